@@ -1,0 +1,228 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+checker.  The GPU library never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liboracle.so")
+
+
+class OParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("moo", "reynolds", "attack", "sideslip", "tref", "twall",
+                                          "cfl", "timedt_rate", "efix", "csrv", "gama", "prl", "prt")] + \
+               [(n, C.c_int) for n in ("nvis", "nlamtur", "nscheme", "nflux", "ntmst")]
+
+
+FIELDS = {"r": 0, "u": 1, "v": 2, "w": 3, "p": 4, "t": 5, "c": 6, "dtdt": 7, "sra": 8, "srb": 9,
+          "src": 10, "srva": 11, "srvb": 12, "srvc": 13, "visl": 14, "vist": 15, "vol": 16,
+          "kcx": 17, "kcy": 18, "kcz": 19, "etx": 20, "ety": 21, "etz": 22, "ctx": 23, "cty": 24,
+          "ctz": 25, "dq": 100, "q": 101, "rhs": 102}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"oracle not built: {LIBPATH} (run make -C oracle)")
+        L = C.CDLL(LIBPATH)
+        dptr = C.POINTER(C.c_double)
+        iptr = C.POINTER(C.c_int)
+        L.o_case_create.restype = C.c_void_p
+        L.o_case_create.argtypes = [C.POINTER(OParams)]
+        L.o_case_destroy.argtypes = [C.c_void_p]
+        L.o_block_add.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, dptr, dptr, dptr]
+        L.o_region_add.argtypes = [C.c_void_p, C.c_int, iptr, C.c_int, iptr, C.c_int, C.c_int]
+        L.o_setup.argtypes = [C.c_void_p]
+        L.o_set_prim.argtypes = [C.c_void_p, C.c_int, dptr, dptr, dptr, dptr, dptr]
+        L.o_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, dptr]
+        L.o_set_field.argtypes = [C.c_void_p, C.c_int, C.c_int, dptr]
+        L.o_g3_size.restype = C.c_long
+        L.o_g3_size.argtypes = [C.c_void_p, C.c_int]
+        L.o_freestream.argtypes = [C.c_void_p, dptr]
+        for n in ("o_boundary_all", "o_initial_q_dq_c_t", "o_timestep", "o_rhs_vis", "o_rhs_invis",
+                  "o_communicate_dq", "o_lhs", "o_communicate_pv"):
+            getattr(L, n).argtypes = [C.c_void_p]
+        L.o_residual.argtypes = [C.c_void_p, dptr, dptr, iptr]
+        L.o_step.argtypes = [C.c_void_p, C.c_int, dptr]
+        L.o_counter.restype = C.c_long
+        L.o_counter.argtypes = [C.c_void_p, C.c_int]
+        L.o_region_images.argtypes = [C.c_void_p, C.c_int, C.c_int, iptr, iptr, iptr, iptr]
+        L.o_multipoint.argtypes = [C.c_void_p, iptr, iptr, iptr, C.c_int]
+        L.o_value_half_node.argtypes = [C.c_int, C.c_int, C.c_int, dptr, dptr]
+        L.o_flux_dxyz.argtypes = [C.c_int, C.c_int, C.c_int, dptr, dptr]
+        L.o_flux_sw.argtypes = [dptr, dptr, C.c_double, C.c_double, C.c_double, C.c_double, dptr,
+                                C.c_double, C.c_double]
+        L.o_flux_roe.argtypes = [dptr, dptr, C.c_double, C.c_double, C.c_double, C.c_double, dptr,
+                                 C.c_double, C.c_double]
+        L.o_wcns_line.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dptr,
+                                  dptr, dptr]
+        L.o_mxdq_std.argtypes = [dptr, dptr, dptr, dptr, C.c_double, C.c_int, C.c_double]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+# ----------------------------------------------------------- pencil routines
+def value_half_node(q):
+    """q (ni, n) -> q_half (ni+1, n): WCNSE5.f90:799-852."""
+    ni, n = q.shape
+    qc = np.ascontiguousarray(q, dtype=np.float64)
+    out = np.zeros((ni + 1, n))
+    lib().o_value_half_node(n, ni, ni, _d(qc), _d(out))
+    return out
+
+
+def flux_dxyz(f):
+    """f (ni+1, nl) -> df (ni, nl): WCNSE5.f90:209-254."""
+    n1, nl = f.shape
+    fc = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros((n1 - 1, nl))
+    lib().o_flux_dxyz(nl, n1 - 1, n1 - 1, _d(fc), _d(out))
+    return out
+
+
+def flux_sw(pl, pr, nv, efix=0.1, gama=1.4):
+    f = np.zeros(5)
+    lib().o_flux_sw(_d(np.ascontiguousarray(pl, float)), _d(np.ascontiguousarray(pr, float)),
+                    nv[0], nv[1], nv[2], nv[3], _d(f), efix, gama)
+    return f
+
+
+def flux_roe(ql, qr, nv, efix=0.1, gama=1.4):
+    f = np.zeros(5)
+    e = lib().o_flux_roe(_d(np.ascontiguousarray(ql, float)), _d(np.ascontiguousarray(qr, float)),
+                         nv[0], nv[1], nv[2], nv[3], _d(f), efix, gama)
+    if e:
+        raise ArithmeticError("Roe average c2 <= 0")
+    return f
+
+
+def wcns_line(scheme, flux, trxyz, qline, efix=0.1, gama=1.4):
+    """trxyz (ni,5), qline (ni+6,5) prim over -2..ni+3 -> (df (ni,5), nerror)."""
+    ni = trxyz.shape[0]
+    q = np.zeros((2, ni + 6, 6))
+    q[0, :, :5] = qline
+    q[:, :, 5] = gama
+    tr = np.ascontiguousarray(trxyz, dtype=np.float64)
+    df = np.zeros((ni, 5))
+    ne = lib().o_wcns_line(scheme, flux, ni, ni, efix, gama, _d(tr), _d(q), _d(df))
+    return df, ne
+
+
+def mxdq_std(prim, gykb, dq, rad, npn, gama=1.4):
+    f = np.zeros(5)
+    lib().o_mxdq_std(_d(np.ascontiguousarray(prim, float)), _d(np.ascontiguousarray(gykb, float)),
+                     _d(np.ascontiguousarray(dq, float)), _d(f), rad, npn, gama)
+    return f
+
+
+# ---------------------------------------------------------------- full case
+class OracleCase:
+    """A multi-block case held by the CPU oracle, built from a wcns_amd.cases.Case."""
+
+    def __init__(self, case):
+        L = lib()
+        p = OParams()
+        for k, _ in OParams._fields_:
+            setattr(p, k, getattr(case.params, k))
+        self.h = L.o_case_create(C.byref(p))
+        self.case = case
+        for blk in case.blocks:
+            x = np.ascontiguousarray(blk.x.transpose(2, 1, 0).ravel())
+            y = np.ascontiguousarray(blk.y.transpose(2, 1, 0).ravel())
+            z = np.ascontiguousarray(blk.z.transpose(2, 1, 0).ravel())
+            L.o_block_add(self.h, blk.ni, blk.nj, blk.nk, _d(x), _d(y), _d(z))
+        for nb, blk in enumerate(case.blocks):
+            for reg in blk.regions:
+                s = np.array(reg.s_raw, dtype=np.int32)
+                t = np.array(reg.t_raw if reg.t_raw is not None else [0] * 6, dtype=np.int32)
+                r = L.o_region_add(self.h, nb, _i(s), reg.bctype, _i(t), reg.nbt, reg.ibcwin)
+                if r < 0:
+                    raise RuntimeError("o_region_add failed")
+        e = L.o_setup(self.h)
+        if e:
+            raise RuntimeError(f"o_setup failed: {e}")
+        for nb, blk in enumerate(case.blocks):
+            prim = [np.ascontiguousarray(a.transpose(2, 1, 0).ravel()) for a in case.initial_prim(nb)]
+            L.o_set_prim(self.h, nb, *[_d(a) for a in prim])
+
+    def close(self):
+        if self.h:
+            lib().o_case_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def field(self, nb, name):
+        blk = self.case.blocks[nb]
+        n = lib().o_g3_size(self.h, nb)
+        shape = (blk.nk + 6, blk.nj + 6, blk.ni + 6)
+        if name in ("dq", "q", "rhs"):
+            out = np.zeros(5 * n)
+            lib().o_get_field(self.h, nb, FIELDS[name], _d(out))
+            return out.reshape(shape + (5,))
+        out = np.zeros(n)
+        lib().o_get_field(self.h, nb, FIELDS[name], _d(out))
+        return out.reshape(shape)
+
+    def set_field(self, nb, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        lib().o_set_field(self.h, nb, FIELDS[name], _d(a))
+
+    def freestream(self):
+        out = np.zeros(8)
+        lib().o_freestream(self.h, _d(out))
+        return out
+
+    def stage(self, name):
+        e = getattr(lib(), "o_" + name)(self.h)
+        if e:
+            raise RuntimeError(f"oracle {name} failed: {e}")
+
+    def step(self, n=1):
+        hist = np.zeros(max(n, 1))
+        e = lib().o_step(self.h, n, _d(hist))
+        if e:
+            raise RuntimeError(f"oracle step failed: {e}")
+        return hist[:n]
+
+    def residual(self):
+        rms, rmax = C.c_double(), C.c_double()
+        loc = np.zeros(5, dtype=np.int32)
+        lib().o_residual(self.h, C.byref(rms), C.byref(rmax), _i(loc))
+        return rms.value, rmax.value, loc
+
+    def counter(self, which):
+        return lib().o_counter(self.h, which)
+
+    def region_images(self, nb, nr):
+        blk = self.case.blocks[nb]
+        n = (blk.ni + 1) * (blk.nj + 1) * (blk.nk + 1)
+        img, jmg, kmg = (np.zeros(n, np.int32) for _ in range(3))
+        box = np.zeros(6, np.int32)
+        lib().o_region_images(self.h, nb, nr, _i(img), _i(jmg), _i(kmg), _i(box))
+        cnt = int(np.prod(box[3:] - box[:3] + 1))
+        return box, img[:cnt], jmg[:cnt], kmg[:cnt]
+
+    def multipoint(self):
+        ng = np.zeros(1, np.int32)
+        npts = np.zeros(1 << 16, np.int32)
+        cap = 4 << 18
+        out = np.zeros(cap, np.int32)
+        tot = lib().o_multipoint(self.h, _i(ng), _i(npts), _i(out), cap)
+        return npts[:ng[0]].copy(), out[:4 * tot].reshape(-1, 4).copy()

@@ -1,0 +1,66 @@
+"""The CPU oracle against golden vectors written by the reference's own Fortran
+(WCNSE5.f90 / flux.f90 compiled from source by oracle/Makefile, vectors made by
+tests/golden/make_golden.py), and, where the reference library is present in
+this container, against it directly on fresh seeded inputs.
+
+The bar is bit-exactness: both sides round every product and sum once
+(-ffp-contract=off, no FMA in the x86-64 baseline ISA) in the same order.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = np.load(os.path.join(HERE, "golden", "pencil_golden.npz"))
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 6, 9, 17])
+def test_value_half_node_golden(n):
+    out = O.value_half_node(GOLD[f"vhn_in_{n}"])
+    np.testing.assert_array_equal(out, GOLD[f"vhn_out_{n}"])
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 6, 9, 17])
+def test_flux_dxyz_golden(n):
+    out = O.flux_dxyz(GOLD[f"fdx_in_{n}"])
+    np.testing.assert_array_equal(out, GOLD[f"fdx_out_{n}"])
+
+
+def test_flux_sw_golden():
+    for row in GOLD["flux_sw"]:
+        pl, pr, nv, f = row[:5], row[5:10], row[10:14], row[14:19]
+        np.testing.assert_array_equal(O.flux_sw(pl, pr, nv), f)
+
+
+@pytest.mark.parametrize("key", [str(k) for k in GOLD["wcns_cases"]])
+def test_wcns_line_golden(key):
+    scheme = 41 if key.startswith("wcns41") else 4
+    df, ne = O.wcns_line(scheme, 1, GOLD[key + "_tr"], GOLD[key + "_q"])
+    np.testing.assert_array_equal(df, GOLD[key + "_df"])
+    if key.endswith("fallback"):
+        assert ne > 0
+
+
+REF = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "libref.so")
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="reference library not built here")
+def test_wcns_line_against_reference_fresh_inputs():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mg", os.path.join(HERE, "golden", "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    lib = mg.load()
+    rng = np.random.default_rng(777)
+    for scheme, fn in ((4, "wcns_e_5_"), (41, "wcns_e_5_41_")):
+        for n in (5, 7, 12, 20, 64):
+            for bl in (0, 1):
+                for br in (0, 1):
+                    q = mg.smooth_line(rng, n, bl, br, amp=0.2, rough=bool(rng.integers(0, 2)))
+                    tr = mg.metrics_line(rng, n)
+                    ref = mg.wcns_line(lib, fn, tr, q)
+                    mine, _ = O.wcns_line(scheme, 1, tr, q)
+                    np.testing.assert_array_equal(mine, ref)
