@@ -1,0 +1,119 @@
+/*
+ * wcns_amd.h -- C-ABI of the MI355X (gfx950) WCNS pseudo-time-step library.
+ *
+ * The reference (zhanghong24/WCNS, Fortran 90 + MPI) runs its hot path as
+ * argument-less subroutines over module globals.  This library exports one
+ * entry point per such subroutine, operating on device-resident state held in
+ * an opaque context.  All functions return 0 on success and a negative
+ * WCNS_E_* code on failure; none of them aborts the process.
+ *
+ * Arrays crossing the boundary are plain host pointers:
+ *   - grid coordinates x,y,z of a block: ni*nj*nk doubles, i fastest,
+ *     node (i,j,k) (1-based) at (i-1) + ni*((j-1) + nj*(k-1))
+ *     (PLOT3D order, reference src/PreProcess.f90:365-412);
+ *   - ghosted fields ("G3", bounds -2..n+3 as in src/PreProcess.f90:720-786):
+ *     (ni+6)*(nj+6)*(nk+6) doubles, node (i,j,k) at
+ *     (i+2) + (ni+6)*((j+2) + (nj+6)*(k+2)).
+ * Inside the library every field is a separate HBM array (structure of
+ * arrays) in that same ghosted order.
+ */
+#ifndef WCNS_AMD_H
+#define WCNS_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WCNS_OK               0
+#define WCNS_E_ARG           -1   /* bad argument, index or size               */
+#define WCNS_E_STATE         -2   /* call out of order (e.g. before setup)      */
+#define WCNS_E_HIP           -3   /* a HIP runtime call failed                  */
+#define WCNS_E_NOMEM         -4   /* device allocation failed                   */
+#define WCNS_E_BC            -5   /* unsupported boundary type (BC.f90:201-205) */
+#define WCNS_E_JACOBIAN      -6   /* non-positive cell volume (PreProcess.f90:2637-2666) */
+#define WCNS_E_ROE           -7   /* Roe average c^2 <= 0 (flux.f90:179-182)     */
+#define WCNS_E_REPAIR        -8   /* too many repaired nodes (Sub2.f90:781-785)  */
+#define WCNS_E_UNSUPPORTED   -9   /* option of the reference not built here      */
+
+/* Namelist values read by the hot path (reference src/param.dat groups inflow,
+ * step, flowtype, turbulence, technic; src/PreProcess.f90:80-159). */
+typedef struct {
+    double moo, reynolds, attack, sideslip, tref, twall;
+    double cfl, timedt_rate, efix, csrv, gama, prl, prt;
+    int nvis;      /* 0 Euler, 1 Navier-Stokes                          */
+    int nlamtur;   /* -1 laminar                                         */
+    int nscheme;   /* 4 (WCNS_E_5) or 41 (WCNS_E_5_41)                   */
+    int nflux;     /* 1 Steger-Warming, 3 Roe                            */
+    int ntmst;     /* 0 local time step                                  */
+} wcns_params;
+
+/* Precision of the implicit operator: the residual, WCNS weights and the
+ * conservative update are always FP64. */
+#define WCNS_LHS_FP64 0
+#define WCNS_LHS_FP32 1
+
+typedef struct wcns_ctx wcns_ctx;
+
+/* ---- context (no reference counterpart: the reference holds module globals,
+ *      src/Globalvar.f90:284-423) ---------------------------------------- */
+int wcns_create(wcns_ctx **ctx, int device, const wcns_params *prm);
+int wcns_destroy(wcns_ctx *ctx);
+const char *wcns_last_error(const wcns_ctx *ctx);
+
+/* ---- setup: replaces read_plot3d_grid / read_bc_parallel / analyze_bc /
+ *      set_grid_derivative (src/PreProcess.f90:365-412, src/parallel.f90:852-1301,
+ *      src/PreProcess.f90:1618-1660) ------------------------------------- */
+/* Adds a block; *nb receives its 0-based index. */
+int wcns_add_block(wcns_ctx *ctx, int ni, int nj, int nk,
+                   const double *x, const double *y, const double *z, int *nb);
+/* One region line of the .inp file with the file's signs, s_raw = (s_st1,
+ * s_ed1, s_st2, s_ed2, s_st3, s_ed3); for bctype -1 the target line t_raw,
+ * target block nbt (1-based) and its region number ibcwin (1-based). */
+int wcns_add_region(wcns_ctx *ctx, int nb, const int s_raw[6], int bctype,
+                    const int t_raw[6], int nbt, int ibcwin);
+/* GCL metrics on the device (GRID_DERIVATIVE_gcl, src/WCNSE5.f90:1478-1728),
+ * boundary order (src/PreProcess.f90:2543-2573), interface maps
+ * (src/parallel.f90:1185-1301) and multi-point groups (src/parallel.f90:4202-4311). */
+int wcns_setup(wcns_ctx *ctx);
+int wcns_set_lhs_precision(wcns_ctx *ctx, int prec);
+
+/* ---- field exchange (host <-> device), G3 layout ------------------------ */
+/* which: 0 r, 1 u, 2 v, 3 w, 4 p, 5 t, 7 dtdt, 8 sra, 9 srb, 10 src,
+ *        11 srva, 12 srvb, 13 srvc, 14 visl, 16 vol, 17..25 kcx..ctz,
+ *        100 dq (5 G3 arrays, variable-major), 102 rhs (the residual R) */
+int wcns_set_prim(wcns_ctx *ctx, int nb, const double *r, const double *u,
+                  const double *v, const double *w, const double *p);
+int wcns_get_field(wcns_ctx *ctx, int nb, int which, double *out);
+int wcns_set_field(wcns_ctx *ctx, int nb, int which, const double *in);
+
+/* ---- one pseudo-time step, stage by stage (src/Main.f90:85-109) --------- */
+int wcns_boundary_all(wcns_ctx *ctx);        /* sub_tgh.f90:123-186     */
+int wcns_initial_q_dq_c_t(wcns_ctx *ctx);    /* sub_tgh.f90:1-32        */
+int wcns_timestep(wcns_ctx *ctx);            /* sub_tgh.f90:1270-1348   */
+int wcns_r_h_s_vis(wcns_ctx *ctx);           /* RHS.f90:2-30            */
+int wcns_r_h_s_invis(wcns_ctx *ctx);         /* RHS.f90:35-72           */
+int wcns_communicate_dq(wcns_ctx *ctx);      /* parallel.f90:3020-3164  */
+int wcns_l_h_s(wcns_ctx *ctx);               /* LHS.f90:1-34            */
+int wcns_communicate_pv(wcns_ctx *ctx);      /* parallel.f90:3166-3326  */
+/* residual_curve (PostProcess.f90:1195-1347): RMS and max of dq/(dtdt*vol)
+ * over nodes 2..n-1; loc = i,j,k,m,nb (1-based).  Synchronises the stream. */
+int wcns_residual(wcns_ctx *ctx, double *rms, double *resmax, int loc[5]);
+
+/* nsteps whole steps; rms_hist (may be NULL) receives each step's RMS.
+ * With rms_hist == NULL the residual is still computed on the device each
+ * step but only read back at the end (no per-step host synchronisation). */
+int wcns_step(wcns_ctx *ctx, int nsteps, double *rms_hist);
+
+/* device error/diagnostic counters: 0 WCNS positivity fallbacks
+ * (WCNSE5.f90:2224-2245), 1 repaired nodes (Sub2.f90:661-767), 2 Roe c^2<=0 */
+int wcns_get_counter(wcns_ctx *ctx, int which, long long *value);
+int wcns_sync(wcns_ctx *ctx);
+
+/* debugging/inspection of the host-side setup */
+int wcns_region_images(wcns_ctx *ctx, int nb, int nr, int *img, int *jmg, int *kmg, int box[6]);
+int wcns_multipoint(wcns_ctx *ctx, int *ngroups, int *npts, int *pts, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

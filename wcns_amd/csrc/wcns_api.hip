@@ -1,0 +1,824 @@
+// wcns_api.hip -- the C-ABI (include/wcns_amd.h): context, host-side setup
+// (boundary topology, interface maps, multi-point groups) and the ordering of
+// the per-step kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <set>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/wcns_amd.h"
+#include "wcns_kernels.hpp"
+
+using namespace wcns;
+
+namespace {
+
+struct HRegion {
+    int bctype;
+    int raw_s[6], raw_t[6];
+    int nbt, ibcwin;         // 1-based as in the .inp file
+    DRegion d;               // analysed form
+};
+
+struct HBlock {
+    int ni, nj, nk;
+    std::vector<double> x, y, z;
+    std::vector<HRegion> regs;
+    std::vector<int> bcorder;  // region indices in -1,4,5,6,2,3 order
+    DBlock db;
+    long g3;
+};
+
+#define CK(call)                                                                    \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);           \
+            return WCNS_E_HIP;                                                      \
+        }                                                                           \
+    } while (0)
+
+}  // namespace
+
+struct wcns_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    wcns_params prm{};
+    DParams dp{};
+    std::vector<HBlock> blocks;
+    bool setup_done = false;
+    int lhs_prec = WCNS_LHS_FP64;
+    std::string err;
+
+    long total = 0;
+    double *prim[2][5] = {{nullptr}};
+    int cur = 0;
+    double *t = nullptr, *metric[10] = {nullptr}, *rhs = nullptr, *dq = nullptr;
+    double *dtdt = nullptr, *sr[3] = {nullptr}, *srv[3] = {nullptr}, *visl = nullptr;
+    unsigned long long *counters = nullptr;
+    DBlock *d_blocks = nullptr;
+    int *d_flag = nullptr;
+    int flag_cap = 512;
+    double *d_partial = nullptr;
+    int max_parts = 1;
+    double *d_resblk = nullptr;
+    double *d_hist = nullptr;
+    int hist_cap = 0;
+
+    std::vector<DRegion> iface;              // interfaces in (nb, nr) order
+    std::vector<DRegion> phys;               // bctype > 0 regions
+    int npp = 0, ngroups = 0;
+    int *d_npp_pts = nullptr, *d_grp_start = nullptr, *d_grp_n = nullptr, *d_write_ok = nullptr;
+    double *d_npp_buf = nullptr;
+    std::vector<int> h_npp_pts, h_grp_n;
+
+    DFields fields() const
+    {
+        DFields F{};
+        F.r = prim[cur][0]; F.u = prim[cur][1]; F.v = prim[cur][2]; F.w = prim[cur][3]; F.p = prim[cur][4];
+        F.rn = prim[cur ^ 1][0]; F.un = prim[cur ^ 1][1]; F.vn = prim[cur ^ 1][2];
+        F.wn = prim[cur ^ 1][3]; F.pn = prim[cur ^ 1][4];
+        F.t = t;
+        F.kcx = metric[0]; F.kcy = metric[1]; F.kcz = metric[2];
+        F.etx = metric[3]; F.ety = metric[4]; F.etz = metric[5];
+        F.ctx = metric[6]; F.cty = metric[7]; F.ctz = metric[8]; F.vol = metric[9];
+        F.rhs = rhs; F.dq = dq; F.dtdt = dtdt;
+        F.sra = sr[0]; F.srb = sr[1]; F.src = sr[2];
+        F.srva = srv[0]; F.srvb = srv[1]; F.srvc = srv[2];
+        F.visl = visl; F.counters = counters; F.total = total;
+        return F;
+    }
+};
+
+namespace {
+
+int fail(wcns_ctx *ctx, int code, const std::string &msg)
+{
+    ctx->err = msg;
+    return code;
+}
+
+// src/parallel.f90:1123-1301 analyze_bc + analyze_bc_connect (method = 1)
+int analyze_region(wcns_ctx *ctx, int nb, int nr, HRegion &r)
+{
+    DRegion &d = r.d;
+    memset(&d, 0, sizeof d);
+    d.bctype = r.bctype;
+    d.nb = nb;
+    int s_st[3], s_ed[3];
+    for (int m = 0; m < 3; m++) { s_st[m] = r.raw_s[2 * m]; s_ed[m] = r.raw_s[2 * m + 1]; }
+    d.s_nd = 0;
+    for (int m = 0; m < 3; m++) {
+        d.s_lo[m] = std::min(std::abs(s_st[m]), std::abs(s_ed[m]));
+        d.s_hi[m] = std::max(std::abs(s_st[m]), std::abs(s_ed[m]));
+        d.l3[m] = 0;
+        if (s_st[m] == s_ed[m]) {
+            d.s_nd = m + 1;
+            if (s_st[m] == 1) { d.s_lr = -1; d.l3[m] = -1; }
+            else { d.s_lr = 1; d.l3[m] = 1; }
+        }
+    }
+    if (d.s_nd == 0) return fail(ctx, WCNS_E_ARG, "region " + std::to_string(nr + 1) + " of block " +
+                                                   std::to_string(nb + 1) + " is not a face");
+    const HBlock &b = ctx->blocks[nb];
+    int dims[3] = {b.ni, b.nj, b.nk};
+    for (int m = 0; m < 3; m++)
+        if (d.s_lo[m] < 1 || d.s_hi[m] > dims[m])
+            return fail(ctx, WCNS_E_ARG, "region outside its block");
+    d.npts = (d.s_hi[0] - d.s_lo[0] + 1) * (d.s_hi[1] - d.s_lo[1] + 1) * (d.s_hi[2] - d.s_lo[2] + 1);
+    if (r.bctype >= 0) {
+        if (r.bctype < 2 || r.bctype > 6) return fail(ctx, WCNS_E_BC, "unsupported bctype " + std::to_string(r.bctype));
+        d.nbt = nb;
+        return 0;
+    }
+    if (r.bctype != -1) return fail(ctx, WCNS_E_BC, "only bctype -1 interfaces (PreProcess.f90:2566)");
+    if (r.nbt < 1 || r.nbt > (int)ctx->blocks.size()) return fail(ctx, WCNS_E_ARG, "bad target block");
+    d.nbt = r.nbt - 1;
+    int t_st[3], t_ed[3], t_nd = 0;
+    for (int m = 0; m < 3; m++) { t_st[m] = r.raw_t[2 * m]; t_ed[m] = r.raw_t[2 * m + 1]; }
+    for (int m = 0; m < 3; m++) {
+        d.t_l3[m] = 0;
+        if (t_st[m] == t_ed[m]) {
+            t_nd = m + 1;
+            d.t_l3[m] = t_st[m] == 1 ? -1 : 1;
+        }
+    }
+    if (t_nd == 0) return fail(ctx, WCNS_E_ARG, "interface target is not a face");
+    int sdir[3][3] = {{0}};
+    bool found = false;
+    for (int m = 1; m <= 3 && !found; m++)
+        for (int n = 1; n <= 3 && !found; n++) {
+            if (m == d.s_nd || n == t_nd) continue;
+            int js1 = s_st[m - 1];
+            if (std::abs(js1) < std::abs(s_ed[m - 1])) js1 = s_ed[m - 1];
+            int js2 = t_st[n - 1];
+            if (std::abs(js2) < std::abs(t_ed[n - 1])) js2 = t_ed[n - 1];
+            if (js1 * js2 > 0) {
+                sdir[n - 1][m - 1] = 1;
+                sdir[t_nd - 1][d.s_nd - 1] = 1;
+                sdir[6 - n - t_nd - 1][6 - m - d.s_nd - 1] = 1;
+                found = true;
+            }
+        }
+    int s_sign[3], t_sign[3], st_sign[3];
+    for (int m = 0; m < 3; m++) {
+        s_sign[m] = 1; s_st[m] = std::abs(s_st[m]); s_ed[m] = std::abs(s_ed[m]);
+        if (m + 1 != d.s_nd && s_st[m] > s_ed[m]) s_sign[m] = -1;
+        t_sign[m] = 1; t_st[m] = std::abs(t_st[m]); t_ed[m] = std::abs(t_ed[m]);
+        if (m + 1 != t_nd && t_st[m] > t_ed[m]) t_sign[m] = -1;
+    }
+    for (int m = 0; m < 3; m++) {
+        int co = 0;
+        for (int n = 0; n < 3; n++) if (sdir[n][m] == 1) co = n + 1;
+        st_sign[m] = (co > 0 ? t_sign[co - 1] : 1) * s_sign[m];
+    }
+    // image = t_st + sum_m sdir[a][m] * (idx_m - s_st_m) * st_sign_m
+    for (int a = 0; a < 3; a++) {
+        d.t0[a] = t_st[a];
+        d.s0[a] = s_st[a];
+        for (int m = 0; m < 3; m++) d.M[3 * a + m] = sdir[a][m] * st_sign[m];
+    }
+    // every image must land on the target block
+    const HBlock &bt = ctx->blocks[d.nbt];
+    int tdims[3] = {bt.ni, bt.nj, bt.nk};
+    for (int t = 0; t < d.npts; t += std::max(1, d.npts - 1)) {
+        int i, j, k, it, jt, kt;
+        region_point(d, t, i, j, k);
+        region_image(d, i, j, k, it, jt, kt);
+        int im[3] = {it, jt, kt};
+        for (int m = 0; m < 3; m++)
+            if (im[m] < 1 || im[m] > tdims[m]) return fail(ctx, WCNS_E_ARG, "interface image outside target block");
+    }
+    return 0;
+}
+
+typedef std::tuple<int, int, int, int> Pt;
+
+// parallel.f90:177-393 + 4202-4311: greedy multi-point groups, kept in order
+void build_groups(wcns_ctx *ctx, std::vector<std::vector<Pt>> &groups)
+{
+    int nbk = (int)ctx->blocks.size();
+    std::vector<std::vector<int>> iused(nbk);
+    for (int nb = 0; nb < nbk; nb++) iused[nb].assign(ctx->blocks[nb].regs.size(), 0);
+    for (int nb = 0; nb < nbk; nb++)
+        for (size_t r = 0; r < ctx->blocks[nb].regs.size(); r++) {
+            const HRegion &g = ctx->blocks[nb].regs[r];
+            if (g.bctype < 0 && iused[nb][r] == 0) {
+                iused[nb][r] = 1;
+                int tb = g.nbt - 1, tr = g.ibcwin - 1;
+                if (tb >= 0 && tb < nbk && tr >= 0 && tr < (int)iused[tb].size()) iused[tb][tr] = -1;
+            }
+        }
+    std::vector<std::vector<Pt>> comp;
+    std::map<Pt, std::vector<int>> where;        // point -> groups containing it (ascending)
+    const int mcyc[5] = {1, 2, 3, 1, 2};
+    for (int nb = 0; nb < nbk; nb++)
+        for (size_t r = 0; r < ctx->blocks[nb].regs.size(); r++) {
+            const HRegion &g = ctx->blocks[nb].regs[r];
+            if (g.bctype >= 0 || iused[nb][r] != 1) continue;
+            const DRegion &d = g.d;
+            int id2 = mcyc[d.s_nd] - 1, id3 = mcyc[d.s_nd + 1] - 1;
+            for (int k = d.s_lo[2]; k <= d.s_hi[2]; k++)
+                for (int j = d.s_lo[1]; j <= d.s_hi[1]; j++)
+                    for (int i = d.s_lo[0]; i <= d.s_hi[0]; i++) {
+                        int idx[3] = {i, j, k};
+                        bool inner = true;
+                        for (int m = 0; m < 3; m++) {
+                            int lo = d.s_lo[m], hi = d.s_hi[m];
+                            if (m == id2 || m == id3) { lo++; hi--; }
+                            if (idx[m] < lo || idx[m] > hi) inner = false;
+                        }
+                        if (inner) continue;
+                        int it, jt, kt;
+                        region_image(d, i, j, k, it, jt, kt);
+                        Pt a{nb, i, j, k}, bt{d.nbt, it, jt, kt};
+                        int first = -1;
+                        auto ia = where.find(a), ib = where.find(bt);
+                        if (ia != where.end()) first = ia->second.front();
+                        if (ib != where.end() && (first < 0 || ib->second.front() < first)) first = ib->second.front();
+                        if (first >= 0) {
+                            auto &cg = comp[first];
+                            bool ea = std::find(cg.begin(), cg.end(), a) != cg.end();
+                            bool eb = std::find(cg.begin(), cg.end(), bt) != cg.end();
+                            if (!ea) { cg.push_back(a); auto &w = where[a]; w.push_back(first); std::sort(w.begin(), w.end()); }
+                            if (!eb) { cg.push_back(bt); auto &w = where[bt]; w.push_back(first); std::sort(w.begin(), w.end()); }
+                        } else {
+                            comp.push_back({a, bt});
+                            int gi = (int)comp.size() - 1;
+                            where[a].push_back(gi);
+                            where[bt].push_back(gi);
+                        }
+                    }
+        }
+    groups.clear();
+    for (auto &c : comp)
+        if (c.size() > 2) groups.push_back(c);
+}
+
+long g3_of(int ni, int nj, int nk) { return (long)(ni + 6) * (nj + 6) * (nk + 6); }
+
+template <typename T>
+int dalloc(wcns_ctx *ctx, T **p, size_t n)
+{
+    if (hipMalloc((void **)p, sizeof(T) * (n ? n : 1)) != hipSuccess) {
+        ctx->err = "hipMalloc of " + std::to_string(sizeof(T) * n) + " bytes failed";
+        return WCNS_E_NOMEM;
+    }
+    if (hipMemset(*p, 0, sizeof(T) * (n ? n : 1)) != hipSuccess) return WCNS_E_HIP;
+    return 0;
+}
+
+#define TRY(x)                 \
+    do {                       \
+        int e__ = (x);         \
+        if (e__) return e__;   \
+    } while (0)
+
+int check_counters(wcns_ctx *ctx)
+{
+    unsigned long long c[8];
+    CK(hipMemcpyAsync(c, ctx->counters, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (c[2]) return fail(ctx, WCNS_E_ROE, "sqrt(c2) is a math error in flux_Roe (flux.f90:179-182)");
+    if (c[4]) return fail(ctx, WCNS_E_REPAIR, "too many abnormal nodes in update_limited (Sub2.f90:781-785)");
+    return 0;
+}
+
+}  // namespace
+
+// ======================================================================
+extern "C" {
+
+int wcns_create(wcns_ctx **out, int device, const wcns_params *prm)
+{
+    if (!out || !prm) return WCNS_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return WCNS_E_HIP;
+    if (hipSetDevice(device) != hipSuccess) return WCNS_E_HIP;
+    wcns_ctx *ctx = new wcns_ctx();
+    ctx->device = device;
+    ctx->prm = *prm;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return WCNS_E_HIP;
+    }
+    if (prm->nscheme != 4 && prm->nscheme != 41) { ctx->err = "nscheme must be 4 or 41"; }
+    *out = ctx;
+    return WCNS_OK;
+}
+
+int wcns_destroy(wcns_ctx *ctx)
+{
+    if (!ctx) return WCNS_E_ARG;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (int b = 0; b < 2; b++) for (int m = 0; m < 5; m++) hipFree(ctx->prim[b][m]);
+    for (int m = 0; m < 10; m++) hipFree(ctx->metric[m]);
+    for (int m = 0; m < 3; m++) { hipFree(ctx->sr[m]); hipFree(ctx->srv[m]); }
+    void *ps[] = {ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
+                  ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
+                  ctx->d_write_ok, ctx->d_npp_buf};
+    for (void *p : ps) if (p) hipFree(p);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return WCNS_OK;
+}
+
+const char *wcns_last_error(const wcns_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int wcns_add_block(wcns_ctx *ctx, int ni, int nj, int nk, const double *x, const double *y, const double *z, int *nb)
+{
+    if (!ctx) return WCNS_E_ARG;
+    if (ctx->setup_done) return fail(ctx, WCNS_E_STATE, "blocks must be added before wcns_setup");
+    if (ni < 2 || nj < 2 || nk < 2 || !x || !y || !z) return fail(ctx, WCNS_E_ARG, "bad block");
+    HBlock b;
+    b.ni = ni; b.nj = nj; b.nk = nk;
+    size_t n = (size_t)ni * nj * nk;
+    b.x.assign(x, x + n); b.y.assign(y, y + n); b.z.assign(z, z + n);
+    b.g3 = g3_of(ni, nj, nk);
+    ctx->blocks.push_back(std::move(b));
+    if (nb) *nb = (int)ctx->blocks.size() - 1;
+    return WCNS_OK;
+}
+
+int wcns_add_region(wcns_ctx *ctx, int nb, const int s_raw[6], int bctype, const int t_raw[6], int nbt, int ibcwin)
+{
+    if (!ctx || !s_raw) return WCNS_E_ARG;
+    if (ctx->setup_done) return fail(ctx, WCNS_E_STATE, "regions must be added before wcns_setup");
+    if (nb < 0 || nb >= (int)ctx->blocks.size()) return fail(ctx, WCNS_E_ARG, "bad block index");
+    HRegion r{};
+    r.bctype = bctype;
+    memcpy(r.raw_s, s_raw, sizeof r.raw_s);
+    if (bctype < 0) {
+        if (!t_raw) return WCNS_E_ARG;
+        memcpy(r.raw_t, t_raw, sizeof r.raw_t);
+        r.nbt = nbt;
+        r.ibcwin = ibcwin;
+    }
+    ctx->blocks[nb].regs.push_back(r);
+    return WCNS_OK;
+}
+
+int wcns_set_lhs_precision(wcns_ctx *ctx, int prec)
+{
+    if (!ctx || (prec != WCNS_LHS_FP64 && prec != WCNS_LHS_FP32)) return WCNS_E_ARG;
+    if (prec == WCNS_LHS_FP32) return fail(ctx, WCNS_E_UNSUPPORTED, "FP32 implicit operator not built yet");
+    ctx->lhs_prec = prec;
+    return WCNS_OK;
+}
+
+int wcns_setup(wcns_ctx *ctx)
+{
+    if (!ctx) return WCNS_E_ARG;
+    if (ctx->setup_done) return fail(ctx, WCNS_E_STATE, "setup already done");
+    if (ctx->blocks.empty()) return fail(ctx, WCNS_E_STATE, "no blocks");
+    CK(hipSetDevice(ctx->device));
+    const wcns_params &p = ctx->prm;
+    if (p.nscheme != 4 && p.nscheme != 41) return fail(ctx, WCNS_E_UNSUPPORTED, "nscheme must be 4 or 41");
+    if (p.nflux != 1 && p.nflux != 3) return fail(ctx, WCNS_E_UNSUPPORTED, "nflux must be 1 (SW) or 3 (Roe)");
+    if (p.nvis != 0) return fail(ctx, WCNS_E_UNSUPPORTED, "viscous terms not built yet");
+    if (p.ntmst != 0) return fail(ctx, WCNS_E_UNSUPPORTED, "only local time stepping (ntmst = 0)");
+    // layout
+    long off = 0;
+    for (auto &b : ctx->blocks) {
+        for (int n : {b.ni, b.nj, b.nk})
+            if (n > 4 && n < 12) return fail(ctx, WCNS_E_UNSUPPORTED, "directions with 5..11 nodes are not supported");
+        b.db.ni = b.ni; b.db.nj = b.nj; b.db.nk = b.nk; b.db.pad = 0;
+        b.db.sx = b.ni + 6;
+        b.db.sy = (long)(b.ni + 6) * (b.nj + 6);
+        b.db.off = off;
+        off += b.g3;
+    }
+    ctx->total = off;
+    // regions
+    static const int order[6] = {-1, 4, 5, 6, 2, 3};      // PreProcess.f90:2543-2573
+    for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
+        HBlock &b = ctx->blocks[nb];
+        for (size_t r = 0; r < b.regs.size(); r++) TRY(analyze_region(ctx, (int)nb, (int)r, b.regs[r]));
+        b.bcorder.clear();
+        for (int o : order)
+            for (size_t r = 0; r < b.regs.size(); r++)
+                if ((b.regs[r].bctype < 0 ? -1 : b.regs[r].bctype) == o) b.bcorder.push_back((int)r);
+        if (b.bcorder.size() != b.regs.size()) return fail(ctx, WCNS_E_BC, "incorrect BC types");
+    }
+    // device fields
+    long T = ctx->total;
+    for (int bb = 0; bb < 2; bb++) for (int m = 0; m < 5; m++) TRY(dalloc(ctx, &ctx->prim[bb][m], T));
+    for (int m = 0; m < 10; m++) TRY(dalloc(ctx, &ctx->metric[m], T));
+    for (int m = 0; m < 3; m++) { TRY(dalloc(ctx, &ctx->sr[m], T)); TRY(dalloc(ctx, &ctx->srv[m], T)); }
+    TRY(dalloc(ctx, &ctx->t, T));
+    TRY(dalloc(ctx, &ctx->rhs, 5 * T));
+    TRY(dalloc(ctx, &ctx->dq, 5 * T));
+    TRY(dalloc(ctx, &ctx->dtdt, T));
+    TRY(dalloc(ctx, &ctx->visl, T));
+    TRY(dalloc(ctx, &ctx->counters, 8));
+    TRY(dalloc(ctx, &ctx->d_flag, ctx->flag_cap));
+    std::vector<DBlock> hb;
+    int maxp = 1;
+    for (auto &b : ctx->blocks) { hb.push_back(b.db); maxp = std::max(maxp, residual_parts(b.db)); }
+    ctx->max_parts = maxp;
+    TRY(dalloc(ctx, &ctx->d_blocks, hb.size()));
+    CK(hipMemcpy(ctx->d_blocks, hb.data(), sizeof(DBlock) * hb.size(), hipMemcpyHostToDevice));
+    TRY(dalloc(ctx, &ctx->d_partial, 3 * maxp));
+    TRY(dalloc(ctx, &ctx->d_resblk, 3 * ctx->blocks.size()));
+
+    // freestream (PreProcess.f90:968-1113, height < 0)
+    DParams &d = ctx->dp;
+    const double pai = 3.14159265;
+    double att = p.attack * pai / 180.0, ssl = p.sideslip * pai / 180.0;
+    d.gama = p.gama; d.efix = p.efix; d.cfl = p.cfl; d.timedt_rate = p.timedt_rate; d.moo = p.moo;
+    d.csrv = p.csrv; d.re = 1.0 / p.reynolds; d.visc = 110.4 / p.tref; d.prl = p.prl; d.prt = p.prt;
+    d.roo = 1.0; d.uoo = cos(att) * cos(ssl); d.voo = sin(att) * cos(ssl); d.woo = sin(ssl);
+    d.poo = 1.0 / (p.gama * p.moo * p.moo);
+    d.coo = sqrt(p.gama * d.poo / d.roo);
+    d.twall_fixed = p.twall > 0.0;
+    d.twall_nd = p.twall / p.tref;
+    d.nvis = p.nvis; d.nscheme = p.nscheme; d.nflux = p.nflux;
+
+    // GCL metrics per block on the device
+    DFields F = ctx->fields();
+    for (auto &b : ctx->blocks) {
+        long n = (long)b.ni * b.nj * b.nk;
+        double *dx, *dy, *dz, *work;
+        TRY(dalloc(ctx, &dx, n)); TRY(dalloc(ctx, &dy, n)); TRY(dalloc(ctx, &dz, n));
+        TRY(dalloc(ctx, &work, 21 * b.g3 + 1));
+        CK(hipMemcpy(dx, b.x.data(), 8 * n, hipMemcpyHostToDevice));
+        CK(hipMemcpy(dy, b.y.data(), 8 * n, hipMemcpyHostToDevice));
+        CK(hipMemcpy(dz, b.z.data(), 8 * n, hipMemcpyHostToDevice));
+        if (launch_gcl_metrics(b.db, dx, dy, dz, F, work, ctx->stream)) return fail(ctx, WCNS_E_HIP, "gcl launch");
+        int nbad = 0;
+        CK(hipMemcpyAsync(&nbad, (int *)(work + 21 * b.g3), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        hipFree(dx); hipFree(dy); hipFree(dz); hipFree(work);
+        if (nbad) return fail(ctx, WCNS_E_JACOBIAN, std::to_string(nbad) + " nodes with non-positive Jacobian");
+    }
+    // symmetry-plane normals (BC.f90:381-396) need the metrics at the face centre
+    for (auto &b : ctx->blocks)
+        for (auto &r : b.regs) {
+            if (r.bctype != 3) continue;
+            DRegion &g = r.d;
+            int dims[3] = {b.ni, b.nj, b.nk};
+            if (dims[g.s_nd - 1] <= 4) return fail(ctx, WCNS_E_UNSUPPORTED, "2-D symmetry branch not built");
+            int c[3] = {(g.s_lo[0] + g.s_hi[0]) / 2, (g.s_lo[1] + g.s_hi[1]) / 2, (g.s_lo[2] + g.s_hi[2]) / 2};
+            long q = gidx(b.db, c[0], c[1], c[2]);
+            double m[9];
+            for (int a = 0; a < 9; a++) CK(hipMemcpy(&m[a], ctx->metric[a] + q, 8, hipMemcpyDeviceToHost));
+            double x = 0, y = 0, z = 0;
+            for (int a = 0; a < 3; a++)
+                if (g.l3[a]) { x = x + m[3 * a]; y = y + m[3 * a + 1]; z = z + m[3 * a + 2]; }
+            double s = std::max(sqrt(x * x + y * y + z * z), 1.0e-30);
+            double nx = x / s, ny = y / s, nz = z / s;
+            double s2 = sqrt(nx * nx + ny * ny + nz * nz);
+            g.nrm[0] = nx / s2; g.nrm[1] = ny / s2; g.nrm[2] = nz / s2;
+        }
+    // interface lists: 2pm regions in (nb, nr) order, serial when a region reads its own writes
+    ctx->iface.clear();
+    ctx->phys.clear();
+    for (auto &b : ctx->blocks)
+        for (auto &r : b.regs) {
+            if (r.bctype > 0) ctx->phys.push_back(r.d);
+            if (r.bctype >= 0) continue;
+            DRegion g = r.d;
+            if (g.nbt == g.nb) {
+                std::set<std::tuple<int, int, int>> w;
+                for (int t = 0; t < g.npts; t++) { int i, j, k; region_point(g, t, i, j, k); w.insert({i, j, k}); }
+                for (int t = 0; t < g.npts && !g.serial; t++) {
+                    int i, j, k, it, jt, kt;
+                    region_point(g, t, i, j, k);
+                    region_image(g, i, j, k, it, jt, kt);
+                    if (w.count({it, jt, kt}) && !(it == i && jt == j && kt == k)) g.serial = 1;
+                }
+            }
+            ctx->iface.push_back(g);
+        }
+    // multi-point groups
+    std::vector<std::vector<Pt>> groups;
+    build_groups(ctx, groups);
+    std::vector<int> pts, gstart, gn, ok;
+    std::map<Pt, int> last;
+    for (auto &g : groups) {
+        gstart.push_back((int)pts.size() / 4);
+        gn.push_back((int)g.size());
+        for (auto &q : g) {
+            last[q] = (int)pts.size() / 4;
+            pts.push_back(std::get<0>(q)); pts.push_back(std::get<1>(q));
+            pts.push_back(std::get<2>(q)); pts.push_back(std::get<3>(q));
+        }
+    }
+    int np = (int)pts.size() / 4;
+    ok.assign(np, 0);
+    for (auto &kv : last) ok[kv.second] = 1;   // a point written by several groups keeps the last write
+    ctx->npp = np;
+    ctx->ngroups = (int)groups.size();
+    ctx->h_npp_pts = pts;
+    ctx->h_grp_n = gn;
+    TRY(dalloc(ctx, &ctx->d_npp_pts, std::max<size_t>(pts.size(), 4)));
+    TRY(dalloc(ctx, &ctx->d_grp_start, std::max<size_t>(gstart.size(), 1)));
+    TRY(dalloc(ctx, &ctx->d_grp_n, std::max<size_t>(gn.size(), 1)));
+    TRY(dalloc(ctx, &ctx->d_write_ok, std::max<size_t>(ok.size(), 1)));
+    TRY(dalloc(ctx, &ctx->d_npp_buf, 6 * std::max(np, 1)));
+    if (np) {
+        CK(hipMemcpy(ctx->d_npp_pts, pts.data(), 4 * pts.size(), hipMemcpyHostToDevice));
+        CK(hipMemcpy(ctx->d_grp_start, gstart.data(), 4 * gstart.size(), hipMemcpyHostToDevice));
+        CK(hipMemcpy(ctx->d_grp_n, gn.data(), 4 * gn.size(), hipMemcpyHostToDevice));
+        CK(hipMemcpy(ctx->d_write_ok, ok.data(), 4 * ok.size(), hipMemcpyHostToDevice));
+    }
+    ctx->setup_done = true;
+    return WCNS_OK;
+}
+
+static int field_ptr(wcns_ctx *ctx, int which, double **p, int *ncomp)
+{
+    *ncomp = 1;
+    switch (which) {
+    case 0: case 1: case 2: case 3: case 4: *p = ctx->prim[ctx->cur][which]; return 0;
+    case 5: *p = ctx->t; return 0;
+    case 7: *p = ctx->dtdt; return 0;
+    case 8: case 9: case 10: *p = ctx->sr[which - 8]; return 0;
+    case 11: case 12: case 13: *p = ctx->srv[which - 11]; return 0;
+    case 14: *p = ctx->visl; return 0;
+    case 16: *p = ctx->metric[9]; return 0;
+    case 100: *p = ctx->dq; *ncomp = 5; return 0;
+    case 102: *p = ctx->rhs; *ncomp = 5; return 0;
+    default:
+        if (which >= 17 && which <= 25) { *p = ctx->metric[which - 17]; return 0; }
+        return WCNS_E_ARG;
+    }
+}
+
+int wcns_set_prim(wcns_ctx *ctx, int nb, const double *r, const double *u, const double *v, const double *w,
+                  const double *p)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (nb < 0 || nb >= (int)ctx->blocks.size()) return WCNS_E_ARG;
+    HBlock &b = ctx->blocks[nb];
+    const double *src[5] = {r, u, v, w, p};
+    for (int bb = 0; bb < 2; bb++)
+        for (int m = 0; m < 5; m++)
+            CK(hipMemcpy(ctx->prim[bb][m] + b.db.off, src[m], 8 * b.g3, hipMemcpyHostToDevice));
+    // initial_t (sub_tgh.f90:36-58): temperature on the interior for nvis == 1
+    std::vector<double> t(b.g3, 0.0);
+    if (ctx->prm.nvis == 1) {
+        double moo2 = ctx->prm.moo * ctx->prm.moo;
+        for (int k = 1; k <= b.nk; k++) for (int j = 1; j <= b.nj; j++) for (int i = 1; i <= b.ni; i++) {
+            long q = gidx(b.db, i, j, k) - b.db.off;
+            t[q] = moo2 * (ctx->prm.gama * p[q] / r[q]);
+        }
+    }
+    CK(hipMemcpy(ctx->t + b.db.off, t.data(), 8 * b.g3, hipMemcpyHostToDevice));
+    return WCNS_OK;
+}
+
+int wcns_get_field(wcns_ctx *ctx, int nb, int which, double *out)
+{
+    if (!ctx || !ctx->setup_done || !out) return WCNS_E_STATE;
+    if (nb < 0 || nb >= (int)ctx->blocks.size()) return WCNS_E_ARG;
+    double *p;
+    int nc;
+    if (field_ptr(ctx, which, &p, &nc)) return fail(ctx, WCNS_E_ARG, "unknown field");
+    HBlock &b = ctx->blocks[nb];
+    CK(hipStreamSynchronize(ctx->stream));
+    for (int m = 0; m < nc; m++)
+        CK(hipMemcpy(out + m * b.g3, p + m * ctx->total + b.db.off, 8 * b.g3, hipMemcpyDeviceToHost));
+    return WCNS_OK;
+}
+
+int wcns_set_field(wcns_ctx *ctx, int nb, int which, const double *in)
+{
+    if (!ctx || !ctx->setup_done || !in) return WCNS_E_STATE;
+    if (nb < 0 || nb >= (int)ctx->blocks.size()) return WCNS_E_ARG;
+    double *p;
+    int nc;
+    if (field_ptr(ctx, which, &p, &nc)) return fail(ctx, WCNS_E_ARG, "unknown field");
+    HBlock &b = ctx->blocks[nb];
+    CK(hipStreamSynchronize(ctx->stream));
+    for (int m = 0; m < nc; m++)
+        CK(hipMemcpy(p + m * ctx->total + b.db.off, in + m * b.g3, 8 * b.g3, hipMemcpyHostToDevice));
+    return WCNS_OK;
+}
+
+// ---------------------------------------------------------------- stages
+int wcns_boundary_all(wcns_ctx *ctx)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    DFields F = ctx->fields();
+    std::vector<DBlock> dbs;
+    for (auto &b : ctx->blocks) dbs.push_back(b.db);
+    for (auto &b : ctx->blocks)
+        for (int r : b.bcorder)
+            if (launch_bc_region(b.regs[r].d, dbs.data(), F, ctx->dp, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "bc launch");
+    return WCNS_OK;
+}
+
+int wcns_initial_q_dq_c_t(wcns_ctx *ctx)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    DFields F = ctx->fields();
+    for (auto &b : ctx->blocks)
+        if (launch_init_stage(b.db, F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "init launch");
+    return WCNS_OK;
+}
+
+int wcns_timestep(wcns_ctx *ctx)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    DFields F = ctx->fields();
+    for (auto &b : ctx->blocks)
+        if (launch_timestep(b.db, F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "timestep launch");
+    return WCNS_OK;
+}
+
+int wcns_r_h_s_vis(wcns_ctx *ctx)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (ctx->prm.nvis != 1) return WCNS_OK;
+    return fail(ctx, WCNS_E_UNSUPPORTED, "viscous terms not built yet");
+}
+
+int wcns_r_h_s_invis(wcns_ctx *ctx)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    DFields F = ctx->fields();
+    for (auto &b : ctx->blocks)
+        if (launch_rhs_inv(b.db, F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "rhs launch");
+    return WCNS_OK;
+}
+
+static int communicate(wcns_ctx *ctx, int pv)
+{
+    DFields F = ctx->fields();
+    std::vector<DBlock> dbs;
+    for (auto &b : ctx->blocks) dbs.push_back(b.db);
+    int nvar = pv ? 6 : 5;
+    if (launch_npp_gather(ctx->d_npp_pts, ctx->npp, ctx->d_blocks, F, ctx->d_npp_buf, nvar, !pv, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "npp gather");
+    for (auto &g : ctx->iface)
+        if (launch_avg2pm(g, dbs.data(), F, pv, ctx->stream)) return fail(ctx, WCNS_E_HIP, "avg2pm");
+    if (launch_npp_assign(ctx->d_grp_start, ctx->d_grp_n, ctx->d_npp_pts, ctx->d_write_ok, ctx->ngroups,
+                          ctx->d_npp_buf, ctx->d_blocks, F, nvar, !pv, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "npp assign");
+    return WCNS_OK;
+}
+
+int wcns_communicate_dq(wcns_ctx *ctx)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    return communicate(ctx, 0);
+}
+
+int wcns_communicate_pv(wcns_ctx *ctx)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    return communicate(ctx, 1);
+}
+
+int wcns_l_h_s(wcns_ctx *ctx)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    DFields F = ctx->fields();
+    std::vector<DBlock> dbs;
+    for (auto &b : ctx->blocks) dbs.push_back(b.db);
+    for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
+        const DBlock &db = ctx->blocks[nb].db;
+        for (auto &g : ctx->phys)
+            if (g.nb == (int)nb) launch_dq_zero_face(g, db, F.rhs, F.total, ctx->stream);
+        if (launch_sweeps(db, F, ctx->dp, ctx->lhs_prec, ctx->stream)) return fail(ctx, WCNS_E_HIP, "sweeps");
+        for (auto &g : ctx->phys)
+            if (g.nb == (int)nb) launch_dq_zero_face(g, db, F.dq, F.total, ctx->stream);
+        CK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(unsigned long long), ctx->stream));
+        if (launch_update(db, F, ctx->dp, ctx->d_flag, ctx->flag_cap, ctx->stream)) return fail(ctx, WCNS_E_HIP, "update");
+        if (launch_repair(db, F, ctx->dp, ctx->d_flag, ctx->flag_cap, ctx->stream)) return fail(ctx, WCNS_E_HIP, "repair");
+    }
+    ctx->cur ^= 1;
+    return WCNS_OK;
+}
+
+int wcns_residual(wcns_ctx *ctx, double *rms, double *resmax, int loc[5])
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    DFields F = ctx->fields();
+    std::vector<double> res(3 * ctx->blocks.size());
+    for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
+        const DBlock &db = ctx->blocks[nb].db;
+        int np = residual_parts(db);
+        launch_residual(db, F, ctx->d_partial, np, ctx->stream);
+        launch_residual_final(ctx->d_partial, np, ctx->d_resblk + 3 * nb, ctx->stream);
+    }
+    CK(hipMemcpyAsync(res.data(), ctx->d_resblk, 8 * res.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    TRY(check_counters(ctx));
+    double sum = 0.0, mx = 0.0;
+    long npt = 0;
+    int l[5] = {2, 2, 2, 1, 1};
+    for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
+        const HBlock &b = ctx->blocks[nb];
+        sum = sum + res[3 * nb];
+        npt += (long)(b.ni - 2) * (b.nj - 2) * (b.nk - 2) * 5;
+        if (mx < res[3 * nb + 1]) {
+            mx = res[3 * nb + 1];
+            long a = (long)res[3 * nb + 2];
+            int m = (int)(a % 5);
+            long t = a / 5;
+            int mi = b.ni - 2, mj = b.nj - 2;
+            l[0] = (int)(t % mi) + 2; l[1] = (int)((t / mi) % mj) + 2; l[2] = (int)(t / ((long)mi * mj)) + 2;
+            l[3] = m + 1; l[4] = (int)nb + 1;
+        }
+    }
+    if (rms) *rms = sqrt(sum / (double)npt);
+    if (resmax) *resmax = mx;
+    if (loc) memcpy(loc, l, sizeof l);
+    return WCNS_OK;
+}
+
+int wcns_step(wcns_ctx *ctx, int nsteps, double *rms_hist)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (nsteps < 0) return WCNS_E_ARG;
+    for (int s = 0; s < nsteps; s++) {
+        TRY(wcns_boundary_all(ctx));
+        TRY(wcns_initial_q_dq_c_t(ctx));
+        TRY(wcns_timestep(ctx));
+        if (ctx->prm.nvis == 1) {
+            TRY(wcns_r_h_s_vis(ctx));
+            TRY(wcns_communicate_dq(ctx));
+        }
+        TRY(wcns_r_h_s_invis(ctx));
+        TRY(wcns_communicate_dq(ctx));
+        TRY(wcns_l_h_s(ctx));
+        TRY(wcns_communicate_pv(ctx));
+        if (rms_hist) {
+            double rms, rmax;
+            int loc[5];
+            TRY(wcns_residual(ctx, &rms, &rmax, loc));
+            rms_hist[s] = rms;
+        }
+    }
+    if (!rms_hist) {
+        CK(hipStreamSynchronize(ctx->stream));
+        TRY(check_counters(ctx));
+    }
+    return WCNS_OK;
+}
+
+int wcns_get_counter(wcns_ctx *ctx, int which, long long *value)
+{
+    if (!ctx || !value || which < 0 || which > 2) return WCNS_E_ARG;
+    if (!ctx->setup_done) return WCNS_E_STATE;
+    unsigned long long c[8];
+    CK(hipStreamSynchronize(ctx->stream));
+    CK(hipMemcpy(c, ctx->counters, sizeof c, hipMemcpyDeviceToHost));
+    *value = (long long)c[which];
+    return WCNS_OK;
+}
+
+int wcns_sync(wcns_ctx *ctx)
+{
+    if (!ctx) return WCNS_E_ARG;
+    CK(hipStreamSynchronize(ctx->stream));
+    return WCNS_OK;
+}
+
+int wcns_region_images(wcns_ctx *ctx, int nb, int nr, int *img, int *jmg, int *kmg, int box[6])
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (nb < 0 || nb >= (int)ctx->blocks.size() || nr < 0 || nr >= (int)ctx->blocks[nb].regs.size())
+        return WCNS_E_ARG;
+    const DRegion &g = ctx->blocks[nb].regs[nr].d;
+    for (int m = 0; m < 3; m++) { box[m] = g.s_lo[m]; box[3 + m] = g.s_hi[m]; }
+    if (g.bctype >= 0) return 1;
+    int o = 0;
+    for (int k = g.s_lo[2]; k <= g.s_hi[2]; k++)
+        for (int j = g.s_lo[1]; j <= g.s_hi[1]; j++)
+            for (int i = g.s_lo[0]; i <= g.s_hi[0]; i++) {
+                int it, jt, kt;
+                region_image(g, i, j, k, it, jt, kt);
+                long q = (i - g.s_lo[0]) + (long)(g.s_hi[0] - g.s_lo[0] + 1) *
+                                               ((j - g.s_lo[1]) + (long)(g.s_hi[1] - g.s_lo[1] + 1) * (k - g.s_lo[2]));
+                img[q] = it; jmg[q] = jt; kmg[q] = kt;
+                o++;
+            }
+    return WCNS_OK;
+}
+
+int wcns_multipoint(wcns_ctx *ctx, int *ngroups, int *npts, int *pts, int cap)
+{
+    if (!ctx || !ctx->setup_done || !ngroups) return WCNS_E_STATE;
+    *ngroups = ctx->ngroups;
+    for (int g = 0; g < ctx->ngroups; g++) if (npts) npts[g] = ctx->h_grp_n[g];
+    int n = (int)ctx->h_npp_pts.size();
+    for (int a = 0; a < n && a < cap; a++) {
+        int v = ctx->h_npp_pts[a];
+        pts[a] = (a % 4 == 0) ? v + 1 : v;     // block 1-based like the reference
+    }
+    return n / 4;
+}
+
+}  // extern "C"

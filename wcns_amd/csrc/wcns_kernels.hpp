@@ -1,0 +1,62 @@
+// wcns_kernels.hpp -- launchers of the gfx950 kernels (one translation unit
+// per kernel family); all take plain structs by value and a stream.
+#pragma once
+#include "wcns_device.hpp"
+
+namespace wcns {
+
+// Boundary region as the kernels see it.  Face box s_lo..s_hi (1-based,
+// inclusive), outward direction l3, and for interfaces the affine map
+// image = t0 + M*(idx - s0) of src/parallel.f90:1185-1301.
+struct DRegion {
+    int bctype, nb, nbt, s_nd;
+    int s_lo[3], s_hi[3], l3[3], t_l3[3];
+    int s0[3], t0[3], M[9];
+    int s_lr;
+    int npts;                 // face points
+    int serial;               // 1: reads overlap writes inside this region -> one thread
+    double nrm[3];            // symmetry plane normal (BC 3, BC.f90:381-396)
+};
+
+__host__ __device__ inline void region_point(const DRegion &g, int t, int &i, int &j, int &k)
+{
+    int bi = g.s_hi[0] - g.s_lo[0] + 1, bj = g.s_hi[1] - g.s_lo[1] + 1;
+    i = g.s_lo[0] + t % bi;
+    j = g.s_lo[1] + (t / bi) % bj;
+    k = g.s_lo[2] + t / (bi * bj);
+}
+
+__host__ __device__ inline void region_image(const DRegion &g, int i, int j, int k, int &it, int &jt, int &kt)
+{
+    int d0 = i - g.s0[0], d1 = j - g.s0[1], d2 = k - g.s0[2];
+    it = g.t0[0] + g.M[0] * d0 + g.M[1] * d1 + g.M[2] * d2;
+    jt = g.t0[1] + g.M[3] * d0 + g.M[4] * d1 + g.M[5] * d2;
+    kt = g.t0[2] + g.M[6] * d0 + g.M[7] * d1 + g.M[8] * d2;
+}
+
+int launch_rhs_inv(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
+
+int launch_bc_region(const DRegion &g, const DBlock *blocks, const DFields &F, const DParams &P, hipStream_t s);
+int launch_init_stage(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
+int launch_timestep(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
+int launch_zero_rhs(const DBlock &b, const DFields &F, hipStream_t s);
+int launch_dq_zero_face(const DRegion &g, const DBlock &b, double *arr, long total, hipStream_t s);
+int launch_npp_gather(const int *pts, int npts, const DBlock *dblocks, const DFields &F, double *buf,
+                      int nvar, int use_vol, hipStream_t s);
+int launch_avg2pm(const DRegion &g, const DBlock *blocks, const DFields &F, int pv, hipStream_t s);
+int launch_npp_assign(const int *grp_start, const int *grp_n, const int *pts, const int *write_ok, int ngroups,
+                      const double *buf, const DBlock *dblocks, const DFields &F, int nvar, int use_vol,
+                      hipStream_t s);
+int launch_sweeps(const DBlock &b, const DFields &F, const DParams &P, int prec, hipStream_t s);
+int launch_update(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s);
+int launch_repair(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s);
+int launch_residual(const DBlock &b, const DFields &F, double *partial, int nparts, hipStream_t s);
+int residual_parts(const DBlock &b);
+int launch_residual_final(const double *partial, int nparts, double *out, hipStream_t s);
+int launch_copy_interior(const DBlock &b, const double *src, double *dst, hipStream_t s);
+
+// setup
+int launch_gcl_metrics(const DBlock &b, const double *x, const double *y, const double *z, DFields F,
+                       double *work, hipStream_t s);
+
+}  // namespace wcns

@@ -1,0 +1,758 @@
+// wcns_step.hip -- per-step kernels around the inviscid residual: boundary
+// ghost fill, speed of sound / temperature, spectral radii and local time
+// step, interface averaging, LU-SGS + SGS sweeps, limited update and the
+// residual monitor.
+#include "wcns_kernels.hpp"
+
+namespace wcns {
+
+namespace {
+
+// ------------------------------------------------------------------ BCs
+__device__ inline void setp(const DFields &F, long g, double r, double u, double v, double w, double p)
+{
+    F.r[g] = r; F.u[g] = u; F.v[g] = v; F.w[g] = w; F.p[g] = p;
+}
+
+// Sub1.f90:750-780 getnxyz_mml
+__device__ inline void getnxyz(const DFields &F, long g, const int *l3, double &nx, double &ny, double &nz)
+{
+    double x = 0, y = 0, z = 0;
+    if (l3[0]) { x = x + F.kcx[g]; y = y + F.kcy[g]; z = z + F.kcz[g]; }
+    if (l3[1]) { x = x + F.etx[g]; y = y + F.ety[g]; z = z + F.etz[g]; }
+    if (l3[2]) { x = x + F.ctx[g]; y = y + F.cty[g]; z = z + F.ctz[g]; }
+    double s = sqrt(x * x + y * y + z * z);
+    s = s < SML_SSS ? SML_SSS : s;
+    nx = x / s; ny = y / s; nz = z / s;
+}
+
+__device__ void bc_point(const DRegion &g, const DBlock &b, const DBlock &bt, const DFields &F,
+                         const DParams &P, int i, int j, int k)
+{
+    const int *l3 = g.l3;
+    long g0 = gidx(b, i, j, k);
+    long s1 = gidx(b, i + l3[0], j + l3[1], k + l3[2]);
+    long s2 = gidx(b, i + 2 * l3[0], j + 2 * l3[1], k + 2 * l3[2]);
+    long s3 = gidx(b, i + 3 * l3[0], j + 3 * l3[1], k + 3 * l3[2]);
+    long t1 = gidx(b, i - l3[0], j - l3[1], k - l3[2]);
+    long t2 = gidx(b, i - 2 * l3[0], j - 2 * l3[1], k - 2 * l3[2]);
+    long t3 = gidx(b, i - 3 * l3[0], j - 3 * l3[1], k - 3 * l3[2]);
+    const double gama = P.gama;
+    if (g.bctype < 0) {
+        // Sub2.f90:206-322 boundary_n1_vir3 (cic1 = 0)
+        int it, jt, kt;
+        region_image(g, i, j, k, it, jt, kt);
+        const int *tl = g.t_l3;
+        long q1 = gidx(bt, it - tl[0], jt - tl[1], kt - tl[2]);
+        long q2 = gidx(bt, it - 2 * tl[0], jt - 2 * tl[1], kt - 2 * tl[2]);
+        long q3 = gidx(bt, it - 3 * tl[0], jt - 3 * tl[1], kt - 3 * tl[2]);
+        setp(F, s1, F.r[q1], F.u[q1], F.v[q1], F.w[q1], F.p[q1]);
+        setp(F, s2, F.r[q2], F.u[q2], F.v[q2], F.w[q2], F.p[q2]);
+        setp(F, s3, F.r[q3], F.u[q3], F.v[q3], F.w[q3], F.p[q3]);
+        return;
+    }
+    switch (g.bctype) {
+    case 4: {   // BC.f90:460-607 boundary_4
+        double nx, ny, nz;
+        getnxyz(F, g0, l3, nx, ny, nz);
+        double rin = F.r[t1], uin = F.u[t1], vin = F.v[t1], win = F.w[t1], pin = F.p[t1];
+        double s_in = pin / pow(rin, gama);
+        double s_oo = P.poo / pow(P.roo, gama);
+        double cin = sqrt(gama * pin / rin);
+        double sq = sqrt(nx * nx + ny * ny + nz * nz);
+        sq = sq < SML_SSS ? SML_SSS : sq;
+        double cgm1 = (double)g.s_lr / sq;
+        double nxa = nx * cgm1, nya = ny * cgm1, nza = nz * cgm1;
+        double vnoo = nxa * P.uoo + nya * P.voo + nza * P.woo;
+        double vnin = nxa * uin + nya * vin + nza * win;
+        double mach = sqrt(uin * uin + vin * vin + win * win) / cin;
+        double rb, ub, vb, wb, pb;
+        if (mach < 1.0) {
+            double rp = vnin + 2.0 * cin / (gama - 1.0);
+            double rn = vnoo - 2.0 * P.coo / (gama - 1.0);
+            double vnb = 0.5 * (rp + rn);
+            double ab = 0.25 * (gama - 1.0) * (rp - rn);
+            double sb, uref, vref, wref, vnref;
+            if (vnb > 0.0) { sb = s_in; uref = uin; vref = vin; wref = win; vnref = vnin; }
+            else { sb = s_oo; uref = P.uoo; vref = P.voo; wref = P.woo; vnref = vnoo; }
+            rb = pow(ab * ab / gama / sb, 1.0 / (gama - 1.0));
+            ub = uref + nxa * (vnb - vnref);
+            vb = vref + nya * (vnb - vnref);
+            wb = wref + nza * (vnb - vnref);
+            pb = sb * pow(rb, gama);
+        } else if (vnin > 0.0) {
+            rb = rin; ub = uin; vb = vin; wb = win; pb = pin;
+        } else {
+            rb = P.roo; ub = P.uoo; vb = P.voo; wb = P.woo; pb = P.poo;
+        }
+        setp(F, g0, rb, ub, vb, wb, pb);
+        setp(F, s1, rb, ub, vb, wb, pb);
+        setp(F, s2, rb, ub, vb, wb, pb);
+        setp(F, s3, -rb, ub, vb, wb, pb);
+        break;
+    }
+    case 5:     // BC.f90:609-665 boundary_5
+        setp(F, g0, P.roo, P.uoo, P.voo, P.woo, P.poo);
+        setp(F, s1, P.roo, P.uoo, P.voo, P.woo, P.poo);
+        setp(F, s2, P.roo, P.uoo, P.voo, P.woo, P.poo);
+        setp(F, s3, P.roo, P.uoo, P.voo, P.woo, P.poo);
+        break;
+    case 6: {   // BC.f90:667-749 boundary_6
+        double rm = F.r[t1], um = F.u[t1], vm = F.v[t1], wm = F.w[t1], pm = F.p[t1];
+        setp(F, g0, rm, um, vm, wm, pm);
+        setp(F, s1, rm, um, vm, wm, pm);
+        setp(F, s2, rm, um, vm, wm, pm);
+        setp(F, s3, rm, um, vm, wm, pm);
+        break;
+    }
+    case 2:
+        if (P.nvis == 1) {   // BC.f90:1244-1409 viscid_wall
+            double moocp = P.moo * P.moo * gama;
+            setp(F, s1, F.r[t1], -F.u[t1], -F.v[t1], -F.w[t1], F.p[t1]);
+            double pres, tres;
+            if (P.twall_fixed) {
+                pres = (4.0 * F.p[t1] - F.p[t2]) / 3.0;
+                if (pres < PMIN_LIMIT) pres = F.p[t1];
+                tres = P.twall_nd;
+            } else {
+                pres = (4.0 * F.p[t1] - F.p[t2]) / 3.0;
+                tres = (4.0 * F.t[t1] - F.t[t2]) / 3.0;
+                if (pres < PMIN_LIMIT) pres = F.p[t1];
+                if (tres < TMIN_LIMIT) tres = F.t[t1];
+            }
+            double dres = moocp * pres / tres;
+            setp(F, g0, dres, 0.0, 0.0, 0.0, pres);
+            setp(F, s2, F.r[t2], -F.u[t2], -F.v[t2], -F.w[t2], F.p[t2]);
+            setp(F, s3, -F.r[t3], -F.u[t3], -F.v[t3], -F.w[t3], F.p[t3]);
+        } else {             // BC.f90:1412-1486 inviscid_wall + dif_average
+            double nx, ny, nz;
+            getnxyz(F, g0, l3, nx, ny, nz);
+            long ss[3] = {s1, s2, s3}, tt[3] = {t1, t2, t3};
+            for (int l = 0; l < 3; l++) {
+                long a = tt[l];
+                double sss1 = 2.0 * (F.u[a] * nx + F.v[a] * ny + F.w[a] * nz);
+                setp(F, ss[l], l == 2 ? -F.r[a] : F.r[a], F.u[a] - sss1 * nx, F.v[a] - sss1 * ny,
+                     F.w[a] - sss1 * nz, F.p[a]);
+            }
+            setp(F, g0, 0.5 * (F.r[s1] + F.r[t1]), 0.5 * (F.u[s1] + F.u[t1]), 0.5 * (F.v[s1] + F.v[t1]),
+                 0.5 * (F.w[s1] + F.w[t1]), 0.5 * (F.p[s1] + F.p[t1]));
+        }
+        break;
+    case 3: {   // BC.f90:227-457 boundary_3, 3-D branch (plane normal from the face centre)
+        double nx = g.nrm[0], ny = g.nrm[1], nz = g.nrm[2];
+        long ss[3] = {s1, s2, s3}, tt[3] = {t1, t2, t3};
+        for (int l = 0; l < 3; l++) {
+            long a = tt[l];
+            double sss1 = 2.0 * (nx * F.u[a] + ny * F.v[a] + nz * F.w[a]);
+            setp(F, ss[l], F.r[a], F.u[a] - sss1 * nx, F.v[a] - sss1 * ny, F.w[a] - sss1 * nz, F.p[a]);
+        }
+        setp(F, g0, 0.5 * (F.r[s1] + F.r[t1]), 0.5 * (F.u[s1] + F.u[t1]), 0.5 * (F.v[s1] + F.v[t1]),
+             0.5 * (F.w[s1] + F.w[t1]), 0.5 * (F.p[s1] + F.p[t1]));
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+__global__ void k_bc_region(DRegion g, DBlock b, DBlock bt, DFields F, DParams P)
+{
+    if (g.serial) {
+        if (blockIdx.x != 0 || threadIdx.x != 0) return;
+        for (int t = 0; t < g.npts; t++) {
+            int i, j, k;
+            region_point(g, t, i, j, k);
+            bc_point(g, b, bt, F, P, i, j, k);
+        }
+        return;
+    }
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.npts) return;
+    int i, j, k;
+    region_point(g, t, i, j, k);
+    bc_point(g, b, bt, F, P, i, j, k);
+}
+
+// --------------------------------------------------------- init stage
+// set_dq_to_0 (PreProcess.f90:1189-1205) on the residual; with nvis == 1 also
+// get_c_t_only's temperature (RHS.f90:307-409) and compute_visl_ns
+// (GasModel.f90:219-235).
+__global__ void k_init_interior(DBlock b, DFields F, DParams P)
+{
+    long n = (long)b.ni * b.nj * b.nk;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
+    long g = gidx(b, i, j, k);
+#pragma unroll
+    for (int m = 0; m < 5; m++) F.rhs[m * F.total + g] = 0.0;
+    if (P.nvis == 1) {
+        double moo2 = P.moo * P.moo;
+        double a2 = P.gama * F.p[g] / F.r[g];
+        double tm = moo2 * a2;
+        F.t[g] = tm;
+        F.visl[g] = tm * sqrt(tm) * (1.0 + P.visc) / (tm + P.visc);
+    }
+}
+
+__global__ void k_init_ghost_t(DBlock b, DFields F, DParams P, int dir)
+{
+    int na = dir == 0 ? b.nj : b.ni, nb = dir == 2 ? b.nj : b.nk;
+    int nd = dir == 0 ? b.ni : (dir == 1 ? b.nj : b.nk);
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2L * na * nb) return;
+    int side = (int)(t % 2);
+    long r = t / 2;
+    int ia = (int)(r % na) + 1, ib = (int)(r / na) + 1;
+    double moo2 = P.moo * P.moo, gama = P.gama;
+    int l1 = side == 0 ? 0 : nd + 1, l2 = side == 0 ? -1 : nd + 2, l3 = side == 0 ? -2 : nd + 3;
+    auto at = [&](int l) {
+        int i = dir == 0 ? l : ia, j = dir == 1 ? l : (dir == 0 ? ia : ib), k = dir == 2 ? l : ib;
+        return gidx(b, i, j, k);
+    };
+    long q1 = at(l1), q2 = at(l2), q3 = at(l3);
+    double a2 = gama * F.p[q1] / F.r[q1];
+    F.t[q1] = moo2 * a2;
+    a2 = gama * F.p[q2] / F.r[q2];
+    F.t[q2] = moo2 * a2;
+    if (dir != 0) a2 = gama * F.p[q3] / fabs(F.r[q3]);
+    F.t[q3] = moo2 * a2;
+}
+
+// ---------------------------------------------------- spectral radius, dt
+// Convect.f90:1338-1461 spectinv (interior) + LHS.f90:1343-1377 localdt0
+__global__ void k_timestep(DBlock b, DFields F, DParams P)
+{
+    long n = (long)b.ni * b.nj * b.nk;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
+    long g = gidx(b, i, j, k);
+    double um = F.u[g], vm = F.v[g], wm = F.w[g];
+    double cm = sqrt(P.gama * F.p[g] / F.r[g]);
+    double kx = F.kcx[g], ky = F.kcy[g], kz = F.kcz[g];
+    double ex = F.etx[g], ey = F.ety[g], ez = F.etz[g];
+    double cx = F.ctx[g], cy = F.cty[g], cz = F.ctz[g];
+    double cita = kx * um + ky * vm + kz * wm + 0.0;
+    double citb = ex * um + ey * vm + ez * wm + 0.0;
+    double citc = cx * um + cy * vm + cz * wm + 0.0;
+    double sra = fabs(cita) + cm * sqrt(kx * kx + ky * ky + kz * kz);
+    double srb = fabs(citb) + cm * sqrt(ex * ex + ey * ey + ez * ez);
+    double src = fabs(citc) + cm * sqrt(cx * cx + cy * cy + cz * cz);
+    F.sra[g] = sra;
+    F.srb[g] = srb;
+    F.src[g] = src;
+    double rv = F.srva[g] + F.srvb[g] + F.srvc[g];
+    double rabc = sra + srb + src + rv;
+    double dt;
+    if (P.timedt_rate > SMALL) {
+        double a = P.cfl / rabc, c = P.timedt_rate / F.vol[g];
+        dt = a < c ? a : c;
+    } else {
+        dt = P.cfl / rabc;
+    }
+    F.dtdt[g] = dt;
+}
+
+// ------------------------------------------------- interface averaging
+__global__ void k_dq_zero_face(DRegion g, DBlock b, double *arr, long total)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.npts) return;
+    int i, j, k;
+    region_point(g, t, i, j, k);
+    long q = gidx(b, i, j, k);
+#pragma unroll
+    for (int m = 0; m < 5; m++) arr[m * total + q] = 0.0;
+}
+
+// parallel.f90:3028-3036 / 3174-3188: pre-averaging values of the multi-points
+__global__ void k_npp_gather(const int *pts, int npts, const DBlock *blocks, DFields F, double *buf,
+                             int nvar, int use_vol)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts) return;
+    const int *p = pts + 4 * t;
+    const DBlock b = blocks[p[0]];
+    long q = gidx(b, p[1], p[2], p[3]);
+    if (use_vol) {
+        double vol_p = F.vol[q];
+        for (int m = 0; m < 5; m++) buf[nvar * t + m] = F.rhs[m * F.total + q] / vol_p;
+    } else {
+        double vol_p = 1.0;
+        buf[nvar * t + 0] = F.r[q] / vol_p;
+        buf[nvar * t + 1] = F.u[q] / vol_p;
+        buf[nvar * t + 2] = F.v[q] / vol_p;
+        buf[nvar * t + 3] = F.w[q] / vol_p;
+        buf[nvar * t + 4] = F.p[q] / vol_p;
+        buf[nvar * t + 5] = F.t[q] / vol_p;
+    }
+}
+
+// parallel.f90:3060-3120 boundary_match_dq_2pm / :3217-3280 boundary_match_pv_2pm
+__device__ inline void avg2pm_point(const DRegion &g, const DBlock &b, const DBlock &bt, const DFields &F,
+                                    int pv, int i, int j, int k)
+{
+    int it, jt, kt;
+    region_image(g, i, j, k, it, jt, kt);
+    long qs = gidx(b, i, j, k), qt = gidx(bt, it, jt, kt);
+    if (!pv) {
+        double vol_p = F.vol[qs] / F.vol[qt];
+#pragma unroll
+        for (int m = 0; m < 5; m++)
+            F.rhs[m * F.total + qs] = 0.5 * (F.rhs[m * F.total + qt] * vol_p + F.rhs[m * F.total + qs]);
+    } else {
+        F.r[qs] = 0.5 * (F.r[qt] + F.r[qs]);
+        F.u[qs] = 0.5 * (F.u[qt] + F.u[qs]);
+        F.v[qs] = 0.5 * (F.v[qt] + F.v[qs]);
+        F.w[qs] = 0.5 * (F.w[qt] + F.w[qs]);
+        F.p[qs] = 0.5 * (F.p[qt] + F.p[qs]);
+        F.t[qs] = 0.5 * (F.t[qt] + F.t[qs]);
+    }
+}
+
+__global__ void k_avg2pm(DRegion g, DBlock b, DBlock bt, DFields F, int pv)
+{
+    if (g.serial) {
+        if (blockIdx.x != 0 || threadIdx.x != 0) return;
+        for (int k = g.s_lo[2]; k <= g.s_hi[2]; k++)
+            for (int j = g.s_lo[1]; j <= g.s_hi[1]; j++)
+                for (int i = g.s_lo[0]; i <= g.s_hi[0]; i++) avg2pm_point(g, b, bt, F, pv, i, j, k);
+        return;
+    }
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.npts) return;
+    int i, j, k;
+    region_point(g, t, i, j, k);
+    avg2pm_point(g, b, bt, F, pv, i, j, k);
+}
+
+// parallel.f90:3122-3164 boundary_match_dq_3pp / :3282-3326 boundary_match_pv_3pp
+__global__ void k_npp_assign(const int *grp_start, const int *grp_n, const int *pts, const int *write_ok,
+                             int ngroups, const double *buf, const DBlock *blocks, DFields F, int nvar,
+                             int use_vol)
+{
+    int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ngroups) return;
+    int s0 = grp_start[gi], n = grp_n[gi];
+    double d[6] = {0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < n; s++)
+        for (int m = 0; m < nvar; m++) d[m] = d[m] + buf[nvar * (s0 + s) + m];
+    for (int m = 0; m < nvar; m++) d[m] = d[m] / (n * 1.0);
+    for (int s = 0; s < n; s++) {
+        if (!write_ok[s0 + s]) continue;
+        const int *p = pts + 4 * (s0 + s);
+        const DBlock b = blocks[p[0]];
+        long q = gidx(b, p[1], p[2], p[3]);
+        if (use_vol) {
+            double vol_p = F.vol[q];
+            for (int m = 0; m < 5; m++) F.rhs[m * F.total + q] = d[m] * vol_p;
+        } else {
+            F.r[q] = d[0] * 1.0; F.u[q] = d[1] * 1.0; F.v[q] = d[2] * 1.0;
+            F.w[q] = d[3] * 1.0; F.p[q] = d[4] * 1.0; F.t[q] = d[5] * 1.0;
+        }
+    }
+}
+
+// ------------------------------------------------------------ sweeps
+// Node (i,j,k) of a sweep reads its lower neighbours (i-1,j,k), (i,j-1,k),
+// (i,j,k-1) and its upper ones.  Processing the hyperplanes i+j+k = s in
+// order, in place, gives every node exactly the operands of the reference's
+// lexicographic loops (LHS.f90:422-574, 1797-2043), so the sweeps below are
+// the reference's arithmetic node for node.
+struct NbrGeo {
+    double prim[5];
+    double gy[3];
+    double rad;
+};
+
+__device__ inline void load_prim(const DFields &F, long g, double *prim)
+{
+    prim[0] = F.r[g]; prim[1] = F.u[g]; prim[2] = F.v[g]; prim[3] = F.w[g]; prim[4] = F.p[g];
+}
+
+// Sub1.f90:676-717 getrkec_mml clamp + LHS.f90:647-678 getrarbrc clamp
+__device__ inline int clampi(int v, int n) { return v < 1 ? 1 : (v > n ? n : v); }
+
+template <typename T>
+__device__ inline void neighbour_term(const DBlock &b, const DFields &F, const DParams &P, int i, int j, int k,
+                                      int dir, int off, const T *dqn, T *out)
+{
+    int in = i + (dir == 0 ? off : 0), jn = j + (dir == 1 ? off : 0), kn = k + (dir == 2 ? off : 0);
+    long gp = gidx(b, in, jn, kn);
+    // metrics at the neighbour, ghost indices clamped to the face (getrkec_mml)
+    int im = in, jm = jn, km = kn;
+    if (im == 0) im = 1;
+    if (im == b.ni + 1) im = b.ni;
+    if (jm == 0) jm = 1;
+    if (jm == b.nj + 1) jm = b.nj;
+    if (km == 0) km = 1;
+    if (km == b.nk + 1) km = b.nk;
+    long gm = gidx(b, im, jm, km);
+    // spectral radius of the neighbour along dir, clamped to 1..n (getrarbrc)
+    long gr;
+    if (dir == 0) gr = gidx(b, clampi(i + off, b.ni), j, k);
+    else if (dir == 1) gr = gidx(b, i, clampi(j + off, b.nj), k);
+    else gr = gidx(b, i, j, clampi(k + off, b.nk));
+    double pr[5];
+    load_prim(F, gp, pr);
+    T prim[5], gy[3];
+#pragma unroll
+    for (int m = 0; m < 5; m++) prim[m] = (T)pr[m];
+    const double *mx = dir == 0 ? F.kcx : (dir == 1 ? F.etx : F.ctx);
+    const double *my = dir == 0 ? F.kcy : (dir == 1 ? F.ety : F.cty);
+    const double *mz = dir == 0 ? F.kcz : (dir == 1 ? F.etz : F.ctz);
+    gy[0] = (T)mx[gm]; gy[1] = (T)my[gm]; gy[2] = (T)mz[gm];
+    const double *sr = dir == 0 ? F.sra : (dir == 1 ? F.srb : F.src);
+    T rad = (T)sr[gr];
+    mxdq_std<T>(prim, gy, dqn, out, rad, off < 0 ? T(1.0) : T(-1.0), (T)P.gama);
+}
+
+// LHS.f90:602-645 getcoed
+__device__ inline double coed_at(const DFields &F, const DParams &P, long g)
+{
+    double rad_ns = F.sra[g] + F.srb[g] + F.src[g];
+    if (P.nvis >= 1) rad_ns = rad_ns + F.srva[g] + F.srvb[g] + F.srvc[g];
+    double ct = 1.0 / F.dtdt[g];
+    return 1.0 / (ct + 1.0 * 1.0 * rad_ns);
+}
+
+// kind 0 lusgs_l (LHS.f90:381-478), 1 lusgs_u (:480-577), 2 gs_pr_l (:1777-1866), 3 gs_pr_u (:1956-2046)
+template <int KIND>
+__global__ void __launch_bounds__(256) k_sweep_plane(DBlock b, DFields F, DParams P, int s)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int jlo = s - b.ni - b.nk, jhi = s - 2;
+    jlo = jlo < 1 ? 1 : jlo;
+    jhi = jhi > b.nj ? b.nj : jhi;
+    if (jhi < jlo) return;
+    long nj_ = jhi - jlo + 1;
+    if (t >= nj_ * b.nk) return;
+    int j = jlo + (int)(t % nj_);
+    int k = (int)(t / nj_) + 1;
+    int i = s - j - k;
+    if (i < 1 || i > b.ni) return;
+    long g = gidx(b, i, j, k);
+    double coed = coed_at(F, P, g);
+    const long T5 = F.total;
+    double *dq = F.dq;
+    double rhs0[5];
+    if (KIND == 0 || KIND == 1) {
+        int off = KIND == 0 ? -1 : 1;
+        double dn[3][5], fo[3][5];
+        long gn[3] = {gidx(b, i + off, j, k), gidx(b, i, j + off, k), gidx(b, i, j, k + off)};
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int m = 0; m < 5; m++) dn[d][m] = dq[m * T5 + gn[d]];
+#pragma unroll
+        for (int d = 0; d < 3; d++) neighbour_term<double>(b, F, P, i, j, k, d, off, dn[d], fo[d]);
+#pragma unroll
+        for (int m = 0; m < 5; m++) rhs0[m] = 1.0 * (fo[0][m] + fo[1][m] + fo[2][m]);
+        if (P.nvis > 0) {
+            double rva = F.srva[gn[0]], rvb = F.srvb[gn[1]], rvc = F.srvc[gn[2]];
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                double v = 0.5 * (rva * dn[0][m] + rvb * dn[1][m] + rvc * dn[2][m]);
+                rhs0[m] = KIND == 0 ? rhs0[m] + v : rhs0[m] - v;
+            }
+        }
+        if (KIND == 0) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) dq[m * T5 + g] = (-F.rhs[m * T5 + g] + rhs0[m]) * coed;
+        } else {
+#pragma unroll
+            for (int m = 0; m < 5; m++) dq[m * T5 + g] = dq[m * T5 + g] - rhs0[m] * coed;
+        }
+    } else {
+        double dl[3][5], du[3][5], fl[3][5], fu[3][5];
+        long gl[3] = {gidx(b, i - 1, j, k), gidx(b, i, j - 1, k), gidx(b, i, j, k - 1)};
+        long gu[3] = {gidx(b, i + 1, j, k), gidx(b, i, j + 1, k), gidx(b, i, j, k + 1)};
+        bool lo[3] = {i == 1, j == 1, k == 1}, hi[3] = {i == b.ni, j == b.nj, k == b.nk};
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                dl[d][m] = lo[d] ? 0.0 : dq[m * T5 + gl[d]];
+                du[d][m] = hi[d] ? 0.0 : dq[m * T5 + gu[d]];
+            }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            neighbour_term<double>(b, F, P, i, j, k, d, -1, dl[d], fl[d]);
+            neighbour_term<double>(b, F, P, i, j, k, d, 1, du[d], fu[d]);
+        }
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            double r0 = 1.0 * (fl[0][m] + fl[1][m] + fl[2][m] - fu[0][m] - fu[1][m] - fu[2][m]);
+            dq[m * T5 + g] = (r0 - F.rhs[m * T5 + g]) * coed;
+        }
+    }
+}
+
+// -------------------------------------------------------------- update
+// Sub2.f90:603-796 update_limited.  q is prim_to_q(prim) of the step's start
+// (get_q_only, RHS.f90:75-100), recomputed here instead of stored.  New
+// primitives go to the other buffer; nodes outside the limits are listed for
+// the ordered repair below.
+__global__ void k_update(DBlock b, DFields F, DParams P, int *flag_list, int cap)
+{
+    long n = (long)b.ni * b.nj * b.nk;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
+    long g = gidx(b, i, j, k);
+    const double gama = P.gama, gamam1 = gama - 1.0;
+    double r0 = F.r[g], u0 = F.u[g], v0 = F.v[g], w0 = F.w[g], p0 = F.p[g];
+    double q[5];
+    q[0] = r0; q[1] = r0 * u0; q[2] = r0 * v0; q[3] = r0 * w0; q[4] = getem(r0, u0, v0, w0, p0, gama);
+    double d[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) d[m] = F.dq[m * F.total + g];
+    double qt[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) qt[m] = q[m] + d[m];
+    double rm = qt[0], rm1 = 1.0 / rm;
+    double um = qt[1] * rm1, vm = qt[2] * rm1, wm = qt[3] * rm1, em = qt[4];
+    double pm = gamam1 * (em - 0.5 * rm * (um * um + vm * vm + wm * wm));
+    double dp = -0.3 + fabs((pm - p0) / p0);
+    double falpha = 1. / (1. + 2. * (0.0 > dp ? 0.0 : dp));
+#pragma unroll
+    for (int m = 0; m < 5; m++) qt[m] = q[m] + d[m] * falpha;
+    rm = qt[0]; rm1 = 1.0 / rm;
+    um = qt[1] * rm1; vm = qt[2] * rm1; wm = qt[3] * rm1; em = qt[4];
+    pm = gamam1 * (em - 0.5 * rm * (um * um + vm * vm + wm * wm));
+    F.rn[g] = rm; F.un[g] = um; F.vn[g] = vm; F.wn[g] = wm; F.pn[g] = pm;
+    if (pm <= PMIN_LIMIT || rm <= RMIN_LIMIT || pm >= PMAX_LIMIT || rm >= RMAX_LIMIT) {
+        unsigned long long slot = atomicAdd(&F.counters[3], 1ull);
+        if (slot < (unsigned long long)cap) flag_list[slot] = (int)t;
+    }
+}
+
+// Sub2.f90:661-767: a flagged node takes the mean of its six neighbours, those
+// earlier in lexicographic order already updated (new buffer), later ones and
+// ghosts as before the update (old buffer).  One thread, in order.
+__global__ void k_repair(DBlock b, DFields F, DParams P, int *flag_list, int cap)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    unsigned long long nf = F.counters[3];
+    if (nf == 0) return;
+    int n = (int)(nf < (unsigned long long)cap ? nf : (unsigned long long)cap);
+    for (int a = 1; a < n; a++) {          // insertion sort: lexicographic = linear order
+        int v = flag_list[a], c = a - 1;
+        while (c >= 0 && flag_list[c] > v) { flag_list[c + 1] = flag_list[c]; c--; }
+        flag_list[c + 1] = v;
+    }
+    for (int a = 0; a < n; a++) {
+        long t = flag_list[a];
+        int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
+        double rm = 0.0, um = 0.0, vm = 0.0, wm = 0.0, pm = 0.0;
+        int np = 0;
+        for (int ii = -1; ii <= 1; ii++)
+            for (int jj = -1; jj <= 1; jj++)
+                for (int kk = -1; kk <= 1; kk++) {
+                    if (abs(ii) + abs(jj) + abs(kk) != 1) continue;
+                    np++;
+                    int i0 = i + ii, j0 = j + jj, k0 = k + kk;
+                    long g2 = gidx(b, i0, j0, k0);
+                    bool interior = i0 >= 1 && i0 <= b.ni && j0 >= 1 && j0 <= b.nj && k0 >= 1 && k0 <= b.nk;
+                    bool earlier = (ii + jj + kk) < 0;
+                    bool use_new = interior && earlier;
+                    rm = rm + (use_new ? F.rn[g2] : F.r[g2]);
+                    um = um + (use_new ? F.un[g2] : F.u[g2]);
+                    vm = vm + (use_new ? F.vn[g2] : F.v[g2]);
+                    wm = wm + (use_new ? F.wn[g2] : F.w[g2]);
+                    pm = pm + (use_new ? F.pn[g2] : F.p[g2]);
+                }
+        double rnp = (double)np;
+        rm = rm / rnp; um = um / rnp; vm = vm / rnp; wm = wm / rnp; pm = pm / rnp;
+        long g = gidx(b, i, j, k);
+        F.rn[g] = rm; F.un[g] = um; F.vn[g] = vm; F.wn[g] = wm; F.pn[g] = pm;
+    }
+    F.counters[1] += nf;
+    // Sub2.f90:781-785: too many abnormal nodes stops the run
+    if (nf > (unsigned long long)(((long)b.ni * b.nj * b.nk) / 64) || nf >= 500ull) F.counters[4] = 1ull;
+}
+
+// ------------------------------------------------------------ residual
+// PostProcess.f90:1240-1293: sum and max of (|dq|/(dtdt*vol))^2 over nodes 2..n-1.
+constexpr int RES_THREADS = 256;
+
+__global__ void __launch_bounds__(RES_THREADS) k_residual(DBlock b, DFields F, double *partial)
+{
+    __shared__ double ssum[RES_THREADS];
+    __shared__ double smax[RES_THREADS];
+    __shared__ long sarg[RES_THREADS];
+    int mi = b.ni - 2, mj = b.nj - 2, mk = b.nk - 2;
+    long n = (long)mi * mj * mk;
+    double sum = 0.0, mx = 0.0;
+    long arg = -1;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        int i = (int)(t % mi) + 2, j = (int)((t / mi) % mj) + 2, k = (int)(t / ((long)mi * mj)) + 2;
+        long g = gidx(b, i, j, k);
+        double dtime1 = 1.0 / (F.dtdt[g] * F.vol[g]);
+        for (int m = 0; m < 5; m++) {
+            double dresm = fabs(F.dq[m * F.total + g]) * dtime1;
+            if (dresm > mx) { mx = dresm; arg = t * 5 + m; }
+            sum = sum + dresm * dresm;
+        }
+    }
+    ssum[threadIdx.x] = sum; smax[threadIdx.x] = mx; sarg[threadIdx.x] = arg;
+    __syncthreads();
+    for (int w = RES_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + w];
+            double om = smax[threadIdx.x + w];
+            long oa = sarg[threadIdx.x + w];
+            if (om > smax[threadIdx.x] || (om == smax[threadIdx.x] && oa >= 0 && (sarg[threadIdx.x] < 0 || oa < sarg[threadIdx.x]))) {
+                smax[threadIdx.x] = om; sarg[threadIdx.x] = oa;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[3 * blockIdx.x + 0] = ssum[0];
+        partial[3 * blockIdx.x + 1] = smax[0];
+        partial[3 * blockIdx.x + 2] = (double)sarg[0];
+    }
+}
+
+__global__ void k_residual_final(const double *partial, int nparts, double *out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0, mx = 0.0, arg = -1.0;
+    for (int p = 0; p < nparts; p++) {
+        s += partial[3 * p];
+        if (partial[3 * p + 1] > mx) { mx = partial[3 * p + 1]; arg = partial[3 * p + 2]; }
+    }
+    out[0] = s; out[1] = mx; out[2] = arg;
+}
+
+__global__ void k_zero_interior5(DBlock b, double *arr, long total)
+{
+    long n = (long)b.ni * b.nj * b.nk;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
+    long g = gidx(b, i, j, k);
+#pragma unroll
+    for (int m = 0; m < 5; m++) arr[m * total + g] = 0.0;
+}
+
+inline dim3 grid1(long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+}  // namespace
+
+int launch_bc_region(const DRegion &g, const DBlock *blocks, const DFields &F, const DParams &P, hipStream_t s)
+{
+    const DBlock &b = blocks[g.nb];
+    const DBlock &bt = blocks[g.bctype < 0 ? g.nbt : g.nb];
+    if (g.serial) hipLaunchKernelGGL(k_bc_region, dim3(1), dim3(64), 0, s, g, b, bt, F, P);
+    else hipLaunchKernelGGL(k_bc_region, grid1(g.npts, 128), dim3(128), 0, s, g, b, bt, F, P);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_init_stage(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
+{
+    long n = (long)b.ni * b.nj * b.nk;
+    hipLaunchKernelGGL(k_init_interior, grid1(n, 256), dim3(256), 0, s, b, F, P);
+    if (P.nvis == 1) {
+        for (int dir = 0; dir < 3; dir++) {
+            long na = dir == 0 ? b.nj : b.ni, nb = dir == 2 ? b.nj : b.nk;
+            hipLaunchKernelGGL(k_init_ghost_t, grid1(2 * na * nb, 128), dim3(128), 0, s, b, F, P, dir);
+        }
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_timestep(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
+{
+    long n = (long)b.ni * b.nj * b.nk;
+    hipLaunchKernelGGL(k_timestep, grid1(n, 256), dim3(256), 0, s, b, F, P);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_zero_rhs(const DBlock &b, const DFields &F, hipStream_t s)
+{
+    long n = (long)b.ni * b.nj * b.nk;
+    hipLaunchKernelGGL(k_zero_interior5, grid1(n, 256), dim3(256), 0, s, b, F.rhs, F.total);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_dq_zero_face(const DRegion &g, const DBlock &b, double *arr, long total, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dq_zero_face, grid1(g.npts, 128), dim3(128), 0, s, g, b, arr, total);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_npp_gather(const int *pts, int npts, const DBlock *dblocks, const DFields &F, double *buf, int nvar,
+                      int use_vol, hipStream_t s)
+{
+    if (npts == 0) return 0;
+    hipLaunchKernelGGL(k_npp_gather, grid1(npts, 128), dim3(128), 0, s, pts, npts, dblocks, F, buf, nvar, use_vol);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_avg2pm(const DRegion &g, const DBlock *blocks, const DFields &F, int pv, hipStream_t s)
+{
+    const DBlock &b = blocks[g.nb], &bt = blocks[g.nbt];
+    if (g.serial) hipLaunchKernelGGL(k_avg2pm, dim3(1), dim3(64), 0, s, g, b, bt, F, pv);
+    else hipLaunchKernelGGL(k_avg2pm, grid1(g.npts, 128), dim3(128), 0, s, g, b, bt, F, pv);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_npp_assign(const int *grp_start, const int *grp_n, const int *pts, const int *write_ok, int ngroups,
+                      const double *buf, const DBlock *dblocks, const DFields &F, int nvar, int use_vol,
+                      hipStream_t s)
+{
+    if (ngroups == 0) return 0;
+    hipLaunchKernelGGL(k_npp_assign, grid1(ngroups, 64), dim3(64), 0, s, grp_start, grp_n, pts, write_ok,
+                       ngroups, buf, dblocks, F, nvar, use_vol);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_sweeps(const DBlock &b, const DFields &F, const DParams &P, int prec, hipStream_t s)
+{
+    (void)prec;
+    int smin = 3, smax = b.ni + b.nj + b.nk;
+    long nthr = (long)b.nj * b.nk;
+    dim3 g = grid1(nthr, 256), bl(256);
+    for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<0>, g, bl, 0, s, b, F, P, p);
+    for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<1>, g, bl, 0, s, b, F, P, p);
+    for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<2>, g, bl, 0, s, b, F, P, p);
+    for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<3>, g, bl, 0, s, b, F, P, p);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_update(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s)
+{
+    long n = (long)b.ni * b.nj * b.nk;
+    hipLaunchKernelGGL(k_update, grid1(n, 256), dim3(256), 0, s, b, F, P, flag_list, cap);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_repair(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_repair, dim3(1), dim3(64), 0, s, b, F, P, flag_list, cap);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int residual_parts(const DBlock &b)
+{
+    long n = (long)(b.ni - 2) * (b.nj - 2) * (b.nk - 2);
+    long p = (n + RES_THREADS * 8 - 1) / (RES_THREADS * 8);
+    return (int)(p < 1 ? 1 : (p > 1024 ? 1024 : p));
+}
+
+int launch_residual(const DBlock &b, const DFields &F, double *partial, int nparts, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_residual, dim3(nparts), dim3(RES_THREADS), 0, s, b, F, partial);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_residual_final(const double *partial, int nparts, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, partial, nparts, out);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+}  // namespace wcns

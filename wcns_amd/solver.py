@@ -1,0 +1,221 @@
+"""Host-side mirror of the reference's hot-path interface over the C-ABI of
+libwcns_amd.so (include/wcns_amd.h).
+
+The reference drives one pseudo-time step from src/Main.f90:85-109 by calling
+argument-less subroutines over module globals.  `Solver` keeps the same names
+and order -- boundary_all, initial_q_dq_c_t, timestep_tgh, r_h_s_vis,
+communicate_dq_npp, r_h_s_invis, l_h_s_tgh, communicate_pv_npp,
+residual_curve -- each one a call into the GPU library on device-resident
+state.  Errors the reference turns into `stop` raise WcnsError here.
+
+There is no CPU fallback: if the gfx950 library is missing, construction fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libwcns_amd.so")
+
+E_NAMES = {-1: "WCNS_E_ARG", -2: "WCNS_E_STATE", -3: "WCNS_E_HIP", -4: "WCNS_E_NOMEM", -5: "WCNS_E_BC",
+           -6: "WCNS_E_JACOBIAN", -7: "WCNS_E_ROE", -8: "WCNS_E_REPAIR", -9: "WCNS_E_UNSUPPORTED"}
+
+FIELDS = {"r": 0, "u": 1, "v": 2, "w": 3, "p": 4, "t": 5, "dtdt": 7, "sra": 8, "srb": 9, "src": 10,
+          "srva": 11, "srvb": 12, "srvc": 13, "visl": 14, "vol": 16, "kcx": 17, "kcy": 18, "kcz": 19,
+          "etx": 20, "ety": 21, "etz": 22, "ctx": 23, "cty": 24, "ctz": 25, "dq": 100, "rhs": 102}
+
+EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "wcns_add_region", "wcns_setup",
+           "wcns_set_lhs_precision", "wcns_set_prim", "wcns_get_field", "wcns_set_field", "wcns_boundary_all",
+           "wcns_initial_q_dq_c_t", "wcns_timestep", "wcns_r_h_s_vis", "wcns_r_h_s_invis", "wcns_communicate_dq",
+           "wcns_l_h_s", "wcns_communicate_pv", "wcns_residual", "wcns_step", "wcns_get_counter", "wcns_sync",
+           "wcns_region_images", "wcns_multipoint"]
+
+
+class WcnsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{E_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("moo", "reynolds", "attack", "sideslip", "tref", "twall",
+                                          "cfl", "timedt_rate", "efix", "csrv", "gama", "prl", "prt")] + \
+               [(n, C.c_int) for n in ("nvis", "nlamtur", "nscheme", "nflux", "ntmst")]
+
+
+_lib = None
+
+
+def load_library(path=LIBPATH):
+    """Load libwcns_amd.so (built by wcns_amd/build.py); raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"gfx950 library not built: {path} (run python -m wcns_amd.build)")
+    L = C.CDLL(path)
+    dptr, iptr, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
+    L.wcns_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(Params)]
+    L.wcns_destroy.argtypes = [vp]
+    L.wcns_last_error.restype = C.c_char_p
+    L.wcns_last_error.argtypes = [vp]
+    L.wcns_add_block.argtypes = [vp, C.c_int, C.c_int, C.c_int, dptr, dptr, dptr, iptr]
+    L.wcns_add_region.argtypes = [vp, C.c_int, iptr, C.c_int, iptr, C.c_int, C.c_int]
+    L.wcns_setup.argtypes = [vp]
+    L.wcns_set_lhs_precision.argtypes = [vp, C.c_int]
+    L.wcns_set_prim.argtypes = [vp, C.c_int, dptr, dptr, dptr, dptr, dptr]
+    L.wcns_get_field.argtypes = [vp, C.c_int, C.c_int, dptr]
+    L.wcns_set_field.argtypes = [vp, C.c_int, C.c_int, dptr]
+    for n in ("wcns_boundary_all", "wcns_initial_q_dq_c_t", "wcns_timestep", "wcns_r_h_s_vis", "wcns_r_h_s_invis",
+              "wcns_communicate_dq", "wcns_l_h_s", "wcns_communicate_pv", "wcns_sync"):
+        getattr(L, n).argtypes = [vp]
+    L.wcns_residual.argtypes = [vp, dptr, dptr, iptr]
+    L.wcns_step.argtypes = [vp, C.c_int, dptr]
+    L.wcns_get_counter.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong)]
+    L.wcns_region_images.argtypes = [vp, C.c_int, C.c_int, iptr, iptr, iptr, iptr]
+    L.wcns_multipoint.argtypes = [vp, iptr, iptr, iptr, C.c_int]
+    _lib = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def g3(a):
+    """[i+2, j+2, k+2]-indexed array -> flat G3 order (i fastest)."""
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).transpose(2, 1, 0)).ravel()
+
+
+class Solver:
+    """One GPU context holding a multi-block case (wcns_amd.cases.Case)."""
+
+    def __init__(self, case, device=0, lhs_precision=0):
+        self.L = load_library()
+        self.case = case
+        prm = Params()
+        for k, _ in Params._fields_:
+            setattr(prm, k, getattr(case.params, k))
+        self.h = C.c_void_p()
+        self._check(self.L.wcns_create(C.byref(self.h), device, C.byref(prm)), "wcns_create")
+        for blk in case.blocks:
+            x, y, z = (np.ascontiguousarray(a.transpose(2, 1, 0)).ravel() for a in (blk.x, blk.y, blk.z))
+            nb = C.c_int()
+            self._check(self.L.wcns_add_block(self.h, blk.ni, blk.nj, blk.nk, _d(x), _d(y), _d(z), C.byref(nb)))
+        for nb, blk in enumerate(case.blocks):
+            for reg in blk.regions:
+                s = np.array(reg.s_raw, dtype=np.int32)
+                t = np.array(reg.t_raw if reg.t_raw is not None else [0] * 6, dtype=np.int32)
+                self._check(self.L.wcns_add_region(self.h, nb, _i(s), reg.bctype, _i(t), reg.nbt, reg.ibcwin))
+        self._check(self.L.wcns_setup(self.h), "wcns_setup")
+        if lhs_precision:
+            self._check(self.L.wcns_set_lhs_precision(self.h, lhs_precision))
+        for nb in range(len(case.blocks)):
+            prim = [g3(a) for a in case.initial_prim(nb)]
+            self._check(self.L.wcns_set_prim(self.h, nb, *[_d(a) for a in prim]))
+
+    def _check(self, rc, what=""):
+        if rc < 0:
+            msg = self.L.wcns_last_error(self.h) if self.h else b""
+            raise WcnsError(rc, f"{what} {msg.decode(errors='replace') if msg else ''}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.wcns_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- the reference's step routines (src/Main.f90:85-109)
+    def boundary_all(self):
+        self._check(self.L.wcns_boundary_all(self.h), "boundary_all")
+
+    def initial_q_dq_c_t(self):
+        self._check(self.L.wcns_initial_q_dq_c_t(self.h), "initial_q_dq_c_t")
+
+    def timestep_tgh(self):
+        self._check(self.L.wcns_timestep(self.h), "timestep_tgh")
+
+    def r_h_s_vis(self):
+        self._check(self.L.wcns_r_h_s_vis(self.h), "r_h_s_vis")
+
+    def r_h_s_invis(self):
+        self._check(self.L.wcns_r_h_s_invis(self.h), "r_h_s_invis")
+
+    def communicate_dq_npp(self):
+        self._check(self.L.wcns_communicate_dq(self.h), "communicate_dq_npp")
+
+    def l_h_s_tgh(self):
+        self._check(self.L.wcns_l_h_s(self.h), "l_h_s_tgh")
+
+    def communicate_pv_npp(self):
+        self._check(self.L.wcns_communicate_pv(self.h), "communicate_pv_npp")
+
+    def residual_curve(self):
+        rms, rmax = C.c_double(), C.c_double()
+        loc = np.zeros(5, dtype=np.int32)
+        self._check(self.L.wcns_residual(self.h, C.byref(rms), C.byref(rmax), _i(loc)), "residual_curve")
+        return rms.value, rmax.value, loc
+
+    def step(self, n=1, history=True):
+        """n whole pseudo-time steps; returns the RMS history when requested."""
+        if history:
+            hist = np.zeros(max(n, 1))
+            self._check(self.L.wcns_step(self.h, n, _d(hist)), "step")
+            return hist[:n]
+        self._check(self.L.wcns_step(self.h, n, None), "step")
+        return None
+
+    def sync(self):
+        self._check(self.L.wcns_sync(self.h))
+
+    # ---- inspection
+    def field(self, nb, name):
+        blk = self.case.blocks[nb]
+        n = (blk.ni + 6) * (blk.nj + 6) * (blk.nk + 6)
+        ncomp = 5 if name in ("dq", "rhs") else 1
+        out = np.zeros(ncomp * n)
+        self._check(self.L.wcns_get_field(self.h, nb, FIELDS[name], _d(out)), "get_field")
+        shape = (blk.nk + 6, blk.nj + 6, blk.ni + 6)
+        if ncomp == 5:
+            return out.reshape((5,) + shape).transpose(1, 2, 3, 0)
+        return out.reshape(shape)
+
+    def set_field(self, nb, name, arr):
+        a = np.asarray(arr, dtype=np.float64)
+        if name in ("dq", "rhs"):
+            a = np.ascontiguousarray(a.transpose(3, 0, 1, 2))
+        a = np.ascontiguousarray(a).ravel()
+        self._check(self.L.wcns_set_field(self.h, nb, FIELDS[name], _d(a)), "set_field")
+
+    def counter(self, which):
+        v = C.c_longlong()
+        self._check(self.L.wcns_get_counter(self.h, which, C.byref(v)))
+        return v.value
+
+    def region_images(self, nb, nr):
+        blk = self.case.blocks[nb]
+        n = (blk.ni + 1) * (blk.nj + 1) * (blk.nk + 1)
+        img, jmg, kmg = (np.zeros(n, np.int32) for _ in range(3))
+        box = np.zeros(6, np.int32)
+        self._check(self.L.wcns_region_images(self.h, nb, nr, _i(img), _i(jmg), _i(kmg), _i(box)))
+        cnt = int(np.prod(box[3:] - box[:3] + 1))
+        return box, img[:cnt], jmg[:cnt], kmg[:cnt]
+
+    def multipoint(self):
+        ng = C.c_int()
+        npts = np.zeros(1 << 16, np.int32)
+        cap = 4 << 18
+        pts = np.zeros(cap, np.int32)
+        tot = self._check(self.L.wcns_multipoint(self.h, C.byref(ng), _i(npts), _i(pts), cap))
+        return npts[:ng.value].copy(), pts[:4 * tot].reshape(-1, 4).copy()
