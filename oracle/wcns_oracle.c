@@ -1884,3 +1884,29 @@ int o_step(o_case *c, int nsteps, double *rms_hist)
     }
     return 0;
 }
+
+/* views for wcns_oracle_vis.c (field pointers of one block, namelist values) */
+typedef struct { int ni, nj, nk; long g3; double *x, *y, *z; double *r, *u, *v, *w, *p, *t, *c;
+    double *q, *dq, *rhs_nb; double *kcx, *kcy, *kcz, *etx, *ety, *etz, *ctx, *cty, *ctz, *vol;
+    double *sra, *srb, *src, *srva, *srvb, *srvc, *dtdt, *visl, *vist; } oblock_view;
+
+int o_view_block(o_case *c, int nb, oblock_view *v)
+{
+    if (nb < 0 || nb >= c->nblocks) return -1;
+    oblock *b = &c->blk[nb];
+    v->ni = b->ni; v->nj = b->nj; v->nk = b->nk; v->g3 = b->g3;
+    v->x = b->x; v->y = b->y; v->z = b->z;
+    v->r = b->r; v->u = b->u; v->v = b->v; v->w = b->w; v->p = b->p; v->t = b->t; v->c = b->c;
+    v->q = b->q; v->dq = b->dq; v->rhs_nb = b->rhs_nb;
+    v->kcx = b->kcx; v->kcy = b->kcy; v->kcz = b->kcz; v->etx = b->etx; v->ety = b->ety; v->etz = b->etz;
+    v->ctx = b->ctx; v->cty = b->cty; v->ctz = b->ctz; v->vol = b->vol;
+    v->sra = b->sra; v->srb = b->srb; v->src = b->src; v->srva = b->srva; v->srvb = b->srvb; v->srvc = b->srvc;
+    v->dtdt = b->dtdt; v->visl = b->visl; v->vist = b->vist;
+    return 0;
+}
+
+void o_view_params(const o_case *c, o_params *p, double *visc)
+{
+    *p = c->prm;
+    *visc = c->visc;
+}

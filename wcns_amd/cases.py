@@ -165,3 +165,91 @@ def cartesian_periodic_case(ni=64, nj=32, nk=16, seed=1, nscheme=41, nflux=1, cf
     prm = Params(moo=moo, cfl=cfl, nscheme=nscheme, nflux=nflux, nvis=0)
     return Case(prm, [blk], init=fourier_perturbed_init(modes, amp),
                 name=f"cart{ni}x{nj}x{nk}", seed=seed)
+
+
+def wavy_periodic_case(ni=24, nj=16, nk=12, seed=2, nscheme=41, nflux=1, cfl=5.0, moo=0.8, amp_grid=0.04,
+                       lx=2.0, ly=1.0, lz=0.5, nmodes=8, amp=0.01, attack=3.0):
+    """Curvilinear variant of cartesian_periodic_case: every coordinate carries a
+    smooth perturbation that is periodic in i and k (so node 1 and node n still
+    coincide) -- all nine metric terms are non-zero."""
+    c = cartesian_periodic_case(ni, nj, nk, seed, nscheme, nflux, cfl, moo, 1.05, lx, ly, lz, nmodes, amp)
+    b = c.blocks[0]
+    I, J, K = np.meshgrid(np.arange(ni) / (ni - 1), np.arange(nj) / (nj - 1), np.arange(nk) / (nk - 1),
+                          indexing="ij")
+    tw = 2 * np.pi
+    b.x = b.x + amp_grid * lx * np.sin(tw * I) * np.cos(np.pi * J) * np.cos(tw * K)
+    b.y = b.y + amp_grid * ly * np.sin(tw * I + 0.3) * np.sin(np.pi * J) * np.cos(tw * K + 0.2)
+    b.z = b.z + amp_grid * lz * np.cos(tw * I) * np.sin(np.pi * J + 0.1) * np.sin(tw * K)
+    c.params.attack = attack
+    c.name = f"wavy{ni}x{nj}x{nk}"
+    return c
+
+
+def uniform_init(case, nb):
+    """Freestream everywhere (ghosts included)."""
+    b = case.blocks[nb]
+    roo, uoo, voo, woo, poo = case.freestream()
+    shp = (b.ni + 6, b.nj + 6, b.nk + 6)
+    return (np.full(shp, roo), np.full(shp, uoo), np.full(shp, voo), np.full(shp, woo), np.full(shp, poo))
+
+
+def bump_channel_case(ni=256, nj=128, nk=64, bump=0.05, dy_wall=1.0e-5, growth=1.1, lx=4.0, lz=0.5,
+                      reynolds=1.0e5, moo=0.5, nflux=3, nscheme=41, cfl=5.0, seed=0):
+    """Sinusoidal-bump channel: x in [-1.5, 2.5] (i), a bump y_w = bump*sin^2(pi x)
+    on 0 <= x <= 1 of the bottom wall (BC 2, adiabatic no-slip, j = 1), wall-normal
+    spacing dy_wall growing geometrically by `growth` (j), span lz periodic (k,
+    self-connected).  Far field (BC 4) on inflow, outflow and top.  Laminar
+    Navier-Stokes at `reynolds` (per unit length), Mach `moo`, Roe flux, WCNS-E5."""
+    x = -1.5 + lx * np.arange(ni) / (ni - 1)
+    s = np.concatenate([[0.0], np.cumsum(dy_wall * growth ** np.arange(nj - 1))])
+    H = s[-1]
+    yw = np.where((x >= 0) & (x <= 1), bump * np.sin(np.pi * x) ** 2, 0.0)
+    z = lz * np.arange(nk) / (nk - 1)
+    X = np.repeat(x[:, None, None], nj, 1).repeat(nk, 2)
+    Y = yw[:, None, None] + (H - yw[:, None, None]) * (s[None, :, None] / H)
+    Y = np.repeat(Y, nk, 2)
+    Z = np.broadcast_to(z[None, None, :], (ni, nj, nk)).copy()
+    regs = [Region((1, 1, 1, nj, 1, nk), 4), Region((ni, ni, 1, nj, 1, nk), 4),
+            Region((1, ni, 1, 1, 1, nk), 2), Region((1, ni, nj, nj, 1, nk), 4),
+            Region((1, ni, 1, nj, 1, 1), -1, (1, ni, 1, nj, nk, nk), 1, 6),
+            Region((1, ni, 1, nj, nk, nk), -1, (1, ni, 1, nj, 1, 1), 1, 5)]
+    blk = Block(ni, nj, nk, X, Y, Z, regs, name="bump")
+    prm = Params(moo=moo, reynolds=reynolds, tref=288.15, twall=-1.0, cfl=cfl, nvis=1, nflux=nflux,
+                 nscheme=nscheme)
+    return Case(prm, [blk], init=uniform_init, name=f"bump{ni}x{nj}x{nk}", seed=seed)
+
+
+def linear_field_init(A, moo=0.5):
+    """u = A x (A 3x3, any), rho = 1, p = freestream -- ghosts included, all positive."""
+    def init(case, nb):
+        b = case.blocks[nb]
+        shp = (b.ni + 6, b.nj + 6, b.nk + 6)
+
+        def ext(a):
+            out = np.zeros(shp)
+            out[3:-3, 3:-3, 3:-3] = a
+            # linear extrapolation into the ghosts keeps the field exactly linear on a uniform grid
+            for ax in range(3):
+                sl = [slice(None)] * 3
+                n = out.shape[ax]
+                for g in range(3):
+                    lo = [slice(None)] * 3
+                    lo[ax] = 2 - g
+                    i1 = [slice(None)] * 3
+                    i1[ax] = 3 - g
+                    i2 = [slice(None)] * 3
+                    i2[ax] = 4 - g
+                    out[tuple(lo)] = 2 * out[tuple(i1)] - out[tuple(i2)]
+                    hi = [slice(None)] * 3
+                    hi[ax] = n - 3 + g
+                    j1 = [slice(None)] * 3
+                    j1[ax] = n - 4 + g
+                    j2 = [slice(None)] * 3
+                    j2[ax] = n - 5 + g
+                    out[tuple(hi)] = 2 * out[tuple(j1)] - out[tuple(j2)]
+            return out
+        X, Y, Z = b.x, b.y, b.z
+        vel = [A[i][0] * X + A[i][1] * Y + A[i][2] * Z for i in range(3)]
+        poo = 1.0 / (case.params.gama * case.params.moo ** 2)
+        return (np.ones(shp), ext(vel[0]), ext(vel[1]), ext(vel[2]), np.full(shp, poo))
+    return init
