@@ -1,0 +1,159 @@
+"""Throughput of the WCNS pseudo-time step on one MI355X.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bump|cart] [--lhs fp64|fp32]
+
+Workloads (synthetic inputs generated by wcns_amd.cases, seeded):
+  bump  256x128x64-node sinusoidal-bump channel (bump 0.05 L, wall spacing 1e-5 L,
+        growth 1.1), laminar Navier-Stokes Re 1e5, Mach 0.5, Roe flux, WCNS-E5
+        (scheme 41), adiabatic no-slip wall, far field elsewhere, periodic span
+  cart  256x128x64-node Cartesian box, i/k periodic, far field in j, Euler,
+        Steger-Warming (the inviscid path on the same node count)
+
+A step is the whole pseudo-time step of src/Main.f90:85-109: ghost fill, speed
+of sound/temperature, spectral radii and local dt, viscous and inviscid
+residual, interface averaging, LU-SGS + SGS, limited update, interface
+primitive averaging and the residual monitor.  Mcell-iter/s = nodes x steps /
+seconds / 1e6, inputs resident in HBM before timing starts.
+
+The dominant kernel is timed live with HIP events recorded on the library's
+own stream over the timed region; its roofline entry is its algorithmic HBM
+bytes per launch over its mean launch time, against 8 TB/s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per node for one launch of each timed kernel family
+# (FP64 words; each field counted once per pass).
+ALG_BYTES_PER_NODE = {
+    # R += dF/dxi along one direction: read P(5) + metric triple(3) + R(5), write R(5)
+    "rhs_inv_i": 8 * 18, "rhs_inv_j": 8 * 18, "rhs_inv_k": 8 * 18,
+    # spectral radii + dt: read P(5) + metrics(9) + vol + srv(3), write sr(3) + dt(1)
+    "timestep": 8 * 22,
+    # update: read P(5) + dq(5), write P(5)
+    "update": 8 * 15,
+}
+
+
+def build_case(workload):
+    from wcns_amd import cases
+    if workload == "bump":
+        return cases.bump_channel_case(ni=256, nj=128, nk=64), \
+            "bump-channel 256x128x64 laminar Re1e5 M0.5 Roe WCNS-E5"
+    if workload == "cart":
+        return cases.cartesian_periodic_case(ni=256, nj=128, nk=64), \
+            "cartesian-periodic 256x128x64 Euler M0.8 SW WCNS-E5"
+    raise SystemExit(f"unknown workload {workload}")
+
+
+def cpu_baseline(case, budget_s=25.0):
+    """The C oracle (single thread) on a bounded sample of the same workload:
+    whole steps of the full grid until about budget_s seconds have passed."""
+    from oracle import oracle as O
+    o = O.OracleCase(case)
+    o.step(1)                        # first step outside the clock (allocation, first touch)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        o.step(1)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 50:
+            break
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": case.nodes * n / dt / 1e6, "unit": "Mcell-iter/s", "cores": 1, "kind": "port",
+            "sample": f"{n} full step(s) of the same {case.nodes}-node workload after one untimed step, "
+                      f"{dt:.1f} s on one host core (oracle/wcns_oracle.c, gcc -O2)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="bump")
+    ap.add_argument("--lhs", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--breakdown", action="store_true", help="print the per-kernel table to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding by block is a design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from wcns_amd.solver import Solver
+
+    case, wl_name = build_case(args.workload)
+    s = Solver(case, device=0, lhs_precision=1 if args.lhs == "fp32" else 0)
+    s.step(args.warmup, history=False)
+    s.sync()
+    torch.cuda.synchronize()
+
+    s.timing(True)
+    s.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(args.steps, history=False)
+    s.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tim = s.timings()
+    s.timing(False)
+    rms, rmax, loc = s.residual_curve()
+
+    sec = t1 - t0
+    value = case.nodes * args.steps / sec / 1e6
+    # dominant kernel family
+    dom = max(tim, key=lambda k: tim[k][0])
+    dms, dlaunch = tim[dom]
+    nodes = case.nodes
+    roof = None
+    if dom in ALG_BYTES_PER_NODE and dlaunch:
+        per_launch = ALG_BYTES_PER_NODE[dom] * nodes
+        avg_s = dms / dlaunch / 1e3
+        ach = per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "alg_bytes_per_launch": per_launch, "avg_launch_ms": round(dms / dlaunch, 4)}
+    elif dlaunch:
+        # sweep planes: one launch per hyperplane; bytes per launch = plane share of a sweep pass
+        nb = {"sweep_lu_l": 26, "sweep_lu_u": 26, "sweep_gs_l": 31, "sweep_gs_u": 31}.get(dom, 0) * 8
+        per_launch = nb * nodes / max(dlaunch / (args.steps * len(case.blocks)), 1)
+        avg_s = dms / dlaunch / 1e3
+        ach = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "alg_bytes_per_launch": per_launch, "avg_launch_ms": round(dms / dlaunch, 5)}
+    if args.breakdown:
+        tot = sum(v[0] for v in tim.values())
+        for k, (ms, n) in sorted(tim.items(), key=lambda kv: -kv[1][0]):
+            print(f"{k:12s} {ms / args.steps:9.3f} ms/step  {100 * ms / max(tot, 1e-9):5.1f}%  launches {n}",
+                  file=sys.stderr)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(case)
+    line = {
+        "metric": "Mcell-iter/s (RHS+LU-SGS), mixed vs FP64 at equal residual drop; % HBM peak",
+        "value": round(value, 2), "unit": "Mcell-iter/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * sec / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.lhs == "fp64" else "f64+f32-lhs",
+        "data": "synthetic (seeded generator wcns_amd.cases)",
+        "config": {"workload": wl_name, "nodes": nodes, "blocks": len(case.blocks), "lhs": args.lhs,
+                   "residual_rms_end": rms},
+        "roofline": roof, "cpu_baseline": cpu,
+        "stage_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in tim.items()},
+    }
+    print(json.dumps(line))
+    s.close()
+
+
+if __name__ == "__main__":
+    main()

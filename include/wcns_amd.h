@@ -109,6 +109,29 @@ int wcns_step(wcns_ctx *ctx, int nsteps, double *rms_hist);
 int wcns_get_counter(wcns_ctx *ctx, int which, long long *value);
 int wcns_sync(wcns_ctx *ctx);
 
+/* per-kernel timing with HIP events recorded on the context's own stream
+ * (no counterpart in the reference, which only reads CPU_TIME, Main.f90:72).
+ * wcns_timing(ctx, 1) clears and starts recording; wcns_timing_get returns
+ * the summed elapsed time of a slot and the number of kernel launches in it. */
+#define WCNS_T_BC           0
+#define WCNS_T_INIT         1
+#define WCNS_T_TIMESTEP     2
+#define WCNS_T_VIS          3
+#define WCNS_T_INV_I        4
+#define WCNS_T_INV_J        5
+#define WCNS_T_INV_K        6
+#define WCNS_T_COMM_DQ      7
+#define WCNS_T_SWEEP_LU_L   8
+#define WCNS_T_SWEEP_LU_U   9
+#define WCNS_T_SWEEP_GS_L  10
+#define WCNS_T_SWEEP_GS_U  11
+#define WCNS_T_UPDATE      12
+#define WCNS_T_COMM_PV     13
+#define WCNS_T_RESIDUAL    14
+#define WCNS_T_NSLOT       15
+int wcns_timing(wcns_ctx *ctx, int on);
+int wcns_timing_get(wcns_ctx *ctx, int slot, double *total_ms, long long *launches);
+
 /* debugging/inspection of the host-side setup */
 int wcns_region_images(wcns_ctx *ctx, int nb, int nr, int *img, int *jmg, int *kmg, int box[6]);
 int wcns_multipoint(wcns_ctx *ctx, int *ngroups, int *npts, int *pts, int cap);

@@ -103,3 +103,61 @@ def test_cartesian_64x32x16_100_steps():
     _cmp_prim(s, o, blk, 1e-11, ghost=0)
     assert s.counter(0) == o.counter(0)
     assert s.counter(1) == o.counter(1)
+
+
+VIS_CASES = [
+    dict(ni=24, nj=16, nk=12, nflux=3, nscheme=41),
+    dict(ni=20, nj=18, nk=13, nflux=1, nscheme=4),
+]
+
+
+@pytest.mark.parametrize("kw", VIS_CASES, ids=lambda k: f"{k['ni']}x{k['nj']}x{k['nk']}-s{k['nscheme']}-f{k['nflux']}")
+def test_viscous_stage_by_stage(kw):
+    """Laminar bump channel: adiabatic no-slip wall (BC 2), far field (BC 4),
+    periodic span -- viscous residual, viscous spectral radii, wall ghosts."""
+    case = cases.bump_channel_case(dy_wall=1e-3, growth=1.2, reynolds=1e3, **kw)
+    blk = case.blocks[0]
+    print(f"viscous case {kw}", flush=True)
+    s = _solver(case)
+    o = O.OracleCase(case)
+    for step in range(3):
+        s.boundary_all(); o.stage("boundary_all")
+        _cmp_prim(s, o, blk, REL)
+        s.initial_q_dq_c_t(); o.stage("initial_q_dq_c_t")
+        for m in ("t", "visl"):
+            e = _rel_err(_interior(s.field(0, m), blk, 1 if m == "t" else 0),
+                         _interior(o.field(0, m), blk, 1 if m == "t" else 0))
+            assert e <= REL, f"{m}: {e:.3e}"
+        s.timestep_tgh(); o.stage("timestep")
+        for m in ("srva", "srvb", "srvc", "dtdt"):
+            e = _rel_err(_interior(s.field(0, m), blk), _interior(o.field(0, m), blk))
+            assert e <= REL, f"{m}: {e:.3e}"
+        s.r_h_s_vis(); o.stage("rhs_vis")
+        rg = _interior(s.field(0, "rhs"), blk)
+        ro = _interior(o.field(0, "dq"), blk)
+        assert _rel_err(rg, ro) <= REL, f"viscous rhs step {step}: {_rel_err(rg, ro):.3e}"
+        s.communicate_dq_npp(); o.stage("communicate_dq")
+        s.r_h_s_invis(); o.stage("rhs_invis")
+        s.communicate_dq_npp(); o.stage("communicate_dq")
+        rg = _interior(s.field(0, "rhs"), blk)
+        ro = _interior(o.field(0, "dq"), blk)
+        assert _rel_err(rg, ro) <= REL, f"rhs step {step}: {_rel_err(rg, ro):.3e}"
+        s.l_h_s_tgh(); o.stage("lhs")
+        dg = _interior(s.field(0, "dq"), blk)
+        do = _interior(o.field(0, "dq"), blk)
+        assert _rel_err(dg, do) <= REL, f"dq step {step}: {_rel_err(dg, do):.3e}"
+        s.communicate_pv_npp(); o.stage("communicate_pv")
+        _cmp_prim(s, o, blk, REL, ghost=0)
+
+
+def test_viscous_bump_30_steps():
+    case = cases.bump_channel_case(ni=32, nj=24, nk=12, dy_wall=1e-3, growth=1.15, reynolds=1e4)
+    blk = case.blocks[0]
+    s = _solver(case)
+    o = O.OracleCase(case)
+    hs = s.step(30)
+    ho = o.step(30)
+    rel = np.abs(hs - ho) / ho
+    print("viscous residual history rel err max", rel.max(), flush=True)
+    assert rel.max() <= 1e-10
+    _cmp_prim(s, o, blk, 1e-11, ghost=0)

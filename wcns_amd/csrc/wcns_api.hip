@@ -62,6 +62,7 @@ struct wcns_ctx {
     int cur = 0;
     double *t = nullptr, *metric[10] = {nullptr}, *rhs = nullptr, *dq = nullptr;
     double *dtdt = nullptr, *sr[3] = {nullptr}, *srv[3] = {nullptr}, *visl = nullptr;
+    double *dn = nullptr;                   // 12 node derivatives (viscous)
     unsigned long long *counters = nullptr;
     DBlock *d_blocks = nullptr;
     int *d_flag = nullptr;
@@ -78,6 +79,22 @@ struct wcns_ctx {
     int *d_npp_pts = nullptr, *d_grp_start = nullptr, *d_grp_n = nullptr, *d_write_ok = nullptr;
     double *d_npp_buf = nullptr;
     std::vector<int> h_npp_pts, h_grp_n;
+
+    // per-kernel timing with HIP events on this context's stream
+    bool timing = false;
+    struct Rec { int slot, launches; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    hipEvent_t event()
+    {
+        if (pool_used == pool.size()) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            pool.push_back(e);
+        }
+        return pool[pool_used++];
+    }
 
     DFields fields() const
     {
@@ -98,6 +115,25 @@ struct wcns_ctx {
 };
 
 namespace {
+
+// Records HIP events around the launches of one timing slot.
+struct Scope {
+    wcns_ctx *c;
+    int slot, launches;
+    hipEvent_t a{};
+    Scope(wcns_ctx *ctx, int s, int n = 1) : c(ctx), slot(s), launches(n)
+    {
+        if (c->timing) { a = c->event(); hipEventRecord(a, c->stream); }
+    }
+    ~Scope()
+    {
+        if (c->timing) {
+            hipEvent_t b = c->event();
+            hipEventRecord(b, c->stream);
+            c->recs.push_back({slot, launches, a, b});
+        }
+    }
+};
 
 int fail(wcns_ctx *ctx, int code, const std::string &msg)
 {
@@ -322,10 +358,11 @@ int wcns_destroy(wcns_ctx *ctx)
     for (int b = 0; b < 2; b++) for (int m = 0; m < 5; m++) hipFree(ctx->prim[b][m]);
     for (int m = 0; m < 10; m++) hipFree(ctx->metric[m]);
     for (int m = 0; m < 3; m++) { hipFree(ctx->sr[m]); hipFree(ctx->srv[m]); }
-    void *ps[] = {ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
+    void *ps[] = {ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
                   ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
+    for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return WCNS_OK;
@@ -383,13 +420,16 @@ int wcns_setup(wcns_ctx *ctx)
     const wcns_params &p = ctx->prm;
     if (p.nscheme != 4 && p.nscheme != 41) return fail(ctx, WCNS_E_UNSUPPORTED, "nscheme must be 4 or 41");
     if (p.nflux != 1 && p.nflux != 3) return fail(ctx, WCNS_E_UNSUPPORTED, "nflux must be 1 (SW) or 3 (Roe)");
-    if (p.nvis != 0) return fail(ctx, WCNS_E_UNSUPPORTED, "viscous terms not built yet");
+    if (p.nvis != 0 && p.nvis != 1) return fail(ctx, WCNS_E_ARG, "nvis must be 0 or 1");
+    if (p.nlamtur >= 0) return fail(ctx, WCNS_E_UNSUPPORTED, "turbulence models (nlamtur >= 0) are not built");
     if (p.ntmst != 0) return fail(ctx, WCNS_E_UNSUPPORTED, "only local time stepping (ntmst = 0)");
     // layout
     long off = 0;
     for (auto &b : ctx->blocks) {
-        for (int n : {b.ni, b.nj, b.nk})
+        for (int n : {b.ni, b.nj, b.nk}) {
             if (n > 4 && n < 12) return fail(ctx, WCNS_E_UNSUPPORTED, "directions with 5..11 nodes are not supported");
+            if (p.nvis == 1 && n <= 4) return fail(ctx, WCNS_E_UNSUPPORTED, "2-D (n <= 4) viscous directions are not supported");
+        }
         b.db.ni = b.ni; b.db.nj = b.nj; b.db.nk = b.nk; b.db.pad = 0;
         b.db.sx = b.ni + 6;
         b.db.sy = (long)(b.ni + 6) * (b.nj + 6);
@@ -418,6 +458,7 @@ int wcns_setup(wcns_ctx *ctx)
     TRY(dalloc(ctx, &ctx->dq, 5 * T));
     TRY(dalloc(ctx, &ctx->dtdt, T));
     TRY(dalloc(ctx, &ctx->visl, T));
+    if (p.nvis == 1) TRY(dalloc(ctx, &ctx->dn, 12 * T));
     TRY(dalloc(ctx, &ctx->counters, 8));
     TRY(dalloc(ctx, &ctx->d_flag, ctx->flag_cap));
     std::vector<DBlock> hb;
@@ -434,7 +475,7 @@ int wcns_setup(wcns_ctx *ctx)
     const double pai = 3.14159265;
     double att = p.attack * pai / 180.0, ssl = p.sideslip * pai / 180.0;
     d.gama = p.gama; d.efix = p.efix; d.cfl = p.cfl; d.timedt_rate = p.timedt_rate; d.moo = p.moo;
-    d.csrv = p.csrv; d.re = 1.0 / p.reynolds; d.visc = 110.4 / p.tref; d.prl = p.prl; d.prt = p.prt;
+    d.csrv = p.csrv; d.re = 1.0 / p.reynolds; d.reynolds = p.reynolds; d.visc = 110.4 / p.tref; d.prl = p.prl; d.prt = p.prt;
     d.roo = 1.0; d.uoo = cos(att) * cos(ssl); d.voo = sin(att) * cos(ssl); d.woo = sin(ssl);
     d.poo = 1.0 / (p.gama * p.moo * p.moo);
     d.coo = sqrt(p.gama * d.poo / d.roo);
@@ -611,6 +652,7 @@ int wcns_boundary_all(wcns_ctx *ctx)
     DFields F = ctx->fields();
     std::vector<DBlock> dbs;
     for (auto &b : ctx->blocks) dbs.push_back(b.db);
+    Scope sc(ctx, WCNS_T_BC);
     for (auto &b : ctx->blocks)
         for (int r : b.bcorder)
             if (launch_bc_region(b.regs[r].d, dbs.data(), F, ctx->dp, ctx->stream))
@@ -622,6 +664,7 @@ int wcns_initial_q_dq_c_t(wcns_ctx *ctx)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     DFields F = ctx->fields();
+    Scope sc(ctx, WCNS_T_INIT);
     for (auto &b : ctx->blocks)
         if (launch_init_stage(b.db, F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "init launch");
     return WCNS_OK;
@@ -631,8 +674,12 @@ int wcns_timestep(wcns_ctx *ctx)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     DFields F = ctx->fields();
-    for (auto &b : ctx->blocks)
+    Scope sc(ctx, WCNS_T_TIMESTEP);
+    for (auto &b : ctx->blocks) {
+        if (ctx->prm.nvis == 1 && ctx->prm.csrv > 0.0 && launch_spectvis(b.db, F, ctx->dp, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "spectvis launch");
         if (launch_timestep(b.db, F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "timestep launch");
+    }
     return WCNS_OK;
 }
 
@@ -640,20 +687,31 @@ int wcns_r_h_s_vis(wcns_ctx *ctx)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     if (ctx->prm.nvis != 1) return WCNS_OK;
-    return fail(ctx, WCNS_E_UNSUPPORTED, "viscous terms not built yet");
+    DFields F = ctx->fields();
+    Scope sc(ctx, WCNS_T_VIS, 4 * (int)ctx->blocks.size());
+    for (auto &b : ctx->blocks) {
+        if (launch_vis_node_deriv(b.db, F, ctx->dn, ctx->stream)) return fail(ctx, WCNS_E_HIP, "vis deriv launch");
+        for (int dir = 0; dir < 3; dir++)
+            if (launch_rhs_vis_dir(dir, b.db, F, ctx->dp, ctx->dn, ctx->stream)) return fail(ctx, WCNS_E_HIP, "vis launch");
+    }
+    return WCNS_OK;
 }
 
 int wcns_r_h_s_invis(wcns_ctx *ctx)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     DFields F = ctx->fields();
-    for (auto &b : ctx->blocks)
-        if (launch_rhs_inv(b.db, F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "rhs launch");
+    for (int dir = 0; dir < 3; dir++) {
+        Scope sc(ctx, WCNS_T_INV_I + dir, (int)ctx->blocks.size());
+        for (auto &b : ctx->blocks)
+            if (launch_rhs_inv_dir(dir, b.db, F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "rhs launch");
+    }
     return WCNS_OK;
 }
 
 static int communicate(wcns_ctx *ctx, int pv)
 {
+    Scope sc(ctx, pv ? WCNS_T_COMM_PV : WCNS_T_COMM_DQ);
     DFields F = ctx->fields();
     std::vector<DBlock> dbs;
     for (auto &b : ctx->blocks) dbs.push_back(b.db);
@@ -690,10 +748,15 @@ int wcns_l_h_s(wcns_ctx *ctx)
         const DBlock &db = ctx->blocks[nb].db;
         for (auto &g : ctx->phys)
             if (g.nb == (int)nb) launch_dq_zero_face(g, db, F.rhs, F.total, ctx->stream);
-        if (launch_sweeps(db, F, ctx->dp, ctx->lhs_prec, ctx->stream)) return fail(ctx, WCNS_E_HIP, "sweeps");
+        int nplanes = db.ni + db.nj + db.nk - 2;
+        for (int kind = 0; kind < 4; kind++) {
+            Scope sc(ctx, WCNS_T_SWEEP_LU_L + kind, nplanes);
+            if (launch_sweep(kind, db, F, ctx->dp, ctx->lhs_prec, ctx->stream)) return fail(ctx, WCNS_E_HIP, "sweeps");
+        }
         for (auto &g : ctx->phys)
             if (g.nb == (int)nb) launch_dq_zero_face(g, db, F.dq, F.total, ctx->stream);
         CK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(unsigned long long), ctx->stream));
+        Scope sc(ctx, WCNS_T_UPDATE);
         if (launch_update(db, F, ctx->dp, ctx->d_flag, ctx->flag_cap, ctx->stream)) return fail(ctx, WCNS_E_HIP, "update");
         if (launch_repair(db, F, ctx->dp, ctx->d_flag, ctx->flag_cap, ctx->stream)) return fail(ctx, WCNS_E_HIP, "repair");
     }
@@ -706,12 +769,14 @@ int wcns_residual(wcns_ctx *ctx, double *rms, double *resmax, int loc[5])
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     DFields F = ctx->fields();
     std::vector<double> res(3 * ctx->blocks.size());
+    Scope *sc = new Scope(ctx, WCNS_T_RESIDUAL);
     for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
         const DBlock &db = ctx->blocks[nb].db;
         int np = residual_parts(db);
         launch_residual(db, F, ctx->d_partial, np, ctx->stream);
         launch_residual_final(ctx->d_partial, np, ctx->d_resblk + 3 * nb, ctx->stream);
     }
+    delete sc;
     CK(hipMemcpyAsync(res.data(), ctx->d_resblk, 8 * res.size(), hipMemcpyDeviceToHost, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     TRY(check_counters(ctx));
@@ -776,6 +841,34 @@ int wcns_get_counter(wcns_ctx *ctx, int which, long long *value)
     CK(hipStreamSynchronize(ctx->stream));
     CK(hipMemcpy(c, ctx->counters, sizeof c, hipMemcpyDeviceToHost));
     *value = (long long)c[which];
+    return WCNS_OK;
+}
+
+int wcns_timing(wcns_ctx *ctx, int on)
+{
+    if (!ctx) return WCNS_E_ARG;
+    CK(hipStreamSynchronize(ctx->stream));
+    ctx->recs.clear();
+    ctx->pool_used = 0;
+    ctx->timing = on != 0;
+    return WCNS_OK;
+}
+
+int wcns_timing_get(wcns_ctx *ctx, int slot, double *total_ms, long long *launches)
+{
+    if (!ctx || !total_ms || !launches || slot < 0 || slot >= WCNS_T_NSLOT) return WCNS_E_ARG;
+    CK(hipStreamSynchronize(ctx->stream));
+    double t = 0.0;
+    long long n = 0;
+    for (auto &r : ctx->recs) {
+        if (r.slot != slot) continue;
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, r.a, r.b));
+        t += ms;
+        n += r.launches;
+    }
+    *total_ms = t;
+    *launches = n;
     return WCNS_OK;
 }
 

@@ -31,7 +31,7 @@ __host__ __device__ inline long gidx(const DBlock &b, int i, int j, int k)
 
 // Scalars every kernel may need (freestream, namelist values).
 struct DParams {
-    double gama, efix, cfl, timedt_rate, moo, csrv, re, visc, prl, prt;
+    double gama, efix, cfl, timedt_rate, moo, csrv, re, reynolds, visc, prl, prt;
     double roo, uoo, voo, woo, poo, coo, twall_nd;
     int nvis, nscheme, nflux, twall_fixed;
 };

@@ -35,6 +35,12 @@ __host__ __device__ inline void region_image(const DRegion &g, int i, int j, int
 }
 
 int launch_rhs_inv(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
+int launch_rhs_inv_dir(int dir, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
+int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, int prec, hipStream_t s);
+
+int launch_vis_node_deriv(const DBlock &b, const DFields &F, double *dn, hipStream_t s);
+int launch_rhs_vis_dir(int dir, const DBlock &b, const DFields &F, const DParams &P, const double *dn, hipStream_t s);
+int launch_spectvis(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
 
 int launch_bc_region(const DRegion &g, const DBlock *blocks, const DFields &F, const DParams &P, hipStream_t s);
 int launch_init_stage(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);

@@ -103,7 +103,7 @@ __device__ inline void checked_flux(const Pencil &pc, int f, double *ql, double 
 
 // The WCNS_E_5_41 left boundary rows 1..3 (WCNSE5.f90:2269-2513).
 template <int FLUX>
-__device__ void left41(const Pencil &pc, double *df1, double *df2, double *df3, double efix, double gama,
+__device__ __noinline__ void left41(const Pencil &pc, double *df1, double *df2, double *df3, double efix, double gama,
                        int &nerr, int &bad)
 {
     double q1[5], q2[5], q3[5], q4[5], urold[5][5], tmp[5], nx[3], f[6][5], ql[5], qr[5];
@@ -162,7 +162,7 @@ __device__ void left41(const Pencil &pc, double *df1, double *df2, double *df3, 
 
 // The WCNS_E_5_41 right boundary rows n-2..n (WCNSE5.f90:2518-2751).  f[k] holds face n-k.
 template <int FLUX>
-__device__ void right41(const Pencil &pc, double *dfn, double *dfn1, double *dfn2, double efix, double gama,
+__device__ __noinline__ void right41(const Pencil &pc, double *dfn, double *dfn1, double *dfn2, double efix, double gama,
                         int &nerr, int &bad)
 {
     int n = pc.n;
@@ -223,6 +223,22 @@ __device__ void right41(const Pencil &pc, double *dfn, double *dfn1, double *dfn
     for (int m = 0; m < 5; m++) dfn2[m] = -(f[1][m] - 27.0 * f[2][m] + 27.0 * f[3][m] - f[4][m]) / 24.0;
 }
 
+// The scheme-4 boundary states (WCNSE5.f90:2192-2213): which 0,1,2 = qwl(0), qwl(1)=qwr(1), qwl(2)
+// at a left physical boundary; 3,4,5 = qwr(n)=qwl(n), qwr(n-1)=qwl(n-1), qwr(n-2) at a right one.
+__device__ __noinline__ void closure_state(const Pencil &pc, int which, double *out)
+{
+    double q1[5], q2[5], q3[5], q4[5];
+    int n = pc.n;
+    if (which < 3) { load_q(pc, 1, q1); load_q(pc, 2, q2); load_q(pc, 3, q3); load_q(pc, 4, q4); }
+    else { load_q(pc, n, q1); load_q(pc, n - 1, q2); load_q(pc, n - 2, q3); load_q(pc, n - 3, q4); }
+    int c = which % 3;
+    for (int m = 0; m < 5; m++) {
+        if (c == 0) out[m] = (35.0 * q1[m] - 35.0 * q2[m] + 21.0 * q3[m] - 5.0 * q4[m]) / 16.0;
+        else if (c == 1) out[m] = (5.0 * q1[m] + 15.0 * q2[m] - 5.0 * q3[m] + q4[m]) / 16.0;
+        else out[m] = (-q1[m] + 9.0 * q2[m] + 9.0 * q3[m] - q4[m]) / 16.0;
+    }
+}
+
 template <int DIR, int FLUX>
 __global__ void __launch_bounds__(128) k_rhs_inv(DBlock b, DFields F, DParams P, int nseg)
 {
@@ -262,31 +278,7 @@ __global__ void __launch_bounds__(128) k_rhs_inv(DBlock b, DFields F, DParams P,
     if (a <= 2 && F1 < 3) F1 = 3;
     if (bb >= n - 1 && F0 > n - 3) F0 = n - 3;
 
-    // scheme-4 closure states (WCNSE5.f90:2192-2213)
-    double cl0[5], cl1[5], cl2[5], cr0[5], cr1[5], cr2[5];
-    if (lclos && F0 <= 2) {
-        double q1[5], q2[5], q3[5], q4[5];
-        load_q(pc, 1, q1); load_q(pc, 2, q2); load_q(pc, 3, q3); load_q(pc, 4, q4);
-#pragma unroll
-        for (int m = 0; m < 5; m++) {
-            cl0[m] = (35.0 * q1[m] - 35.0 * q2[m] + 21.0 * q3[m] - 5.0 * q4[m]) / 16.0;
-            cl1[m] = (5.0 * q1[m] + 15.0 * q2[m] - 5.0 * q3[m] + q4[m]) / 16.0;
-            cl2[m] = (-q1[m] + 9.0 * q2[m] + 9.0 * q3[m] - q4[m]) / 16.0;
-        }
-    }
-    if (rclos && F1 >= n - 2) {
-        double q1[5], q2[5], q3[5], q4[5];
-        load_q(pc, n, q1); load_q(pc, n - 1, q2); load_q(pc, n - 2, q3); load_q(pc, n - 3, q4);
-#pragma unroll
-        for (int m = 0; m < 5; m++) {
-            cr0[m] = (35.0 * q1[m] - 35.0 * q2[m] + 21.0 * q3[m] - 5.0 * q4[m]) / 16.0;
-            cr1[m] = (5.0 * q1[m] + 15.0 * q2[m] - 5.0 * q3[m] + q4[m]) / 16.0;
-            cr2[m] = (-q1[m] + 9.0 * q2[m] + 9.0 * q3[m] - q4[m]) / 16.0;
-        }
-    }
-
     double qw[5][5];       // nodes nd-2..nd+2
-    double tw[4][3];       // metrics of nodes f-1..f+2 for the face f = nd-1
     double fw[6][5];       // faces fc-5..fc
     double qwl_prev[5];
     int nerr = 0, bad = 0;
@@ -307,22 +299,21 @@ __global__ void __launch_bounds__(128) k_rhs_inv(DBlock b, DFields F, DParams P,
         }
         // closure overrides of qwl(nd) and qwr(nd-1); the right side wins on overlap
         {
-            int f = nd;
-            const double *src = nullptr;
-            if (rclos && f == n) src = cr0;
-            else if (rclos && f == n - 1) src = cr1;
-            else if (lclos && f == 0) src = cl0;
-            else if (lclos && f == 1) src = cl1;
-            else if (lclos && f == 2) src = cl2;
-            if (src) for (int m = 0; m < 5; m++) qwl_cur[m] = src[m];
+            int f = nd, c = -1;
+            if (rclos && f == n) c = 3;
+            else if (rclos && f == n - 1) c = 4;
+            else if (lclos && f == 0) c = 0;
+            else if (lclos && f == 1) c = 1;
+            else if (lclos && f == 2) c = 2;
+            if (c >= 0) closure_state(pc, c, qwl_cur);
             f = nd - 1;
-            src = nullptr;
-            if (rclos && f == n) src = cr0;
-            else if (rclos && f == n - 1) src = cr1;
-            else if (rclos && f == n - 2) src = cr2;
-            else if (lclos && f == 0) src = cl0;
-            else if (lclos && f == 1) src = cl1;
-            if (src) for (int m = 0; m < 5; m++) qwr_m1[m] = src[m];
+            c = -1;
+            if (rclos && f == n) c = 3;
+            else if (rclos && f == n - 1) c = 4;
+            else if (rclos && f == n - 2) c = 5;
+            else if (lclos && f == 0) c = 0;
+            else if (lclos && f == 1) c = 1;
+            if (c >= 0) closure_state(pc, c, qwr_m1);
         }
         int fc = nd - 1;
         if (fc >= F0) {
@@ -333,8 +324,9 @@ __global__ void __launch_bounds__(128) k_rhs_inv(DBlock b, DFields F, DParams P,
             for (int m = 0; m < 5; m++) { ql[m] = qwl_prev[m]; qr[m] = qwr_m1[m]; }
             bool fl = ql[0] <= RMIN_LIMIT || ql[4] <= PMIN_LIMIT;
             bool fr = qr[0] <= RMIN_LIMIT || qr[4] <= PMIN_LIMIT;
-            if (fl) { nerr++; int s = fc > 1 ? fc : 1; for (int m = 0; m < 5; m++) ql[m] = qw[s - nd + 2][m]; }
-            if (fr) { nerr++; int s = fc + 1 < n ? fc + 1 : n; for (int m = 0; m < 5; m++) qr[m] = qw[s - nd + 2][m]; }
+            // window slot 1 holds node nd-1 = fc, slot 2 node nd = fc+1 (no dynamic register indexing)
+            if (fl) { nerr++; bool s2 = fc < 1; for (int m = 0; m < 5; m++) ql[m] = s2 ? qw[2][m] : qw[1][m]; }
+            if (fr) { nerr++; bool s1 = fc + 1 > n; for (int m = 0; m < 5; m++) qr[m] = s1 ? qw[1][m] : qw[2][m]; }
 #pragma unroll
             for (int l = 0; l < 5; l++)
 #pragma unroll
@@ -393,7 +385,6 @@ __global__ void __launch_bounds__(128) k_rhs_inv(DBlock b, DFields F, DParams P,
 #pragma unroll
         for (int m = 0; m < 5; m++) qwl_prev[m] = qwl_cur[m];
     }
-    (void)tw;
 
     if (s41 && lclos && a == 1) {
         double d1[5], d2[5], d3[5];
@@ -424,26 +415,31 @@ __global__ void __launch_bounds__(128) k_rhs_inv(DBlock b, DFields F, DParams P,
 
 }  // namespace
 
-int launch_rhs_inv(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
+int launch_rhs_inv_dir(int dir, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
 {
-    for (int dir = 0; dir < 3; dir++) {
-        int n = dir == 0 ? b.ni : (dir == 1 ? b.nj : b.nk);
-        if (n <= 4) continue;                           // Convect.f90:150,314,479
-        long npen = dir == 0 ? (long)b.nj * b.nk : (dir == 1 ? (long)b.ni * b.nk : (long)b.ni * b.nj);
-        int nseg = (n + SEG - 1) / SEG;
-        long nthr = npen * nseg;
-        dim3 grid((unsigned)((nthr + 127) / 128)), blk(128);
-        if (P.nflux == 3) {
-            if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 3>), grid, blk, 0, s, b, F, P, nseg);
-            if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 3>), grid, blk, 0, s, b, F, P, nseg);
-            if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 3>), grid, blk, 0, s, b, F, P, nseg);
-        } else {
-            if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 1>), grid, blk, 0, s, b, F, P, nseg);
-            if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 1>), grid, blk, 0, s, b, F, P, nseg);
-            if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 1>), grid, blk, 0, s, b, F, P, nseg);
-        }
+    int n = dir == 0 ? b.ni : (dir == 1 ? b.nj : b.nk);
+    if (n <= 4) return 0;                               // Convect.f90:150,314,479
+    long npen = dir == 0 ? (long)b.nj * b.nk : (dir == 1 ? (long)b.ni * b.nk : (long)b.ni * b.nj);
+    int nseg = (n + SEG - 1) / SEG;
+    long nthr = npen * nseg;
+    dim3 grid((unsigned)((nthr + 127) / 128)), blk(128);
+    if (P.nflux == 3) {
+        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 3>), grid, blk, 0, s, b, F, P, nseg);
+        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 3>), grid, blk, 0, s, b, F, P, nseg);
+        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 3>), grid, blk, 0, s, b, F, P, nseg);
+    } else {
+        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 1>), grid, blk, 0, s, b, F, P, nseg);
+        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 1>), grid, blk, 0, s, b, F, P, nseg);
+        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 1>), grid, blk, 0, s, b, F, P, nseg);
     }
     return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_rhs_inv(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
+{
+    for (int dir = 0; dir < 3; dir++)
+        if (launch_rhs_inv_dir(dir, b, F, P, s)) return -3;
+    return 0;
 }
 
 }  // namespace wcns

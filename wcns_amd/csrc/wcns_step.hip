@@ -710,17 +710,26 @@ int launch_npp_assign(const int *grp_start, const int *grp_n, const int *pts, co
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_sweeps(const DBlock &b, const DFields &F, const DParams &P, int prec, hipStream_t s)
+int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, int prec, hipStream_t s)
 {
     (void)prec;
     int smin = 3, smax = b.ni + b.nj + b.nk;
     long nthr = (long)b.nj * b.nk;
     dim3 g = grid1(nthr, 256), bl(256);
-    for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<0>, g, bl, 0, s, b, F, P, p);
-    for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<1>, g, bl, 0, s, b, F, P, p);
-    for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<2>, g, bl, 0, s, b, F, P, p);
-    for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<3>, g, bl, 0, s, b, F, P, p);
+    switch (kind) {
+    case 0: for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<0>, g, bl, 0, s, b, F, P, p); break;
+    case 1: for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<1>, g, bl, 0, s, b, F, P, p); break;
+    case 2: for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<2>, g, bl, 0, s, b, F, P, p); break;
+    default: for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<3>, g, bl, 0, s, b, F, P, p); break;
+    }
     return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_sweeps(const DBlock &b, const DFields &F, const DParams &P, int prec, hipStream_t s)
+{
+    for (int kind = 0; kind < 4; kind++)
+        if (launch_sweep(kind, b, F, P, prec, s)) return -3;
+    return 0;
 }
 
 int launch_update(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s)

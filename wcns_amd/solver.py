@@ -29,7 +29,10 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_set_lhs_precision", "wcns_set_prim", "wcns_get_field", "wcns_set_field", "wcns_boundary_all",
            "wcns_initial_q_dq_c_t", "wcns_timestep", "wcns_r_h_s_vis", "wcns_r_h_s_invis", "wcns_communicate_dq",
            "wcns_l_h_s", "wcns_communicate_pv", "wcns_residual", "wcns_step", "wcns_get_counter", "wcns_sync",
-           "wcns_region_images", "wcns_multipoint"]
+           "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get"]
+
+TIMING_SLOTS = ["bc", "init", "timestep", "rhs_vis", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
+                "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual"]
 
 
 class WcnsError(RuntimeError):
@@ -75,6 +78,8 @@ def load_library(path=LIBPATH):
     L.wcns_get_counter.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong)]
     L.wcns_region_images.argtypes = [vp, C.c_int, C.c_int, iptr, iptr, iptr, iptr]
     L.wcns_multipoint.argtypes = [vp, iptr, iptr, iptr, C.c_int]
+    L.wcns_timing.argtypes = [vp, C.c_int]
+    L.wcns_timing_get.argtypes = [vp, C.c_int, dptr, C.POINTER(C.c_longlong)]
     _lib = L
     return L
 
@@ -197,6 +202,19 @@ class Solver:
             a = np.ascontiguousarray(a.transpose(3, 0, 1, 2))
         a = np.ascontiguousarray(a).ravel()
         self._check(self.L.wcns_set_field(self.h, nb, FIELDS[name], _d(a)), "set_field")
+
+    def timing(self, on=True):
+        """Start (clear) or stop recording HIP events around every kernel slot."""
+        self._check(self.L.wcns_timing(self.h, 1 if on else 0))
+
+    def timings(self):
+        """{slot: (total_ms, launches)} since the last timing(True)."""
+        out = {}
+        for i, name in enumerate(TIMING_SLOTS):
+            ms, n = C.c_double(), C.c_longlong()
+            self._check(self.L.wcns_timing_get(self.h, i, C.byref(ms), C.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
 
     def counter(self, which):
         v = C.c_longlong()
