@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -63,6 +64,11 @@ struct wcns_ctx {
     double *t = nullptr, *metric[10] = {nullptr}, *rhs = nullptr, *dq = nullptr;
     double *dtdt = nullptr, *sr[3] = {nullptr}, *srv[3] = {nullptr}, *visl = nullptr;
     double *dn = nullptr;                   // 12 node derivatives (viscous)
+    double *coed = nullptr;                 // LU-SGS diagonal per node
+    float *dq32 = nullptr;                  // dQ of the FP32 implicit operator
+    std::vector<SweepTiles> tiles;          // per block
+    int *d_sweep_err = nullptr;
+    bool tiled_sweeps = true;
     unsigned long long *counters = nullptr;
     DBlock *d_blocks = nullptr;
     int *d_flag = nullptr;
@@ -109,7 +115,8 @@ struct wcns_ctx {
         F.rhs = rhs; F.dq = dq; F.dtdt = dtdt;
         F.sra = sr[0]; F.srb = sr[1]; F.src = sr[2];
         F.srva = srv[0]; F.srvb = srv[1]; F.srvc = srv[2];
-        F.visl = visl; F.counters = counters; F.total = total;
+        F.visl = visl; F.counters = counters; F.total = total; F.coed = coed;
+        F.dq32 = lhs_prec == WCNS_LHS_FP32 ? dq32 : nullptr;
         return F;
     }
 };
@@ -324,6 +331,11 @@ int check_counters(wcns_ctx *ctx)
     CK(hipStreamSynchronize(ctx->stream));
     if (c[2]) return fail(ctx, WCNS_E_ROE, "sqrt(c2) is a math error in flux_Roe (flux.f90:179-182)");
     if (c[4]) return fail(ctx, WCNS_E_REPAIR, "too many abnormal nodes in update_limited (Sub2.f90:781-785)");
+    if (ctx->d_sweep_err) {
+        int e = 0;
+        CK(hipMemcpy(&e, ctx->d_sweep_err, sizeof(int), hipMemcpyDeviceToHost));
+        if (e) return fail(ctx, WCNS_E_HIP, "a pipelined sweep wait expired");
+    }
     return 0;
 }
 
@@ -341,6 +353,7 @@ int wcns_create(wcns_ctx **out, int device, const wcns_params *prm)
     wcns_ctx *ctx = new wcns_ctx();
     ctx->device = device;
     ctx->prm = *prm;
+    if (const char *sw = getenv("WCNS_SWEEP")) ctx->tiled_sweeps = strcmp(sw, "plane") != 0;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return WCNS_E_HIP;
@@ -358,7 +371,8 @@ int wcns_destroy(wcns_ctx *ctx)
     for (int b = 0; b < 2; b++) for (int m = 0; m < 5; m++) hipFree(ctx->prim[b][m]);
     for (int m = 0; m < 10; m++) hipFree(ctx->metric[m]);
     for (int m = 0; m < 3; m++) { hipFree(ctx->sr[m]); hipFree(ctx->srv[m]); }
-    void *ps[] = {ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
+    for (auto &st : ctx->tiles) { hipFree(st.order_fwd); hipFree(st.order_bwd); hipFree(st.progress); }
+    void *ps[] = {ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
                   ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
@@ -406,7 +420,9 @@ int wcns_add_region(wcns_ctx *ctx, int nb, const int s_raw[6], int bctype, const
 int wcns_set_lhs_precision(wcns_ctx *ctx, int prec)
 {
     if (!ctx || (prec != WCNS_LHS_FP64 && prec != WCNS_LHS_FP32)) return WCNS_E_ARG;
-    if (prec == WCNS_LHS_FP32) return fail(ctx, WCNS_E_UNSUPPORTED, "FP32 implicit operator not built yet");
+    if (prec == WCNS_LHS_FP32 && !ctx->tiled_sweeps)
+        return fail(ctx, WCNS_E_UNSUPPORTED, "the FP32 implicit operator runs in the pipelined sweeps only");
+    if (prec == WCNS_LHS_FP32 && ctx->setup_done && !ctx->dq32) TRY(dalloc(ctx, &ctx->dq32, 5 * ctx->total));
     ctx->lhs_prec = prec;
     return WCNS_OK;
 }
@@ -459,6 +475,30 @@ int wcns_setup(wcns_ctx *ctx)
     TRY(dalloc(ctx, &ctx->dtdt, T));
     TRY(dalloc(ctx, &ctx->visl, T));
     if (p.nvis == 1) TRY(dalloc(ctx, &ctx->dn, 12 * T));
+    TRY(dalloc(ctx, &ctx->coed, T));
+    TRY(dalloc(ctx, &ctx->d_sweep_err, 1));
+    // sweep tiles: TJ x TK columns per workgroup, dispatched in topological order
+    for (auto &b : ctx->blocks) {
+        SweepTiles st{};
+        st.TJ = std::min(16, b.nj);
+        st.TK = std::max(1, std::min(64 / st.TJ, b.nk));
+        st.nTJ = (b.nj + st.TJ - 1) / st.TJ;
+        st.nTK = (b.nk + st.TK - 1) / st.TK;
+        int nt = st.nTJ * st.nTK;
+        std::vector<int> tl(nt);
+        for (int t = 0; t < nt; t++) tl[t] = t;
+        auto key = [&](int t) { return std::make_pair(t % st.nTJ + t / st.nTJ, t % st.nTJ); };
+        std::sort(tl.begin(), tl.end(), [&](int a, int c) { return key(a) < key(c); });
+        std::vector<int> tb(tl.rbegin(), tl.rend());
+        TRY(dalloc(ctx, &st.order_fwd, nt));
+        TRY(dalloc(ctx, &st.order_bwd, nt));
+        TRY(dalloc(ctx, &st.progress, nt));
+        CK(hipMemcpy(st.order_fwd, tl.data(), 4 * nt, hipMemcpyHostToDevice));
+        CK(hipMemcpy(st.order_bwd, tb.data(), 4 * nt, hipMemcpyHostToDevice));
+        st.err = ctx->d_sweep_err;
+        st.spin_limit = 2000000;
+        ctx->tiles.push_back(st);
+    }
     TRY(dalloc(ctx, &ctx->counters, 8));
     TRY(dalloc(ctx, &ctx->d_flag, ctx->flag_cap));
     std::vector<DBlock> hb;
@@ -586,7 +626,9 @@ static int field_ptr(wcns_ctx *ctx, int which, double **p, int *ncomp)
     case 11: case 12: case 13: *p = ctx->srv[which - 11]; return 0;
     case 14: *p = ctx->visl; return 0;
     case 16: *p = ctx->metric[9]; return 0;
-    case 100: *p = ctx->dq; *ncomp = 5; return 0;
+    case 100:
+        if (ctx->lhs_prec == WCNS_LHS_FP32) return WCNS_E_UNSUPPORTED;
+        *p = ctx->dq; *ncomp = 5; return 0;
     case 102: *p = ctx->rhs; *ncomp = 5; return 0;
     default:
         if (which >= 17 && which <= 25) { *p = ctx->metric[which - 17]; return 0; }
@@ -750,11 +792,16 @@ int wcns_l_h_s(wcns_ctx *ctx)
             if (g.nb == (int)nb) launch_dq_zero_face(g, db, F.rhs, F.total, ctx->stream);
         int nplanes = db.ni + db.nj + db.nk - 2;
         for (int kind = 0; kind < 4; kind++) {
-            Scope sc(ctx, WCNS_T_SWEEP_LU_L + kind, nplanes);
-            if (launch_sweep(kind, db, F, ctx->dp, ctx->lhs_prec, ctx->stream)) return fail(ctx, WCNS_E_HIP, "sweeps");
+            Scope sc(ctx, WCNS_T_SWEEP_LU_L + kind, ctx->tiled_sweeps ? 1 : nplanes);
+            int e = ctx->tiled_sweeps ? launch_sweep_tiled(kind, db, F, ctx->dp, ctx->tiles[nb], ctx->lhs_prec, ctx->stream)
+                                      : launch_sweep(kind, db, F, ctx->dp, ctx->lhs_prec, ctx->stream);
+            if (e) return fail(ctx, WCNS_E_HIP, "sweeps");
         }
         for (auto &g : ctx->phys)
-            if (g.nb == (int)nb) launch_dq_zero_face(g, db, F.dq, F.total, ctx->stream);
+            if (g.nb == (int)nb) {
+                if (F.dq32) launch_dq_zero_face32(g, db, F.dq32, F.total, ctx->stream);
+                else launch_dq_zero_face(g, db, F.dq, F.total, ctx->stream);
+            }
         CK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(unsigned long long), ctx->stream));
         Scope sc(ctx, WCNS_T_UPDATE);
         if (launch_update(db, F, ctx->dp, ctx->d_flag, ctx->flag_cap, ctx->stream)) return fail(ctx, WCNS_E_HIP, "update");

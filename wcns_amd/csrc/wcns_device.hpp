@@ -45,6 +45,7 @@ struct DFields {
     const double *kcx, *kcy, *kcz, *etx, *ety, *etz, *ctx, *cty, *ctz, *vol;
     double *rhs, *dq;
     double *dtdt, *sra, *srb, *src, *srva, *srvb, *srvc, *visl;
+    double *coed;                       // LU-SGS diagonal 1/(1/dt + sum of radii), per node
     float *dq32;                        // FP32 sweep operand (mixed mode)
     unsigned long long *counters;       // 0 fallbacks, 1 repairs, 2 roe, 3 nrepair list len
     long total;

@@ -36,6 +36,18 @@ __host__ __device__ inline void region_image(const DRegion &g, int i, int j, int
 
 int launch_rhs_inv(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
 int launch_rhs_inv_dir(int dir, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
+// Tile decomposition of one block's (j,k) columns for the pipelined sweeps.
+struct SweepTiles {
+    int TJ, TK, nTJ, nTK;
+    int *order_fwd, *order_bwd;   // blockIdx -> tile, topological for each sweep direction
+    int *progress;                // per-tile last published step
+    int *err;                     // set when a bounded wait expires
+    long spin_limit;
+};
+int launch_sweep_tiled(int kind, const DBlock &b, const DFields &F, const DParams &P, const SweepTiles &T,
+                       int prec, hipStream_t s);
+int launch_dq_zero_face32(const DRegion &g, const DBlock &b, float *arr, long total, hipStream_t s);
+int launch_fill_int(int *a, int n, int v, hipStream_t s);
 int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, int prec, hipStream_t s);
 
 int launch_vis_node_deriv(const DBlock &b, const DFields &F, double *dn, hipStream_t s);

@@ -252,9 +252,27 @@ __global__ void k_timestep(DBlock b, DFields F, DParams P)
         dt = P.cfl / rabc;
     }
     F.dtdt[g] = dt;
+    if (F.coed) {
+        // LHS.f90:602-645 getcoed, the same for all four sweeps of the step
+        double rad_ns = sra + srb + src;
+        if (P.nvis >= 1) rad_ns = rad_ns + F.srva[g] + F.srvb[g] + F.srvc[g];
+        double ct = 1.0 / dt;
+        F.coed[g] = 1.0 / (ct + 1.0 * 1.0 * rad_ns);
+    }
 }
 
 // ------------------------------------------------- interface averaging
+__global__ void k_dq_zero_face32(DRegion g, DBlock b, float *arr, long total)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.npts) return;
+    int i, j, k;
+    region_point(g, t, i, j, k);
+    long q = gidx(b, i, j, k);
+#pragma unroll
+    for (int m = 0; m < 5; m++) arr[m * total + q] = 0.0f;
+}
+
 __global__ void k_dq_zero_face(DRegion g, DBlock b, double *arr, long total)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -418,6 +436,73 @@ __device__ inline double coed_at(const DFields &F, const DParams &P, long g)
 }
 
 // kind 0 lusgs_l (LHS.f90:381-478), 1 lusgs_u (:480-577), 2 gs_pr_l (:1777-1866), 3 gs_pr_u (:1956-2046)
+template <typename T> __device__ inline T *dq_of(const DFields &F);
+template <> __device__ inline double *dq_of<double>(const DFields &F) { return F.dq; }
+template <> __device__ inline float *dq_of<float>(const DFields &F) { return F.dq32; }
+
+// One node of a sweep in arithmetic type T (double: the reference's FP64
+// operator; float: the mixed-precision operator, with R, the primitives and
+// the metrics rounded to FP32 on load and dQ stored in FP32).
+template <int KIND, typename T>
+__device__ inline void sweep_node(const DBlock &b, const DFields &F, const DParams &P, int i, int j, int k)
+{
+    long g = gidx(b, i, j, k);
+    T coed = (T)(F.coed ? F.coed[g] : coed_at(F, P, g));
+    const long T5 = F.total;
+    T *dq = dq_of<T>(F);
+    T rhs0[5];
+    if (KIND == 0 || KIND == 1) {
+        int off = KIND == 0 ? -1 : 1;
+        T dn[3][5], fo[3][5];
+        long gn[3] = {gidx(b, i + off, j, k), gidx(b, i, j + off, k), gidx(b, i, j, k + off)};
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int m = 0; m < 5; m++) dn[d][m] = dq[m * T5 + gn[d]];
+#pragma unroll
+        for (int d = 0; d < 3; d++) neighbour_term<T>(b, F, P, i, j, k, d, off, dn[d], fo[d]);
+#pragma unroll
+        for (int m = 0; m < 5; m++) rhs0[m] = T(1.0) * (fo[0][m] + fo[1][m] + fo[2][m]);
+        if (P.nvis > 0) {
+            T rva = (T)F.srva[gn[0]], rvb = (T)F.srvb[gn[1]], rvc = (T)F.srvc[gn[2]];
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                T v = T(0.5) * (rva * dn[0][m] + rvb * dn[1][m] + rvc * dn[2][m]);
+                rhs0[m] = KIND == 0 ? rhs0[m] + v : rhs0[m] - v;
+            }
+        }
+        if (KIND == 0) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) dq[m * T5 + g] = (-(T)F.rhs[m * T5 + g] + rhs0[m]) * coed;
+        } else {
+#pragma unroll
+            for (int m = 0; m < 5; m++) dq[m * T5 + g] = dq[m * T5 + g] - rhs0[m] * coed;
+        }
+    } else {
+        T dl[3][5], du[3][5], fl[3][5], fu[3][5];
+        long gl[3] = {gidx(b, i - 1, j, k), gidx(b, i, j - 1, k), gidx(b, i, j, k - 1)};
+        long gu[3] = {gidx(b, i + 1, j, k), gidx(b, i, j + 1, k), gidx(b, i, j, k + 1)};
+        bool lo[3] = {i == 1, j == 1, k == 1}, hi[3] = {i == b.ni, j == b.nj, k == b.nk};
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                dl[d][m] = lo[d] ? T(0.0) : dq[m * T5 + gl[d]];
+                du[d][m] = hi[d] ? T(0.0) : dq[m * T5 + gu[d]];
+            }
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            neighbour_term<T>(b, F, P, i, j, k, d, -1, dl[d], fl[d]);
+            neighbour_term<T>(b, F, P, i, j, k, d, 1, du[d], fu[d]);
+        }
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            T r0 = T(1.0) * (fl[0][m] + fl[1][m] + fl[2][m] - fu[0][m] - fu[1][m] - fu[2][m]);
+            dq[m * T5 + g] = (r0 - (T)F.rhs[m * T5 + g]) * coed;
+        }
+    }
+}
+
 template <int KIND>
 __global__ void __launch_bounds__(256) k_sweep_plane(DBlock b, DFields F, DParams P, int s)
 {
@@ -432,61 +517,75 @@ __global__ void __launch_bounds__(256) k_sweep_plane(DBlock b, DFields F, DParam
     int k = (int)(t / nj_) + 1;
     int i = s - j - k;
     if (i < 1 || i > b.ni) return;
-    long g = gidx(b, i, j, k);
-    double coed = coed_at(F, P, g);
-    const long T5 = F.total;
-    double *dq = F.dq;
-    double rhs0[5];
-    if (KIND == 0 || KIND == 1) {
-        int off = KIND == 0 ? -1 : 1;
-        double dn[3][5], fo[3][5];
-        long gn[3] = {gidx(b, i + off, j, k), gidx(b, i, j + off, k), gidx(b, i, j, k + off)};
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int m = 0; m < 5; m++) dn[d][m] = dq[m * T5 + gn[d]];
-#pragma unroll
-        for (int d = 0; d < 3; d++) neighbour_term<double>(b, F, P, i, j, k, d, off, dn[d], fo[d]);
-#pragma unroll
-        for (int m = 0; m < 5; m++) rhs0[m] = 1.0 * (fo[0][m] + fo[1][m] + fo[2][m]);
-        if (P.nvis > 0) {
-            double rva = F.srva[gn[0]], rvb = F.srvb[gn[1]], rvc = F.srvc[gn[2]];
-#pragma unroll
-            for (int m = 0; m < 5; m++) {
-                double v = 0.5 * (rva * dn[0][m] + rvb * dn[1][m] + rvc * dn[2][m]);
-                rhs0[m] = KIND == 0 ? rhs0[m] + v : rhs0[m] - v;
-            }
-        }
-        if (KIND == 0) {
-#pragma unroll
-            for (int m = 0; m < 5; m++) dq[m * T5 + g] = (-F.rhs[m * T5 + g] + rhs0[m]) * coed;
-        } else {
-#pragma unroll
-            for (int m = 0; m < 5; m++) dq[m * T5 + g] = dq[m * T5 + g] - rhs0[m] * coed;
-        }
+    sweep_node<KIND, double>(b, F, P, i, j, k);
+}
+
+// Persistent, tile-pipelined sweep.  Workgroup = a TJ x TK tile of (j,k) columns,
+// one column per thread; it walks the hyperplanes s = i+j+k of its columns in
+// sweep order, each thread updating node i = s-j-k of its column.  Every node
+// of plane s reads only nodes of planes s-1 (its own column and the j-1, k-1
+// columns, i.e. its own tile or the two predecessor tiles) and, in the GS
+// sweeps, old values of plane s+1.  A tile therefore starts plane s once both
+// predecessor tiles have published plane s-1 (per-tile progress counters,
+// release/acquire at agent scope); successors cannot overwrite plane s+1
+// before this tile publishes plane s.  Workgroups are dispatched in a
+// topological order of the tile graph, so every awaited tile is already
+// running or done; each wait is bounded and reports through err.
+template <int KIND, typename T>
+__global__ void __launch_bounds__(256) k_sweep_tiled(DBlock b, DFields F, DParams P, const int *order, int *progress,
+                                                   int nTJ, int nTK, int TJ, int TK, long spin_limit, int *err)
+{
+    const bool fwd = (KIND == 0 || KIND == 2);
+    int tile = order[blockIdx.x];
+    int J = tile % nTJ, K = tile / nTJ;
+    int jl = threadIdx.x % TJ, kl = threadIdx.x / TJ;
+    int j = 1 + J * TJ + jl, k = 1 + K * TK + kl;
+    bool col = (jl < TJ) && (kl < TK) && j <= b.nj && k <= b.nk;
+    int jlo = 1 + J * TJ, klo = 1 + K * TK;
+    int jhi = min(b.nj, (J + 1) * TJ), khi = min(b.nk, (K + 1) * TK);
+    int smin = jlo + klo + 1, smax = jhi + khi + b.ni;
+    const int SMAXG = b.ni + b.nj + b.nk + 1;
+    // predecessor tiles in this sweep's order
+    int p1 = -1, p2 = -1;
+    if (fwd) {
+        if (J > 0) p1 = tile - 1;
+        if (K > 0) p2 = tile - nTJ;
     } else {
-        double dl[3][5], du[3][5], fl[3][5], fu[3][5];
-        long gl[3] = {gidx(b, i - 1, j, k), gidx(b, i, j - 1, k), gidx(b, i, j, k - 1)};
-        long gu[3] = {gidx(b, i + 1, j, k), gidx(b, i, j + 1, k), gidx(b, i, j, k + 1)};
-        bool lo[3] = {i == 1, j == 1, k == 1}, hi[3] = {i == b.ni, j == b.nj, k == b.nk};
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int m = 0; m < 5; m++) {
-                dl[d][m] = lo[d] ? 0.0 : dq[m * T5 + gl[d]];
-                du[d][m] = hi[d] ? 0.0 : dq[m * T5 + gu[d]];
-            }
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            neighbour_term<double>(b, F, P, i, j, k, d, -1, dl[d], fl[d]);
-            neighbour_term<double>(b, F, P, i, j, k, d, 1, du[d], fu[d]);
-        }
-#pragma unroll
-        for (int m = 0; m < 5; m++) {
-            double r0 = 1.0 * (fl[0][m] + fl[1][m] + fl[2][m] - fu[0][m] - fu[1][m] - fu[2][m]);
-            dq[m * T5 + g] = (r0 - F.rhs[m * T5 + g]) * coed;
-        }
+        if (J < nTJ - 1) p1 = tile + 1;
+        if (K < nTK - 1) p2 = tile + nTJ;
     }
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int step = 0; step <= smax - smin; step++) {
+        int s = fwd ? smin + step : smax - step;
+        int t = fwd ? s : SMAXG - s;            // monotone step index
+        if (threadIdx.x == 0 && !bad) {
+            long spins = 0;
+            while ((p1 >= 0 && __hip_atomic_load(&progress[p1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < t - 1) ||
+                   (p2 >= 0 && __hip_atomic_load(&progress[p2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < t - 1)) {
+                if (++spins > spin_limit) { bad = 1; atomicExch(err, 1); break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // later loads see the predecessors' planes
+        }
+        __syncthreads();
+        if (col) {
+            int i = s - j - k;
+            if (i >= 1 && i <= b.ni) sweep_node<KIND, T>(b, F, P, i, j, k);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&progress[tile], t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&progress[tile], 0x7fffffff, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_fill_int(int *a, int n, int v)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
 }
 
 // -------------------------------------------------------------- update
@@ -507,7 +606,7 @@ __global__ void k_update(DBlock b, DFields F, DParams P, int *flag_list, int cap
     q[0] = r0; q[1] = r0 * u0; q[2] = r0 * v0; q[3] = r0 * w0; q[4] = getem(r0, u0, v0, w0, p0, gama);
     double d[5];
 #pragma unroll
-    for (int m = 0; m < 5; m++) d[m] = F.dq[m * F.total + g];
+    for (int m = 0; m < 5; m++) d[m] = F.dq32 ? (double)F.dq32[m * F.total + g] : F.dq[m * F.total + g];
     double qt[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) qt[m] = q[m] + d[m];
@@ -591,7 +690,8 @@ __global__ void __launch_bounds__(RES_THREADS) k_residual(DBlock b, DFields F, d
         long g = gidx(b, i, j, k);
         double dtime1 = 1.0 / (F.dtdt[g] * F.vol[g]);
         for (int m = 0; m < 5; m++) {
-            double dresm = fabs(F.dq[m * F.total + g]) * dtime1;
+            double dqm = F.dq32 ? (double)F.dq32[m * F.total + g] : F.dq[m * F.total + g];
+            double dresm = fabs(dqm) * dtime1;
             if (dresm > mx) { mx = dresm; arg = t * 5 + m; }
             sum = sum + dresm * dresm;
         }
@@ -684,6 +784,12 @@ int launch_dq_zero_face(const DRegion &g, const DBlock &b, double *arr, long tot
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
+int launch_dq_zero_face32(const DRegion &g, const DBlock &b, float *arr, long total, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dq_zero_face32, grid1(g.npts, 128), dim3(128), 0, s, g, b, arr, total);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
 int launch_npp_gather(const int *pts, int npts, const DBlock *dblocks, const DFields &F, double *buf, int nvar,
                       int use_vol, hipStream_t s)
 {
@@ -722,6 +828,40 @@ int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, 
     case 2: for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<2>, g, bl, 0, s, b, F, P, p); break;
     default: for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<3>, g, bl, 0, s, b, F, P, p); break;
     }
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_sweep_tiled(int kind, const DBlock &b, const DFields &F, const DParams &P, const SweepTiles &T,
+                       int prec, hipStream_t s)
+{
+    int ntiles = T.nTJ * T.nTK;
+    hipLaunchKernelGGL(k_fill_int, grid1(ntiles, 256), dim3(256), 0, s, T.progress, ntiles, 0);
+    const int *ord = (kind == 0 || kind == 2) ? T.order_fwd : T.order_bwd;
+    dim3 g(ntiles), bl(T.TJ * T.TK);
+#define WCNS_SWEEP_LAUNCH(K, TY) hipLaunchKernelGGL((k_sweep_tiled<K, TY>), g, bl, 0, s, b, F, P, ord, T.progress, \
+                                                    T.nTJ, T.nTK, T.TJ, T.TK, T.spin_limit, T.err)
+    if (prec == 1) {
+        switch (kind) {
+        case 0: WCNS_SWEEP_LAUNCH(0, float); break;
+        case 1: WCNS_SWEEP_LAUNCH(1, float); break;
+        case 2: WCNS_SWEEP_LAUNCH(2, float); break;
+        default: WCNS_SWEEP_LAUNCH(3, float); break;
+        }
+    } else {
+        switch (kind) {
+        case 0: WCNS_SWEEP_LAUNCH(0, double); break;
+        case 1: WCNS_SWEEP_LAUNCH(1, double); break;
+        case 2: WCNS_SWEEP_LAUNCH(2, double); break;
+        default: WCNS_SWEEP_LAUNCH(3, double); break;
+        }
+    }
+#undef WCNS_SWEEP_LAUNCH
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_fill_int(int *a, int n, int v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_int, grid1(n, 256), dim3(256), 0, s, a, n, v);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
