@@ -82,13 +82,24 @@ def main():
     ap.add_argument("--lhs", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="print the per-kernel table to stderr")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding by block is a design, see DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU run would shard the grid by block, one rank per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n}))
+        return
     from wcns_amd.solver import Solver
 
     case, wl_name = build_case(args.workload)

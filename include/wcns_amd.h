@@ -45,6 +45,9 @@ typedef struct {
     int nscheme;   /* 4 (WCNS_E_5) or 41 (WCNS_E_5_41)                   */
     int nflux;     /* 1 Steger-Warming, 3 Roe                            */
     int ntmst;     /* 0 local time step                                  */
+    /* namelist force (param.dat): reference area, length and moment centre */
+    double sref, lfref, xref, yref, zref;
+    int nwholefield; /* 1 whole body; 0 half model (forces doubled)      */
 } wcns_params;
 
 /* Precision of the implicit operator: the residual, WCNS weights and the
@@ -103,6 +106,13 @@ int wcns_residual(wcns_ctx *ctx, double *rms, double *resmax, int loc[5]);
  * With rms_hist == NULL the residual is still computed on the device each
  * step but only read back at the end (no per-step host synchronisation). */
 int wcns_step(wcns_ctx *ctx, int nsteps, double *rms_hist);
+
+/* Force and moment coefficients over every no-slip wall (bctype 2 or 2x):
+ * out = cfx, cfy, cfz, cmx, cmy, cmz, cd, cl, xcp.  Replaces force + force_dif
+ * (src/PostProcess.f90:1415-1503, 1505-1680); the quad quadrature is getforce
+ * (src/PostProcess.f90:1684-1796), the stress Sub1.f90:181-198 with the
+ * one-sided wall derivatives of getuvwtder (Sub1.f90:383-480). */
+int wcns_force(wcns_ctx *ctx, double out[9]);
 
 /* device error/diagnostic counters: 0 WCNS positivity fallbacks
  * (WCNSE5.f90:2224-2245), 1 repaired nodes (Sub2.f90:661-767), 2 Roe c^2<=0 */

@@ -15,7 +15,8 @@ LIBPATH = os.path.join(HERE, "liboracle.so")
 class OParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("moo", "reynolds", "attack", "sideslip", "tref", "twall",
                                           "cfl", "timedt_rate", "efix", "csrv", "gama", "prl", "prt")] + \
-               [(n, C.c_int) for n in ("nvis", "nlamtur", "nscheme", "nflux", "ntmst")]
+               [(n, C.c_int) for n in ("nvis", "nlamtur", "nscheme", "nflux", "ntmst")] + \
+               [(n, C.c_double) for n in ("sref", "lfref", "xref", "yref", "zref")] + [("nwholefield", C.c_int)]
 
 
 FIELDS = {"r": 0, "u": 1, "v": 2, "w": 3, "p": 4, "t": 5, "c": 6, "dtdt": 7, "sra": 8, "srb": 9,
@@ -63,6 +64,7 @@ def lib():
                                  C.c_double, C.c_double]
         L.o_wcns_line.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dptr,
                                   dptr, dptr]
+        L.o_force.argtypes = [C.c_void_p, dptr]
         L.o_mxdq_std.argtypes = [dptr, dptr, dptr, dptr, C.c_double, C.c_int, C.c_double]
         _lib = L
     return _lib
@@ -209,6 +211,12 @@ class OracleCase:
 
     def counter(self, which):
         return lib().o_counter(self.h, which)
+
+    def force(self):
+        """cfx, cfy, cfz, cmx, cmy, cmz, cd, cl, xcp (PostProcess.f90:1415-1503)."""
+        out = np.zeros(9)
+        lib().o_force(self.h, _d(out))
+        return out
 
     def region_images(self, nb, nr):
         blk = self.case.blocks[nb]

@@ -1910,3 +1910,149 @@ void o_view_params(const o_case *c, o_params *p, double *visc)
     *p = c->prm;
     *visc = c->visc;
 }
+
+
+/* ======================================================= wall force */
+/* PostProcess.f90:1684-1796 getforce (4-triangle Heron quadrature of a wall quad) */
+static double aver3(double a, double b, double c) { return (a + b + c) / 3.0; }
+static double halen(double a, double b, double c)
+{
+    double p = 0.5 * (a + b + c);
+    double S = p * (p - a) * (p - b) * (p - c);
+    if (S < 0) return 0.0;
+    return sqrt(S);
+}
+static double lenth(double x1, double y1, double z1, double x2, double y2, double z2)
+{
+    double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+static void getforce(const double *P /*12: p1..p4 xyz*/, const double *X /*12*/, const double *ref, double *d /*6*/)
+{
+    double p5[3], x5[3];
+    for (int a = 0; a < 3; a++) {
+        p5[a] = 0.25 * (P[a] + P[3 + a] + P[6 + a] + P[9 + a]);
+        x5[a] = 0.25 * (X[a] + X[3 + a] + X[6 + a] + X[9 + a]);
+    }
+#define XX(n) X[3 * (n)], X[3 * (n) + 1], X[3 * (n) + 2]
+    double r12 = lenth(XX(0), XX(1)), r23 = lenth(XX(1), XX(2)), r34 = lenth(XX(2), XX(3)), r41 = lenth(XX(3), XX(0));
+    double r15 = lenth(XX(0), x5[0], x5[1], x5[2]), r25 = lenth(XX(1), x5[0], x5[1], x5[2]);
+    double r35 = lenth(XX(2), x5[0], x5[1], x5[2]), r45 = lenth(XX(3), x5[0], x5[1], x5[2]);
+#undef XX
+    double sarea[4] = {halen(r12, r15, r25), halen(r23, r25, r35), halen(r34, r35, r45), halen(r41, r45, r15)};
+    const int ia[4] = {0, 1, 2, 3}, ib[4] = {1, 2, 3, 0};
+    for (int a = 0; a < 6; a++) d[a] = 0.0;
+    for (int t = 0; t < 4; t++) {
+        double pc[3], xc[3];
+        for (int a = 0; a < 3; a++) {
+            pc[a] = aver3(P[3 * ia[t] + a], P[3 * ib[t] + a], p5[a]);
+            xc[a] = aver3(X[3 * ia[t] + a], X[3 * ib[t] + a], x5[a]);
+        }
+        double fx = pc[0] * sarea[t], fy = pc[1] * sarea[t], fz = pc[2] * sarea[t];
+        double mx = (xc[1] - ref[1]) * fz - (xc[2] - ref[2]) * fy;
+        double my = (xc[2] - ref[2]) * fx - (xc[0] - ref[0]) * fz;
+        double mz = (xc[0] - ref[0]) * fy - (xc[1] - ref[1]) * fx;
+        d[0] = d[0] + fx; d[1] = d[1] + fy; d[2] = d[2] + fz;
+        d[3] = d[3] + mx; d[4] = d[4] + my; d[5] = d[5] + mz;
+    }
+}
+
+/* Sub1.f90:383-480 getuvwtder: second-order one-sided at block faces, central inside */
+static void uvwtder(const oblock *b, int i, int j, int k, int dir, double *d /*4*/)
+{
+    int n = dir == 0 ? b->ni : (dir == 1 ? b->nj : b->nk);
+    int l = dir == 0 ? i : (dir == 1 ? j : k);
+    const double aa = 1.5, bb = -2.0, cc = 0.5;
+#define AT(a, o) ((dir) == 0 ? (a)[G(b, (o), j, k)] : ((dir) == 1 ? (a)[G(b, i, (o), k)] : (a)[G(b, i, j, (o))]))
+    const double *fs[4] = {b->u, b->v, b->w, b->t};
+    for (int m = 0; m < 4; m++) {
+        const double *a = fs[m];
+        if (l == 1) d[m] = -(aa * AT(a, l) + bb * AT(a, 2) + cc * AT(a, 3));
+        else if (l == n) d[m] = aa * AT(a, l) + bb * AT(a, l - 1) + cc * AT(a, l - 2);
+        else d[m] = 0.5 * (AT(a, l + 1) - AT(a, l - 1));
+    }
+#undef AT
+}
+
+int o_force(o_case *c, double out[9])
+{
+    double cfx = 0, cfy = 0, cfz = 0, cmx = 0, cmy = 0, cmz = 0;
+    double ref[3] = {c->prm.xref, c->prm.yref, c->prm.zref};
+    for (int nb = 0; nb < c->nblocks; nb++) {
+        oblock *b = &c->blk[nb];
+        for (int r = 0; r < b->nregions; r++) {
+            oregion *g = &b->reg[r];
+            if (!(g->bctype == 2 || g->bctype / 10 == 2)) continue;
+            int bi = g->s_ed[0] - g->s_st[0] + 1, bj = g->s_ed[1] - g->s_st[1] + 1, bk = g->s_ed[2] - g->s_st[2] + 1;
+            double *pn3 = (double *)malloc(sizeof(double) * 3 * bi * bj * bk);
+#define PN(a, i, j, k) pn3[3 * (((i) - g->s_st[0]) + bi * (((j) - g->s_st[1]) + bj * ((k) - g->s_st[2]))) + (a)]
+            for (int is = g->s_st[0]; is <= g->s_ed[0]; is++)
+                for (int js = g->s_st[1]; js <= g->s_ed[1]; js++)
+                    for (int ks = g->s_st[2]; ks <= g->s_ed[2]; ks++) {
+                        double nx, ny, nz;
+                        getnxyz(b, is, js, ks, g->s_lr3d[0], g->s_lr3d[1], g->s_lr3d[2], &nx, &ny, &nz);
+                        long q = G(b, is, js, ks);
+                        double pn = 2.0 * (b->p[q] - c->poo);
+                        PN(0, is, js, ks) = g->s_lr * nx * pn;
+                        PN(1, is, js, ks) = g->s_lr * ny * pn;
+                        PN(2, is, js, ks) = g->s_lr * nz * pn;
+                        if (c->prm.nvis == 1) {
+                            double dk[4], de[4], dc[4];
+                            uvwtder(b, is, js, ks, 0, dk); uvwtder(b, is, js, ks, 1, de); uvwtder(b, is, js, ks, 2, dc);
+                            double kx = b->kcx[q], ky = b->kcy[q], kz = b->kcz[q], ex = b->etx[q], ey = b->ety[q];
+                            double ez = b->etz[q], cx = b->ctx[q], cy = b->cty[q], cz = b->ctz[q], vj = b->vol[q];
+                            /* Sub1.f90:251-270 getduvwdxyz */
+                            double dudx = (kx * dk[0] + ex * de[0] + cx * dc[0]) / vj, dudy = (ky * dk[0] + ey * de[0] + cy * dc[0]) / vj;
+                            double dudz = (kz * dk[0] + ez * de[0] + cz * dc[0]) / vj;
+                            double dvdx = (kx * dk[1] + ex * de[1] + cx * dc[1]) / vj, dvdy = (ky * dk[1] + ey * de[1] + cy * dc[1]) / vj;
+                            double dvdz = (kz * dk[1] + ez * de[1] + cz * dc[1]) / vj;
+                            double dwdx = (kx * dk[2] + ex * de[2] + cx * dc[2]) / vj, dwdy = (ky * dk[2] + ey * de[2] + cy * dc[2]) / vj;
+                            double dwdz = (kz * dk[2] + ez * de[2] + cz * dc[2]) / vj;
+                            /* Sub1.f90:181-198 stress, c23 = 0.66666666667 as written */
+                            double vis = b->visl[q], c23 = 0.66666666667;
+                            double txx = vis * c23 * (2.0 * dudx - dvdy - dwdz);
+                            double tyy = vis * c23 * (2.0 * dvdy - dwdz - dudx);
+                            double tzz = vis * c23 * (2.0 * dwdz - dudx - dvdy);
+                            double txy = vis * (dudy + dvdx), txz = vis * (dudz + dwdx), tyz = vis * (dvdz + dwdy);
+                            double tyx = txy, tzx = txz, tzy = tyz;
+                            double pv = 2.0 * g->s_lr;
+                            PN(0, is, js, ks) = PN(0, is, js, ks) - pv * (nx * txx + ny * tyx + nz * tzx) * c->re;
+                            PN(1, is, js, ks) = PN(1, is, js, ks) - pv * (nx * txy + ny * tyy + nz * tzy) * c->re;
+                            PN(2, is, js, ks) = PN(2, is, js, ks) - pv * (nx * txz + ny * tyz + nz * tzz) * c->re;
+                        }
+                    }
+            int id = 1 - abs(g->s_lr3d[0]), jd = 1 - abs(g->s_lr3d[1]), kd = 1 - abs(g->s_lr3d[2]);
+            for (int is = g->s_st[0]; is <= g->s_ed[0] - id; is++)
+                for (int js = g->s_st[1]; js <= g->s_ed[1] - jd; js++)
+                    for (int ks = g->s_st[2]; ks <= g->s_ed[2] - kd; ks++) {
+                        int q4[4][3];
+                        q4[0][0] = is; q4[0][1] = js; q4[0][2] = ks;
+                        if (id == 0) { int t[3][3] = {{is, js + 1, ks}, {is, js + 1, ks + 1}, {is, js, ks + 1}}; memcpy(&q4[1], t, sizeof t); }
+                        if (jd == 0) { int t[3][3] = {{is + 1, js, ks}, {is + 1, js, ks + 1}, {is, js, ks + 1}}; memcpy(&q4[1], t, sizeof t); }
+                        if (kd == 0) { int t[3][3] = {{is + 1, js, ks}, {is + 1, js + 1, ks}, {is, js + 1, ks}}; memcpy(&q4[1], t, sizeof t); }
+                        double P[12], X[12], d[6];
+                        for (int n = 0; n < 4; n++) {
+                            for (int a = 0; a < 3; a++) P[3 * n + a] = PN(a, q4[n][0], q4[n][1], q4[n][2]);
+                            long xi = (q4[n][0] - 1) + (long)b->ni * ((q4[n][1] - 1) + (long)b->nj * (q4[n][2] - 1));
+                            X[3 * n] = b->x[xi]; X[3 * n + 1] = b->y[xi]; X[3 * n + 2] = b->z[xi];
+                        }
+                        getforce(P, X, ref, d);
+                        cfx = cfx + d[0]; cfy = cfy + d[1]; cfz = cfz + d[2];
+                        cmx = cmx + d[3]; cmy = cmy + d[4]; cmz = cmz + d[5];
+                    }
+#undef PN
+            free(pn3);
+        }
+    }
+    double scf = 1.0 / c->prm.sref, vcm = 1.0 / (c->prm.sref * c->prm.lfref);
+    int w = 2 - c->prm.nwholefield;
+    cfx = w * cfx * scf; cfy = w * cfy * scf; cfz = w * cfz * scf;
+    cmx = w * cmx * vcm; cmy = w * cmy * vcm; cmz = w * cmz * vcm;
+    double sina = sin(c->attack_r), cosa = cos(c->attack_r), sinb = sin(c->sideslip_r), cosb = cos(c->sideslip_r);
+    double cl = cfy * cosa - cfx * sina;
+    double cd = cosb * (cfy * sina + cfx * cosa) + sinb * cfz;
+    double xcp = (cfy >= 0 ? 1.0 : -1.0) * cmz / (fabs(cfy) + SMALL);
+    out[0] = cfx; out[1] = cfy; out[2] = cfz; out[3] = cmx; out[4] = cmy; out[5] = cmz;
+    out[6] = cd; out[7] = cl; out[8] = xcp;
+    return 0;
+}

@@ -25,6 +25,9 @@ typedef struct {
     double moo, reynolds, attack, sideslip, tref, twall;
     double cfl, timedt_rate, efix, csrv, gama, prl, prt;
     int nvis, nlamtur, nscheme, nflux, ntmst;
+    /* namelist force (param.dat:34-44) */
+    double sref, lfref, xref, yref, zref;
+    int nwholefield;
 } o_params;
 
 typedef struct o_case o_case;
@@ -57,6 +60,8 @@ int     o_communicate_pv(o_case *c);
 int     o_residual(o_case *c, double *rms, double *resmax, int loc[5]);
 int     o_step(o_case *c, int nsteps, double *rms_hist);
 long    o_counter(const o_case *c, int which);   /* 0 wcns fallbacks, 1 repairs */
+/* force + force_dif (PostProcess.f90:1415-1503, 1505-1680): cfx,cfy,cfz,cmx,cmy,cmz,cd,cl,xcp */
+int     o_force(o_case *c, double out[9]);
 
 /* image/jmage/kmage of region nr of block nb over its face box (s_st..s_ed) */
 int     o_region_images(const o_case *c, int nb, int nr, int *img, int *jmg, int *kmg, int *box);

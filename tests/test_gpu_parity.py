@@ -192,3 +192,21 @@ def test_mixed_precision_lhs_converges_like_fp64():
     pa = a.field(0, "p")[3:-3, 3:-3, 3:-3]
     pb = b.field(0, "p")[3:-3, 3:-3, 3:-3]
     print("pressure difference", np.max(np.abs(pa - pb)) / np.max(np.abs(pa)), flush=True)
+
+
+def test_wall_force_matches_oracle():
+    """Force and moment coefficients on the bump wall after 5 viscous steps
+    (pressure plus Stokes stress, four-triangle quadrature) within 1e-10."""
+    case = cases.bump_channel_case(ni=32, nj=24, nk=12, dy_wall=1e-3, growth=1.15, reynolds=1e4)
+    case.params.xref, case.params.yref = 0.25, 0.1
+    case.params.sref, case.params.lfref, case.params.attack = 0.5, 1.0, 2.0
+    s = _solver(case)
+    o = O.OracleCase(case)
+    s.step(5)
+    o.step(5)
+    fs = s.force()
+    fo = o.force()
+    print("force gpu", fs, "oracle", fo, flush=True)
+    scale = np.max(np.abs(fo[:8]))
+    assert np.max(np.abs(fs[:8] - fo[:8])) <= 1e-10 * scale
+    assert abs(fs[8] - fo[8]) <= 1e-10 * abs(fo[8])

@@ -234,3 +234,79 @@ def test_viscous_bump_runs_and_converges():
     o = O.OracleCase(c)
     h = o.step(20)
     assert np.all(np.isfinite(h))
+
+
+# ------------------------------------------------------------- wall force
+def _flat_wall(**prm):
+    """Flat no-slip wall y = 0, x in [-1.5, 2.5], z in [0, 0.5] (area 2), uniform spacing."""
+    c = cases.bump_channel_case(ni=12, nj=10, nk=8, bump=0.0, dy_wall=0.1, growth=1.0)
+    for k, v in prm.items():
+        setattr(c.params, k, v)
+    return c
+
+
+def _set_state(o, c, u, v, w, p, mu):
+    b = c.blocks[0]
+    shp = (b.nk + 6, b.nj + 6, b.ni + 6)
+    x = np.zeros(shp); y = np.zeros(shp)
+    x[3:-3, 3:-3, 3:-3] = b.x.transpose(2, 1, 0)
+    y[3:-3, 3:-3, 3:-3] = b.y.transpose(2, 1, 0)
+    o.set_field(0, "r", np.ones(shp))
+    for name, f in (("u", u), ("v", v), ("w", w), ("p", p)):
+        o.set_field(0, name, f(x, y) * np.ones(shp))
+    o.set_field(0, "visl", np.full(shp, mu))
+
+
+def test_force_constant_pressure_closed_form():
+    """Uniform overpressure c on the flat wall: force -2c*A on the wall normal,
+    moments from the plate centroid, centre of pressure at the plate middle."""
+    c = _flat_wall(sref=1.0, lfref=1.0, nwholefield=1)
+    o = O.OracleCase(c)
+    poo = o.freestream()[4]
+    cc = 0.03
+    _set_state(o, c, lambda x, y: 0.0, lambda x, y: 0.0, lambda x, y: 0.0, lambda x, y: poo + cc, 1.0)
+    f = o.force()
+    # int x dA = 1.0, int z dA = 0.5 over the plate
+    assert abs(f[0]) < 1e-14 and abs(f[2]) < 1e-14
+    assert f[1] == pytest.approx(-4.0 * cc, rel=1e-12)
+    assert f[5] == pytest.approx(-2.0 * cc * 1.0, rel=1e-12)      # cmz = int x fy
+    assert f[3] == pytest.approx(2.0 * cc * 0.5, rel=1e-12)       # cmx = -int z fy
+    assert f[7] == pytest.approx(f[1], rel=1e-12)                  # cl at zero incidence
+    assert f[8] == pytest.approx(0.5, rel=1e-10)                   # xcp = plate middle
+
+
+def test_force_scaling_by_reference_values():
+    c1 = _flat_wall(sref=1.0, lfref=1.0, nwholefield=1)
+    c2 = _flat_wall(sref=2.0, lfref=0.5, nwholefield=0, attack=30.0)
+    out = []
+    for c in (c1, c2):
+        o = O.OracleCase(c)
+        poo = o.freestream()[4]
+        _set_state(o, c, lambda x, y: 0.0, lambda x, y: 0.0, lambda x, y: 0.0, lambda x, y: poo + 0.01 * (1 + x), 1.0)
+        out.append(o.force())
+    a, b = out
+    np.testing.assert_allclose(b[:3], a[:3] * 2 / 2.0, rtol=1e-12)           # (2-0)/sref
+    np.testing.assert_allclose(b[3:6], a[3:6] * 2 / (2.0 * 0.5), rtol=1e-12)  # (2-0)/(sref lfref)
+    al = 30.0 * 3.14159265 / 180.0
+    assert b[7] == pytest.approx(b[1] * np.cos(al) - b[0] * np.sin(al), rel=1e-12)
+    assert b[6] == pytest.approx(b[1] * np.sin(al) + b[0] * np.cos(al), rel=1e-12)
+
+
+def test_force_viscous_stress_closed_form():
+    """u = a y, v = b x + d y, w = e y over a uniform grid: every one-sided and
+    central difference is exact, so the wall traction is the Stokes stress:
+    fx = 2 re mu (a+b), fy = 4 re mu c23 d, fz = 2 re mu e per unit area."""
+    c = _flat_wall(sref=1.0, lfref=1.0, nwholefield=1, yref=0.3)
+    o = O.OracleCase(c)
+    poo = o.freestream()[4]
+    a, bb, d, e, mu = 0.3, -0.2, 0.05, 0.7, 1.7
+    _set_state(o, c, lambda x, y: a * y, lambda x, y: bb * x + d * y, lambda x, y: e * y, lambda x, y: poo, mu)
+    re = 1.0 / c.params.reynolds
+    c23 = 0.66666666667
+    f = o.force()
+    A = 2.0
+    assert f[0] == pytest.approx(2 * re * mu * (a + bb) * A, rel=1e-10)
+    assert f[1] == pytest.approx(4 * re * mu * c23 * d * A, rel=1e-10)
+    assert f[2] == pytest.approx(2 * re * mu * e * A, rel=1e-10)
+    # the wall is y = 0: cmz = int x fy dA - (0 - yref) int fx dA
+    assert f[5] == pytest.approx(4 * re * mu * c23 * d * 1.0 + 0.3 * f[0], rel=1e-9)

@@ -34,6 +34,12 @@ class Params:
     nscheme: int = 41
     nflux: int = 1
     ntmst: int = 0
+    sref: float = 1.0
+    lfref: float = 1.0
+    xref: float = 0.0
+    yref: float = 0.0
+    zref: float = 0.0
+    nwholefield: int = 1
 
 
 @dataclass

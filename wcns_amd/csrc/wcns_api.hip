@@ -81,6 +81,9 @@ struct wcns_ctx {
 
     std::vector<DRegion> iface;              // interfaces in (nb, nr) order
     std::vector<DRegion> phys;               // bctype > 0 regions
+    struct Wall { DRegion d; double *xyz; };  // no-slip walls with their face-node coordinates
+    std::vector<Wall> walls;
+    double *d_force_pn = nullptr, *d_force_part = nullptr, *d_force_out = nullptr;
     int npp = 0, ngroups = 0;
     int *d_npp_pts = nullptr, *d_grp_start = nullptr, *d_grp_n = nullptr, *d_write_ok = nullptr;
     double *d_npp_buf = nullptr;
@@ -376,6 +379,9 @@ int wcns_destroy(wcns_ctx *ctx)
                   ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
+    for (auto &w : ctx->walls) hipFree(w.xyz);
+    void *fs[] = {ctx->d_force_pn, ctx->d_force_part, ctx->d_force_out};
+    for (void *p : fs) if (p) hipFree(p);
     for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -565,6 +571,7 @@ int wcns_setup(wcns_ctx *ctx)
     for (auto &b : ctx->blocks)
         for (auto &r : b.regs) {
             if (r.bctype > 0) ctx->phys.push_back(r.d);
+            if (r.bctype == 2 || r.bctype / 10 == 2) ctx->walls.push_back({r.d, nullptr});
             if (r.bctype >= 0) continue;
             DRegion g = r.d;
             if (g.nbt == g.nb) {
@@ -611,6 +618,25 @@ int wcns_setup(wcns_ctx *ctx)
         CK(hipMemcpy(ctx->d_grp_n, gn.data(), 4 * gn.size(), hipMemcpyHostToDevice));
         CK(hipMemcpy(ctx->d_write_ok, ok.data(), 4 * ok.size(), hipMemcpyHostToDevice));
     }
+    // walls: face-node coordinates in region-point order, force scratch
+    size_t maxw = 1, maxq = 1;
+    for (auto &w : ctx->walls) {
+        const HBlock &b = ctx->blocks[w.d.nb - 1];
+        std::vector<double> xyz(3 * (size_t)w.d.npts);
+        for (int t = 0; t < w.d.npts; t++) {
+            int i, j, k;
+            region_point(w.d, t, i, j, k);
+            size_t q = (size_t)(i - 1) + (size_t)b.ni * ((size_t)(j - 1) + (size_t)b.nj * (size_t)(k - 1));
+            xyz[t] = b.x[q]; xyz[w.d.npts + t] = b.y[q]; xyz[2 * (size_t)w.d.npts + t] = b.z[q];
+        }
+        TRY(dalloc(ctx, &w.xyz, xyz.size()));
+        CK(hipMemcpy(w.xyz, xyz.data(), 8 * xyz.size(), hipMemcpyHostToDevice));
+        maxw = std::max(maxw, (size_t)w.d.npts);
+        maxq = std::max(maxq, (size_t)force_parts(w.d));
+    }
+    TRY(dalloc(ctx, &ctx->d_force_pn, 3 * maxw));
+    TRY(dalloc(ctx, &ctx->d_force_part, 6 * maxq));
+    TRY(dalloc(ctx, &ctx->d_force_out, 6 * std::max<size_t>(ctx->walls.size(), 1)));
     ctx->setup_done = true;
     return WCNS_OK;
 }
@@ -877,6 +903,44 @@ int wcns_step(wcns_ctx *ctx, int nsteps, double *rms_hist)
         CK(hipStreamSynchronize(ctx->stream));
         TRY(check_counters(ctx));
     }
+    return WCNS_OK;
+}
+
+int wcns_force(wcns_ctx *ctx, double out[9])
+{
+    if (!ctx || !out) return WCNS_E_ARG;
+    if (!ctx->setup_done) return WCNS_E_STATE;
+    if (ctx->prm.sref == 0.0 || ctx->prm.lfref == 0.0) return fail(ctx, WCNS_E_ARG, "sref and lfref must be non-zero");
+    DFields F = ctx->fields();
+    const double ref[3] = {ctx->prm.xref, ctx->prm.yref, ctx->prm.zref};
+    for (size_t w = 0; w < ctx->walls.size(); w++) {
+        const auto &wr = ctx->walls[w];
+        if (launch_force_region(wr.d, ctx->blocks[wr.d.nb - 1].db, F, ctx->dp, wr.xyz, ctx->d_force_pn,
+                                ctx->d_force_part, ctx->d_force_out + 6 * w, ref, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "force launch failed");
+    }
+    std::vector<double> part(6 * ctx->walls.size());
+    if (!part.empty()) {
+        CK(hipMemcpyAsync(part.data(), ctx->d_force_out, 8 * part.size(), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    CK(hipStreamSynchronize(ctx->stream));
+    // PostProcess.f90:1415-1477: region sums in (block, region) order, then coefficients
+    double f[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t w = 0; w < ctx->walls.size(); w++)
+        for (int m = 0; m < 6; m++) f[m] = f[m] + part[6 * w + m];
+    const wcns_params &p = ctx->prm;
+    double scf = 1.0 / p.sref, vcm = 1.0 / (p.sref * p.lfref);
+    int wf = 2 - p.nwholefield;
+    for (int m = 0; m < 3; m++) f[m] = wf * f[m] * scf;
+    for (int m = 3; m < 6; m++) f[m] = wf * f[m] * vcm;
+    const double pai = 3.14159265;   // Globalvar.f90:236, as the freestream set-up converts attack
+    double att = p.attack * pai / 180.0, ssl = p.sideslip * pai / 180.0;
+    double sina = sin(att), cosa = cos(att), sinb = sin(ssl), cosb = cos(ssl);
+    double cl = f[1] * cosa - f[0] * sina;
+    double cd = cosb * (f[1] * sina + f[0] * cosa) + sinb * f[2];
+    double xcp = (f[1] >= 0 ? 1.0 : -1.0) * f[5] / (fabs(f[1]) + SMALL);
+    for (int m = 0; m < 6; m++) out[m] = f[m];
+    out[6] = cd; out[7] = cl; out[8] = xcp;
     return WCNS_OK;
 }
 

@@ -51,6 +51,18 @@ struct DFields {
     long total;
 };
 
+// Sub1.f90:750-780 getnxyz_mml: unit normal of the face whose index offset is l3
+__device__ inline void getnxyz(const DFields &F, long g, const int *l3, double &nx, double &ny, double &nz)
+{
+    double x = 0, y = 0, z = 0;
+    if (l3[0]) { x = x + F.kcx[g]; y = y + F.kcy[g]; z = z + F.kcz[g]; }
+    if (l3[1]) { x = x + F.etx[g]; y = y + F.ety[g]; z = z + F.etz[g]; }
+    if (l3[2]) { x = x + F.ctx[g]; y = y + F.cty[g]; z = z + F.ctz[g]; }
+    double s = sqrt(x * x + y * y + z * z);
+    s = s < SML_SSS ? SML_SSS : s;
+    nx = x / s; ny = y / s; nz = z / s;
+}
+
 // ------------------------------------------------------------- gas model
 // GasModel.f90:358-373 getem
 __device__ inline double getem(double rm, double um, double vm, double wm, double pm, double gama)

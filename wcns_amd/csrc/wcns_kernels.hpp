@@ -73,6 +73,12 @@ int residual_parts(const DBlock &b);
 int launch_residual_final(const double *partial, int nparts, double *out, hipStream_t s);
 int launch_copy_interior(const DBlock &b, const double *src, double *dst, hipStream_t s);
 
+// wall force
+long force_quads(const DRegion &g);
+int force_parts(const DRegion &g);
+int launch_force_region(const DRegion &g, const DBlock &b, const DFields &F, const DParams &P, const double *xyz,
+                        double *pn3, double *partial, double *out6, const double ref[3], hipStream_t s);
+
 // setup
 int launch_gcl_metrics(const DBlock &b, const double *x, const double *y, const double *z, DFields F,
                        double *work, hipStream_t s);

@@ -29,7 +29,7 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_set_lhs_precision", "wcns_set_prim", "wcns_get_field", "wcns_set_field", "wcns_boundary_all",
            "wcns_initial_q_dq_c_t", "wcns_timestep", "wcns_r_h_s_vis", "wcns_r_h_s_invis", "wcns_communicate_dq",
            "wcns_l_h_s", "wcns_communicate_pv", "wcns_residual", "wcns_step", "wcns_get_counter", "wcns_sync",
-           "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get"]
+           "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get", "wcns_force"]
 
 TIMING_SLOTS = ["bc", "init", "timestep", "rhs_vis", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual"]
@@ -44,7 +44,8 @@ class WcnsError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("moo", "reynolds", "attack", "sideslip", "tref", "twall",
                                           "cfl", "timedt_rate", "efix", "csrv", "gama", "prl", "prt")] + \
-               [(n, C.c_int) for n in ("nvis", "nlamtur", "nscheme", "nflux", "ntmst")]
+               [(n, C.c_int) for n in ("nvis", "nlamtur", "nscheme", "nflux", "ntmst")] + \
+               [(n, C.c_double) for n in ("sref", "lfref", "xref", "yref", "zref")] + [("nwholefield", C.c_int)]
 
 
 _lib = None
@@ -80,6 +81,7 @@ def load_library(path=LIBPATH):
     L.wcns_multipoint.argtypes = [vp, iptr, iptr, iptr, C.c_int]
     L.wcns_timing.argtypes = [vp, C.c_int]
     L.wcns_timing_get.argtypes = [vp, C.c_int, dptr, C.POINTER(C.c_longlong)]
+    L.wcns_force.argtypes = [vp, dptr]
     _lib = L
     return L
 
@@ -214,6 +216,13 @@ class Solver:
             ms, n = C.c_double(), C.c_longlong()
             self._check(self.L.wcns_timing_get(self.h, i, C.byref(ms), C.byref(n)))
             out[name] = (ms.value, n.value)
+        return out
+
+    def force(self):
+        """cfx, cfy, cfz, cmx, cmy, cmz, cd, cl, xcp over the no-slip walls
+        (src/PostProcess.f90:1415-1503 force)."""
+        out = np.zeros(9)
+        self._check(self.L.wcns_force(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
     def counter(self, which):
