@@ -621,7 +621,7 @@ int wcns_setup(wcns_ctx *ctx)
     // walls: face-node coordinates in region-point order, force scratch
     size_t maxw = 1, maxq = 1;
     for (auto &w : ctx->walls) {
-        const HBlock &b = ctx->blocks[w.d.nb - 1];
+        const HBlock &b = ctx->blocks[w.d.nb];
         std::vector<double> xyz(3 * (size_t)w.d.npts);
         for (int t = 0; t < w.d.npts; t++) {
             int i, j, k;
@@ -915,7 +915,7 @@ int wcns_force(wcns_ctx *ctx, double out[9])
     const double ref[3] = {ctx->prm.xref, ctx->prm.yref, ctx->prm.zref};
     for (size_t w = 0; w < ctx->walls.size(); w++) {
         const auto &wr = ctx->walls[w];
-        if (launch_force_region(wr.d, ctx->blocks[wr.d.nb - 1].db, F, ctx->dp, wr.xyz, ctx->d_force_pn,
+        if (launch_force_region(wr.d, ctx->blocks[wr.d.nb].db, F, ctx->dp, wr.xyz, ctx->d_force_pn,
                                 ctx->d_force_part, ctx->d_force_out + 6 * w, ref, ctx->stream))
             return fail(ctx, WCNS_E_HIP, "force launch failed");
     }
