@@ -66,6 +66,8 @@ struct wcns_ctx {
     double *dn = nullptr;                   // 12 node derivatives (viscous)
     double *coed = nullptr;                 // LU-SGS diagonal per node
     float *dq32 = nullptr;                  // dQ of the FP32 implicit operator
+    double *dqB = nullptr;                  // second dQ buffer of the SGS sweep pair
+    float *dqB32 = nullptr;
     std::vector<SweepTiles> tiles;          // per block
     int *d_sweep_err = nullptr;
     bool tiled_sweeps = true;
@@ -374,8 +376,8 @@ int wcns_destroy(wcns_ctx *ctx)
     for (int b = 0; b < 2; b++) for (int m = 0; m < 5; m++) hipFree(ctx->prim[b][m]);
     for (int m = 0; m < 10; m++) hipFree(ctx->metric[m]);
     for (int m = 0; m < 3; m++) { hipFree(ctx->sr[m]); hipFree(ctx->srv[m]); }
-    for (auto &st : ctx->tiles) { hipFree(st.order_fwd); hipFree(st.order_bwd); hipFree(st.progress); }
-    void *ps[] = {ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
+    for (auto &st : ctx->tiles) { hipFree(st.order_fwd); hipFree(st.order_bwd); hipFree(st.xj); }
+    void *ps[] = {ctx->dqB, ctx->dqB32, ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
                   ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
@@ -429,6 +431,7 @@ int wcns_set_lhs_precision(wcns_ctx *ctx, int prec)
     if (prec == WCNS_LHS_FP32 && !ctx->tiled_sweeps)
         return fail(ctx, WCNS_E_UNSUPPORTED, "the FP32 implicit operator runs in the pipelined sweeps only");
     if (prec == WCNS_LHS_FP32 && ctx->setup_done && !ctx->dq32) TRY(dalloc(ctx, &ctx->dq32, 5 * ctx->total));
+    if (prec == WCNS_LHS_FP32 && ctx->setup_done && !ctx->dqB32) TRY(dalloc(ctx, &ctx->dqB32, 5 * ctx->total));
     ctx->lhs_prec = prec;
     return WCNS_OK;
 }
@@ -478,6 +481,11 @@ int wcns_setup(wcns_ctx *ctx)
     TRY(dalloc(ctx, &ctx->t, T));
     TRY(dalloc(ctx, &ctx->rhs, 5 * T));
     TRY(dalloc(ctx, &ctx->dq, 5 * T));
+    TRY(dalloc(ctx, &ctx->dqB, 5 * T));
+    if (ctx->lhs_prec == WCNS_LHS_FP32) {
+        TRY(dalloc(ctx, &ctx->dq32, 5 * T));
+        TRY(dalloc(ctx, &ctx->dqB32, 5 * T));
+    }
     TRY(dalloc(ctx, &ctx->dtdt, T));
     TRY(dalloc(ctx, &ctx->visl, T));
     if (p.nvis == 1) TRY(dalloc(ctx, &ctx->dn, 12 * T));
@@ -498,7 +506,9 @@ int wcns_setup(wcns_ctx *ctx)
         std::vector<int> tb(tl.rbegin(), tl.rend());
         TRY(dalloc(ctx, &st.order_fwd, nt));
         TRY(dalloc(ctx, &st.order_bwd, nt));
-        TRY(dalloc(ctx, &st.progress, nt));
+        long nx = sweep_exchange_words(st, b.ni);
+        TRY(dalloc(ctx, &st.xj, nx));
+        st.xk = st.xj + (long)nt * st.TK * b.ni * 5;
         CK(hipMemcpy(st.order_fwd, tl.data(), 4 * nt, hipMemcpyHostToDevice));
         CK(hipMemcpy(st.order_bwd, tb.data(), 4 * nt, hipMemcpyHostToDevice));
         st.err = ctx->d_sweep_err;
@@ -819,8 +829,10 @@ int wcns_l_h_s(wcns_ctx *ctx)
         int nplanes = db.ni + db.nj + db.nk - 2;
         for (int kind = 0; kind < 4; kind++) {
             Scope sc(ctx, WCNS_T_SWEEP_LU_L + kind, ctx->tiled_sweeps ? 1 : nplanes);
-            int e = ctx->tiled_sweeps ? launch_sweep_tiled(kind, db, F, ctx->dp, ctx->tiles[nb], ctx->lhs_prec, ctx->stream)
-                                      : launch_sweep(kind, db, F, ctx->dp, ctx->lhs_prec, ctx->stream);
+            void *A = F.dq32 ? (void *)F.dq32 : (void *)F.dq;
+            void *B = F.dq32 ? (void *)ctx->dqB32 : (void *)ctx->dqB;
+            int e = ctx->tiled_sweeps ? launch_sweep_tiled(kind, db, F, ctx->dp, ctx->tiles[nb], ctx->lhs_prec, A, B, ctx->stream)
+                                      : launch_sweep(kind, db, F, ctx->dp, ctx->stream);
             if (e) return fail(ctx, WCNS_E_HIP, "sweeps");
         }
         for (auto &g : ctx->phys)

@@ -227,9 +227,18 @@ __device__ inline void wcns_node(double qm2, double qm1, double q0, double qp1, 
 }
 
 // LHS.f90:756-828 mxdq_std (kt = 0), generic over the arithmetic type T of the
-// implicit operator.  prim/gykb/rad arrive in FP64 and are rounded to T.
+// implicit operator.  The routine is linear in dQ; it is split here into the
+// dQ-independent coefficients (mxdq_coef: the node's eigenvalues, projections
+// and normal) and their application to dQ (mxdq_apply), so that a sweep can
+// form the coefficients before the neighbour's dQ is known.  The two parts
+// perform exactly the reference's operations in its order.
 template <typename T>
-__device__ inline void mxdq_std(const T *prim, const T *gykb, const T *dq, T *f, T rad, T npn, T gama)
+struct MxCoef {
+    T l1, x1, x2, nx, ny, nz, ct, af, ae, c2, um, vm, wm, hm;
+};
+
+template <typename T>
+__device__ inline void mxdq_coef(const T *prim, const T *gykb, T rad, T npn, T gama, MxCoef<T> &c)
 {
     T rm = prim[0], um = prim[1], vm = prim[2], wm = prim[3], pm = prim[4];
     T hint = gama / (gama - T(1.0)) * pm / rm;
@@ -245,23 +254,37 @@ __device__ inline void mxdq_std(const T *prim, const T *gykb, const T *dq, T *f,
     l1 = T(0.5) * (l1 + npn * rad);
     l4 = T(0.5) * (l4 + npn * rad);
     l5 = T(0.5) * (l5 + npn * rad);
-    T x1 = (T(2.0) * l1 - l4 - l5) / (T(2.0) * c2);
-    T x2 = (l4 - l5) / (T(2.0) * cm);
+    c.x1 = (T(2.0) * l1 - l4 - l5) / (T(2.0) * c2);
+    c.x2 = (l4 - l5) / (T(2.0) * cm);
     T cgm1 = T(1.0) / cgm;
-    nx = nx * cgm1;
-    ny = ny * cgm1;
-    nz = nz * cgm1;
-    ct = (ct - T(0.0)) * cgm1;
-    T ae = gama - T(1.0);
-    T af = T(0.5) * ae * v2;
-    T dc = ct * dq[0] - nx * dq[1] - ny * dq[2] - nz * dq[3];
-    T dh = af * dq[0] - ae * (um * dq[1] + vm * dq[2] + wm * dq[3] - dq[4]);
-    T c2dc = c2 * dc;
-    f[0] = l1 * dq[0] - dh * x1 - dc * x2;
-    f[1] = l1 * dq[1] + (nx * c2dc - um * dh) * x1 + (nx * dh - um * dc) * x2;
-    f[2] = l1 * dq[2] + (ny * c2dc - vm * dh) * x1 + (ny * dh - vm * dc) * x2;
-    f[3] = l1 * dq[3] + (nz * c2dc - wm * dh) * x1 + (nz * dh - wm * dc) * x2;
-    f[4] = l1 * dq[4] + (ct * c2dc - hm * dh) * x1 + (ct * dh - hm * dc) * x2;
+    c.nx = nx * cgm1;
+    c.ny = ny * cgm1;
+    c.nz = nz * cgm1;
+    c.ct = (ct - T(0.0)) * cgm1;
+    c.ae = gama - T(1.0);
+    c.af = T(0.5) * c.ae * v2;
+    c.l1 = l1; c.c2 = c2; c.um = um; c.vm = vm; c.wm = wm; c.hm = hm;
+}
+
+template <typename T>
+__device__ inline void mxdq_apply(const MxCoef<T> &c, const T *dq, T *f)
+{
+    T dc = c.ct * dq[0] - c.nx * dq[1] - c.ny * dq[2] - c.nz * dq[3];
+    T dh = c.af * dq[0] - c.ae * (c.um * dq[1] + c.vm * dq[2] + c.wm * dq[3] - dq[4]);
+    T c2dc = c.c2 * dc;
+    f[0] = c.l1 * dq[0] - dh * c.x1 - dc * c.x2;
+    f[1] = c.l1 * dq[1] + (c.nx * c2dc - c.um * dh) * c.x1 + (c.nx * dh - c.um * dc) * c.x2;
+    f[2] = c.l1 * dq[2] + (c.ny * c2dc - c.vm * dh) * c.x1 + (c.ny * dh - c.vm * dc) * c.x2;
+    f[3] = c.l1 * dq[3] + (c.nz * c2dc - c.wm * dh) * c.x1 + (c.nz * dh - c.wm * dc) * c.x2;
+    f[4] = c.l1 * dq[4] + (c.ct * c2dc - c.hm * dh) * c.x1 + (c.ct * dh - c.hm * dc) * c.x2;
+}
+
+template <typename T>
+__device__ inline void mxdq_std(const T *prim, const T *gykb, const T *dq, T *f, T rad, T npn, T gama)
+{
+    MxCoef<T> c;
+    mxdq_coef<T>(prim, gykb, rad, npn, gama, c);
+    mxdq_apply<T>(c, dq, f);
 }
 
 }  // namespace wcns

@@ -40,15 +40,16 @@ int launch_rhs_inv_dir(int dir, const DBlock &b, const DFields &F, const DParams
 struct SweepTiles {
     int TJ, TK, nTJ, nTK;
     int *order_fwd, *order_bwd;   // blockIdx -> tile, topological for each sweep direction
-    int *progress;                // per-tile last published step
+    double *xj, *xk;              // boundary exchange slots: [tile][TK][ni][5], then [tile][TJ][ni][5]
     int *err;                     // set when a bounded wait expires
     long spin_limit;
 };
+long sweep_exchange_words(const SweepTiles &T, int ni);
+// A: dQ (double or float by prec); B: the second buffer of the SGS pair, same type
 int launch_sweep_tiled(int kind, const DBlock &b, const DFields &F, const DParams &P, const SweepTiles &T,
-                       int prec, hipStream_t s);
+                       int prec, void *A, void *B, hipStream_t s);
+int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
 int launch_dq_zero_face32(const DRegion &g, const DBlock &b, float *arr, long total, hipStream_t s);
-int launch_fill_int(int *a, int n, int v, hipStream_t s);
-int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, int prec, hipStream_t s);
 
 int launch_vis_node_deriv(const DBlock &b, const DFields &F, double *dn, hipStream_t s);
 int launch_rhs_vis_dir(int dir, const DBlock &b, const DFields &F, const DParams &P, const double *dn, hipStream_t s);
@@ -65,7 +66,6 @@ int launch_avg2pm(const DRegion &g, const DBlock *blocks, const DFields &F, int 
 int launch_npp_assign(const int *grp_start, const int *grp_n, const int *pts, const int *write_ok, int ngroups,
                       const double *buf, const DBlock *dblocks, const DFields &F, int nvar, int use_vol,
                       hipStream_t s);
-int launch_sweeps(const DBlock &b, const DFields &F, const DParams &P, int prec, hipStream_t s);
 int launch_update(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s);
 int launch_repair(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s);
 int launch_residual(const DBlock &b, const DFields &F, double *partial, int nparts, hipStream_t s);

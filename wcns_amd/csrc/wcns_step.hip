@@ -360,222 +360,6 @@ __global__ void k_npp_assign(const int *grp_start, const int *grp_n, const int *
     }
 }
 
-// ------------------------------------------------------------ sweeps
-// Node (i,j,k) of a sweep reads its lower neighbours (i-1,j,k), (i,j-1,k),
-// (i,j,k-1) and its upper ones.  Processing the hyperplanes i+j+k = s in
-// order, in place, gives every node exactly the operands of the reference's
-// lexicographic loops (LHS.f90:422-574, 1797-2043), so the sweeps below are
-// the reference's arithmetic node for node.
-struct NbrGeo {
-    double prim[5];
-    double gy[3];
-    double rad;
-};
-
-__device__ inline void load_prim(const DFields &F, long g, double *prim)
-{
-    prim[0] = F.r[g]; prim[1] = F.u[g]; prim[2] = F.v[g]; prim[3] = F.w[g]; prim[4] = F.p[g];
-}
-
-// Sub1.f90:676-717 getrkec_mml clamp + LHS.f90:647-678 getrarbrc clamp
-__device__ inline int clampi(int v, int n) { return v < 1 ? 1 : (v > n ? n : v); }
-
-template <typename T>
-__device__ inline void neighbour_term(const DBlock &b, const DFields &F, const DParams &P, int i, int j, int k,
-                                      int dir, int off, const T *dqn, T *out)
-{
-    int in = i + (dir == 0 ? off : 0), jn = j + (dir == 1 ? off : 0), kn = k + (dir == 2 ? off : 0);
-    long gp = gidx(b, in, jn, kn);
-    // metrics at the neighbour, ghost indices clamped to the face (getrkec_mml)
-    int im = in, jm = jn, km = kn;
-    if (im == 0) im = 1;
-    if (im == b.ni + 1) im = b.ni;
-    if (jm == 0) jm = 1;
-    if (jm == b.nj + 1) jm = b.nj;
-    if (km == 0) km = 1;
-    if (km == b.nk + 1) km = b.nk;
-    long gm = gidx(b, im, jm, km);
-    // spectral radius of the neighbour along dir, clamped to 1..n (getrarbrc)
-    long gr;
-    if (dir == 0) gr = gidx(b, clampi(i + off, b.ni), j, k);
-    else if (dir == 1) gr = gidx(b, i, clampi(j + off, b.nj), k);
-    else gr = gidx(b, i, j, clampi(k + off, b.nk));
-    double pr[5];
-    load_prim(F, gp, pr);
-    T prim[5], gy[3];
-#pragma unroll
-    for (int m = 0; m < 5; m++) prim[m] = (T)pr[m];
-    const double *mx = dir == 0 ? F.kcx : (dir == 1 ? F.etx : F.ctx);
-    const double *my = dir == 0 ? F.kcy : (dir == 1 ? F.ety : F.cty);
-    const double *mz = dir == 0 ? F.kcz : (dir == 1 ? F.etz : F.ctz);
-    gy[0] = (T)mx[gm]; gy[1] = (T)my[gm]; gy[2] = (T)mz[gm];
-    const double *sr = dir == 0 ? F.sra : (dir == 1 ? F.srb : F.src);
-    T rad = (T)sr[gr];
-    mxdq_std<T>(prim, gy, dqn, out, rad, off < 0 ? T(1.0) : T(-1.0), (T)P.gama);
-}
-
-// LHS.f90:602-645 getcoed
-__device__ inline double coed_at(const DFields &F, const DParams &P, long g)
-{
-    double rad_ns = F.sra[g] + F.srb[g] + F.src[g];
-    if (P.nvis >= 1) rad_ns = rad_ns + F.srva[g] + F.srvb[g] + F.srvc[g];
-    double ct = 1.0 / F.dtdt[g];
-    return 1.0 / (ct + 1.0 * 1.0 * rad_ns);
-}
-
-// kind 0 lusgs_l (LHS.f90:381-478), 1 lusgs_u (:480-577), 2 gs_pr_l (:1777-1866), 3 gs_pr_u (:1956-2046)
-template <typename T> __device__ inline T *dq_of(const DFields &F);
-template <> __device__ inline double *dq_of<double>(const DFields &F) { return F.dq; }
-template <> __device__ inline float *dq_of<float>(const DFields &F) { return F.dq32; }
-
-// One node of a sweep in arithmetic type T (double: the reference's FP64
-// operator; float: the mixed-precision operator, with R, the primitives and
-// the metrics rounded to FP32 on load and dQ stored in FP32).
-template <int KIND, typename T>
-__device__ inline void sweep_node(const DBlock &b, const DFields &F, const DParams &P, int i, int j, int k)
-{
-    long g = gidx(b, i, j, k);
-    T coed = (T)(F.coed ? F.coed[g] : coed_at(F, P, g));
-    const long T5 = F.total;
-    T *dq = dq_of<T>(F);
-    T rhs0[5];
-    if (KIND == 0 || KIND == 1) {
-        int off = KIND == 0 ? -1 : 1;
-        T dn[3][5], fo[3][5];
-        long gn[3] = {gidx(b, i + off, j, k), gidx(b, i, j + off, k), gidx(b, i, j, k + off)};
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int m = 0; m < 5; m++) dn[d][m] = dq[m * T5 + gn[d]];
-#pragma unroll
-        for (int d = 0; d < 3; d++) neighbour_term<T>(b, F, P, i, j, k, d, off, dn[d], fo[d]);
-#pragma unroll
-        for (int m = 0; m < 5; m++) rhs0[m] = T(1.0) * (fo[0][m] + fo[1][m] + fo[2][m]);
-        if (P.nvis > 0) {
-            T rva = (T)F.srva[gn[0]], rvb = (T)F.srvb[gn[1]], rvc = (T)F.srvc[gn[2]];
-#pragma unroll
-            for (int m = 0; m < 5; m++) {
-                T v = T(0.5) * (rva * dn[0][m] + rvb * dn[1][m] + rvc * dn[2][m]);
-                rhs0[m] = KIND == 0 ? rhs0[m] + v : rhs0[m] - v;
-            }
-        }
-        if (KIND == 0) {
-#pragma unroll
-            for (int m = 0; m < 5; m++) dq[m * T5 + g] = (-(T)F.rhs[m * T5 + g] + rhs0[m]) * coed;
-        } else {
-#pragma unroll
-            for (int m = 0; m < 5; m++) dq[m * T5 + g] = dq[m * T5 + g] - rhs0[m] * coed;
-        }
-    } else {
-        T dl[3][5], du[3][5], fl[3][5], fu[3][5];
-        long gl[3] = {gidx(b, i - 1, j, k), gidx(b, i, j - 1, k), gidx(b, i, j, k - 1)};
-        long gu[3] = {gidx(b, i + 1, j, k), gidx(b, i, j + 1, k), gidx(b, i, j, k + 1)};
-        bool lo[3] = {i == 1, j == 1, k == 1}, hi[3] = {i == b.ni, j == b.nj, k == b.nk};
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int m = 0; m < 5; m++) {
-                dl[d][m] = lo[d] ? T(0.0) : dq[m * T5 + gl[d]];
-                du[d][m] = hi[d] ? T(0.0) : dq[m * T5 + gu[d]];
-            }
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            neighbour_term<T>(b, F, P, i, j, k, d, -1, dl[d], fl[d]);
-            neighbour_term<T>(b, F, P, i, j, k, d, 1, du[d], fu[d]);
-        }
-#pragma unroll
-        for (int m = 0; m < 5; m++) {
-            T r0 = T(1.0) * (fl[0][m] + fl[1][m] + fl[2][m] - fu[0][m] - fu[1][m] - fu[2][m]);
-            dq[m * T5 + g] = (r0 - (T)F.rhs[m * T5 + g]) * coed;
-        }
-    }
-}
-
-template <int KIND>
-__global__ void __launch_bounds__(256) k_sweep_plane(DBlock b, DFields F, DParams P, int s)
-{
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int jlo = s - b.ni - b.nk, jhi = s - 2;
-    jlo = jlo < 1 ? 1 : jlo;
-    jhi = jhi > b.nj ? b.nj : jhi;
-    if (jhi < jlo) return;
-    long nj_ = jhi - jlo + 1;
-    if (t >= nj_ * b.nk) return;
-    int j = jlo + (int)(t % nj_);
-    int k = (int)(t / nj_) + 1;
-    int i = s - j - k;
-    if (i < 1 || i > b.ni) return;
-    sweep_node<KIND, double>(b, F, P, i, j, k);
-}
-
-// Persistent, tile-pipelined sweep.  Workgroup = a TJ x TK tile of (j,k) columns,
-// one column per thread; it walks the hyperplanes s = i+j+k of its columns in
-// sweep order, each thread updating node i = s-j-k of its column.  Every node
-// of plane s reads only nodes of planes s-1 (its own column and the j-1, k-1
-// columns, i.e. its own tile or the two predecessor tiles) and, in the GS
-// sweeps, old values of plane s+1.  A tile therefore starts plane s once both
-// predecessor tiles have published plane s-1 (per-tile progress counters,
-// release/acquire at agent scope); successors cannot overwrite plane s+1
-// before this tile publishes plane s.  Workgroups are dispatched in a
-// topological order of the tile graph, so every awaited tile is already
-// running or done; each wait is bounded and reports through err.
-template <int KIND, typename T>
-__global__ void __launch_bounds__(256) k_sweep_tiled(DBlock b, DFields F, DParams P, const int *order, int *progress,
-                                                   int nTJ, int nTK, int TJ, int TK, long spin_limit, int *err)
-{
-    const bool fwd = (KIND == 0 || KIND == 2);
-    int tile = order[blockIdx.x];
-    int J = tile % nTJ, K = tile / nTJ;
-    int jl = threadIdx.x % TJ, kl = threadIdx.x / TJ;
-    int j = 1 + J * TJ + jl, k = 1 + K * TK + kl;
-    bool col = (jl < TJ) && (kl < TK) && j <= b.nj && k <= b.nk;
-    int jlo = 1 + J * TJ, klo = 1 + K * TK;
-    int jhi = min(b.nj, (J + 1) * TJ), khi = min(b.nk, (K + 1) * TK);
-    int smin = jlo + klo + 1, smax = jhi + khi + b.ni;
-    const int SMAXG = b.ni + b.nj + b.nk + 1;
-    // predecessor tiles in this sweep's order
-    int p1 = -1, p2 = -1;
-    if (fwd) {
-        if (J > 0) p1 = tile - 1;
-        if (K > 0) p2 = tile - nTJ;
-    } else {
-        if (J < nTJ - 1) p1 = tile + 1;
-        if (K < nTK - 1) p2 = tile + nTJ;
-    }
-    __shared__ int bad;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
-    for (int step = 0; step <= smax - smin; step++) {
-        int s = fwd ? smin + step : smax - step;
-        int t = fwd ? s : SMAXG - s;            // monotone step index
-        if (threadIdx.x == 0 && !bad) {
-            long spins = 0;
-            while ((p1 >= 0 && __hip_atomic_load(&progress[p1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < t - 1) ||
-                   (p2 >= 0 && __hip_atomic_load(&progress[p2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < t - 1)) {
-                if (++spins > spin_limit) { bad = 1; atomicExch(err, 1); break; }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // later loads see the predecessors' planes
-        }
-        __syncthreads();
-        if (col) {
-            int i = s - j - k;
-            if (i >= 1 && i <= b.ni) sweep_node<KIND, T>(b, F, P, i, j, k);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0)
-            __hip_atomic_store(&progress[tile], t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (threadIdx.x == 0)
-        __hip_atomic_store(&progress[tile], 0x7fffffff, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ void k_fill_int(int *a, int n, int v)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) a[t] = v;
-}
-
 // -------------------------------------------------------------- update
 // Sub2.f90:603-796 update_limited.  q is prim_to_q(prim) of the step's start
 // (get_q_only, RHS.f90:75-100), recomputed here instead of stored.  New
@@ -802,62 +586,6 @@ int launch_npp_assign(const int *grp_start, const int *grp_n, const int *pts, co
     hipLaunchKernelGGL(k_npp_assign, grid1(ngroups, 64), dim3(64), 0, s, grp_start, grp_n, pts, write_ok,
                        ngroups, buf, dblocks, F, nvar, use_vol);
     return hipGetLastError() == hipSuccess ? 0 : -3;
-}
-
-int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, int prec, hipStream_t s)
-{
-    (void)prec;
-    int smin = 3, smax = b.ni + b.nj + b.nk;
-    long nthr = (long)b.nj * b.nk;
-    dim3 g = grid1(nthr, 256), bl(256);
-    switch (kind) {
-    case 0: for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<0>, g, bl, 0, s, b, F, P, p); break;
-    case 1: for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<1>, g, bl, 0, s, b, F, P, p); break;
-    case 2: for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<2>, g, bl, 0, s, b, F, P, p); break;
-    default: for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<3>, g, bl, 0, s, b, F, P, p); break;
-    }
-    return hipGetLastError() == hipSuccess ? 0 : -3;
-}
-
-int launch_sweep_tiled(int kind, const DBlock &b, const DFields &F, const DParams &P, const SweepTiles &T,
-                       int prec, hipStream_t s)
-{
-    int ntiles = T.nTJ * T.nTK;
-    hipLaunchKernelGGL(k_fill_int, grid1(ntiles, 256), dim3(256), 0, s, T.progress, ntiles, 0);
-    const int *ord = (kind == 0 || kind == 2) ? T.order_fwd : T.order_bwd;
-    dim3 g(ntiles), bl(T.TJ * T.TK);
-#define WCNS_SWEEP_LAUNCH(K, TY) hipLaunchKernelGGL((k_sweep_tiled<K, TY>), g, bl, 0, s, b, F, P, ord, T.progress, \
-                                                    T.nTJ, T.nTK, T.TJ, T.TK, T.spin_limit, T.err)
-    if (prec == 1) {
-        switch (kind) {
-        case 0: WCNS_SWEEP_LAUNCH(0, float); break;
-        case 1: WCNS_SWEEP_LAUNCH(1, float); break;
-        case 2: WCNS_SWEEP_LAUNCH(2, float); break;
-        default: WCNS_SWEEP_LAUNCH(3, float); break;
-        }
-    } else {
-        switch (kind) {
-        case 0: WCNS_SWEEP_LAUNCH(0, double); break;
-        case 1: WCNS_SWEEP_LAUNCH(1, double); break;
-        case 2: WCNS_SWEEP_LAUNCH(2, double); break;
-        default: WCNS_SWEEP_LAUNCH(3, double); break;
-        }
-    }
-#undef WCNS_SWEEP_LAUNCH
-    return hipGetLastError() == hipSuccess ? 0 : -3;
-}
-
-int launch_fill_int(int *a, int n, int v, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_fill_int, grid1(n, 256), dim3(256), 0, s, a, n, v);
-    return hipGetLastError() == hipSuccess ? 0 : -3;
-}
-
-int launch_sweeps(const DBlock &b, const DFields &F, const DParams &P, int prec, hipStream_t s)
-{
-    for (int kind = 0; kind < 4; kind++)
-        if (launch_sweep(kind, b, F, P, prec, s)) return -3;
-    return 0;
 }
 
 int launch_update(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s)
