@@ -30,16 +30,32 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per node for one launch of each timed kernel family
-# (FP64 words; each field counted once per pass).
-ALG_BYTES_PER_NODE = {
-    # R += dF/dxi along one direction: read P(5) + metric triple(3) + R(5), write R(5)
-    "rhs_inv_i": 8 * 18, "rhs_inv_j": 8 * 18, "rhs_inv_k": 8 * 18,
-    # spectral radii + dt: read P(5) + metrics(9) + vol + srv(3), write sr(3) + dt(1)
-    "timestep": 8 * 22,
-    # update: read P(5) + dq(5), write P(5)
-    "update": 8 * 15,
-}
+# Algorithmic HBM bytes per node for one launch of each single-kernel timing
+# slot (one launch per block per step), FP64 words, each field counted once
+# per pass.  Sweep operands are stored at the implicit operator's width.
+def alg_bytes_per_node(slot, lhs_bytes):
+    table = {
+        # R += dF/dxi along one direction: read P 5, metric triple 3, R 5; write R 5
+        "rhs_inv_i": 8 * 18, "rhs_inv_j": 8 * 18, "rhs_inv_k": 8 * 18,
+        # R -= Re^-1 dFv/dxi along one direction: read u v w T 4, the 8 cross
+        # derivatives, 9 metrics, vol, mu, R 5; write R 5
+        "rhs_vis_i": 8 * 33, "rhs_vis_j": 8 * 33, "rhs_vis_k": 8 * 33,
+        # node derivatives: read u v w T, write 12
+        "vis_deriv": 8 * 16,
+        # sweeps (P 5, Geo 10, R 5, D 1, dQ in/out): 26 / 26 / 31 / 31 words
+        "sweep_lu_l": lhs_bytes * 26, "sweep_lu_u": lhs_bytes * 26,
+        "sweep_gs_l": lhs_bytes * 31, "sweep_gs_u": lhs_bytes * 31,
+        # plane-major staging: read 17 FP64 fields, write them at the operator's width
+        "sweep_stage": 8 * 17 + lhs_bytes * 17,
+        # update: read P 5 + dQ 5, write P 5
+        "update": 8 * 15,
+    }
+    return table.get(slot)
+
+
+# Whole-step algorithmic bytes per node (Euler / laminar Navier-Stokes, FP64):
+# RHS 21 + LU 26 + 26 + SGS 31 + 31 + update 15 = 150 words.
+STEP_BYTES_PER_NODE = 8 * 150
 
 
 def build_case(workload):
@@ -122,27 +138,22 @@ def main():
 
     sec = t1 - t0
     value = case.nodes * args.steps / sec / 1e6
-    # dominant kernel family
-    dom = max(tim, key=lambda k: tim[k][0])
-    dms, dlaunch = tim[dom]
+    # dominant single-kernel slot: the one with the most time per step
+    lhs_bytes = 4 if args.lhs == "fp32" else 8
     nodes = case.nodes
+    priced = {k: v for k, v in tim.items() if alg_bytes_per_node(k, lhs_bytes) and v[1]}
     roof = None
-    if dom in ALG_BYTES_PER_NODE and dlaunch:
-        per_launch = ALG_BYTES_PER_NODE[dom] * nodes
+    if priced:
+        dom = max(priced, key=lambda k: priced[k][0])
+        dms, dlaunch = priced[dom]
+        # the launches of a slot in one step cover every node once (one per
+        # block, or one batched launch for all blocks)
+        per_launch = alg_bytes_per_node(dom, lhs_bytes) * nodes * args.steps / dlaunch
         avg_s = dms / dlaunch / 1e3
         ach = per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                 "alg_bytes_per_launch": per_launch, "avg_launch_ms": round(dms / dlaunch, 4)}
-    elif dlaunch:
-        # sweep planes: one launch per hyperplane; bytes per launch = plane share of a sweep pass
-        nb = {"sweep_lu_l": 26, "sweep_lu_u": 26, "sweep_gs_l": 31, "sweep_gs_u": 31}.get(dom, 0) * 8
-        per_launch = nb * nodes / max(dlaunch / (args.steps * len(case.blocks)), 1)
-        avg_s = dms / dlaunch / 1e3
-        ach = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "alg_bytes_per_launch": per_launch, "avg_launch_ms": round(dms / dlaunch, 5)}
     if args.breakdown:
         tot = sum(v[0] for v in tim.values())
         for k, (ms, n) in sorted(tim.items(), key=lambda kv: -kv[1][0]):
@@ -160,6 +171,7 @@ def main():
         "config": {"workload": wl_name, "nodes": nodes, "blocks": len(case.blocks), "lhs": args.lhs,
                    "residual_rms_end": rms},
         "roofline": roof, "cpu_baseline": cpu,
+        "step_hbm_frac": round(STEP_BYTES_PER_NODE * nodes * args.steps / sec / (HBM_PEAK_GBS * 1e9), 5),
         "stage_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in tim.items()},
     }
     print(json.dumps(line))

@@ -126,7 +126,7 @@ int wcns_sync(wcns_ctx *ctx);
 #define WCNS_T_BC           0
 #define WCNS_T_INIT         1
 #define WCNS_T_TIMESTEP     2
-#define WCNS_T_VIS          3
+#define WCNS_T_VIS          3    /* node derivatives of u, v, w, T (viscous) */
 #define WCNS_T_INV_I        4
 #define WCNS_T_INV_J        5
 #define WCNS_T_INV_K        6
@@ -138,7 +138,11 @@ int wcns_sync(wcns_ctx *ctx);
 #define WCNS_T_UPDATE      12
 #define WCNS_T_COMM_PV     13
 #define WCNS_T_RESIDUAL    14
-#define WCNS_T_NSLOT       15
+#define WCNS_T_SWEEP_STAGE 15   /* plane-major staging of the sweep operands */
+#define WCNS_T_VIS_I       16   /* viscous flux derivative along i, j, k */
+#define WCNS_T_VIS_J       17
+#define WCNS_T_VIS_K       18
+#define WCNS_T_NSLOT       19
 int wcns_timing(wcns_ctx *ctx, int on);
 int wcns_timing_get(wcns_ctx *ctx, int slot, double *total_ms, long long *launches);
 

@@ -226,3 +226,64 @@ def test_pipelined_sweeps_match_plane_sweeps_full_size(monkeypatch):
     assert np.array_equal(ha, hb)
     for name in ("r", "u", "v", "w", "p"):
         assert np.array_equal(a.field(0, name), b.field(0, name))
+
+
+def _cmp_all_blocks(s, o, case, names, tol, ghost=0):
+    for nb, blk in enumerate(case.blocks):
+        for name in names:
+            a = _interior(s.field(nb, name), blk, ghost)
+            b = _interior(o.field(nb, name), blk, ghost)
+            e = _rel_err(a, b)
+            assert e <= tol, f"block {nb} {name}: rel err {e:.3e}"
+
+
+MB_CASES = [
+    dict(nx=(14, 18), ny=(13, 15), nz=(12,), nvis=0, nflux=3),
+    dict(nx=(14, 16), ny=(13, 14), nz=(12, 13), nvis=1, wall=True, nflux=1, nscheme=4),
+]
+
+
+@pytest.mark.parametrize("kw", MB_CASES, ids=["2x2x1-euler-roe", "2x2x2-wall-sw"])
+def test_multiblock_stage_by_stage(kw):
+    """Uneven blocks with 1:1 interfaces in every direction (plus periodic
+    faces and, in the second case, a no-slip wall): every stage of three
+    steps against the oracle, all blocks."""
+    case = cases.multiblock_case(**kw)
+    print(f"multi-block case {kw}", flush=True)
+    s = _solver(case)
+    o = O.OracleCase(case)
+    prim = ("r", "u", "v", "w", "p")
+    for step in range(3):
+        s.boundary_all(); o.stage("boundary_all")
+        _cmp_all_blocks(s, o, case, prim, REL, ghost=3)
+        s.initial_q_dq_c_t(); o.stage("initial_q_dq_c_t")
+        s.timestep_tgh(); o.stage("timestep")
+        _cmp_all_blocks(s, o, case, ("sra", "srb", "src", "dtdt"), REL)
+        if kw["nvis"]:
+            s.r_h_s_vis(); o.stage("rhs_vis")
+            s.communicate_dq_npp(); o.stage("communicate_dq")
+        s.r_h_s_invis(); o.stage("rhs_invis")
+        s.communicate_dq_npp(); o.stage("communicate_dq")
+        for nb, blk in enumerate(case.blocks):
+            e = _rel_err(_interior(s.field(nb, "rhs"), blk), _interior(o.field(nb, "dq"), blk))
+            assert e <= REL, f"block {nb} rhs step {step}: {e:.3e}"
+        s.l_h_s_tgh(); o.stage("lhs")
+        _cmp_all_blocks(s, o, case, ("dq",), REL)
+        s.communicate_pv_npp(); o.stage("communicate_pv")
+        _cmp_all_blocks(s, o, case, prim, REL)
+        rs = s.residual_curve()
+        ro = o.residual()
+        assert abs(rs[0] - ro[0]) <= 1e-12 * ro[0]
+        assert list(rs[2]) == list(ro[2])
+
+
+def test_multiblock_40_steps():
+    case = cases.multiblock_case(nx=(14, 17, 12), ny=(13, 15), nz=(12, 14), nvis=1, wall=True)
+    s = _solver(case)
+    o = O.OracleCase(case)
+    hs = s.step(40)
+    ho = o.step(40)
+    rel = np.abs(hs - ho) / ho
+    print("multi-block residual history rel err max", rel.max(), flush=True)
+    assert rel.max() <= 1e-10
+    _cmp_all_blocks(s, o, case, ("r", "u", "v", "w", "p"), 1e-11)

@@ -259,3 +259,97 @@ def linear_field_init(A, moo=0.5):
         poo = 1.0 / (case.params.gama * case.params.moo ** 2)
         return (np.ones(shp), ext(vel[0]), ext(vel[1]), ext(vel[2]), np.full(shp, poo))
     return init
+
+
+def multiblock_case(nx=(14, 18), ny=(13, 15), nz=(12,), seed=5, nvis=0, wall=False, periodic_i=True,
+                    periodic_k=True, nscheme=41, nflux=3, moo=0.6, reynolds=1.0e4, cfl=5.0, lx=2.0, ly=1.0,
+                    lz=0.6, amp_grid=0.03, nmodes=6, amp=0.01, attack=2.0):
+    """A box split into len(nx) x len(ny) x len(nz) blocks of uneven node counts
+    (block (a,b,c) has nx[a] x ny[b] x nz[c] nodes) meeting at 1:1
+    point-matched interfaces (bctype -1).  The outer i and k faces are
+    connected periodically; the j faces are far field (BC 4), or a no-slip wall
+    (BC 2) at j = 1 when `wall`.  A smooth coordinate perturbation makes all
+    nine metric terms non-zero; rho and p carry seeded Fourier modes.  Regions
+    of every block are listed as i-lo, i-hi, j-lo, j-hi, k-lo, k-hi."""
+    rng = np.random.default_rng(seed)
+    A, B, C = len(nx), len(ny), len(nz)
+    Nx, Ny, Nz = sum(nx) - A + 1, sum(ny) - B + 1, sum(nz) - C + 1
+    gi, gj, gk = np.arange(Nx) / (Nx - 1), np.arange(Ny) / (Ny - 1), np.arange(Nz) / (Nz - 1)
+    I, J, K = np.meshgrid(gi, gj, gk, indexing="ij")
+    tw = 2 * np.pi
+    if wall:
+        eta = (np.exp(2.5 * J) - 1.0) / (np.exp(2.5) - 1.0)          # clustered at the wall
+    else:
+        eta = J
+    X = lx * I + amp_grid * lx * np.sin(tw * I) * np.sin(np.pi * J) * np.cos(tw * K)
+    Y = ly * eta + amp_grid * ly * np.sin(tw * I + 0.3) * np.sin(np.pi * J) * np.cos(tw * K + 0.2)
+    Z = lz * K + amp_grid * lz * np.cos(tw * I) * np.sin(np.pi * J + 0.1) * np.sin(tw * K)
+    ox = np.concatenate([[0], np.cumsum(np.array(nx) - 1)])
+    oy = np.concatenate([[0], np.cumsum(np.array(ny) - 1)])
+    oz = np.concatenate([[0], np.cumsum(np.array(nz) - 1)])
+
+    def bid(a, b, c):
+        return a + A * (b + B * c)
+
+    def faces(ni, nj, nk):
+        return [(1, 1, 1, nj, 1, nk), (ni, ni, 1, nj, 1, nk), (1, ni, 1, 1, 1, nk), (1, ni, nj, nj, 1, nk),
+                (1, ni, 1, nj, 1, 1), (1, ni, 1, nj, nk, nk)]
+
+    blocks = [None] * (A * B * C)
+    for c in range(C):
+        for b in range(B):
+            for a in range(A):
+                ni, nj, nk = nx[a], ny[b], nz[c]
+                sl = (slice(ox[a], ox[a] + ni), slice(oy[b], oy[b] + nj), slice(oz[c], oz[c] + nk))
+                fs = faces(ni, nj, nk)
+                regs = []
+                for f in range(6):
+                    ax, hi = f // 2, f % 2
+                    idx, n_ax = [a, b, c], [A, B, C][ax]
+                    nbr = idx[ax] + (1 if hi else -1)
+                    periodic = (ax == 0 and periodic_i) or (ax == 2 and periodic_k)
+                    if 0 <= nbr < n_ax or periodic:
+                        idx[ax] = nbr % n_ax
+                        t = bid(*idx)
+                        tdims = (nx[idx[0]], ny[idx[1]], nz[idx[2]])
+                        tf = 2 * ax + (1 - hi)
+                        regs.append(Region(fs[f], -1, faces(*tdims)[tf], t + 1, tf + 1))
+                    elif ax == 1 and hi == 0 and wall:
+                        regs.append(Region(fs[f], 2))
+                    else:
+                        regs.append(Region(fs[f], 4))
+                blocks[bid(a, b, c)] = Block(ni, nj, nk, X[sl].copy(), Y[sl].copy(), Z[sl].copy(), regs,
+                                             name=f"b{a}{b}{c}")
+    a_m = rng.random(nmodes)
+    a_m = a_m / a_m.sum()
+    modes = [(int(rng.integers(1, 3)), float(rng.random()), int(rng.integers(0, 2)), float(a_m[m]),
+              float(rng.random() * 2 * np.pi), float(rng.random() * 2 * np.pi)) for m in range(nmodes)]
+    prm = Params(moo=moo, reynolds=reynolds, cfl=cfl, nscheme=nscheme, nflux=nflux, nvis=nvis, attack=attack)
+    gx = (X.min(), X.max() - X.min()), (Y.min(), Y.max() - Y.min()), (Z.min(), Z.max() - Z.min())
+
+    def init(case, nb):
+        blk = case.blocks[nb]
+        roo, uoo, voo, woo, poo = case.freestream()
+        shp = (blk.ni + 6, blk.nj + 6, blk.nk + 6)
+        r, p = np.full(shp, roo), np.full(shp, poo)
+        pr = np.zeros_like(blk.x)
+        pp = np.zeros_like(blk.x)
+        for (kx, ky, kz, am, phr, php) in modes:
+            arg = 2 * np.pi * (kx * (blk.x - gx[0][0]) / gx[0][1] + ky * (blk.y - gx[1][0]) / gx[1][1]
+                               + kz * (blk.z - gx[2][0]) / gx[2][1])
+            pr += am * np.sin(arg + phr)
+            pp += am * np.sin(arg + php)
+        r[3:-3, 3:-3, 3:-3] = roo * (1.0 + amp * pr)
+        p[3:-3, 3:-3, 3:-3] = poo * (1.0 + amp * pp)
+        return r, np.full(shp, uoo), np.full(shp, voo), np.full(shp, woo), p
+
+    return Case(prm, blocks, init=init, name=f"mb{A}x{B}x{C}", seed=seed)
+
+
+def uneven_blocks_case(nblk=(4, 4, 4), lo=24, hi=96, seed=7, **kw):
+    """nblk[0] x nblk[1] x nblk[2] blocks whose node counts per axis are drawn
+    from [lo, hi] with the seeded law s = lo + (hi - lo) U^(1/3) (mean 0.75 of
+    the way up), so that 64 blocks of sides in [24, 96] hold about 30 M nodes."""
+    rng = np.random.default_rng(seed)
+    draw = lambda n: tuple(int(lo + (hi - lo) * u ** (1.0 / 3.0)) for u in rng.random(n))
+    return multiblock_case(draw(nblk[0]), draw(nblk[1]), draw(nblk[2]), seed=seed, **kw)

@@ -66,9 +66,14 @@ struct wcns_ctx {
     double *dn = nullptr;                   // 12 node derivatives (viscous)
     double *coed = nullptr;                 // LU-SGS diagonal per node
     float *dq32 = nullptr;                  // dQ of the FP32 implicit operator
-    double *dqB = nullptr;                  // second dQ buffer of the SGS sweep pair
-    float *dqB32 = nullptr;
-    std::vector<SweepTiles> tiles;          // per block
+    void *skew[2] = {nullptr, nullptr};     // plane-major sweep operands, FP64 / FP32 implicit operator
+    std::vector<long> skoff;                // per block offset in each plane-major array
+    long sktot = 0;
+    SweepPlan plan{};                       // batched pipelined sweeps over all blocks
+    std::vector<std::pair<int, int>> tile_shape;   // per block (TJ, TK)
+    SweepBlock *d_sblocks = nullptr;
+    int2 *d_work = nullptr;                 // forward list, then backward list
+    unsigned long long *d_xch = nullptr;
     int *d_sweep_err = nullptr;
     bool tiled_sweeps = true;
     unsigned long long *counters = nullptr;
@@ -312,6 +317,9 @@ void build_groups(wcns_ctx *ctx, std::vector<std::vector<Pt>> &groups)
 
 long g3_of(int ni, int nj, int nk) { return (long)(ni + 6) * (nj + 6) * (nk + 6); }
 
+// Every device operation of a context goes through its (non-blocking) stream,
+// so that fills and copies are ordered with the kernels around them; the
+// copies wait for completion because the host buffers are the caller's.
 template <typename T>
 int dalloc(wcns_ctx *ctx, T **p, size_t n)
 {
@@ -319,7 +327,17 @@ int dalloc(wcns_ctx *ctx, T **p, size_t n)
         ctx->err = "hipMalloc of " + std::to_string(sizeof(T) * n) + " bytes failed";
         return WCNS_E_NOMEM;
     }
-    if (hipMemset(*p, 0, sizeof(T) * (n ? n : 1)) != hipSuccess) return WCNS_E_HIP;
+    if (hipMemsetAsync(*p, 0, sizeof(T) * (n ? n : 1), ctx->stream) != hipSuccess) return WCNS_E_HIP;
+    return 0;
+}
+
+int copy_sync(wcns_ctx *ctx, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    if (hipMemcpyAsync(dst, src, bytes, kind, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        ctx->err = "device copy failed";
+        return WCNS_E_HIP;
+    }
     return 0;
 }
 
@@ -338,7 +356,7 @@ int check_counters(wcns_ctx *ctx)
     if (c[4]) return fail(ctx, WCNS_E_REPAIR, "too many abnormal nodes in update_limited (Sub2.f90:781-785)");
     if (ctx->d_sweep_err) {
         int e = 0;
-        CK(hipMemcpy(&e, ctx->d_sweep_err, sizeof(int), hipMemcpyDeviceToHost));
+        TRY(copy_sync(ctx, &e, ctx->d_sweep_err, sizeof(int), hipMemcpyDeviceToHost));
         if (e) return fail(ctx, WCNS_E_HIP, "a pipelined sweep wait expired");
     }
     return 0;
@@ -376,8 +394,9 @@ int wcns_destroy(wcns_ctx *ctx)
     for (int b = 0; b < 2; b++) for (int m = 0; m < 5; m++) hipFree(ctx->prim[b][m]);
     for (int m = 0; m < 10; m++) hipFree(ctx->metric[m]);
     for (int m = 0; m < 3; m++) { hipFree(ctx->sr[m]); hipFree(ctx->srv[m]); }
-    for (auto &st : ctx->tiles) { hipFree(st.order_fwd); hipFree(st.order_bwd); hipFree(st.xj); }
-    void *ps[] = {ctx->dqB, ctx->dqB32, ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
+    void *pl[] = {ctx->d_sblocks, ctx->d_work, ctx->d_xch};
+    for (void *p : pl) if (p) hipFree(p);
+    void *ps[] = {ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
                   ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
@@ -425,13 +444,89 @@ int wcns_add_region(wcns_ctx *ctx, int nb, const int s_raw[6], int bctype, const
     return WCNS_OK;
 }
 
+// Plane-major copy of the sweep operands for the pipelined sweeps; the
+// metrics are staged once here, the per-step operands before every sweep set.
+static int alloc_skew(wcns_ctx *ctx, int prec)
+{
+    size_t n = (size_t)SK_N * ctx->sktot;
+    if (prec == 1) TRY(dalloc(ctx, (float **)&ctx->skew[1], n));
+    else TRY(dalloc(ctx, (double **)&ctx->skew[0], n));
+    DFields F = ctx->fields();
+    for (size_t nb = 0; nb < ctx->blocks.size(); nb++)
+        if (launch_skew_stage(SKEW_METRICS, ctx->blocks[nb].db, ctx->skoff[nb], F, prec, ctx->skew[prec], ctx->sktot,
+                              ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "staging the metrics");
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// (Re)builds the batched sweep plan from ctx->tile_shape: per-block tile
+// grids and exchange slots, and the work lists.  The forward list orders all
+// tiles of all blocks by tile diagonal J+K, then block, then J -- a
+// topological order of every block's tile graph, which interleaves the blocks
+// so that their sweeps run side by side; the backward list is its reverse.
+static int build_sweep_plan(wcns_ctx *ctx)
+{
+    std::vector<SweepBlock> sb;
+    std::vector<long> xoff;
+    long nx = 0;
+    int lanes = 1;
+    struct W { int key, nb, J, t; };
+    std::vector<W> wl;
+    for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
+        const HBlock &b = ctx->blocks[nb];
+        SweepBlock q{};
+        q.b = b.db;
+        q.soff = ctx->skoff[nb];
+        q.TJ = ctx->tile_shape[nb].first;
+        q.TK = ctx->tile_shape[nb].second;
+        if (q.TJ < 1 || q.TK < 1 || q.TJ * q.TK > 64) return fail(ctx, WCNS_E_ARG, "bad sweep tile shape");
+        q.nTJ = (b.nj + q.TJ - 1) / q.TJ;
+        q.nTK = (b.nk + q.TK - 1) / q.TK;
+        lanes = std::max(lanes, q.TJ * q.TK);
+        xoff.push_back(nx);
+        nx += sweep_exchange_words(q.TJ, q.TK, q.nTJ, q.nTK, b.ni);
+        for (int t = 0; t < q.nTJ * q.nTK; t++) wl.push_back({t % q.nTJ + t / q.nTJ, (int)nb, t % q.nTJ, t});
+        sb.push_back(q);
+    }
+    std::sort(wl.begin(), wl.end(), [](const W &a, const W &c) {
+        return std::make_tuple(a.key, a.nb, a.J) < std::make_tuple(c.key, c.nb, c.J);
+    });
+    std::vector<int2> work;
+    for (auto &w : wl) work.push_back(make_int2(w.nb, w.t));
+    for (auto it = wl.rbegin(); it != wl.rend(); ++it) work.push_back(make_int2(it->nb, it->t));
+    if (ctx->d_sblocks) { hipFree(ctx->d_sblocks); ctx->d_sblocks = nullptr; }
+    if (ctx->d_work) { hipFree(ctx->d_work); ctx->d_work = nullptr; }
+    if (ctx->d_xch) { hipFree(ctx->d_xch); ctx->d_xch = nullptr; }
+    TRY(dalloc(ctx, &ctx->d_xch, nx));
+    for (size_t nb = 0; nb < sb.size(); nb++) {
+        sb[nb].xj = (double *)(ctx->d_xch + xoff[nb]);
+        sb[nb].xk = sb[nb].xj + (long)sb[nb].nTJ * sb[nb].nTK * sb[nb].TK * sb[nb].b.ni * 5;
+    }
+    TRY(dalloc(ctx, &ctx->d_sblocks, sb.size()));
+    TRY(dalloc(ctx, &ctx->d_work, work.size()));
+    TRY(copy_sync(ctx, ctx->d_sblocks, sb.data(), sizeof(SweepBlock) * sb.size(), hipMemcpyHostToDevice));
+    TRY(copy_sync(ctx, ctx->d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice));
+    SweepPlan &P = ctx->plan;
+    P.blocks = ctx->d_sblocks;
+    P.nwork = (int)wl.size();
+    P.work_fwd = ctx->d_work;
+    P.work_bwd = ctx->d_work + wl.size();
+    P.lanes = lanes;
+    P.xch = ctx->d_xch;
+    P.nxch = nx;
+    P.err = ctx->d_sweep_err;
+    P.spin_limit = 2000000;
+    return 0;
+}
+
 int wcns_set_lhs_precision(wcns_ctx *ctx, int prec)
 {
     if (!ctx || (prec != WCNS_LHS_FP64 && prec != WCNS_LHS_FP32)) return WCNS_E_ARG;
     if (prec == WCNS_LHS_FP32 && !ctx->tiled_sweeps)
         return fail(ctx, WCNS_E_UNSUPPORTED, "the FP32 implicit operator runs in the pipelined sweeps only");
     if (prec == WCNS_LHS_FP32 && ctx->setup_done && !ctx->dq32) TRY(dalloc(ctx, &ctx->dq32, 5 * ctx->total));
-    if (prec == WCNS_LHS_FP32 && ctx->setup_done && !ctx->dqB32) TRY(dalloc(ctx, &ctx->dqB32, 5 * ctx->total));
+    if (prec == WCNS_LHS_FP32 && ctx->setup_done && ctx->tiled_sweeps && !ctx->skew[1]) TRY(alloc_skew(ctx, 1));
     ctx->lhs_prec = prec;
     return WCNS_OK;
 }
@@ -481,39 +576,17 @@ int wcns_setup(wcns_ctx *ctx)
     TRY(dalloc(ctx, &ctx->t, T));
     TRY(dalloc(ctx, &ctx->rhs, 5 * T));
     TRY(dalloc(ctx, &ctx->dq, 5 * T));
-    TRY(dalloc(ctx, &ctx->dqB, 5 * T));
-    if (ctx->lhs_prec == WCNS_LHS_FP32) {
-        TRY(dalloc(ctx, &ctx->dq32, 5 * T));
-        TRY(dalloc(ctx, &ctx->dqB32, 5 * T));
-    }
+    if (ctx->lhs_prec == WCNS_LHS_FP32) TRY(dalloc(ctx, &ctx->dq32, 5 * T));
     TRY(dalloc(ctx, &ctx->dtdt, T));
     TRY(dalloc(ctx, &ctx->visl, T));
     if (p.nvis == 1) TRY(dalloc(ctx, &ctx->dn, 12 * T));
     TRY(dalloc(ctx, &ctx->coed, T));
     TRY(dalloc(ctx, &ctx->d_sweep_err, 1));
-    // sweep tiles: TJ x TK columns per workgroup, dispatched in topological order
+    // sweep tiles: TJ x TK columns per workgroup (16 x 4 unless tuned)
+    ctx->tile_shape.clear();
     for (auto &b : ctx->blocks) {
-        SweepTiles st{};
-        st.TJ = std::min(16, b.nj);
-        st.TK = std::max(1, std::min(64 / st.TJ, b.nk));
-        st.nTJ = (b.nj + st.TJ - 1) / st.TJ;
-        st.nTK = (b.nk + st.TK - 1) / st.TK;
-        int nt = st.nTJ * st.nTK;
-        std::vector<int> tl(nt);
-        for (int t = 0; t < nt; t++) tl[t] = t;
-        auto key = [&](int t) { return std::make_pair(t % st.nTJ + t / st.nTJ, t % st.nTJ); };
-        std::sort(tl.begin(), tl.end(), [&](int a, int c) { return key(a) < key(c); });
-        std::vector<int> tb(tl.rbegin(), tl.rend());
-        TRY(dalloc(ctx, &st.order_fwd, nt));
-        TRY(dalloc(ctx, &st.order_bwd, nt));
-        long nx = sweep_exchange_words(st, b.ni);
-        TRY(dalloc(ctx, &st.xj, nx));
-        st.xk = st.xj + (long)nt * st.TK * b.ni * 5;
-        CK(hipMemcpy(st.order_fwd, tl.data(), 4 * nt, hipMemcpyHostToDevice));
-        CK(hipMemcpy(st.order_bwd, tb.data(), 4 * nt, hipMemcpyHostToDevice));
-        st.err = ctx->d_sweep_err;
-        st.spin_limit = 2000000;
-        ctx->tiles.push_back(st);
+        int TJ = std::min(16, b.nj);
+        ctx->tile_shape.push_back({TJ, std::max(1, std::min(64 / TJ, b.nk))});
     }
     TRY(dalloc(ctx, &ctx->counters, 8));
     TRY(dalloc(ctx, &ctx->d_flag, ctx->flag_cap));
@@ -522,7 +595,7 @@ int wcns_setup(wcns_ctx *ctx)
     for (auto &b : ctx->blocks) { hb.push_back(b.db); maxp = std::max(maxp, residual_parts(b.db)); }
     ctx->max_parts = maxp;
     TRY(dalloc(ctx, &ctx->d_blocks, hb.size()));
-    CK(hipMemcpy(ctx->d_blocks, hb.data(), sizeof(DBlock) * hb.size(), hipMemcpyHostToDevice));
+    TRY(copy_sync(ctx, ctx->d_blocks, hb.data(), sizeof(DBlock) * hb.size(), hipMemcpyHostToDevice));
     TRY(dalloc(ctx, &ctx->d_partial, 3 * maxp));
     TRY(dalloc(ctx, &ctx->d_resblk, 3 * ctx->blocks.size()));
 
@@ -546,9 +619,9 @@ int wcns_setup(wcns_ctx *ctx)
         double *dx, *dy, *dz, *work;
         TRY(dalloc(ctx, &dx, n)); TRY(dalloc(ctx, &dy, n)); TRY(dalloc(ctx, &dz, n));
         TRY(dalloc(ctx, &work, 21 * b.g3 + 1));
-        CK(hipMemcpy(dx, b.x.data(), 8 * n, hipMemcpyHostToDevice));
-        CK(hipMemcpy(dy, b.y.data(), 8 * n, hipMemcpyHostToDevice));
-        CK(hipMemcpy(dz, b.z.data(), 8 * n, hipMemcpyHostToDevice));
+        TRY(copy_sync(ctx, dx, b.x.data(), 8 * n, hipMemcpyHostToDevice));
+        TRY(copy_sync(ctx, dy, b.y.data(), 8 * n, hipMemcpyHostToDevice));
+        TRY(copy_sync(ctx, dz, b.z.data(), 8 * n, hipMemcpyHostToDevice));
         if (launch_gcl_metrics(b.db, dx, dy, dz, F, work, ctx->stream)) return fail(ctx, WCNS_E_HIP, "gcl launch");
         int nbad = 0;
         CK(hipMemcpyAsync(&nbad, (int *)(work + 21 * b.g3), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -566,7 +639,7 @@ int wcns_setup(wcns_ctx *ctx)
             int c[3] = {(g.s_lo[0] + g.s_hi[0]) / 2, (g.s_lo[1] + g.s_hi[1]) / 2, (g.s_lo[2] + g.s_hi[2]) / 2};
             long q = gidx(b.db, c[0], c[1], c[2]);
             double m[9];
-            for (int a = 0; a < 9; a++) CK(hipMemcpy(&m[a], ctx->metric[a] + q, 8, hipMemcpyDeviceToHost));
+            for (int a = 0; a < 9; a++) TRY(copy_sync(ctx, &m[a], ctx->metric[a] + q, 8, hipMemcpyDeviceToHost));
             double x = 0, y = 0, z = 0;
             for (int a = 0; a < 3; a++)
                 if (g.l3[a]) { x = x + m[3 * a]; y = y + m[3 * a + 1]; z = z + m[3 * a + 2]; }
@@ -575,6 +648,15 @@ int wcns_setup(wcns_ctx *ctx)
             double s2 = sqrt(nx * nx + ny * ny + nz * nz);
             g.nrm[0] = nx / s2; g.nrm[1] = ny / s2; g.nrm[2] = nz / s2;
         }
+    // plane-major sweep operands
+    ctx->skoff.clear();
+    ctx->sktot = 0;
+    for (auto &b : ctx->blocks) { ctx->skoff.push_back(ctx->sktot); ctx->sktot += skew_size(b.db); }
+    if (ctx->tiled_sweeps) {
+        TRY(alloc_skew(ctx, 0));
+        if (ctx->lhs_prec == WCNS_LHS_FP32) TRY(alloc_skew(ctx, 1));
+        TRY(build_sweep_plan(ctx));
+    }
     // interface lists: 2pm regions in (nb, nr) order, serial when a region reads its own writes
     ctx->iface.clear();
     ctx->phys.clear();
@@ -623,10 +705,10 @@ int wcns_setup(wcns_ctx *ctx)
     TRY(dalloc(ctx, &ctx->d_write_ok, std::max<size_t>(ok.size(), 1)));
     TRY(dalloc(ctx, &ctx->d_npp_buf, 6 * std::max(np, 1)));
     if (np) {
-        CK(hipMemcpy(ctx->d_npp_pts, pts.data(), 4 * pts.size(), hipMemcpyHostToDevice));
-        CK(hipMemcpy(ctx->d_grp_start, gstart.data(), 4 * gstart.size(), hipMemcpyHostToDevice));
-        CK(hipMemcpy(ctx->d_grp_n, gn.data(), 4 * gn.size(), hipMemcpyHostToDevice));
-        CK(hipMemcpy(ctx->d_write_ok, ok.data(), 4 * ok.size(), hipMemcpyHostToDevice));
+        TRY(copy_sync(ctx, ctx->d_npp_pts, pts.data(), 4 * pts.size(), hipMemcpyHostToDevice));
+        TRY(copy_sync(ctx, ctx->d_grp_start, gstart.data(), 4 * gstart.size(), hipMemcpyHostToDevice));
+        TRY(copy_sync(ctx, ctx->d_grp_n, gn.data(), 4 * gn.size(), hipMemcpyHostToDevice));
+        TRY(copy_sync(ctx, ctx->d_write_ok, ok.data(), 4 * ok.size(), hipMemcpyHostToDevice));
     }
     // walls: face-node coordinates in region-point order, force scratch
     size_t maxw = 1, maxq = 1;
@@ -640,7 +722,7 @@ int wcns_setup(wcns_ctx *ctx)
             xyz[t] = b.x[q]; xyz[w.d.npts + t] = b.y[q]; xyz[2 * (size_t)w.d.npts + t] = b.z[q];
         }
         TRY(dalloc(ctx, &w.xyz, xyz.size()));
-        CK(hipMemcpy(w.xyz, xyz.data(), 8 * xyz.size(), hipMemcpyHostToDevice));
+        TRY(copy_sync(ctx, w.xyz, xyz.data(), 8 * xyz.size(), hipMemcpyHostToDevice));
         maxw = std::max(maxw, (size_t)w.d.npts);
         maxq = std::max(maxq, (size_t)force_parts(w.d));
     }
@@ -681,7 +763,7 @@ int wcns_set_prim(wcns_ctx *ctx, int nb, const double *r, const double *u, const
     const double *src[5] = {r, u, v, w, p};
     for (int bb = 0; bb < 2; bb++)
         for (int m = 0; m < 5; m++)
-            CK(hipMemcpy(ctx->prim[bb][m] + b.db.off, src[m], 8 * b.g3, hipMemcpyHostToDevice));
+            TRY(copy_sync(ctx, ctx->prim[bb][m] + b.db.off, src[m], 8 * b.g3, hipMemcpyHostToDevice));
     // initial_t (sub_tgh.f90:36-58): temperature on the interior for nvis == 1
     std::vector<double> t(b.g3, 0.0);
     if (ctx->prm.nvis == 1) {
@@ -691,7 +773,7 @@ int wcns_set_prim(wcns_ctx *ctx, int nb, const double *r, const double *u, const
             t[q] = moo2 * (ctx->prm.gama * p[q] / r[q]);
         }
     }
-    CK(hipMemcpy(ctx->t + b.db.off, t.data(), 8 * b.g3, hipMemcpyHostToDevice));
+    TRY(copy_sync(ctx, ctx->t + b.db.off, t.data(), 8 * b.g3, hipMemcpyHostToDevice));
     return WCNS_OK;
 }
 
@@ -705,7 +787,7 @@ int wcns_get_field(wcns_ctx *ctx, int nb, int which, double *out)
     HBlock &b = ctx->blocks[nb];
     CK(hipStreamSynchronize(ctx->stream));
     for (int m = 0; m < nc; m++)
-        CK(hipMemcpy(out + m * b.g3, p + m * ctx->total + b.db.off, 8 * b.g3, hipMemcpyDeviceToHost));
+        TRY(copy_sync(ctx, out + m * b.g3, p + m * ctx->total + b.db.off, 8 * b.g3, hipMemcpyDeviceToHost));
     return WCNS_OK;
 }
 
@@ -719,7 +801,7 @@ int wcns_set_field(wcns_ctx *ctx, int nb, int which, const double *in)
     HBlock &b = ctx->blocks[nb];
     CK(hipStreamSynchronize(ctx->stream));
     for (int m = 0; m < nc; m++)
-        CK(hipMemcpy(p + m * ctx->total + b.db.off, in + m * b.g3, 8 * b.g3, hipMemcpyHostToDevice));
+        TRY(copy_sync(ctx, p + m * ctx->total + b.db.off, in + m * b.g3, 8 * b.g3, hipMemcpyHostToDevice));
     return WCNS_OK;
 }
 
@@ -766,10 +848,14 @@ int wcns_r_h_s_vis(wcns_ctx *ctx)
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     if (ctx->prm.nvis != 1) return WCNS_OK;
     DFields F = ctx->fields();
-    Scope sc(ctx, WCNS_T_VIS, 4 * (int)ctx->blocks.size());
-    for (auto &b : ctx->blocks) {
-        if (launch_vis_node_deriv(b.db, F, ctx->dn, ctx->stream)) return fail(ctx, WCNS_E_HIP, "vis deriv launch");
-        for (int dir = 0; dir < 3; dir++)
+    {
+        Scope sc(ctx, WCNS_T_VIS, (int)ctx->blocks.size());
+        for (auto &b : ctx->blocks)
+            if (launch_vis_node_deriv(b.db, F, ctx->dn, ctx->stream)) return fail(ctx, WCNS_E_HIP, "vis deriv launch");
+    }
+    for (int dir = 0; dir < 3; dir++) {
+        Scope sc(ctx, WCNS_T_VIS_I + dir, (int)ctx->blocks.size());
+        for (auto &b : ctx->blocks)
             if (launch_rhs_vis_dir(dir, b.db, F, ctx->dp, ctx->dn, ctx->stream)) return fail(ctx, WCNS_E_HIP, "vis launch");
     }
     return WCNS_OK;
@@ -822,19 +908,32 @@ int wcns_l_h_s(wcns_ctx *ctx)
     DFields F = ctx->fields();
     std::vector<DBlock> dbs;
     for (auto &b : ctx->blocks) dbs.push_back(b.db);
+    const int prec = F.dq32 ? 1 : 0;
+    for (auto &g : ctx->phys) launch_dq_zero_face(g, ctx->blocks[g.nb].db, F.rhs, F.total, ctx->stream);
+    if (ctx->tiled_sweeps) {
+        {
+            Scope sc(ctx, WCNS_T_SWEEP_STAGE, (int)ctx->blocks.size());
+            for (size_t nb = 0; nb < ctx->blocks.size(); nb++)
+                if (launch_skew_stage(SKEW_STEP, ctx->blocks[nb].db, ctx->skoff[nb], F, prec, ctx->skew[prec],
+                                      ctx->sktot, ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "staging the sweep operands");
+        }
+        for (int kind = 0; kind < 4; kind++) {
+            Scope sc(ctx, WCNS_T_SWEEP_LU_L + kind);
+            if (launch_sweep_batched(kind, ctx->plan, F, ctx->dp, prec, ctx->skew[prec], ctx->sktot, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "sweeps");
+        }
+    } else {
+        for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
+            const DBlock &db = ctx->blocks[nb].db;
+            for (int kind = 0; kind < 4; kind++) {
+                Scope sc(ctx, WCNS_T_SWEEP_LU_L + kind, db.ni + db.nj + db.nk - 2);
+                if (launch_sweep(kind, db, F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "sweeps");
+            }
+        }
+    }
     for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
         const DBlock &db = ctx->blocks[nb].db;
-        for (auto &g : ctx->phys)
-            if (g.nb == (int)nb) launch_dq_zero_face(g, db, F.rhs, F.total, ctx->stream);
-        int nplanes = db.ni + db.nj + db.nk - 2;
-        for (int kind = 0; kind < 4; kind++) {
-            Scope sc(ctx, WCNS_T_SWEEP_LU_L + kind, ctx->tiled_sweeps ? 1 : nplanes);
-            void *A = F.dq32 ? (void *)F.dq32 : (void *)F.dq;
-            void *B = F.dq32 ? (void *)ctx->dqB32 : (void *)ctx->dqB;
-            int e = ctx->tiled_sweeps ? launch_sweep_tiled(kind, db, F, ctx->dp, ctx->tiles[nb], ctx->lhs_prec, A, B, ctx->stream)
-                                      : launch_sweep(kind, db, F, ctx->dp, ctx->stream);
-            if (e) return fail(ctx, WCNS_E_HIP, "sweeps");
-        }
         for (auto &g : ctx->phys)
             if (g.nb == (int)nb) {
                 if (F.dq32) launch_dq_zero_face32(g, db, F.dq32, F.total, ctx->stream);
@@ -962,7 +1061,7 @@ int wcns_get_counter(wcns_ctx *ctx, int which, long long *value)
     if (!ctx->setup_done) return WCNS_E_STATE;
     unsigned long long c[8];
     CK(hipStreamSynchronize(ctx->stream));
-    CK(hipMemcpy(c, ctx->counters, sizeof c, hipMemcpyDeviceToHost));
+    TRY(copy_sync(ctx, c, ctx->counters, sizeof c, hipMemcpyDeviceToHost));
     *value = (long long)c[which];
     return WCNS_OK;
 }

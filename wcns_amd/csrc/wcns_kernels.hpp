@@ -36,18 +36,36 @@ __host__ __device__ inline void region_image(const DRegion &g, int i, int j, int
 
 int launch_rhs_inv(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
 int launch_rhs_inv_dir(int dir, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
-// Tile decomposition of one block's (j,k) columns for the pipelined sweeps.
-struct SweepTiles {
+// The pipelined sweeps of all blocks run in one persistent launch per sweep:
+// every block's (j,k) columns are cut into TJ x TK tiles, one workgroup per
+// tile, and the workgroups follow a work list that is a topological order of
+// every block's tile graph.
+struct SweepBlock {
+    DBlock b;
+    long soff;                    // offset of the block in the plane-major arrays
     int TJ, TK, nTJ, nTK;
-    int *order_fwd, *order_bwd;   // blockIdx -> tile, topological for each sweep direction
-    double *xj, *xk;              // boundary exchange slots: [tile][TK][ni][5], then [tile][TJ][ni][5]
+    double *xj, *xk;              // exchange slots: [tile][TK][ni][5], [tile][TJ][ni][5]
+};
+struct SweepPlan {
+    const SweepBlock *blocks;     // device copy, one per block
+    const int2 *work_fwd, *work_bwd;  // (block, tile) per workgroup, forward / backward sweeps
+    int nwork, lanes;             // workgroups; lanes per workgroup (largest TJ*TK)
+    unsigned long long *xch;      // all exchange slots
+    long nxch;
     int *err;                     // set when a bounded wait expires
     long spin_limit;
 };
-long sweep_exchange_words(const SweepTiles &T, int ni);
-// A: dQ (double or float by prec); B: the second buffer of the SGS pair, same type
-int launch_sweep_tiled(int kind, const DBlock &b, const DFields &F, const DParams &P, const SweepTiles &T,
-                       int prec, void *A, void *B, hipStream_t s);
+long sweep_exchange_words(int TJ, int TK, int nTJ, int nTK, int ni);
+// Plane-major staging copy of the sweep operands: arrays of skew_size(b)
+// elements per block (element (i,j,k), i,j,k in 0..n+1, at (i+j+k)*(nj+2)*(nk+2)
+// + j + (nj+2)*k), SK_N arrays of sktot elements each, double or float (prec).
+enum { SK_P = 0, SK_M = 5, SK_SR = 14, SK_SRV = 17, SK_RHS = 20, SK_COED = 25, SK_A = 26, SK_B = 31, SK_N = 36 };
+enum { SKEW_STEP = 0, SKEW_METRICS = 1 };
+long skew_size(const DBlock &b);
+int launch_skew_stage(int what, const DBlock &b, long soff, const DFields &F, int prec, void *pool, long sktot,
+                      hipStream_t s);
+int launch_sweep_batched(int kind, const SweepPlan &S, const DFields &F, const DParams &P, int prec, void *pool,
+                         long sktot, hipStream_t s);
 int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
 int launch_dq_zero_face32(const DRegion &g, const DBlock &b, float *arr, long total, hipStream_t s);
 

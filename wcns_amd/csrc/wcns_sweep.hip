@@ -27,7 +27,11 @@
 //                  except the neighbours' new dQ -- primitives, metrics,
 //                  spectral radii, the split-Jacobian coefficients, the
 //                  complete terms of the previous-sweep neighbours -- is formed
-//                  one step ahead, off the dependency chain.
+//                  one step ahead, off the dependency chain.  The operands are
+//                  read from a plane-major copy (index (i+j+k, j, k), one ghost
+//                  layer) staged once per step by k_skew_stage, so the lanes of
+//                  a tile read a few contiguous rows per array at every step
+//                  instead of one cache line per lane.
 #include "wcns_kernels.hpp"
 
 namespace wcns {
@@ -36,18 +40,59 @@ namespace {
 
 inline dim3 grid1(long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
+// Five component arrays of a dQ field, passed by value (indexing a kernel
+// argument through a pointer would copy it to scratch).
+template <typename T>
+struct TPtrs {
+    T *c[5];
+};
+
 // Sub1.f90:676-717 getrkec_mml clamp + LHS.f90:647-678 getrarbrc clamp
 __device__ inline int clampi(int v, int n) { return v < 1 ? 1 : (v > n ? n : v); }
 
-// Coefficients of the split Jacobian of neighbour (i,j,k)+off*e_dir: the
+// Where a sweep reads its operands: base pointers of the primitive, metric,
+// spectral-radius, residual and diagonal arrays (element type S) and an index
+// map IX from (i,j,k) to an element offset.  The per-hyperplane schedule reads
+// the field pools directly (S = double, G3 index); the pipelined one reads the
+// plane-major staging copy (S = T).
+template <typename S>
+struct Operands {
+    const S *p[5];
+    const S *m[3][3];     // [direction][x, y, z]: kcx kcy kcz / etx ety etz / ctx cty ctz
+    const S *sr[3];       // sra, srb, src
+    const S *srv[3];      // srva, srvb, srvc
+    const S *rhs[5];
+    const S *coed;
+};
+
+struct G3Index {
+    DBlock b;
+    __device__ long operator()(int i, int j, int k) const { return gidx(b, i, j, k); }
+};
+
+struct SkIndex {
+    long off;
+    int pl, nj2;
+    __device__ long operator()(int i, int j, int k) const
+    {
+        return off + (long)(i + j + k) * pl + j + (long)nj2 * k;
+    }
+};
+
+// Raw operands of the split Jacobian of neighbour (i,j,k)+off*e_dir: the
 // neighbour's primitives, the metric at the neighbour with ghost indices
 // clamped to the face, and the spectral radius clamped to 1..n.
 template <typename T>
-__device__ inline void nbr_coef(const DBlock &b, const DFields &F, const DParams &P, int i, int j, int k, int dir,
-                                int off, MxCoef<T> &c)
+struct NbrRaw {
+    T prim[5], gy[3], rad;
+};
+
+template <typename T, typename S, typename IX>
+__device__ inline void nbr_load(const DBlock &b, const Operands<S> &O, const IX &ix, int i, int j, int k, int dir,
+                                int off, NbrRaw<T> &r)
 {
     int in = i + (dir == 0 ? off : 0), jn = j + (dir == 1 ? off : 0), kn = k + (dir == 2 ? off : 0);
-    long gp = gidx(b, in, jn, kn);
+    long gp = ix(in, jn, kn);
     int im = in, jm = jn, km = kn;
     if (im == 0) im = 1;
     if (im == b.ni + 1) im = b.ni;
@@ -55,25 +100,33 @@ __device__ inline void nbr_coef(const DBlock &b, const DFields &F, const DParams
     if (jm == b.nj + 1) jm = b.nj;
     if (km == 0) km = 1;
     if (km == b.nk + 1) km = b.nk;
-    long gm = gidx(b, im, jm, km);
+    long gm = ix(im, jm, km);
     long gr;
-    if (dir == 0) gr = gidx(b, clampi(i + off, b.ni), j, k);
-    else if (dir == 1) gr = gidx(b, i, clampi(j + off, b.nj), k);
-    else gr = gidx(b, i, j, clampi(k + off, b.nk));
-    T prim[5] = {(T)F.r[gp], (T)F.u[gp], (T)F.v[gp], (T)F.w[gp], (T)F.p[gp]};
-    const double *mx = dir == 0 ? F.kcx : (dir == 1 ? F.etx : F.ctx);
-    const double *my = dir == 0 ? F.kcy : (dir == 1 ? F.ety : F.cty);
-    const double *mz = dir == 0 ? F.kcz : (dir == 1 ? F.etz : F.ctz);
-    T gy[3] = {(T)mx[gm], (T)my[gm], (T)mz[gm]};
-    const double *sr = dir == 0 ? F.sra : (dir == 1 ? F.srb : F.src);
-    mxdq_coef<T>(prim, gy, (T)sr[gr], off < 0 ? T(1.0) : T(-1.0), (T)P.gama, c);
+    if (dir == 0) gr = ix(clampi(i + off, b.ni), j, k);
+    else if (dir == 1) gr = ix(i, clampi(j + off, b.nj), k);
+    else gr = ix(i, j, clampi(k + off, b.nk));
+#pragma unroll
+    for (int m = 0; m < 5; m++) r.prim[m] = (T)O.p[m][gp];
+    // direction chosen by value, not by indexing the argument arrays (which
+    // would copy them to scratch for a run-time dir)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const S *mc = dir == 0 ? O.m[0][c] : (dir == 1 ? O.m[1][c] : O.m[2][c]);
+        r.gy[c] = (T)mc[gm];
+    }
+    const S *sr = dir == 0 ? O.sr[0] : (dir == 1 ? O.sr[1] : O.sr[2]);
+    r.rad = (T)sr[gr];
 }
 
-// Operands of one node that do not depend on this sweep's new dQ.
+template <int KIND>
+constexpr int sweep_off() { return (KIND == 0 || KIND == 2) ? -1 : 1; }
+
+// Every operand one node reads from memory, as loaded (no arithmetic yet).
 template <int KIND, typename T>
-struct NodeIn {
-    MxCoef<T> cn[3];      // neighbours updated in this sweep, directions i, j, k
-    T fold[3][5];         // SGS: complete terms of the three previous-sweep neighbours
+struct NodeRaw {
+    NbrRaw<T> nn[3];      // neighbours updated in this sweep, directions i, j, k
+    NbrRaw<T> no[3];      // SGS: previous-sweep neighbours
+    T dqo[3][5];          // SGS: their dQ (zero outside the block)
     T ghost[3][5];        // LU: dQ of a this-sweep neighbour outside the block
     T own[5];             // lusgs_u: the node's dQ from lusgs_l
     T rhs[5];
@@ -81,59 +134,106 @@ struct NodeIn {
     T rv[3];              // LU, viscous: neighbours' viscous spectral radii
 };
 
-template <int KIND>
-constexpr int sweep_off() { return (KIND == 0 || KIND == 2) ? -1 : 1; }
+// Operands of one node that do not depend on this sweep's new dQ.
+template <int KIND, typename T>
+struct NodeIn {
+    MxCoef<T> cn[3];      // neighbours updated in this sweep, directions i, j, k
+    T fold[3][5];         // SGS: complete terms of the three previous-sweep neighbours
+    T ghost[3][5];
+    T own[5];
+    T rhs[5];
+    T coed;
+    T rv[3];
+};
 
 // A: the dQ array the sweep pair works in; B: the SGS second buffer (A for the
-// in-place schedule).  gs_pr_l reads its previous-sweep neighbours from A,
-// gs_pr_u from B.
-template <int KIND, typename T>
-__device__ inline void node_inputs(const DBlock &b, const DFields &F, const DParams &P, const T *A, const T *B,
-                                   int i, int j, int k, NodeIn<KIND, T> &in)
+// in-place schedule), both as five component pointers under the same index
+// map.  gs_pr_l reads its previous-sweep neighbours from A, gs_pr_u from B.
+template <int KIND, typename T, typename S, typename IX>
+__device__ inline void node_load(const DBlock &b, const Operands<S> &O, const IX &ix, TPtrs<T> A, TPtrs<T> B,
+                                 const DParams &P, int i, int j, int k, NodeRaw<KIND, T> &r)
 {
     constexpr int off = sweep_off<KIND>();
-    const long T5 = F.total;
-    const long g = gidx(b, i, j, k);
+    const long g = ix(i, j, k);
     const int nn[3] = {b.ni, b.nj, b.nk};
     const int ijk[3] = {i, j, k};
 #pragma unroll
-    for (int d = 0; d < 3; d++) nbr_coef<T>(b, F, P, i, j, k, d, off, in.cn[d]);
+    for (int d = 0; d < 3; d++) nbr_load<T>(b, O, ix, i, j, k, d, off, r.nn[d]);
 #pragma unroll
-    for (int m = 0; m < 5; m++) in.rhs[m] = (T)F.rhs[m * T5 + g];
-    in.coed = (T)F.coed[g];
+    for (int m = 0; m < 5; m++) r.rhs[m] = (T)O.rhs[m][g];
+    r.coed = (T)O.coed[g];
     if (KIND < 2) {
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             int c = ijk[d] + off;
             if (c < 1 || c > nn[d]) {
-                long gn = gidx(b, i + (d == 0 ? off : 0), j + (d == 1 ? off : 0), k + (d == 2 ? off : 0));
+                long gn = ix(i + (d == 0 ? off : 0), j + (d == 1 ? off : 0), k + (d == 2 ? off : 0));
 #pragma unroll
-                for (int m = 0; m < 5; m++) in.ghost[d][m] = A[m * T5 + gn];
+                for (int m = 0; m < 5; m++) r.ghost[d][m] = A.c[m][gn];
             }
         }
         if (KIND == 1) {
 #pragma unroll
-            for (int m = 0; m < 5; m++) in.own[m] = A[m * T5 + g];
+            for (int m = 0; m < 5; m++) r.own[m] = A.c[m][g];
         }
         if (P.nvis > 0) {
-            in.rv[0] = (T)F.srva[gidx(b, i + off, j, k)];
-            in.rv[1] = (T)F.srvb[gidx(b, i, j + off, k)];
-            in.rv[2] = (T)F.srvc[gidx(b, i, j, k + off)];
+            r.rv[0] = (T)O.srv[0][ix(i + off, j, k)];
+            r.rv[1] = (T)O.srv[1][ix(i, j + off, k)];
+            r.rv[2] = (T)O.srv[2][ix(i, j, k + off)];
         }
     } else {
-        const T *old = KIND == 2 ? A : B;
+        const TPtrs<T> old = KIND == 2 ? A : B;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             int c = ijk[d] - off;
-            T dqo[5];
-            long gn = gidx(b, i - (d == 0 ? off : 0), j - (d == 1 ? off : 0), k - (d == 2 ? off : 0));
+            long gn = ix(i - (d == 0 ? off : 0), j - (d == 1 ? off : 0), k - (d == 2 ? off : 0));
 #pragma unroll
-            for (int m = 0; m < 5; m++) dqo[m] = (c < 1 || c > nn[d]) ? T(0.0) : old[m * T5 + gn];
-            MxCoef<T> co;
-            nbr_coef<T>(b, F, P, i, j, k, d, -off, co);
-            mxdq_apply<T>(co, dqo, in.fold[d]);
+            for (int m = 0; m < 5; m++) r.dqo[d][m] = (c < 1 || c > nn[d]) ? T(0.0) : old.c[m][gn];
+            nbr_load<T>(b, O, ix, i, j, k, d, -off, r.no[d]);
         }
     }
+}
+
+// The dQ-independent arithmetic of a node: split-Jacobian coefficients of its
+// this-sweep neighbours and, in the SGS sweeps, the complete terms of the
+// previous-sweep ones.
+template <int KIND, typename T>
+__device__ inline void node_prep(const DParams &P, const NodeRaw<KIND, T> &r, NodeIn<KIND, T> &in)
+{
+    constexpr int off = sweep_off<KIND>();
+    const T gama = (T)P.gama;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        mxdq_coef<T>(r.nn[d].prim, r.nn[d].gy, r.nn[d].rad, off < 0 ? T(1.0) : T(-1.0), gama, in.cn[d]);
+#pragma unroll
+    for (int m = 0; m < 5; m++) in.rhs[m] = r.rhs[m];
+    in.coed = r.coed;
+    if (KIND < 2) {
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int m = 0; m < 5; m++) in.ghost[d][m] = r.ghost[d][m];
+#pragma unroll
+        for (int m = 0; m < 5; m++) in.own[m] = r.own[m];
+#pragma unroll
+        for (int d = 0; d < 3; d++) in.rv[d] = r.rv[d];
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            MxCoef<T> co;
+            mxdq_coef<T>(r.no[d].prim, r.no[d].gy, r.no[d].rad, -off < 0 ? T(1.0) : T(-1.0), gama, co);
+            mxdq_apply<T>(co, r.dqo[d], in.fold[d]);
+        }
+    }
+}
+
+template <int KIND, typename T, typename S, typename IX>
+__device__ inline void node_inputs(const DBlock &b, const Operands<S> &O, const IX &ix, TPtrs<T> A, TPtrs<T> B,
+                                   const DParams &P, int i, int j, int k, NodeIn<KIND, T> &in)
+{
+    NodeRaw<KIND, T> r;
+    node_load<KIND, T>(b, O, ix, A, B, P, i, j, k, r);
+    node_prep<KIND, T>(P, r, in);
 }
 
 // The node's new dQ from its operands and the this-sweep neighbours' dQ nw.
@@ -179,8 +279,24 @@ __device__ inline void outside(const NodeIn<KIND, T> &in, int d, T *v)
     for (int m = 0; m < 5; m++) v[m] = KIND < 2 ? in.ghost[d][m] : T(0.0);
 }
 
+Operands<double> g3_operands(const DFields &F)
+{
+    Operands<double> O;
+    const double *pr[5] = {F.r, F.u, F.v, F.w, F.p};
+    const double *mt[3][3] = {{F.kcx, F.kcy, F.kcz}, {F.etx, F.ety, F.etz}, {F.ctx, F.cty, F.ctz}};
+    for (int m = 0; m < 5; m++) { O.p[m] = pr[m]; O.rhs[m] = F.rhs + m * F.total; }
+    for (int d = 0; d < 3; d++)
+        for (int c = 0; c < 3; c++) O.m[d][c] = mt[d][c];
+    O.sr[0] = F.sra; O.sr[1] = F.srb; O.sr[2] = F.src;
+    O.srv[0] = F.srva; O.srv[1] = F.srvb; O.srv[2] = F.srvc;
+    O.coed = F.coed;
+    return O;
+}
+
+using DqPtrs = TPtrs<double>;
+
 template <int KIND>
-__global__ void __launch_bounds__(256) k_sweep_plane(DBlock b, DFields F, DParams P, int s)
+__global__ void __launch_bounds__(256) k_sweep_plane(DBlock b, Operands<double> O, DqPtrs A, DParams P, int s)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int jlo = s - b.ni - b.nk, jhi = s - 2;
@@ -194,10 +310,9 @@ __global__ void __launch_bounds__(256) k_sweep_plane(DBlock b, DFields F, DParam
     int i = s - j - k;
     if (i < 1 || i > b.ni) return;
     constexpr int off = sweep_off<KIND>();
-    double *A = F.dq;
-    const long T5 = F.total;
+    G3Index ix{b};
     NodeIn<KIND, double> in;
-    node_inputs<KIND, double>(b, F, P, A, A, i, j, k, in);
+    node_inputs<KIND, double>(b, O, ix, A, A, P, i, j, k, in);
     const int nn[3] = {b.ni, b.nj, b.nk};
     const int ijk[3] = {i, j, k};
     double nw[3][5];
@@ -207,16 +322,16 @@ __global__ void __launch_bounds__(256) k_sweep_plane(DBlock b, DFields F, DParam
         if (c < 1 || c > nn[d]) {
             outside<KIND, double>(in, d, nw[d]);
         } else {
-            long gn = gidx(b, i + (d == 0 ? off : 0), j + (d == 1 ? off : 0), k + (d == 2 ? off : 0));
+            long gn = ix(i + (d == 0 ? off : 0), j + (d == 1 ? off : 0), k + (d == 2 ? off : 0));
 #pragma unroll
-            for (int m = 0; m < 5; m++) nw[d][m] = A[m * T5 + gn];
+            for (int m = 0; m < 5; m++) nw[d][m] = A.c[m][gn];
         }
     }
     double out[5];
     node_compute<KIND, double>(P, in, nw, out);
-    long g = gidx(b, i, j, k);
+    long g = ix(i, j, k);
 #pragma unroll
-    for (int m = 0; m < 5; m++) A[m * T5 + g] = out[m];
+    for (int m = 0; m < 5; m++) A.c[m][g] = out[m];
 }
 
 // Empty exchange slot: a signalling NaN; arithmetic only ever yields quiet NaNs.
@@ -271,101 +386,356 @@ __device__ inline bool xch_finish(double *slot, unsigned long long *raw, T *v, l
     return ok;
 }
 
-constexpr int PIPE_MAX_LANES = 256;
+constexpr int PIPE_COLS = 64;                  // tile columns = lanes of the chain wavefront
+constexpr int PIPE_MAX_THREADS = 7 * PIPE_COLS;  // chain + up to six helper wavefronts
+
+// Operands of one tile's next hyperplane, formed by the helper wavefronts:
+// two slots, structure of arrays over the tile's columns.  The 13 stored
+// fields of a coefficient set are MxCoef without ae (= gama - 1).
+template <typename T>
+struct PipeLds {
+    T cn[2][3][13][PIPE_COLS];
+    T fold[2][3][5][PIPE_COLS];
+    T rhs[2][5][PIPE_COLS];
+    T coed[2][PIPE_COLS];
+    T ghost[2][3][5][PIPE_COLS];
+    T own[2][5][PIPE_COLS];
+    T rv[2][3][PIPE_COLS];
+    T sh[2][5][PIPE_COLS];       // this step's results, for the j and k neighbours in the tile
+};
+
+template <typename T>
+__device__ inline void coef_put(PipeLds<T> &L, int slot, int d, int c, const MxCoef<T> &x)
+{
+    const T v[13] = {x.l1, x.x1, x.x2, x.nx, x.ny, x.nz, x.ct, x.af, x.c2, x.um, x.vm, x.wm, x.hm};
+#pragma unroll
+    for (int f = 0; f < 13; f++) L.cn[slot][d][f][c] = v[f];
+}
+
+template <typename T>
+__device__ inline void coef_get(const PipeLds<T> &L, int slot, int d, int c, T gama, MxCoef<T> &x)
+{
+    x.l1 = L.cn[slot][d][0][c]; x.x1 = L.cn[slot][d][1][c]; x.x2 = L.cn[slot][d][2][c];
+    x.nx = L.cn[slot][d][3][c]; x.ny = L.cn[slot][d][4][c]; x.nz = L.cn[slot][d][5][c];
+    x.ct = L.cn[slot][d][6][c]; x.af = L.cn[slot][d][7][c]; x.c2 = L.cn[slot][d][8][c];
+    x.um = L.cn[slot][d][9][c]; x.vm = L.cn[slot][d][10][c]; x.wm = L.cn[slot][d][11][c];
+    x.hm = L.cn[slot][d][12][c];
+    x.ae = gama - T(1.0);
+}
+
+// One helper task: part p of the operands of node (i,j,k).  p = 0..2: the
+// coefficients of the this-sweep neighbour in direction p, plus a share of
+// the node's other operands (p = 0: R and the diagonal; p = 1: lusgs_u's own
+// dQ; p = 2: the viscous radii; and, for LU, the dQ outside the block in
+// direction p); p = 3..5 (SGS): the complete term of the previous-sweep
+// neighbour in direction p-3.  Loaded into TaskRaw, formed into the LDS slot
+// by task_prep.  The share sits at the same places of x for every p
+// (x[0..4] a 5-vector, x[5] the diagonal, x[6..10] the outside dQ), so that
+// no index depends on p.
+template <typename T>
+struct TaskRaw {
+    NbrRaw<T> nb;
+    T x[11];
+};
+
+template <int KIND, typename T, typename S, typename IX>
+__device__ inline void task_load(int p, const DBlock &b, const Operands<S> &O, const IX &ix, TPtrs<T> A, TPtrs<T> B,
+                                 const DParams &P, int i, int j, int k, TaskRaw<T> &r)
+{
+    constexpr int off = sweep_off<KIND>();
+    const long g = ix(i, j, k);
+    if (p < 3) {
+        nbr_load<T>(b, O, ix, i, j, k, p, off, r.nb);
+        if (p == 0) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) r.x[m] = (T)O.rhs[m][g];
+            r.x[5] = (T)O.coed[g];
+        } else if (p == 1 && KIND == 1) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) r.x[m] = A.c[m][g];
+        } else if (p == 2 && KIND < 2 && P.nvis > 0) {
+            r.x[0] = (T)O.srv[0][ix(i + off, j, k)];
+            r.x[1] = (T)O.srv[1][ix(i, j + off, k)];
+            r.x[2] = (T)O.srv[2][ix(i, j, k + off)];
+        }
+        if (KIND < 2) {
+            int cc = (p == 0 ? i : (p == 1 ? j : k)) + off;
+            if (cc < 1 || cc > (p == 0 ? b.ni : (p == 1 ? b.nj : b.nk))) {
+                long gn = ix(i + (p == 0 ? off : 0), j + (p == 1 ? off : 0), k + (p == 2 ? off : 0));
+#pragma unroll
+                for (int m = 0; m < 5; m++) r.x[6 + m] = A.c[m][gn];
+            }
+        }
+    } else {
+        const int d = p - 3;
+        const TPtrs<T> old = KIND == 2 ? A : B;
+        int cc = (d == 0 ? i : (d == 1 ? j : k)) - off;
+        const int nd = d == 0 ? b.ni : (d == 1 ? b.nj : b.nk);
+        long gn = ix(i - (d == 0 ? off : 0), j - (d == 1 ? off : 0), k - (d == 2 ? off : 0));
+#pragma unroll
+        for (int m = 0; m < 5; m++) r.x[m] = (cc < 1 || cc > nd) ? T(0.0) : old.c[m][gn];
+        nbr_load<T>(b, O, ix, i, j, k, d, -off, r.nb);
+    }
+}
 
 template <int KIND, typename T>
-__global__ void __launch_bounds__(PIPE_MAX_LANES) k_sweep_pipe(DBlock b, DFields F, DParams P, SweepTiles S,
-                                                               T *A, T *B)
+__device__ inline void task_prep(int p, const DParams &P, int i, int j, int k, const DBlock &b, const TaskRaw<T> &r,
+                                 PipeLds<T> &L, int slot, int c)
+{
+    constexpr int off = sweep_off<KIND>();
+    const T gama = (T)P.gama;
+    if (p < 3) {
+        MxCoef<T> x;
+        mxdq_coef<T>(r.nb.prim, r.nb.gy, r.nb.rad, off < 0 ? T(1.0) : T(-1.0), gama, x);
+        coef_put<T>(L, slot, p, c, x);
+        if (p == 0) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) L.rhs[slot][m][c] = r.x[m];
+            L.coed[slot][c] = r.x[5];
+        } else if (p == 1 && KIND == 1) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) L.own[slot][m][c] = r.x[m];
+        } else if (p == 2 && KIND < 2 && P.nvis > 0) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) L.rv[slot][d][c] = r.x[d];
+        }
+        if (KIND < 2) {
+            int cc = (p == 0 ? i : (p == 1 ? j : k)) + off;
+            if (cc < 1 || cc > (p == 0 ? b.ni : (p == 1 ? b.nj : b.nk))) {
+#pragma unroll
+                for (int m = 0; m < 5; m++) L.ghost[slot][p][m][c] = r.x[6 + m];
+            }
+        }
+    } else {
+        MxCoef<T> x;
+        mxdq_coef<T>(r.nb.prim, r.nb.gy, r.nb.rad, -off < 0 ? T(1.0) : T(-1.0), gama, x);
+        T f[5];
+        mxdq_apply<T>(x, r.x, f);
+#pragma unroll
+        for (int m = 0; m < 5; m++) L.fold[slot][p - 3][m][c] = f[m];
+    }
+}
+
+// node_compute with the operands read from the LDS slot as they are needed
+// (the same operations in the same order).
+template <int KIND, typename T>
+__device__ inline void chain_compute(const DParams &P, const PipeLds<T> &L, int slot, int c, T (&nw)[3][5], T *out)
+{
+    const T gama = (T)P.gama;
+    T f[3][5];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        MxCoef<T> x;
+        coef_get<T>(L, slot, d, c, gama, x);
+        mxdq_apply<T>(x, nw[d], f[d]);
+    }
+    const T coed = L.coed[slot][c];
+    if (KIND < 2) {
+        T rhs0[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++) rhs0[m] = T(1.0) * (f[0][m] + f[1][m] + f[2][m]);
+        if (P.nvis > 0) {
+            const T rv0 = L.rv[slot][0][c], rv1 = L.rv[slot][1][c], rv2 = L.rv[slot][2][c];
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                T v = T(0.5) * (rv0 * nw[0][m] + rv1 * nw[1][m] + rv2 * nw[2][m]);
+                rhs0[m] = KIND == 0 ? rhs0[m] + v : rhs0[m] - v;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 5; m++)
+            out[m] = KIND == 0 ? (-L.rhs[slot][m][c] + rhs0[m]) * coed : L.own[slot][m][c] - rhs0[m] * coed;
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            const T fo0 = L.fold[slot][0][m][c], fo1 = L.fold[slot][1][m][c], fo2 = L.fold[slot][2][m][c];
+            T r0;
+            if (KIND == 2)
+                r0 = T(1.0) * (f[0][m] + f[1][m] + f[2][m] - fo0 - fo1 - fo2);
+            else
+                r0 = T(1.0) * (fo0 + fo1 + fo2 - f[0][m] - f[1][m] - f[2][m]);
+            out[m] = (r0 - L.rhs[slot][m][c]) * coed;
+        }
+    }
+}
+
+// Pipelined sweep, all blocks in one launch.  Workgroup w takes tile
+// work[w].y of block work[w].x.  Wavefront 0 is the chain: lane c = (jl,kl)
+// updates its column's node of the current hyperplane from the operands in
+// LDS and the neighbours' new dQ.  Wavefronts 1..H are helpers: helper lane
+// (c, p) forms part p of column c's operands for the next hyperplane (H = 3
+// for LU, 6 for SGS) and, meanwhile, loads those of the hyperplane after.
+// O: the plane-major operands (every block at its own offset); A, B: the
+// plane-major dQ pair; OUT: the field-pool dQ that gs_pr_u (the last sweep of
+// the step) writes for the update.
+template <int KIND, typename T>
+__global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBlock *blocks, const int2 *work,
+                                                                 Operands<T> O, TPtrs<T> A, TPtrs<T> B,
+                                                                 TPtrs<T> OUT, DParams P, int *err, long spin_limit)
 {
     constexpr bool fwd = (KIND == 0 || KIND == 2);
     constexpr int off = sweep_off<KIND>();
-    const int TJ = S.TJ, TK = S.TK, ni = b.ni;
-    const long T5 = F.total;
-    const int tile = (fwd ? S.order_fwd : S.order_bwd)[blockIdx.x];
-    const int J = tile % S.nTJ, K = tile / S.nTJ;
-    const int lane = threadIdx.x, jl = lane % TJ, kl = lane / TJ;
+    constexpr int H = KIND < 2 ? 3 : 6;
+    const int2 wk = work[blockIdx.x];
+    const SweepBlock sb = blocks[wk.x];
+    const DBlock b = sb.b;
+    const SkIndex ix{sb.soff, (b.nj + 2) * (b.nk + 2), b.nj + 2};
+    const int TJ = sb.TJ, TK = sb.TK, nTJ = sb.nTJ, nTK = sb.nTK, ni = b.ni;
+    const int tile = wk.y;
+    const int J = tile % nTJ, K = tile / nTJ;
+    const int wave = threadIdx.x / PIPE_COLS;
+    const int c = threadIdx.x % PIPE_COLS;          // column of this lane
+    const int p = wave - 1;                         // helper part (chain: -1)
+    const int jl = c % TJ, kl = c / TJ;
     const int jlo = 1 + J * TJ, klo = 1 + K * TK;
     const int jn = min(TJ, b.nj - jlo + 1), kn = min(TK, b.nk - klo + 1);
     const bool col = kl < kn && jl < jn;
     const int j = jlo + jl, k = klo + kl;
     const int nsteps = ni + jn + kn - 2;
-    // source of the this-sweep j and k neighbours: 0 this tile, 1 adjacent tile, 2 outside the block
-    const int jsrc = fwd ? (jl > 0 ? 0 : (J > 0 ? 1 : 2)) : (jl < jn - 1 ? 0 : (J < S.nTJ - 1 ? 1 : 2));
-    const int ksrc = fwd ? (kl > 0 ? 0 : (K > 0 ? 1 : 2)) : (kl < kn - 1 ? 0 : (K < S.nTK - 1 ? 1 : 2));
-    // this lane's results are the adjacent tile's neighbours
-    const bool putj = fwd ? (jl == jn - 1 && J < S.nTJ - 1) : (jl == 0 && J > 0);
-    const bool putk = fwd ? (kl == kn - 1 && K < S.nTK - 1) : (kl == 0 && K > 0);
-    const long rowlen = (long)ni * 5;
-    double *xj_in = S.xj + (jsrc == 1 ? ((long)(tile + off) * TK + kl) * rowlen : 0);
-    double *xk_in = S.xk + (ksrc == 1 ? ((long)(tile + off * S.nTJ) * TJ + jl) * rowlen : 0);
-    double *xj_out = S.xj + ((long)tile * TK + kl) * rowlen;
-    double *xk_out = S.xk + ((long)tile * TJ + jl) * rowlen;
-    T *W = KIND == 2 ? B : A;
+    const int i0 = fwd ? 1 - jl - kl : ni + (jn - 1 - jl) + (kn - 1 - kl);
+    auto i_at = [&](int st) { return fwd ? i0 + st : i0 - st; };
+    auto live = [&](int st) { int ii = i_at(st); return col && st < nsteps && ii >= 1 && ii <= ni; };
+    const T gama = (T)P.gama;
 
-    __shared__ T sh[2][5][PIPE_MAX_LANES];
+    __shared__ PipeLds<T> L;
+
+    // chain-wavefront state
+    const int jsrc = fwd ? (jl > 0 ? 0 : (J > 0 ? 1 : 2)) : (jl < jn - 1 ? 0 : (J < nTJ - 1 ? 1 : 2));
+    const int ksrc = fwd ? (kl > 0 ? 0 : (K > 0 ? 1 : 2)) : (kl < kn - 1 ? 0 : (K < nTK - 1 ? 1 : 2));
+    const bool putj = col && (fwd ? (jl == jn - 1 && J < nTJ - 1) : (jl == 0 && J > 0));
+    const bool putk = col && (fwd ? (kl == kn - 1 && K < nTK - 1) : (kl == 0 && K > 0));
+    const long rowlen = (long)ni * 5;
+    double *xj_in = sb.xj + (jsrc == 1 ? ((long)(tile + off) * TK + kl) * rowlen : 0);
+    double *xk_in = sb.xk + (ksrc == 1 ? ((long)(tile + off * nTJ) * TJ + jl) * rowlen : 0);
+    double *xj_out = sb.xj + ((long)tile * TK + kl) * rowlen;
+    double *xk_out = sb.xk + ((long)tile * TJ + jl) * rowlen;
+    const TPtrs<T> W = KIND == 2 ? B : (KIND == 3 ? OUT : A);
     T prev[5] = {T(0), T(0), T(0), T(0), T(0)};
-    NodeIn<KIND, T> in;
     unsigned long long rj[5], rk[5];
     bool bad = false;                 // a bounded wait of this lane expired: stop waiting
-    const int i0 = fwd ? 1 - jl - kl : ni + (jn - 1 - jl) + (kn - 1 - kl);
-    int i = i0;
-    bool act = col && i >= 1 && i <= ni;
-    if (act) {
-        node_inputs<KIND, T>(b, F, P, A, B, i, j, k, in);
+
+    // helper state: the raw operands of the hyperplane after next
+    TaskRaw<T> raw;
+    const bool helper = p >= 0 && p < H;
+    if (helper) {
+        if (live(0)) {
+            task_load<KIND, T>(p, b, O, ix, A, B, P, i_at(0), j, k, raw);
+            task_prep<KIND, T>(p, P, i_at(0), j, k, b, raw, L, 0, c);
+        }
+        if (live(1)) task_load<KIND, T>(p, b, O, ix, A, B, P, i_at(1), j, k, raw);
+    } else if (p < 0 && live(0)) {
+        int i = i_at(0);
         if (jsrc == 1) xch_issue(xj_in + (long)(i - 1) * 5, rj);
         if (ksrc == 1) xch_issue(xk_in + (long)(i - 1) * 5, rk);
     }
+    __syncthreads();
     for (int step = 0; step < nsteps; step++) {
-        const int buf = step & 1;
-        if (act) {
-            T nw[3][5];
-            if (i + off >= 1 && i + off <= ni) {
+        const int slot = step & 1;
+        if (p < 0) {
+            if (live(step)) {
+                const int i = i_at(step);
+                T nw[3][5];
+                if (i + off >= 1 && i + off <= ni) {
 #pragma unroll
-                for (int m = 0; m < 5; m++) nw[0][m] = prev[m];
-            } else {
-                outside<KIND, T>(in, 0, nw[0]);
-            }
-            if (jsrc == 0) {
+                    for (int m = 0; m < 5; m++) nw[0][m] = prev[m];
+                } else {
 #pragma unroll
-                for (int m = 0; m < 5; m++) nw[1][m] = sh[buf ^ 1][m][lane + off];
-            } else if (jsrc == 1) {
-                if (!xch_finish<T>(xj_in + (long)(i - 1) * 5, rj, nw[1], bad ? 0 : S.spin_limit, S.err)) {
-                    if (!bad) atomicExch(S.err, 1);
-                    bad = true;
+                    for (int m = 0; m < 5; m++) nw[0][m] = KIND < 2 ? L.ghost[slot][0][m][c] : T(0.0);
                 }
-            } else {
-                outside<KIND, T>(in, 1, nw[1]);
-            }
-            if (ksrc == 0) {
+                if (jsrc == 0) {
 #pragma unroll
-                for (int m = 0; m < 5; m++) nw[2][m] = sh[buf ^ 1][m][lane + off * TJ];
-            } else if (ksrc == 1) {
-                if (!xch_finish<T>(xk_in + (long)(i - 1) * 5, rk, nw[2], bad ? 0 : S.spin_limit, S.err)) {
-                    if (!bad) atomicExch(S.err, 1);
-                    bad = true;
+                    for (int m = 0; m < 5; m++) nw[1][m] = L.sh[slot ^ 1][m][c + off];
+                } else if (jsrc == 1) {
+                    if (!xch_finish<T>(xj_in + (long)(i - 1) * 5, rj, nw[1], bad ? 0 : spin_limit, err)) {
+                        if (!bad) atomicExch(err, 1);
+                        bad = true;
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 5; m++) nw[1][m] = KIND < 2 ? L.ghost[slot][1][m][c] : T(0.0);
                 }
-            } else {
-                outside<KIND, T>(in, 2, nw[2]);
-            }
-            T out[5];
-            node_compute<KIND, T>(P, in, nw, out);
-            const long g = gidx(b, i, j, k);
-            if (putj) xch_put<T>(xj_out + (long)(i - 1) * 5, out);
-            if (putk) xch_put<T>(xk_out + (long)(i - 1) * 5, out);
+                if (ksrc == 0) {
 #pragma unroll
-            for (int m = 0; m < 5; m++) {
-                sh[buf][m][lane] = out[m];
-                prev[m] = out[m];
-                W[m * T5 + g] = out[m];
+                    for (int m = 0; m < 5; m++) nw[2][m] = L.sh[slot ^ 1][m][c + off * TJ];
+                } else if (ksrc == 1) {
+                    if (!xch_finish<T>(xk_in + (long)(i - 1) * 5, rk, nw[2], bad ? 0 : spin_limit, err)) {
+                        if (!bad) atomicExch(err, 1);
+                        bad = true;
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 5; m++) nw[2][m] = KIND < 2 ? L.ghost[slot][2][m][c] : T(0.0);
+                }
+                T out[5];
+                chain_compute<KIND, T>(P, L, slot, c, nw, out);
+                if (putj) xch_put<T>(xj_out + (long)(i - 1) * 5, out);
+                if (putk) xch_put<T>(xk_out + (long)(i - 1) * 5, out);
+                const long g = KIND == 3 ? gidx(b, i, j, k) : ix(i, j, k);
+#pragma unroll
+                for (int m = 0; m < 5; m++) {
+                    L.sh[slot][m][c] = out[m];
+                    prev[m] = out[m];
+                    W.c[m][g] = out[m];
+                }
             }
+            if (live(step + 1)) {
+                const int i = i_at(step + 1);
+                if (jsrc == 1) xch_issue(xj_in + (long)(i - 1) * 5, rj);
+                if (ksrc == 1) xch_issue(xk_in + (long)(i - 1) * 5, rk);
+            }
+        } else if (helper) {
+            if (live(step + 1)) task_prep<KIND, T>(p, P, i_at(step + 1), j, k, b, raw, L, slot ^ 1, c);
+            if (live(step + 2)) task_load<KIND, T>(p, b, O, ix, A, B, P, i_at(step + 2), j, k, raw);
         }
-        // the next step's operands, formed before the barrier
-        i = fwd ? i0 + step + 1 : i0 - step - 1;
-        act = col && i >= 1 && i <= ni && step + 1 < nsteps;
-        if (act) {
-            node_inputs<KIND, T>(b, F, P, A, B, i, j, k, in);
-            if (jsrc == 1) xch_issue(xj_in + (long)(i - 1) * 5, rj);
-            if (ksrc == 1) xch_issue(xk_in + (long)(i - 1) * 5, rk);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------- plane-major staging
+// Copies field-pool arrays (i fastest) into the plane-major copy: a workgroup
+// takes rows j0..j0+15 of plane k and 64 consecutive hyperplanes s0..s0+63,
+// reads each row's 64 i-values contiguously into LDS and writes each
+// hyperplane's 16 j-values contiguously.  Nodes i,j,k in 0..n+1.
+constexpr int STG_J = 16, STG_S = 64, STG_MAX = 24;
+
+struct StageList {
+    const void *src[STG_MAX];   // field-pool array (double, or float when f32 is set)
+    int dst[STG_MAX];           // plane-major array index
+    unsigned char f32[STG_MAX];
+    unsigned char shell[STG_MAX];  // 1: only nodes on the ghost shell (i, j or k = 0 or n+1)
+    int n;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_skew_stage(DBlock b, SkIndex ix, StageList L, T *pool, long sktot)
+{
+    __shared__ T tile[STG_J][STG_S + 1];
+    const int k = blockIdx.z;
+    const int j0 = blockIdx.y * STG_J, s0 = blockIdx.x * STG_S;
+    const int t = threadIdx.x;
+    for (int a = 0; a < L.n; a++) {
+        // load: row jr = t / 16 handles 4 hyperplanes each
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int jr = t / 16, sc = (t % 16) + 16 * q;
+            int j = j0 + jr, sv = s0 + sc, i = sv - j - k;
+            T v = T(0);
+            if (j <= b.nj + 1 && i >= 0 && i <= b.ni + 1) {
+                long g = gidx(b, i, j, k);
+                v = L.f32[a] ? (T)((const float *)L.src[a])[g] : (T)((const double *)L.src[a])[g];
+            }
+            tile[jr][sc] = v;
+        }
+        __syncthreads();
+        // store: hyperplane sc, rows jr contiguous
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int jr = t % 16, sc = t / 16 + 16 * q;
+            int j = j0 + jr, sv = s0 + sc, i = sv - j - k;
+            if (j <= b.nj + 1 && i >= 0 && i <= b.ni + 1) {
+                bool on_shell = i == 0 || i == b.ni + 1 || j == 0 || j == b.nj + 1 || k == 0 || k == b.nk + 1;
+                if (!L.shell[a] || on_shell) pool[(long)L.dst[a] * sktot + ix(i, j, k)] = tile[jr][sc];
+            }
         }
         __syncthreads();
     }
@@ -378,46 +748,101 @@ int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, 
     int smin = 3, smax = b.ni + b.nj + b.nk;
     long nthr = (long)b.nj * b.nk;
     dim3 g = grid1(nthr, 256), bl(256);
+    Operands<double> O = g3_operands(F);
+    DqPtrs A;
+    for (int m = 0; m < 5; m++) A.c[m] = F.dq + m * F.total;
     switch (kind) {
-    case 0: for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<0>, g, bl, 0, s, b, F, P, p); break;
-    case 1: for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<1>, g, bl, 0, s, b, F, P, p); break;
-    case 2: for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<2>, g, bl, 0, s, b, F, P, p); break;
-    default: for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<3>, g, bl, 0, s, b, F, P, p); break;
+    case 0: for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<0>, g, bl, 0, s, b, O, A, P, p); break;
+    case 1: for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<1>, g, bl, 0, s, b, O, A, P, p); break;
+    case 2: for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_sweep_plane<2>, g, bl, 0, s, b, O, A, P, p); break;
+    default: for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_sweep_plane<3>, g, bl, 0, s, b, O, A, P, p); break;
     }
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-long sweep_exchange_words(const SweepTiles &T, int ni)
+long sweep_exchange_words(int TJ, int TK, int nTJ, int nTK, int ni)
 {
-    return (long)T.nTJ * T.nTK * (T.TJ + T.TK) * ni * 5;
+    return (long)nTJ * nTK * (TJ + TK) * ni * 5;
 }
 
-int launch_sweep_tiled(int kind, const DBlock &b, const DFields &F, const DParams &P, const SweepTiles &T,
-                       int prec, void *A, void *B, hipStream_t s)
+long skew_size(const DBlock &b)
 {
-    int ntiles = T.nTJ * T.nTK;
-    if (T.TJ * T.TK > PIPE_MAX_LANES || T.TJ < 1 || T.TK < 1) return -2;
-    long nx = sweep_exchange_words(T, b.ni);
-    hipLaunchKernelGGL(k_fill_u64, grid1(nx, 256), dim3(256), 0, s, (unsigned long long *)T.xj, nx, XCH_EMPTY);
-    dim3 g(ntiles), bl(T.TJ * T.TK);
-#define WCNS_PIPE(K, TY) hipLaunchKernelGGL((k_sweep_pipe<K, TY>), g, bl, 0, s, b, F, P, T, (TY *)A, (TY *)B)
-    if (prec == 1) {
-        switch (kind) {
-        case 0: WCNS_PIPE(0, float); break;
-        case 1: WCNS_PIPE(1, float); break;
-        case 2: WCNS_PIPE(2, float); break;
-        default: WCNS_PIPE(3, float); break;
-        }
+    return (long)(b.ni + b.nj + b.nk + 4) * (b.nj + 2) * (b.nk + 2);
+}
+
+static SkIndex skew_index(const DBlock &b, long soff)
+{
+    return SkIndex{soff, (b.nj + 2) * (b.nk + 2), b.nj + 2};
+}
+
+int launch_skew_stage(int what, const DBlock &b, long soff, const DFields &F, int prec, void *pool, long sktot,
+                      hipStream_t s)
+{
+    StageList L{};
+    auto add = [&](const void *src, int dst, int f32, int shell) {
+        L.src[L.n] = src; L.dst[L.n] = dst; L.f32[L.n] = (unsigned char)f32; L.shell[L.n] = (unsigned char)shell;
+        L.n++;
+    };
+    if (what == SKEW_METRICS) {
+        const double *mt[9] = {F.kcx, F.kcy, F.kcz, F.etx, F.ety, F.etz, F.ctx, F.cty, F.ctz};
+        for (int c = 0; c < 9; c++) add(mt[c], SK_M + c, 0, 0);
     } else {
-        switch (kind) {
-        case 0: WCNS_PIPE(0, double); break;
-        case 1: WCNS_PIPE(1, double); break;
-        case 2: WCNS_PIPE(2, double); break;
-        default: WCNS_PIPE(3, double); break;
+        const double *pr[5] = {F.r, F.u, F.v, F.w, F.p};
+        for (int m = 0; m < 5; m++) add(pr[m], SK_P + m, 0, 0);
+        add(F.sra, SK_SR + 0, 0, 0); add(F.srb, SK_SR + 1, 0, 0); add(F.src, SK_SR + 2, 0, 0);
+        if (F.srva) { add(F.srva, SK_SRV + 0, 0, 0); add(F.srvb, SK_SRV + 1, 0, 0); add(F.srvc, SK_SRV + 2, 0, 0); }
+        for (int m = 0; m < 5; m++) add(F.rhs + m * F.total, SK_RHS + m, 0, 0);
+        add(F.coed, SK_COED, 0, 0);
+        // dQ outside the block (read by the LU sweeps) comes from the field pool's ghost layer
+        for (int m = 0; m < 5; m++) {
+            if (prec == 1) add(F.dq32 + m * F.total, SK_A + m, 1, 1);
+            else add(F.dq + m * F.total, SK_A + m, 0, 1);
         }
     }
-#undef WCNS_PIPE
+    int ns = b.ni + b.nj + b.nk + 4;
+    dim3 g((ns + STG_S - 1) / STG_S, (b.nj + 2 + STG_J - 1) / STG_J, b.nk + 2);
+    SkIndex ix = skew_index(b, soff);
+    if (prec == 1) hipLaunchKernelGGL(k_skew_stage<float>, g, dim3(256), 0, s, b, ix, L, (float *)pool, sktot);
+    else hipLaunchKernelGGL(k_skew_stage<double>, g, dim3(256), 0, s, b, ix, L, (double *)pool, sktot);
     return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <typename T>
+static int launch_pipe(int kind, const SweepPlan &S, const DFields &F, const DParams &P, T *pool, long sktot,
+                       T *g3dq, hipStream_t s)
+{
+    Operands<T> O;
+    for (int m = 0; m < 5; m++) { O.p[m] = pool + (SK_P + m) * sktot; O.rhs[m] = pool + (SK_RHS + m) * sktot; }
+    for (int d = 0; d < 3; d++) {
+        for (int c = 0; c < 3; c++) O.m[d][c] = pool + (SK_M + 3 * d + c) * sktot;
+        O.sr[d] = pool + (SK_SR + d) * sktot;
+        O.srv[d] = pool + (SK_SRV + d) * sktot;
+    }
+    O.coed = pool + SK_COED * sktot;
+    TPtrs<T> A, B, OUT;
+    for (int m = 0; m < 5; m++) {
+        A.c[m] = pool + (SK_A + m) * sktot;
+        B.c[m] = pool + (SK_B + m) * sktot;
+        OUT.c[m] = g3dq + m * F.total;
+    }
+    const int2 *work = (kind == 0 || kind == 2) ? S.work_fwd : S.work_bwd;
+    dim3 g(S.nwork), bl(PIPE_COLS * (kind < 2 ? 4 : 7));
+    switch (kind) {
+    case 0: hipLaunchKernelGGL((k_sweep_pipe<0, T>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
+    case 1: hipLaunchKernelGGL((k_sweep_pipe<1, T>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
+    case 2: hipLaunchKernelGGL((k_sweep_pipe<2, T>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
+    default: hipLaunchKernelGGL((k_sweep_pipe<3, T>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_sweep_batched(int kind, const SweepPlan &S, const DFields &F, const DParams &P, int prec, void *pool,
+                         long sktot, hipStream_t s)
+{
+    if (S.lanes > PIPE_COLS || S.lanes < 1 || S.nwork < 1) return -2;
+    hipLaunchKernelGGL(k_fill_u64, grid1(S.nxch, 256), dim3(256), 0, s, S.xch, S.nxch, XCH_EMPTY);
+    if (prec == 1) return launch_pipe<float>(kind, S, F, P, (float *)pool, sktot, F.dq32, s);
+    return launch_pipe<double>(kind, S, F, P, (double *)pool, sktot, F.dq, s);
 }
 
 }  // namespace wcns

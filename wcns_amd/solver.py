@@ -31,8 +31,9 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_l_h_s", "wcns_communicate_pv", "wcns_residual", "wcns_step", "wcns_get_counter", "wcns_sync",
            "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get", "wcns_force"]
 
-TIMING_SLOTS = ["bc", "init", "timestep", "rhs_vis", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
-                "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual"]
+TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
+                "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
+                "sweep_stage", "rhs_vis_i", "rhs_vis_j", "rhs_vis_k"]
 
 
 class WcnsError(RuntimeError):
