@@ -16,13 +16,14 @@ class OParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("moo", "reynolds", "attack", "sideslip", "tref", "twall",
                                           "cfl", "timedt_rate", "efix", "csrv", "gama", "prl", "prt")] + \
                [(n, C.c_int) for n in ("nvis", "nlamtur", "nscheme", "nflux", "ntmst")] + \
-               [(n, C.c_double) for n in ("sref", "lfref", "xref", "yref", "zref")] + [("nwholefield", C.c_int)]
+               [(n, C.c_double) for n in ("sref", "lfref", "xref", "yref", "zref")] + [("nwholefield", C.c_int)] + \
+               [(n, C.c_double) for n in ("timedt_turb", "kmaxlim", "xk", "xb")]
 
 
 FIELDS = {"r": 0, "u": 1, "v": 2, "w": 3, "p": 4, "t": 5, "c": 6, "dtdt": 7, "sra": 8, "srb": 9,
           "src": 10, "srva": 11, "srvb": 12, "srvc": 13, "visl": 14, "vist": 15, "vol": 16,
           "kcx": 17, "kcy": 18, "kcz": 19, "etx": 20, "ety": 21, "etz": 22, "ctx": 23, "cty": 24,
-          "ctz": 25, "dq": 100, "q": 101, "rhs": 102}
+          "ctz": 25, "qke": 26, "dist": 27, "dq": 100, "q": 101, "rhs": 102}
 
 _lib = None
 
@@ -65,6 +66,8 @@ def lib():
         L.o_wcns_line.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dptr,
                                   dptr, dptr]
         L.o_force.argtypes = [C.c_void_p, dptr]
+        L.o_turbulence.argtypes = [C.c_void_p, C.c_int]
+        L.o_wall_distance.argtypes = [C.c_void_p]
         L.o_mxdq_std.argtypes = [dptr, dptr, dptr, dptr, C.c_double, C.c_int, C.c_double]
         _lib = L
     return _lib
@@ -211,6 +214,12 @@ class OracleCase:
 
     def counter(self, which):
         return lib().o_counter(self.h, which)
+
+    def turbulence(self, nb):
+        """One SA block step on block nb (turblence.f90:15-103) on the fields as they are."""
+        e = lib().o_turbulence(self.h, nb)
+        if e:
+            raise RuntimeError(f"oracle turbulence failed: {e}")
 
     def force(self):
         """cfx, cfy, cfz, cmx, cmy, cmz, cd, cl, xcp (PostProcess.f90:1415-1503)."""

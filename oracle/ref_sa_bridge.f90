@@ -1,0 +1,124 @@
+! ref_sa_bridge.f90 -- TEST INFRASTRUCTURE ONLY, never part of the library.
+!
+! Loads one block handed over from C into the reference's module variables
+! (global_variables, Globalvar.f90:284-423, with the bounds of
+! PreProcess.f90:744-807) and runs the reference's own per-block
+! Spalart-Allmaras step, turbulence(nb) (turblence.f90:15-49), with
+! nameturb = 'SA', ncmpcor = 0, ndes = 0, ntrans = 0, nwtmax = 0, method = 1.
+! oracle/Makefile links it with the reference's turblence.f90, Sub1.f90,
+! Sub2.f90, Convect.f90, BL.f90 and Globalvar.f90, compiled from source in the
+! reference's serial configuration (PARALLEL undefined), into _ref/libref_sa.so.
+! The tests compare the C restatement of the same step against it.
+!
+! Every field comes in the C oracle's frame: (ni+6)*(nj+6)*(nk+6) doubles,
+! i fastest, index 0 = node -2.
+module ref_sa_store
+  implicit none
+  real, target, allocatable, save :: s_r(:,:,:), s_u(:,:,:), s_v(:,:,:), s_w(:,:,:)
+  real, target, allocatable, save :: s_qke(:,:,:,:), s_vist(:,:,:), s_visl(:,:,:)
+  real, target, allocatable, save :: s_vol(:,:,:), s_dtdt(:,:,:), s_dist(:,:,:)
+  real, target, allocatable, save :: s_kcx(:,:,:), s_kcy(:,:,:), s_kcz(:,:,:)
+  real, target, allocatable, save :: s_etx(:,:,:), s_ety(:,:,:), s_etz(:,:,:)
+  real, target, allocatable, save :: s_ctx(:,:,:), s_cty(:,:,:), s_ctz(:,:,:)
+  real, target, allocatable, save :: s_p(:,:,:), s_kct(:,:,:), s_ett(:,:,:), s_ctt(:,:,:)
+contains
+  subroutine release()
+    if (allocated(s_r)) deallocate(s_r, s_u, s_v, s_w, s_qke, s_vist, s_visl, s_vol, &
+         s_dtdt, s_dist, s_kcx, s_kcy, s_kcz, s_etx, s_ety, s_etz, s_ctx, s_cty, s_ctz, &
+         s_p, s_kct, s_ett, s_ctt)
+  end subroutine release
+
+  ! copy nodes lo..n+hi of a C-frame field into dst(lo:ni+hi, lo:nj+hi, lo:nk+hi)
+  subroutine load(dst, src, ni, nj, nk, lo, hi)
+    integer, intent(in) :: ni, nj, nk, lo, hi
+    real, intent(in) :: src(-2:ni+3, -2:nj+3, -2:nk+3)
+    real, intent(out) :: dst(lo:ni+hi, lo:nj+hi, lo:nk+hi)
+    dst = src(lo:ni+hi, lo:nj+hi, lo:nk+hi)
+  end subroutine load
+end module ref_sa_store
+
+! dims = (ni, nj, nk); scal = (reynolds, muoo, kmaxlim, timedt_turb, xk, xb, efix)
+! fld = pointers to r, u, v, w, qke, vist, visl, vol, dtdt, dist,
+!       kcx, kcy, kcz, etx, ety, etz, ctx, cty, ctz, p (C frame); the grid is
+!       fixed, so the time metrics kct, ett, ctt are zero
+! qke_out, vist_out: C frame, written on the nodes the reference stores
+! (qke, vist: -2..n+3).  Returns 0.
+integer(c_int) function ref_sa_step(dims, scal, fld, qke_out, vist_out) bind(C, name="ref_sa_step")
+  use iso_c_binding
+  use global_variables, only: g_ni => ni, g_nj => nj, g_nk => nk, nmax, method, nlamtur, &
+       nameturb, ncmpcor, ndes, ntrans, nwtmax, reynolds, re, muoo, kmaxlim, timedt_turb, &
+       xk, xb, efix, r, u, v, w, qke, vist, visl, vol, dtdt, ds_turbulence, &
+       kcx, kcy, kcz, etx, ety, etz, ctx, cty, ctz, p, kct, ett, ctt
+  use ref_sa_store
+  implicit none
+  integer(c_int), intent(in) :: dims(3)
+  real(c_double), intent(in) :: scal(7)
+  type(c_ptr), intent(in) :: fld(20)
+  real(c_double), intent(out) :: qke_out(-2:dims(1)+3, -2:dims(2)+3, -2:dims(3)+3)
+  real(c_double), intent(out) :: vist_out(-2:dims(1)+3, -2:dims(2)+3, -2:dims(3)+3)
+  real(c_double), pointer :: f(:,:,:)
+  integer :: ni, nj, nk
+  external :: turbulence
+
+  ni = dims(1); nj = dims(2); nk = dims(3)
+  call release()
+  allocate(s_r(-2:ni+3, -2:nj+3, -2:nk+3), s_u(-2:ni+3, -2:nj+3, -2:nk+3))
+  allocate(s_v(-2:ni+3, -2:nj+3, -2:nk+3), s_w(-2:ni+3, -2:nj+3, -2:nk+3))
+  allocate(s_qke(-2:ni+3, -2:nj+3, -2:nk+3, 1), s_vist(-2:ni+3, -2:nj+3, -2:nk+3))
+  allocate(s_visl(-1:ni+1, -1:nj+1, -1:nk+1), s_vol(-1:ni+1, -1:nj+1, -1:nk+1))
+  allocate(s_dtdt(1:ni, 1:nj, 1:nk), s_dist(1:ni, 1:nj, 1:nk))
+  allocate(s_kcx(-1:ni+1, -1:nj+1, -1:nk+1), s_kcy(-1:ni+1, -1:nj+1, -1:nk+1))
+  allocate(s_kcz(-1:ni+1, -1:nj+1, -1:nk+1), s_etx(-1:ni+1, -1:nj+1, -1:nk+1))
+  allocate(s_ety(-1:ni+1, -1:nj+1, -1:nk+1), s_etz(-1:ni+1, -1:nj+1, -1:nk+1))
+  allocate(s_ctx(-1:ni+1, -1:nj+1, -1:nk+1), s_cty(-1:ni+1, -1:nj+1, -1:nk+1))
+  allocate(s_ctz(-1:ni+1, -1:nj+1, -1:nk+1), s_p(-2:ni+3, -2:nj+3, -2:nk+3))
+  allocate(s_kct(-1:ni+1, -1:nj+1, -1:nk+1), s_ett(-1:ni+1, -1:nj+1, -1:nk+1))
+  allocate(s_ctt(-1:ni+1, -1:nj+1, -1:nk+1))
+  s_kct = 0.0; s_ett = 0.0; s_ctt = 0.0
+
+  call c_f_pointer(fld(1), f, [ni + 6, nj + 6, nk + 6]); call load(s_r, f, ni, nj, nk, -2, 3)
+  call c_f_pointer(fld(2), f, [ni + 6, nj + 6, nk + 6]); call load(s_u, f, ni, nj, nk, -2, 3)
+  call c_f_pointer(fld(3), f, [ni + 6, nj + 6, nk + 6]); call load(s_v, f, ni, nj, nk, -2, 3)
+  call c_f_pointer(fld(4), f, [ni + 6, nj + 6, nk + 6]); call load(s_w, f, ni, nj, nk, -2, 3)
+  call c_f_pointer(fld(5), f, [ni + 6, nj + 6, nk + 6]); call load(s_qke(:,:,:,1), f, ni, nj, nk, -2, 3)
+  call c_f_pointer(fld(6), f, [ni + 6, nj + 6, nk + 6]); call load(s_vist, f, ni, nj, nk, -2, 3)
+  call c_f_pointer(fld(7), f, [ni + 6, nj + 6, nk + 6]); call load(s_visl, f, ni, nj, nk, -1, 1)
+  call c_f_pointer(fld(8), f, [ni + 6, nj + 6, nk + 6]); call load(s_vol, f, ni, nj, nk, -1, 1)
+  call c_f_pointer(fld(9), f, [ni + 6, nj + 6, nk + 6]); call load(s_dtdt, f, ni, nj, nk, 1, 0)
+  call c_f_pointer(fld(10), f, [ni + 6, nj + 6, nk + 6]); call load(s_dist, f, ni, nj, nk, 1, 0)
+  call c_f_pointer(fld(11), f, [ni + 6, nj + 6, nk + 6]); call load(s_kcx, f, ni, nj, nk, -1, 1)
+  call c_f_pointer(fld(12), f, [ni + 6, nj + 6, nk + 6]); call load(s_kcy, f, ni, nj, nk, -1, 1)
+  call c_f_pointer(fld(13), f, [ni + 6, nj + 6, nk + 6]); call load(s_kcz, f, ni, nj, nk, -1, 1)
+  call c_f_pointer(fld(14), f, [ni + 6, nj + 6, nk + 6]); call load(s_etx, f, ni, nj, nk, -1, 1)
+  call c_f_pointer(fld(15), f, [ni + 6, nj + 6, nk + 6]); call load(s_ety, f, ni, nj, nk, -1, 1)
+  call c_f_pointer(fld(16), f, [ni + 6, nj + 6, nk + 6]); call load(s_etz, f, ni, nj, nk, -1, 1)
+  call c_f_pointer(fld(17), f, [ni + 6, nj + 6, nk + 6]); call load(s_ctx, f, ni, nj, nk, -1, 1)
+  call c_f_pointer(fld(18), f, [ni + 6, nj + 6, nk + 6]); call load(s_cty, f, ni, nj, nk, -1, 1)
+  call c_f_pointer(fld(19), f, [ni + 6, nj + 6, nk + 6]); call load(s_ctz, f, ni, nj, nk, -1, 1)
+  call c_f_pointer(fld(20), f, [ni + 6, nj + 6, nk + 6]); call load(s_p, f, ni, nj, nk, -2, 3)
+
+  ! the module state recast_grid / recast_field (Sub2.f90:324-402) would set
+  g_ni = ni; g_nj = nj; g_nk = nk
+  nmax = max(ni, nj, nk)
+  r => s_r; u => s_u; v => s_v; w => s_w
+  qke => s_qke; vist => s_vist; visl => s_visl
+  vol => s_vol; dtdt => s_dtdt; ds_turbulence => s_dist
+  kcx => s_kcx; kcy => s_kcy; kcz => s_kcz
+  etx => s_etx; ety => s_ety; etz => s_etz
+  ctx => s_ctx; cty => s_cty; ctz => s_ctz
+  p => s_p; kct => s_kct; ett => s_ett; ctt => s_ctt
+
+  ! run options of the SA path (param.dat namelists; re as init_inflow sets it,
+  ! PreProcess.f90:1106)
+  method = 1; nlamtur = 1; nameturb = 'SA'
+  ncmpcor = 0; ndes = 0; ntrans = 0; nwtmax = 0
+  reynolds = scal(1); re = 1.0 / reynolds
+  muoo = scal(2); kmaxlim = scal(3); timedt_turb = scal(4)
+  xk = scal(5); xb = scal(6); efix = scal(7)
+
+  call turbulence(1)
+
+  qke_out = s_qke(:,:,:,1)
+  vist_out = s_vist
+  ref_sa_step = 0
+end function ref_sa_step

@@ -8,7 +8,7 @@
  *
  * This file is the checker: the GPU library never links or calls it.
  */
-#include "wcns_oracle.h"
+#include "wcns_oracle_priv.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -49,6 +49,7 @@ typedef struct {
     double *q, *dq, *rhs_nb;
     double *kcx, *kcy, *kcz, *etx, *ety, *etz, *ctx, *cty, *ctz, *vol;
     double *sra, *srb, *src, *srva, *srvb, *srvc, *dtdt, *visl, *vist;
+    double *qke, *dist;                    /* SA working variable (G3), wall distance (G3 shape, 1..n used) */
     int nregions;
     oregion reg[MAXREG];
     int bcindex[MAXREG];
@@ -60,6 +61,7 @@ typedef struct { int n, cap; opnt *p; } ogroup;
 struct o_case {
     o_params prm;
     double roo, uoo, voo, woo, poo, coo, too, eoo, visc, re, attack_r, sideslip_r;
+    double muoo;                           /* SA freestream value, turblence.f90:3166-3181 */
     int nblocks, cap;
     oblock *blk;
     int ngroups;          /* groups with more than two points (multi-points) */
@@ -624,7 +626,7 @@ static void block_free(oblock *b)
     double **fs[] = {&b->x, &b->y, &b->z, &b->r, &b->u, &b->v, &b->w, &b->p, &b->t, &b->c,
                      &b->q, &b->dq, &b->rhs_nb, &b->kcx, &b->kcy, &b->kcz, &b->etx, &b->ety,
                      &b->etz, &b->ctx, &b->cty, &b->ctz, &b->vol, &b->sra, &b->srb, &b->src,
-                     &b->srva, &b->srvb, &b->srvc, &b->dtdt, &b->visl, &b->vist};
+                     &b->srva, &b->srvb, &b->srvc, &b->dtdt, &b->visl, &b->vist, &b->qke, &b->dist};
     for (size_t n = 0; n < sizeof fs / sizeof fs[0]; n++) { free(*fs[n]); *fs[n] = NULL; }
     for (int r = 0; r < b->nregions; r++) {
         free(b->reg[r].image); free(b->reg[r].jmage); free(b->reg[r].kmage);
@@ -661,7 +663,8 @@ int o_block_add(o_case *c, int ni, int nj, int nk, const double *x, const double
     memcpy(b->z, z, sizeof(double) * n);
     double **g3f[] = {&b->r, &b->u, &b->v, &b->w, &b->p, &b->t, &b->c, &b->kcx, &b->kcy, &b->kcz,
                       &b->etx, &b->ety, &b->etz, &b->ctx, &b->cty, &b->ctz, &b->vol, &b->sra,
-                      &b->srb, &b->src, &b->srva, &b->srvb, &b->srvc, &b->dtdt, &b->visl, &b->vist};
+                      &b->srb, &b->src, &b->srva, &b->srvb, &b->srvc, &b->dtdt, &b->visl, &b->vist,
+                      &b->qke, &b->dist};
     for (size_t f = 0; f < sizeof g3f / sizeof g3f[0]; f++) *g3f[f] = (double *)calloc(b->g3, sizeof(double));
     b->q = (double *)calloc(5 * b->g3, sizeof(double));
     b->dq = (double *)calloc(5 * b->g3, sizeof(double));
@@ -960,6 +963,67 @@ static void init_inflow(o_case *c)
     c->eoo = c->poo / (gama - 1.0) + 0.5 * c->roo * (c->uoo * c->uoo + c->voo * c->voo + c->woo * c->woo);
     c->visc = 110.4 / c->prm.tref;
     c->re = 1.0 / c->prm.reynolds;
+    c->muoo = 1.0e-1;                      /* turblence.f90:3166-3181 inflow_turbulence */
+}
+
+/* turblence.f90:2710-2734 init_SA: nu~ = muoo, mu_t = muoo fv1(muoo) on -1..n+1 */
+static void init_sa(o_case *c)
+{
+    const double cv13 = pow(7.1, 3.0);
+    double kafan = c->muoo;
+    double kf3 = pow(kafan, 3.0);
+    double fv1 = kf3 / (kf3 + cv13);
+    double vt = c->muoo * fv1;
+    for (int nb = 0; nb < c->nblocks; nb++) {
+        oblock *b = &c->blk[nb];
+        for (int k = -1; k <= b->nk + 1; k++) for (int j = -1; j <= b->nj + 1; j++) for (int i = -1; i <= b->ni + 1; i++) {
+            long q = G(b, i, j, k);
+            b->qke[q] = c->muoo;
+            b->vist[q] = vt;
+        }
+    }
+}
+
+/* parallel.f90:4313-4468 calc_wall_dists_parallel: squared distance to the
+ * nearest node of any no-slip wall region (bctype 2, 20, 21), then sqrt + small */
+int o_wall_distance(o_case *c)
+{
+    for (int nb = 0; nb < c->nblocks; nb++) {
+        oblock *b = &c->blk[nb];
+        for (int k = 1; k <= b->nk; k++) for (int j = 1; j <= b->nj; j++) for (int i = 1; i <= b->ni; i++)
+            b->dist[G(b, i, j, k)] = 1.0e30;
+    }
+    for (int wb = 0; wb < c->nblocks; wb++) {
+        oblock *w = &c->blk[wb];
+        for (int r = 0; r < w->nregions; r++) {
+            oregion *g = &w->reg[r];
+            if (!(g->bctype == 2 || g->bctype == 20 || g->bctype == 21)) continue;
+            for (int kw = g->s_st[2]; kw <= g->s_ed[2]; kw++)
+                for (int jw = g->s_st[1]; jw <= g->s_ed[1]; jw++)
+                    for (int iw = g->s_st[0]; iw <= g->s_ed[0]; iw++) {
+                        long qw = (iw - 1) + (long)w->ni * ((jw - 1) + (long)w->nj * (kw - 1));
+                        double xw = w->x[qw], yw = w->y[qw], zw = w->z[qw];
+                        for (int nb = 0; nb < c->nblocks; nb++) {
+                            oblock *b = &c->blk[nb];
+                            for (int k = 1; k <= b->nk; k++) for (int j = 1; j <= b->nj; j++) for (int i = 1; i <= b->ni; i++) {
+                                long q = (i - 1) + (long)b->ni * ((j - 1) + (long)b->nj * (k - 1));
+                                double dx = b->x[q] - xw, dy = b->y[q] - yw, dz = b->z[q] - zw;
+                                double dist = dx * dx + dy * dy + dz * dz;
+                                long qg = G(b, i, j, k);
+                                if (dist < b->dist[qg]) b->dist[qg] = dist;
+                            }
+                        }
+                    }
+        }
+    }
+    for (int nb = 0; nb < c->nblocks; nb++) {
+        oblock *b = &c->blk[nb];
+        for (int k = 1; k <= b->nk; k++) for (int j = 1; j <= b->nj; j++) for (int i = 1; i <= b->ni; i++) {
+            long q = G(b, i, j, k);
+            b->dist[q] = sqrt(b->dist[q]) + SMALL;
+        }
+    }
+    return 0;
 }
 
 int o_setup(o_case *c)
@@ -985,6 +1049,12 @@ int o_setup(o_case *c)
     }
     build_groups(c);
     init_inflow(c);
+    if (c->prm.nlamtur >= 0) {
+        /* turblence.f90:1735-1785 init_turbulence: wall distance, uniform SA field */
+        if (c->prm.nlamtur != 1 || c->prm.nvis != 1) return -12;
+        o_wall_distance(c);
+        init_sa(c);
+    }
     return 0;
 }
 
@@ -1024,6 +1094,7 @@ static double *field_ptr(const oblock *b, int which)
     case 17: return b->kcx; case 18: return b->kcy; case 19: return b->kcz;
     case 20: return b->etx; case 21: return b->ety; case 22: return b->etz;
     case 23: return b->ctx; case 24: return b->cty; case 25: return b->ctz;
+    case 26: return b->qke; case 27: return b->dist;
     default: return NULL;
     }
 }
@@ -1095,6 +1166,8 @@ static void getnxyz(const oblock *b, int i, int j, int k, int is, int js, int ks
     *nx = x / s; *ny = y / s; *nz = z / s;
 }
 
+/* the SA variable is carried through the boundary conditions (nvis = 1, nlamtur >= 0) */
+#define TURB(c) ((c)->prm.nvis == 1 && (c)->prm.nlamtur >= 0)
 #define SETP(b,is,js,ks,rv,uv,vv,wv,pv) do { long _g = G(b,is,js,ks); (b)->r[_g]=(rv); (b)->u[_g]=(uv); (b)->v[_g]=(vv); (b)->w[_g]=(wv); (b)->p[_g]=(pv); } while (0)
 
 /* Sub2.f90:206-322 boundary_n1_vir3 (cic1 = 0; turbulence copies when nvis==1 & nlamtur>=0) */
@@ -1107,6 +1180,15 @@ static void bc_interface(o_case *c, oblock *b, oregion *g)
                 long o = rbox(g, i, j, k);
                 int it = g->image[o] - g->t_lr3d[0], jt = g->jmage[o] - g->t_lr3d[1], kt = g->kmage[o] - g->t_lr3d[2];
                 int is = i + g->s_lr3d[0], js = j + g->s_lr3d[1], ks = k + g->s_lr3d[2];
+                if (TURB(c)) {
+                    /* Sub2.f90:266-280: first ghost from the partner, face node = mean of the two neighbours */
+                    long gt = G(bt, it, jt, kt), gs = G(b, is, js, ks), g0 = G(b, i, j, k);
+                    long gi = G(b, i - g->s_lr3d[0], j - g->s_lr3d[1], k - g->s_lr3d[2]);
+                    b->vist[gs] = bt->vist[gt];
+                    b->vist[g0] = (b->vist[gs] + b->vist[gi]) * 0.5;
+                    b->qke[gs] = bt->qke[gt];
+                    b->qke[g0] = (b->qke[gs] + b->qke[gi]) * 0.5;
+                }
                 for (int l = 1; l <= 3; l++) {
                     long gt = G(bt, it, jt, kt), gs = G(b, is, js, ks);
                     b->r[gs] = bt->r[gt]; b->u[gs] = bt->u[gt]; b->v[gs] = bt->v[gt];
@@ -1160,6 +1242,20 @@ static void bc_farfield(o_case *c, oblock *b, oregion *g)
                     rb = c->roo; ub = c->uoo; vb = c->voo; wb = c->woo; pb = c->poo;
                 }
                 SETP(b, i, j, k, rb, ub, vb, wb, pb);
+                if (TURB(c)) {
+                    /* sub_tgh.f90:713-787 BC_farfield_turbulence_tgh: inflow takes the
+                     * freestream value, outflow the interior neighbour's, on the face
+                     * node and the first ghost */
+                    long g0 = G(b, i, j, k);
+                    double tem_lr = (double)l3[g->s_nd - 1], ut;
+                    if (g->s_nd == 1) ut = tem_lr * (b->kcx[g0] * b->u[g0] + b->kcy[g0] * b->v[g0] + b->kcz[g0] * b->w[g0]);
+                    else if (g->s_nd == 2) ut = tem_lr * (b->etx[g0] * b->u[g0] + b->ety[g0] * b->v[g0] + b->etz[g0] * b->w[g0]);
+                    else ut = tem_lr * (b->ctx[g0] * b->u[g0] + b->cty[g0] * b->v[g0] + b->ctz[g0] * b->w[g0]);
+                    for (int m = 0; m <= 1; m++) {
+                        long gm = G(b, i + m * l3[0], j + m * l3[1], k + m * l3[2]);
+                        b->qke[gm] = ut < 0.0 ? c->muoo : b->qke[G(b, it, jt, kt)];
+                    }
+                }
                 SETP(b, i + l3[0], j + l3[1], k + l3[2], rb, ub, vb, wb, pb);
                 SETP(b, i + 2 * l3[0], j + 2 * l3[1], k + 2 * l3[2], rb, ub, vb, wb, pb);
                 SETP(b, i + 3 * l3[0], j + 3 * l3[1], k + 3 * l3[2], -rb, ub, vb, wb, pb);
@@ -1172,9 +1268,13 @@ static void bc_fixed(o_case *c, oblock *b, oregion *g)
     for (int i = g->s_st[0]; i <= g->s_ed[0]; i++)
         for (int j = g->s_st[1]; j <= g->s_ed[1]; j++)
             for (int k = g->s_st[2]; k <= g->s_ed[2]; k++)
-                for (int n = 0; n <= 3; n++)
+                for (int n = 0; n <= 3; n++) {
                     SETP(b, i + n * g->s_lr3d[0], j + n * g->s_lr3d[1], k + n * g->s_lr3d[2],
                          c->roo, c->uoo, c->voo, c->woo, c->poo);
+                    /* turblence.f90:2321-2338 BC_farfield_turbulence on the face node and first ghost */
+                    if (TURB(c) && n <= 1)
+                        b->qke[G(b, i + n * g->s_lr3d[0], j + n * g->s_lr3d[1], k + n * g->s_lr3d[2])] = c->muoo;
+                }
 }
 
 /* BC.f90:667-749 boundary_6 */
@@ -1187,13 +1287,17 @@ static void bc_outflow(o_case *c, oblock *b, oregion *g)
                 const int *l3 = g->s_lr3d;
                 int it = i - l3[0], jt = j - l3[1], kt = k - l3[2];
                 double rm = R_(b,it,jt,kt), um = U_(b,it,jt,kt), vm = V_(b,it,jt,kt), wm = W_(b,it,jt,kt), pm = P_(b,it,jt,kt);
-                for (int n = 0; n <= 3; n++)
+                for (int n = 0; n <= 3; n++) {
                     SETP(b, i + n * l3[0], j + n * l3[1], k + n * l3[2], rm, um, vm, wm, pm);
+                    /* turblence.f90:2340-2355 BC_outflow_turbulence on the face node and first ghost */
+                    if (TURB(c) && n <= 1)
+                        b->qke[G(b, i + n * l3[0], j + n * l3[1], k + n * l3[2])] = b->qke[G(b, it, jt, kt)];
+                }
             }
 }
 
-/* BC.f90:1972-2008 dif_average */
-static void dif_average(oblock *b, oregion *g)
+/* BC.f90:1972-2008 dif_average (+ BC_face_dif_turbulence, turblence.f90:2252-2265) */
+static void dif_average(o_case *c, oblock *b, oregion *g)
 {
     for (int i = g->s_st[0]; i <= g->s_ed[0]; i++)
         for (int j = g->s_st[1]; j <= g->s_ed[1]; j++)
@@ -1206,11 +1310,12 @@ static void dif_average(oblock *b, oregion *g)
                 b->v[g0] = 0.5 * (b->v[gs] + b->v[gt]);
                 b->w[g0] = 0.5 * (b->w[gs] + b->w[gt]);
                 b->p[g0] = 0.5 * (b->p[gs] + b->p[gt]);
+                if (TURB(c)) b->qke[g0] = 0.5 * (b->qke[gt] + b->qke[gs]);
             }
 }
 
 /* BC.f90:1412-1486 inviscid_wall */
-static void bc_inviscid_wall(oblock *b, oregion *g)
+static void bc_inviscid_wall(o_case *c, oblock *b, oregion *g)
 {
     const int *l3 = g->s_lr3d;
     for (int k = g->s_st[2]; k <= g->s_ed[2]; k++)
@@ -1227,7 +1332,7 @@ static void bc_inviscid_wall(oblock *b, oregion *g)
                          V_(b,it,jt,kt) - sss1 * ny, W_(b,it,jt,kt) - sss1 * nz, P_(b,it,jt,kt));
                 }
             }
-    dif_average(b, g);
+    dif_average(c, b, g);
 }
 
 /* BC.f90:1244-1409 viscid_wall */
@@ -1241,6 +1346,8 @@ static void bc_viscous_wall(o_case *c, oblock *b, oregion *g)
                 int is = i + l3[0], js = j + l3[1], ks = k + l3[2];
                 int it = i - l3[0], jt = j - l3[1], kt = k - l3[2];
                 SETP(b, is, js, ks, R_(b,it,jt,kt), -U_(b,it,jt,kt), -V_(b,it,jt,kt), -W_(b,it,jt,kt), P_(b,it,jt,kt));
+                /* turblence.f90:2267-2306 BC_wall_turbulence (SA): odd reflection of the first ghost */
+                if (TURB(c)) b->qke[G(b, is, js, ks)] = -b->qke[G(b, it, jt, kt)];
                 double pres, tres;
                 int i2 = it - l3[0], j2 = jt - l3[1], k2 = kt - l3[2];
                 if (c->prm.twall > 0.0) {
@@ -1255,6 +1362,8 @@ static void bc_viscous_wall(o_case *c, oblock *b, oregion *g)
                 }
                 double dres = moocp * pres / tres;
                 SETP(b, i, j, k, dres, 0.0, 0.0, 0.0, pres);
+                /* BC_face_dif_turbulence: face value = mean of first ghost and first interior node */
+                if (TURB(c)) b->qke[G(b, i, j, k)] = 0.5 * (b->qke[G(b, it, jt, kt)] + b->qke[G(b, is, js, ks)]);
                 is += l3[0]; js += l3[1]; ks += l3[2]; it -= l3[0]; jt -= l3[1]; kt -= l3[2];
                 SETP(b, is, js, ks, R_(b,it,jt,kt), -U_(b,it,jt,kt), -V_(b,it,jt,kt), -W_(b,it,jt,kt), P_(b,it,jt,kt));
                 is += l3[0]; js += l3[1]; ks += l3[2]; it -= l3[0]; jt -= l3[1]; kt -= l3[2];
@@ -1263,7 +1372,7 @@ static void bc_viscous_wall(o_case *c, oblock *b, oregion *g)
 }
 
 /* BC.f90:227-457 boundary_3 (method = 1, 3-D branch) */
-static void bc_symmetry(oblock *b, oregion *g)
+static void bc_symmetry(o_case *c, oblock *b, oregion *g)
 {
     const int *l3 = g->s_lr3d;
     int ni = b->ni, nj = b->nj, nk = b->nk;
@@ -1280,10 +1389,11 @@ static void bc_symmetry(oblock *b, oregion *g)
                     double rm = R_(b,it,jt,kt), um = U_(b,it,jt,kt), vm = V_(b,it,jt,kt);
                     double wm = -W_(b,it,jt,kt), pm = P_(b,it,jt,kt);
                     SETP(b, i + l3[0], j + l3[1], k + l3[2], rm, um, vm, wm, pm);
+                    if (TURB(c)) b->qke[G(b, i + l3[0], j + l3[1], k + l3[2])] = b->qke[G(b, it, jt, kt)];
                     SETP(b, i + 2 * l3[0], j + 2 * l3[1], k + 2 * l3[2], rm, um, vm, 0.0, pm);
                     SETP(b, i + 3 * l3[0], j + 3 * l3[1], k + 3 * l3[2], rm, um, vm, 0.0, pm);
                 }
-        dif_average(b, g);
+        dif_average(c, b, g);
         return;
     }
     int is = (g->s_st[0] + g->s_ed[0]) / 2, js = (g->s_st[1] + g->s_ed[1]) / 2, ks = (g->s_st[2] + g->s_ed[2]) / 2;
@@ -1299,8 +1409,11 @@ static void bc_symmetry(oblock *b, oregion *g)
                     double s = 2.0 * (nx * U_(b,it,jt,kt) + ny * V_(b,it,jt,kt) + nz * W_(b,it,jt,kt));
                     SETP(b, i + l * l3[0], j + l * l3[1], k + l * l3[2], R_(b,it,jt,kt),
                          U_(b,it,jt,kt) - s * nx, V_(b,it,jt,kt) - s * ny, W_(b,it,jt,kt) - s * nz, P_(b,it,jt,kt));
+                    /* turblence.f90:2308-2319 BC_symmetry_turbulence on the first ghost */
+                    if (TURB(c) && l == 1)
+                        b->qke[G(b, i + l3[0], j + l3[1], k + l3[2])] = b->qke[G(b, it, jt, kt)];
                 }
-    dif_average(b, g);
+    dif_average(c, b, g);
 }
 
 /* sub_tgh.f90:123-186 boundary_all -> BC.f90:169-210 boundary_sequence */
@@ -1311,8 +1424,8 @@ int o_boundary_all(o_case *c)
         for (int ib = 0; ib < b->nregions; ib++) {
             oregion *g = &b->reg[b->bcindex[ib]];
             switch (g->bctype) {
-            case 2: if (c->prm.nvis == 1) bc_viscous_wall(c, b, g); else bc_inviscid_wall(b, g); break;
-            case 3: bc_symmetry(b, g); break;
+            case 2: if (c->prm.nvis == 1) bc_viscous_wall(c, b, g); else bc_inviscid_wall(c, b, g); break;
+            case 3: bc_symmetry(c, b, g); break;
             case 4: bc_farfield(c, b, g); break;
             case 5: bc_fixed(c, b, g); break;
             case 6: bc_outflow(c, b, g); break;
@@ -1886,9 +1999,7 @@ int o_step(o_case *c, int nsteps, double *rms_hist)
 }
 
 /* views for wcns_oracle_vis.c (field pointers of one block, namelist values) */
-typedef struct { int ni, nj, nk; long g3; double *x, *y, *z; double *r, *u, *v, *w, *p, *t, *c;
-    double *q, *dq, *rhs_nb; double *kcx, *kcy, *kcz, *etx, *ety, *etz, *ctx, *cty, *ctz, *vol;
-    double *sra, *srb, *src, *srva, *srvb, *srvc, *dtdt, *visl, *vist; } oblock_view;
+/* oblock_view: wcns_oracle_priv.h */
 
 int o_view_block(o_case *c, int nb, oblock_view *v)
 {
@@ -1902,8 +2013,11 @@ int o_view_block(o_case *c, int nb, oblock_view *v)
     v->ctx = b->ctx; v->cty = b->cty; v->ctz = b->ctz; v->vol = b->vol;
     v->sra = b->sra; v->srb = b->srb; v->src = b->src; v->srva = b->srva; v->srvb = b->srvb; v->srvc = b->srvc;
     v->dtdt = b->dtdt; v->visl = b->visl; v->vist = b->vist;
+    v->qke = b->qke; v->dist = b->dist;
     return 0;
 }
+
+double o_view_muoo(const o_case *c) { return c->muoo; }
 
 void o_view_params(const o_case *c, o_params *p, double *visc)
 {

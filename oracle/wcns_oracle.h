@@ -28,6 +28,9 @@ typedef struct {
     /* namelist force (param.dat:34-44) */
     double sref, lfref, xref, yref, zref;
     int nwholefield;
+    /* SA model (nlamtur = 1): step timedt_turb, turbulence group kmaxlim,
+     * interplate group xk, xb (NND limiter of the convection term) */
+    double timedt_turb, kmaxlim, xk, xb;
 } o_params;
 
 typedef struct o_case o_case;
@@ -60,6 +63,11 @@ int     o_communicate_pv(o_case *c);
 int     o_residual(o_case *c, double *rms, double *resmax, int loc[5]);
 int     o_step(o_case *c, int nsteps, double *rms_hist);
 long    o_counter(const o_case *c, int which);   /* 0 wcns fallbacks, 1 repairs */
+/* Spalart-Allmaras one-equation model (turblence.f90), per block, as called
+ * from r_h_s_vis before the mean-flow viscous flux (RHS.f90:22) */
+int     o_turbulence(o_case *c, int nb);
+/* calc_wall_dists_parallel (parallel.f90:4313-4468): nearest no-slip wall node */
+int     o_wall_distance(o_case *c);
 /* force + force_dif (PostProcess.f90:1415-1503, 1505-1680): cfx,cfy,cfz,cmx,cmy,cmz,cd,cl,xcp */
 int     o_force(o_case *c, double out[9]);
 

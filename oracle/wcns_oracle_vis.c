@@ -19,14 +19,7 @@
 #define SMALL 1.0e-38
 #define NIJK2ND 4
 
-/* The oracle's private block layout (must match wcns_oracle.c). */
-typedef struct { int ni, nj, nk; long g3; double *x, *y, *z; double *r, *u, *v, *w, *p, *t, *c;
-    double *q, *dq, *rhs_nb; double *kcx, *kcy, *kcz, *etx, *ety, *etz, *ctx, *cty, *ctz, *vol;
-    double *sra, *srb, *src, *srva, *srvb, *srvc, *dtdt, *visl, *vist; } oblock_view;
-
-/* accessors provided by wcns_oracle.c */
-extern int o_view_block(o_case *c, int nb, oblock_view *v);
-extern void o_view_params(const o_case *c, o_params *p, double *visc);
+#include "wcns_oracle_priv.h"
 
 static inline long G(const oblock_view *b, int i, int j, int k)
 {
@@ -170,6 +163,8 @@ int o_rhs_vis(o_case *c)
         oblock_view *b = &B;
         int ni = b->ni, nj = b->nj, nk = b->nk;
         if (ni <= NIJK2ND || nj <= NIJK2ND || nk <= NIJK2ND) return -100;
+        /* RHS.f90:22: the turbulence model advances first, in the same block loop */
+        if (prm.nlamtur >= 0 && o_turbulence(c, nb)) return -101;
         int nmax = ni > nj ? ni : nj; nmax = nmax > nk ? nmax : nk;
         long N = (long)ni * nj * nk;
         double *duvwt = (double *)calloc(12 * N, sizeof(double));        /* duvwt(12,i,j,k) */
@@ -270,7 +265,7 @@ int o_rhs_vis(o_case *c)
     return 0;
 }
 
-/* LHS.f90:243-343 spectvisl (laminar) */
+/* LHS.f90:243-343 spectvisl (laminar); LHS.f90:141-241 spectvist (mu + mu_t) */
 int o_spectvis(o_case *c, int nb)
 {
     o_params prm;
@@ -288,7 +283,7 @@ int o_spectvis(o_case *c, int nb)
         double cx = b->ctx[qq], cy = b->cty[qq], cz = b->ctz[qq];                       \
         double ri = kx * kx + ky * ky + kz * kz, rj = ex * ex + ey * ey + ez * ez;      \
         double rk = cx * cx + cy * cy + cz * cz;                                        \
-        double vis = b->visl[qq];                                                        \
+        double vis = prm.nlamtur >= 0 ? b->visl[qq] + b->vist[qq] : b->visl[qq];          \
         double coef = 2.0 * vis / (prm.reynolds * rm_vol + SMALL);                      \
         double coef1 = prm.csrv * coef;                                                  \
         b->srva[q] = coef1 * ri; b->srvb[q] = coef1 * rj; b->srvc[q] = coef1 * rk;       \

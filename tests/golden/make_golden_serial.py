@@ -1,0 +1,184 @@
+"""Write golden vectors from the reference's serial build, oracle/_ref/libref_sa.so
+(oracle/Makefile: Globalvar, GasModel, flux, WCNSE5, Convect, Sub1, Sub2, BL and
+turblence compiled from source with PARALLEL undefined, plus the bridge
+oracle/ref_sa_bridge.f90).
+
+Routines exercised (reference src/ file:line):
+  flux_Roe         flux.f90:116-244 (with flux, Convect.f90:1245-1272)
+  WCNS_E_5         WCNSE5.f90:2-206      with flux_Roe
+  WCNS_E_5_41      WCNSE5.f90:2087-2759  with flux_Roe
+  turbulence(nb)   turblence.f90:15-103, SA one-equation model, one block step
+                   (sources, diffusion, NND convection, scalar LU-SGS, update,
+                   cal_vist_SA, max_control_turbulence)
+
+The SA inputs are states of a small wall-bounded block made by the C oracle
+(a few pseudo-time steps, then qke and velocity perturbed so that the
+limiters and floors are reached); the expected outputs come from the reference.
+
+Usage:  python tests/golden/make_golden_serial.py   (needs oracle/_ref/libref_sa.so)
+Output: tests/golden/serial_golden.npz
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+LIB = os.path.join(REPO, "oracle", "_ref", "libref_sa.so")
+
+GAMA = 1.4
+EFIX = 0.1
+MUOO = 0.1          # freestream SA working variable (init_inflow_turbulence value in the oracle)
+
+# the bridge's field order (oracle/ref_sa_bridge.f90)
+SA_FIELDS = ["r", "u", "v", "w", "qke", "vist", "visl", "vol", "dtdt", "dist",
+             "kcx", "kcy", "kcz", "etx", "ety", "etz", "ctx", "cty", "ctz", "p"]
+
+
+def load():
+    # lazy binding: routines of reference files not compiled into it stay unresolved
+    lib = C.CDLL(LIB, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    C.c_int.in_dll(lib, "_QMglobal_constEnijk2nd").value = 4
+    C.c_int.in_dll(lib, "_QMglobal_constEnijk2d").value = 4
+    C.c_double.in_dll(lib, "_QMglobal_constEgama").value = GAMA
+    lib.ref_sa_step.restype = C.c_int
+    return lib
+
+
+def dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def ip(v):
+    return C.byref(C.c_int(v))
+
+
+def dv(v):
+    return C.byref(C.c_double(v))
+
+
+def flux_roe(lib, pl, pr, nvec):
+    f = np.zeros(5)
+    lib.flux_roe_(dp(pl.copy()), dp(pr.copy()), ip(5), dv(nvec[0]), dv(nvec[1]), dv(nvec[2]),
+                  dv(nvec[3]), dp(f), dv(EFIX), dv(GAMA))
+    return f
+
+
+def wcns_line_roe(lib, fn, trxyz, qline):
+    """trxyz (ni,5); qline (ni+6, 5) prim over -2..ni+3; returns df (ni,5)."""
+    ni = trxyz.shape[0]
+    tr = np.asfortranarray(trxyz.T.copy())
+    q = np.zeros((6, ni + 6, 2), order="F")
+    q[:5, :, 0] = qline.T
+    q[5, :, :] = GAMA
+    df = np.zeros((5, ni), order="F")
+    roe = C.cast(lib.flux_roe_, C.c_void_p)
+    f = getattr(lib, fn)
+    f.argtypes = None
+    f(dv(0.0), dv(0.0), ip(ni), ip(ni), ip(5), ip(1), ip(2), roe, C.c_void_p(0),
+      dv(EFIX), dp(tr), dp(q), dp(df))
+    return df.T.copy()
+
+
+def sa_step(lib, dims, scal, fields):
+    """One reference SA block step on C-frame fields; returns (qke, vist)."""
+    arrs = [np.ascontiguousarray(fields[n], dtype=np.float64) for n in SA_FIELDS]
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    qo = np.zeros_like(arrs[SA_FIELDS.index("qke")])
+    vo = np.zeros_like(qo)
+    d = (C.c_int * 3)(*dims)
+    s = (C.c_double * 7)(*scal)
+    rc = lib.ref_sa_step(d, s, ptrs, qo.ctypes.data_as(C.c_void_p), vo.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise RuntimeError(f"ref_sa_step returned {rc}")
+    return qo, vo
+
+
+def sa_scal(prm):
+    return (prm.reynolds, MUOO, prm.kmaxlim, prm.timedt_turb, prm.xk, prm.xb, prm.efix)
+
+
+def sa_case(ni, nj, nk, reynolds=1.0e5, moo=0.5):
+    from wcns_amd import cases
+    case = cases.bump_channel_case(ni=ni, nj=nj, nk=nk, dy_wall=0.01, growth=1.2,
+                                   reynolds=reynolds, moo=moo)
+    case.params.nlamtur = 1
+    return case
+
+
+def sa_inputs(case, steps, seed, kick):
+    """Oracle state of block 0 just before its SA step, after `steps` steps, with
+    qke scaled by exp(kick*N(0,1)) and the velocity perturbed by 5 % of kick."""
+    from oracle import oracle as O
+    oc = O.OracleCase(case)
+    if steps:
+        oc.step(steps)
+    for st in ("boundary_all", "initial_q_dq_c_t", "timestep"):
+        oc.stage(st)
+    rng = np.random.default_rng(seed)
+    f = {n: oc.field(0, n).copy() for n in SA_FIELDS}
+    if kick:
+        f["qke"] = f["qke"] * np.exp(kick * rng.standard_normal(f["qke"].shape))
+        for n in ("u", "v", "w"):
+            f[n] = f[n] + 0.05 * kick * rng.standard_normal(f[n].shape)
+    oc.close()
+    return f
+
+
+def main():
+    lib = load()
+    rng = np.random.default_rng(20261204)
+    out = {}
+    rows = []
+    for _ in range(64):
+        pl = np.array([1 + 0.3 * rng.random(), rng.standard_normal(), rng.standard_normal(),
+                       rng.standard_normal(), 0.5 + rng.random()])
+        pr = np.array([1 + 0.3 * rng.random(), rng.standard_normal(), rng.standard_normal(),
+                       rng.standard_normal(), 0.5 + rng.random()])
+        nv = np.array([rng.standard_normal(), rng.standard_normal(), rng.standard_normal(), 0.0])
+        rows.append(np.concatenate([pl, pr, nv, flux_roe(lib, pl, pr, nv)]))
+    out["flux_roe"] = np.array(rows)
+
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mg", os.path.join(HERE, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    lines = []
+    for fn, tag in (("wcns_e_5_", 4), ("wcns_e_5_41_", 41)):
+        for n in (6, 9, 16):
+            for bl, br in ((0, 0), (1, 1), (1, 0)):
+                for rough in (False, True):
+                    q = mg.smooth_line(rng, n, bl, br, rough=rough)
+                    tr = mg.metrics_line(rng, n)
+                    key = f"roe{tag}_n{n}_{bl}{br}{int(rough)}"
+                    out[key + "_q"] = q
+                    out[key + "_tr"] = tr
+                    out[key + "_df"] = wcns_line_roe(lib, fn, tr, q)
+                    lines.append(key)
+    out["roe_cases"] = np.array(lines)
+
+    sa = []
+    for tag, dims, steps, seed, kick in (("sa_a", (12, 10, 8), 2, 1, 0.0),
+                                          ("sa_b", (12, 10, 8), 2, 2, 1.5),
+                                          ("sa_c", (14, 9, 7), 1, 3, 4.0)):
+        case = sa_case(*dims)
+        f = sa_inputs(case, steps, seed, kick)
+        qo, vo = sa_step(lib, dims, sa_scal(case.params), f)
+        for n in SA_FIELDS:
+            out[f"{tag}_in_{n}"] = f[n]
+        out[f"{tag}_dims"] = np.array(dims)
+        out[f"{tag}_scal"] = np.array(sa_scal(case.params))
+        out[f"{tag}_qke"] = qo
+        out[f"{tag}_vist"] = vo
+        sa.append(tag)
+    out["sa_cases"] = np.array(sa)
+    path = os.path.join(HERE, "serial_golden.npz")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path}: {len(lines)} Roe WCNS lines, {len(sa)} SA block steps, {len(out)} arrays")
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -64,3 +64,78 @@ def test_wcns_line_against_reference_fresh_inputs():
                     ref = mg.wcns_line(lib, fn, tr, q)
                     mine, _ = O.wcns_line(scheme, 1, tr, q)
                     np.testing.assert_array_equal(mine, ref)
+
+
+# ---- serial build of the reference: flux_Roe, WCNS with Roe, the SA block step
+SER = np.load(os.path.join(HERE, "golden", "serial_golden.npz"))
+REF_SA = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "libref_sa.so")
+
+
+def _serial_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mgs", os.path.join(HERE, "golden", "make_golden_serial.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_flux_roe_golden():
+    for row in SER["flux_roe"]:
+        pl, pr, nv, f = row[:5], row[5:10], row[10:14], row[14:19]
+        np.testing.assert_array_equal(O.flux_roe(pl, pr, nv), f)
+
+
+@pytest.mark.parametrize("key", [str(k) for k in SER["roe_cases"]])
+def test_wcns_line_roe_golden(key):
+    scheme = 41 if key.startswith("roe41") else 4
+    df, _ = O.wcns_line(scheme, 3, SER[key + "_tr"], SER[key + "_q"])
+    np.testing.assert_array_equal(df, SER[key + "_df"])
+
+
+def _oracle_sa_step(mgs, dims, fields):
+    oc = O.OracleCase(mgs.sa_case(*dims))
+    for n in mgs.SA_FIELDS:
+        oc.set_field(0, n, fields[n])
+    oc.turbulence(0)
+    q, v = oc.field(0, "qke"), oc.field(0, "vist")
+    oc.close()
+    return q, v
+
+
+@pytest.mark.parametrize("tag", [str(k) for k in SER["sa_cases"]])
+def test_sa_block_step_golden(tag):
+    """Sources, diffusion, NND convection, scalar LU-SGS, update, cal_vist_SA and
+    max_control_turbulence of one block, bit for bit against the reference."""
+    mgs = _serial_module()
+    dims = tuple(int(d) for d in SER[f"{tag}_dims"])
+    fields = {n: SER[f"{tag}_in_{n}"] for n in mgs.SA_FIELDS}
+    q, v = _oracle_sa_step(mgs, dims, fields)
+    assert (q != fields["qke"]).sum() > 0
+    np.testing.assert_array_equal(q, SER[f"{tag}_qke"])
+    np.testing.assert_array_equal(v, SER[f"{tag}_vist"])
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SA), reason="serial reference library not built here")
+@pytest.mark.parametrize("seed,kick,dims,steps", [(11, 0.5, (16, 12, 9), 3), (12, 3.0, (13, 11, 10), 1),
+                                                  (13, 6.0, (12, 10, 8), 2)])
+def test_sa_block_step_against_reference_fresh_inputs(seed, kick, dims, steps):
+    mgs = _serial_module()
+    lib = mgs.load()
+    case = mgs.sa_case(*dims)
+    f = mgs.sa_inputs(case, steps, seed, kick)
+    qr, vr = mgs.sa_step(lib, dims, mgs.sa_scal(case.params), f)
+    q, v = _oracle_sa_step(mgs, dims, f)
+    np.testing.assert_array_equal(q, qr)
+    np.testing.assert_array_equal(v, vr)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SA), reason="serial reference library not built here")
+def test_flux_roe_against_reference_fresh_inputs():
+    mgs = _serial_module()
+    lib = mgs.load()
+    rng = np.random.default_rng(4242)
+    for _ in range(500):
+        pl = np.array([0.2 + 2 * rng.random(), *(3 * rng.standard_normal(3)), 0.1 + 3 * rng.random()])
+        pr = np.array([0.2 + 2 * rng.random(), *(3 * rng.standard_normal(3)), 0.1 + 3 * rng.random()])
+        nv = np.array([*rng.standard_normal(3), 0.0])
+        np.testing.assert_array_equal(O.flux_roe(pl, pr, nv), mgs.flux_roe(lib, pl, pr, nv))

@@ -40,6 +40,10 @@ class Params:
     yref: float = 0.0
     zref: float = 0.0
     nwholefield: int = 1
+    timedt_turb: float = 10.0
+    kmaxlim: float = 30000.0
+    xk: float = -1.0
+    xb: float = 1.0
 
 
 @dataclass
