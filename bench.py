@@ -53,6 +53,36 @@ def alg_bytes_per_node(slot, lhs_bytes):
     return table.get(slot)
 
 
+# Kernel of each single-kernel timing slot, as rocprofv3 names it.
+SLOT_KERNEL = {"rhs_inv_i": "k_rhs_inv<0,", "rhs_inv_j": "k_rhs_inv<1,", "rhs_inv_k": "k_rhs_inv<2,",
+               "rhs_vis_i": "k_rhs_vis<0>", "rhs_vis_j": "k_rhs_vis<1>", "rhs_vis_k": "k_rhs_vis<2>",
+               "vis_deriv": "k_vis_node_deriv", "sweep_lu_l": "k_sweep_pipe<0,", "sweep_lu_u": "k_sweep_pipe<1,",
+               "sweep_gs_l": "k_sweep_pipe<2,", "sweep_gs_u": "k_sweep_pipe<3,", "sweep_stage": "k_skew_stage<",
+               "update": "k_update"}
+PROFILE_TAG = {"bump": "bump256x128x64", "cart": "cart256x128x64"}
+
+
+def measured_traffic(workload, lhs, slot):
+    """HBM bytes per launch of the slot's kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload (profiles/), or None."""
+    tag = PROFILE_TAG.get(workload)
+    name = SLOT_KERNEL.get(slot)
+    if not tag or not name:
+        return None, None
+    path = os.path.join(REPO, "profiles", f"{tag}_{lhs}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    key = name.replace(" ", "")
+    for line in open(path):
+        cells = [c.strip().strip("`") for c in line.split("|")]
+        if len(cells) > 4 and cells[1].replace(" ", "").startswith(key):
+            try:
+                return float(cells[4]) * 1e6, os.path.relpath(path, REPO)
+            except ValueError:
+                return None, None
+    return None, None
+
+
 # Whole-step algorithmic bytes per node (Euler / laminar Navier-Stokes, FP64):
 # RHS 21 + LU 26 + 26 + SGS 31 + 31 + update 15 = 150 words.
 STEP_BYTES_PER_NODE = 8 * 150
@@ -151,8 +181,9 @@ def main():
         per_launch = alg_bytes_per_node(dom, lhs_bytes) * nodes * args.steps / dlaunch
         avg_s = dms / dlaunch / 1e3
         ach = per_launch / avg_s / 1e9
+        traffic, tsrc = measured_traffic(args.workload, args.lhs, dom)
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc, "kernel": dom,
                 "alg_bytes_per_launch": per_launch, "avg_launch_ms": round(dms / dlaunch, 4)}
     if args.breakdown:
         tot = sum(v[0] for v in tim.values())
