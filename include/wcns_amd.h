@@ -41,13 +41,17 @@ typedef struct {
     double moo, reynolds, attack, sideslip, tref, twall;
     double cfl, timedt_rate, efix, csrv, gama, prl, prt;
     int nvis;      /* 0 Euler, 1 Navier-Stokes                          */
-    int nlamtur;   /* -1 laminar                                         */
+    int nlamtur;   /* -1 laminar, 1 Spalart-Allmaras (nameturb 'SA')      */
     int nscheme;   /* 4 (WCNS_E_5) or 41 (WCNS_E_5_41)                   */
     int nflux;     /* 1 Steger-Warming, 3 Roe                            */
     int ntmst;     /* 0 local time step                                  */
     /* namelist force (param.dat): reference area, length and moment centre */
     double sref, lfref, xref, yref, zref;
     int nwholefield; /* 1 whole body; 0 half model (forces doubled)      */
+    /* namelist turbulence (param.dat, turblence.f90): SA time-step ratio,
+     * mu_t/mu cap of max_control_turbulence, NND parameters of the SA
+     * convection (Convect.f90:1281, turblence.f90:371-435) */
+    double timedt_turb, kmaxlim, xk, xb;
 } wcns_params;
 
 /* Precision of the implicit operator: the residual, WCNS weights and the
@@ -83,7 +87,9 @@ int wcns_set_lhs_precision(wcns_ctx *ctx, int prec);
 /* ---- field exchange (host <-> device), G3 layout ------------------------ */
 /* which: 0 r, 1 u, 2 v, 3 w, 4 p, 5 t, 7 dtdt, 8 sra, 9 srb, 10 src,
  *        11 srva, 12 srvb, 13 srvc, 14 visl, 16 vol, 17..25 kcx..ctz,
- *        100 dq (5 G3 arrays, variable-major), 102 rhs (the residual R) */
+ *        100 dq (5 G3 arrays, variable-major), 102 rhs (the residual R);
+ *        with the SA model (nlamtur = 1) also 15 vist (mu_t), 26 qke (the SA
+ *        working variable nu~, mb_qke), 27 dist (wall distance, nodes 1..n) */
 int wcns_set_prim(wcns_ctx *ctx, int nb, const double *r, const double *u,
                   const double *v, const double *w, const double *p);
 int wcns_get_field(wcns_ctx *ctx, int nb, int which, double *out);
@@ -142,7 +148,11 @@ int wcns_sync(wcns_ctx *ctx);
 #define WCNS_T_VIS_I       16   /* viscous flux derivative along i, j, k */
 #define WCNS_T_VIS_J       17
 #define WCNS_T_VIS_K       18
-#define WCNS_T_NSLOT       19
+#define WCNS_T_SA_RHS      19   /* SA: gradients, sources, diffusion, NND convection, diagonal */
+#define WCNS_T_SA_SWEEP_L  20   /* SA: scalar LU-SGS forward / backward sweep */
+#define WCNS_T_SA_SWEEP_U  21
+#define WCNS_T_SA_UPDATE   22   /* SA: update, mu_t, max_control */
+#define WCNS_T_NSLOT       23
 int wcns_timing(wcns_ctx *ctx, int on);
 int wcns_timing_get(wcns_ctx *ctx, int slot, double *total_ms, long long *launches);
 

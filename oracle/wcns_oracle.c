@@ -971,7 +971,7 @@ static void init_sa(o_case *c)
 {
     const double cv13 = pow(7.1, 3.0);
     double kafan = c->muoo;
-    double kf3 = pow(kafan, 3.0);
+    double kf3 = kafan * kafan * kafan;     /* kafan**3.0, as the flang build evaluates it */
     double fv1 = kf3 / (kf3 + cv13);
     double vt = c->muoo * fv1;
     for (int nb = 0; nb < c->nblocks; nb++) {

@@ -287,3 +287,99 @@ def test_multiblock_40_steps():
     print("multi-block residual history rel err max", rel.max(), flush=True)
     assert rel.max() <= 1e-10
     _cmp_all_blocks(s, o, case, ("r", "u", "v", "w", "p"), 1e-11)
+
+
+# ---------------------------------------------------------------- Spalart-Allmaras
+def _sa_bump(**kw):
+    case = cases.bump_channel_case(dy_wall=1e-3, growth=1.2, reynolds=1e5, **kw)
+    case.params.nlamtur = 1
+    return case
+
+
+def _sa_mixed_bc():
+    """Two x two blocks, no-slip wall at j = 1, symmetry on top, fixed
+    freestream inflow (BC 5) and outflow (BC 6) on the i faces, k periodic."""
+    case = cases.multiblock_case(nvis=1, wall=True, periodic_i=False, ilo_bc=5, ihi_bc=6, jhi_bc=3)
+    case.params.nlamtur = 1
+    return case
+
+
+SA_CASES = [dict(ni=24, nj=16, nk=12, nflux=3, nscheme=41), dict(ni=20, nj=18, nk=13, nflux=1, nscheme=4)]
+
+
+@pytest.mark.parametrize("kw", SA_CASES, ids=lambda k: f"{k['ni']}x{k['nj']}x{k['nk']}-s{k['nscheme']}-f{k['nflux']}")
+def test_sa_stage_by_stage(kw):
+    """SA bump channel: wall distance and initial field at set-up; each step the
+    SA ghosts (wall, far field, periodic), the radii with mu_t, the SA step
+    inside r_h_s_vis (nu~ and mu_t after it) and the viscous residual with mu_t."""
+    case = _sa_bump(**kw)
+    blk = case.blocks[0]
+    print(f"SA case {kw}", flush=True)
+    s = _solver(case)
+    o = O.OracleCase(case)
+    e = _rel_err(_interior(s.field(0, "dist"), blk), _interior(o.field(0, "dist"), blk))
+    assert e <= REL, f"wall distance: {e:.3e}"
+    for name in ("qke", "vist"):
+        e = _rel_err(_interior(s.field(0, name), blk, 1), _interior(o.field(0, name), blk, 1))
+        assert e == 0.0, f"initial {name}: {e:.3e}"
+    for step in range(3):
+        s.boundary_all(); o.stage("boundary_all")
+        _cmp_prim(s, o, blk, REL)
+        for name in ("qke", "vist"):
+            e = _rel_err(_interior(s.field(0, name), blk, 1), _interior(o.field(0, name), blk, 1))
+            assert e <= REL, f"{name} after BCs, step {step}: {e:.3e}"
+        s.initial_q_dq_c_t(); o.stage("initial_q_dq_c_t")
+        s.timestep_tgh(); o.stage("timestep")
+        for m in ("srva", "srvb", "srvc", "dtdt"):
+            e = _rel_err(_interior(s.field(0, m), blk), _interior(o.field(0, m), blk))
+            assert e <= REL, f"{m}: {e:.3e}"
+        s.r_h_s_vis(); o.stage("rhs_vis")
+        for name in ("qke", "vist"):
+            e = _rel_err(_interior(s.field(0, name), blk, 1), _interior(o.field(0, name), blk, 1))
+            assert e <= REL, f"{name} after the SA step, step {step}: {e:.3e}"
+        rg = _interior(s.field(0, "rhs"), blk)
+        ro = _interior(o.field(0, "dq"), blk)
+        assert _rel_err(rg, ro) <= REL, f"viscous rhs step {step}: {_rel_err(rg, ro):.3e}"
+        s.communicate_dq_npp(); o.stage("communicate_dq")
+        s.r_h_s_invis(); o.stage("rhs_invis")
+        s.communicate_dq_npp(); o.stage("communicate_dq")
+        s.l_h_s_tgh(); o.stage("lhs")
+        dg = _interior(s.field(0, "dq"), blk)
+        do = _interior(o.field(0, "dq"), blk)
+        assert _rel_err(dg, do) <= REL, f"dq step {step}: {_rel_err(dg, do):.3e}"
+        s.communicate_pv_npp(); o.stage("communicate_pv")
+        _cmp_prim(s, o, blk, REL, ghost=0)
+
+
+def test_sa_bump_30_steps():
+    case = _sa_bump(ni=32, nj=24, nk=12)
+    blk = case.blocks[0]
+    s = _solver(case)
+    o = O.OracleCase(case)
+    hs = s.step(30)
+    ho = o.step(30)
+    rel = np.abs(hs - ho) / ho
+    print("SA residual history rel err max", rel.max(), flush=True)
+    assert rel.max() <= 1e-10
+    _cmp_prim(s, o, blk, 1e-11, ghost=0)
+    for name in ("qke", "vist"):
+        e = _rel_err(_interior(s.field(0, name), blk), _interior(o.field(0, name), blk))
+        assert e <= 1e-11, f"{name}: {e:.3e}"
+
+
+def test_sa_multiblock_mixed_bcs_20_steps():
+    """Four blocks: SA across 1:1 interfaces, symmetry, fixed inflow and outflow
+    ghosts, and the batched scalar sweeps over all blocks at once."""
+    case = _sa_mixed_bc()
+    print(f"SA multiblock case: {len(case.blocks)} blocks", flush=True)
+    s = _solver(case)
+    o = O.OracleCase(case)
+    for nb, blk in enumerate(case.blocks):
+        e = _rel_err(_interior(s.field(nb, "dist"), blk), _interior(o.field(nb, "dist"), blk))
+        assert e <= REL, f"wall distance block {nb}: {e:.3e}"
+    hs = s.step(20)
+    ho = o.step(20)
+    rel = np.abs(hs - ho) / ho
+    print("SA multiblock residual history rel err max", rel.max(), flush=True)
+    assert rel.max() <= 1e-10
+    _cmp_all_blocks(s, o, case, ("r", "u", "v", "w", "p", "qke", "vist"), 1e-11)

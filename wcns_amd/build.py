@@ -16,7 +16,8 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=o
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-Wno-unused-variable"]
 
-SOURCES = ["wcns_api.hip", "wcns_rhs_inv.hip", "wcns_rhs_vis.hip", "wcns_step.hip", "wcns_metrics.hip", "wcns_force.hip", "wcns_sweep.hip"]
+SOURCES = ["wcns_api.hip", "wcns_rhs_inv.hip", "wcns_rhs_vis.hip", "wcns_step.hip", "wcns_metrics.hip", "wcns_force.hip",
+           "wcns_sweep.hip", "wcns_sa.hip"]
 
 
 def _compile(src):

@@ -267,12 +267,14 @@ def linear_field_init(A, moo=0.5):
 
 def multiblock_case(nx=(14, 18), ny=(13, 15), nz=(12,), seed=5, nvis=0, wall=False, periodic_i=True,
                     periodic_k=True, nscheme=41, nflux=3, moo=0.6, reynolds=1.0e4, cfl=5.0, lx=2.0, ly=1.0,
-                    lz=0.6, amp_grid=0.03, nmodes=6, amp=0.01, attack=2.0):
+                    lz=0.6, amp_grid=0.03, nmodes=6, amp=0.01, attack=2.0, ilo_bc=4, ihi_bc=4, jhi_bc=4):
     """A box split into len(nx) x len(ny) x len(nz) blocks of uneven node counts
     (block (a,b,c) has nx[a] x ny[b] x nz[c] nodes) meeting at 1:1
     point-matched interfaces (bctype -1).  The outer i and k faces are
     connected periodically; the j faces are far field (BC 4), or a no-slip wall
-    (BC 2) at j = 1 when `wall`.  A smooth coordinate perturbation makes all
+    (BC 2) at j = 1 when `wall`.  Outer faces that are not periodic take
+    ilo_bc / ihi_bc (i) and jhi_bc (j = top, a flat plane), e.g. 5 (fixed
+    freestream), 6 (outflow) or 3 (symmetry).  A smooth coordinate perturbation makes all
     nine metric terms non-zero; rho and p carry seeded Fourier modes.  Regions
     of every block are listed as i-lo, i-hi, j-lo, j-hi, k-lo, k-hi."""
     rng = np.random.default_rng(seed)
@@ -320,6 +322,10 @@ def multiblock_case(nx=(14, 18), ny=(13, 15), nz=(12,), seed=5, nvis=0, wall=Fal
                         regs.append(Region(fs[f], -1, faces(*tdims)[tf], t + 1, tf + 1))
                     elif ax == 1 and hi == 0 and wall:
                         regs.append(Region(fs[f], 2))
+                    elif ax == 0:
+                        regs.append(Region(fs[f], ihi_bc if hi else ilo_bc))
+                    elif ax == 1 and hi == 1:
+                        regs.append(Region(fs[f], jhi_bc))
                     else:
                         regs.append(Region(fs[f], 4))
                 blocks[bid(a, b, c)] = Block(ni, nj, nk, X[sl].copy(), Y[sl].copy(), Z[sl].copy(), regs,

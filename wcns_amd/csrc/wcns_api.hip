@@ -63,7 +63,10 @@ struct wcns_ctx {
     int cur = 0;
     double *t = nullptr, *metric[10] = {nullptr}, *rhs = nullptr, *dq = nullptr;
     double *dtdt = nullptr, *sr[3] = {nullptr}, *srv[3] = {nullptr}, *visl = nullptr;
-    double *dn = nullptr;                   // 12 node derivatives (viscous)
+    double *dn = nullptr;                   // 12 node derivatives (viscous); SA gradients before them
+    double *qke = nullptr, *vist = nullptr, *dist = nullptr;   // Spalart-Allmaras fields
+    SaWork sa{};                            // SA scratch (grad aliases dn)
+    double *sa_pool = nullptr;              // viseq, un[3], dqke, spec
     double *coed = nullptr;                 // LU-SGS diagonal per node
     float *dq32 = nullptr;                  // dQ of the FP32 implicit operator
     void *skew[2] = {nullptr, nullptr};     // plane-major sweep operands, FP64 / FP32 implicit operator
@@ -127,6 +130,7 @@ struct wcns_ctx {
         F.srva = srv[0]; F.srvb = srv[1]; F.srvc = srv[2];
         F.visl = visl; F.counters = counters; F.total = total; F.coed = coed;
         F.dq32 = lhs_prec == WCNS_LHS_FP32 ? dq32 : nullptr;
+        F.qke = qke; F.vist = vist; F.dist = dist;
         return F;
     }
 };
@@ -396,7 +400,7 @@ int wcns_destroy(wcns_ctx *ctx)
     for (int m = 0; m < 3; m++) { hipFree(ctx->sr[m]); hipFree(ctx->srv[m]); }
     void *pl[] = {ctx->d_sblocks, ctx->d_work, ctx->d_xch};
     for (void *p : pl) if (p) hipFree(p);
-    void *ps[] = {ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
+    void *ps[] = {ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
                   ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
@@ -541,7 +545,9 @@ int wcns_setup(wcns_ctx *ctx)
     if (p.nscheme != 4 && p.nscheme != 41) return fail(ctx, WCNS_E_UNSUPPORTED, "nscheme must be 4 or 41");
     if (p.nflux != 1 && p.nflux != 3) return fail(ctx, WCNS_E_UNSUPPORTED, "nflux must be 1 (SW) or 3 (Roe)");
     if (p.nvis != 0 && p.nvis != 1) return fail(ctx, WCNS_E_ARG, "nvis must be 0 or 1");
-    if (p.nlamtur >= 0) return fail(ctx, WCNS_E_UNSUPPORTED, "turbulence models (nlamtur >= 0) are not built");
+    if (p.nlamtur >= 0 && !(p.nlamtur == 1 && p.nvis == 1))
+        return fail(ctx, WCNS_E_UNSUPPORTED, "of the turbulence models only SA (nlamtur = 1, nvis = 1) is built");
+    const bool turb = p.nlamtur == 1;
     if (p.ntmst != 0) return fail(ctx, WCNS_E_UNSUPPORTED, "only local time stepping (ntmst = 0)");
     // layout
     long off = 0;
@@ -580,6 +586,17 @@ int wcns_setup(wcns_ctx *ctx)
     TRY(dalloc(ctx, &ctx->dtdt, T));
     TRY(dalloc(ctx, &ctx->visl, T));
     if (p.nvis == 1) TRY(dalloc(ctx, &ctx->dn, 12 * T));
+    if (turb) {
+        TRY(dalloc(ctx, &ctx->qke, T));
+        TRY(dalloc(ctx, &ctx->vist, T));
+        TRY(dalloc(ctx, &ctx->dist, T));
+        TRY(dalloc(ctx, &ctx->sa_pool, 6 * T));
+        ctx->sa.grad = ctx->dn;
+        ctx->sa.viseq = ctx->sa_pool;
+        for (int d = 0; d < 3; d++) ctx->sa.un[d] = ctx->sa_pool + (1 + d) * T;
+        ctx->sa.dqke = ctx->sa_pool + 4 * T;
+        ctx->sa.spec = ctx->sa_pool + 5 * T;
+    }
     TRY(dalloc(ctx, &ctx->coed, T));
     TRY(dalloc(ctx, &ctx->d_sweep_err, 1));
     // sweep tiles: TJ x TK columns per workgroup (16 x 4 unless tuned)
@@ -611,6 +628,34 @@ int wcns_setup(wcns_ctx *ctx)
     d.twall_fixed = p.twall > 0.0;
     d.twall_nd = p.twall / p.tref;
     d.nvis = p.nvis; d.nscheme = p.nscheme; d.nflux = p.nflux;
+    d.turb = turb ? 1 : 0;
+    d.muoo = 1.0e-1;                        // turblence.f90:3166-3181 inflow_turbulence (SA)
+    d.kmaxlim = p.kmaxlim; d.timedt_turb = p.timedt_turb; d.xk = p.xk; d.xb = p.xb;
+
+    // wall distance (SA): every node of every no-slip wall region (bctype 2, 20, 21),
+    // parallel.f90:4313-4468
+    double *d_wall = nullptr;
+    int nwall = 0;
+    if (turb) {
+        std::vector<double> wx, wy, wz;
+        for (auto &b : ctx->blocks)
+            for (auto &r : b.regs) {
+                if (!(r.bctype == 2 || r.bctype == 20 || r.bctype == 21)) continue;
+                for (int t = 0; t < r.d.npts; t++) {
+                    int i, j, k;
+                    region_point(r.d, t, i, j, k);
+                    size_t q = (size_t)(i - 1) + (size_t)b.ni * ((size_t)(j - 1) + (size_t)b.nj * (size_t)(k - 1));
+                    wx.push_back(b.x[q]); wy.push_back(b.y[q]); wz.push_back(b.z[q]);
+                }
+            }
+        nwall = (int)wx.size();
+        TRY(dalloc(ctx, &d_wall, 3 * (size_t)std::max(nwall, 1)));
+        if (nwall) {
+            wx.insert(wx.end(), wy.begin(), wy.end());
+            wx.insert(wx.end(), wz.begin(), wz.end());
+            TRY(copy_sync(ctx, d_wall, wx.data(), 8 * wx.size(), hipMemcpyHostToDevice));
+        }
+    }
 
     // GCL metrics per block on the device
     DFields F = ctx->fields();
@@ -623,11 +668,22 @@ int wcns_setup(wcns_ctx *ctx)
         TRY(copy_sync(ctx, dy, b.y.data(), 8 * n, hipMemcpyHostToDevice));
         TRY(copy_sync(ctx, dz, b.z.data(), 8 * n, hipMemcpyHostToDevice));
         if (launch_gcl_metrics(b.db, dx, dy, dz, F, work, ctx->stream)) return fail(ctx, WCNS_E_HIP, "gcl launch");
+        if (turb && launch_wall_dist(b.db, dx, dy, dz, d_wall, nwall, ctx->dist, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "wall distance launch");
         int nbad = 0;
         CK(hipMemcpyAsync(&nbad, (int *)(work + 21 * b.g3), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
         hipFree(dx); hipFree(dy); hipFree(dz); hipFree(work);
         if (nbad) return fail(ctx, WCNS_E_JACOBIAN, std::to_string(nbad) + " nodes with non-positive Jacobian");
+    }
+    if (d_wall) hipFree(d_wall);
+    if (turb) {
+        // init_SA (turblence.f90:2701-2734): nu~ = muoo, mu_t = muoo fv1(muoo) on -1..n+1
+        const double cv13 = 7.1 * 7.1 * 7.1, kafan = d.muoo;
+        const double kf3 = kafan * kafan * kafan;
+        const double fv1 = kf3 / (kf3 + cv13);
+        for (auto &b : ctx->blocks)
+            if (launch_sa_init(b.db, F, d.muoo, d.muoo * fv1, ctx->stream)) return fail(ctx, WCNS_E_HIP, "SA init");
     }
     // symmetry-plane normals (BC.f90:381-396) need the metrics at the face centre
     for (auto &b : ctx->blocks)
@@ -655,8 +711,8 @@ int wcns_setup(wcns_ctx *ctx)
     if (ctx->tiled_sweeps) {
         TRY(alloc_skew(ctx, 0));
         if (ctx->lhs_prec == WCNS_LHS_FP32) TRY(alloc_skew(ctx, 1));
-        TRY(build_sweep_plan(ctx));
     }
+    if (ctx->tiled_sweeps || turb) TRY(build_sweep_plan(ctx));
     // interface lists: 2pm regions in (nb, nr) order, serial when a region reads its own writes
     ctx->iface.clear();
     ctx->phys.clear();
@@ -743,7 +799,10 @@ static int field_ptr(wcns_ctx *ctx, int which, double **p, int *ncomp)
     case 8: case 9: case 10: *p = ctx->sr[which - 8]; return 0;
     case 11: case 12: case 13: *p = ctx->srv[which - 11]; return 0;
     case 14: *p = ctx->visl; return 0;
+    case 15: if (!ctx->vist) return WCNS_E_ARG; *p = ctx->vist; return 0;
     case 16: *p = ctx->metric[9]; return 0;
+    case 26: if (!ctx->qke) return WCNS_E_ARG; *p = ctx->qke; return 0;
+    case 27: if (!ctx->dist) return WCNS_E_ARG; *p = ctx->dist; return 0;
     case 100:
         if (ctx->lhs_prec == WCNS_LHS_FP32) return WCNS_E_UNSUPPORTED;
         *p = ctx->dq; *ncomp = 5; return 0;
@@ -848,6 +907,22 @@ int wcns_r_h_s_vis(wcns_ctx *ctx)
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     if (ctx->prm.nvis != 1) return WCNS_OK;
     DFields F = ctx->fields();
+    if (ctx->dp.turb) {
+        // RHS.f90:22: the turbulence model advances first; its update (nu~ on 2..n-1,
+        // mu_t on 0..n) only touches the block's own nodes, so all blocks go together
+        {
+            Scope sc(ctx, WCNS_T_SA_RHS, 2 * (int)ctx->blocks.size());
+            for (auto &b : ctx->blocks)
+                if (launch_sa_rhs(b.db, F, ctx->sa, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "SA rhs launch");
+        }
+        for (int fwd = 1; fwd >= 0; fwd--) {
+            Scope sc(ctx, fwd ? WCNS_T_SA_SWEEP_L : WCNS_T_SA_SWEEP_U);
+            if (launch_sa_sweep(fwd, ctx->plan, ctx->sa, ctx->stream)) return fail(ctx, WCNS_E_HIP, "SA sweep launch");
+        }
+        Scope sc(ctx, WCNS_T_SA_UPDATE, (int)ctx->blocks.size());
+        for (auto &b : ctx->blocks)
+            if (launch_sa_update(b.db, F, ctx->sa, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "SA update launch");
+    }
     {
         Scope sc(ctx, WCNS_T_VIS, (int)ctx->blocks.size());
         for (auto &b : ctx->blocks)

@@ -56,6 +56,10 @@ struct SweepPlan {
     long spin_limit;
 };
 long sweep_exchange_words(int TJ, int TK, int nTJ, int nTK, int ni);
+// An exchange word nobody has written yet: a signalling-NaN pattern that no
+// arithmetic result has.  Every slot is reset to it before each sweep.
+constexpr unsigned long long XCH_EMPTY = 0x7ff4deadbeef0001ull;
+int launch_fill_xch(const SweepPlan &S, hipStream_t s);
 // Plane-major staging copy of the sweep operands: arrays of skew_size(b)
 // elements per block (element (i,j,k), i,j,k in 0..n+1, at (i+j+k)*(nj+2)*(nk+2)
 // + j + (nj+2)*k), SK_N arrays of sktot elements each, double or float (prec).
@@ -90,6 +94,20 @@ int launch_residual(const DBlock &b, const DFields &F, double *partial, int npar
 int residual_parts(const DBlock &b);
 int launch_residual_final(const double *partial, int nparts, double *out, hipStream_t s);
 int launch_copy_interior(const DBlock &b, const double *src, double *dst, hipStream_t s);
+
+// Spalart-Allmaras: scratch of one step (field-pool frame, total elements each)
+struct SaWork {
+    double *grad;          // 12 arrays: d(u, v, w, nu~)/d(x, y, z), variable-major
+    double *viseq;         // 1.5 (mu/rho + nu~)
+    double *un[3];         // contravariant velocities along i, j, k
+    double *dqke, *spec;   // right-hand side / sweep unknown, diagonal
+};
+int launch_wall_dist(const DBlock &b, const double *x, const double *y, const double *z, const double *wall, int nw,
+                     double *dist, hipStream_t s);
+int launch_sa_init(const DBlock &b, const DFields &F, double muoo, double vt, hipStream_t s);
+int launch_sa_rhs(const DBlock &b, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s);
+int launch_sa_sweep(int fwd, const SweepPlan &S, const SaWork &W, hipStream_t s);
+int launch_sa_update(const DBlock &b, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s);
 
 // wall force
 long force_quads(const DRegion &g);

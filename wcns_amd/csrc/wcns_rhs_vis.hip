@@ -120,7 +120,7 @@ __device__ inline double dhalf(const double *q, const VLine &L, int f, bool fl, 
 
 template <int DIR>
 __device__ void face_visc(const DFields &F, const DParams &P, const double *dn, const VLine &L, int f, bool fl,
-                          bool fr, double cp_prl, double *fv)
+                          bool fr, double cp_prl, double cp_prt, double *fv)
 {
     const double *q4[4] = {F.u, F.v, F.w, F.t};
     double uvwt[4], du[12], kx[9];
@@ -139,12 +139,14 @@ __device__ void face_visc(const DFields &F, const DParams &P, const double *dn, 
 #pragma unroll
     for (int a = 0; a < 9; a++) kx[a] = vhn(ms[a], L, f);
     double vol = vhn(F.vol, L, f);
-    // viscosities: vslt1 = visl (+ vist), vslt2 = visl*cp/prl (+ vist*cp/prt); laminar vist = 0
+    // viscosities (sub_tgh.f90:1562-1566): vslt1 = visl + vist, vslt2 = visl*cp/prl + vist*cp/prt;
+    // laminar: vist = 0
     double s1, s2;
     {
         int n = L.n;
-        auto V1 = [&](int l) { return F.visl[L.at(l)] + 0.0; };
-        auto V2 = [&](int l) { return F.visl[L.at(l)] * cp_prl + 0.0; };
+        const bool turb = P.turb != 0;
+        auto V1 = [&](int l) { return F.visl[L.at(l)] + (turb ? F.vist[L.at(l)] : 0.0); };
+        auto V2 = [&](int l) { return F.visl[L.at(l)] * cp_prl + (turb ? F.vist[L.at(l)] * cp_prt : 0.0); };
         if (f >= 2 && f <= n - 2) {
             s1 = (9.0 * (V1(f) + V1(f + 1)) + -1.0 * (V1(f + 2) + V1(f - 1))) / 16.0;
             s2 = (9.0 * (V2(f) + V2(f + 1)) + -1.0 * (V2(f + 2) + V2(f - 1))) / 16.0;
@@ -209,7 +211,7 @@ __global__ void __launch_bounds__(128) k_rhs_vis(DBlock b, DFields F, DParams P,
     pencil_flags(F, L, fl, fr);
     const double re = P.re;
     const double cp = 1.0 / ((P.gama - 1.0) * P.moo * P.moo);
-    const double cp_prl = cp / P.prl;
+    const double cp_prl = cp / P.prl, cp_prt = cp / P.prt;
 
     int a = seg * SEG + 1;
     int bb = a + SEG - 1;
@@ -225,7 +227,7 @@ __global__ void __launch_bounds__(128) k_rhs_vis(DBlock b, DFields F, DParams P,
         for (int l = 0; l < 5; l++)
 #pragma unroll
             for (int m = 0; m < 5; m++) fw[l][m] = fw[l + 1][m];
-        face_visc<DIR>(F, P, dn, L, fc, fl, fr, cp_prl, fw[5]);
+        face_visc<DIR>(F, P, dn, L, fc, fl, fr, cp_prl, cp_prt, fw[5]);
         // FLUX_DXYZ rows (WCNSE5.f90:229-251); R -= re * dfv
         double df[5];
         if (fc == 3) {
@@ -273,7 +275,8 @@ __global__ void __launch_bounds__(128) k_rhs_vis(DBlock b, DFields F, DParams P,
     }
 }
 
-// LHS.f90:243-343 spectvisl, interior nodes (ghost-layer radii only ever multiply a zero dq)
+// LHS.f90:243-343 spectvisl (laminar) / LHS.f90:141-241 spectvist (mu + mu_t), interior
+// nodes (ghost-layer radii only ever multiply a zero dq)
 __global__ void k_spectvis(DBlock b, DFields F, DParams P)
 {
     long n = (long)b.ni * b.nj * b.nk;
@@ -288,7 +291,7 @@ __global__ void k_spectvis(DBlock b, DFields F, DParams P)
     double ri = kx * kx + ky * ky + kz * kz;
     double rj = ex * ex + ey * ey + ez * ez;
     double rk = cx * cx + cy * cy + cz * cz;
-    double vis = F.visl[g];
+    double vis = P.turb ? F.visl[g] + F.vist[g] : F.visl[g];
     double coef = 2.0 * vis / (P.reynolds * rm_vol + SMALL);
     double coef1 = P.csrv * coef;
     F.srva[g] = coef1 * ri;

@@ -1,0 +1,521 @@
+// wcns_sa.hip -- the Spalart-Allmaras one-equation model as the reference runs
+// it inside r_h_s_vis (RHS.f90:22 -> turblence.f90:15-103 turbulence ->
+// differential_model_turbulence), nameturb 'SA', ncmpcor 0, ndes 0, method 1,
+// and its set-up (wall distance, uniform initial field).
+//
+//   k_wall_dist   parallel.f90:4313-4468 calc_wall_dists_parallel: squared
+//                 distance to every no-slip wall node, minimum, sqrt + small;
+//                 wall points staged through LDS in chunks.
+//   k_sa_init     turblence.f90:2701-2734 init_SA.
+//   k_sa_node     per node 1..n: the central-difference gradients of u, v, w
+//                 and nu~ (dfidxyz_center, turblence.f90:753-849), the
+//                 equivalent viscosity (viscosity_equivalent_SA_r,
+//                 turblence.f90:2785-2808) and the contravariant velocities
+//                 that the scalar LU-SGS coefficients are made of.
+//   k_sa_rhs      per interior node 2..n-1, fused in the reference's order of
+//                 accumulation: source (cal_rhs_source_SA, turblence.f90:
+//                 2506-2699), diffusion along i, j, k (cal_rhs_viscous_
+//                 turbulence_dif_r, :1022-1185), NND convection along i, j, k
+//                 (cal_rhs_inviscous_turbulence + NND_turbulence +
+//                 flux_turbulence, :291-459) and the diagonal (source part,
+//                 then cal_spec_turbulence_r, :218-259).  Face quantities that
+//                 two nodes share are formed by both, with the same operations.
+//   k_sa_sweep    lusgs_turbulence (turblence.f90:477-583): the scalar
+//                 forward and backward sweeps, pipelined over the same
+//                 (j,k)-column tiles, work list and exchange slots as the
+//                 five-variable sweeps (one word per boundary node).
+//   k_sa_update   update_turbulence (:585-623), cal_vist_SA (:2735-2760) and
+//                 max_control_turbulence (:625-660), node by node.
+#include "wcns_kernels.hpp"
+
+namespace wcns {
+
+namespace {
+
+inline dim3 grid1(long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
+
+// cal_rhs_source_SA model constants (turblence.f90:2509-2513)
+constexpr double SA_CB1 = 0.1355, SA_CB2 = 0.622, SA_CV1 = 7.1, SA_EPS = 1.0e-20;
+constexpr double SA_CGA = 2. / 3.0, SA_KI = 0.419, SA_CW2 = 0.3, SA_CW3 = 2.0;
+
+__device__ inline double amin1(double a, double b) { return a < b ? a : b; }
+__device__ inline double amax1(double a, double b) { return a > b ? a : b; }
+
+// Convect.f90:1281-1286 minmod
+__device__ inline double minmod(double x, double y)
+{
+    return 0.5 * (copysign(1.0, x) + copysign(1.0, y)) * amin1(fabs(x), fabs(y));
+}
+
+__device__ inline void node_of(const DBlock &b, long t, int lo, int ext_i, int ext_j, int &i, int &j, int &k)
+{
+    i = (int)(t % ext_i) + lo;
+    j = (int)((t / ext_i) % ext_j) + lo;
+    k = (int)(t / ((long)ext_i * ext_j)) + lo;
+}
+
+// ------------------------------------------------------------ wall distance
+constexpr int WD_THREADS = 256;
+
+__global__ void __launch_bounds__(WD_THREADS) k_wall_dist(DBlock b, const double *x, const double *y,
+                                                          const double *z, const double *wall, int nw,
+                                                          double *dist)
+{
+    __shared__ double sx[WD_THREADS], sy[WD_THREADS], sz[WD_THREADS];
+    const long n = (long)b.ni * b.nj * b.nk;
+    const long t = (long)blockIdx.x * WD_THREADS + threadIdx.x;
+    const bool act = t < n;
+    double xc = 0.0, yc = 0.0, zc = 0.0;
+    if (act) { xc = x[t]; yc = y[t]; zc = z[t]; }
+    double best = 1.0e30;
+    for (int w0 = 0; w0 < nw; w0 += WD_THREADS) {
+        const int cnt = nw - w0 < WD_THREADS ? nw - w0 : WD_THREADS;
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) {
+            sx[threadIdx.x] = wall[w0 + threadIdx.x];
+            sy[threadIdx.x] = wall[(long)nw + w0 + threadIdx.x];
+            sz[threadIdx.x] = wall[2L * nw + w0 + threadIdx.x];
+        }
+        __syncthreads();
+        for (int m = 0; m < cnt; m++) {
+            const double dx = xc - sx[m], dy = yc - sy[m], dz = zc - sz[m];
+            const double d = dx * dx + dy * dy + dz * dz;
+            best = d < best ? d : best;
+        }
+    }
+    if (act) {
+        int i, j, k;
+        node_of(b, t, 1, b.ni, b.nj, i, j, k);
+        dist[gidx(b, i, j, k)] = sqrt(best) + SMALL;
+    }
+}
+
+// ------------------------------------------------------------------ init
+__global__ void k_sa_init(DBlock b, DFields F, double muoo, double vt)
+{
+    const long n = (long)(b.ni + 3) * (b.nj + 3) * (b.nk + 3);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    node_of(b, t, -1, b.ni + 3, b.nj + 3, i, j, k);
+    const long g = gidx(b, i, j, k);
+    F.qke[g] = muoo;
+    F.vist[g] = vt;
+}
+
+// ------------------------------------------------------------- node pass
+// gradient array m of variable v (u, v, w, nu~) at component c: grad[(3v+c)*total + g]
+__global__ void k_sa_node(DBlock b, DFields F, SaWork W)
+{
+    const long n = (long)b.ni * b.nj * b.nk;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    node_of(b, t, 1, b.ni, b.nj, i, j, k);
+    const long g = gidx(b, i, j, k);
+    const long gi0 = gidx(b, i - 1, j, k), gi1 = gidx(b, i + 1, j, k);
+    const long gj0 = gidx(b, i, j - 1, k), gj1 = gidx(b, i, j + 1, k);
+    const long gk0 = gidx(b, i, j, k - 1), gk1 = gidx(b, i, j, k + 1);
+    const double kx = F.kcx[g], ky = F.kcy[g], kz = F.kcz[g];
+    const double ex = F.etx[g], ey = F.ety[g], ez = F.etz[g];
+    const double cx = F.ctx[g], cy = F.cty[g], cz = F.ctz[g];
+    const double vol2 = 0.5 / F.vol[g];
+    const double *fis[4] = {F.u, F.v, F.w, F.qke};
+    const long T = F.total;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const double *fi = fis[v];
+        const double dkc = fi[gi1] - fi[gi0];
+        const double det = fi[gj1] - fi[gj0];
+        const double dct = fi[gk1] - fi[gk0];
+        const double ddx = dkc * kx + det * ex + dct * cx;
+        const double ddy = dkc * ky + det * ey + dct * cy;
+        const double ddz = dkc * kz + det * ez + dct * cz;
+        W.grad[(3 * v) * T + g] = ddx * vol2;
+        W.grad[(3 * v + 1) * T + g] = ddy * vol2;
+        W.grad[(3 * v + 2) * T + g] = ddz * vol2;
+    }
+    const double r = F.r[g], um = F.u[g], vm = F.v[g], wm = F.w[g];
+    W.viseq[g] = 1.5 * (F.visl[g] / r + F.qke[g]);
+    // mxdq_std_turbulence's l1 = g . (u, v, w) with g the metric of each direction
+    W.un[0][g] = kx * um + ky * vm + kz * wm;
+    W.un[1][g] = ex * um + ey * vm + ez * wm;
+    W.un[2][g] = cx * um + cy * vm + cz * wm;
+}
+
+// -------------------------------------------------------------- RHS pass
+// flux_turbulence (turblence.f90:437-459): prim = (r, u, v, w, nu~)
+__device__ inline double flux_turb(const double *q, double nx, double ny, double nz, double npn)
+{
+    double l1 = nx * q[1] + ny * q[2] + nz * q[3];
+    l1 = 0.5 * (l1 + npn * sqrt(l1 * l1));
+    return l1 * q[4] * q[0];
+}
+
+// node at offset d from (i,j,k) along direction DIR
+template <int DIR>
+__device__ inline long nbr(const DBlock &b, int i, int j, int k, int d)
+{
+    return gidx(b, i + (DIR == 0 ? d : 0), j + (DIR == 1 ? d : 0), k + (DIR == 2 ? d : 0));
+}
+
+template <int DIR>
+__device__ inline double sa_diffusion(const DBlock &b, const DFields &F, const SaWork &W, int i, int j, int k,
+                                      long g)
+{
+    const double *mx = DIR == 0 ? F.kcx : (DIR == 1 ? F.etx : F.ctx);
+    const double *my = DIR == 0 ? F.kcy : (DIR == 1 ? F.ety : F.cty);
+    const double *mz = DIR == 0 ? F.kcz : (DIR == 1 ? F.etz : F.ctz);
+    const long T = F.total;
+    const double *gx = W.grad + 9 * T, *gy = W.grad + 10 * T, *gz = W.grad + 11 * T;
+    double fv[2];
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        // face between nodes l-1 and l, l = node + f
+        const long qa = nbr<DIR>(b, i, j, k, f), qb = nbr<DIR>(b, i, j, k, f - 1);
+        const double nx = 0.5 * (mx[qa] + mx[qb]), ny = 0.5 * (my[qa] + my[qb]), nz = 0.5 * (mz[qa] + mz[qb]);
+        const double dmudx = gx[qa] + gx[qb], dmudy = gy[qa] + gy[qb], dmudz = gz[qa] + gz[qb];
+        const double vis = W.viseq[qa] + W.viseq[qb];
+        fv[f] = (dmudx * nx + dmudy * ny + dmudz * nz) * vis * 0.25;
+    }
+    return fv[1] - fv[0];
+}
+
+template <int DIR>
+__device__ inline double sa_nnd(const DBlock &b, const DFields &F, const DParams &P, int i, int j, int k)
+{
+    const double *mx = DIR == 0 ? F.kcx : (DIR == 1 ? F.etx : F.ctx);
+    const double *my = DIR == 0 ? F.kcy : (DIR == 1 ? F.ety : F.cty);
+    const double *mz = DIR == 0 ? F.kcz : (DIR == 1 ? F.etz : F.ctz);
+    const double c1 = 0.25 * (1.0 + P.xk), c2 = 0.25 * (1.0 - P.xk), xb = P.xb;
+    // Q at offsets -2..2, DQ(p) = Q(p) - Q(p-1) at -1..2, CL/CR at -1..1
+    double q[5][5];
+#pragma unroll
+    for (int o = 0; o < 5; o++) {
+        const long g = nbr<DIR>(b, i, j, k, o - 2);
+        q[o][0] = F.r[g]; q[o][1] = F.u[g]; q[o][2] = F.v[g]; q[o][3] = F.w[g]; q[o][4] = F.qke[g];
+    }
+    double cl[3][5], cr[3][5];
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            const double dq0 = q[p + 1][m] - q[p][m];       // DQ at offset p-1
+            const double dq1 = q[p + 2][m] - q[p + 1][m];   // DQ at offset p
+            cl[p][m] = minmod(xb * dq1, dq0);
+            cr[p][m] = minmod(dq1, xb * dq0);
+        }
+    double fc[2];
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        // face between offsets f-1 and f
+        const long ga = nbr<DIR>(b, i, j, k, f - 1), gb = nbr<DIR>(b, i, j, k, f);
+        const double nx = 0.50 * (mx[ga] + mx[gb]);
+        const double ny = 0.50 * (my[ga] + my[gb]);
+        const double nz = 0.50 * (mz[ga] + mz[gb]);
+        double ql[5], qr[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            ql[m] = q[f + 1][m] + c1 * cr[f][m] + c2 * cl[f][m];
+            qr[m] = q[f + 2][m] - c1 * cl[f + 1][m] - c2 * cr[f + 1][m];
+        }
+        fc[f] = flux_turb(ql, nx, ny, nz, 1.0) + flux_turb(qr, nx, ny, nz, -1.0);
+    }
+    return fc[1] - fc[0];
+}
+
+__global__ void __launch_bounds__(128) k_sa_rhs(DBlock b, DFields F, SaWork W, DParams P)
+{
+    const int mi = b.ni - 2, mj = b.nj - 2;
+    const long n = (long)mi * mj * (b.nk - 2);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    node_of(b, t, 2, mi, mj, i, j, k);
+    const long g = gidx(b, i, j, k);
+    const long T = F.total;
+    const double re = P.re, reynolds = P.reynolds;
+    const double cw1 = SA_CB1 / SA_KI / SA_KI + (1.0 + SA_CB2) / SA_CGA;
+    const double cv13 = SA_CV1 * SA_CV1 * SA_CV1, ki2 = SA_KI * SA_KI;
+    const double cw36 = pow(SA_CW3, 6.0), onesix = 1.0 / 6.0;
+    const double ft2 = 0.0;
+
+    // cal_rhs_source_SA
+    const double ro = F.r[g], volume = F.vol[g], vsl = F.visl[g], vtbar = F.qke[g];
+    const double romu = ro * vtbar;
+    const double dn = amax1(F.dist[g], re);
+    const double dn2 = dn * dn;
+    const double reki2dn2 = reynolds * ki2 * dn2;
+    const double kafan = romu / vsl;
+    const double kf3 = kafan * kafan * kafan;
+    const double fv1 = kf3 / (kf3 + cv13);
+    const double fv2 = 1.0 - kafan / (1.0 + kafan * fv1);
+    const double dudy = W.grad[1 * T + g], dudz = W.grad[2 * T + g];
+    const double dvdx = W.grad[3 * T + g], dvdz = W.grad[5 * T + g];
+    const double dwdx = W.grad[6 * T + g], dwdy = W.grad[7 * T + g];
+    const double a1 = dudy - dvdx, a2 = dvdz - dwdy, a3 = dwdx - dudz;
+    const double omg = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+    double sbar = omg + vtbar / reki2dn2 * fv2;
+    sbar = amax1(sbar, SA_EPS);
+    double rbar = vtbar / (sbar * reki2dn2);
+    rbar = amin1(rbar, 10.0);
+    const double rbar5 = pow(rbar, 5.0);
+    const double gbar = rbar + SA_CW2 * rbar * (rbar5 - 1.0);
+    const double gbar6 = pow(gbar, 6.0);
+    const double fw = gbar * pow((1.0 + cw36) / (gbar6 + cw36), onesix);
+    const double cpmu = SA_CB1 * (1.0 - ft2) * sbar;
+    const double cdmu = (cw1 * fw - SA_CB1 * ft2 / ki2) * vtbar / dn2 / reynolds;
+    const double Sp = cpmu * vtbar, Sd = cdmu * vtbar;
+    const double dmudx = W.grad[9 * T + g], dmudy = W.grad[10 * T + g], dmudz = W.grad[11 * T + g];
+    const double Dbar = SA_CB2 / SA_CGA * (dmudx * dmudx + dmudy * dmudy + dmudz * dmudz) * re;
+    double dq = 0.0 - ro * (Sp - Sd + Dbar) * volume;
+    double sp = 0.0 - amin1(cpmu - 2.0 * cdmu, 0.0) * volume;
+
+    // cal_rhs_viscous_turbulence_dif_r
+    dq = dq - re * ro * sa_diffusion<0>(b, F, W, i, j, k, g);
+    dq = dq - re * ro * sa_diffusion<1>(b, F, W, i, j, k, g);
+    dq = dq - re * ro * sa_diffusion<2>(b, F, W, i, j, k, g);
+
+    // cal_spec_turbulence_r
+    {
+        const double um = F.u[g], vm = F.v[g], wm = F.w[g];
+        const double kx = F.kcx[g], ky = F.kcy[g], kz = F.kcz[g];
+        const double ex = F.etx[g], ey = F.ety[g], ez = F.etz[g];
+        const double cx = F.ctx[g], cy = F.cty[g], cz = F.ctz[g];
+        const double unx = um * kx + vm * ky + wm * kz;
+        const double uny = um * ex + vm * ey + wm * ez;
+        const double unz = um * cx + vm * cy + wm * cz;
+        sp = sp + fabs(unx) + fabs(uny) + fabs(unz) + 1.0 / (F.dtdt[g] * P.timedt_turb);
+        const double nx = kx * kx + ky * ky + kz * kz;
+        const double ny = ex * ex + ey * ey + ez * ez;
+        const double nz = cx * cx + cy * cy + cz * cz;
+        const double vis = W.viseq[g] / volume;
+        sp = sp + 2.0 * re * vis * (nx + ny + nz);
+    }
+
+    // cal_rhs_inviscous_turbulence
+    dq = dq + sa_nnd<0>(b, F, P, i, j, k);
+    dq = dq + sa_nnd<1>(b, F, P, i, j, k);
+    dq = dq + sa_nnd<2>(b, F, P, i, j, k);
+
+    W.dqke[g] = dq;
+    W.spec[g] = sp;
+}
+
+// ------------------------------------------------------------ the sweeps
+__device__ inline void xch_put1(double *slot, double v)
+{
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(slot), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline unsigned long long xch_issue1(double *slot)
+{
+    return __hip_atomic_load(reinterpret_cast<unsigned long long *>(slot), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded wait for a slot read issued earlier (see the five-variable sweeps).
+__device__ inline bool xch_finish1(double *slot, unsigned long long raw, double &v, long spin_limit, int *err)
+{
+    unsigned long long *p = reinterpret_cast<unsigned long long *>(slot);
+    long spins = 0;
+    unsigned long long x = raw;
+    while (x == XCH_EMPTY) {
+        if (++spins > spin_limit ||
+            ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            v = 0.0;
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        x = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    v = __longlong_as_double((long long)x);
+    return true;
+}
+
+struct SaOp {
+    double d, sp, ui, uj, uk;
+};
+
+// Scalar LU-SGS, one tile of (j,k) columns per workgroup of one wavefront; lane
+// (jl,kl) walks its column, at step t updating node i = t+1-jl-kl (forward).
+// Nodes outside 2..n-1 take no part in lusgs_turbulence: their lanes publish 0,
+// the value dqke holds there.  Forward: dq = (A+ dq_lower - dq) / spec;
+// backward: dq = dq - (A- dq_upper) / spec, A+- = (l1 +- |l1|)/2 of the
+// neighbour (mxdq_std_turbulence, turblence.f90:560-583).
+template <int FWD>
+__global__ void __launch_bounds__(64) k_sa_sweep(const SweepBlock *blocks, const int2 *work, SaWork W, int *err,
+                                                 long spin_limit)
+{
+    constexpr int off = FWD ? -1 : 1;
+    const double npn = FWD ? 1.0 : -1.0;
+    const int2 wk = work[blockIdx.x];
+    const SweepBlock sb = blocks[wk.x];
+    const DBlock b = sb.b;
+    const int TJ = sb.TJ, TK = sb.TK, nTJ = sb.nTJ, nTK = sb.nTK, ni = b.ni;
+    const int tile = wk.y;
+    const int J = tile % nTJ, K = tile / nTJ;
+    const int c = threadIdx.x;
+    const int jl = c % TJ, kl = c / TJ;
+    const int jlo = 1 + J * TJ, klo = 1 + K * TK;
+    const int jn = min(TJ, b.nj - jlo + 1), kn = min(TK, b.nk - klo + 1);
+    const bool col = kl < kn && jl < jn;
+    const int j = jlo + jl, k = klo + kl;
+    const int nsteps = ni + jn + kn - 2;
+    const int i0 = FWD ? 1 - jl - kl : ni + (jn - 1 - jl) + (kn - 1 - kl);
+    auto i_at = [&](int st) { return FWD ? i0 + st : i0 - st; };
+    auto live = [&](int st) { int ii = i_at(st); return col && st < nsteps && ii >= 1 && ii <= ni; };
+    const bool colin = j >= 2 && j <= b.nj - 1 && k >= 2 && k <= b.nk - 1;
+    auto inner = [&](int ii) { return colin && ii >= 2 && ii <= ni - 1; };
+
+    __shared__ double sh[2][64];
+    const int jsrc = FWD ? (jl > 0 ? 0 : (J > 0 ? 1 : 2)) : (jl < jn - 1 ? 0 : (J < nTJ - 1 ? 1 : 2));
+    const int ksrc = FWD ? (kl > 0 ? 0 : (K > 0 ? 1 : 2)) : (kl < kn - 1 ? 0 : (K < nTK - 1 ? 1 : 2));
+    const bool putj = col && (FWD ? (jl == jn - 1 && J < nTJ - 1) : (jl == 0 && J > 0));
+    const bool putk = col && (FWD ? (kl == kn - 1 && K < nTK - 1) : (kl == 0 && K > 0));
+    const long rowlen = ni;
+    double *xj_in = sb.xj + (jsrc == 1 ? ((long)(tile + off) * TK + kl) * rowlen : 0);
+    double *xk_in = sb.xk + (ksrc == 1 ? ((long)(tile + off * nTJ) * TJ + jl) * rowlen : 0);
+    double *xj_out = sb.xj + ((long)tile * TK + kl) * rowlen;
+    double *xk_out = sb.xk + ((long)tile * TJ + jl) * rowlen;
+
+    auto load = [&](int st, SaOp &o) {
+        const int i = i_at(st);
+        if (!inner(i)) return;
+        const long g = gidx(b, i, j, k);
+        o.d = W.dqke[g];
+        o.sp = W.spec[g];
+        o.ui = W.un[0][gidx(b, i + off, j, k)];
+        o.uj = W.un[1][gidx(b, i, j + off, k)];
+        o.uk = W.un[2][gidx(b, i, j, k + off)];
+    };
+    // operands of this step (o0) and the next two (o1, o2), rotated by value so
+    // that no register array is indexed at run time
+    SaOp o0{}, o1{}, o2{};
+    if (live(0)) load(0, o0);
+    if (live(1)) load(1, o1);
+    double prev = 0.0;
+    unsigned long long rj = 0, rk = 0;
+    bool bad = false;
+    if (live(0) && inner(i_at(0))) {
+        const int i = i_at(0);
+        if (jsrc == 1) rj = xch_issue1(xj_in + (i - 1));
+        if (ksrc == 1) rk = xch_issue1(xk_in + (i - 1));
+    }
+    for (int step = 0; step < nsteps; step++) {
+        const int slot = step & 1;
+        if (live(step + 2)) load(step + 2, o2);
+        if (live(step)) {
+            const int i = i_at(step);
+            double out = 0.0;
+            if (inner(i)) {
+                const SaOp &o = o0;
+                double nj_ = 0.0, nk_ = 0.0;
+                if (jsrc == 0) nj_ = sh[slot ^ 1][c + off];
+                else if (jsrc == 1 && !xch_finish1(xj_in + (i - 1), rj, nj_, bad ? 0 : spin_limit, err)) {
+                    if (!bad) atomicExch(err, 1);
+                    bad = true;
+                }
+                if (ksrc == 0) nk_ = sh[slot ^ 1][c + off * TJ];
+                else if (ksrc == 1 && !xch_finish1(xk_in + (i - 1), rk, nk_, bad ? 0 : spin_limit, err)) {
+                    if (!bad) atomicExch(err, 1);
+                    bad = true;
+                }
+                const double ai = 0.5 * (o.ui + npn * fabs(o.ui));
+                const double aj = 0.5 * (o.uj + npn * fabs(o.uj));
+                const double ak = 0.5 * (o.uk + npn * fabs(o.uk));
+                const double rhs0 = ai * prev + aj * nj_ + ak * nk_;
+                out = FWD ? (rhs0 - o.d) / o.sp : o.d - rhs0 / o.sp;
+                W.dqke[gidx(b, i, j, k)] = out;
+            }
+            if (putj) xch_put1(xj_out + (i - 1), out);
+            if (putk) xch_put1(xk_out + (i - 1), out);
+            sh[slot][c] = out;
+            prev = out;
+        }
+        if (live(step + 1) && inner(i_at(step + 1))) {
+            const int i = i_at(step + 1);
+            if (jsrc == 1) rj = xch_issue1(xj_in + (i - 1));
+            if (ksrc == 1) rk = xch_issue1(xk_in + (i - 1));
+        }
+        o0 = o1;
+        o1 = o2;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- update
+__global__ void k_sa_update(DBlock b, DFields F, SaWork W, DParams P)
+{
+    const long n = (long)(b.ni + 1) * (b.nj + 1) * (b.nk + 1);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int i, j, k;
+    node_of(b, t, 0, b.ni + 1, b.nj + 1, i, j, k);
+    const long g = gidx(b, i, j, k);
+    const bool in1 = i >= 1 && j >= 1 && k >= 1;
+    const bool in2 = i >= 2 && i <= b.ni - 1 && j >= 2 && j <= b.nj - 1 && k >= 2 && k <= b.nk - 1;
+    const double r = F.r[g];
+    double q = F.qke[g];
+    if (in2) {
+        q = amax1(q + W.dqke[g] / r, 1.0e-20 * P.muoo);
+        F.qke[g] = q;
+    }
+    double vt = F.vist[g];
+    if (in1) {
+        const double cv13 = SA_CV1 * SA_CV1 * SA_CV1;
+        const double romu = r * q;
+        const double kafan = romu / F.visl[g];
+        const double kf3 = kafan * kafan * kafan;
+        const double fv1 = kf3 / (kf3 + cv13);
+        vt = romu * fv1;
+    }
+    if (P.kmaxlim > 1.0e-15) vt = amin1(vt, P.kmaxlim * F.visl[g]);
+    F.vist[g] = vt;
+}
+
+}  // namespace
+
+int launch_wall_dist(const DBlock &b, const double *x, const double *y, const double *z, const double *wall, int nw,
+                     double *dist, hipStream_t s)
+{
+    long n = (long)b.ni * b.nj * b.nk;
+    hipLaunchKernelGGL(k_wall_dist, grid1(n, WD_THREADS), dim3(WD_THREADS), 0, s, b, x, y, z, wall, nw, dist);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_sa_init(const DBlock &b, const DFields &F, double muoo, double vt, hipStream_t s)
+{
+    long n = (long)(b.ni + 3) * (b.nj + 3) * (b.nk + 3);
+    hipLaunchKernelGGL(k_sa_init, grid1(n, 256), dim3(256), 0, s, b, F, muoo, vt);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_sa_rhs(const DBlock &b, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s)
+{
+    long n = (long)b.ni * b.nj * b.nk;
+    hipLaunchKernelGGL(k_sa_node, grid1(n, 256), dim3(256), 0, s, b, F, W);
+    long m = (long)(b.ni - 2) * (b.nj - 2) * (b.nk - 2);
+    if (m > 0) hipLaunchKernelGGL(k_sa_rhs, grid1(m, 128), dim3(128), 0, s, b, F, W, P);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_sa_sweep(int fwd, const SweepPlan &S, const SaWork &W, hipStream_t s)
+{
+    if (S.lanes > 64 || S.lanes < 1 || S.nwork < 1) return -2;
+    if (launch_fill_xch(S, s)) return -3;     // every exchange word back to the empty pattern
+    const int2 *work = fwd ? S.work_fwd : S.work_bwd;
+    if (fwd) hipLaunchKernelGGL(k_sa_sweep<1>, dim3(S.nwork), dim3(64), 0, s, S.blocks, work, W, S.err, S.spin_limit);
+    else hipLaunchKernelGGL(k_sa_sweep<0>, dim3(S.nwork), dim3(64), 0, s, S.blocks, work, W, S.err, S.spin_limit);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_sa_update(const DBlock &b, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s)
+{
+    long n = (long)(b.ni + 1) * (b.nj + 1) * (b.nk + 1);
+    hipLaunchKernelGGL(k_sa_update, grid1(n, 256), dim3(256), 0, s, b, F, W, P);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+}  // namespace wcns

@@ -334,9 +334,7 @@ __global__ void __launch_bounds__(256) k_sweep_plane(DBlock b, Operands<double> 
     for (int m = 0; m < 5; m++) A.c[m][g] = out[m];
 }
 
-// Empty exchange slot: a signalling NaN; arithmetic only ever yields quiet NaNs.
-constexpr unsigned long long XCH_EMPTY = 0x7ff4deadbeef0001ull;
-
+// resets exchange slots to XCH_EMPTY (arithmetic only ever yields quiet NaNs)
 __global__ void k_fill_u64(unsigned long long *a, long n, unsigned long long v)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -833,6 +831,12 @@ static int launch_pipe(int kind, const SweepPlan &S, const DFields &F, const DPa
     case 2: hipLaunchKernelGGL((k_sweep_pipe<2, T>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
     default: hipLaunchKernelGGL((k_sweep_pipe<3, T>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
     }
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_fill_xch(const SweepPlan &S, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_u64, grid1(S.nxch, 256), dim3(256), 0, s, S.xch, S.nxch, XCH_EMPTY);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -23,7 +23,8 @@ E_NAMES = {-1: "WCNS_E_ARG", -2: "WCNS_E_STATE", -3: "WCNS_E_HIP", -4: "WCNS_E_N
 
 FIELDS = {"r": 0, "u": 1, "v": 2, "w": 3, "p": 4, "t": 5, "dtdt": 7, "sra": 8, "srb": 9, "src": 10,
           "srva": 11, "srvb": 12, "srvc": 13, "visl": 14, "vol": 16, "kcx": 17, "kcy": 18, "kcz": 19,
-          "etx": 20, "ety": 21, "etz": 22, "ctx": 23, "cty": 24, "ctz": 25, "dq": 100, "rhs": 102}
+          "etx": 20, "ety": 21, "etz": 22, "ctx": 23, "cty": 24, "ctz": 25, "dq": 100, "rhs": 102,
+          "vist": 15, "qke": 26, "dist": 27}
 
 EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "wcns_add_region", "wcns_setup",
            "wcns_set_lhs_precision", "wcns_set_prim", "wcns_get_field", "wcns_set_field", "wcns_boundary_all",
@@ -33,7 +34,8 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
 
 TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
-                "sweep_stage", "rhs_vis_i", "rhs_vis_j", "rhs_vis_k"]
+                "sweep_stage", "rhs_vis_i", "rhs_vis_j", "rhs_vis_k", "sa_rhs", "sa_sweep_l", "sa_sweep_u",
+                "sa_update"]
 
 
 class WcnsError(RuntimeError):
@@ -46,7 +48,8 @@ class Params(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("moo", "reynolds", "attack", "sideslip", "tref", "twall",
                                           "cfl", "timedt_rate", "efix", "csrv", "gama", "prl", "prt")] + \
                [(n, C.c_int) for n in ("nvis", "nlamtur", "nscheme", "nflux", "ntmst")] + \
-               [(n, C.c_double) for n in ("sref", "lfref", "xref", "yref", "zref")] + [("nwholefield", C.c_int)]
+               [(n, C.c_double) for n in ("sref", "lfref", "xref", "yref", "zref")] + [("nwholefield", C.c_int)] + \
+               [(n, C.c_double) for n in ("timedt_turb", "kmaxlim", "xk", "xb")]
 
 
 _lib = None
