@@ -1,6 +1,7 @@
 """Throughput of the WCNS pseudo-time step on one MI355X.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bump|cart] [--lhs fp64|fp32]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload bump|cart|bump-sa|mb64-sa]
+                  [--lhs fp64|fp32] [--launch graph|plain]
 
 Workloads (synthetic inputs generated by wcns_amd.cases, seeded):
   bump  256x128x64-node sinusoidal-bump channel (bump 0.05 L, wall spacing 1e-5 L,
@@ -15,9 +16,14 @@ residual, interface averaging, LU-SGS + SGS, limited update, interface
 primitive averaging and the residual monitor.  Mcell-iter/s = nodes x steps /
 seconds / 1e6, inputs resident in HBM before timing starts.
 
-The dominant kernel is timed live with HIP events recorded on the library's
-own stream over the timed region; its roofline entry is its algorithmic HBM
-bytes per launch over its mean launch time, against 8 TB/s.
+The timed steps are replayed from a captured HIP graph (--launch graph, the
+default) or launched one by one (--launch plain).  The per-kernel times come
+from a few further plain steps with HIP events recorded on the library's own
+stream; the dominant kernel's roofline entry is its algorithmic HBM bytes per
+launch over its mean launch time, against 8 TB/s.
+
+Extra workloads: bump-sa (the bump channel with the Spalart-Allmaras model) and
+mb64-sa (64 uneven blocks, sides 24-96, SA, about 30 M nodes; use --no-cpu).
 """
 import argparse
 import json
@@ -93,6 +99,15 @@ def build_case(workload):
     if workload == "bump":
         return cases.bump_channel_case(ni=256, nj=128, nk=64), \
             "bump-channel 256x128x64 laminar Re1e5 M0.5 Roe WCNS-E5"
+    if workload == "bump-sa":
+        case = cases.bump_channel_case(ni=256, nj=128, nk=64)
+        case.params.nlamtur = 1
+        return case, "bump-channel 256x128x64 Spalart-Allmaras Re1e5 M0.5 Roe WCNS-E5"
+    if workload == "mb64-sa":
+        case = cases.uneven_blocks_case(nblk=(4, 4, 4), lo=24, hi=96, seed=7, nvis=1, wall=True, nflux=3,
+                                        reynolds=3.0e6, moo=0.6)
+        case.params.nlamtur = 1
+        return case, f"64 uneven blocks (sides 24-96, seed 7) {case.nodes} nodes, SA Re3e6 M0.6 Roe WCNS-E5"
     if workload == "cart":
         return cases.cartesian_periodic_case(ni=256, nj=128, nk=64), \
             "cartesian-periodic 256x128x64 Euler M0.8 SW WCNS-E5"
@@ -127,6 +142,8 @@ def main():
     ap.add_argument("--workload", default="bump")
     ap.add_argument("--lhs", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--launch", default="graph", choices=["graph", "plain"],
+                    help="timed steps replayed from a captured HIP graph, or plain launches")
     ap.add_argument("--breakdown", action="store_true", help="print the per-kernel table to stderr")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters)")
@@ -151,20 +168,29 @@ def main():
     case, wl_name = build_case(args.workload)
     s = Solver(case, device=0, lhs_precision=1 if args.lhs == "fp32" else 0)
     s.step(args.warmup, history=False)
+    if args.launch == "graph":
+        s.step_graph(2)              # capture outside the clock
     s.sync()
     torch.cuda.synchronize()
 
-    s.timing(True)
-    s.sync()
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    s.step(args.steps, history=False)
+    if args.launch == "graph":
+        s.step_graph(args.steps)
+    else:
+        s.step(args.steps, history=False)
     s.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    rms, rmax, loc = s.residual_curve()
+
+    # per-kernel times: plain launches of the same step with HIP events on the
+    # library's stream (graph replays carry no events)
+    kt = max(2, min(args.steps, 10))
+    s.timing(True)
+    s.step(kt, history=False)
+    s.sync()
     tim = s.timings()
     s.timing(False)
-    rms, rmax, loc = s.residual_curve()
 
     sec = t1 - t0
     value = case.nodes * args.steps / sec / 1e6
@@ -178,7 +204,7 @@ def main():
         dms, dlaunch = priced[dom]
         # the launches of a slot in one step cover every node once (one per
         # block, or one batched launch for all blocks)
-        per_launch = alg_bytes_per_node(dom, lhs_bytes) * nodes * args.steps / dlaunch
+        per_launch = alg_bytes_per_node(dom, lhs_bytes) * nodes * kt / dlaunch
         avg_s = dms / dlaunch / 1e3
         ach = per_launch / avg_s / 1e9
         traffic, tsrc = measured_traffic(args.workload, args.lhs, dom)
@@ -188,7 +214,7 @@ def main():
     if args.breakdown:
         tot = sum(v[0] for v in tim.values())
         for k, (ms, n) in sorted(tim.items(), key=lambda kv: -kv[1][0]):
-            print(f"{k:12s} {ms / args.steps:9.3f} ms/step  {100 * ms / max(tot, 1e-9):5.1f}%  launches {n}",
+            print(f"{k:12s} {ms / kt:9.3f} ms/step  {100 * ms / max(tot, 1e-9):5.1f}%  launches {n}",
                   file=sys.stderr)
     cpu = None
     if not args.no_cpu:
@@ -200,10 +226,10 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.lhs == "fp64" else "f64+f32-lhs",
         "data": "synthetic (seeded generator wcns_amd.cases)",
         "config": {"workload": wl_name, "nodes": nodes, "blocks": len(case.blocks), "lhs": args.lhs,
-                   "residual_rms_end": rms},
+                   "launch": args.launch, "residual_rms_end": rms},
         "roofline": roof, "cpu_baseline": cpu,
         "step_hbm_frac": round(STEP_BYTES_PER_NODE * nodes * args.steps / sec / (HBM_PEAK_GBS * 1e9), 5),
-        "stage_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in tim.items()},
+        "stage_ms_per_step": {k: round(v[0] / kt, 4) for k, v in tim.items()},
     }
     print(json.dumps(line))
     s.close()

@@ -113,6 +113,13 @@ int wcns_residual(wcns_ctx *ctx, double *rms, double *resmax, int loc[5]);
  * step but only read back at the end (no per-step host synchronisation). */
 int wcns_step(wcns_ctx *ctx, int nsteps, double *rms_hist);
 
+/* The same nsteps steps as wcns_step(ctx, nsteps, NULL), replayed from a HIP
+ * graph that holds two whole steps (captured on first use, kept until the
+ * implicit-operator precision changes; an odd last step runs as plain
+ * launches).  No counterpart in the reference; the residual is not monitored
+ * inside (call wcns_residual after).  Synchronises the stream at the end. */
+int wcns_step_graph(wcns_ctx *ctx, int nsteps);
+
 /* Force and moment coefficients over every no-slip wall (bctype 2 or 2x):
  * out = cfx, cfy, cfz, cmx, cmy, cmz, cd, cl, xcp.  Replaces force + force_dif
  * (src/PostProcess.f90:1415-1503, 1505-1680); the quad quadrature is getforce

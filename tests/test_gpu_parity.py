@@ -383,3 +383,33 @@ def test_sa_multiblock_mixed_bcs_20_steps():
     print("SA multiblock residual history rel err max", rel.max(), flush=True)
     assert rel.max() <= 1e-10
     _cmp_all_blocks(s, o, case, ("r", "u", "v", "w", "p", "qke", "vist"), 1e-11)
+
+
+# ---------------------------------------------------------------- graph replay
+GRAPH_CASES = {
+    "bump-laminar": lambda: cases.bump_channel_case(ni=24, nj=16, nk=12, dy_wall=1e-3, growth=1.2, reynolds=1e4),
+    "sa-multiblock": _sa_mixed_bc,
+    "cart-euler": lambda: cases.cartesian_periodic_case(ni=24, nj=16, nk=12, seed=3, amp=0.02),
+}
+
+
+@pytest.mark.parametrize("name", list(GRAPH_CASES))
+def test_graph_replay_matches_plain_launches(name):
+    """wcns_step_graph replays a captured two-step graph: with plain steps and
+    host reads between the replays, every field equals the plain launches'."""
+    case = GRAPH_CASES[name]()
+    print(f"graph case {name}", flush=True)
+    a = _solver(case)
+    b = _solver(case)
+    a.step(9, history=False)
+    b.step_graph(4)
+    b.field(0, "r")                  # host read between replays
+    b.step(1, history=False)
+    b.step_graph(3)                  # odd: one graph pair + one plain step
+    b.step_graph(1)
+    names = ["r", "u", "v", "w", "p"] + (["qke", "vist"] if case.params.nlamtur == 1 else [])
+    for nb in range(len(case.blocks)):
+        for nm in names:
+            np.testing.assert_array_equal(a.field(nb, nm), b.field(nb, nm), err_msg=f"block {nb} {nm}")
+    ra, rb = a.residual_curve(), b.residual_curve()
+    assert ra[0] == rb[0] and ra[1] == rb[1]

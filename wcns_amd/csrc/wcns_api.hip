@@ -81,6 +81,8 @@ struct wcns_ctx {
     bool tiled_sweeps = true;
     unsigned long long *counters = nullptr;
     DBlock *d_blocks = nullptr;
+    BlockGrid grid[GRID_N] = {};            // block work-lists of the batched launches
+    int2 *d_grid = nullptr;
     int *d_flag = nullptr;
     int flag_cap = 512;
     double *d_partial = nullptr;
@@ -88,6 +90,8 @@ struct wcns_ctx {
     double *d_resblk = nullptr;
     double *d_hist = nullptr;
     int hist_cap = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   // two-step graphs, by starting primitive buffer
+    int graph_prec = -1;                            // implicit-operator precision they were captured with
 
     std::vector<DRegion> iface;              // interfaces in (nb, nr) order
     std::vector<DRegion> phys;               // bctype > 0 regions
@@ -400,7 +404,7 @@ int wcns_destroy(wcns_ctx *ctx)
     for (int m = 0; m < 3; m++) { hipFree(ctx->sr[m]); hipFree(ctx->srv[m]); }
     void *pl[] = {ctx->d_sblocks, ctx->d_work, ctx->d_xch};
     for (void *p : pl) if (p) hipFree(p);
-    void *ps[] = {ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
+    void *ps[] = {ctx->d_grid, ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
                   ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
@@ -408,6 +412,8 @@ int wcns_destroy(wcns_ctx *ctx)
     void *fs[] = {ctx->d_force_pn, ctx->d_force_part, ctx->d_force_out};
     for (void *p : fs) if (p) hipFree(p);
     for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
+    for (int c = 0; c < 2; c++)
+        if (ctx->graph[c]) hipGraphExecDestroy(ctx->graph[c]);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return WCNS_OK;
@@ -613,6 +619,26 @@ int wcns_setup(wcns_ctx *ctx)
     ctx->max_parts = maxp;
     TRY(dalloc(ctx, &ctx->d_blocks, hb.size()));
     TRY(copy_sync(ctx, ctx->d_blocks, hb.data(), sizeof(DBlock) * hb.size(), hipMemcpyHostToDevice));
+    // block work-lists: every block's workgroups of a launch kind, block after block
+    {
+        std::vector<int2> all;
+        size_t start[GRID_N];
+        for (int kind = 0; kind < GRID_N; kind++) {
+            start[kind] = all.size();
+            const long bs = grid_block_size(kind);
+            for (size_t nb = 0; nb < hb.size(); nb++) {
+                const long nw = (grid_threads(kind, hb[nb]) + bs - 1) / bs;
+                for (long w = 0; w < nw; w++) all.push_back(make_int2((int)nb, (int)w));
+            }
+        }
+        TRY(dalloc(ctx, &ctx->d_grid, std::max<size_t>(all.size(), 1)));
+        if (!all.empty())
+            TRY(copy_sync(ctx, ctx->d_grid, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice));
+        for (int kind = 0; kind < GRID_N; kind++) {
+            const size_t end = kind + 1 < GRID_N ? start[kind + 1] : all.size();
+            ctx->grid[kind] = BlockGrid{ctx->d_blocks, ctx->d_grid + start[kind], (int)(end - start[kind])};
+        }
+    }
     TRY(dalloc(ctx, &ctx->d_partial, 3 * maxp));
     TRY(dalloc(ctx, &ctx->d_resblk, 3 * ctx->blocks.size()));
 
@@ -884,8 +910,8 @@ int wcns_initial_q_dq_c_t(wcns_ctx *ctx)
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     DFields F = ctx->fields();
     Scope sc(ctx, WCNS_T_INIT);
-    for (auto &b : ctx->blocks)
-        if (launch_init_stage(b.db, F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "init launch");
+    if (launch_init_stage(ctx->grid[GRID_NODES], &ctx->grid[GRID_GHOST_T], F, ctx->dp, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "init launch");
     return WCNS_OK;
 }
 
@@ -894,11 +920,9 @@ int wcns_timestep(wcns_ctx *ctx)
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     DFields F = ctx->fields();
     Scope sc(ctx, WCNS_T_TIMESTEP);
-    for (auto &b : ctx->blocks) {
-        if (ctx->prm.nvis == 1 && ctx->prm.csrv > 0.0 && launch_spectvis(b.db, F, ctx->dp, ctx->stream))
-            return fail(ctx, WCNS_E_HIP, "spectvis launch");
-        if (launch_timestep(b.db, F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "timestep launch");
-    }
+    if (ctx->prm.nvis == 1 && ctx->prm.csrv > 0.0 && launch_spectvis(ctx->grid[GRID_NODES], F, ctx->dp, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "spectvis launch");
+    if (launch_timestep(ctx->grid[GRID_NODES], F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "timestep launch");
     return WCNS_OK;
 }
 
@@ -911,27 +935,27 @@ int wcns_r_h_s_vis(wcns_ctx *ctx)
         // RHS.f90:22: the turbulence model advances first; its update (nu~ on 2..n-1,
         // mu_t on 0..n) only touches the block's own nodes, so all blocks go together
         {
-            Scope sc(ctx, WCNS_T_SA_RHS, 2 * (int)ctx->blocks.size());
-            for (auto &b : ctx->blocks)
-                if (launch_sa_rhs(b.db, F, ctx->sa, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "SA rhs launch");
+            Scope sc(ctx, WCNS_T_SA_RHS, 2);
+            if (launch_sa_rhs(ctx->grid[GRID_NODES], ctx->grid[GRID_SA_RHS], F, ctx->sa, ctx->dp, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "SA rhs launch");
         }
         for (int fwd = 1; fwd >= 0; fwd--) {
             Scope sc(ctx, fwd ? WCNS_T_SA_SWEEP_L : WCNS_T_SA_SWEEP_U);
             if (launch_sa_sweep(fwd, ctx->plan, ctx->sa, ctx->stream)) return fail(ctx, WCNS_E_HIP, "SA sweep launch");
         }
-        Scope sc(ctx, WCNS_T_SA_UPDATE, (int)ctx->blocks.size());
-        for (auto &b : ctx->blocks)
-            if (launch_sa_update(b.db, F, ctx->sa, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "SA update launch");
+        Scope sc(ctx, WCNS_T_SA_UPDATE);
+        if (launch_sa_update(ctx->grid[GRID_SA_UPD], F, ctx->sa, ctx->dp, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "SA update launch");
     }
     {
-        Scope sc(ctx, WCNS_T_VIS, (int)ctx->blocks.size());
-        for (auto &b : ctx->blocks)
-            if (launch_vis_node_deriv(b.db, F, ctx->dn, ctx->stream)) return fail(ctx, WCNS_E_HIP, "vis deriv launch");
+        Scope sc(ctx, WCNS_T_VIS);
+        if (launch_vis_node_deriv(ctx->grid[GRID_NODES], F, ctx->dn, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "vis deriv launch");
     }
     for (int dir = 0; dir < 3; dir++) {
-        Scope sc(ctx, WCNS_T_VIS_I + dir, (int)ctx->blocks.size());
-        for (auto &b : ctx->blocks)
-            if (launch_rhs_vis_dir(dir, b.db, F, ctx->dp, ctx->dn, ctx->stream)) return fail(ctx, WCNS_E_HIP, "vis launch");
+        Scope sc(ctx, WCNS_T_VIS_I + dir);
+        if (launch_rhs_vis_dir(dir, ctx->grid[GRID_RHS_VIS + dir], F, ctx->dp, ctx->dn, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "vis launch");
     }
     return WCNS_OK;
 }
@@ -941,9 +965,9 @@ int wcns_r_h_s_invis(wcns_ctx *ctx)
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     DFields F = ctx->fields();
     for (int dir = 0; dir < 3; dir++) {
-        Scope sc(ctx, WCNS_T_INV_I + dir, (int)ctx->blocks.size());
-        for (auto &b : ctx->blocks)
-            if (launch_rhs_inv_dir(dir, b.db, F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "rhs launch");
+        Scope sc(ctx, WCNS_T_INV_I + dir);
+        if (launch_rhs_inv_dir(dir, ctx->grid[GRID_RHS_INV + dir], F, ctx->dp, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "rhs launch");
     }
     return WCNS_OK;
 }
@@ -1014,7 +1038,7 @@ int wcns_l_h_s(wcns_ctx *ctx)
                 if (F.dq32) launch_dq_zero_face32(g, db, F.dq32, F.total, ctx->stream);
                 else launch_dq_zero_face(g, db, F.dq, F.total, ctx->stream);
             }
-        CK(hipMemsetAsync(ctx->counters + 3, 0, sizeof(unsigned long long), ctx->stream));
+        if (launch_fill_u64(ctx->counters + 3, 1, 0ull, ctx->stream)) return fail(ctx, WCNS_E_HIP, "counter reset");
         Scope sc(ctx, WCNS_T_UPDATE);
         if (launch_update(db, F, ctx->dp, ctx->d_flag, ctx->flag_cap, ctx->stream)) return fail(ctx, WCNS_E_HIP, "update");
         if (launch_repair(db, F, ctx->dp, ctx->d_flag, ctx->flag_cap, ctx->stream)) return fail(ctx, WCNS_E_HIP, "repair");
@@ -1062,22 +1086,84 @@ int wcns_residual(wcns_ctx *ctx, double *rms, double *resmax, int loc[5])
     return WCNS_OK;
 }
 
+}  // extern "C"
+
+// The launches of one pseudo-time step (Main.f90:85-109), no host wait.
+static int step_once(wcns_ctx *ctx)
+{
+    TRY(wcns_boundary_all(ctx));
+    TRY(wcns_initial_q_dq_c_t(ctx));
+    TRY(wcns_timestep(ctx));
+    if (ctx->prm.nvis == 1) {
+        TRY(wcns_r_h_s_vis(ctx));
+        TRY(wcns_communicate_dq(ctx));
+    }
+    TRY(wcns_r_h_s_invis(ctx));
+    TRY(wcns_communicate_dq(ctx));
+    TRY(wcns_l_h_s(ctx));
+    TRY(wcns_communicate_pv(ctx));
+    return WCNS_OK;
+}
+
+// Captures two consecutive steps (the primitive buffers alternate, so a pair
+// returns to the buffer it started from) into an executable graph.
+static int capture_pair(wcns_ctx *ctx, hipGraphExec_t *out)
+{
+    const bool tm = ctx->timing;
+    const int c0 = ctx->cur;
+    ctx->timing = false;
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        ctx->timing = tm;
+        return fail(ctx, WCNS_E_HIP, "hipStreamBeginCapture failed");
+    }
+    int e = step_once(ctx);
+    if (!e) e = step_once(ctx);
+    hipError_t ee = hipStreamEndCapture(ctx->stream, &g);
+    ctx->timing = tm;
+    ctx->cur = c0;
+    if (e || ee != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return e ? e : fail(ctx, WCNS_E_HIP, std::string("stream capture: ") + hipGetErrorString(ee));
+    }
+    hipGraphExec_t x = nullptr;
+    hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ie != hipSuccess) return fail(ctx, WCNS_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+    *out = x;
+    return WCNS_OK;
+}
+
+static void drop_graphs(wcns_ctx *ctx)
+{
+    for (int c = 0; c < 2; c++)
+        if (ctx->graph[c]) { hipGraphExecDestroy(ctx->graph[c]); ctx->graph[c] = nullptr; }
+}
+
+extern "C" {
+
+int wcns_step_graph(wcns_ctx *ctx, int nsteps)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (nsteps < 0) return WCNS_E_ARG;
+    if (ctx->graph_prec != ctx->lhs_prec) { drop_graphs(ctx); ctx->graph_prec = ctx->lhs_prec; }
+    int done = 0;
+    for (; nsteps - done >= 2; done += 2) {
+        hipGraphExec_t &ge = ctx->graph[ctx->cur];
+        if (!ge) TRY(capture_pair(ctx, &ge));
+        CK(hipGraphLaunch(ge, ctx->stream));
+    }
+    if (done < nsteps) TRY(step_once(ctx));
+    CK(hipStreamSynchronize(ctx->stream));
+    return check_counters(ctx);
+}
+
 int wcns_step(wcns_ctx *ctx, int nsteps, double *rms_hist)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     if (nsteps < 0) return WCNS_E_ARG;
     for (int s = 0; s < nsteps; s++) {
-        TRY(wcns_boundary_all(ctx));
-        TRY(wcns_initial_q_dq_c_t(ctx));
-        TRY(wcns_timestep(ctx));
-        if (ctx->prm.nvis == 1) {
-            TRY(wcns_r_h_s_vis(ctx));
-            TRY(wcns_communicate_dq(ctx));
-        }
-        TRY(wcns_r_h_s_invis(ctx));
-        TRY(wcns_communicate_dq(ctx));
-        TRY(wcns_l_h_s(ctx));
-        TRY(wcns_communicate_pv(ctx));
+        TRY(step_once(ctx));
         if (rms_hist) {
             double rms, rmax;
             int loc[5];

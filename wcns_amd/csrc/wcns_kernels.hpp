@@ -34,8 +34,34 @@ __host__ __device__ inline void region_image(const DRegion &g, int i, int j, int
     kt = g.t0[2] + g.M[6] * d0 + g.M[7] * d1 + g.M[8] * d2;
 }
 
-int launch_rhs_inv(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
-int launch_rhs_inv_dir(int dir, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
+// All blocks in one launch: workgroup w does the work that workgroup wg[w].y
+// of a launch over block wg[w].x alone would do.  One map per kind of launch
+// (the number of threads per block differs); blocks with no work have no entry.
+struct BlockGrid {
+    const DBlock *blocks;     // device copy of every block
+    const int2 *wg;           // per workgroup: (block, workgroup within the block)
+    int nwg;
+};
+__device__ inline void grid_thread(const BlockGrid &G, DBlock &b, long &t)
+{
+    const int2 w = G.wg[blockIdx.x];
+    b = G.blocks[w.x];
+    t = (long)w.y * blockDim.x + threadIdx.x;
+}
+constexpr int PENCIL_SEG = 32;    // rows per thread of the pencil kernels
+enum {
+    GRID_NODES = 0,               // nodes 1..n, 256 threads per workgroup
+    GRID_GHOST_T = 1,             // + dir: ghost temperatures of the i, j, k faces
+    GRID_RHS_VIS = 4,             // + dir: viscous pencil segments
+    GRID_RHS_INV = 7,             // + dir: inviscid pencil segments
+    GRID_SA_RHS = 10,             // SA nodes 2..n-1
+    GRID_SA_UPD = 11,             // SA nodes 0..n
+    GRID_N = 12
+};
+long grid_threads(int kind, const DBlock &b);
+int grid_block_size(int kind);
+
+int launch_rhs_inv_dir(int dir, const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s);
 // The pipelined sweeps of all blocks run in one persistent launch per sweep:
 // every block's (j,k) columns are cut into TJ x TK tiles, one workgroup per
 // tile, and the workgroups follow a work list that is a topological order of
@@ -60,6 +86,8 @@ long sweep_exchange_words(int TJ, int TK, int nTJ, int nTK, int ni);
 // arithmetic result has.  Every slot is reset to it before each sweep.
 constexpr unsigned long long XCH_EMPTY = 0x7ff4deadbeef0001ull;
 int launch_fill_xch(const SweepPlan &S, hipStream_t s);
+// fills n words with v by a kernel (no memset node, so it may sit in a captured graph)
+int launch_fill_u64(unsigned long long *a, long n, unsigned long long v, hipStream_t s);
 // Plane-major staging copy of the sweep operands: arrays of skew_size(b)
 // elements per block (element (i,j,k), i,j,k in 0..n+1, at (i+j+k)*(nj+2)*(nk+2)
 // + j + (nj+2)*k), SK_N arrays of sktot elements each, double or float (prec).
@@ -73,13 +101,14 @@ int launch_sweep_batched(int kind, const SweepPlan &S, const DFields &F, const D
 int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
 int launch_dq_zero_face32(const DRegion &g, const DBlock &b, float *arr, long total, hipStream_t s);
 
-int launch_vis_node_deriv(const DBlock &b, const DFields &F, double *dn, hipStream_t s);
-int launch_rhs_vis_dir(int dir, const DBlock &b, const DFields &F, const DParams &P, const double *dn, hipStream_t s);
-int launch_spectvis(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
+int launch_vis_node_deriv(const BlockGrid &G, const DFields &F, double *dn, hipStream_t s);
+int launch_rhs_vis_dir(int dir, const BlockGrid &G, const DFields &F, const DParams &P, const double *dn, hipStream_t s);
+int launch_spectvis(const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s);
 
 int launch_bc_region(const DRegion &g, const DBlock *blocks, const DFields &F, const DParams &P, hipStream_t s);
-int launch_init_stage(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
-int launch_timestep(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
+int launch_init_stage(const BlockGrid &nodes, const BlockGrid *ghost_t, const DFields &F, const DParams &P,
+                      hipStream_t s);
+int launch_timestep(const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s);
 int launch_zero_rhs(const DBlock &b, const DFields &F, hipStream_t s);
 int launch_dq_zero_face(const DRegion &g, const DBlock &b, double *arr, long total, hipStream_t s);
 int launch_npp_gather(const int *pts, int npts, const DBlock *dblocks, const DFields &F, double *buf,
@@ -105,9 +134,10 @@ struct SaWork {
 int launch_wall_dist(const DBlock &b, const double *x, const double *y, const double *z, const double *wall, int nw,
                      double *dist, hipStream_t s);
 int launch_sa_init(const DBlock &b, const DFields &F, double muoo, double vt, hipStream_t s);
-int launch_sa_rhs(const DBlock &b, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s);
+int launch_sa_rhs(const BlockGrid &nodes, const BlockGrid &inner, const DFields &F, const SaWork &W,
+                  const DParams &P, hipStream_t s);
 int launch_sa_sweep(int fwd, const SweepPlan &S, const SaWork &W, hipStream_t s);
-int launch_sa_update(const DBlock &b, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s);
+int launch_sa_update(const BlockGrid &G, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s);
 
 // wall force
 long force_quads(const DRegion &g);

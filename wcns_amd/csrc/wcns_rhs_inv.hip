@@ -6,7 +6,8 @@
 // Mapping: one thread marches one segment of one pencil, keeping the last
 // five nodes (5 primitives each), four metric triples and six face fluxes in
 // registers.  Adjacent threads own adjacent pencils, so for j- and k-pencils
-// every load is coalesced along i.  A segment of SEG rows recomputes the
+// every load is coalesced along i.  All blocks go in one launch over a block
+// work-list (BlockGrid).  A segment of SEG rows recomputes the
 // (at most) 3+3 faces it shares with its neighbours; the segment that owns
 // rows 1..3 (resp. n-2..n) also applies the WCNS_E_5_41 boundary rows.
 #include "wcns_kernels.hpp"
@@ -15,7 +16,7 @@ namespace wcns {
 
 namespace {
 
-constexpr int SEG = 32;
+constexpr int SEG = PENCIL_SEG;
 
 struct Pencil {
     const double *r, *u, *v, *w, *p;
@@ -240,13 +241,16 @@ __device__ __noinline__ void closure_state(const Pencil &pc, int which, double *
 }
 
 template <int DIR, int FLUX>
-__global__ void __launch_bounds__(128) k_rhs_inv(DBlock b, DFields F, DParams P, int nseg)
+__global__ void __launch_bounds__(128) k_rhs_inv(BlockGrid G, DFields F, DParams P)
 {
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
     const int n = DIR == 0 ? b.ni : (DIR == 1 ? b.nj : b.nk);
     const int na = DIR == 0 ? b.nj : b.ni;              // fast transverse extent
     const int nb = DIR == 2 ? b.nj : b.nk;              // slow transverse extent
     const long npen = (long)na * nb;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nseg = (n + SEG - 1) / SEG;
     if (t >= npen * nseg) return;
     int seg = (int)(t / npen);
     long pen = t - (long)seg * npen;
@@ -415,31 +419,30 @@ __global__ void __launch_bounds__(128) k_rhs_inv(DBlock b, DFields F, DParams P,
 
 }  // namespace
 
-int launch_rhs_inv_dir(int dir, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
+// pencil segments of one block along dir; none when the direction has <= 4
+// nodes (Convect.f90:150,314,479)
+long rhs_inv_threads(int dir, const DBlock &b)
 {
     int n = dir == 0 ? b.ni : (dir == 1 ? b.nj : b.nk);
-    if (n <= 4) return 0;                               // Convect.f90:150,314,479
+    if (n <= 4) return 0;
     long npen = dir == 0 ? (long)b.nj * b.nk : (dir == 1 ? (long)b.ni * b.nk : (long)b.ni * b.nj);
-    int nseg = (n + SEG - 1) / SEG;
-    long nthr = npen * nseg;
-    dim3 grid((unsigned)((nthr + 127) / 128)), blk(128);
-    if (P.nflux == 3) {
-        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 3>), grid, blk, 0, s, b, F, P, nseg);
-        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 3>), grid, blk, 0, s, b, F, P, nseg);
-        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 3>), grid, blk, 0, s, b, F, P, nseg);
-    } else {
-        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 1>), grid, blk, 0, s, b, F, P, nseg);
-        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 1>), grid, blk, 0, s, b, F, P, nseg);
-        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 1>), grid, blk, 0, s, b, F, P, nseg);
-    }
-    return hipGetLastError() == hipSuccess ? 0 : -3;
+    return npen * ((n + SEG - 1) / SEG);
 }
 
-int launch_rhs_inv(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
+int launch_rhs_inv_dir(int dir, const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s)
 {
-    for (int dir = 0; dir < 3; dir++)
-        if (launch_rhs_inv_dir(dir, b, F, P, s)) return -3;
-    return 0;
+    if (G.nwg == 0) return 0;
+    dim3 grid((unsigned)G.nwg), blk(128);
+    if (P.nflux == 3) {
+        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 3>), grid, blk, 0, s, G, F, P);
+        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 3>), grid, blk, 0, s, G, F, P);
+        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 3>), grid, blk, 0, s, G, F, P);
+    } else {
+        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 1>), grid, blk, 0, s, G, F, P);
+        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 1>), grid, blk, 0, s, G, F, P);
+        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 1>), grid, blk, 0, s, G, F, P);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 }  // namespace wcns

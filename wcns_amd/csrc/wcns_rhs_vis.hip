@@ -17,7 +17,7 @@ namespace wcns {
 
 namespace {
 
-constexpr int SEG = 32;
+constexpr int SEG = PENCIL_SEG;
 
 struct VLine {
     long base, stride;
@@ -72,10 +72,12 @@ __device__ inline VLine line_of(const DBlock &b, int dir, int i, int j, int k)
 }
 
 // dn layout: component c (0..11 = 4*dir + {u,v,w,T}) of node g at dn[c*total + g]
-__global__ void k_vis_node_deriv(DBlock b, DFields F, double *dn)
+__global__ void k_vis_node_deriv(BlockGrid G, DFields F, double *dn)
 {
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
     long n = (long)b.ni * b.nj * b.nk;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
     long g = gidx(b, i, j, k);
@@ -194,13 +196,16 @@ __device__ void face_visc(const DFields &F, const DParams &P, const double *dn, 
 }
 
 template <int DIR>
-__global__ void __launch_bounds__(128) k_rhs_vis(DBlock b, DFields F, DParams P, const double *dn, int nseg)
+__global__ void __launch_bounds__(128) k_rhs_vis(BlockGrid G, DFields F, DParams P, const double *dn)
 {
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
     const int n = DIR == 0 ? b.ni : (DIR == 1 ? b.nj : b.nk);
     const int na = DIR == 0 ? b.nj : b.ni;
     const int nbk = DIR == 2 ? b.nj : b.nk;
     const long npen = (long)na * nbk;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nseg = (n + SEG - 1) / SEG;
     if (t >= npen * nseg) return;
     int seg = (int)(t / npen);
     long pen = t - (long)seg * npen;
@@ -277,10 +282,12 @@ __global__ void __launch_bounds__(128) k_rhs_vis(DBlock b, DFields F, DParams P,
 
 // LHS.f90:243-343 spectvisl (laminar) / LHS.f90:141-241 spectvist (mu + mu_t), interior
 // nodes (ghost-layer radii only ever multiply a zero dq)
-__global__ void k_spectvis(DBlock b, DFields F, DParams P)
+__global__ void k_spectvis(BlockGrid G, DFields F, DParams P)
 {
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
     long n = (long)b.ni * b.nj * b.nk;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
     long g = gidx(b, i, j, k);
@@ -299,33 +306,37 @@ __global__ void k_spectvis(DBlock b, DFields F, DParams P)
     F.srvc[g] = coef1 * rk;
 }
 
-inline dim3 grid1(long n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)); }
 
 }  // namespace
 
-int launch_vis_node_deriv(const DBlock &b, const DFields &F, double *dn, hipStream_t s)
+int launch_vis_node_deriv(const BlockGrid &G, const DFields &F, double *dn, hipStream_t s)
 {
-    long n = (long)b.ni * b.nj * b.nk;
-    hipLaunchKernelGGL(k_vis_node_deriv, grid1(n, 256), dim3(256), 0, s, b, F, dn);
+    if (G.nwg == 0) return 0;
+    hipLaunchKernelGGL(k_vis_node_deriv, dim3(G.nwg), dim3(256), 0, s, G, F, dn);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_rhs_vis_dir(int dir, const DBlock &b, const DFields &F, const DParams &P, const double *dn, hipStream_t s)
+long rhs_vis_threads(int dir, const DBlock &b)
 {
     int n = dir == 0 ? b.ni : (dir == 1 ? b.nj : b.nk);
     long npen = dir == 0 ? (long)b.nj * b.nk : (dir == 1 ? (long)b.ni * b.nk : (long)b.ni * b.nj);
-    int nseg = (n + SEG - 1) / SEG;
-    dim3 grid = grid1(npen * nseg, 128), blk(128);
-    if (dir == 0) hipLaunchKernelGGL(k_rhs_vis<0>, grid, blk, 0, s, b, F, P, dn, nseg);
-    if (dir == 1) hipLaunchKernelGGL(k_rhs_vis<1>, grid, blk, 0, s, b, F, P, dn, nseg);
-    if (dir == 2) hipLaunchKernelGGL(k_rhs_vis<2>, grid, blk, 0, s, b, F, P, dn, nseg);
+    return npen * ((n + SEG - 1) / SEG);
+}
+
+int launch_rhs_vis_dir(int dir, const BlockGrid &G, const DFields &F, const DParams &P, const double *dn, hipStream_t s)
+{
+    if (G.nwg == 0) return 0;
+    dim3 grid(G.nwg), blk(128);
+    if (dir == 0) hipLaunchKernelGGL(k_rhs_vis<0>, grid, blk, 0, s, G, F, P, dn);
+    if (dir == 1) hipLaunchKernelGGL(k_rhs_vis<1>, grid, blk, 0, s, G, F, P, dn);
+    if (dir == 2) hipLaunchKernelGGL(k_rhs_vis<2>, grid, blk, 0, s, G, F, P, dn);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_spectvis(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
+int launch_spectvis(const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s)
 {
-    long n = (long)b.ni * b.nj * b.nk;
-    hipLaunchKernelGGL(k_spectvis, grid1(n, 256), dim3(256), 0, s, b, F, P);
+    if (G.nwg == 0) return 0;
+    hipLaunchKernelGGL(k_spectvis, dim3(G.nwg), dim3(256), 0, s, G, F, P);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -105,10 +105,12 @@ __global__ void k_sa_init(DBlock b, DFields F, double muoo, double vt)
 
 // ------------------------------------------------------------- node pass
 // gradient array m of variable v (u, v, w, nu~) at component c: grad[(3v+c)*total + g]
-__global__ void k_sa_node(DBlock b, DFields F, SaWork W)
+__global__ void k_sa_node(BlockGrid G, DFields F, SaWork W)
 {
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
     const long n = (long)b.ni * b.nj * b.nk;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i, j, k;
     node_of(b, t, 1, b.ni, b.nj, i, j, k);
@@ -224,11 +226,13 @@ __device__ inline double sa_nnd(const DBlock &b, const DFields &F, const DParams
     return fc[1] - fc[0];
 }
 
-__global__ void __launch_bounds__(128) k_sa_rhs(DBlock b, DFields F, SaWork W, DParams P)
+__global__ void __launch_bounds__(128) k_sa_rhs(BlockGrid G, DFields F, SaWork W, DParams P)
 {
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
     const int mi = b.ni - 2, mj = b.nj - 2;
     const long n = (long)mi * mj * (b.nk - 2);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i, j, k;
     node_of(b, t, 2, mi, mj, i, j, k);
@@ -446,10 +450,12 @@ __global__ void __launch_bounds__(64) k_sa_sweep(const SweepBlock *blocks, const
 }
 
 // ---------------------------------------------------------------- update
-__global__ void k_sa_update(DBlock b, DFields F, SaWork W, DParams P)
+__global__ void k_sa_update(BlockGrid G, DFields F, SaWork W, DParams P)
 {
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
     const long n = (long)(b.ni + 1) * (b.nj + 1) * (b.nk + 1);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i, j, k;
     node_of(b, t, 0, b.ni + 1, b.nj + 1, i, j, k);
@@ -492,12 +498,11 @@ int launch_sa_init(const DBlock &b, const DFields &F, double muoo, double vt, hi
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_sa_rhs(const DBlock &b, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s)
+int launch_sa_rhs(const BlockGrid &nodes, const BlockGrid &inner, const DFields &F, const SaWork &W,
+                  const DParams &P, hipStream_t s)
 {
-    long n = (long)b.ni * b.nj * b.nk;
-    hipLaunchKernelGGL(k_sa_node, grid1(n, 256), dim3(256), 0, s, b, F, W);
-    long m = (long)(b.ni - 2) * (b.nj - 2) * (b.nk - 2);
-    if (m > 0) hipLaunchKernelGGL(k_sa_rhs, grid1(m, 128), dim3(128), 0, s, b, F, W, P);
+    if (nodes.nwg) hipLaunchKernelGGL(k_sa_node, dim3(nodes.nwg), dim3(256), 0, s, nodes, F, W);
+    if (inner.nwg) hipLaunchKernelGGL(k_sa_rhs, dim3(inner.nwg), dim3(128), 0, s, inner, F, W, P);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -511,10 +516,9 @@ int launch_sa_sweep(int fwd, const SweepPlan &S, const SaWork &W, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_sa_update(const DBlock &b, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s)
+int launch_sa_update(const BlockGrid &G, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s)
 {
-    long n = (long)(b.ni + 1) * (b.nj + 1) * (b.nk + 1);
-    hipLaunchKernelGGL(k_sa_update, grid1(n, 256), dim3(256), 0, s, b, F, W, P);
+    if (G.nwg) hipLaunchKernelGGL(k_sa_update, dim3(G.nwg), dim3(256), 0, s, G, F, W, P);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

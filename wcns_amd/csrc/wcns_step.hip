@@ -198,10 +198,12 @@ __global__ void k_bc_region(DRegion g, DBlock b, DBlock bt, DFields F, DParams P
 // set_dq_to_0 (PreProcess.f90:1189-1205) on the residual; with nvis == 1 also
 // get_c_t_only's temperature (RHS.f90:307-409) and compute_visl_ns
 // (GasModel.f90:219-235).
-__global__ void k_init_interior(DBlock b, DFields F, DParams P)
+__global__ void k_init_interior(BlockGrid G, DFields F, DParams P)
 {
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
     long n = (long)b.ni * b.nj * b.nk;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
     long g = gidx(b, i, j, k);
@@ -216,11 +218,13 @@ __global__ void k_init_interior(DBlock b, DFields F, DParams P)
     }
 }
 
-__global__ void k_init_ghost_t(DBlock b, DFields F, DParams P, int dir)
+__global__ void k_init_ghost_t(BlockGrid G, DFields F, DParams P, int dir)
 {
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
     int na = dir == 0 ? b.nj : b.ni, nb = dir == 2 ? b.nj : b.nk;
     int nd = dir == 0 ? b.ni : (dir == 1 ? b.nj : b.nk);
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 2L * na * nb) return;
     int side = (int)(t % 2);
     long r = t / 2;
@@ -242,10 +246,12 @@ __global__ void k_init_ghost_t(DBlock b, DFields F, DParams P, int dir)
 
 // ---------------------------------------------------- spectral radius, dt
 // Convect.f90:1338-1461 spectinv (interior) + LHS.f90:1343-1377 localdt0
-__global__ void k_timestep(DBlock b, DFields F, DParams P)
+__global__ void k_timestep(BlockGrid G, DFields F, DParams P)
 {
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
     long n = (long)b.ni * b.nj * b.nk;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
     long g = gidx(b, i, j, k);
@@ -556,24 +562,46 @@ int launch_bc_region(const DRegion &g, const DBlock *blocks, const DFields &F, c
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_init_stage(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
+int launch_init_stage(const BlockGrid &nodes, const BlockGrid *ghost_t, const DFields &F, const DParams &P,
+                      hipStream_t s)
 {
-    long n = (long)b.ni * b.nj * b.nk;
-    hipLaunchKernelGGL(k_init_interior, grid1(n, 256), dim3(256), 0, s, b, F, P);
+    if (nodes.nwg) hipLaunchKernelGGL(k_init_interior, dim3(nodes.nwg), dim3(256), 0, s, nodes, F, P);
     if (P.nvis == 1) {
-        for (int dir = 0; dir < 3; dir++) {
-            long na = dir == 0 ? b.nj : b.ni, nb = dir == 2 ? b.nj : b.nk;
-            hipLaunchKernelGGL(k_init_ghost_t, grid1(2 * na * nb, 128), dim3(128), 0, s, b, F, P, dir);
-        }
+        for (int dir = 0; dir < 3; dir++)
+            if (ghost_t[dir].nwg)
+                hipLaunchKernelGGL(k_init_ghost_t, dim3(ghost_t[dir].nwg), dim3(128), 0, s, ghost_t[dir], F, P, dir);
     }
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_timestep(const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
+int launch_timestep(const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s)
 {
-    long n = (long)b.ni * b.nj * b.nk;
-    hipLaunchKernelGGL(k_timestep, grid1(n, 256), dim3(256), 0, s, b, F, P);
+    if (G.nwg) hipLaunchKernelGGL(k_timestep, dim3(G.nwg), dim3(256), 0, s, G, F, P);
     return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+long rhs_inv_threads(int dir, const DBlock &b);
+long rhs_vis_threads(int dir, const DBlock &b);
+
+long grid_threads(int kind, const DBlock &b)
+{
+    const long n = (long)b.ni * b.nj * b.nk;
+    if (kind == GRID_NODES) return n;
+    if (kind >= GRID_GHOST_T && kind < GRID_GHOST_T + 3) {
+        const int dir = kind - GRID_GHOST_T;
+        const long na = dir == 0 ? b.nj : b.ni, nb = dir == 2 ? b.nj : b.nk;
+        return 2 * na * nb;
+    }
+    if (kind >= GRID_RHS_VIS && kind < GRID_RHS_VIS + 3) return rhs_vis_threads(kind - GRID_RHS_VIS, b);
+    if (kind >= GRID_RHS_INV && kind < GRID_RHS_INV + 3) return rhs_inv_threads(kind - GRID_RHS_INV, b);
+    if (kind == GRID_SA_RHS) return b.ni > 2 && b.nj > 2 && b.nk > 2 ? (long)(b.ni - 2) * (b.nj - 2) * (b.nk - 2) : 0;
+    if (kind == GRID_SA_UPD) return (long)(b.ni + 1) * (b.nj + 1) * (b.nk + 1);
+    return 0;
+}
+
+int grid_block_size(int kind)
+{
+    return (kind == GRID_NODES || kind == GRID_SA_UPD) ? 256 : 128;
 }
 
 int launch_zero_rhs(const DBlock &b, const DFields &F, hipStream_t s)

@@ -834,11 +834,14 @@ static int launch_pipe(int kind, const SweepPlan &S, const DFields &F, const DPa
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_fill_xch(const SweepPlan &S, hipStream_t s)
+int launch_fill_u64(unsigned long long *a, long n, unsigned long long v, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fill_u64, grid1(S.nxch, 256), dim3(256), 0, s, S.xch, S.nxch, XCH_EMPTY);
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_fill_u64, grid1(n, 256), dim3(256), 0, s, a, n, v);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
+
+int launch_fill_xch(const SweepPlan &S, hipStream_t s) { return launch_fill_u64(S.xch, S.nxch, XCH_EMPTY, s); }
 
 int launch_sweep_batched(int kind, const SweepPlan &S, const DFields &F, const DParams &P, int prec, void *pool,
                          long sktot, hipStream_t s)

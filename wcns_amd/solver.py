@@ -30,7 +30,8 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_set_lhs_precision", "wcns_set_prim", "wcns_get_field", "wcns_set_field", "wcns_boundary_all",
            "wcns_initial_q_dq_c_t", "wcns_timestep", "wcns_r_h_s_vis", "wcns_r_h_s_invis", "wcns_communicate_dq",
            "wcns_l_h_s", "wcns_communicate_pv", "wcns_residual", "wcns_step", "wcns_get_counter", "wcns_sync",
-           "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get", "wcns_force"]
+           "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get", "wcns_force",
+           "wcns_step_graph"]
 
 TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
@@ -86,6 +87,7 @@ def load_library(path=LIBPATH):
     L.wcns_timing.argtypes = [vp, C.c_int]
     L.wcns_timing_get.argtypes = [vp, C.c_int, dptr, C.POINTER(C.c_longlong)]
     L.wcns_force.argtypes = [vp, dptr]
+    L.wcns_step_graph.argtypes = [vp, C.c_int]
     _lib = L
     return L
 
@@ -186,6 +188,10 @@ class Solver:
             return hist[:n]
         self._check(self.L.wcns_step(self.h, n, None), "step")
         return None
+
+    def step_graph(self, n=1):
+        """n whole steps replayed from a captured HIP graph (wcns_step_graph)."""
+        self._check(self.L.wcns_step_graph(self.h, n), "step_graph")
 
     def sync(self):
         self._check(self.L.wcns_sync(self.h))
