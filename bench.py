@@ -142,6 +142,7 @@ def main():
     ap.add_argument("--workload", default="bump")
     ap.add_argument("--lhs", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default sweep tile shape")
     ap.add_argument("--launch", default="graph", choices=["graph", "plain"],
                     help="timed steps replayed from a captured HIP graph, or plain launches")
     ap.add_argument("--breakdown", action="store_true", help="print the per-kernel table to stderr")
@@ -168,6 +169,9 @@ def main():
     case, wl_name = build_case(args.workload)
     s = Solver(case, device=0, lhs_precision=1 if args.lhs == "fp32" else 0)
     s.step(args.warmup, history=False)
+    tile = None
+    if not args.no_tune:
+        tile = s.autotune(reps=2)    # sweep tile shape, timed on the warmed-up state
     if args.launch == "graph":
         s.step_graph(2)              # capture outside the clock
     s.sync()
@@ -226,7 +230,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.lhs == "fp64" else "f64+f32-lhs",
         "data": "synthetic (seeded generator wcns_amd.cases)",
         "config": {"workload": wl_name, "nodes": nodes, "blocks": len(case.blocks), "lhs": args.lhs,
-                   "launch": args.launch, "residual_rms_end": rms},
+                   "launch": args.launch, "sweep_tile": list(tile[:2]) if tile else None, "residual_rms_end": rms},
         "roofline": roof, "cpu_baseline": cpu,
         "step_hbm_frac": round(STEP_BYTES_PER_NODE * nodes * args.steps / sec / (HBM_PEAK_GBS * 1e9), 5),
         "stage_ms_per_step": {k: round(v[0] / kt, 4) for k, v in tim.items()},

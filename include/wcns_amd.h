@@ -120,6 +120,14 @@ int wcns_step(wcns_ctx *ctx, int nsteps, double *rms_hist);
  * inside (call wcns_residual after).  Synchronises the stream at the end. */
 int wcns_step_graph(wcns_ctx *ctx, int nsteps);
 
+/* Tile-shape autotuner of the pipelined sweeps (no counterpart in the
+ * reference): times the four sweeps of one step, reps times, for each
+ * candidate (j, k) tile shape of at most 64 columns on the current state and
+ * keeps the fastest for every block (the SA sweeps share it).  Leaves the
+ * primitive, SA and residual fields unchanged; the sweep unknowns dq are
+ * scratch.  Returns the shape chosen and its time per sweep set in ms. */
+int wcns_autotune(wcns_ctx *ctx, int reps, int *tj, int *tk, double *ms);
+
 /* Force and moment coefficients over every no-slip wall (bctype 2 or 2x):
  * out = cfx, cfy, cfz, cmx, cmy, cmz, cd, cl, xcp.  Replaces force + force_dif
  * (src/PostProcess.f90:1415-1503, 1505-1680); the quad quadrature is getforce

@@ -413,3 +413,25 @@ def test_graph_replay_matches_plain_launches(name):
             np.testing.assert_array_equal(a.field(nb, nm), b.field(nb, nm), err_msg=f"block {nb} {nm}")
     ra, rb = a.residual_curve(), b.residual_curve()
     assert ra[0] == rb[0] and ra[1] == rb[1]
+
+
+@pytest.mark.parametrize("name", ["bump-laminar", "sa-multiblock"])
+def test_autotuned_tiles_give_identical_steps(name):
+    """The autotuner only changes the sweep tile shape: the steps after it are
+    bit-identical to those of an untuned solver, and it leaves the state alone."""
+    case = GRAPH_CASES[name]()
+    print(f"autotune case {name}", flush=True)
+    a = _solver(case)
+    b = _solver(case)
+    a.step(2, history=False)
+    r_before = a.field(0, "r")
+    tj, tk, ms = a.autotune(reps=1)
+    print("tile", tj, tk, "ms", ms, flush=True)
+    assert 1 <= tj * tk <= 64 and ms > 0
+    np.testing.assert_array_equal(a.field(0, "r"), r_before)
+    a.step(3, history=False)
+    b.step(5, history=False)
+    names = ["r", "u", "v", "w", "p"] + (["qke", "vist"] if case.params.nlamtur == 1 else [])
+    for nb in range(len(case.blocks)):
+        for nm in names:
+            np.testing.assert_array_equal(a.field(nb, nm), b.field(nb, nm), err_msg=f"block {nb} {nm}")

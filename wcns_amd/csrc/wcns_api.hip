@@ -1227,6 +1227,57 @@ int wcns_get_counter(wcns_ctx *ctx, int which, long long *value)
     return WCNS_OK;
 }
 
+int wcns_autotune(wcns_ctx *ctx, int reps, int *tj, int *tk, double *ms)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (!ctx->tiled_sweeps) return fail(ctx, WCNS_E_UNSUPPORTED, "the autotuner tunes the pipelined sweeps");
+    reps = std::max(1, std::min(reps, 20));
+    static const int cand[][2] = {{16, 4}, {8, 8}, {32, 2}, {64, 1}, {4, 16}, {8, 4}, {16, 2}, {32, 1}};
+    DFields F = ctx->fields();
+    const int prec = F.dq32 ? 1 : 0;
+    // the sweeps read the plane-major copy of the current step's operands
+    for (size_t nb = 0; nb < ctx->blocks.size(); nb++)
+        if (launch_skew_stage(SKEW_STEP, ctx->blocks[nb].db, ctx->skoff[nb], F, prec, ctx->skew[prec], ctx->sktot,
+                              ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "staging the sweep operands");
+    const auto saved = ctx->tile_shape;
+    double best = 1e300;
+    int bj = 0, bk = 0;
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    for (auto &c : cand) {
+        ctx->tile_shape.clear();
+        for (auto &b : ctx->blocks) ctx->tile_shape.push_back({std::min(c[0], b.nj), std::min(c[1], b.nk)});
+        TRY(build_sweep_plan(ctx));
+        for (int kind = 0; kind < 4; kind++)      // warm-up
+            if (launch_sweep_batched(kind, ctx->plan, F, ctx->dp, prec, ctx->skew[prec], ctx->sktot, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "sweeps");
+        CK(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps; r++)
+            for (int kind = 0; kind < 4; kind++)
+                if (launch_sweep_batched(kind, ctx->plan, F, ctx->dp, prec, ctx->skew[prec], ctx->sktot, ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "sweeps");
+        CK(hipEventRecord(e1, ctx->stream));
+        CK(hipEventSynchronize(e1));
+        float t = 0.f;
+        CK(hipEventElapsedTime(&t, e0, e1));
+        TRY(check_counters(ctx));
+        if (t / reps < best) { best = t / reps; bj = c[0]; bk = c[1]; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    ctx->tile_shape.clear();
+    if (bj) for (auto &b : ctx->blocks) ctx->tile_shape.push_back({std::min(bj, b.nj), std::min(bk, b.nk)});
+    else ctx->tile_shape = saved;
+    TRY(build_sweep_plan(ctx));
+    drop_graphs(ctx);               // captured graphs hold the old plan
+    if (tj) *tj = bj;
+    if (tk) *tk = bk;
+    if (ms) *ms = best;
+    return WCNS_OK;
+}
+
 int wcns_timing(wcns_ctx *ctx, int on)
 {
     if (!ctx) return WCNS_E_ARG;

@@ -31,7 +31,7 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_initial_q_dq_c_t", "wcns_timestep", "wcns_r_h_s_vis", "wcns_r_h_s_invis", "wcns_communicate_dq",
            "wcns_l_h_s", "wcns_communicate_pv", "wcns_residual", "wcns_step", "wcns_get_counter", "wcns_sync",
            "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get", "wcns_force",
-           "wcns_step_graph"]
+           "wcns_step_graph", "wcns_autotune"]
 
 TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
@@ -88,6 +88,7 @@ def load_library(path=LIBPATH):
     L.wcns_timing_get.argtypes = [vp, C.c_int, dptr, C.POINTER(C.c_longlong)]
     L.wcns_force.argtypes = [vp, dptr]
     L.wcns_step_graph.argtypes = [vp, C.c_int]
+    L.wcns_autotune.argtypes = [vp, C.c_int, iptr, iptr, dptr]
     _lib = L
     return L
 
@@ -188,6 +189,12 @@ class Solver:
             return hist[:n]
         self._check(self.L.wcns_step(self.h, n, None), "step")
         return None
+
+    def autotune(self, reps=3):
+        """Pick the sweep tile shape by timing (wcns_autotune); returns (TJ, TK, ms per sweep set)."""
+        tj, tk, ms = C.c_int(), C.c_int(), C.c_double()
+        self._check(self.L.wcns_autotune(self.h, reps, C.byref(tj), C.byref(tk), C.byref(ms)), "autotune")
+        return tj.value, tk.value, ms.value
 
     def step_graph(self, n=1):
         """n whole steps replayed from a captured HIP graph (wcns_step_graph)."""
