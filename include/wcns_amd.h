@@ -52,6 +52,9 @@ typedef struct {
      * mu_t/mu cap of max_control_turbulence, NND parameters of the SA
      * convection (Convect.f90:1281, turblence.f90:371-435) */
     double timedt_turb, kmaxlim, xk, xb;
+    /* namelist force: length unit of the grid; the Reynolds number becomes
+     * reynolds/lfref*lref (init_inflow, PreProcess.f90:994) */
+    double lref;
 } wcns_params;
 
 /* Precision of the implicit operator: the residual, WCNS weights and the

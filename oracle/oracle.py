@@ -17,13 +17,13 @@ class OParams(C.Structure):
                                           "cfl", "timedt_rate", "efix", "csrv", "gama", "prl", "prt")] + \
                [(n, C.c_int) for n in ("nvis", "nlamtur", "nscheme", "nflux", "ntmst")] + \
                [(n, C.c_double) for n in ("sref", "lfref", "xref", "yref", "zref")] + [("nwholefield", C.c_int)] + \
-               [(n, C.c_double) for n in ("timedt_turb", "kmaxlim", "xk", "xb")]
+               [(n, C.c_double) for n in ("timedt_turb", "kmaxlim", "xk", "xb", "lref")]
 
 
 FIELDS = {"r": 0, "u": 1, "v": 2, "w": 3, "p": 4, "t": 5, "c": 6, "dtdt": 7, "sra": 8, "srb": 9,
           "src": 10, "srva": 11, "srvb": 12, "srvc": 13, "visl": 14, "vist": 15, "vol": 16,
           "kcx": 17, "kcy": 18, "kcz": 19, "etx": 20, "ety": 21, "etz": 22, "ctx": 23, "cty": 24,
-          "ctz": 25, "qke": 26, "dist": 27, "dq": 100, "q": 101, "rhs": 102}
+          "ctz": 25, "qke": 26, "dist": 27, "qkw": 28, "dq": 100, "q": 101, "rhs": 102}
 
 _lib = None
 

@@ -3,8 +3,9 @@
 ! Loads one block handed over from C into the reference's module variables
 ! (global_variables, Globalvar.f90:284-423, with the bounds of
 ! PreProcess.f90:744-807) and runs the reference's own per-block
-! Spalart-Allmaras step, turbulence(nb) (turblence.f90:15-49), with
-! nameturb = 'SA', ncmpcor = 0, ndes = 0, ntrans = 0, nwtmax = 0, method = 1.
+! turbulence-model step, turbulence(nb) (turblence.f90:15-49), with
+! nameturb = 'SA' (ref_sa_step) or 'SST' (ref_sst_step), ncmpcor = 0, ndes = 0,
+! ntrans = 0, nwtmax = 0, method = 1.
 ! oracle/Makefile links it with the reference's turblence.f90, Sub1.f90,
 ! Sub2.f90, Convect.f90, BL.f90 and Globalvar.f90, compiled from source in the
 ! reference's serial configuration (PARALLEL undefined), into _ref/libref_sa.so.
@@ -21,11 +22,12 @@ module ref_sa_store
   real, target, allocatable, save :: s_etx(:,:,:), s_ety(:,:,:), s_etz(:,:,:)
   real, target, allocatable, save :: s_ctx(:,:,:), s_cty(:,:,:), s_ctz(:,:,:)
   real, target, allocatable, save :: s_p(:,:,:), s_kct(:,:,:), s_ett(:,:,:), s_ctt(:,:,:)
+  real, target, allocatable, save :: s_c(:,:,:)
 contains
   subroutine release()
     if (allocated(s_r)) deallocate(s_r, s_u, s_v, s_w, s_qke, s_vist, s_visl, s_vol, &
          s_dtdt, s_dist, s_kcx, s_kcy, s_kcz, s_etx, s_ety, s_etz, s_ctx, s_cty, s_ctz, &
-         s_p, s_kct, s_ett, s_ctt)
+         s_p, s_kct, s_ett, s_ctt, s_c)
   end subroutine release
 
   ! copy nodes lo..n+hi of a C-frame field into dst(lo:ni+hi, lo:nj+hi, lo:nk+hi)
@@ -37,34 +39,23 @@ contains
   end subroutine load
 end module ref_sa_store
 
-! dims = (ni, nj, nk); scal = (reynolds, muoo, kmaxlim, timedt_turb, xk, xb, efix)
-! fld = pointers to r, u, v, w, qke, vist, visl, vol, dtdt, dist,
-!       kcx, kcy, kcz, etx, ety, etz, ctx, cty, ctz, p (C frame); the grid is
-!       fixed, so the time metrics kct, ett, ctt are zero
-! qke_out, vist_out: C frame, written on the nodes the reference stores
-! (qke, vist: -2..n+3).  Returns 0.
-integer(c_int) function ref_sa_step(dims, scal, fld, qke_out, vist_out) bind(C, name="ref_sa_step")
+! Loads one block: fld = pointers to r, u, v, w, qke(1), vist, visl, vol, dtdt,
+! dist, kcx, kcy, kcz, etx, ety, etz, ctx, cty, ctz, p and, for nl = 2, qke(2)
+! (C frame); the grid is fixed, so the time metrics kct, ett, ctt are zero.
+subroutine ref_turb_load(ni, nj, nk, nl, fld)
   use iso_c_binding
-  use global_variables, only: g_ni => ni, g_nj => nj, g_nk => nk, nmax, method, nlamtur, &
-       nameturb, ncmpcor, ndes, ntrans, nwtmax, reynolds, re, muoo, kmaxlim, timedt_turb, &
-       xk, xb, efix, r, u, v, w, qke, vist, visl, vol, dtdt, ds_turbulence, &
-       kcx, kcy, kcz, etx, ety, etz, ctx, cty, ctz, p, kct, ett, ctt
+  use global_variables, only: g_ni => ni, g_nj => nj, g_nk => nk, nmax, r, u, v, w, qke, vist, visl, &
+       vol, dtdt, ds_turbulence, kcx, kcy, kcz, etx, ety, etz, ctx, cty, ctz, p, kct, ett, ctt, c
   use ref_sa_store
   implicit none
-  integer(c_int), intent(in) :: dims(3)
-  real(c_double), intent(in) :: scal(7)
-  type(c_ptr), intent(in) :: fld(20)
-  real(c_double), intent(out) :: qke_out(-2:dims(1)+3, -2:dims(2)+3, -2:dims(3)+3)
-  real(c_double), intent(out) :: vist_out(-2:dims(1)+3, -2:dims(2)+3, -2:dims(3)+3)
+  integer, intent(in) :: ni, nj, nk, nl
+  type(c_ptr), intent(in) :: fld(*)
   real(c_double), pointer :: f(:,:,:)
-  integer :: ni, nj, nk
-  external :: turbulence
 
-  ni = dims(1); nj = dims(2); nk = dims(3)
   call release()
   allocate(s_r(-2:ni+3, -2:nj+3, -2:nk+3), s_u(-2:ni+3, -2:nj+3, -2:nk+3))
   allocate(s_v(-2:ni+3, -2:nj+3, -2:nk+3), s_w(-2:ni+3, -2:nj+3, -2:nk+3))
-  allocate(s_qke(-2:ni+3, -2:nj+3, -2:nk+3, 1), s_vist(-2:ni+3, -2:nj+3, -2:nk+3))
+  allocate(s_qke(-2:ni+3, -2:nj+3, -2:nk+3, nl), s_vist(-2:ni+3, -2:nj+3, -2:nk+3))
   allocate(s_visl(-1:ni+1, -1:nj+1, -1:nk+1), s_vol(-1:ni+1, -1:nj+1, -1:nk+1))
   allocate(s_dtdt(1:ni, 1:nj, 1:nk), s_dist(1:ni, 1:nj, 1:nk))
   allocate(s_kcx(-1:ni+1, -1:nj+1, -1:nk+1), s_kcy(-1:ni+1, -1:nj+1, -1:nk+1))
@@ -73,8 +64,8 @@ integer(c_int) function ref_sa_step(dims, scal, fld, qke_out, vist_out) bind(C, 
   allocate(s_ctx(-1:ni+1, -1:nj+1, -1:nk+1), s_cty(-1:ni+1, -1:nj+1, -1:nk+1))
   allocate(s_ctz(-1:ni+1, -1:nj+1, -1:nk+1), s_p(-2:ni+3, -2:nj+3, -2:nk+3))
   allocate(s_kct(-1:ni+1, -1:nj+1, -1:nk+1), s_ett(-1:ni+1, -1:nj+1, -1:nk+1))
-  allocate(s_ctt(-1:ni+1, -1:nj+1, -1:nk+1))
-  s_kct = 0.0; s_ett = 0.0; s_ctt = 0.0
+  allocate(s_ctt(-1:ni+1, -1:nj+1, -1:nk+1), s_c(-1:ni+1, -1:nj+1, -1:nk+1))
+  s_kct = 0.0; s_ett = 0.0; s_ctt = 0.0; s_c = 0.0
 
   call c_f_pointer(fld(1), f, [ni + 6, nj + 6, nk + 6]); call load(s_r, f, ni, nj, nk, -2, 3)
   call c_f_pointer(fld(2), f, [ni + 6, nj + 6, nk + 6]); call load(s_u, f, ni, nj, nk, -2, 3)
@@ -96,6 +87,9 @@ integer(c_int) function ref_sa_step(dims, scal, fld, qke_out, vist_out) bind(C, 
   call c_f_pointer(fld(18), f, [ni + 6, nj + 6, nk + 6]); call load(s_cty, f, ni, nj, nk, -1, 1)
   call c_f_pointer(fld(19), f, [ni + 6, nj + 6, nk + 6]); call load(s_ctz, f, ni, nj, nk, -1, 1)
   call c_f_pointer(fld(20), f, [ni + 6, nj + 6, nk + 6]); call load(s_p, f, ni, nj, nk, -2, 3)
+  if (nl == 2) then
+    call c_f_pointer(fld(21), f, [ni + 6, nj + 6, nk + 6]); call load(s_qke(:,:,:,2), f, ni, nj, nk, -2, 3)
+  end if
 
   ! the module state recast_grid / recast_field (Sub2.f90:324-402) would set
   g_ni = ni; g_nj = nj; g_nk = nk
@@ -106,8 +100,25 @@ integer(c_int) function ref_sa_step(dims, scal, fld, qke_out, vist_out) bind(C, 
   kcx => s_kcx; kcy => s_kcy; kcz => s_kcz
   etx => s_etx; ety => s_ety; etz => s_etz
   ctx => s_ctx; cty => s_cty; ctz => s_ctz
-  p => s_p; kct => s_kct; ett => s_ett; ctt => s_ctt
+  p => s_p; kct => s_kct; ett => s_ett; ctt => s_ctt; c => s_c
+end subroutine ref_turb_load
 
+! SA: dims = (ni, nj, nk); scal = (reynolds, muoo, kmaxlim, timedt_turb, xk, xb, efix);
+! fld as ref_turb_load (20 pointers).  qke_out, vist_out: C frame.  Returns 0.
+integer(c_int) function ref_sa_step(dims, scal, fld, qke_out, vist_out) bind(C, name="ref_sa_step")
+  use iso_c_binding
+  use global_variables, only: method, nlamtur, nameturb, ncmpcor, ndes, ntrans, nwtmax, reynolds, re, &
+       muoo, kmaxlim, timedt_turb, xk, xb, efix
+  use ref_sa_store
+  implicit none
+  integer(c_int), intent(in) :: dims(3)
+  real(c_double), intent(in) :: scal(7)
+  type(c_ptr), intent(in) :: fld(20)
+  real(c_double), intent(out) :: qke_out(-2:dims(1)+3, -2:dims(2)+3, -2:dims(3)+3)
+  real(c_double), intent(out) :: vist_out(-2:dims(1)+3, -2:dims(2)+3, -2:dims(3)+3)
+  external :: turbulence
+
+  call ref_turb_load(dims(1), dims(2), dims(3), 1, fld)
   ! run options of the SA path (param.dat namelists; re as init_inflow sets it,
   ! PreProcess.f90:1106)
   method = 1; nlamtur = 1; nameturb = 'SA'
@@ -122,3 +133,34 @@ integer(c_int) function ref_sa_step(dims, scal, fld, qke_out, vist_out) bind(C, 
   vist_out = s_vist
   ref_sa_step = 0
 end function ref_sa_step
+
+! SST: scal = (reynolds, koo, omgoo, kmaxlim, timedt_turb, xk, xb, efix); fld as
+! ref_turb_load with qke(2) = omega (21 pointers).  Returns 0.
+integer(c_int) function ref_sst_step(dims, scal, fld, k_out, w_out, vist_out) bind(C, name="ref_sst_step")
+  use iso_c_binding
+  use global_variables, only: method, nlamtur, nameturb, ncmpcor, ndes, ntrans, nwtmax, reynolds, re, &
+       koo, omgoo, kmaxlim, timedt_turb, xk, xb, efix
+  use ref_sa_store
+  implicit none
+  integer(c_int), intent(in) :: dims(3)
+  real(c_double), intent(in) :: scal(8)
+  type(c_ptr), intent(in) :: fld(21)
+  real(c_double), intent(out) :: k_out(-2:dims(1)+3, -2:dims(2)+3, -2:dims(3)+3)
+  real(c_double), intent(out) :: w_out(-2:dims(1)+3, -2:dims(2)+3, -2:dims(3)+3)
+  real(c_double), intent(out) :: vist_out(-2:dims(1)+3, -2:dims(2)+3, -2:dims(3)+3)
+  external :: turbulence
+
+  call ref_turb_load(dims(1), dims(2), dims(3), 2, fld)
+  method = 1; nlamtur = 2; nameturb = 'SST'
+  ncmpcor = 0; ndes = 0; ntrans = 0; nwtmax = 0
+  reynolds = scal(1); re = 1.0 / reynolds
+  koo = scal(2); omgoo = scal(3); kmaxlim = scal(4); timedt_turb = scal(5)
+  xk = scal(6); xb = scal(7); efix = scal(8)
+
+  call turbulence(1)
+
+  k_out = s_qke(:,:,:,1)
+  w_out = s_qke(:,:,:,2)
+  vist_out = s_vist
+  ref_sst_step = 0
+end function ref_sst_step

@@ -49,7 +49,8 @@ typedef struct {
     double *q, *dq, *rhs_nb;
     double *kcx, *kcy, *kcz, *etx, *ety, *etz, *ctx, *cty, *ctz, *vol;
     double *sra, *srb, *src, *srva, *srvb, *srvc, *dtdt, *visl, *vist;
-    double *qke, *dist;                    /* SA working variable (G3), wall distance (G3 shape, 1..n used) */
+    double *qke, *dist;                    /* SA nu~ / SST k (G3), wall distance (G3 shape, 1..n used) */
+    double *qkw;                           /* SST omega (G3) */
     int nregions;
     oregion reg[MAXREG];
     int bcindex[MAXREG];
@@ -62,6 +63,7 @@ struct o_case {
     o_params prm;
     double roo, uoo, voo, woo, poo, coo, too, eoo, visc, re, attack_r, sideslip_r;
     double muoo;                           /* SA freestream value, turblence.f90:3166-3181 */
+    double koo, omgoo, vtoo;               /* SST freestream k, omega and mu_t, same place */
     int nblocks, cap;
     oblock *blk;
     int ngroups;          /* groups with more than two points (multi-points) */
@@ -626,7 +628,7 @@ static void block_free(oblock *b)
     double **fs[] = {&b->x, &b->y, &b->z, &b->r, &b->u, &b->v, &b->w, &b->p, &b->t, &b->c,
                      &b->q, &b->dq, &b->rhs_nb, &b->kcx, &b->kcy, &b->kcz, &b->etx, &b->ety,
                      &b->etz, &b->ctx, &b->cty, &b->ctz, &b->vol, &b->sra, &b->srb, &b->src,
-                     &b->srva, &b->srvb, &b->srvc, &b->dtdt, &b->visl, &b->vist, &b->qke, &b->dist};
+                     &b->srva, &b->srvb, &b->srvc, &b->dtdt, &b->visl, &b->vist, &b->qke, &b->dist, &b->qkw};
     for (size_t n = 0; n < sizeof fs / sizeof fs[0]; n++) { free(*fs[n]); *fs[n] = NULL; }
     for (int r = 0; r < b->nregions; r++) {
         free(b->reg[r].image); free(b->reg[r].jmage); free(b->reg[r].kmage);
@@ -664,7 +666,7 @@ int o_block_add(o_case *c, int ni, int nj, int nk, const double *x, const double
     double **g3f[] = {&b->r, &b->u, &b->v, &b->w, &b->p, &b->t, &b->c, &b->kcx, &b->kcy, &b->kcz,
                       &b->etx, &b->ety, &b->etz, &b->ctx, &b->cty, &b->ctz, &b->vol, &b->sra,
                       &b->srb, &b->src, &b->srva, &b->srvb, &b->srvc, &b->dtdt, &b->visl, &b->vist,
-                      &b->qke, &b->dist};
+                      &b->qke, &b->dist, &b->qkw};
     for (size_t f = 0; f < sizeof g3f / sizeof g3f[0]; f++) *g3f[f] = (double *)calloc(b->g3, sizeof(double));
     b->q = (double *)calloc(5 * b->g3, sizeof(double));
     b->dq = (double *)calloc(5 * b->g3, sizeof(double));
@@ -962,8 +964,31 @@ static void init_inflow(o_case *c)
     c->coo = sqrt(gama * c->poo / c->roo);
     c->eoo = c->poo / (gama - 1.0) + 0.5 * c->roo * (c->uoo * c->uoo + c->voo * c->voo + c->woo * c->woo);
     c->visc = 110.4 / c->prm.tref;
+    /* PreProcess.f90:994: Reynolds number per grid length unit */
+    c->prm.reynolds = (c->prm.reynolds / c->prm.lfref) * c->prm.lref;
     c->re = 1.0 / c->prm.reynolds;
-    c->muoo = 1.0e-1;                      /* turblence.f90:3166-3181 inflow_turbulence */
+    /* turblence.f90:3166-3181 inflow_turbulence */
+    c->vtoo = 1.0e-6;
+    c->muoo = 1.0e-1;
+    if (c->prm.nlamtur == 2) {
+        double lref_lfref = c->prm.lref / c->prm.lfref;
+        c->omgoo = 10.0 * lref_lfref;
+        c->koo = c->omgoo * c->vtoo / c->prm.reynolds;
+    }
+}
+
+/* turblence.f90:3091-3117 init_SST: k = koo, omega = omgoo, mu_t = vtoo on -1..n+1 */
+static void init_sst(o_case *c)
+{
+    for (int nb = 0; nb < c->nblocks; nb++) {
+        oblock *b = &c->blk[nb];
+        for (int k = -1; k <= b->nk + 1; k++) for (int j = -1; j <= b->nj + 1; j++) for (int i = -1; i <= b->ni + 1; i++) {
+            long q = G(b, i, j, k);
+            b->qke[q] = c->koo;
+            b->qkw[q] = c->omgoo;
+            b->vist[q] = c->vtoo;
+        }
+    }
 }
 
 /* turblence.f90:2710-2734 init_SA: nu~ = muoo, mu_t = muoo fv1(muoo) on -1..n+1 */
@@ -1050,10 +1075,12 @@ int o_setup(o_case *c)
     build_groups(c);
     init_inflow(c);
     if (c->prm.nlamtur >= 0) {
-        /* turblence.f90:1735-1785 init_turbulence: wall distance, uniform SA field */
-        if (c->prm.nlamtur != 1 || c->prm.nvis != 1) return -12;
+        /* turblence.f90:1735-1785 init_turbulence: wall distance, uniform field
+         * (nlamtur 1: SA, 2: SST) */
+        if ((c->prm.nlamtur != 1 && c->prm.nlamtur != 2) || c->prm.nvis != 1) return -12;
         o_wall_distance(c);
-        init_sa(c);
+        if (c->prm.nlamtur == 1) init_sa(c);
+        else init_sst(c);
     }
     return 0;
 }
@@ -1094,7 +1121,7 @@ static double *field_ptr(const oblock *b, int which)
     case 17: return b->kcx; case 18: return b->kcy; case 19: return b->kcz;
     case 20: return b->etx; case 21: return b->ety; case 22: return b->etz;
     case 23: return b->ctx; case 24: return b->cty; case 25: return b->ctz;
-    case 26: return b->qke; case 27: return b->dist;
+    case 26: return b->qke; case 27: return b->dist; case 28: return b->qkw;
     default: return NULL;
     }
 }
@@ -1166,8 +1193,13 @@ static void getnxyz(const oblock *b, int i, int j, int k, int is, int js, int ks
     *nx = x / s; *ny = y / s; *nz = z / s;
 }
 
-/* the SA variable is carried through the boundary conditions (nvis = 1, nlamtur >= 0) */
+/* the turbulence variables are carried through the boundary conditions (nvis = 1,
+ * nlamtur >= 0): nu~ (SA), or k and omega (SST) */
 #define TURB(c) ((c)->prm.nvis == 1 && (c)->prm.nlamtur >= 0)
+static int nturb(const o_case *c) { return TURB(c) ? c->prm.nlamtur : 0; }
+static double *tvar(const oblock *b, int m) { return m == 0 ? b->qke : b->qkw; }
+/* BC_farfield_turbulence's inflow values (turblence.f90:2321-2338) */
+static double tinf(const o_case *c, int m) { return c->prm.nlamtur == 1 ? c->muoo : (m == 0 ? c->koo : c->omgoo); }
 #define SETP(b,is,js,ks,rv,uv,vv,wv,pv) do { long _g = G(b,is,js,ks); (b)->r[_g]=(rv); (b)->u[_g]=(uv); (b)->v[_g]=(vv); (b)->w[_g]=(wv); (b)->p[_g]=(pv); } while (0)
 
 /* Sub2.f90:206-322 boundary_n1_vir3 (cic1 = 0; turbulence copies when nvis==1 & nlamtur>=0) */
@@ -1186,8 +1218,11 @@ static void bc_interface(o_case *c, oblock *b, oregion *g)
                     long gi = G(b, i - g->s_lr3d[0], j - g->s_lr3d[1], k - g->s_lr3d[2]);
                     b->vist[gs] = bt->vist[gt];
                     b->vist[g0] = (b->vist[gs] + b->vist[gi]) * 0.5;
-                    b->qke[gs] = bt->qke[gt];
-                    b->qke[g0] = (b->qke[gs] + b->qke[gi]) * 0.5;
+                    for (int m = 0; m < nturb(c); m++) {
+                        double *q = tvar(b, m);
+                        q[gs] = tvar(bt, m)[gt];
+                        q[g0] = (q[gs] + q[gi]) * 0.5;
+                    }
                 }
                 for (int l = 1; l <= 3; l++) {
                     long gt = G(bt, it, jt, kt), gs = G(b, is, js, ks);
@@ -1253,7 +1288,8 @@ static void bc_farfield(o_case *c, oblock *b, oregion *g)
                     else ut = tem_lr * (b->ctx[g0] * b->u[g0] + b->cty[g0] * b->v[g0] + b->ctz[g0] * b->w[g0]);
                     for (int m = 0; m <= 1; m++) {
                         long gm = G(b, i + m * l3[0], j + m * l3[1], k + m * l3[2]);
-                        b->qke[gm] = ut < 0.0 ? c->muoo : b->qke[G(b, it, jt, kt)];
+                        for (int v = 0; v < nturb(c); v++)
+                            tvar(b, v)[gm] = ut < 0.0 ? tinf(c, v) : tvar(b, v)[G(b, it, jt, kt)];
                     }
                 }
                 SETP(b, i + l3[0], j + l3[1], k + l3[2], rb, ub, vb, wb, pb);
@@ -1273,7 +1309,8 @@ static void bc_fixed(o_case *c, oblock *b, oregion *g)
                          c->roo, c->uoo, c->voo, c->woo, c->poo);
                     /* turblence.f90:2321-2338 BC_farfield_turbulence on the face node and first ghost */
                     if (TURB(c) && n <= 1)
-                        b->qke[G(b, i + n * g->s_lr3d[0], j + n * g->s_lr3d[1], k + n * g->s_lr3d[2])] = c->muoo;
+                        for (int v = 0; v < nturb(c); v++)
+                            tvar(b, v)[G(b, i + n * g->s_lr3d[0], j + n * g->s_lr3d[1], k + n * g->s_lr3d[2])] = tinf(c, v);
                 }
 }
 
@@ -1291,7 +1328,8 @@ static void bc_outflow(o_case *c, oblock *b, oregion *g)
                     SETP(b, i + n * l3[0], j + n * l3[1], k + n * l3[2], rm, um, vm, wm, pm);
                     /* turblence.f90:2340-2355 BC_outflow_turbulence on the face node and first ghost */
                     if (TURB(c) && n <= 1)
-                        b->qke[G(b, i + n * l3[0], j + n * l3[1], k + n * l3[2])] = b->qke[G(b, it, jt, kt)];
+                        for (int v = 0; v < nturb(c); v++)
+                            tvar(b, v)[G(b, i + n * l3[0], j + n * l3[1], k + n * l3[2])] = tvar(b, v)[G(b, it, jt, kt)];
                 }
             }
 }
@@ -1310,7 +1348,7 @@ static void dif_average(o_case *c, oblock *b, oregion *g)
                 b->v[g0] = 0.5 * (b->v[gs] + b->v[gt]);
                 b->w[g0] = 0.5 * (b->w[gs] + b->w[gt]);
                 b->p[g0] = 0.5 * (b->p[gs] + b->p[gt]);
-                if (TURB(c)) b->qke[g0] = 0.5 * (b->qke[gt] + b->qke[gs]);
+                for (int v = 0; v < nturb(c); v++) tvar(b, v)[g0] = 0.5 * (tvar(b, v)[gt] + tvar(b, v)[gs]);
             }
 }
 
@@ -1346,8 +1384,17 @@ static void bc_viscous_wall(o_case *c, oblock *b, oregion *g)
                 int is = i + l3[0], js = j + l3[1], ks = k + l3[2];
                 int it = i - l3[0], jt = j - l3[1], kt = k - l3[2];
                 SETP(b, is, js, ks, R_(b,it,jt,kt), -U_(b,it,jt,kt), -V_(b,it,jt,kt), -W_(b,it,jt,kt), P_(b,it,jt,kt));
-                /* turblence.f90:2267-2306 BC_wall_turbulence (SA): odd reflection of the first ghost */
+                /* turblence.f90:2267-2306 BC_wall_turbulence: odd reflection of nu~ / k in the
+                 * first ghost; SST omega there: 1600 mu/(rho Re d^2) - omega(first interior),
+                 * with mu from the face node's temperature and the face node's old density */
                 if (TURB(c)) b->qke[G(b, is, js, ks)] = -b->qke[G(b, it, jt, kt)];
+                if (TURB(c) && c->prm.nlamtur == 2) {
+                    long g0 = G(b, i, j, k), gt = G(b, it, jt, kt);
+                    double tm = b->t[g0];
+                    double vsl = tm * sqrt(tm) * (1.0 + c->visc) / (tm + c->visc);
+                    double d = b->dist[gt];
+                    b->qkw[G(b, is, js, ks)] = 1600.0 * vsl / b->r[g0] / c->prm.reynolds / (d * d) - b->qkw[gt];
+                }
                 double pres, tres;
                 int i2 = it - l3[0], j2 = jt - l3[1], k2 = kt - l3[2];
                 if (c->prm.twall > 0.0) {
@@ -1363,7 +1410,8 @@ static void bc_viscous_wall(o_case *c, oblock *b, oregion *g)
                 double dres = moocp * pres / tres;
                 SETP(b, i, j, k, dres, 0.0, 0.0, 0.0, pres);
                 /* BC_face_dif_turbulence: face value = mean of first ghost and first interior node */
-                if (TURB(c)) b->qke[G(b, i, j, k)] = 0.5 * (b->qke[G(b, it, jt, kt)] + b->qke[G(b, is, js, ks)]);
+                for (int v = 0; v < nturb(c); v++)
+                    tvar(b, v)[G(b, i, j, k)] = 0.5 * (tvar(b, v)[G(b, it, jt, kt)] + tvar(b, v)[G(b, is, js, ks)]);
                 is += l3[0]; js += l3[1]; ks += l3[2]; it -= l3[0]; jt -= l3[1]; kt -= l3[2];
                 SETP(b, is, js, ks, R_(b,it,jt,kt), -U_(b,it,jt,kt), -V_(b,it,jt,kt), -W_(b,it,jt,kt), P_(b,it,jt,kt));
                 is += l3[0]; js += l3[1]; ks += l3[2]; it -= l3[0]; jt -= l3[1]; kt -= l3[2];
@@ -1389,7 +1437,8 @@ static void bc_symmetry(o_case *c, oblock *b, oregion *g)
                     double rm = R_(b,it,jt,kt), um = U_(b,it,jt,kt), vm = V_(b,it,jt,kt);
                     double wm = -W_(b,it,jt,kt), pm = P_(b,it,jt,kt);
                     SETP(b, i + l3[0], j + l3[1], k + l3[2], rm, um, vm, wm, pm);
-                    if (TURB(c)) b->qke[G(b, i + l3[0], j + l3[1], k + l3[2])] = b->qke[G(b, it, jt, kt)];
+                    for (int v = 0; v < nturb(c); v++)
+                        tvar(b, v)[G(b, i + l3[0], j + l3[1], k + l3[2])] = tvar(b, v)[G(b, it, jt, kt)];
                     SETP(b, i + 2 * l3[0], j + 2 * l3[1], k + 2 * l3[2], rm, um, vm, 0.0, pm);
                     SETP(b, i + 3 * l3[0], j + 3 * l3[1], k + 3 * l3[2], rm, um, vm, 0.0, pm);
                 }
@@ -1411,7 +1460,8 @@ static void bc_symmetry(o_case *c, oblock *b, oregion *g)
                          U_(b,it,jt,kt) - s * nx, V_(b,it,jt,kt) - s * ny, W_(b,it,jt,kt) - s * nz, P_(b,it,jt,kt));
                     /* turblence.f90:2308-2319 BC_symmetry_turbulence on the first ghost */
                     if (TURB(c) && l == 1)
-                        b->qke[G(b, i + l3[0], j + l3[1], k + l3[2])] = b->qke[G(b, it, jt, kt)];
+                        for (int v = 0; v < nturb(c); v++)
+                            tvar(b, v)[G(b, i + l3[0], j + l3[1], k + l3[2])] = tvar(b, v)[G(b, it, jt, kt)];
                 }
     dif_average(c, b, g);
 }
@@ -2013,11 +2063,12 @@ int o_view_block(o_case *c, int nb, oblock_view *v)
     v->ctx = b->ctx; v->cty = b->cty; v->ctz = b->ctz; v->vol = b->vol;
     v->sra = b->sra; v->srb = b->srb; v->src = b->src; v->srva = b->srva; v->srvb = b->srvb; v->srvc = b->srvc;
     v->dtdt = b->dtdt; v->visl = b->visl; v->vist = b->vist;
-    v->qke = b->qke; v->dist = b->dist;
+    v->qke = b->qke; v->dist = b->dist; v->qkw = b->qkw;
     return 0;
 }
 
 double o_view_muoo(const o_case *c) { return c->muoo; }
+void o_view_turb_inflow(const o_case *c, double *koo, double *omgoo) { *koo = c->koo; *omgoo = c->omgoo; }
 
 void o_view_params(const o_case *c, o_params *p, double *visc)
 {

@@ -31,6 +31,7 @@ typedef struct {
     /* SA model (nlamtur = 1): step timedt_turb, turbulence group kmaxlim,
      * interplate group xk, xb (NND limiter of the convection term) */
     double timedt_turb, kmaxlim, xk, xb;
+    double lref;          /* namelist force: grid length unit (PreProcess.f90:93, 994) */
 } o_params;
 
 typedef struct o_case o_case;

@@ -17,11 +17,13 @@ typedef struct {
     double *kcx, *kcy, *kcz, *etx, *ety, *etz, *ctx, *cty, *ctz, *vol;
     double *sra, *srb, *src, *srva, *srvb, *srvc, *dtdt, *visl, *vist;
     double *qke, *dist;
+    double *qkw;          /* SST omega */
 } oblock_view;
 
 int o_view_block(o_case *c, int nb, oblock_view *v);
 void o_view_params(const o_case *c, o_params *p, double *visc);
 double o_view_muoo(const o_case *c);
+void o_view_turb_inflow(const o_case *c, double *koo, double *omgoo);
 
 static inline long OG(const oblock_view *b, int i, int j, int k)
 {

@@ -122,8 +122,15 @@ static double mxdq_turb(const double *prim, const double *g, double dq, int npn)
     return l1 * dq;
 }
 
+static int sst_step(o_case *c, int nb);
+
 int o_turbulence(o_case *c, int nb)
 {
+    o_params prm0;
+    double visc0;
+    o_view_params(c, &prm0, &visc0);
+    if (prm0.nlamtur == 2) return sst_step(c, nb);
+    if (prm0.nlamtur != 1) return -1;
     o_params prm;
     double visc;
     o_view_params(c, &prm, &visc);
@@ -319,6 +326,259 @@ int o_turbulence(o_case *c, int nb)
 #undef N3V
 #undef OGV
     free(dqke); free(spec); free(dudxyz); free(dvdxyz); free(dwdxyz); free(dmudxyz); free(viseq);
+    free(fv); free(ql); free(tr); free(dfc); free(fc); free(dqw); free(clw); free(crw);
+    return 0;
+}
+
+/* ===================================================================== SST */
+/* cal_rhs_source_SST constants (turblence.f90:2813-2818) */
+static const double SST_CGAK[2] = {0.85, 1.0}, SST_CGAW[2] = {0.5, 0.856}, SST_BETA[2] = {0.075, 0.0828};
+#define SST_CDMIN 1.0e-20
+#define SST_KI 0.41
+
+/* The SST step of one block (turblence.f90:15-103 with nameturb 'SST', nlamtur 2,
+ * ncmpcor 0, ndes 0): cal_rhs_source_SST (:2810-3089), cal_rhs_viscous_turbulence_dif
+ * (:851-1019), cal_spec_turbulence (:175-215), cal_rhs_inviscous_turbulence per
+ * variable, lusgs_turbulence per variable, update_turbulence with cal_vist_SST
+ * (:3119-3164), max_control_turbulence. */
+static int sst_step(o_case *c, int nb)
+{
+    o_params prm;
+    double visc;
+    o_view_params(c, &prm, &visc);
+    oblock_view B;
+    if (o_view_block(c, nb, &B)) return -1;
+    oblock_view *b = &B;
+    const int ni = b->ni, nj = b->nj, nk = b->nk;
+    const long N = (long)ni * nj * nk;
+    const double re = 1.0 / prm.reynolds, reynolds = prm.reynolds;
+    double koo, omgoo;
+    o_view_turb_inflow(c, &koo, &omgoo);
+    const double tinf[2] = {koo, omgoo};
+    double *q2[2] = {b->qke, b->qkw};
+    const double ki2 = SST_KI * SST_KI, twothd = 0.2 / 0.3;
+    const double betax0 = 0.09, betax = betax0, obetx = 1.0 / betax, ki2osqrtbetx = ki2 / sqrt(betax);
+#define N3(i, j, k) ((long)((i) - 1) + (long)ni * ((long)((j) - 1) + (long)nj * ((k) - 1)))
+#define N3V(...) N3(__VA_ARGS__)
+#define OGV(b, ...) OG(b, __VA_ARGS__)
+    double *dqke[2], *spec[2], *dmu[2], *viseq[2];
+    for (int m = 0; m < 2; m++) {
+        dqke[m] = (double *)calloc(b->g3, sizeof(double));
+        spec[m] = (double *)calloc(b->g3, sizeof(double));
+        dmu[m] = (double *)calloc(3 * N, sizeof(double));
+        viseq[m] = (double *)calloc(N, sizeof(double));
+    }
+    double *dudxyz = (double *)calloc(3 * N, sizeof(double)), *dvdxyz = (double *)calloc(3 * N, sizeof(double));
+    double *dwdxyz = (double *)calloc(3 * N, sizeof(double)), *F1 = (double *)calloc(N, sizeof(double));
+
+    /* cal_rhs_source_SST */
+    dfidxyz_center(b, b->u, dudxyz);
+    dfidxyz_center(b, b->v, dvdxyz);
+    dfidxyz_center(b, b->w, dwdxyz);
+    dfidxyz_center(b, b->qke, dmu[0]);
+    dfidxyz_center(b, b->qkw, dmu[1]);
+    const double re500 = re * 500.0, cgaw24 = SST_CGAW[1] * 4.0, cgaw22 = SST_CGAW[1] * 2.0;
+    for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
+        long q = OG(b, i, j, k), n = N3(i, j, k);
+        double ro = b->r[q], qk = b->qke[q], qw = b->qkw[q], vsl = b->visl[q], vst = b->vist[q];
+        double rok = ro * qk, row = ro * qw;
+        double dn = b->dist[q], dn2 = dn * dn;
+        double dkxdw = dmu[0][3 * n] * dmu[1][3 * n] + dmu[0][3 * n + 1] * dmu[1][3 * n + 1] +
+                       dmu[0][3 * n + 2] * dmu[1][3 * n + 2];
+        double rowdkdw = ro / qw * dkxdw * cgaw22;
+        double CDkw = amax1(SST_CDMIN, rowdkdw);
+        double sqrtkwdn = obetx * sqrt(qk) / qw / dn;
+        double vlrodn2w = re500 * vsl / row / dn2;
+        double rokdn2 = cgaw24 * rok / CDkw / dn2;
+        double arg2 = amax1(sqrtkwdn, vlrodn2w);
+        double arg1 = amin1(arg2, rokdn2);
+        F1[n] = arg1 >= 2.0 ? 1.0 : tanh(pow(arg1, 4.0));
+        double F2 = 1.0 - F1[n];
+        double cga0 = F1[n] * SST_CGAK[0] + F2 * SST_CGAK[1];
+        double cga1 = F1[n] * SST_CGAW[0] + F2 * SST_CGAW[1];
+        viseq[0][n] = vsl + cga0 * vst;
+        viseq[1][n] = vsl + cga1 * vst;
+    }
+    for (int k = 2; k <= nk - 1; k++) for (int j = 2; j <= nj - 1; j++) for (int i = 2; i <= ni - 1; i++) {
+        long q = OG(b, i, j, k), n = N3(i, j, k);
+        double ro = b->r[q], qk = b->qke[q], qw = b->qkw[q], volume = b->vol[q], vst = b->vist[q];
+        double rok = ro * qk, row = ro * qw;
+        double F2 = 1.0 - F1[n];
+        double beita = F1[n] * SST_BETA[0] + F2 * SST_BETA[1];
+        double cga1 = F1[n] * SST_CGAW[0] + F2 * SST_CGAW[1];
+        double ga = beita * obetx - cga1 * ki2osqrtbetx;
+        double ux = dudxyz[3 * n], uy = dudxyz[3 * n + 1], uz = dudxyz[3 * n + 2];
+        double vx = dvdxyz[3 * n], vy = dvdxyz[3 * n + 1], vz = dvdxyz[3 * n + 2];
+        double wx = dwdxyz[3 * n], wy = dwdxyz[3 * n + 1], wz = dwdxyz[3 * n + 2];
+        double div_u = ux + vy + wz;
+        double dkxdw = dmu[0][3 * n] * dmu[1][3 * n] + dmu[0][3 * n + 1] * dmu[1][3 * n + 1] +
+                       dmu[0][3 * n + 2] * dmu[1][3 * n + 2];
+        double omg = vst * ((2.0 * (ux * ux + vy * vy + wz * wz + uy * vx + uz * wx + wy * vz) +
+                             uy * uy + uz * uz + vx * vx + vz * vz + wx * wx + wy * wy) -
+                            twothd * div_u * div_u) * re -
+                     twothd * div_u * rok;
+        double cdk = betax * qw, cdw = beita * qw;
+        double Sd0 = cdk * rok, Sd1 = cdw * row;
+        double Sp0 = amin1(omg, 20.0 * Sd0);
+        double Sp1 = ga * ro * omg / vst * reynolds;
+        double dbw = cgaw22 * F2 * dkxdw / qw / qw;
+        double Dbar0 = 0.0, Dbar1 = dbw * row;
+        spec[0][q] = spec[0][q] + 2.0 * (cdk + twothd * amax1(ux + vy + wz, 0.0)) * volume;
+        spec[1][q] = spec[1][q] + 2.0 * (2.0 * cdw + fabs(dbw)) * volume;
+        dqke[0][q] = dqke[0][q] - (Sp0 - Sd0 + Dbar0) * volume;
+        dqke[1][q] = dqke[1][q] - (Sp1 - Sd1 + Dbar1) * volume;
+    }
+
+    /* cal_rhs_viscous_turbulence_dif: faces between l-1 and l, l = 2..n; nodes 2..n-1 */
+    int nmax = ni > nj ? ni : nj;
+    nmax = nmax > nk ? nmax : nk;
+    double *fv = (double *)calloc(2 * (nmax + 2), sizeof(double));
+    for (int dir = 0; dir < 3; dir++) {
+        int n = dir == 0 ? ni : (dir == 1 ? nj : nk);
+        int na = dir == 0 ? nj : ni, nbb = dir == 2 ? nj : nk;
+        const double *mx = dir == 0 ? b->kcx : (dir == 1 ? b->etx : b->ctx);
+        const double *my = dir == 0 ? b->kcy : (dir == 1 ? b->ety : b->cty);
+        const double *mz = dir == 0 ? b->kcz : (dir == 1 ? b->etz : b->ctz);
+        for (int b2 = 2; b2 <= nbb - 1; b2++)
+            for (int b1 = 2; b1 <= na - 1; b1++) {
+#define IJK(l) (dir == 0 ? (l) : b1), (dir == 1 ? (l) : (dir == 0 ? b1 : b2)), (dir == 2 ? (l) : b2)
+                for (int l = 2; l <= n; l++) {
+                    long qa = OGV(b, IJK(l)), qb = OGV(b, IJK(l - 1));
+                    long na_ = N3V(IJK(l)), nb_ = N3V(IJK(l - 1));
+                    double nx = 0.5 * (mx[qa] + mx[qb]), ny = 0.5 * (my[qa] + my[qb]), nz = 0.5 * (mz[qa] + mz[qb]);
+                    for (int m = 0; m < 2; m++) {
+                        double dmudx = dmu[m][3 * na_] + dmu[m][3 * nb_];
+                        double dmudy = dmu[m][3 * na_ + 1] + dmu[m][3 * nb_ + 1];
+                        double dmudz = dmu[m][3 * na_ + 2] + dmu[m][3 * nb_ + 2];
+                        double vis = viseq[m][na_] + viseq[m][nb_];
+                        fv[2 * l + m] = (dmudx * nx + dmudy * ny + dmudz * nz) * vis * 0.25;
+                    }
+                }
+                for (int m = 0; m < 2; m++)
+                    for (int l = 2; l <= n - 1; l++) {
+                        long q = OGV(b, IJK(l));
+                        dqke[m][q] = dqke[m][q] - re * (fv[2 * (l + 1) + m] - fv[2 * l + m]);
+                    }
+            }
+    }
+
+    /* cal_spec_turbulence on nodes 2..n-1 */
+    const double re2 = re * 2.0;
+    for (int m = 0; m < 2; m++)
+        for (int k = 2; k <= nk - 1; k++) for (int j = 2; j <= nj - 1; j++) for (int i = 2; i <= ni - 1; i++) {
+            long q = OG(b, i, j, k);
+            double um = b->u[q], vm = b->v[q], wm = b->w[q];
+            double unx = um * b->kcx[q] + vm * b->kcy[q] + wm * b->kcz[q];
+            double uny = um * b->etx[q] + vm * b->ety[q] + wm * b->etz[q];
+            double unz = um * b->ctx[q] + vm * b->cty[q] + wm * b->ctz[q];
+            spec[m][q] = spec[m][q] + fabs(unx) + fabs(uny) + fabs(unz) + 1.0 / (b->dtdt[q] * prm.timedt_turb);
+            double nx = b->kcx[q] * b->kcx[q] + b->kcy[q] * b->kcy[q] + b->kcz[q] * b->kcz[q];
+            double ny = b->etx[q] * b->etx[q] + b->ety[q] * b->ety[q] + b->etz[q] * b->etz[q];
+            double nz = b->ctx[q] * b->ctx[q] + b->cty[q] * b->cty[q] + b->ctz[q] * b->ctz[q];
+            double vis = viseq[m][N3(i, j, k)] / b->vol[q] / b->r[q];
+            spec[m][q] = spec[m][q] + re2 * vis * (nx + ny + nz);
+        }
+
+    /* cal_rhs_inviscous_turbulence, variable by variable */
+    double *ql = (double *)calloc(5 * (nmax + 3), sizeof(double)), *tr = (double *)calloc(3 * (nmax + 1), sizeof(double));
+    double *dfc = (double *)calloc(nmax + 2, sizeof(double)), *fc = (double *)calloc(nmax + 2, sizeof(double));
+    double *dqw = (double *)calloc(5 * (nmax + 3), sizeof(double)), *clw = (double *)calloc(5 * (nmax + 2), sizeof(double));
+    double *crw = (double *)calloc(5 * (nmax + 2), sizeof(double));
+    for (int m = 0; m < 2; m++)
+        for (int dir = 0; dir < 3; dir++) {
+            int n = dir == 0 ? ni : (dir == 1 ? nj : nk);
+            int na = dir == 0 ? nj : ni, nbb = dir == 2 ? nj : nk;
+            const double *mx = dir == 0 ? b->kcx : (dir == 1 ? b->etx : b->ctx);
+            const double *my = dir == 0 ? b->kcy : (dir == 1 ? b->ety : b->cty);
+            const double *mz = dir == 0 ? b->kcz : (dir == 1 ? b->etz : b->ctz);
+            for (int b2 = 2; b2 <= nbb - 1; b2++)
+                for (int b1 = 2; b1 <= na - 1; b1++) {
+                    for (int l = 0; l <= n + 1; l++) {
+                        long q = OGV(b, IJK(l));
+                        double *e = ql + 5 * (l + 1);
+                        e[0] = b->r[q]; e[1] = b->u[q]; e[2] = b->v[q]; e[3] = b->w[q]; e[4] = q2[m][q];
+                    }
+                    for (int l = 1; l <= n; l++) {
+                        long q = OGV(b, IJK(l));
+                        tr[3 * (l - 1)] = mx[q]; tr[3 * (l - 1) + 1] = my[q]; tr[3 * (l - 1) + 2] = mz[q];
+                    }
+                    nnd_turbulence(prm.xk, prm.xb, n, ql, tr, dfc, fc, dqw, clw, crw);
+                    for (int l = 2; l <= n - 1; l++) {
+                        long q = OGV(b, IJK(l));
+                        dqke[m][q] = dqke[m][q] + dfc[l];
+                    }
+                }
+        }
+#undef IJK
+
+    /* lusgs_turbulence, variable by variable: forward then backward over nodes 2..n-1 */
+    for (int m = 0; m < 2; m++) {
+        double *dq = dqke[m], *sp = spec[m];
+        for (int k = 2; k <= nk - 1; k++) for (int j = 2; j <= nj - 1; j++) for (int i = 2; i <= ni - 1; i++) {
+            long q = OG(b, i, j, k);
+            long qi = OG(b, i - 1, j, k), qj = OG(b, i, j - 1, k), qk = OG(b, i, j, k - 1);
+            double pi[5] = {b->r[qi], b->u[qi], b->v[qi], b->w[qi], b->p[qi]};
+            double pj[5] = {b->r[qj], b->u[qj], b->v[qj], b->w[qj], b->p[qj]};
+            double pk[5] = {b->r[qk], b->u[qk], b->v[qk], b->w[qk], b->p[qk]};
+            double gi[3] = {b->kcx[qi], b->kcy[qi], b->kcz[qi]};
+            double gj[3] = {b->etx[qj], b->ety[qj], b->etz[qj]};
+            double gk[3] = {b->ctx[qk], b->cty[qk], b->ctz[qk]};
+            double de = mxdq_turb(pi, gi, dq[qi], 1), df = mxdq_turb(pj, gj, dq[qj], 1);
+            double dg = mxdq_turb(pk, gk, dq[qk], 1);
+            double rhs0 = de + df + dg;
+            dq[q] = (rhs0 - dq[q]) / sp[q];
+        }
+        for (int k = nk - 1; k >= 2; k--) for (int j = nj - 1; j >= 2; j--) for (int i = ni - 1; i >= 2; i--) {
+            long q = OG(b, i, j, k);
+            long qi = OG(b, i + 1, j, k), qj = OG(b, i, j + 1, k), qk = OG(b, i, j, k + 1);
+            double pi[5] = {b->r[qi], b->u[qi], b->v[qi], b->w[qi], b->p[qi]};
+            double pj[5] = {b->r[qj], b->u[qj], b->v[qj], b->w[qj], b->p[qj]};
+            double pk[5] = {b->r[qk], b->u[qk], b->v[qk], b->w[qk], b->p[qk]};
+            double gi[3] = {b->kcx[qi], b->kcy[qi], b->kcz[qi]};
+            double gj[3] = {b->etx[qj], b->ety[qj], b->etz[qj]};
+            double gk[3] = {b->ctx[qk], b->cty[qk], b->ctz[qk]};
+            double de = mxdq_turb(pi, gi, dq[qi], -1), df = mxdq_turb(pj, gj, dq[qj], -1);
+            double dg = mxdq_turb(pk, gk, dq[qk], -1);
+            double rhs0 = de + df + dg;
+            dq[q] = dq[q] - rhs0 / sp[q];
+        }
+    }
+
+    /* update_turbulence (nodes 2..n-1), cal_vist_SST (1..n) */
+    const double minlimiter = 1.0e-20;
+    for (int m = 0; m < 2; m++)
+        for (int k = 2; k <= nk - 1; k++) for (int j = 2; j <= nj - 1; j++) for (int i = 2; i <= ni - 1; i++) {
+            long q = OG(b, i, j, k);
+            q2[m][q] = amax1(q2[m][q] + dqke[m][q] / b->r[q], minlimiter * tinf[m]);
+        }
+    {
+        const double a1 = 0.31, obetx_v = 2.0 / 0.09;
+        const double a1reynolds = a1 * reynolds, b1 = 500.0 * re;
+        for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
+            long q = OG(b, i, j, k), n = N3(i, j, k);
+            double ro = b->r[q], qk = b->qke[q], qw = b->qkw[q], vsl = b->visl[q];
+            double dn = b->dist[q], dn2 = dn * dn;
+            double sqrtkwdn = obetx_v * sqrt(qk) / qw / dn;
+            double vlrodn2w = b1 * vsl / ro / dn2 / qw;
+            double arg2 = amax1(sqrtkwdn, vlrodn2w);
+            double F2 = arg2 >= 4.0 ? 1.0 : tanh(arg2 * arg2);
+            double a_ = dudxyz[3 * n + 1] - dvdxyz[3 * n], b_ = dvdxyz[3 * n + 2] - dwdxyz[3 * n + 1];
+            double c_ = dwdxyz[3 * n] - dudxyz[3 * n + 2];
+            double omg = sqrt(a_ * a_ + b_ * b_ + c_ * c_);
+            b->vist[q] = a1reynolds * ro * qk / amax1(a1 * qw, omg * F2);
+        }
+    }
+    /* max_control_turbulence: nodes 0..n */
+    if (prm.kmaxlim > 1.0e-15)
+        for (int k = 0; k <= nk; k++) for (int j = 0; j <= nj; j++) for (int i = 0; i <= ni; i++) {
+            long q = OG(b, i, j, k);
+            b->vist[q] = amin1(b->vist[q], prm.kmaxlim * b->visl[q]);
+        }
+#undef N3
+#undef N3V
+#undef OGV
+    for (int m = 0; m < 2; m++) { free(dqke[m]); free(spec[m]); free(dmu[m]); free(viseq[m]); }
+    free(dudxyz); free(dvdxyz); free(dwdxyz); free(F1);
     free(fv); free(ql); free(tr); free(dfc); free(fc); free(dqw); free(clw); free(crw);
     return 0;
 }

@@ -7,9 +7,12 @@ Routines exercised (reference src/ file:line):
   flux_Roe         flux.f90:116-244 (with flux, Convect.f90:1245-1272)
   WCNS_E_5         WCNSE5.f90:2-206      with flux_Roe
   WCNS_E_5_41      WCNSE5.f90:2087-2759  with flux_Roe
-  turbulence(nb)   turblence.f90:15-103, SA one-equation model, one block step
-                   (sources, diffusion, NND convection, scalar LU-SGS, update,
-                   cal_vist_SA, max_control_turbulence)
+  turbulence(nb)   turblence.f90:15-103, one block step of the SA one-equation
+                   model (sources, diffusion, NND convection, scalar LU-SGS,
+                   update, cal_vist_SA, max_control_turbulence) and of the SST
+                   k-omega model (cal_rhs_source_SST, cal_rhs_viscous_turbulence_dif,
+                   cal_spec_turbulence, the same convection and sweeps per
+                   variable, update, cal_vist_SST)
 
 The SA inputs are states of a small wall-bounded block made by the C oracle
 (a few pseudo-time steps, then qke and velocity perturbed so that the
@@ -97,6 +100,62 @@ def sa_step(lib, dims, scal, fields):
     return qo, vo
 
 
+def sst_step(lib, dims, scal, fields):
+    """One reference SST block step on C-frame fields; returns (k, omega, vist)."""
+    names = SA_FIELDS + ["qkw"]
+    arrs = [np.ascontiguousarray(fields[n], dtype=np.float64) for n in names]
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    ko, wo, vo = (np.zeros_like(arrs[0]) for _ in range(3))
+    d = (C.c_int * 3)(*dims)
+    s = (C.c_double * 8)(*scal)
+    lib.ref_sst_step.restype = C.c_int
+    rc = lib.ref_sst_step(d, s, ptrs, ko.ctypes.data_as(C.c_void_p), wo.ctypes.data_as(C.c_void_p),
+                          vo.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise RuntimeError(f"ref_sst_step returned {rc}")
+    return ko, wo, vo
+
+
+def sst_inflow(prm):
+    """inflow_turbulence (turblence.f90:3166-3181) for SST, after the Reynolds
+    rescaling of init_inflow (PreProcess.f90:994)."""
+    reynolds = (prm.reynolds / prm.lfref) * prm.lref
+    omgoo = 10.0 * (prm.lref / prm.lfref)
+    return reynolds, omgoo * 1.0e-6 / reynolds, omgoo
+
+
+def sst_scal(prm):
+    reynolds, koo, omgoo = sst_inflow(prm)
+    return (reynolds, koo, omgoo, prm.kmaxlim, prm.timedt_turb, prm.xk, prm.xb, prm.efix)
+
+
+def sst_case(ni, nj, nk, reynolds=1.0e5, moo=0.5):
+    from wcns_amd import cases
+    case = cases.bump_channel_case(ni=ni, nj=nj, nk=nk, dy_wall=0.01, growth=1.2,
+                                   reynolds=reynolds, moo=moo)
+    case.params.nlamtur = 2
+    return case
+
+
+def sst_inputs(case, steps, seed, kick):
+    """As sa_inputs, with k and omega both scaled by exp(kick*N(0,1))."""
+    from oracle import oracle as O
+    oc = O.OracleCase(case)
+    if steps:
+        oc.step(steps)
+    for st in ("boundary_all", "initial_q_dq_c_t", "timestep"):
+        oc.stage(st)
+    rng = np.random.default_rng(seed)
+    f = {n: oc.field(0, n).copy() for n in SA_FIELDS + ["qkw"]}
+    if kick:
+        for n in ("qke", "qkw"):
+            f[n] = f[n] * np.exp(kick * rng.standard_normal(f[n].shape))
+        for n in ("u", "v", "w"):
+            f[n] = f[n] + 0.05 * kick * rng.standard_normal(f[n].shape)
+    oc.close()
+    return f
+
+
 def sa_scal(prm):
     return (prm.reynolds, MUOO, prm.kmaxlim, prm.timedt_turb, prm.xk, prm.xb, prm.efix)
 
@@ -175,9 +234,26 @@ def main():
         out[f"{tag}_vist"] = vo
         sa.append(tag)
     out["sa_cases"] = np.array(sa)
+
+    sst = []
+    for tag, dims, steps, seed, kick in (("sst_a", (12, 10, 8), 2, 4, 0.0),
+                                          ("sst_b", (12, 10, 8), 2, 5, 1.0),
+                                          ("sst_c", (14, 9, 7), 1, 6, 3.0)):
+        case = sst_case(*dims)
+        f = sst_inputs(case, steps, seed, kick)
+        ko, wo, vo = sst_step(lib, dims, sst_scal(case.params), f)
+        for n in SA_FIELDS + ["qkw"]:
+            out[f"{tag}_in_{n}"] = f[n]
+        out[f"{tag}_dims"] = np.array(dims)
+        out[f"{tag}_qke"] = ko
+        out[f"{tag}_qkw"] = wo
+        out[f"{tag}_vist"] = vo
+        sst.append(tag)
+    out["sst_cases"] = np.array(sst)
     path = os.path.join(HERE, "serial_golden.npz")
     np.savez_compressed(path, **out)
-    print(f"wrote {path}: {len(lines)} Roe WCNS lines, {len(sa)} SA block steps, {len(out)} arrays")
+    print(f"wrote {path}: {len(lines)} Roe WCNS lines, {len(sa)} SA and {len(sst)} SST block steps, "
+          f"{len(out)} arrays")
 
 
 if __name__ == "__main__":

@@ -139,3 +139,42 @@ def test_flux_roe_against_reference_fresh_inputs():
         pr = np.array([0.2 + 2 * rng.random(), *(3 * rng.standard_normal(3)), 0.1 + 3 * rng.random()])
         nv = np.array([*rng.standard_normal(3), 0.0])
         np.testing.assert_array_equal(O.flux_roe(pl, pr, nv), mgs.flux_roe(lib, pl, pr, nv))
+
+
+def _oracle_sst_step(mgs, dims, fields):
+    oc = O.OracleCase(mgs.sst_case(*dims))
+    for n in mgs.SA_FIELDS + ["qkw"]:
+        oc.set_field(0, n, fields[n])
+    oc.turbulence(0)
+    out = oc.field(0, "qke"), oc.field(0, "qkw"), oc.field(0, "vist")
+    oc.close()
+    return out
+
+
+@pytest.mark.parametrize("tag", [str(k) for k in SER["sst_cases"]])
+def test_sst_block_step_golden(tag):
+    """The SST k-omega step of one block (sources with the F1 blending, diffusion,
+    diagonal, NND convection and scalar LU-SGS per variable, update, mu_t of
+    cal_vist_SST, max_control), bit for bit against the reference."""
+    mgs = _serial_module()
+    dims = tuple(int(d) for d in SER[f"{tag}_dims"])
+    fields = {n: SER[f"{tag}_in_{n}"] for n in mgs.SA_FIELDS + ["qkw"]}
+    k, w, v = _oracle_sst_step(mgs, dims, fields)
+    assert (k != fields["qke"]).sum() > 0 and (w != fields["qkw"]).sum() > 0
+    np.testing.assert_array_equal(k, SER[f"{tag}_qke"])
+    np.testing.assert_array_equal(w, SER[f"{tag}_qkw"])
+    np.testing.assert_array_equal(v, SER[f"{tag}_vist"])
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SA), reason="serial reference library not built here")
+@pytest.mark.parametrize("seed,kick,dims,steps", [(21, 0.5, (16, 12, 9), 3), (22, 2.5, (13, 11, 10), 1)])
+def test_sst_block_step_against_reference_fresh_inputs(seed, kick, dims, steps):
+    mgs = _serial_module()
+    lib = mgs.load()
+    case = mgs.sst_case(*dims)
+    f = mgs.sst_inputs(case, steps, seed, kick)
+    kr, wr, vr = mgs.sst_step(lib, dims, mgs.sst_scal(case.params), f)
+    k, w, v = _oracle_sst_step(mgs, dims, f)
+    np.testing.assert_array_equal(k, kr)
+    np.testing.assert_array_equal(w, wr)
+    np.testing.assert_array_equal(v, vr)

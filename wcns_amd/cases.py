@@ -44,6 +44,7 @@ class Params:
     kmaxlim: float = 30000.0
     xk: float = -1.0
     xb: float = 1.0
+    lref: float = 1.0
 
 
 @dataclass

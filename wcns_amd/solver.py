@@ -50,7 +50,7 @@ class Params(C.Structure):
                                           "cfl", "timedt_rate", "efix", "csrv", "gama", "prl", "prt")] + \
                [(n, C.c_int) for n in ("nvis", "nlamtur", "nscheme", "nflux", "ntmst")] + \
                [(n, C.c_double) for n in ("sref", "lfref", "xref", "yref", "zref")] + [("nwholefield", C.c_int)] + \
-               [(n, C.c_double) for n in ("timedt_turb", "kmaxlim", "xk", "xb")]
+               [(n, C.c_double) for n in ("timedt_turb", "kmaxlim", "xk", "xb", "lref")]
 
 
 _lib = None
