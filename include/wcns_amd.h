@@ -41,7 +41,7 @@ typedef struct {
     double moo, reynolds, attack, sideslip, tref, twall;
     double cfl, timedt_rate, efix, csrv, gama, prl, prt;
     int nvis;      /* 0 Euler, 1 Navier-Stokes                          */
-    int nlamtur;   /* -1 laminar, 1 Spalart-Allmaras (nameturb 'SA')      */
+    int nlamtur;   /* -1 laminar, 1 Spalart-Allmaras, 2 SST k-omega      */
     int nscheme;   /* 4 (WCNS_E_5) or 41 (WCNS_E_5_41)                   */
     int nflux;     /* 1 Steger-Warming, 3 Roe                            */
     int ntmst;     /* 0 local time step                                  */
@@ -91,8 +91,9 @@ int wcns_set_lhs_precision(wcns_ctx *ctx, int prec);
 /* which: 0 r, 1 u, 2 v, 3 w, 4 p, 5 t, 7 dtdt, 8 sra, 9 srb, 10 src,
  *        11 srva, 12 srvb, 13 srvc, 14 visl, 16 vol, 17..25 kcx..ctz,
  *        100 dq (5 G3 arrays, variable-major), 102 rhs (the residual R);
- *        with the SA model (nlamtur = 1) also 15 vist (mu_t), 26 qke (the SA
- *        working variable nu~, mb_qke), 27 dist (wall distance, nodes 1..n) */
+ *        with a turbulence model (nlamtur = 1 SA, 2 SST) also 15 vist (mu_t),
+ *        26 qke (mb_qke(:,:,:,1): SA nu~ or SST k), 27 dist (wall distance,
+ *        nodes 1..n) and, for SST, 28 qkw (mb_qke(:,:,:,2), omega) */
 int wcns_set_prim(wcns_ctx *ctx, int nb, const double *r, const double *u,
                   const double *v, const double *w, const double *p);
 int wcns_get_field(wcns_ctx *ctx, int nb, int which, double *out);

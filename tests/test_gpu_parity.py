@@ -390,6 +390,7 @@ GRAPH_CASES = {
     "bump-laminar": lambda: cases.bump_channel_case(ni=24, nj=16, nk=12, dy_wall=1e-3, growth=1.2, reynolds=1e4),
     "sa-multiblock": _sa_mixed_bc,
     "cart-euler": lambda: cases.cartesian_periodic_case(ni=24, nj=16, nk=12, seed=3, amp=0.02),
+    "sst-multiblock": lambda: _sst_mixed_bc(),
 }
 
 
@@ -407,7 +408,7 @@ def test_graph_replay_matches_plain_launches(name):
     b.step(1, history=False)
     b.step_graph(3)                  # odd: one graph pair + one plain step
     b.step_graph(1)
-    names = ["r", "u", "v", "w", "p"] + (["qke", "vist"] if case.params.nlamtur == 1 else [])
+    names = ["r", "u", "v", "w", "p"] + {1: ["qke", "vist"], 2: ["qke", "qkw", "vist"]}.get(case.params.nlamtur, [])
     for nb in range(len(case.blocks)):
         for nm in names:
             np.testing.assert_array_equal(a.field(nb, nm), b.field(nb, nm), err_msg=f"block {nb} {nm}")
@@ -431,7 +432,69 @@ def test_autotuned_tiles_give_identical_steps(name):
     np.testing.assert_array_equal(a.field(0, "r"), r_before)
     a.step(3, history=False)
     b.step(5, history=False)
-    names = ["r", "u", "v", "w", "p"] + (["qke", "vist"] if case.params.nlamtur == 1 else [])
+    names = ["r", "u", "v", "w", "p"] + {1: ["qke", "vist"], 2: ["qke", "qkw", "vist"]}.get(case.params.nlamtur, [])
     for nb in range(len(case.blocks)):
         for nm in names:
             np.testing.assert_array_equal(a.field(nb, nm), b.field(nb, nm), err_msg=f"block {nb} {nm}")
+
+
+# ---------------------------------------------------------------- SST k-omega
+def _sst_bump(**kw):
+    case = cases.bump_channel_case(dy_wall=1e-3, growth=1.2, reynolds=1e5, **kw)
+    case.params.nlamtur = 2
+    return case
+
+
+def _sst_mixed_bc():
+    case = cases.multiblock_case(nvis=1, wall=True, periodic_i=False, ilo_bc=5, ihi_bc=6, jhi_bc=3)
+    case.params.nlamtur = 2
+    return case
+
+
+@pytest.mark.parametrize("kw", SA_CASES, ids=lambda k: f"{k['ni']}x{k['nj']}x{k['nk']}-s{k['nscheme']}-f{k['nflux']}")
+def test_sst_stage_by_stage(kw):
+    """SST bump channel: initial k, omega, mu_t; each step the k/omega ghosts
+    (wall omega value, far field, periodic), the SST step inside r_h_s_vis and
+    the viscous residual with its mu_t."""
+    case = _sst_bump(**kw)
+    blk = case.blocks[0]
+    print(f"SST case {kw}", flush=True)
+    s = _solver(case)
+    o = O.OracleCase(case)
+    for name in ("qke", "qkw", "vist"):
+        e = _rel_err(_interior(s.field(0, name), blk, 1), _interior(o.field(0, name), blk, 1))
+        assert e == 0.0, f"initial {name}: {e:.3e}"
+    for step in range(3):
+        s.boundary_all(); o.stage("boundary_all")
+        _cmp_prim(s, o, blk, REL)
+        for name in ("qke", "qkw", "vist"):
+            e = _rel_err(_interior(s.field(0, name), blk, 1), _interior(o.field(0, name), blk, 1))
+            assert e <= REL, f"{name} after BCs, step {step}: {e:.3e}"
+        s.initial_q_dq_c_t(); o.stage("initial_q_dq_c_t")
+        s.timestep_tgh(); o.stage("timestep")
+        s.r_h_s_vis(); o.stage("rhs_vis")
+        for name in ("qke", "qkw", "vist"):
+            e = _rel_err(_interior(s.field(0, name), blk, 1), _interior(o.field(0, name), blk, 1))
+            assert e <= REL, f"{name} after the SST step, step {step}: {e:.3e}"
+        rg = _interior(s.field(0, "rhs"), blk)
+        ro = _interior(o.field(0, "dq"), blk)
+        assert _rel_err(rg, ro) <= REL, f"viscous rhs step {step}: {_rel_err(rg, ro):.3e}"
+        s.communicate_dq_npp(); o.stage("communicate_dq")
+        s.r_h_s_invis(); o.stage("rhs_invis")
+        s.communicate_dq_npp(); o.stage("communicate_dq")
+        s.l_h_s_tgh(); o.stage("lhs")
+        s.communicate_pv_npp(); o.stage("communicate_pv")
+        _cmp_prim(s, o, blk, REL, ghost=0)
+
+
+def test_sst_multiblock_mixed_bcs_20_steps():
+    case = _sst_mixed_bc()
+    print(f"SST multiblock case: {len(case.blocks)} blocks", flush=True)
+    s = _solver(case)
+    o = O.OracleCase(case)
+    hs = s.step(20)
+    ho = o.step(20)
+    rel = np.abs(hs - ho) / ho
+    print("SST multiblock residual history rel err max", rel.max(), flush=True)
+    assert rel.max() <= 1e-10
+    _cmp_all_blocks(s, o, case, ("r", "u", "v", "w", "p", "qke", "qkw", "vist"), 1e-11)

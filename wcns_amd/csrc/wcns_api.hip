@@ -64,9 +64,10 @@ struct wcns_ctx {
     double *t = nullptr, *metric[10] = {nullptr}, *rhs = nullptr, *dq = nullptr;
     double *dtdt = nullptr, *sr[3] = {nullptr}, *srv[3] = {nullptr}, *visl = nullptr;
     double *dn = nullptr;                   // 12 node derivatives (viscous); SA gradients before them
-    double *qke = nullptr, *vist = nullptr, *dist = nullptr;   // Spalart-Allmaras fields
-    SaWork sa{};                            // SA scratch (grad aliases dn)
-    double *sa_pool = nullptr;              // viseq, un[3], dqke, spec
+    double *qke = nullptr, *vist = nullptr, *dist = nullptr;   // turbulence-model fields
+    double *qkw = nullptr;                  // SST omega
+    SaWork sa{};                            // turbulence scratch (grad aliases dn)
+    double *sa_pool = nullptr;              // viseq, un[3], dqke, spec (+ SST arrays)
     double *coed = nullptr;                 // LU-SGS diagonal per node
     float *dq32 = nullptr;                  // dQ of the FP32 implicit operator
     void *skew[2] = {nullptr, nullptr};     // plane-major sweep operands, FP64 / FP32 implicit operator
@@ -134,7 +135,7 @@ struct wcns_ctx {
         F.srva = srv[0]; F.srvb = srv[1]; F.srvc = srv[2];
         F.visl = visl; F.counters = counters; F.total = total; F.coed = coed;
         F.dq32 = lhs_prec == WCNS_LHS_FP32 ? dq32 : nullptr;
-        F.qke = qke; F.vist = vist; F.dist = dist;
+        F.qke = qke; F.vist = vist; F.dist = dist; F.qkw = qkw;
         return F;
     }
 };
@@ -404,7 +405,7 @@ int wcns_destroy(wcns_ctx *ctx)
     for (int m = 0; m < 3; m++) { hipFree(ctx->sr[m]); hipFree(ctx->srv[m]); }
     void *pl[] = {ctx->d_sblocks, ctx->d_work, ctx->d_xch};
     for (void *p : pl) if (p) hipFree(p);
-    void *ps[] = {ctx->d_grid, ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
+    void *ps[] = {ctx->d_grid, ctx->qkw, ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
                   ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
@@ -551,9 +552,11 @@ int wcns_setup(wcns_ctx *ctx)
     if (p.nscheme != 4 && p.nscheme != 41) return fail(ctx, WCNS_E_UNSUPPORTED, "nscheme must be 4 or 41");
     if (p.nflux != 1 && p.nflux != 3) return fail(ctx, WCNS_E_UNSUPPORTED, "nflux must be 1 (SW) or 3 (Roe)");
     if (p.nvis != 0 && p.nvis != 1) return fail(ctx, WCNS_E_ARG, "nvis must be 0 or 1");
-    if (p.nlamtur >= 0 && !(p.nlamtur == 1 && p.nvis == 1))
-        return fail(ctx, WCNS_E_UNSUPPORTED, "of the turbulence models only SA (nlamtur = 1, nvis = 1) is built");
-    const bool turb = p.nlamtur == 1;
+    if (p.nlamtur >= 0 && !((p.nlamtur == 1 || p.nlamtur == 2) && p.nvis == 1))
+        return fail(ctx, WCNS_E_UNSUPPORTED, "turbulence models: SA (nlamtur = 1) or SST (nlamtur = 2), with nvis = 1");
+    const bool turb = p.nlamtur == 1 || p.nlamtur == 2;
+    const bool sst = p.nlamtur == 2;
+    if (p.lfref <= 0.0 || p.lref <= 0.0) return fail(ctx, WCNS_E_ARG, "lfref and lref must be positive");
     if (p.ntmst != 0) return fail(ctx, WCNS_E_UNSUPPORTED, "only local time stepping (ntmst = 0)");
     // layout
     long off = 0;
@@ -596,12 +599,20 @@ int wcns_setup(wcns_ctx *ctx)
         TRY(dalloc(ctx, &ctx->qke, T));
         TRY(dalloc(ctx, &ctx->vist, T));
         TRY(dalloc(ctx, &ctx->dist, T));
-        TRY(dalloc(ctx, &ctx->sa_pool, 6 * T));
+        TRY(dalloc(ctx, &ctx->sa_pool, (sst ? 13 : 6) * T));
         ctx->sa.grad = ctx->dn;
-        ctx->sa.viseq = ctx->sa_pool;
+        ctx->sa.viseq[0] = ctx->sa_pool;
         for (int d = 0; d < 3; d++) ctx->sa.un[d] = ctx->sa_pool + (1 + d) * T;
-        ctx->sa.dqke = ctx->sa_pool + 4 * T;
-        ctx->sa.spec = ctx->sa_pool + 5 * T;
+        ctx->sa.dqke[0] = ctx->sa_pool + 4 * T;
+        ctx->sa.spec[0] = ctx->sa_pool + 5 * T;
+        if (sst) {
+            TRY(dalloc(ctx, &ctx->qkw, T));
+            ctx->sa.gradw = ctx->sa_pool + 6 * T;
+            ctx->sa.viseq[1] = ctx->sa_pool + 9 * T;
+            ctx->sa.dqke[1] = ctx->sa_pool + 10 * T;
+            ctx->sa.spec[1] = ctx->sa_pool + 11 * T;
+            ctx->sa.f1 = ctx->sa_pool + 12 * T;
+        }
     }
     TRY(dalloc(ctx, &ctx->coed, T));
     TRY(dalloc(ctx, &ctx->d_sweep_err, 1));
@@ -647,15 +658,22 @@ int wcns_setup(wcns_ctx *ctx)
     const double pai = 3.14159265;
     double att = p.attack * pai / 180.0, ssl = p.sideslip * pai / 180.0;
     d.gama = p.gama; d.efix = p.efix; d.cfl = p.cfl; d.timedt_rate = p.timedt_rate; d.moo = p.moo;
-    d.csrv = p.csrv; d.re = 1.0 / p.reynolds; d.reynolds = p.reynolds; d.visc = 110.4 / p.tref; d.prl = p.prl; d.prt = p.prt;
+    d.reynolds = (p.reynolds / p.lfref) * p.lref;       // PreProcess.f90:994
+    d.csrv = p.csrv; d.re = 1.0 / d.reynolds; d.visc = 110.4 / p.tref; d.prl = p.prl; d.prt = p.prt;
     d.roo = 1.0; d.uoo = cos(att) * cos(ssl); d.voo = sin(att) * cos(ssl); d.woo = sin(ssl);
     d.poo = 1.0 / (p.gama * p.moo * p.moo);
     d.coo = sqrt(p.gama * d.poo / d.roo);
     d.twall_fixed = p.twall > 0.0;
     d.twall_nd = p.twall / p.tref;
     d.nvis = p.nvis; d.nscheme = p.nscheme; d.nflux = p.nflux;
-    d.turb = turb ? 1 : 0;
-    d.muoo = 1.0e-1;                        // turblence.f90:3166-3181 inflow_turbulence (SA)
+    d.turb = turb ? p.nlamtur : 0;
+    // turblence.f90:3166-3181 inflow_turbulence
+    d.vtoo = 1.0e-6;
+    d.muoo = 1.0e-1;
+    if (sst) {
+        d.omgoo = 10.0 * (p.lref / p.lfref);
+        d.koo = d.omgoo * d.vtoo / d.reynolds;
+    }
     d.kmaxlim = p.kmaxlim; d.timedt_turb = p.timedt_turb; d.xk = p.xk; d.xb = p.xb;
 
     // wall distance (SA): every node of every no-slip wall region (bctype 2, 20, 21),
@@ -704,12 +722,17 @@ int wcns_setup(wcns_ctx *ctx)
     }
     if (d_wall) hipFree(d_wall);
     if (turb) {
-        // init_SA (turblence.f90:2701-2734): nu~ = muoo, mu_t = muoo fv1(muoo) on -1..n+1
+        // init_SA (turblence.f90:2701-2734): nu~ = muoo, mu_t = muoo fv1(muoo);
+        // init_SST (:3091-3117): k = koo, omega = omgoo, mu_t = vtoo; on -1..n+1
         const double cv13 = 7.1 * 7.1 * 7.1, kafan = d.muoo;
         const double kf3 = kafan * kafan * kafan;
         const double fv1 = kf3 / (kf3 + cv13);
-        for (auto &b : ctx->blocks)
-            if (launch_sa_init(b.db, F, d.muoo, d.muoo * fv1, ctx->stream)) return fail(ctx, WCNS_E_HIP, "SA init");
+        F = ctx->fields();
+        for (auto &b : ctx->blocks) {
+            int e = sst ? launch_sa_init(b.db, F, d.koo, d.omgoo, d.vtoo, ctx->stream)
+                        : launch_sa_init(b.db, F, d.muoo, 0.0, d.muoo * fv1, ctx->stream);
+            if (e) return fail(ctx, WCNS_E_HIP, "turbulence init");
+        }
     }
     // symmetry-plane normals (BC.f90:381-396) need the metrics at the face centre
     for (auto &b : ctx->blocks)
@@ -829,6 +852,7 @@ static int field_ptr(wcns_ctx *ctx, int which, double **p, int *ncomp)
     case 16: *p = ctx->metric[9]; return 0;
     case 26: if (!ctx->qke) return WCNS_E_ARG; *p = ctx->qke; return 0;
     case 27: if (!ctx->dist) return WCNS_E_ARG; *p = ctx->dist; return 0;
+    case 28: if (!ctx->qkw) return WCNS_E_ARG; *p = ctx->qkw; return 0;
     case 100:
         if (ctx->lhs_prec == WCNS_LHS_FP32) return WCNS_E_UNSUPPORTED;
         *p = ctx->dq; *ncomp = 5; return 0;
@@ -939,10 +963,13 @@ int wcns_r_h_s_vis(wcns_ctx *ctx)
             if (launch_sa_rhs(ctx->grid[GRID_NODES], ctx->grid[GRID_SA_RHS], F, ctx->sa, ctx->dp, ctx->stream))
                 return fail(ctx, WCNS_E_HIP, "SA rhs launch");
         }
-        for (int fwd = 1; fwd >= 0; fwd--) {
-            Scope sc(ctx, fwd ? WCNS_T_SA_SWEEP_L : WCNS_T_SA_SWEEP_U);
-            if (launch_sa_sweep(fwd, ctx->plan, ctx->sa, ctx->stream)) return fail(ctx, WCNS_E_HIP, "SA sweep launch");
-        }
+        // lusgs_turbulence: forward then backward sweep per variable (independent variables)
+        for (int var = 0; var < ctx->dp.turb; var++)
+            for (int fwd = 1; fwd >= 0; fwd--) {
+                Scope sc(ctx, fwd ? WCNS_T_SA_SWEEP_L : WCNS_T_SA_SWEEP_U);
+                if (launch_sa_sweep(fwd, var, ctx->plan, ctx->sa, ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "turbulence sweep launch");
+            }
         Scope sc(ctx, WCNS_T_SA_UPDATE);
         if (launch_sa_update(ctx->grid[GRID_SA_UPD], F, ctx->sa, ctx->dp, ctx->stream))
             return fail(ctx, WCNS_E_HIP, "SA update launch");

@@ -34,8 +34,9 @@ struct DParams {
     double gama, efix, cfl, timedt_rate, moo, csrv, re, reynolds, visc, prl, prt;
     double roo, uoo, voo, woo, poo, coo, twall_nd;
     int nvis, nscheme, nflux, twall_fixed;
-    // Spalart-Allmaras (turb = 1): freestream nu~, namelist turbulence values
-    double muoo, kmaxlim, timedt_turb, xk, xb;
+    // turbulence model: turb = 0 laminar, 1 Spalart-Allmaras, 2 SST k-omega;
+    // freestream nu~ (SA), k, omega and mu_t (SST), namelist turbulence values
+    double muoo, koo, omgoo, vtoo, kmaxlim, timedt_turb, xk, xb;
     int turb;
 };
 
@@ -51,7 +52,8 @@ struct DFields {
     double *coed;                       // LU-SGS diagonal 1/(1/dt + sum of radii), per node
     float *dq32;                        // FP32 sweep operand (mixed mode)
     unsigned long long *counters;       // 0 fallbacks, 1 repairs, 2 roe, 3 nrepair list len
-    double *qke, *vist, *dist;          // SA working variable, mu_t, wall distance (null when laminar)
+    double *qke, *vist, *dist;          // SA nu~ / SST k, mu_t, wall distance (null when laminar)
+    double *qkw;                        // SST omega (null otherwise)
     long total;
 };
 

@@ -124,19 +124,23 @@ int residual_parts(const DBlock &b);
 int launch_residual_final(const double *partial, int nparts, double *out, hipStream_t s);
 int launch_copy_interior(const DBlock &b, const double *src, double *dst, hipStream_t s);
 
-// Spalart-Allmaras: scratch of one step (field-pool frame, total elements each)
+// Turbulence model scratch of one step (field-pool frame, total elements each);
+// the second variable's arrays (SST omega) are null for SA.
 struct SaWork {
-    double *grad;          // 12 arrays: d(u, v, w, nu~)/d(x, y, z), variable-major
-    double *viseq;         // 1.5 (mu/rho + nu~)
+    double *grad;          // 12 arrays: d(u, v, w, nu~ or k)/d(x, y, z), variable-major
+    double *gradw;         // 3 arrays: d(omega)/d(x, y, z)
+    double *viseq[2];      // equivalent viscosity per variable
     double *un[3];         // contravariant velocities along i, j, k
-    double *dqke, *spec;   // right-hand side / sweep unknown, diagonal
+    double *dqke[2];       // right-hand side / sweep unknown per variable
+    double *spec[2];       // diagonal per variable
+    double *f1;            // SST blending function F1
 };
 int launch_wall_dist(const DBlock &b, const double *x, const double *y, const double *z, const double *wall, int nw,
                      double *dist, hipStream_t s);
-int launch_sa_init(const DBlock &b, const DFields &F, double muoo, double vt, hipStream_t s);
+int launch_sa_init(const DBlock &b, const DFields &F, double q0, double w0, double vt, hipStream_t s);
 int launch_sa_rhs(const BlockGrid &nodes, const BlockGrid &inner, const DFields &F, const SaWork &W,
                   const DParams &P, hipStream_t s);
-int launch_sa_sweep(int fwd, const SweepPlan &S, const SaWork &W, hipStream_t s);
+int launch_sa_sweep(int fwd, int var, const SweepPlan &S, const SaWork &W, hipStream_t s);
 int launch_sa_update(const BlockGrid &G, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s);
 
 // wall force

@@ -1,12 +1,14 @@
-// wcns_sa.hip -- the Spalart-Allmaras one-equation model as the reference runs
-// it inside r_h_s_vis (RHS.f90:22 -> turblence.f90:15-103 turbulence ->
-// differential_model_turbulence), nameturb 'SA', ncmpcor 0, ndes 0, method 1,
-// and its set-up (wall distance, uniform initial field).
+// wcns_sa.hip -- the turbulence models as the reference runs them inside
+// r_h_s_vis (RHS.f90:22 -> turblence.f90:15-103 turbulence ->
+// differential_model_turbulence), ncmpcor 0, ndes 0, method 1: the
+// Spalart-Allmaras one-equation model (nameturb 'SA', nlamtur 1) and the SST
+// k-omega model (nameturb 'SST', nlamtur 2), and their set-up (wall distance,
+// uniform initial field).
 //
 //   k_wall_dist   parallel.f90:4313-4468 calc_wall_dists_parallel: squared
 //                 distance to every no-slip wall node, minimum, sqrt + small;
 //                 wall points staged through LDS in chunks.
-//   k_sa_init     turblence.f90:2701-2734 init_SA.
+//   k_sa_init     turblence.f90:2701-2734 init_SA / :3091-3117 init_SST.
 //   k_sa_node     per node 1..n: the central-difference gradients of u, v, w
 //                 and nu~ (dfidxyz_center, turblence.f90:753-849), the
 //                 equivalent viscosity (viscosity_equivalent_SA_r,
@@ -26,6 +28,13 @@
 //                 five-variable sweeps (one word per boundary node).
 //   k_sa_update   update_turbulence (:585-623), cal_vist_SA (:2735-2760) and
 //                 max_control_turbulence (:625-660), node by node.
+//   SST: k_sa_node also forms d(omega)/dx, the blending function F1 and the
+//   equivalent viscosities of the first loop of cal_rhs_source_SST
+//   (:2810-2923); k_sst_rhs is the per-node fusion of the rest of
+//   cal_rhs_source_SST (:2925-3086), cal_rhs_viscous_turbulence_dif (:851-1019),
+//   cal_spec_turbulence (:175-215) and the NND convection of both variables;
+//   the scalar sweeps run once per variable; k_sa_update applies
+//   cal_vist_SST (:3119-3164).
 #include "wcns_kernels.hpp"
 
 namespace wcns {
@@ -91,7 +100,7 @@ __global__ void __launch_bounds__(WD_THREADS) k_wall_dist(DBlock b, const double
 }
 
 // ------------------------------------------------------------------ init
-__global__ void k_sa_init(DBlock b, DFields F, double muoo, double vt)
+__global__ void k_sa_init(DBlock b, DFields F, double q0, double w0, double vt)
 {
     const long n = (long)(b.ni + 3) * (b.nj + 3) * (b.nk + 3);
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -99,13 +108,18 @@ __global__ void k_sa_init(DBlock b, DFields F, double muoo, double vt)
     int i, j, k;
     node_of(b, t, -1, b.ni + 3, b.nj + 3, i, j, k);
     const long g = gidx(b, i, j, k);
-    F.qke[g] = muoo;
+    F.qke[g] = q0;
+    if (F.qkw) F.qkw[g] = w0;
     F.vist[g] = vt;
 }
 
+// cal_rhs_source_SST / cal_vist_SST constants (turblence.f90:2813-2818, 3122)
+constexpr double SST_CGAK0 = 0.85, SST_CGAK1 = 1.0, SST_CGAW0 = 0.5, SST_CGAW1 = 0.856;
+constexpr double SST_BETA0 = 0.075, SST_BETA1 = 0.0828, SST_CDMIN = 1.0e-20, SST_KI = 0.41;
+
 // ------------------------------------------------------------- node pass
 // gradient array m of variable v (u, v, w, nu~) at component c: grad[(3v+c)*total + g]
-__global__ void k_sa_node(BlockGrid G, DFields F, SaWork W)
+__global__ void k_sa_node(BlockGrid G, DFields F, SaWork W, DParams P)
 {
     DBlock b;
     long t;
@@ -138,7 +152,35 @@ __global__ void k_sa_node(BlockGrid G, DFields F, SaWork W)
         W.grad[(3 * v + 2) * T + g] = ddz * vol2;
     }
     const double r = F.r[g], um = F.u[g], vm = F.v[g], wm = F.w[g];
-    W.viseq[g] = 1.5 * (F.visl[g] / r + F.qke[g]);
+    if (P.turb == 2) {
+        // d(omega)/dx, then the first loop of cal_rhs_source_SST (nodes 1..n)
+        const double *fi = F.qkw;
+        const double dkc = fi[gi1] - fi[gi0], det = fi[gj1] - fi[gj0], dct = fi[gk1] - fi[gk0];
+        const double gw[3] = {(dkc * kx + det * ex + dct * cx) * vol2, (dkc * ky + det * ey + dct * cy) * vol2,
+                              (dkc * kz + det * ez + dct * cz) * vol2};
+#pragma unroll
+        for (int c = 0; c < 3; c++) W.gradw[c * T + g] = gw[c];
+        const double re500 = P.re * 500.0, cgaw24 = SST_CGAW1 * 4.0, cgaw22 = SST_CGAW1 * 2.0;
+        const double obetx = 1.0 / 0.09;
+        const double qk = F.qke[g], qw = F.qkw[g], vsl = F.visl[g], vst = F.vist[g];
+        const double rok = r * qk, row = r * qw;
+        const double dn = F.dist[g], dn2 = dn * dn;
+        const double dkxdw = W.grad[9 * T + g] * gw[0] + W.grad[10 * T + g] * gw[1] + W.grad[11 * T + g] * gw[2];
+        const double rowdkdw = r / qw * dkxdw * cgaw22;
+        const double CDkw = amax1(SST_CDMIN, rowdkdw);
+        const double sqrtkwdn = obetx * sqrt(qk) / qw / dn;
+        const double vlrodn2w = re500 * vsl / row / dn2;
+        const double rokdn2 = cgaw24 * rok / CDkw / dn2;
+        const double arg2 = amax1(sqrtkwdn, vlrodn2w);
+        const double arg1 = amin1(arg2, rokdn2);
+        const double f1 = arg1 >= 2.0 ? 1.0 : tanh(pow(arg1, 4.0));
+        const double f2 = 1.0 - f1;
+        W.f1[g] = f1;
+        W.viseq[0][g] = vsl + (f1 * SST_CGAK0 + f2 * SST_CGAK1) * vst;
+        W.viseq[1][g] = vsl + (f1 * SST_CGAW0 + f2 * SST_CGAW1) * vst;
+    } else {
+        W.viseq[0][g] = 1.5 * (F.visl[g] / r + F.qke[g]);
+    }
     // mxdq_std_turbulence's l1 = g . (u, v, w) with g the metric of each direction
     W.un[0][g] = kx * um + ky * vm + kz * wm;
     W.un[1][g] = ex * um + ey * vm + ez * wm;
@@ -161,15 +203,15 @@ __device__ inline long nbr(const DBlock &b, int i, int j, int k, int d)
     return gidx(b, i + (DIR == 0 ? d : 0), j + (DIR == 1 ? d : 0), k + (DIR == 2 ? d : 0));
 }
 
+// diffusion flux difference along DIR of the variable whose gradient arrays are
+// gx, gy, gz and whose equivalent viscosity is viseq
 template <int DIR>
-__device__ inline double sa_diffusion(const DBlock &b, const DFields &F, const SaWork &W, int i, int j, int k,
-                                      long g)
+__device__ inline double sa_diffusion(const DBlock &b, const DFields &F, const double *gx, const double *gy,
+                                      const double *gz, const double *viseq, int i, int j, int k)
 {
     const double *mx = DIR == 0 ? F.kcx : (DIR == 1 ? F.etx : F.ctx);
     const double *my = DIR == 0 ? F.kcy : (DIR == 1 ? F.ety : F.cty);
     const double *mz = DIR == 0 ? F.kcz : (DIR == 1 ? F.etz : F.ctz);
-    const long T = F.total;
-    const double *gx = W.grad + 9 * T, *gy = W.grad + 10 * T, *gz = W.grad + 11 * T;
     double fv[2];
 #pragma unroll
     for (int f = 0; f < 2; f++) {
@@ -177,14 +219,15 @@ __device__ inline double sa_diffusion(const DBlock &b, const DFields &F, const S
         const long qa = nbr<DIR>(b, i, j, k, f), qb = nbr<DIR>(b, i, j, k, f - 1);
         const double nx = 0.5 * (mx[qa] + mx[qb]), ny = 0.5 * (my[qa] + my[qb]), nz = 0.5 * (mz[qa] + mz[qb]);
         const double dmudx = gx[qa] + gx[qb], dmudy = gy[qa] + gy[qb], dmudz = gz[qa] + gz[qb];
-        const double vis = W.viseq[qa] + W.viseq[qb];
+        const double vis = viseq[qa] + viseq[qb];
         fv[f] = (dmudx * nx + dmudy * ny + dmudz * nz) * vis * 0.25;
     }
     return fv[1] - fv[0];
 }
 
 template <int DIR>
-__device__ inline double sa_nnd(const DBlock &b, const DFields &F, const DParams &P, int i, int j, int k)
+__device__ inline double sa_nnd(const DBlock &b, const DFields &F, const DParams &P, const double *qv, int i, int j,
+                                int k)
 {
     const double *mx = DIR == 0 ? F.kcx : (DIR == 1 ? F.etx : F.ctx);
     const double *my = DIR == 0 ? F.kcy : (DIR == 1 ? F.ety : F.cty);
@@ -195,7 +238,7 @@ __device__ inline double sa_nnd(const DBlock &b, const DFields &F, const DParams
 #pragma unroll
     for (int o = 0; o < 5; o++) {
         const long g = nbr<DIR>(b, i, j, k, o - 2);
-        q[o][0] = F.r[g]; q[o][1] = F.u[g]; q[o][2] = F.v[g]; q[o][3] = F.w[g]; q[o][4] = F.qke[g];
+        q[o][0] = F.r[g]; q[o][1] = F.u[g]; q[o][2] = F.v[g]; q[o][3] = F.w[g]; q[o][4] = qv[g];
     }
     double cl[3][5], cr[3][5];
 #pragma unroll
@@ -276,9 +319,12 @@ __global__ void __launch_bounds__(128) k_sa_rhs(BlockGrid G, DFields F, SaWork W
     double sp = 0.0 - amin1(cpmu - 2.0 * cdmu, 0.0) * volume;
 
     // cal_rhs_viscous_turbulence_dif_r
-    dq = dq - re * ro * sa_diffusion<0>(b, F, W, i, j, k, g);
-    dq = dq - re * ro * sa_diffusion<1>(b, F, W, i, j, k, g);
-    dq = dq - re * ro * sa_diffusion<2>(b, F, W, i, j, k, g);
+    {
+        const double *gx = W.grad + 9 * T, *gy = W.grad + 10 * T, *gz = W.grad + 11 * T;
+        dq = dq - re * ro * sa_diffusion<0>(b, F, gx, gy, gz, W.viseq[0], i, j, k);
+        dq = dq - re * ro * sa_diffusion<1>(b, F, gx, gy, gz, W.viseq[0], i, j, k);
+        dq = dq - re * ro * sa_diffusion<2>(b, F, gx, gy, gz, W.viseq[0], i, j, k);
+    }
 
     // cal_spec_turbulence_r
     {
@@ -293,17 +339,108 @@ __global__ void __launch_bounds__(128) k_sa_rhs(BlockGrid G, DFields F, SaWork W
         const double nx = kx * kx + ky * ky + kz * kz;
         const double ny = ex * ex + ey * ey + ez * ez;
         const double nz = cx * cx + cy * cy + cz * cz;
-        const double vis = W.viseq[g] / volume;
+        const double vis = W.viseq[0][g] / volume;
         sp = sp + 2.0 * re * vis * (nx + ny + nz);
     }
 
     // cal_rhs_inviscous_turbulence
-    dq = dq + sa_nnd<0>(b, F, P, i, j, k);
-    dq = dq + sa_nnd<1>(b, F, P, i, j, k);
-    dq = dq + sa_nnd<2>(b, F, P, i, j, k);
+    dq = dq + sa_nnd<0>(b, F, P, F.qke, i, j, k);
+    dq = dq + sa_nnd<1>(b, F, P, F.qke, i, j, k);
+    dq = dq + sa_nnd<2>(b, F, P, F.qke, i, j, k);
 
-    W.dqke[g] = dq;
-    W.spec[g] = sp;
+    W.dqke[0][g] = dq;
+    W.spec[0][g] = sp;
+}
+
+// SST, per interior node 2..n-1: both variables' right-hand sides and diagonals
+__global__ void __launch_bounds__(128) k_sst_rhs(BlockGrid G, DFields F, SaWork W, DParams P)
+{
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
+    const int mi = b.ni - 2, mj = b.nj - 2;
+    const long n = (long)mi * mj * (b.nk - 2);
+    if (t >= n) return;
+    int i, j, k;
+    node_of(b, t, 2, mi, mj, i, j, k);
+    const long g = gidx(b, i, j, k);
+    const long T = F.total;
+    const double re = P.re, reynolds = P.reynolds;
+    const double ki2 = SST_KI * SST_KI, twothd = 0.2 / 0.3;
+    const double betax = 0.09, obetx = 1.0 / betax, ki2osqrtbetx = ki2 / sqrt(betax);
+    const double cgaw22 = SST_CGAW1 * 2.0;
+
+    // cal_rhs_source_SST, second loop
+    const double ro = F.r[g], qk = F.qke[g], qw = F.qkw[g], volume = F.vol[g], vst = F.vist[g];
+    const double rok = ro * qk, row = ro * qw;
+    const double f1 = W.f1[g], f2 = 1.0 - f1;
+    const double beita = f1 * SST_BETA0 + f2 * SST_BETA1;
+    const double cga1 = f1 * SST_CGAW0 + f2 * SST_CGAW1;
+    const double ga = beita * obetx - cga1 * ki2osqrtbetx;
+    const double ux = W.grad[0 * T + g], uy = W.grad[1 * T + g], uz = W.grad[2 * T + g];
+    const double vx = W.grad[3 * T + g], vy = W.grad[4 * T + g], vz = W.grad[5 * T + g];
+    const double wx = W.grad[6 * T + g], wy = W.grad[7 * T + g], wz = W.grad[8 * T + g];
+    const double div_u = ux + vy + wz;
+    const double dkxdw = W.grad[9 * T + g] * W.gradw[0 * T + g] + W.grad[10 * T + g] * W.gradw[1 * T + g] +
+                         W.grad[11 * T + g] * W.gradw[2 * T + g];
+    const double omg = vst * ((2.0 * (ux * ux + vy * vy + wz * wz + uy * vx + uz * wx + wy * vz) + uy * uy + uz * uz +
+                               vx * vx + vz * vz + wx * wx + wy * wy) -
+                              twothd * div_u * div_u) * re -
+                       twothd * div_u * rok;
+    const double cdk = betax * qw, cdw = beita * qw;
+    const double Sd0 = cdk * rok, Sd1 = cdw * row;
+    const double Sp0 = amin1(omg, 20.0 * Sd0);
+    const double Sp1 = ga * ro * omg / vst * reynolds;
+    const double dbw = cgaw22 * f2 * dkxdw / qw / qw;
+    const double Dbar0 = 0.0, Dbar1 = dbw * row;
+    double sp0 = 0.0 + 2.0 * (cdk + twothd * amax1(ux + vy + wz, 0.0)) * volume;
+    double sp1 = 0.0 + 2.0 * (2.0 * cdw + fabs(dbw)) * volume;
+    double dq0 = 0.0 - (Sp0 - Sd0 + Dbar0) * volume;
+    double dq1 = 0.0 - (Sp1 - Sd1 + Dbar1) * volume;
+
+    // cal_rhs_viscous_turbulence_dif
+    {
+        const double *gx = W.grad + 9 * T, *gy = W.grad + 10 * T, *gz = W.grad + 11 * T;
+        const double *hx = W.gradw, *hy = W.gradw + T, *hz = W.gradw + 2 * T;
+        dq0 = dq0 - re * sa_diffusion<0>(b, F, gx, gy, gz, W.viseq[0], i, j, k);
+        dq1 = dq1 - re * sa_diffusion<0>(b, F, hx, hy, hz, W.viseq[1], i, j, k);
+        dq0 = dq0 - re * sa_diffusion<1>(b, F, gx, gy, gz, W.viseq[0], i, j, k);
+        dq1 = dq1 - re * sa_diffusion<1>(b, F, hx, hy, hz, W.viseq[1], i, j, k);
+        dq0 = dq0 - re * sa_diffusion<2>(b, F, gx, gy, gz, W.viseq[0], i, j, k);
+        dq1 = dq1 - re * sa_diffusion<2>(b, F, hx, hy, hz, W.viseq[1], i, j, k);
+    }
+
+    // cal_spec_turbulence
+    {
+        const double um = F.u[g], vm = F.v[g], wm = F.w[g];
+        const double kx = F.kcx[g], ky = F.kcy[g], kz = F.kcz[g];
+        const double ex = F.etx[g], ey = F.ety[g], ez = F.etz[g];
+        const double cx = F.ctx[g], cy = F.cty[g], cz = F.ctz[g];
+        const double unx = um * kx + vm * ky + wm * kz;
+        const double uny = um * ex + vm * ey + wm * ez;
+        const double unz = um * cx + vm * cy + wm * cz;
+        const double dtt = 1.0 / (F.dtdt[g] * P.timedt_turb);
+        const double nx = kx * kx + ky * ky + kz * kz;
+        const double ny = ex * ex + ey * ey + ez * ez;
+        const double nz = cx * cx + cy * cy + cz * cz;
+        sp0 = sp0 + fabs(unx) + fabs(uny) + fabs(unz) + dtt;
+        sp0 = sp0 + 2.0 * re * (W.viseq[0][g] / volume / ro) * (nx + ny + nz);
+        sp1 = sp1 + fabs(unx) + fabs(uny) + fabs(unz) + dtt;
+        sp1 = sp1 + 2.0 * re * (W.viseq[1][g] / volume / ro) * (nx + ny + nz);
+    }
+
+    // cal_rhs_inviscous_turbulence, per variable
+    dq0 = dq0 + sa_nnd<0>(b, F, P, F.qke, i, j, k);
+    dq0 = dq0 + sa_nnd<1>(b, F, P, F.qke, i, j, k);
+    dq0 = dq0 + sa_nnd<2>(b, F, P, F.qke, i, j, k);
+    dq1 = dq1 + sa_nnd<0>(b, F, P, F.qkw, i, j, k);
+    dq1 = dq1 + sa_nnd<1>(b, F, P, F.qkw, i, j, k);
+    dq1 = dq1 + sa_nnd<2>(b, F, P, F.qkw, i, j, k);
+
+    W.dqke[0][g] = dq0;
+    W.dqke[1][g] = dq1;
+    W.spec[0][g] = sp0;
+    W.spec[1][g] = sp1;
 }
 
 // ------------------------------------------------------------ the sweeps
@@ -349,8 +486,8 @@ struct SaOp {
 // backward: dq = dq - (A- dq_upper) / spec, A+- = (l1 +- |l1|)/2 of the
 // neighbour (mxdq_std_turbulence, turblence.f90:560-583).
 template <int FWD>
-__global__ void __launch_bounds__(64) k_sa_sweep(const SweepBlock *blocks, const int2 *work, SaWork W, int *err,
-                                                 long spin_limit)
+__global__ void __launch_bounds__(64) k_sa_sweep(const SweepBlock *blocks, const int2 *work, SaWork W,
+                                                 double *dqv, const double *specv, int *err, long spin_limit)
 {
     constexpr int off = FWD ? -1 : 1;
     const double npn = FWD ? 1.0 : -1.0;
@@ -388,8 +525,8 @@ __global__ void __launch_bounds__(64) k_sa_sweep(const SweepBlock *blocks, const
         const int i = i_at(st);
         if (!inner(i)) return;
         const long g = gidx(b, i, j, k);
-        o.d = W.dqke[g];
-        o.sp = W.spec[g];
+        o.d = dqv[g];
+        o.sp = specv[g];
         o.ui = W.un[0][gidx(b, i + off, j, k)];
         o.uj = W.un[1][gidx(b, i, j + off, k)];
         o.uk = W.un[2][gidx(b, i, j, k + off)];
@@ -431,7 +568,7 @@ __global__ void __launch_bounds__(64) k_sa_sweep(const SweepBlock *blocks, const
                 const double ak = 0.5 * (o.uk + npn * fabs(o.uk));
                 const double rhs0 = ai * prev + aj * nj_ + ak * nk_;
                 out = FWD ? (rhs0 - o.d) / o.sp : o.d - rhs0 / o.sp;
-                W.dqke[gidx(b, i, j, k)] = out;
+                dqv[gidx(b, i, j, k)] = out;
             }
             if (putj) xch_put1(xj_out + (i - 1), out);
             if (putk) xch_put1(xk_out + (i - 1), out);
@@ -464,18 +601,45 @@ __global__ void k_sa_update(BlockGrid G, DFields F, SaWork W, DParams P)
     const bool in2 = i >= 2 && i <= b.ni - 1 && j >= 2 && j <= b.nj - 1 && k >= 2 && k <= b.nk - 1;
     const double r = F.r[g];
     double q = F.qke[g];
-    if (in2) {
-        q = amax1(q + W.dqke[g] / r, 1.0e-20 * P.muoo);
-        F.qke[g] = q;
-    }
     double vt = F.vist[g];
-    if (in1) {
-        const double cv13 = SA_CV1 * SA_CV1 * SA_CV1;
-        const double romu = r * q;
-        const double kafan = romu / F.visl[g];
-        const double kf3 = kafan * kafan * kafan;
-        const double fv1 = kf3 / (kf3 + cv13);
-        vt = romu * fv1;
+    if (P.turb == 2) {
+        double qw = F.qkw[g];
+        if (in2) {
+            q = amax1(q + W.dqke[0][g] / r, 1.0e-20 * P.koo);
+            qw = amax1(qw + W.dqke[1][g] / r, 1.0e-20 * P.omgoo);
+            F.qke[g] = q;
+            F.qkw[g] = qw;
+        }
+        if (in1) {
+            // cal_vist_SST, with the velocity gradients of this step's source term
+            const long T = F.total;
+            const double a1 = 0.31, obetx = 2.0 / 0.09;
+            const double a1reynolds = a1 * P.reynolds, b1 = 500.0 * P.re;
+            const double vsl = F.visl[g];
+            const double dn = F.dist[g], dn2 = dn * dn;
+            const double sqrtkwdn = obetx * sqrt(q) / qw / dn;
+            const double vlrodn2w = b1 * vsl / r / dn2 / qw;
+            const double arg2 = amax1(sqrtkwdn, vlrodn2w);
+            const double f2 = arg2 >= 4.0 ? 1.0 : tanh(arg2 * arg2);
+            const double a_ = W.grad[1 * T + g] - W.grad[3 * T + g];
+            const double b_ = W.grad[5 * T + g] - W.grad[7 * T + g];
+            const double c_ = W.grad[6 * T + g] - W.grad[2 * T + g];
+            const double omg = sqrt(a_ * a_ + b_ * b_ + c_ * c_);
+            vt = a1reynolds * r * q / amax1(a1 * qw, omg * f2);
+        }
+    } else {
+        if (in2) {
+            q = amax1(q + W.dqke[0][g] / r, 1.0e-20 * P.muoo);
+            F.qke[g] = q;
+        }
+        if (in1) {
+            const double cv13 = SA_CV1 * SA_CV1 * SA_CV1;
+            const double romu = r * q;
+            const double kafan = romu / F.visl[g];
+            const double kf3 = kafan * kafan * kafan;
+            const double fv1 = kf3 / (kf3 + cv13);
+            vt = romu * fv1;
+        }
     }
     if (P.kmaxlim > 1.0e-15) vt = amin1(vt, P.kmaxlim * F.visl[g]);
     F.vist[g] = vt;
@@ -491,28 +655,38 @@ int launch_wall_dist(const DBlock &b, const double *x, const double *y, const do
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_sa_init(const DBlock &b, const DFields &F, double muoo, double vt, hipStream_t s)
+int launch_sa_init(const DBlock &b, const DFields &F, double q0, double w0, double vt, hipStream_t s)
 {
     long n = (long)(b.ni + 3) * (b.nj + 3) * (b.nk + 3);
-    hipLaunchKernelGGL(k_sa_init, grid1(n, 256), dim3(256), 0, s, b, F, muoo, vt);
+    hipLaunchKernelGGL(k_sa_init, grid1(n, 256), dim3(256), 0, s, b, F, q0, w0, vt);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 int launch_sa_rhs(const BlockGrid &nodes, const BlockGrid &inner, const DFields &F, const SaWork &W,
                   const DParams &P, hipStream_t s)
 {
-    if (nodes.nwg) hipLaunchKernelGGL(k_sa_node, dim3(nodes.nwg), dim3(256), 0, s, nodes, F, W);
-    if (inner.nwg) hipLaunchKernelGGL(k_sa_rhs, dim3(inner.nwg), dim3(128), 0, s, inner, F, W, P);
+    if (nodes.nwg) hipLaunchKernelGGL(k_sa_node, dim3(nodes.nwg), dim3(256), 0, s, nodes, F, W, P);
+    if (inner.nwg) {
+        if (P.turb == 2) hipLaunchKernelGGL(k_sst_rhs, dim3(inner.nwg), dim3(128), 0, s, inner, F, W, P);
+        else hipLaunchKernelGGL(k_sa_rhs, dim3(inner.nwg), dim3(128), 0, s, inner, F, W, P);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_sa_sweep(int fwd, const SweepPlan &S, const SaWork &W, hipStream_t s)
+int launch_sa_sweep(int fwd, int var, const SweepPlan &S, const SaWork &W, hipStream_t s)
 {
+    if (var < 0 || var > 1 || !W.dqke[var] || !W.spec[var]) return -2;
     if (S.lanes > 64 || S.lanes < 1 || S.nwork < 1) return -2;
     if (launch_fill_xch(S, s)) return -3;     // every exchange word back to the empty pattern
     const int2 *work = fwd ? S.work_fwd : S.work_bwd;
-    if (fwd) hipLaunchKernelGGL(k_sa_sweep<1>, dim3(S.nwork), dim3(64), 0, s, S.blocks, work, W, S.err, S.spin_limit);
-    else hipLaunchKernelGGL(k_sa_sweep<0>, dim3(S.nwork), dim3(64), 0, s, S.blocks, work, W, S.err, S.spin_limit);
+    double *dqv = W.dqke[var];
+    const double *specv = W.spec[var];
+    if (fwd)
+        hipLaunchKernelGGL(k_sa_sweep<1>, dim3(S.nwork), dim3(64), 0, s, S.blocks, work, W, dqv, specv, S.err,
+                           S.spin_limit);
+    else
+        hipLaunchKernelGGL(k_sa_sweep<0>, dim3(S.nwork), dim3(64), 0, s, S.blocks, work, W, dqv, specv, S.err,
+                           S.spin_limit);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

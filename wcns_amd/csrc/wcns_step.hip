@@ -14,6 +14,11 @@ __device__ inline void setp(const DFields &F, long g, double r, double u, double
     F.r[g] = r; F.u[g] = u; F.v[g] = v; F.w[g] = w; F.p[g] = p;
 }
 
+// turbulence variable m (nu~ or k, then SST omega) and its BC_farfield_turbulence
+// value (turblence.f90:2321-2338)
+__device__ inline double *tvar(const DFields &F, int m) { return m == 0 ? F.qke : F.qkw; }
+__device__ inline double tinf(const DParams &P, int m) { return P.turb == 1 ? P.muoo : (m == 0 ? P.koo : P.omgoo); }
+
 __device__ void bc_point(const DRegion &g, const DBlock &b, const DBlock &bt, const DFields &F,
                          const DParams &P, int i, int j, int k)
 {
@@ -36,11 +41,15 @@ __device__ void bc_point(const DRegion &g, const DBlock &b, const DBlock &bt, co
         long q3 = gidx(bt, it - 3 * tl[0], jt - 3 * tl[1], kt - 3 * tl[2]);
         if (P.turb) {
             // Sub2.f90:266-280: first ghost from the partner, face node = mean of the two neighbours
-            const double vt = F.vist[q1], qk = F.qke[q1];
+            const double vt = F.vist[q1];
             F.vist[s1] = vt;
             F.vist[g0] = (vt + F.vist[t1]) * 0.5;
-            F.qke[s1] = qk;
-            F.qke[g0] = (qk + F.qke[t1]) * 0.5;
+            for (int m = 0; m < P.turb; m++) {
+                double *q = tvar(F, m);
+                const double qk = q[q1];
+                q[s1] = qk;
+                q[g0] = (qk + q[t1]) * 0.5;
+            }
         }
         setp(F, s1, F.r[q1], F.u[q1], F.v[q1], F.w[q1], F.p[q1]);
         setp(F, s2, F.r[q2], F.u[q2], F.v[q2], F.w[q2], F.p[q2]);
@@ -90,9 +99,12 @@ __device__ void bc_point(const DRegion &g, const DBlock &b, const DBlock &bt, co
             if (g.s_nd == 1) ut = tem_lr * (F.kcx[g0] * ub + F.kcy[g0] * vb + F.kcz[g0] * wb);
             else if (g.s_nd == 2) ut = tem_lr * (F.etx[g0] * ub + F.ety[g0] * vb + F.etz[g0] * wb);
             else ut = tem_lr * (F.ctx[g0] * ub + F.cty[g0] * vb + F.ctz[g0] * wb);
-            const double qv = ut < 0.0 ? P.muoo : F.qke[t1];
-            F.qke[g0] = qv;
-            F.qke[s1] = qv;
+            for (int m = 0; m < P.turb; m++) {
+                double *q = tvar(F, m);
+                const double qv = ut < 0.0 ? tinf(P, m) : q[t1];
+                q[g0] = qv;
+                q[s1] = qv;
+            }
         }
         setp(F, s1, rb, ub, vb, wb, pb);
         setp(F, s2, rb, ub, vb, wb, pb);
@@ -104,7 +116,7 @@ __device__ void bc_point(const DRegion &g, const DBlock &b, const DBlock &bt, co
         setp(F, s1, P.roo, P.uoo, P.voo, P.woo, P.poo);
         setp(F, s2, P.roo, P.uoo, P.voo, P.woo, P.poo);
         setp(F, s3, P.roo, P.uoo, P.voo, P.woo, P.poo);
-        if (P.turb) { F.qke[g0] = P.muoo; F.qke[s1] = P.muoo; }   // turblence.f90:2321-2338
+        for (int m = 0; m < P.turb; m++) { tvar(F, m)[g0] = tinf(P, m); tvar(F, m)[s1] = tinf(P, m); }   // turblence.f90:2321-2338
         break;
     case 6: {   // BC.f90:667-749 boundary_6
         double rm = F.r[t1], um = F.u[t1], vm = F.v[t1], wm = F.w[t1], pm = F.p[t1];
@@ -112,17 +124,31 @@ __device__ void bc_point(const DRegion &g, const DBlock &b, const DBlock &bt, co
         setp(F, s1, rm, um, vm, wm, pm);
         setp(F, s2, rm, um, vm, wm, pm);
         setp(F, s3, rm, um, vm, wm, pm);
-        if (P.turb) { const double qv = F.qke[t1]; F.qke[g0] = qv; F.qke[s1] = qv; }   // turblence.f90:2340-2355
+        for (int m = 0; m < P.turb; m++) {   // turblence.f90:2340-2355
+            double *q = tvar(F, m);
+            const double qv = q[t1];
+            q[g0] = qv;
+            q[s1] = qv;
+        }
         break;
     }
     case 2:
         if (P.nvis == 1) {   // BC.f90:1244-1409 viscid_wall
             double moocp = P.moo * P.moo * gama;
             setp(F, s1, F.r[t1], -F.u[t1], -F.v[t1], -F.w[t1], F.p[t1]);
-            // turblence.f90:2267-2306 BC_wall_turbulence (SA): odd reflection of the first
-            // ghost; BC_face_dif_turbulence: face value = mean of ghost and first interior node
-            double qs1 = 0.0;
+            // turblence.f90:2267-2306 BC_wall_turbulence: odd reflection of nu~ / k in the
+            // first ghost; SST omega there 1600 mu/(rho Re d^2) - omega(first interior), with
+            // mu from the face node's temperature and the face node's density before this
+            // BC sets it.  BC_face_dif_turbulence: face value = mean of ghost and first interior
+            double qs1 = 0.0, ws1 = 0.0;
             if (P.turb) { qs1 = -F.qke[t1]; F.qke[s1] = qs1; }
+            if (P.turb == 2) {
+                const double tm = F.t[g0];
+                const double vsl = tm * sqrt(tm) * (1.0 + P.visc) / (tm + P.visc);
+                const double d = F.dist[t1];
+                ws1 = 1600.0 * vsl / F.r[g0] / P.reynolds / (d * d) - F.qkw[t1];
+                F.qkw[s1] = ws1;
+            }
             double pres, tres;
             if (P.twall_fixed) {
                 pres = (4.0 * F.p[t1] - F.p[t2]) / 3.0;
@@ -137,6 +163,7 @@ __device__ void bc_point(const DRegion &g, const DBlock &b, const DBlock &bt, co
             double dres = moocp * pres / tres;
             setp(F, g0, dres, 0.0, 0.0, 0.0, pres);
             if (P.turb) F.qke[g0] = 0.5 * (F.qke[t1] + qs1);
+            if (P.turb == 2) F.qkw[g0] = 0.5 * (F.qkw[t1] + ws1);
             setp(F, s2, F.r[t2], -F.u[t2], -F.v[t2], -F.w[t2], F.p[t2]);
             setp(F, s3, -F.r[t3], -F.u[t3], -F.v[t3], -F.w[t3], F.p[t3]);
         } else {             // BC.f90:1412-1486 inviscid_wall + dif_average
@@ -163,11 +190,12 @@ __device__ void bc_point(const DRegion &g, const DBlock &b, const DBlock &bt, co
         }
         setp(F, g0, 0.5 * (F.r[s1] + F.r[t1]), 0.5 * (F.u[s1] + F.u[t1]), 0.5 * (F.v[s1] + F.v[t1]),
              0.5 * (F.w[s1] + F.w[t1]), 0.5 * (F.p[s1] + F.p[t1]));
-        if (P.turb) {
+        for (int m = 0; m < P.turb; m++) {
             // turblence.f90:2308-2319 BC_symmetry_turbulence (first ghost) + dif_average
-            const double qt = F.qke[t1];
-            F.qke[s1] = qt;
-            F.qke[g0] = 0.5 * (qt + qt);
+            double *q = tvar(F, m);
+            const double qt = q[t1];
+            q[s1] = qt;
+            q[g0] = 0.5 * (qt + qt);
         }
         break;
     }

@@ -24,7 +24,7 @@ E_NAMES = {-1: "WCNS_E_ARG", -2: "WCNS_E_STATE", -3: "WCNS_E_HIP", -4: "WCNS_E_N
 FIELDS = {"r": 0, "u": 1, "v": 2, "w": 3, "p": 4, "t": 5, "dtdt": 7, "sra": 8, "srb": 9, "src": 10,
           "srva": 11, "srvb": 12, "srvc": 13, "visl": 14, "vol": 16, "kcx": 17, "kcy": 18, "kcz": 19,
           "etx": 20, "ety": 21, "etz": 22, "ctx": 23, "cty": 24, "ctz": 25, "dq": 100, "rhs": 102,
-          "vist": 15, "qke": 26, "dist": 27}
+          "vist": 15, "qke": 26, "dist": 27, "qkw": 28}
 
 EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "wcns_add_region", "wcns_setup",
            "wcns_set_lhs_precision", "wcns_set_prim", "wcns_get_field", "wcns_set_field", "wcns_boundary_all",
