@@ -164,3 +164,64 @@ integer(c_int) function ref_sst_step(dims, scal, fld, k_out, w_out, vist_out) bi
   vist_out = s_vist
   ref_sst_step = 0
 end function ref_sst_step
+
+! Wall values of the turbulence variables: for each of the npts face points
+! (is,js,ks, it,jt,kt, i,j,k in idx, 9 per point; ghost, first interior, face)
+! calls the reference's BC_wall_turbulence and BC_face_dif_turbulence
+! (turblence.f90:2252-2306) on the given state, as viscid_wall (BC.f90:1244-1409)
+! does around its face update.  scal = (reynolds, visc); nl = 1 (SA) or 2 (SST);
+! fld = r, t, qke, qkw (ignored for nl = 1), dist (C frame).
+integer(c_int) function ref_wall_turb(dims, nl, scal, fld, npts, idx, qke_out, qkw_out) bind(C, name="ref_wall_turb")
+  use iso_c_binding
+  use global_variables, only: method, nlamtur, nameturb, reynolds, visc, r, t, mb_qke, mb_distance
+  use ref_sa_store
+  implicit none
+  integer(c_int), intent(in) :: dims(3)
+  integer(c_int), value :: nl, npts
+  real(c_double), intent(in) :: scal(2)
+  type(c_ptr), intent(in) :: fld(5)
+  integer(c_int), intent(in) :: idx(9, npts)
+  real(c_double), intent(out) :: qke_out(-2:dims(1)+3, -2:dims(2)+3, -2:dims(3)+3)
+  real(c_double), intent(out) :: qkw_out(-2:dims(1)+3, -2:dims(2)+3, -2:dims(3)+3)
+  real(c_double), pointer :: f(:,:,:)
+  real, target, allocatable, save :: s_t(:,:,:)
+  integer :: ni, nj, nk, m, q
+  external :: BC_wall_turbulence, BC_face_dif_turbulence
+
+  ni = dims(1); nj = dims(2); nk = dims(3)
+  call release()
+  if (allocated(s_t)) deallocate(s_t)
+  allocate(s_r(-2:ni+3, -2:nj+3, -2:nk+3), s_t(-2:ni+3, -2:nj+3, -2:nk+3))
+  allocate(s_qke(-2:ni+3, -2:nj+3, -2:nk+3, nl), s_dist(1:ni, 1:nj, 1:nk))
+  call c_f_pointer(fld(1), f, [ni + 6, nj + 6, nk + 6]); call load(s_r, f, ni, nj, nk, -2, 3)
+  call c_f_pointer(fld(2), f, [ni + 6, nj + 6, nk + 6]); call load(s_t, f, ni, nj, nk, -2, 3)
+  call c_f_pointer(fld(3), f, [ni + 6, nj + 6, nk + 6]); call load(s_qke(:,:,:,1), f, ni, nj, nk, -2, 3)
+  if (nl == 2) then
+    call c_f_pointer(fld(4), f, [ni + 6, nj + 6, nk + 6]); call load(s_qke(:,:,:,2), f, ni, nj, nk, -2, 3)
+  end if
+  call c_f_pointer(fld(5), f, [ni + 6, nj + 6, nk + 6]); call load(s_dist, f, ni, nj, nk, 1, 0)
+  if (associated(mb_qke)) deallocate(mb_qke)
+  if (associated(mb_distance)) deallocate(mb_distance)
+  allocate(mb_qke(1), mb_distance(1))
+  mb_qke(1)%a4d => s_qke
+  mb_distance(1)%a3d => s_dist
+  r => s_r; t => s_t
+  method = 1; nlamtur = nl
+  if (nl == 2) then
+    nameturb = 'SST'
+  else
+    nameturb = 'SA'
+  end if
+  reynolds = scal(1); visc = scal(2)
+  do q = 1, npts
+    call BC_wall_turbulence(1, idx(1,q), idx(2,q), idx(3,q), idx(4,q), idx(5,q), idx(6,q), &
+                            idx(7,q), idx(8,q), idx(9,q))
+    call BC_face_dif_turbulence(1, 1, idx(1,q), idx(2,q), idx(3,q), idx(4,q), idx(5,q), idx(6,q), &
+                                idx(7,q), idx(8,q), idx(9,q))
+  end do
+  qke_out = s_qke(:,:,:,1)
+  qkw_out = 0.0
+  if (nl == 2) qkw_out = s_qke(:,:,:,2)
+  m = 0
+  ref_wall_turb = m
+end function ref_wall_turb

@@ -178,3 +178,45 @@ def test_sst_block_step_against_reference_fresh_inputs(seed, kick, dims, steps):
     np.testing.assert_array_equal(k, kr)
     np.testing.assert_array_equal(w, wr)
     np.testing.assert_array_equal(v, vr)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SA), reason="serial reference library not built here")
+@pytest.mark.parametrize("nlamtur", [1, 2])
+def test_wall_turbulence_values_against_reference(nlamtur):
+    """The oracle's wall values of nu~ (SA) or k and omega (SST) after
+    boundary_all equal the reference's BC_wall_turbulence + BC_face_dif_turbulence
+    applied to the same pre-BC state, at the face points no other region touches."""
+    import ctypes as C
+    mgs = _serial_module()
+    lib = mgs.load()
+    case = mgs.sst_case(14, 10, 8) if nlamtur == 2 else mgs.sa_case(14, 10, 8)
+    blk = case.blocks[0]
+    oc = O.OracleCase(case)
+    oc.step(2)
+    rng = np.random.default_rng(9)
+    names = ["qke"] + (["qkw"] if nlamtur == 2 else [])
+    for nm in names:
+        f = oc.field(0, nm)
+        oc.set_field(0, nm, f * (1.0 + 0.2 * rng.random(f.shape)))
+    pre = {nm: np.ascontiguousarray(oc.field(0, nm)) for nm in ["r", "t", "qke", "qkw", "dist"]}
+    oc.stage("boundary_all")
+    # wall j = 1: ghost j = 0, first interior j = 2; points i = 2..ni-1, k = 2..nk-1
+    idx = []
+    for k in range(2, blk.nk):
+        for i in range(2, blk.ni):
+            idx += [i, 0, k, i, 2, k, i, 1, k]
+    idx = np.array(idx, dtype=np.int32)
+    qo, wo = np.zeros_like(pre["r"]), np.zeros_like(pre["r"])
+    ptrs = (C.c_void_p * 5)(*[pre[n].ctypes.data for n in ["r", "t", "qke", "qkw", "dist"]])
+    scal = (C.c_double * 2)(case.params.reynolds, 110.4 / case.params.tref)
+    lib.ref_wall_turb.restype = C.c_int
+    rc = lib.ref_wall_turb((C.c_int * 3)(blk.ni, blk.nj, blk.nk), C.c_int(nlamtur), scal, ptrs,
+                           C.c_int(len(idx) // 9), idx.ctypes.data_as(C.c_void_p),
+                           qo.ctypes.data_as(C.c_void_p), wo.ctypes.data_as(C.c_void_p))
+    assert rc == 0
+    # G3 index = node + 2: nodes k = 2..nk-1, j = 0..1 (ghost, face), i = 2..ni-1
+    sl = (slice(2 + 2, 2 + blk.nk), slice(2 + 0, 2 + 2), slice(2 + 2, 2 + blk.ni))
+    np.testing.assert_array_equal(oc.field(0, "qke")[sl], qo[sl])
+    if nlamtur == 2:
+        np.testing.assert_array_equal(oc.field(0, "qkw")[sl], wo[sl])
+        assert np.any(wo[sl][:, 0, :] != 0.0)

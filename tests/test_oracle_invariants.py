@@ -355,3 +355,105 @@ def test_block_interface_images_match_coordinates():
                 if nr >= 4:                         # periodic k: z apart by lz
                     d[2] = 0.0 if abs(abs(d[2]) - 0.6) < 1e-9 else d[2]
                 assert np.max(np.abs(d)) < 1e-12, (nb, nr, (i, j, k), (it, jt, kt), d)
+
+
+# ---------------------------------------------------------------- turbulence models
+def _turb_case(nlamtur, **kw):
+    kw.setdefault("dy_wall", 0.01)
+    kw.setdefault("growth", 1.2)
+    case = cases.bump_channel_case(ni=kw.pop("ni", 14), nj=kw.pop("nj", 10), nk=kw.pop("nk", 8),
+                                   reynolds=kw.pop("reynolds", 1e5), **kw)
+    case.params.nlamtur = nlamtur
+    return case
+
+
+def test_wall_distance_is_brute_force_minimum():
+    """calc_wall_dists_parallel (parallel.f90:4313-4468): sqrt of the least
+    squared distance to any node of a no-slip wall region, plus small -- on a
+    two-block case with the wall split between the blocks."""
+    case = cases.multiblock_case(nvis=1, wall=True, periodic_i=False, ilo_bc=5, ihi_bc=6, jhi_bc=3)
+    case.params.nlamtur = 1
+    oc = O.OracleCase(case)
+    wall = []
+    for blk in case.blocks:
+        for reg in blk.regions:
+            if reg.bctype == 2:
+                s = reg.s_raw
+                sl = (slice(abs(s[0]) - 1, abs(s[1])), slice(abs(s[2]) - 1, abs(s[3])), slice(abs(s[4]) - 1, abs(s[5])))
+                wall.append(np.stack([blk.x[sl].ravel(), blk.y[sl].ravel(), blk.z[sl].ravel()], 1))
+    wall = np.concatenate(wall)
+    for nb, blk in enumerate(case.blocks):
+        pts = np.stack([blk.x.ravel(), blk.y.ravel(), blk.z.ravel()], 1)
+        d2 = ((pts[:, None, :] - wall[None, :, :]) ** 2).sum(-1).min(1)
+        want = (np.sqrt(d2) + 1e-38).reshape(blk.x.shape)
+        got = oc.field(nb, "dist")[3:-3, 3:-3, 3:-3].transpose(2, 1, 0)
+        np.testing.assert_allclose(got, want, rtol=2e-16, atol=0)
+    # a plant: dropping the z term changes the distance of off-plane nodes
+    assert not np.allclose(np.sqrt(((pts[:, None, :2] - wall[None, :, :2]) ** 2).sum(-1).min(1)).reshape(blk.x.shape),
+                           want, rtol=1e-6)
+
+
+def test_sa_freestream_values_and_initial_field():
+    """inflow_turbulence/init_SA: nu~ = 0.1 and mu_t = nu~ fv1(nu~ r/mu) with the
+    reference's constants (cv1 = 7.1) on nodes -1..n+1."""
+    case = _turb_case(1)
+    oc = O.OracleCase(case)
+    q = oc.field(0, "qke")[1:-2, 1:-2, 1:-2]          # nodes -1..n+1
+    v = oc.field(0, "vist")[1:-2, 1:-2, 1:-2]
+    assert np.all(q == 0.1)
+    chi = 0.1
+    np.testing.assert_allclose(v, chi * chi ** 3 / (chi ** 3 + 7.1 ** 3), rtol=1e-15)
+
+
+def test_sst_freestream_values():
+    """inflow_turbulence for SST: omega = 10 lref/lfref, k = omega mu_t/Re with
+    mu_t = 1e-6, Re rescaled by lref/lfref (PreProcess.f90:994)."""
+    case = _turb_case(2)
+    case.params.lfref = 2.0
+    case.params.lref = 0.5
+    oc = O.OracleCase(case)
+    re = 1e5 / 2.0 * 0.5
+    np.testing.assert_allclose(oc.field(0, "qkw")[5, 5, 5], 10.0 * 0.25, rtol=1e-15)
+    np.testing.assert_allclose(oc.field(0, "qke")[5, 5, 5], 2.5 * 1e-6 / re, rtol=1e-15)
+    np.testing.assert_allclose(oc.field(0, "vist")[5, 5, 5], 1e-6, rtol=1e-15)
+
+
+def test_sst_wall_ghosts_closed_form():
+    """BC_wall_turbulence for SST: k odd in the first ghost, omega there =
+    1600 mu(T_face)/(rho_face Re d_1^2) - omega_1, face values the mean of ghost
+    and first interior node (the face node's rho before the wall BC sets it)."""
+    case = _turb_case(2)
+    oc = O.OracleCase(case)
+    oc.step(2)
+    blk = case.blocks[0]
+    # perturb k and omega so that the ghost values are not trivially uniform
+    rng = np.random.default_rng(3)
+    for nm in ("qke", "qkw"):
+        f = oc.field(0, nm)
+        oc.set_field(0, nm, f * (1.0 + 0.1 * rng.random(f.shape)))
+    r0, t0 = oc.field(0, "r").copy(), oc.field(0, "t").copy()
+    k0, w0, d0 = oc.field(0, "qke").copy(), oc.field(0, "qkw").copy(), oc.field(0, "dist")
+    oc.stage("boundary_all")
+    k1, w1 = oc.field(0, "qke"), oc.field(0, "qkw")
+    # the wall is j = 1 (G3 index 3), interior node j = 2 (index 4), ghost j = 0 (index 2)
+    sl = (slice(3 + 1, 3 + blk.nk - 1), slice(None), slice(3 + 1, 3 + blk.ni - 1))   # away from the far-field edges
+    face, inner, ghost = (lambda a, jj: a[sl[0], jj, sl[2]]), 4, 2
+    visc = 110.4 / case.params.tref
+    tm = face(t0, 3)
+    mu = tm * np.sqrt(tm) * (1.0 + visc) / (tm + visc)
+    want_w = 1600.0 * mu / face(r0, 3) / 1e5 / (face(d0, inner) * face(d0, inner)) - face(w0, inner)
+    np.testing.assert_array_equal(face(w1, ghost), want_w)
+    np.testing.assert_array_equal(face(k1, ghost), -face(k0, inner))
+    np.testing.assert_array_equal(face(w1, 3), 0.5 * (face(w0, inner) + want_w))
+    np.testing.assert_array_equal(face(k1, 3), 0.5 * (face(k0, inner) - face(k0, inner)))
+
+
+@pytest.mark.parametrize("nlamtur", [1, 2])
+def test_turbulence_models_run_and_stay_bounded(nlamtur):
+    case = _turb_case(nlamtur, ni=24, nj=16, nk=10, dy_wall=1e-3)
+    oc = O.OracleCase(case)
+    h = oc.step(10)
+    assert np.all(np.isfinite(h)) and h[-1] < h[2]
+    for nm in ("qke", "vist") + (("qkw",) if nlamtur == 2 else ()):
+        f = oc.field(0, nm)[3:-3, 3:-3, 3:-3]
+        assert np.all(np.isfinite(f)) and f.min() >= 0.0
