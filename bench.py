@@ -59,9 +59,10 @@ def alg_bytes_per_node(slot, lhs_bytes):
     return table.get(slot)
 
 
-# Kernel of each single-kernel timing slot, as rocprofv3 names it.
-SLOT_KERNEL = {"rhs_inv_i": "k_rhs_inv<0,", "rhs_inv_j": "k_rhs_inv<1,", "rhs_inv_k": "k_rhs_inv<2,",
-               "rhs_vis_i": "k_rhs_vis<0>", "rhs_vis_j": "k_rhs_vis<1>", "rhs_vis_k": "k_rhs_vis<2>",
+# Kernel of each single-kernel timing slot, as rocprofv3 names it (the i
+# direction runs as instance 3: j pencils of the transposed copies).
+SLOT_KERNEL = {"rhs_inv_i": "k_rhs_inv<3,", "rhs_inv_j": "k_rhs_inv<1,", "rhs_inv_k": "k_rhs_inv<2,",
+               "rhs_vis_i": "k_rhs_vis<3>", "rhs_vis_j": "k_rhs_vis<1>", "rhs_vis_k": "k_rhs_vis<2>",
                "vis_deriv": "k_vis_node_deriv", "sweep_lu_l": "k_sweep_pipe<0,", "sweep_lu_u": "k_sweep_pipe<1,",
                "sweep_gs_l": "k_sweep_pipe<2,", "sweep_gs_u": "k_sweep_pipe<3,", "sweep_stage": "k_skew_stage<",
                "update": "k_update"}

@@ -171,7 +171,8 @@ int wcns_sync(wcns_ctx *ctx);
 #define WCNS_T_SA_SWEEP_L  20   /* SA: scalar LU-SGS forward / backward sweep */
 #define WCNS_T_SA_SWEEP_U  21
 #define WCNS_T_SA_UPDATE   22   /* SA: update, mu_t, max_control */
-#define WCNS_T_NSLOT       23
+#define WCNS_T_TRANSPOSE   23   /* i<->j transposed copies for the i-direction residuals */
+#define WCNS_T_NSLOT       24
 int wcns_timing(wcns_ctx *ctx, int on);
 int wcns_timing_get(wcns_ctx *ctx, int slot, double *total_ms, long long *launches);
 

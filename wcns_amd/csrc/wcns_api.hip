@@ -84,6 +84,14 @@ struct wcns_ctx {
     DBlock *d_blocks = nullptr;
     BlockGrid grid[GRID_N] = {};            // block work-lists of the batched launches
     int2 *d_grid = nullptr;
+    // i-direction residuals on i<->j transposed copies (WCNS_IDIR=direct: in place)
+    bool idir_t = true;
+    DBlock *d_blocks_t = nullptr;           // transposed views
+    double *tpool = nullptr;                // the copies below, total elements each
+    double *mt[10] = {nullptr};             // kcx kcy kcz etx ety etz ctx cty ctz vol (set-up)
+    double *pt[5] = {nullptr}, *t_t = nullptr, *visl_t = nullptr, *vist_t = nullptr;
+    double *dn_t = nullptr;                 // 12 slots; 0..3 <- j derivatives, 8..11 <- k derivatives
+    double *df_t = nullptr;                 // 5: the i-direction contribution, transposed
     int *d_flag = nullptr;
     int flag_cap = 512;
     double *d_partial = nullptr;
@@ -141,6 +149,50 @@ struct wcns_ctx {
 };
 
 namespace {
+
+// The fields as the j-direction kernels see them on the transposed view: the
+// transposed copies, with the i and j metric triples exchanged so that the
+// "j" direction of the view is the original i direction; the residual output
+// goes to the transposed contribution df_t (zeroed before the launch).
+DFields fields_t(const wcns_ctx *ctx, const DFields &F)
+{
+    DFields T = F;
+    T.r = ctx->pt[0]; T.u = ctx->pt[1]; T.v = ctx->pt[2]; T.w = ctx->pt[3]; T.p = ctx->pt[4];
+    T.t = ctx->t_t;
+    T.kcx = ctx->mt[3]; T.kcy = ctx->mt[4]; T.kcz = ctx->mt[5];
+    T.etx = ctx->mt[0]; T.ety = ctx->mt[1]; T.etz = ctx->mt[2];
+    T.ctx = ctx->mt[6]; T.cty = ctx->mt[7]; T.ctz = ctx->mt[8]; T.vol = ctx->mt[9];
+    T.visl = ctx->visl_t;
+    T.vist = ctx->vist_t;
+    T.rhs = ctx->df_t;
+    return T;
+}
+
+// Copies the i-direction inputs to the transposed view, zeroes df_t.
+int stage_transposed(wcns_ctx *ctx, const DFields &F, bool vis)
+{
+    TransList L{};
+    const double *src[5] = {F.r, F.u, F.v, F.w, F.p};
+    for (int m = 0; m < 5; m++) { L.src[L.n] = src[m]; L.dst[L.n++] = ctx->pt[m]; }
+    if (vis) {
+        L.src[L.n] = F.t; L.dst[L.n++] = ctx->t_t;
+        L.src[L.n] = F.visl; L.dst[L.n++] = ctx->visl_t;
+        if (F.vist) { L.src[L.n] = F.vist; L.dst[L.n++] = ctx->vist_t; }
+        // derivative slots: the view's direction 0 is the original j, direction 2 the original k
+        for (int m = 0; m < 4; m++) { L.src[L.n] = ctx->dn + (4 + m) * F.total; L.dst[L.n++] = ctx->dn_t + m * F.total; }
+        for (int m = 0; m < 4; m++) { L.src[L.n] = ctx->dn + (8 + m) * F.total; L.dst[L.n++] = ctx->dn_t + (8 + m) * F.total; }
+    }
+    if (launch_transpose(0, ctx->grid[GRID_TRANS], L, ctx->stream)) return -3;
+    return launch_fill_u64(reinterpret_cast<unsigned long long *>(ctx->df_t), 5 * F.total, 0ull, ctx->stream);
+}
+
+// rhs += the transposed i-direction contribution
+int add_transposed(wcns_ctx *ctx, const DFields &F)
+{
+    TransList L{};
+    for (int m = 0; m < 5; m++) { L.src[L.n] = ctx->df_t + m * F.total; L.dst[L.n++] = F.rhs + m * F.total; }
+    return launch_transpose(1, ctx->grid[GRID_TRANS], L, ctx->stream);
+}
 
 // Records HIP events around the launches of one timing slot.
 struct Scope {
@@ -386,6 +438,7 @@ int wcns_create(wcns_ctx **out, int device, const wcns_params *prm)
     ctx->device = device;
     ctx->prm = *prm;
     if (const char *sw = getenv("WCNS_SWEEP")) ctx->tiled_sweeps = strcmp(sw, "plane") != 0;
+    if (const char *id = getenv("WCNS_IDIR")) ctx->idir_t = strcmp(id, "direct") != 0;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return WCNS_E_HIP;
@@ -405,7 +458,7 @@ int wcns_destroy(wcns_ctx *ctx)
     for (int m = 0; m < 3; m++) { hipFree(ctx->sr[m]); hipFree(ctx->srv[m]); }
     void *pl[] = {ctx->d_sblocks, ctx->d_work, ctx->d_xch};
     for (void *p : pl) if (p) hipFree(p);
-    void *ps[] = {ctx->d_grid, ctx->qkw, ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
+    void *ps[] = {ctx->tpool, ctx->d_blocks_t, ctx->d_grid, ctx->qkw, ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
                   ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
@@ -630,6 +683,11 @@ int wcns_setup(wcns_ctx *ctx)
     ctx->max_parts = maxp;
     TRY(dalloc(ctx, &ctx->d_blocks, hb.size()));
     TRY(copy_sync(ctx, ctx->d_blocks, hb.data(), sizeof(DBlock) * hb.size(), hipMemcpyHostToDevice));
+    // transposed views of the blocks (i <-> j), for the i-direction residuals
+    std::vector<DBlock> hbt;
+    for (auto &b : hb) hbt.push_back(transposed_view(b));
+    TRY(dalloc(ctx, &ctx->d_blocks_t, hbt.size()));
+    TRY(copy_sync(ctx, ctx->d_blocks_t, hbt.data(), sizeof(DBlock) * hbt.size(), hipMemcpyHostToDevice));
     // block work-lists: every block's workgroups of a launch kind, block after block
     {
         std::vector<int2> all;
@@ -637,8 +695,9 @@ int wcns_setup(wcns_ctx *ctx)
         for (int kind = 0; kind < GRID_N; kind++) {
             start[kind] = all.size();
             const long bs = grid_block_size(kind);
+            const bool tview = kind == GRID_RHS_INV_T || kind == GRID_RHS_VIS_T;
             for (size_t nb = 0; nb < hb.size(); nb++) {
-                const long nw = (grid_threads(kind, hb[nb]) + bs - 1) / bs;
+                const long nw = (grid_threads(kind, tview ? hbt[nb] : hb[nb]) + bs - 1) / bs;
                 for (long w = 0; w < nw; w++) all.push_back(make_int2((int)nb, (int)w));
             }
         }
@@ -647,8 +706,23 @@ int wcns_setup(wcns_ctx *ctx)
             TRY(copy_sync(ctx, ctx->d_grid, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice));
         for (int kind = 0; kind < GRID_N; kind++) {
             const size_t end = kind + 1 < GRID_N ? start[kind + 1] : all.size();
-            ctx->grid[kind] = BlockGrid{ctx->d_blocks, ctx->d_grid + start[kind], (int)(end - start[kind])};
+            const bool tview = kind == GRID_RHS_INV_T || kind == GRID_RHS_VIS_T;
+            ctx->grid[kind] = BlockGrid{tview ? ctx->d_blocks_t : ctx->d_blocks, ctx->d_grid + start[kind],
+                                        (int)(end - start[kind])};
         }
+    }
+    if (ctx->idir_t) {
+        // metrics 10, primitives 5, t, visl, vist, 12 derivative slots, 5 residual components
+        const int ntp = 10 + 5 + 3 + 12 + 5;
+        TRY(dalloc(ctx, &ctx->tpool, (size_t)ntp * T));
+        double *q = ctx->tpool;
+        for (int m = 0; m < 10; m++, q += T) ctx->mt[m] = q;
+        for (int m = 0; m < 5; m++, q += T) ctx->pt[m] = q;
+        ctx->t_t = q; q += T;
+        ctx->visl_t = q; q += T;
+        ctx->vist_t = q; q += T;
+        ctx->dn_t = q; q += 12 * T;
+        ctx->df_t = q;
     }
     TRY(dalloc(ctx, &ctx->d_partial, 3 * maxp));
     TRY(dalloc(ctx, &ctx->d_resblk, 3 * ctx->blocks.size()));
@@ -733,6 +807,11 @@ int wcns_setup(wcns_ctx *ctx)
                         : launch_sa_init(b.db, F, d.muoo, 0.0, d.muoo * fv1, ctx->stream);
             if (e) return fail(ctx, WCNS_E_HIP, "turbulence init");
         }
+    }
+    if (ctx->idir_t) {
+        TransList L{};
+        for (int m = 0; m < 10; m++) { L.src[L.n] = ctx->metric[m]; L.dst[L.n++] = ctx->mt[m]; }
+        if (launch_transpose(0, ctx->grid[GRID_TRANS], L, ctx->stream)) return fail(ctx, WCNS_E_HIP, "transposing the metrics");
     }
     // symmetry-plane normals (BC.f90:381-396) need the metrics at the face centre
     for (auto &b : ctx->blocks)
@@ -980,6 +1059,20 @@ int wcns_r_h_s_vis(wcns_ctx *ctx)
             return fail(ctx, WCNS_E_HIP, "vis deriv launch");
     }
     for (int dir = 0; dir < 3; dir++) {
+        if (dir == 0 && ctx->idir_t) {
+            {
+                Scope st(ctx, WCNS_T_TRANSPOSE);
+                if (stage_transposed(ctx, F, true)) return fail(ctx, WCNS_E_HIP, "transposing the viscous inputs");
+            }
+            {
+                Scope sc(ctx, WCNS_T_VIS_I);
+                if (launch_rhs_vis_dir(3, ctx->grid[GRID_RHS_VIS_T], fields_t(ctx, F), ctx->dp, ctx->dn_t, ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "vis launch");
+            }
+            Scope st(ctx, WCNS_T_TRANSPOSE);
+            if (add_transposed(ctx, F)) return fail(ctx, WCNS_E_HIP, "adding the i-direction viscous terms");
+            continue;
+        }
         Scope sc(ctx, WCNS_T_VIS_I + dir);
         if (launch_rhs_vis_dir(dir, ctx->grid[GRID_RHS_VIS + dir], F, ctx->dp, ctx->dn, ctx->stream))
             return fail(ctx, WCNS_E_HIP, "vis launch");
@@ -992,6 +1085,20 @@ int wcns_r_h_s_invis(wcns_ctx *ctx)
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     DFields F = ctx->fields();
     for (int dir = 0; dir < 3; dir++) {
+        if (dir == 0 && ctx->idir_t) {
+            {
+                Scope st(ctx, WCNS_T_TRANSPOSE);
+                if (stage_transposed(ctx, F, false)) return fail(ctx, WCNS_E_HIP, "transposing the inviscid inputs");
+            }
+            {
+                Scope sc(ctx, WCNS_T_INV_I);
+                if (launch_rhs_inv_dir(3, ctx->grid[GRID_RHS_INV_T], fields_t(ctx, F), ctx->dp, ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "rhs launch");
+            }
+            Scope st(ctx, WCNS_T_TRANSPOSE);
+            if (add_transposed(ctx, F)) return fail(ctx, WCNS_E_HIP, "adding the i-direction inviscid terms");
+            continue;
+        }
         Scope sc(ctx, WCNS_T_INV_I + dir);
         if (launch_rhs_inv_dir(dir, ctx->grid[GRID_RHS_INV + dir], F, ctx->dp, ctx->stream))
             return fail(ctx, WCNS_E_HIP, "rhs launch");

@@ -56,7 +56,10 @@ enum {
     GRID_RHS_INV = 7,             // + dir: inviscid pencil segments
     GRID_SA_RHS = 10,             // SA nodes 2..n-1
     GRID_SA_UPD = 11,             // SA nodes 0..n
-    GRID_N = 12
+    GRID_TRANS = 12,              // 32x32 tiles of every k plane (i<->j transposes)
+    GRID_RHS_INV_T = 13,          // inviscid i pencils as j pencils of the transposed view
+    GRID_RHS_VIS_T = 14,          // viscous i pencils, likewise
+    GRID_N = 15
 };
 long grid_threads(int kind, const DBlock &b);
 int grid_block_size(int kind);
@@ -142,6 +145,20 @@ int launch_sa_rhs(const BlockGrid &nodes, const BlockGrid &inner, const DFields 
                   const DParams &P, hipStream_t s);
 int launch_sa_sweep(int fwd, int var, const SweepPlan &S, const SaWork &W, hipStream_t s);
 int launch_sa_update(const BlockGrid &G, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s);
+
+// i<->j transposed copies (wcns_transpose.hip): block b's copy has b's offset
+// and plane size, node (i,j,k) at off + (j+2) + (nj+6)(i+2) + sy(k+2); the
+// transposed view is the DBlock that addresses it with (j, i, k).
+constexpr int TRANS_MAX = 16;
+struct TransList {
+    const double *src[TRANS_MAX];
+    double *dst[TRANS_MAX];
+    int n;
+};
+long transpose_tiles(const DBlock &b);
+DBlock transposed_view(const DBlock &b);
+// back = 0: dst (transposed) = src; back = 1: dst (normal) += src (transposed)
+int launch_transpose(int back, const BlockGrid &G, const TransList &L, hipStream_t s);
 
 // wall force
 long force_quads(const DRegion &g);

@@ -240,9 +240,11 @@ __device__ __noinline__ void closure_state(const Pencil &pc, int which, double *
     }
 }
 
-template <int DIR, int FLUX>
+// DIR0 = 3: the i direction run as j pencils of the transposed view
+template <int DIR0, int FLUX>
 __global__ void __launch_bounds__(128) k_rhs_inv(BlockGrid G, DFields F, DParams P)
 {
+    constexpr int DIR = DIR0 == 3 ? 1 : DIR0;
     DBlock b;
     long t;
     grid_thread(G, b, t);
@@ -437,10 +439,12 @@ int launch_rhs_inv_dir(int dir, const BlockGrid &G, const DFields &F, const DPar
         if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 3>), grid, blk, 0, s, G, F, P);
         if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 3>), grid, blk, 0, s, G, F, P);
         if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 3>), grid, blk, 0, s, G, F, P);
+        if (dir == 3) hipLaunchKernelGGL((k_rhs_inv<3, 3>), grid, blk, 0, s, G, F, P);
     } else {
         if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 1>), grid, blk, 0, s, G, F, P);
         if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 1>), grid, blk, 0, s, G, F, P);
         if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 1>), grid, blk, 0, s, G, F, P);
+        if (dir == 3) hipLaunchKernelGGL((k_rhs_inv<3, 1>), grid, blk, 0, s, G, F, P);
     }
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }

@@ -195,9 +195,12 @@ __device__ void face_visc(const DFields &F, const DParams &P, const double *dn, 
     fv[4] = uvwt[0] * fv[1] + uvwt[1] * fv[2] + uvwt[2] * fv[3] + kcp * (d[9] * nx + d[10] * ny + d[11] * nz);
 }
 
-template <int DIR>
+// DIR0 = 3: the i direction run as j pencils of the transposed view (the same
+// code as DIR = 1 on transposed fields; its own instance so profiles tell it apart)
+template <int DIR0>
 __global__ void __launch_bounds__(128) k_rhs_vis(BlockGrid G, DFields F, DParams P, const double *dn)
 {
+    constexpr int DIR = DIR0 == 3 ? 1 : DIR0;
     DBlock b;
     long t;
     grid_thread(G, b, t);
@@ -330,6 +333,7 @@ int launch_rhs_vis_dir(int dir, const BlockGrid &G, const DFields &F, const DPar
     if (dir == 0) hipLaunchKernelGGL(k_rhs_vis<0>, grid, blk, 0, s, G, F, P, dn);
     if (dir == 1) hipLaunchKernelGGL(k_rhs_vis<1>, grid, blk, 0, s, G, F, P, dn);
     if (dir == 2) hipLaunchKernelGGL(k_rhs_vis<2>, grid, blk, 0, s, G, F, P, dn);
+    if (dir == 3) hipLaunchKernelGGL(k_rhs_vis<3>, grid, blk, 0, s, G, F, P, dn);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -624,12 +624,16 @@ long grid_threads(int kind, const DBlock &b)
     if (kind >= GRID_RHS_INV && kind < GRID_RHS_INV + 3) return rhs_inv_threads(kind - GRID_RHS_INV, b);
     if (kind == GRID_SA_RHS) return b.ni > 2 && b.nj > 2 && b.nk > 2 ? (long)(b.ni - 2) * (b.nj - 2) * (b.nk - 2) : 0;
     if (kind == GRID_SA_UPD) return (long)(b.ni + 1) * (b.nj + 1) * (b.nk + 1);
+    if (kind == GRID_TRANS) return transpose_tiles(b) * 256;
+    // b is the transposed view here: its j pencils are the original i pencils
+    if (kind == GRID_RHS_INV_T) return rhs_inv_threads(1, b);
+    if (kind == GRID_RHS_VIS_T) return rhs_vis_threads(1, b);
     return 0;
 }
 
 int grid_block_size(int kind)
 {
-    return (kind == GRID_NODES || kind == GRID_SA_UPD) ? 256 : 128;
+    return (kind == GRID_NODES || kind == GRID_SA_UPD || kind == GRID_TRANS) ? 256 : 128;
 }
 
 int launch_zero_rhs(const DBlock &b, const DFields &F, hipStream_t s)
