@@ -43,9 +43,11 @@ def alg_bytes_per_node(slot, lhs_bytes):
     table = {
         # R += dF/dxi along one direction: read P 5, metric triple 3, R 5; write R 5
         "rhs_inv_i": 8 * 18, "rhs_inv_j": 8 * 18, "rhs_inv_k": 8 * 18,
-        # R -= Re^-1 dFv/dxi along one direction: read u v w T 4, the 8 cross
-        # derivatives, 9 metrics, vol, mu, R 5; write R 5
-        "rhs_vis_i": 8 * 33, "rhs_vis_j": 8 * 33, "rhs_vis_k": 8 * 33,
+        # viscous half-node fluxes along one direction: read u v w T 4, the 8
+        # cross derivatives, 9 metrics, vol, mu; write 5 fluxes
+        "rhs_vis_i": 8 * 28, "rhs_vis_j": 8 * 28, "rhs_vis_k": 8 * 28,
+        # FLUX_DXYZ rows: read 5 fluxes and R 5, write R 5 (one launch per direction)
+        "vis_rows": 8 * 15,
         # node derivatives: read u v w T, write 12
         "vis_deriv": 8 * 16,
         # sweeps (P 5, Geo 10, R 5, D 1, dQ in/out): 26 / 26 / 31 / 31 words
@@ -62,7 +64,8 @@ def alg_bytes_per_node(slot, lhs_bytes):
 # Kernel of each single-kernel timing slot, as rocprofv3 names it (the i
 # direction runs as instance 3: j pencils of the transposed copies).
 SLOT_KERNEL = {"rhs_inv_i": "k_rhs_inv<3,", "rhs_inv_j": "k_rhs_inv<1,", "rhs_inv_k": "k_rhs_inv<2,",
-               "rhs_vis_i": "k_rhs_vis<3>", "rhs_vis_j": "k_rhs_vis<1>", "rhs_vis_k": "k_rhs_vis<2>",
+               "rhs_vis_i": "k_vis_face<3>", "rhs_vis_j": "k_vis_face<1>", "rhs_vis_k": "k_vis_face<2>",
+               "vis_rows": "k_vis_rows<",
                "vis_deriv": "k_vis_node_deriv", "sweep_lu_l": "k_sweep_pipe<0,", "sweep_lu_u": "k_sweep_pipe<1,",
                "sweep_gs_l": "k_sweep_pipe<2,", "sweep_gs_u": "k_sweep_pipe<3,", "sweep_stage": "k_skew_stage<",
                "update": "k_update"}

@@ -172,7 +172,8 @@ int wcns_sync(wcns_ctx *ctx);
 #define WCNS_T_SA_SWEEP_U  21
 #define WCNS_T_SA_UPDATE   22   /* SA: update, mu_t, max_control */
 #define WCNS_T_TRANSPOSE   23   /* i<->j transposed copies for the i-direction residuals */
-#define WCNS_T_NSLOT       24
+#define WCNS_T_VIS_ROWS    24   /* viscous FLUX_DXYZ rows (WCNS_T_VIS_I/J/K: the half-node fluxes) */
+#define WCNS_T_NSLOT       25
 int wcns_timing(wcns_ctx *ctx, int on);
 int wcns_timing_get(wcns_ctx *ctx, int slot, double *total_ms, long long *launches);
 

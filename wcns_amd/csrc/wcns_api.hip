@@ -92,6 +92,7 @@ struct wcns_ctx {
     double *pt[5] = {nullptr}, *t_t = nullptr, *visl_t = nullptr, *vist_t = nullptr;
     double *dn_t = nullptr;                 // 12 slots; 0..3 <- j derivatives, 8..11 <- k derivatives
     double *df_t = nullptr;                 // 5: the i-direction contribution, transposed
+    double *fvb = nullptr;                  // 5: viscous half-node fluxes of one direction
     int *d_flag = nullptr;
     int flag_cap = 512;
     double *d_partial = nullptr;
@@ -458,7 +459,7 @@ int wcns_destroy(wcns_ctx *ctx)
     for (int m = 0; m < 3; m++) { hipFree(ctx->sr[m]); hipFree(ctx->srv[m]); }
     void *pl[] = {ctx->d_sblocks, ctx->d_work, ctx->d_xch};
     for (void *p : pl) if (p) hipFree(p);
-    void *ps[] = {ctx->tpool, ctx->d_blocks_t, ctx->d_grid, ctx->qkw, ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
+    void *ps[] = {ctx->fvb, ctx->tpool, ctx->d_blocks_t, ctx->d_grid, ctx->qkw, ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
                   ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
@@ -648,6 +649,7 @@ int wcns_setup(wcns_ctx *ctx)
     TRY(dalloc(ctx, &ctx->dtdt, T));
     TRY(dalloc(ctx, &ctx->visl, T));
     if (p.nvis == 1) TRY(dalloc(ctx, &ctx->dn, 12 * T));
+    if (p.nvis == 1) TRY(dalloc(ctx, &ctx->fvb, 5 * T));
     if (turb) {
         TRY(dalloc(ctx, &ctx->qke, T));
         TRY(dalloc(ctx, &ctx->vist, T));
@@ -695,7 +697,7 @@ int wcns_setup(wcns_ctx *ctx)
         for (int kind = 0; kind < GRID_N; kind++) {
             start[kind] = all.size();
             const long bs = grid_block_size(kind);
-            const bool tview = kind == GRID_RHS_INV_T || kind == GRID_RHS_VIS_T;
+            const bool tview = kind == GRID_RHS_INV_T || kind == GRID_RHS_VIS_T || kind == GRID_VFACE || kind == GRID_VROW;
             for (size_t nb = 0; nb < hb.size(); nb++) {
                 const long nw = (grid_threads(kind, tview ? hbt[nb] : hb[nb]) + bs - 1) / bs;
                 for (long w = 0; w < nw; w++) all.push_back(make_int2((int)nb, (int)w));
@@ -706,7 +708,7 @@ int wcns_setup(wcns_ctx *ctx)
             TRY(copy_sync(ctx, ctx->d_grid, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice));
         for (int kind = 0; kind < GRID_N; kind++) {
             const size_t end = kind + 1 < GRID_N ? start[kind + 1] : all.size();
-            const bool tview = kind == GRID_RHS_INV_T || kind == GRID_RHS_VIS_T;
+            const bool tview = kind == GRID_RHS_INV_T || kind == GRID_RHS_VIS_T || kind == GRID_VFACE || kind == GRID_VROW;
             ctx->grid[kind] = BlockGrid{tview ? ctx->d_blocks_t : ctx->d_blocks, ctx->d_grid + start[kind],
                                         (int)(end - start[kind])};
         }
@@ -1064,13 +1066,30 @@ int wcns_r_h_s_vis(wcns_ctx *ctx)
                 Scope st(ctx, WCNS_T_TRANSPOSE);
                 if (stage_transposed(ctx, F, true)) return fail(ctx, WCNS_E_HIP, "transposing the viscous inputs");
             }
+            const DFields FT = fields_t(ctx, F);
             {
                 Scope sc(ctx, WCNS_T_VIS_I);
-                if (launch_rhs_vis_dir(3, ctx->grid[GRID_RHS_VIS_T], fields_t(ctx, F), ctx->dp, ctx->dn_t, ctx->stream))
+                if (launch_vis_faces(3, ctx->grid[GRID_VFACE], FT, ctx->dp, ctx->dn_t, ctx->fvb, ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "vis launch");
+            }
+            {
+                Scope sc(ctx, WCNS_T_VIS_ROWS);
+                if (launch_vis_rows(3, ctx->grid[GRID_VROW], FT, ctx->dp, ctx->fvb, ctx->stream))
                     return fail(ctx, WCNS_E_HIP, "vis launch");
             }
             Scope st(ctx, WCNS_T_TRANSPOSE);
             if (add_transposed(ctx, F)) return fail(ctx, WCNS_E_HIP, "adding the i-direction viscous terms");
+            continue;
+        }
+        if (dir > 0) {
+            {
+                Scope sc(ctx, WCNS_T_VIS_I + dir);
+                if (launch_vis_faces(dir, ctx->grid[GRID_VFACE + dir], F, ctx->dp, ctx->dn, ctx->fvb, ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "vis launch");
+            }
+            Scope sc(ctx, WCNS_T_VIS_ROWS);
+            if (launch_vis_rows(dir, ctx->grid[GRID_VROW + dir], F, ctx->dp, ctx->fvb, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "vis launch");
             continue;
         }
         Scope sc(ctx, WCNS_T_VIS_I + dir);

@@ -59,7 +59,9 @@ enum {
     GRID_TRANS = 12,              // 32x32 tiles of every k plane (i<->j transposes)
     GRID_RHS_INV_T = 13,          // inviscid i pencils as j pencils of the transposed view
     GRID_RHS_VIS_T = 14,          // viscous i pencils, likewise
-    GRID_N = 15
+    GRID_VFACE = 15,              // + d (0: i on the transposed view, 1: j, 2: k): viscous half nodes
+    GRID_VROW = 18,               // + d: viscous FLUX_DXYZ rows
+    GRID_N = 21
 };
 long grid_threads(int kind, const DBlock &b);
 int grid_block_size(int kind);
@@ -107,6 +109,11 @@ int launch_dq_zero_face32(const DRegion &g, const DBlock &b, float *arr, long to
 int launch_vis_node_deriv(const BlockGrid &G, const DFields &F, double *dn, hipStream_t s);
 int launch_rhs_vis_dir(int dir, const BlockGrid &G, const DFields &F, const DParams &P, const double *dn, hipStream_t s);
 int launch_spectvis(const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s);
+// the viscous residual of one direction in two passes (dir 1: j, 2: k, 3: i on the
+// transposed view); fvb: 5 pool arrays of half-node fluxes
+int launch_vis_faces(int dir, const BlockGrid &G, const DFields &F, const DParams &P, const double *dn, double *fvb,
+                     hipStream_t s);
+int launch_vis_rows(int dir, const BlockGrid &G, const DFields &F, const DParams &P, const double *fvb, hipStream_t s);
 
 int launch_bc_region(const DRegion &g, const DBlock *blocks, const DFields &F, const DParams &P, hipStream_t s);
 int launch_init_stage(const BlockGrid &nodes, const BlockGrid *ghost_t, const DFields &F, const DParams &P,

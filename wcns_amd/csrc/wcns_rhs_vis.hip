@@ -283,6 +283,75 @@ __global__ void __launch_bounds__(128) k_rhs_vis(BlockGrid G, DFields F, DParams
     }
 }
 
+// The same viscous residual split in two passes (used for the j, k and, on the
+// transposed view, the i direction): k_vis_face forms the flux of one half node
+// per thread (face f of a pencil stored at the pencil's node f of fvb, 5 pool
+// arrays), k_vis_rows applies the FLUX_DXYZ rows (WCNSE5.f90:229-251) per node.
+// One face per thread keeps the register footprint of face_visc alone, so the
+// kernel runs several waves per SIMD without spilling; the faces that two
+// segments of the marching form would share are formed once.
+template <int DIR0>
+__global__ void __launch_bounds__(256) k_vis_face(BlockGrid G, DFields F, DParams P, const double *dn, double *fvb)
+{
+    constexpr int DIR = DIR0 == 3 ? 1 : DIR0;
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
+    const int n = DIR == 0 ? b.ni : (DIR == 1 ? b.nj : b.nk);
+    const int na = DIR == 0 ? b.nj : b.ni;
+    const int nbk = DIR == 2 ? b.nj : b.nk;
+    const long npen = (long)na * nbk;
+    if (t >= npen * (n + 1)) return;
+    const int f = (int)(t / npen);
+    const long pen = t - (long)f * npen;
+    const int ia = (int)(pen % na) + 1, ib = (int)(pen / na) + 1;
+    const int i0 = DIR == 0 ? 0 : ia, j0 = DIR == 1 ? 0 : (DIR == 0 ? ia : ib), k0 = DIR == 2 ? 0 : ib;
+    const VLine L = line_of(b, DIR, i0, j0, k0);
+    bool fl, fr;
+    pencil_flags(F, L, fl, fr);
+    const double cp = 1.0 / ((P.gama - 1.0) * P.moo * P.moo);
+    const double cp_prl = cp / P.prl, cp_prt = cp / P.prt;
+    double fv[5];
+    face_visc<DIR>(F, P, dn, L, f, fl, fr, cp_prl, cp_prt, fv);
+    const long g = L.at(f);
+#pragma unroll
+    for (int m = 0; m < 5; m++) fvb[m * F.total + g] = fv[m];
+}
+
+template <int DIR0>
+__global__ void __launch_bounds__(256) k_vis_rows(BlockGrid G, DFields F, DParams P, const double *fvb)
+{
+    constexpr int DIR = DIR0 == 3 ? 1 : DIR0;
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
+    const int n = DIR == 0 ? b.ni : (DIR == 1 ? b.nj : b.nk);
+    const int na = DIR == 0 ? b.nj : b.ni;
+    const int nbk = DIR == 2 ? b.nj : b.nk;
+    const long npen = (long)na * nbk;
+    if (t >= npen * n) return;
+    const int l = (int)(t / npen) + 1;
+    const long pen = t - (long)(l - 1) * npen;
+    const int ia = (int)(pen % na) + 1, ib = (int)(pen / na) + 1;
+    const int i0 = DIR == 0 ? 0 : ia, j0 = DIR == 1 ? 0 : (DIR == 0 ? ia : ib), k0 = DIR == 2 ? 0 : ib;
+    const VLine L = line_of(b, DIR, i0, j0, k0);
+    const double re = P.re;
+    const long g = L.at(l);
+    auto fa = [&](int m, int o) { return fvb[m * F.total + L.at(o)]; };
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        double df;
+        if (l == 1) df = (-23.0 * fa(m, 0) + 21.0 * fa(m, 1) + 3.0 * fa(m, 2) - fa(m, 3)) / 24.0;
+        else if (l == 2) df = (fa(m, 0) - 27.0 * fa(m, 1) + 27.0 * fa(m, 2) - fa(m, 3)) / 24.0;
+        else if (l == n - 1) df = -(fa(m, n) - 27.0 * fa(m, n - 1) + 27.0 * fa(m, n - 2) - fa(m, n - 3)) / 24.0;
+        else if (l == n) df = -(-23.0 * fa(m, n) + 21.0 * fa(m, n - 1) + 3.0 * fa(m, n - 2) - fa(m, n - 3)) / 24.0;
+        else
+            df = (2250.0 * (fa(m, l) - fa(m, l - 1)) - 125.0 * (fa(m, l + 1) - fa(m, l - 2)) +
+                  9.0 * (fa(m, l + 2) - fa(m, l - 3))) / 1920.0;
+        F.rhs[m * F.total + g] = F.rhs[m * F.total + g] - re * df;
+    }
+}
+
 // LHS.f90:243-343 spectvisl (laminar) / LHS.f90:141-241 spectvist (mu + mu_t), interior
 // nodes (ghost-layer radii only ever multiply a zero dq)
 __global__ void k_spectvis(BlockGrid G, DFields F, DParams P)
@@ -334,6 +403,39 @@ int launch_rhs_vis_dir(int dir, const BlockGrid &G, const DFields &F, const DPar
     if (dir == 1) hipLaunchKernelGGL(k_rhs_vis<1>, grid, blk, 0, s, G, F, P, dn);
     if (dir == 2) hipLaunchKernelGGL(k_rhs_vis<2>, grid, blk, 0, s, G, F, P, dn);
     if (dir == 3) hipLaunchKernelGGL(k_rhs_vis<3>, grid, blk, 0, s, G, F, P, dn);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+long vis_face_threads(int dir, const DBlock &b)
+{
+    int n = dir == 0 ? b.ni : (dir == 1 ? b.nj : b.nk);
+    long npen = dir == 0 ? (long)b.nj * b.nk : (dir == 1 ? (long)b.ni * b.nk : (long)b.ni * b.nj);
+    return npen * (n + 1);
+}
+
+long vis_row_threads(int dir, const DBlock &b)
+{
+    return (long)b.ni * b.nj * b.nk;
+}
+
+int launch_vis_faces(int dir, const BlockGrid &G, const DFields &F, const DParams &P, const double *dn, double *fvb,
+                     hipStream_t s)
+{
+    if (G.nwg == 0) return 0;
+    if (dir == 1) hipLaunchKernelGGL(k_vis_face<1>, dim3(G.nwg), dim3(256), 0, s, G, F, P, dn, fvb);
+    else if (dir == 2) hipLaunchKernelGGL(k_vis_face<2>, dim3(G.nwg), dim3(256), 0, s, G, F, P, dn, fvb);
+    else if (dir == 3) hipLaunchKernelGGL(k_vis_face<3>, dim3(G.nwg), dim3(256), 0, s, G, F, P, dn, fvb);
+    else return -2;
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_vis_rows(int dir, const BlockGrid &G, const DFields &F, const DParams &P, const double *fvb, hipStream_t s)
+{
+    if (G.nwg == 0) return 0;
+    if (dir == 1) hipLaunchKernelGGL(k_vis_rows<1>, dim3(G.nwg), dim3(256), 0, s, G, F, P, fvb);
+    else if (dir == 2) hipLaunchKernelGGL(k_vis_rows<2>, dim3(G.nwg), dim3(256), 0, s, G, F, P, fvb);
+    else if (dir == 3) hipLaunchKernelGGL(k_vis_rows<3>, dim3(G.nwg), dim3(256), 0, s, G, F, P, fvb);
+    else return -2;
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

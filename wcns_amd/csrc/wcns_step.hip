@@ -610,6 +610,8 @@ int launch_timestep(const BlockGrid &G, const DFields &F, const DParams &P, hipS
 
 long rhs_inv_threads(int dir, const DBlock &b);
 long rhs_vis_threads(int dir, const DBlock &b);
+long vis_face_threads(int dir, const DBlock &b);
+long vis_row_threads(int dir, const DBlock &b);
 
 long grid_threads(int kind, const DBlock &b)
 {
@@ -628,12 +630,15 @@ long grid_threads(int kind, const DBlock &b)
     // b is the transposed view here: its j pencils are the original i pencils
     if (kind == GRID_RHS_INV_T) return rhs_inv_threads(1, b);
     if (kind == GRID_RHS_VIS_T) return rhs_vis_threads(1, b);
+    // d = 0 is the i direction as the transposed view's j pencils (b is that view)
+    if (kind >= GRID_VFACE && kind < GRID_VFACE + 3) return vis_face_threads(kind == GRID_VFACE ? 1 : kind - GRID_VFACE, b);
+    if (kind >= GRID_VROW && kind < GRID_VROW + 3) return vis_row_threads(kind == GRID_VROW ? 1 : kind - GRID_VROW, b);
     return 0;
 }
 
 int grid_block_size(int kind)
 {
-    return (kind == GRID_NODES || kind == GRID_SA_UPD || kind == GRID_TRANS) ? 256 : 128;
+    return (kind == GRID_NODES || kind == GRID_SA_UPD || kind == GRID_TRANS || kind >= GRID_VFACE) ? 256 : 128;
 }
 
 int launch_zero_rhs(const DBlock &b, const DFields &F, hipStream_t s)

@@ -36,7 +36,7 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
 TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
                 "sweep_stage", "rhs_vis_i", "rhs_vis_j", "rhs_vis_k", "sa_rhs", "sa_sweep_l", "sa_sweep_u",
-                "sa_update", "transpose"]
+                "sa_update", "transpose", "vis_rows"]
 
 
 class WcnsError(RuntimeError):
