@@ -177,6 +177,35 @@ int wcns_sync(wcns_ctx *ctx);
 int wcns_timing(wcns_ctx *ctx, int on);
 int wcns_timing_get(wcns_ctx *ctx, int slot, double *total_ms, long long *launches);
 
+/* ---- tangent and adjoint of the residual (no counterpart in the reference) ----
+ * R(P, X) of block nb is the right-hand side that wcns_initial_q_dq_c_t, the
+ * mean-flow part of wcns_r_h_s_vis (mu_t held at its stored value: no
+ * turbulence-model update) and wcns_r_h_s_invis form at the interior nodes from
+ *   P  the primitives r, u, v, w, p of every node -2..n+3, ghosts included as
+ *      wcns_boundary_all leaves them: 5 arrays of (ni+6)(nj+6)(nk+6) doubles,
+ *      back to back, i fastest (the layout of wcns_get_field);
+ *   X  the node coordinates x, y, z: 3 arrays of ni*nj*nk doubles (PLOT3D
+ *      order), through the conservative metrics of GRID_DERIVATIVE_gcl
+ *      (src/WCNSE5.f90:1478-1728).
+ * R, dR and W are 5 arrays of ni*nj*nk doubles (interior nodes, i fastest).
+ * Every array is DEVICE memory.  Each call leaves the block's primitives and
+ * metrics set to (P, X) (the implicit operator's staged metrics follow; the
+ * symmetry-plane normals and wall distances keep their set-up values) and the
+ * residual of that state in place; calls are synchronous.  Blocks need more
+ * than 4 nodes in every direction. */
+int wcns_adj_residual(wcns_ctx *ctx, int nb, const double *P, const double *X, double *R);
+/* tangent: dR = (dR/dP) dP + (dR/dX) dX (dP or dX may be NULL: zero); R may be NULL */
+int wcns_adj_jvp(wcns_ctx *ctx, int nb, const double *P, const double *X, const double *dP,
+                 const double *dX, double *R, double *dR);
+/* adjoint: gP = (dR/dP)^T W (5 G3 arrays), gX = (dR/dX)^T W (3 arrays); either may be NULL */
+int wcns_adj_vjp(wcns_ctx *ctx, int nb, const double *P, const double *X, const double *W,
+                 double *gP, double *gX);
+/* J = sum_c omega[c] * C[c], C the pressure part (src/PostProcess.f90:1560-1564)
+ * of wcns_force's cfx, cfy, cfz, cmx, cmy, cmz summed over the block's no-slip
+ * walls, and its gradient gP (dJ/dP, 5 G3 arrays), gX (dJ/dX); either may be NULL */
+int wcns_adj_force(wcns_ctx *ctx, int nb, const double *P, const double *X, const double omega[6],
+                   double *J, double *gP, double *gX);
+
 /* debugging/inspection of the host-side setup */
 int wcns_region_images(wcns_ctx *ctx, int nb, int nr, int *img, int *jmg, int *kmg, int box[6]);
 int wcns_multipoint(wcns_ctx *ctx, int *ngroups, int *npts, int *pts, int cap);

@@ -108,6 +108,10 @@ struct wcns_ctx {
     struct Wall { DRegion d; double *xyz; };  // no-slip walls with their face-node coordinates
     std::vector<Wall> walls;
     double *d_force_pn = nullptr, *d_force_part = nullptr, *d_force_out = nullptr;
+    // tangent / adjoint of the residual, allocated on first use
+    double *ad_pool = nullptr;              // AD_N arrays of total elements
+    double *ad_work = nullptr;              // metric slots of the largest block
+    double *ad_force = nullptr;             // traction / coordinate gradients and coordinates of a wall
     int npp = 0, ngroups = 0;
     int *d_npp_pts = nullptr, *d_grp_start = nullptr, *d_grp_n = nullptr, *d_write_ok = nullptr;
     double *d_npp_buf = nullptr;
@@ -464,7 +468,7 @@ int wcns_destroy(wcns_ctx *ctx)
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
     for (auto &w : ctx->walls) hipFree(w.xyz);
-    void *fs[] = {ctx->d_force_pn, ctx->d_force_part, ctx->d_force_out};
+    void *fs[] = {ctx->d_force_pn, ctx->d_force_part, ctx->d_force_out, ctx->ad_pool, ctx->ad_work, ctx->ad_force};
     for (void *p : fs) if (p) hipFree(p);
     for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
     for (int c = 0; c < 2; c++)
@@ -1031,6 +1035,8 @@ int wcns_timestep(wcns_ctx *ctx)
     return WCNS_OK;
 }
 
+static int vis_meanflow(wcns_ctx *ctx, const DFields &F);
+
 int wcns_r_h_s_vis(wcns_ctx *ctx)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
@@ -1055,6 +1061,12 @@ int wcns_r_h_s_vis(wcns_ctx *ctx)
         if (launch_sa_update(ctx->grid[GRID_SA_UPD], F, ctx->sa, ctx->dp, ctx->stream))
             return fail(ctx, WCNS_E_HIP, "SA update launch");
     }
+    return vis_meanflow(ctx, F);
+}
+
+// the mean-flow viscous terms of r_h_s_vis (mu_t as stored)
+static int vis_meanflow(wcns_ctx *ctx, const DFields &F)
+{
     {
         Scope sc(ctx, WCNS_T_VIS);
         if (launch_vis_node_deriv(ctx->grid[GRID_NODES], F, ctx->dn, ctx->stream))
@@ -1499,6 +1511,222 @@ int wcns_multipoint(wcns_ctx *ctx, int *ngroups, int *npts, int *pts, int cap)
         pts[a] = (a % 4 == 0) ? v + 1 : v;     // block 1-based like the reference
     }
     return n / 4;
+}
+
+
+// ------------------------------------------------ tangent and adjoint of the residual
+// AD pool: P-direction / P-gradient 5, metrics 10, T 1, mu 1, node derivatives 12,
+// residual tangent 5, adjoint weights 5
+enum { AD_P = 0, AD_M = 5, AD_T = 15, AD_MU = 16, AD_DN = 17, AD_R = 29, AD_W = 34, AD_N = 39 };
+
+static int ad_alloc(wcns_ctx *ctx)
+{
+    if (ctx->ad_pool) return WCNS_OK;
+    long g3max = 1, wmax = 1;
+    for (auto &b : ctx->blocks) g3max = std::max(g3max, (long)b.g3);
+    for (auto &w : ctx->walls) wmax = std::max(wmax, (long)w.d.npts);
+    TRY(dalloc(ctx, &ctx->ad_pool, (size_t)AD_N * ctx->total));
+    TRY(dalloc(ctx, &ctx->ad_work, (size_t)gcl_ad_slots() * g3max + 1));
+    TRY(dalloc(ctx, &ctx->ad_force, (size_t)9 * wmax));
+    return WCNS_OK;
+}
+
+static int ad_check(wcns_ctx *ctx, int nb)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (nb < 0 || nb >= (int)ctx->blocks.size()) return fail(ctx, WCNS_E_ARG, "block index");
+    const HBlock &b = ctx->blocks[nb];
+    if (b.ni <= 4 || b.nj <= 4 || b.nk <= 4)
+        return fail(ctx, WCNS_E_UNSUPPORTED, "the residual adjoint needs more than 4 nodes per direction");
+    return ad_alloc(ctx);
+}
+
+// the block's primitives <- P, metrics <- GCL(X) (and their transposed copies)
+static int ad_state(wcns_ctx *ctx, int nb, const double *P, const double *X)
+{
+    const HBlock &b = ctx->blocks[nb];
+    DFields F = ctx->fields();
+    double *dst[5] = {F.r, F.u, F.v, F.w, F.p};
+    for (int m = 0; m < 5; m++)
+        CK(hipMemcpyAsync(dst[m] + b.db.off, P + (size_t)m * b.g3, 8 * (size_t)b.g3, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    const long nn = (long)b.ni * b.nj * b.nk;
+    if (launch_gcl_metrics(b.db, X, X + nn, X + 2 * nn, F, ctx->ad_work, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "metrics launch");
+    if (ctx->idir_t) {
+        TransList L{};
+        for (int m = 0; m < 10; m++) { L.src[L.n] = ctx->metric[m]; L.dst[L.n++] = ctx->mt[m]; }
+        if (launch_transpose(0, ctx->grid[GRID_TRANS], L, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "transposing the metrics");
+    }
+    // the implicit operator's staged copy of the metrics follows them
+    for (int prec = 0; prec < 2; prec++)
+        if (ctx->skew[prec] && launch_skew_stage(SKEW_METRICS, b.db, ctx->skoff[nb], F, prec, ctx->skew[prec],
+                                                 ctx->sktot, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "staging the metrics");
+    return WCNS_OK;
+}
+
+// the residual of the current state (no turbulence-model update), left in rhs
+static int ad_rhs(wcns_ctx *ctx)
+{
+    int e = wcns_initial_q_dq_c_t(ctx);
+    if (e) return e;
+    if (ctx->prm.nvis == 1 && (e = vis_meanflow(ctx, ctx->fields()))) return e;
+    return wcns_r_h_s_invis(ctx);
+}
+
+static int ad_zero(wcns_ctx *ctx, int first, int count)
+{
+    return launch_fill_u64(reinterpret_cast<unsigned long long *>(ctx->ad_pool + (size_t)first * ctx->total),
+                           (long)count * ctx->total, 0ull, ctx->stream)
+               ? fail(ctx, WCNS_E_HIP, "clearing the adjoint arrays")
+               : WCNS_OK;
+}
+
+int wcns_adj_residual(wcns_ctx *ctx, int nb, const double *P, const double *X, double *R)
+{
+    int e = ad_check(ctx, nb);
+    if (e) return e;
+    if (!P || !X || !R) return fail(ctx, WCNS_E_ARG, "null array");
+    if ((e = ad_state(ctx, nb, P, X)) || (e = ad_rhs(ctx))) return e;
+    if (launch_interior(ctx->blocks[nb].db, ctx->rhs, ctx->total, R, 5, 0, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "residual copy");
+    CK(hipStreamSynchronize(ctx->stream));
+    return WCNS_OK;
+}
+
+int wcns_adj_jvp(wcns_ctx *ctx, int nb, const double *P, const double *X, const double *dP, const double *dX,
+                 double *R, double *dR)
+{
+    int e = ad_check(ctx, nb);
+    if (e) return e;
+    if (!P || !X || !dR) return fail(ctx, WCNS_E_ARG, "null array");
+    const HBlock &b = ctx->blocks[nb];
+    const long T = ctx->total;
+    if ((e = ad_state(ctx, nb, P, X)) || (e = ad_rhs(ctx))) return e;
+    if (R && launch_interior(b.db, ctx->rhs, T, R, 5, 0, ctx->stream)) return fail(ctx, WCNS_E_HIP, "residual copy");
+    if ((e = ad_zero(ctx, 0, AD_W))) return e;
+    double *A = ctx->ad_pool;
+    if (dP)
+        for (int m = 0; m < 5; m++)
+            CK(hipMemcpyAsync(A + (AD_P + m) * T + b.db.off, dP + (size_t)m * b.g3, 8 * (size_t)b.g3,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    DFields F = ctx->fields();
+    if (dX && launch_gcl_jvp(b.db, F, X, dX, A + AD_M * T, ctx->ad_work, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "metric tangent");
+    AdTan D{A + AD_P * T, A + AD_M * T, A + AD_T * T, A + AD_MU * T, A + AD_DN * T, A + AD_R * T};
+    if (ctx->prm.nvis == 1) {
+        if (launch_tmu_jvp(b.db, F, ctx->dp, A + AD_T * T, A + AD_MU * T, A + AD_P * T, ctx->stream) ||
+            launch_dn_jvp(b.db, F, D, A + AD_DN * T, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "viscous tangent");
+        for (int dir = 0; dir < 3; dir++)
+            if (launch_rhs_vis_jvp(dir, b.db, F, ctx->dp, ctx->dn, D, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "viscous tangent");
+    }
+    for (int dir = 0; dir < 3; dir++)
+        if (launch_rhs_inv_jvp(dir, b.db, F, ctx->dp, D, ctx->stream)) return fail(ctx, WCNS_E_HIP, "inviscid tangent");
+    if (launch_interior(b.db, A + AD_R * T, T, dR, 5, 0, ctx->stream)) return fail(ctx, WCNS_E_HIP, "tangent copy");
+    CK(hipStreamSynchronize(ctx->stream));
+    return WCNS_OK;
+}
+
+int wcns_adj_vjp(wcns_ctx *ctx, int nb, const double *P, const double *X, const double *W, double *gP, double *gX)
+{
+    int e = ad_check(ctx, nb);
+    if (e) return e;
+    if (!P || !X || !W) return fail(ctx, WCNS_E_ARG, "null array");
+    const HBlock &b = ctx->blocks[nb];
+    const long T = ctx->total, nn = (long)b.ni * b.nj * b.nk;
+    if ((e = ad_state(ctx, nb, P, X)) || (e = ad_rhs(ctx))) return e;
+    if ((e = ad_zero(ctx, 0, AD_N))) return e;
+    double *A = ctx->ad_pool;
+    if (launch_interior(b.db, A + AD_W * T, T, const_cast<double *>(W), 5, 1, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "weight copy");
+    DFields F = ctx->fields();
+    AdAdj G{A + AD_W * T, A + AD_P * T, A + AD_M * T, A + AD_T * T, A + AD_MU * T, A + AD_DN * T};
+    for (int dir = 0; dir < 3; dir++)
+        if (launch_rhs_inv_vjp(dir, b.db, F, ctx->dp, G, ctx->stream)) return fail(ctx, WCNS_E_HIP, "inviscid adjoint");
+    if (ctx->prm.nvis == 1) {
+        for (int dir = 0; dir < 3; dir++)
+            if (launch_rhs_vis_vjp(dir, b.db, F, ctx->dp, ctx->dn, G, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "viscous adjoint");
+        if (launch_dn_vjp(b.db, F, G, ctx->stream) ||
+            launch_tmu_vjp(b.db, F, ctx->dp, A + AD_T * T, A + AD_MU * T, A + AD_P * T, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "viscous adjoint");
+    }
+    if (gX) {
+        CK(hipMemsetAsync(gX, 0, 8 * 3 * (size_t)nn, ctx->stream));
+        if (launch_gcl_vjp(b.db, F, X, A + AD_M * T, gX, ctx->ad_work, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "metric adjoint");
+    }
+    if (gP)
+        for (int m = 0; m < 5; m++)
+            CK(hipMemcpyAsync(gP + (size_t)m * b.g3, A + (AD_P + m) * T + b.db.off, 8 * (size_t)b.g3,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return WCNS_OK;
+}
+
+int wcns_adj_force(wcns_ctx *ctx, int nb, const double *P, const double *X, const double omega[6], double *J,
+                   double *gP, double *gX)
+{
+    int e = ad_check(ctx, nb);
+    if (e) return e;
+    if (!P || !X || !omega || !J) return fail(ctx, WCNS_E_ARG, "null array");
+    if (ctx->prm.sref == 0.0 || ctx->prm.lfref == 0.0) return fail(ctx, WCNS_E_ARG, "sref and lfref must be non-zero");
+    const HBlock &b = ctx->blocks[nb];
+    const long T = ctx->total, nn = (long)b.ni * b.nj * b.nk;
+    if ((e = ad_state(ctx, nb, P, X))) return e;
+    DFields F = ctx->fields();
+    DParams dpp = ctx->dp;
+    dpp.nvis = 0;                      // the pressure part only
+    const wcns_params &p = ctx->prm;
+    const double ref[3] = {p.xref, p.yref, p.zref};
+    // the coefficients' scaling of wcns_force (PostProcess.f90:1415-1477)
+    const int wf = 2 - p.nwholefield;
+    double om[6];
+    for (int c = 0; c < 6; c++) om[c] = omega[c] * wf * (c < 3 ? 1.0 / p.sref : 1.0 / (p.sref * p.lfref));
+    double *A = ctx->ad_pool;
+    const bool grad = gP || gX;
+    if (grad) {
+        if ((e = ad_zero(ctx, AD_P, 15))) return e;
+        if (gX) CK(hipMemsetAsync(gX, 0, 8 * 3 * (size_t)nn, ctx->stream));
+    }
+    double *lpn3 = ctx->ad_force, *lxyz = nullptr, *xyz = nullptr;
+    double sum = 0.0;
+    for (size_t w = 0; w < ctx->walls.size(); w++) {
+        const auto &wr = ctx->walls[w];
+        if (wr.d.nb != nb) continue;
+        const int np = wr.d.npts;
+        lxyz = lpn3 + 3 * (size_t)np;
+        xyz = lxyz + 3 * (size_t)np;
+        if (launch_wall_xyz(wr.d, b.db, const_cast<double *>(X), xyz, 0, ctx->stream) ||
+            launch_force_region(wr.d, b.db, F, dpp, xyz, ctx->d_force_pn, ctx->d_force_part, ctx->d_force_out, ref,
+                                ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "force launch");
+        double part[6];
+        CK(hipMemcpyAsync(part, ctx->d_force_out, sizeof part, hipMemcpyDeviceToHost, ctx->stream));
+        if (grad) {
+            if (launch_force_vjp(wr.d, b.db, F, dpp, ctx->d_force_pn, xyz, ref, om, lpn3, lxyz, A + AD_P * T,
+                                 A + AD_M * T, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "force adjoint");
+            if (gX && launch_wall_xyz(wr.d, b.db, gX, lxyz, 1, ctx->stream)) return fail(ctx, WCNS_E_HIP, "force adjoint");
+        }
+        CK(hipStreamSynchronize(ctx->stream));
+        for (int c = 0; c < 6; c++) sum = sum + om[c] * part[c];
+    }
+    *J = sum;
+    if (grad) {
+        if (gX && launch_gcl_vjp(b.db, F, X, A + AD_M * T, gX, ctx->ad_work, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "metric adjoint");
+        if (gP)
+            for (int m = 0; m < 5; m++)
+                CK(hipMemcpyAsync(gP + (size_t)m * b.g3, A + (AD_P + m) * T + b.db.off, 8 * (size_t)b.g3,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+    }
+    return WCNS_OK;
 }
 
 }  // extern "C"

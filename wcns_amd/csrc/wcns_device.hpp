@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "wcns_ad.hpp"
 
 namespace wcns {
 
@@ -70,37 +71,41 @@ __device__ inline void getnxyz(const DFields &F, long g, const int *l3, double &
 }
 
 // ------------------------------------------------------------- gas model
+// The residual arithmetic below is generic over the scalar type T: double for
+// the solver, D1 (wcns_ad.hpp) for the tangent and adjoint kernels.
+
 // GasModel.f90:358-373 getem
-__device__ inline double getem(double rm, double um, double vm, double wm, double pm, double gama)
+template <typename T>
+__device__ inline T getem(T rm, T um, T vm, T wm, T pm, double gama)
 {
     return pm / (gama - 1.0) + 0.5 * rm * (um * um + vm * vm + wm * wm);
 }
 
 // flux.f90:17-77 flux_SW_pn (kt = 0: static grid, PreProcess.f90:1649-1657)
-__device__ inline void flux_sw_pn(const double *prim, double nx, double ny, double nz, double *f,
-                                  double efix, double gama, double npn)
+template <typename T>
+__device__ inline void flux_sw_pn(const T *prim, T nx, T ny, T nz, T *f, double efix, double gama, double npn)
 {
-    double rm = prim[0], um = prim[1], vm = prim[2], wm = prim[3], pm = prim[4];
-    double u2 = 0.5 * (um * um + vm * vm + wm * wm);
-    double hint = gama / (gama - 1.0) * pm / rm;
-    double eint = hint - pm / rm;
-    double em = eint + u2;
-    double cm = sqrt(gama * pm / rm);
-    double ct = nx * um + ny * vm + nz * wm + 0.0;
-    double cgm = sqrt(nx * nx + ny * ny + nz * nz);
-    cgm = cgm < SML_SSS ? SML_SSS : cgm;
-    double l1 = ct, l4 = ct + cm * cgm, l5 = ct - cm * cgm;
-    double eps = efix * efix * cgm * cgm;
+    T rm = prim[0], um = prim[1], vm = prim[2], wm = prim[3], pm = prim[4];
+    T u2 = 0.5 * (um * um + vm * vm + wm * wm);
+    T hint = gama / (gama - 1.0) * pm / rm;
+    T eint = hint - pm / rm;
+    T em = eint + u2;
+    T cm = sqrt(gama * pm / rm);
+    T ct = nx * um + ny * vm + nz * wm + 0.0;
+    T cgm = sqrt(nx * nx + ny * ny + nz * nz);
+    cgm = cgm < SML_SSS ? T(SML_SSS) : cgm;
+    T l1 = ct, l4 = ct + cm * cgm, l5 = ct - cm * cgm;
+    T eps = efix * efix * cgm * cgm;
     l1 = 0.5 * (l1 + npn * sqrt(l1 * l1 + eps));
     l4 = 0.5 * (l4 + npn * sqrt(l4 * l4 + eps));
     l5 = 0.5 * (l5 + npn * sqrt(l5 * l5 + eps));
-    double cgm1 = 1.0 / cgm;
-    double nxa = nx * cgm1, nya = ny * cgm1, nza = nz * cgm1;
-    double cta = (ct - 0.0) * cgm1;
-    double c2r = cm * cm / gama;
-    double x1 = c2r * (2.0 * l1 - l4 - l5) / (2.0 * cm * cm);
-    double x2 = c2r * (l4 - l5) / (2.0 * cm);
-    double hm = em + pm / rm;
+    T cgm1 = 1.0 / cgm;
+    T nxa = nx * cgm1, nya = ny * cgm1, nza = nz * cgm1;
+    T cta = (ct - 0.0) * cgm1;
+    T c2r = cm * cm / gama;
+    T x1 = c2r * (2.0 * l1 - l4 - l5) / (2.0 * cm * cm);
+    T x2 = c2r * (l4 - l5) / (2.0 * cm);
+    T hm = em + pm / rm;
     f[0] = (l1 - x1) * rm;
     f[1] = (l1 * um - x1 * um + nxa * x2) * rm;
     f[2] = (l1 * vm - x1 * vm + nya * x2) * rm;
@@ -109,10 +114,10 @@ __device__ inline void flux_sw_pn(const double *prim, double nx, double ny, doub
 }
 
 // flux.f90:2-15 flux_SW
-__device__ inline void flux_sw(const double *ql, const double *qr, double nx, double ny, double nz,
-                               double *f, double efix, double gama)
+template <typename T>
+__device__ inline void flux_sw(const T *ql, const T *qr, T nx, T ny, T nz, T *f, double efix, double gama)
 {
-    double fl[5], fr[5];
+    T fl[5], fr[5];
     flux_sw_pn(ql, nx, ny, nz, fl, efix, gama, 1.0);
     flux_sw_pn(qr, nx, ny, nz, fr, efix, gama, -1.0);
 #pragma unroll
@@ -120,11 +125,12 @@ __device__ inline void flux_sw(const double *ql, const double *qr, double nx, do
 }
 
 // Convect.f90:1245-1272 flux (kt = 0)
-__device__ inline void flux_phys(const double *prim, double gama, double nx, double ny, double nz, double *f)
+template <typename T>
+__device__ inline void flux_phys(const T *prim, double gama, T nx, T ny, T nz, T *f)
 {
-    double rm = prim[0], um = prim[1], vm = prim[2], wm = prim[3], pm = prim[4];
-    double em = getem(rm, um, vm, wm, pm, gama);
-    double cit = nx * um + ny * vm + nz * wm + 0.0;
+    T rm = prim[0], um = prim[1], vm = prim[2], wm = prim[3], pm = prim[4];
+    T em = getem(rm, um, vm, wm, pm, gama);
+    T cit = nx * um + ny * vm + nz * wm + 0.0;
     f[0] = rm * cit;
     f[1] = f[0] * um + nx * pm;
     f[2] = f[0] * vm + ny * pm;
@@ -133,57 +139,57 @@ __device__ inline void flux_phys(const double *prim, double gama, double nx, dou
 }
 
 // flux.f90:116-244 flux_Roe.  Returns false when c^2 <= 0 (the reference stops).
-__device__ inline bool flux_roe(const double *qlin, const double *qrin, double nx, double ny, double nz,
-                                double *f, double efix, double gama)
+template <typename T>
+__device__ inline bool flux_roe(const T *qlin, const T *qrin, T nx, T ny, T nz, T *f, double efix, double gama)
 {
-    double ql[5], qr[5], fl[5], fr[5], dq[5], qroe[5], df[5];
+    T ql[5], qr[5], fl[5], fr[5], dq[5], qroe[5], df[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) { ql[m] = qlin[m]; qr[m] = qrin[m]; }
     flux_phys(ql, gama, nx, ny, nz, fl);
     flux_phys(qr, gama, nx, ny, nz, fr);
 #pragma unroll
     for (int m = 0; m < 5; m++) dq[m] = qr[m] - ql[m];
-    double gama1 = gama - 1.0;
-    double gama2 = gama / gama1;
+    const double gama1 = gama - 1.0;
+    const double gama2 = gama / gama1;
     ql[4] = gama2 * ql[4] / ql[0] + 0.5 * (ql[1] * ql[1] + ql[2] * ql[2] + ql[3] * ql[3]);
     qr[4] = gama2 * qr[4] / qr[0] + 0.5 * (qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
     qroe[0] = sqrt(ql[0] * qr[0]);
-    double rrl = sqrt(qr[0] / ql[0]);
-    double rrl_1 = rrl + 1.0;
+    T rrl = sqrt(qr[0] / ql[0]);
+    T rrl_1 = rrl + 1.0;
 #pragma unroll
     for (int m = 1; m < 5; m++) qroe[m] = (ql[m] + qr[m] * rrl) / rrl_1;
-    double rm = qroe[0], um = qroe[1], vm = qroe[2], wm = qroe[3], hm = qroe[4];
-    double c2 = (hm - 0.5 * (um * um + vm * vm + wm * wm)) * gama1;
+    T rm = qroe[0], um = qroe[1], vm = qroe[2], wm = qroe[3], hm = qroe[4];
+    T c2 = (hm - 0.5 * (um * um + vm * vm + wm * wm)) * gama1;
     bool ok = c2 > 0.0;
-    double c = sqrt(c2);
-    double cta = um * nx + vm * ny + wm * nz + 0.0;
-    double cgm = sqrt(nx * nx + ny * ny + nz * nz);
-    cgm = cgm < SML_SSS ? SML_SSS : cgm;
-    double cgm1 = 1.0 / cgm;
+    T c = sqrt(c2);
+    T cta = um * nx + vm * ny + wm * nz + 0.0;
+    T cgm = sqrt(nx * nx + ny * ny + nz * nz);
+    cgm = cgm < SML_SSS ? T(SML_SSS) : cgm;
+    T cgm1 = 1.0 / cgm;
     nx = nx * cgm1;
     ny = ny * cgm1;
     nz = nz * cgm1;
-    double ccgm = c * cgm;
-    double l1 = fabs(cta - ccgm), l4 = fabs(cta), l5 = fabs(cta + ccgm);
-    double l12 = l1 * l1, l42 = l4 * l4, l52 = l5 * l5;
-    double deti = efix * cgm;
-    double det2 = deti * deti;
+    T ccgm = c * cgm;
+    T l1 = fabs(cta - ccgm), l4 = fabs(cta), l5 = fabs(cta + ccgm);
+    T l12 = l1 * l1, l42 = l4 * l4, l52 = l5 * l5;
+    T deti = efix * cgm;
+    T det2 = deti * deti;
     l1 = sqrt(l12 + det2);
     l4 = sqrt(l42 + det2);
     l5 = sqrt(l52 + det2);
-    double dcta = dq[1] * nx + dq[2] * ny + dq[3] * nz;
-    double cta1 = cta * cgm1;
-    double rodcta = qroe[0] * dcta;
-    double dp_c2 = dq[4] / c2;
-    double rodctac = rodcta / c;
-    double a1 = l4 * (dq[0] - dp_c2);
-    double a2 = l5 * (dp_c2 + rodctac) * 0.5;
-    double a3 = l1 * (dp_c2 - rodctac) * 0.5;
-    double a4 = a1 + a2 + a3;
-    double a5 = c * (a2 - a3);
-    double a6 = l4 * (rm * dq[1] - nx * rodcta);
-    double a7 = l4 * (rm * dq[2] - ny * rodcta);
-    double a8 = l4 * (rm * dq[3] - nz * rodcta);
+    T dcta = dq[1] * nx + dq[2] * ny + dq[3] * nz;
+    T cta1 = cta * cgm1;
+    T rodcta = qroe[0] * dcta;
+    T dp_c2 = dq[4] / c2;
+    T rodctac = rodcta / c;
+    T a1 = l4 * (dq[0] - dp_c2);
+    T a2 = l5 * (dp_c2 + rodctac) * 0.5;
+    T a3 = l1 * (dp_c2 - rodctac) * 0.5;
+    T a4 = a1 + a2 + a3;
+    T a5 = c * (a2 - a3);
+    T a6 = l4 * (rm * dq[1] - nx * rodcta);
+    T a7 = l4 * (rm * dq[2] - ny * rodcta);
+    T a8 = l4 * (rm * dq[3] - nz * rodcta);
     df[0] = a4;
     df[1] = um * a4 + nx * a5 + a6;
     df[2] = vm * a4 + ny * a5 + a7;
@@ -194,9 +200,8 @@ __device__ inline bool flux_roe(const double *qlin, const double *qrin, double n
     return ok;
 }
 
-template <int FLUX>
-__device__ inline bool face_flux(const double *ql, const double *qr, double nx, double ny, double nz,
-                                 double *f, double efix, double gama)
+template <int FLUX, typename T>
+__device__ inline bool face_flux(const T *ql, const T *qr, T nx, T ny, T nz, T *f, double efix, double gama)
 {
     if (FLUX == 3) return flux_roe(ql, qr, nx, ny, nz, f, efix, gama);
     flux_sw(ql, qr, nx, ny, nz, f, efix, gama);
@@ -206,28 +211,28 @@ __device__ inline bool face_flux(const double *ql, const double *qr, double nx, 
 // WCNSE5.f90:2138-2187 (one variable at one node): nonlinear weights from the
 // five values qm2..qp2 around node i, returning the left state at face i+1/2
 // (qwl(i)) and the right state at face i-1/2 (qwr(i-1)).
-__device__ inline void wcns_node(double qm2, double qm1, double q0, double qp1, double qp2,
-                                 double &qwl, double &qwr)
+template <typename T>
+__device__ inline void wcns_node(T qm2, T qm1, T q0, T qp1, T qp2, T &qwl, T &qwr)
 {
     const double EPS = 1.0e-6;
-    double g1 = 0.5 * (qm2 - 4.0 * qm1 + 3.0 * q0);
-    double g2 = 0.5 * (qp1 - qm1);
-    double g3 = 0.5 * (-3.0 * q0 + 4.0 * qp1 - qp2);
-    double s1 = qm2 - 2.0 * qm1 + q0;
-    double s2 = qm1 - 2.0 * q0 + qp1;
-    double s3 = q0 - 2.0 * qp1 + qp2;
-    double is1 = g1 * g1 + s1 * s1;
-    double is2 = g2 * g2 + s2 * s2;
-    double is3 = g3 * g3 + s3 * s3;
-    double e1 = (EPS + is1) * (EPS + is1);
-    double e2 = (EPS + is2) * (EPS + is2);
-    double e3 = (EPS + is3) * (EPS + is3);
-    double bl1 = (1.0 / 16.0) / e1, bl2 = (10.0 / 16.0) / e2, bl3 = (5.0 / 16.0) / e3;
-    double br1 = (5.0 / 16.0) / e1, br2 = (10.0 / 16.0) / e2, br3 = (1.0 / 16.0) / e3;
-    double sl = bl1 + bl2 + bl3;
-    double sr = br1 + br2 + br3;
-    double wl1 = bl1 / sl, wl2 = bl2 / sl, wl3 = bl3 / sl;
-    double wr1 = br1 / sr, wr2 = br2 / sr, wr3 = br3 / sr;
+    T g1 = 0.5 * (qm2 - 4.0 * qm1 + 3.0 * q0);
+    T g2 = 0.5 * (qp1 - qm1);
+    T g3 = 0.5 * (-3.0 * q0 + 4.0 * qp1 - qp2);
+    T s1 = qm2 - 2.0 * qm1 + q0;
+    T s2 = qm1 - 2.0 * q0 + qp1;
+    T s3 = q0 - 2.0 * qp1 + qp2;
+    T is1 = g1 * g1 + s1 * s1;
+    T is2 = g2 * g2 + s2 * s2;
+    T is3 = g3 * g3 + s3 * s3;
+    T e1 = (EPS + is1) * (EPS + is1);
+    T e2 = (EPS + is2) * (EPS + is2);
+    T e3 = (EPS + is3) * (EPS + is3);
+    T bl1 = (1.0 / 16.0) / e1, bl2 = (10.0 / 16.0) / e2, bl3 = (5.0 / 16.0) / e3;
+    T br1 = (5.0 / 16.0) / e1, br2 = (10.0 / 16.0) / e2, br3 = (1.0 / 16.0) / e3;
+    T sl = bl1 + bl2 + bl3;
+    T sr = br1 + br2 + br3;
+    T wl1 = bl1 / sl, wl2 = bl2 / sl, wl3 = bl3 / sl;
+    T wr1 = br1 / sr, wr2 = br2 / sr, wr3 = br3 / sr;
     qwl = q0 + 0.125 * (wl1 * (s1 + 4.0 * g1) + wl2 * (s2 + 4.0 * g2) + wl3 * (s3 + 4.0 * g3));
     qwr = q0 + 0.125 * (wr1 * (s1 - 4.0 * g1) + wr2 * (s2 - 4.0 * g2) + wr3 * (s3 - 4.0 * g3));
 }

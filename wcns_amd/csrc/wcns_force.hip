@@ -80,18 +80,61 @@ __global__ void k_force_pn(DRegion g, DBlock b, DFields F, DParams P, double *pn
     pn3[2 * npts + t] = fz;
 }
 
-__device__ inline double lenth(const double *a, const double *b)
+// region-local corner offsets of quad (a, c, e), corners 1..4 of force_dif
+// (PostProcess.f90:1577-1613)
+__device__ inline void quad_corners(const DRegion &g, int a, int c, int e, int o[4][3])
 {
-    double dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+    int id = 1 - abs(g.l3[0]), jd = 1 - abs(g.l3[1]), kd = 1 - abs(g.l3[2]);
+    for (int n = 0; n < 4; n++) { o[n][0] = a; o[n][1] = c; o[n][2] = e; }
+    if (id == 0) { o[1][1] += 1; o[2][1] += 1; o[2][2] += 1; o[3][2] += 1; }
+    if (jd == 0) { o[1][0] += 1; o[2][0] += 1; o[2][2] += 1; o[3][2] += 1; }
+    if (kd == 0) { o[1][0] += 1; o[2][0] += 1; o[2][1] += 1; o[3][1] += 1; }
+}
+
+template <class T>
+__device__ inline T lenth(const T *a, const T *b)
+{
+    T dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
     return sqrt(dx * dx + dy * dy + dz * dz);
 }
 
 // Sub1.f90:865-878 halen (Heron; a negative radicand gives zero area)
-__device__ inline double halen(double a, double b, double c)
+template <class T>
+__device__ inline T halen(T a, T b, T c)
 {
-    double p = 0.5 * (a + b + c);
-    double S = p * (p - a) * (p - b) * (p - c);
-    return S < 0 ? 0.0 : sqrt(S);
+    T p = 0.5 * (a + b + c);
+    T S = p * (p - a) * (p - b) * (p - c);
+    return S < 0 ? T(0.0) : sqrt(S);
+}
+
+// getforce (PostProcess.f90:1684-1796): four triangles around the quad's centre,
+// traction at each centroid times the Heron area; d[0..2] force, d[3..5] moment
+template <class T>
+__device__ inline void quad_force(const T (&P)[4][3], const T (&Xc)[4][3], const double rf[3], T d[6])
+{
+    T X[5][3], P5[3];
+    for (int n = 0; n < 4; n++)
+        for (int m = 0; m < 3; m++) X[n][m] = Xc[n][m];
+    for (int m = 0; m < 3; m++) {
+        P5[m] = 0.25 * (P[0][m] + P[1][m] + P[2][m] + P[3][m]);
+        X[4][m] = 0.25 * (X[0][m] + X[1][m] + X[2][m] + X[3][m]);
+    }
+    T r12 = lenth(X[0], X[1]), r23 = lenth(X[1], X[2]), r34 = lenth(X[2], X[3]), r41 = lenth(X[3], X[0]);
+    T r15 = lenth(X[0], X[4]), r25 = lenth(X[1], X[4]), r35 = lenth(X[2], X[4]), r45 = lenth(X[3], X[4]);
+    T s[4] = {halen(r12, r15, r25), halen(r23, r25, r35), halen(r34, r35, r45), halen(r41, r45, r15)};
+    for (int tr = 0; tr < 4; tr++) {
+        int ia = tr, ib = (tr + 1) & 3;
+        T pc[3], xc[3];
+        for (int m = 0; m < 3; m++) {
+            pc[m] = (P[ia][m] + P[ib][m] + P5[m]) / 3.0;
+            xc[m] = (X[ia][m] + X[ib][m] + X[4][m]) / 3.0;
+        }
+        T fx = pc[0] * s[tr], fy = pc[1] * s[tr], fz = pc[2] * s[tr];
+        d[0] = d[0] + fx; d[1] = d[1] + fy; d[2] = d[2] + fz;
+        d[3] = d[3] + ((xc[1] - rf[1]) * fz - (xc[2] - rf[2]) * fy);
+        d[4] = d[4] + ((xc[2] - rf[2]) * fx - (xc[0] - rf[0]) * fz);
+        d[5] = d[5] + ((xc[0] - rf[0]) * fy - (xc[1] - rf[1]) * fx);
+    }
 }
 
 __global__ void __launch_bounds__(FORCE_THREADS)
@@ -106,12 +149,9 @@ k_force_quad(DRegion g, const double *pn3, const double *xyz, int npts, double3 
     double d[6] = {0, 0, 0, 0, 0, 0};
     if (t < nq) {
         int a = (int)(t % qi), c = (int)((t / qi) % qj), e = (int)(t / ((long)qi * qj));
-        // corners 1..4 of force_dif (PostProcess.f90:1577-1613), in region-local offsets
-        int o[4][3] = {{a, c, e}, {a, c, e}, {a, c, e}, {a, c, e}};
-        if (id == 0) { o[1][1] += 1; o[2][1] += 1; o[2][2] += 1; o[3][2] += 1; }
-        if (jd == 0) { o[1][0] += 1; o[2][0] += 1; o[2][2] += 1; o[3][2] += 1; }
-        if (kd == 0) { o[1][0] += 1; o[2][0] += 1; o[2][1] += 1; o[3][1] += 1; }
-        double P[4][3], X[5][3];
+        int o[4][3];
+        quad_corners(g, a, c, e, o);
+        double P[4][3], X[4][3];
         for (int n = 0; n < 4; n++) {
             int pt = o[n][0] + bi * (o[n][1] + bj * o[n][2]);
             for (int m = 0; m < 3; m++) {
@@ -119,28 +159,8 @@ k_force_quad(DRegion g, const double *pn3, const double *xyz, int npts, double3 
                 X[n][m] = xyz[(long)m * npts + pt];
             }
         }
-        double P5[3];
-        for (int m = 0; m < 3; m++) {
-            P5[m] = 0.25 * (P[0][m] + P[1][m] + P[2][m] + P[3][m]);
-            X[4][m] = 0.25 * (X[0][m] + X[1][m] + X[2][m] + X[3][m]);
-        }
-        double r12 = lenth(X[0], X[1]), r23 = lenth(X[1], X[2]), r34 = lenth(X[2], X[3]), r41 = lenth(X[3], X[0]);
-        double r15 = lenth(X[0], X[4]), r25 = lenth(X[1], X[4]), r35 = lenth(X[2], X[4]), r45 = lenth(X[3], X[4]);
-        double s[4] = {halen(r12, r15, r25), halen(r23, r25, r35), halen(r34, r35, r45), halen(r41, r45, r15)};
-        double rf[3] = {ref.x, ref.y, ref.z};
-        for (int tr = 0; tr < 4; tr++) {
-            int ia = tr, ib = (tr + 1) & 3;
-            double pc[3], xc[3];
-            for (int m = 0; m < 3; m++) {
-                pc[m] = (P[ia][m] + P[ib][m] + P5[m]) / 3.0;
-                xc[m] = (X[ia][m] + X[ib][m] + X[4][m]) / 3.0;
-            }
-            double fx = pc[0] * s[tr], fy = pc[1] * s[tr], fz = pc[2] * s[tr];
-            d[0] = d[0] + fx; d[1] = d[1] + fy; d[2] = d[2] + fz;
-            d[3] = d[3] + ((xc[1] - rf[1]) * fz - (xc[2] - rf[2]) * fy);
-            d[4] = d[4] + ((xc[2] - rf[2]) * fx - (xc[0] - rf[0]) * fz);
-            d[5] = d[5] + ((xc[0] - rf[0]) * fy - (xc[1] - rf[1]) * fx);
-        }
+        const double rf[3] = {ref.x, ref.y, ref.z};
+        quad_force(P, X, rf, d);
     }
     for (int m = 0; m < 6; m++) red[m][threadIdx.x] = d[m];
     __syncthreads();
@@ -158,6 +178,132 @@ __global__ void k_force_final(const double *partial, int nparts, double *out)
     double s = 0.0;
     for (int p = 0; p < nparts; p++) s = s + partial[p * 6 + threadIdx.x];
     out[threadIdx.x] = s;
+}
+
+
+// ------------------------------------------- adjoint of the pressure force
+// J = sum_c omega[c] * (region sums)[c] of the pressure traction only
+// (PostProcess.f90:1560-1564); its gradient w.r.t. the traction and the
+// coordinates of each region point gathers, over the up to four quads that
+// hold the point, the quad's derivative along each of the point's six inputs.
+struct Omega6 {
+    double w[6];
+};
+
+__global__ void k_force_quad_vjp(DRegion g, const double *pn3, const double *xyz, int npts, double3 ref, Omega6 om,
+                                 double *lpn3, double *lxyz)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts) return;
+    int bi = g.s_hi[0] - g.s_lo[0] + 1, bj = g.s_hi[1] - g.s_lo[1] + 1, bk = g.s_hi[2] - g.s_lo[2] + 1;
+    int tg[3] = {1 - abs(g.l3[0]), 1 - abs(g.l3[1]), 1 - abs(g.l3[2])};   // 1 along the tangential dims
+    int qn[3] = {bi - tg[0], bj - tg[1], bk - tg[2]};
+    int pt[3] = {t % bi, (t / bi) % bj, t / (bi * bj)};
+    const double rf[3] = {ref.x, ref.y, ref.z};
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int cand = 0; cand < 8; cand++) {
+        int q[3];
+        bool ok = true;
+        for (int d = 0; d < 3; d++) {
+            int off = (cand >> d) & 1;
+            if (off && !tg[d]) { ok = false; break; }
+            q[d] = pt[d] - off;
+            if (q[d] < 0 || q[d] >= qn[d]) { ok = false; break; }
+        }
+        if (!ok) continue;
+        int o[4][3];
+        quad_corners(g, q[0], q[1], q[2], o);
+        int hit = -1;
+        for (int n = 0; n < 4; n++)
+            if (o[n][0] == pt[0] && o[n][1] == pt[1] && o[n][2] == pt[2]) hit = n;
+        if (hit < 0) continue;
+        double P[4][3], X[4][3];
+        for (int n = 0; n < 4; n++) {
+            int ptn = o[n][0] + bi * (o[n][1] + bj * o[n][2]);
+            for (int m = 0; m < 3; m++) {
+                P[n][m] = pn3[(long)m * npts + ptn];
+                X[n][m] = xyz[(long)m * npts + ptn];
+            }
+        }
+        for (int sd = 0; sd < 6; sd++) {
+            D1 PD[4][3], XD[4][3], d6[6];
+            for (int n = 0; n < 4; n++)
+                for (int m = 0; m < 3; m++) {
+                    PD[n][m] = D1(P[n][m], (n == hit && sd == m) ? 1.0 : 0.0);
+                    XD[n][m] = D1(X[n][m], (n == hit && sd == 3 + m) ? 1.0 : 0.0);
+                }
+            for (int c = 0; c < 6; c++) d6[c] = D1(0.0);
+            quad_force(PD, XD, rf, d6);
+            double v = 0.0;
+            for (int c = 0; c < 6; c++) v = v + om.w[c] * d6[c].d;
+            acc[sd] = acc[sd] + v;
+        }
+    }
+    for (int m = 0; m < 3; m++) {
+        lpn3[(long)m * npts + t] = acc[m];
+        lxyz[(long)m * npts + t] = acc[3 + m];
+    }
+}
+
+// pressure traction of a face node: s_lr * n * 2 (p - poo), n the unit normal of
+// the summed metric triples of the face directions (Sub1.f90:750-780)
+template <class T>
+__device__ inline void pn_pressure(T p, T x, T y, T z, int s_lr, double poo, T *out)
+{
+    T s = sqrt(x * x + y * y + z * z);
+    s = s < SML_SSS ? T(SML_SSS) : s;
+    T nx = x / s, ny = y / s, nz = z / s;
+    T pn = 2.0 * (p - poo);
+    out[0] = (double)s_lr * nx * pn;
+    out[1] = (double)s_lr * ny * pn;
+    out[2] = (double)s_lr * nz * pn;
+}
+
+// gradient of the traction w.r.t. p and the metrics of each region point, contracted with lpn3
+__global__ void k_force_pn_vjp(DRegion g, DBlock b, DFields F, DParams P, const double *lpn3, int npts, double *gp,
+                               double *gm)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts) return;
+    int i, j, k;
+    region_point(g, t, i, j, k);
+    long q = gidx(b, i, j, k);
+    const double *tri[3][3] = {{F.kcx, F.kcy, F.kcz}, {F.etx, F.ety, F.etz}, {F.ctx, F.cty, F.ctz}};
+    double x = 0, y = 0, z = 0;
+    for (int d = 0; d < 3; d++)
+        if (g.l3[d]) { x = x + tri[d][0][q]; y = y + tri[d][1][q]; z = z + tri[d][2][q]; }
+    double lam[3] = {lpn3[t], lpn3[npts + t], lpn3[2L * npts + t]};
+    double gsd[4];
+    for (int sd = 0; sd < 4; sd++) {
+        D1 out[3];
+        pn_pressure(D1(F.p[q], sd == 0 ? 1.0 : 0.0), D1(x, sd == 1 ? 1.0 : 0.0), D1(y, sd == 2 ? 1.0 : 0.0),
+                    D1(z, sd == 3 ? 1.0 : 0.0), g.s_lr, P.poo, out);
+        gsd[sd] = lam[0] * out[0].d + lam[1] * out[1].d + lam[2] * out[2].d;
+    }
+    gp[4 * F.total + q] = gp[4 * F.total + q] + gsd[0];
+    for (int d = 0; d < 3; d++)
+        if (g.l3[d])
+            for (int c = 0; c < 3; c++) gm[(3 * d + c) * F.total + q] = gm[(3 * d + c) * F.total + q] + gsd[1 + c];
+}
+
+__device__ inline long xidx_of(const DBlock &b, int i, int j, int k)
+{
+    return (long)(i - 1) + (long)b.ni * ((long)(j - 1) + (long)b.nj * (long)(k - 1));
+}
+
+// face-node coordinates of a region from X (PLOT3D order): back = 0 gathers
+// xyz <- X, back = 1 adds xyz into X (the transpose, for gradients)
+__global__ void k_wall_xyz(DRegion g, DBlock b, double *X, double *xyz, int npts, int back)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts) return;
+    int i, j, k;
+    region_point(g, t, i, j, k);
+    long nn = (long)b.ni * b.nj * b.nk, xi = xidx_of(b, i, j, k);
+    for (int m = 0; m < 3; m++) {
+        if (back) X[m * nn + xi] = X[m * nn + xi] + xyz[(long)m * npts + t];
+        else xyz[(long)m * npts + t] = X[m * nn + xi];
+    }
 }
 
 }  // namespace
@@ -185,6 +331,30 @@ int launch_force_region(const DRegion &g, const DBlock &b, const DFields &F, con
                            make_double3(ref[0], ref[1], ref[2]), partial);
     hipLaunchKernelGGL(k_force_final, dim3(1), dim3(64), 0, s, (const double *)partial, np, out6);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_wall_xyz(const DRegion &g, const DBlock &b, double *X, double *xyz, int back, hipStream_t s)
+{
+    if (g.npts <= 0) return 0;
+    hipLaunchKernelGGL(k_wall_xyz, dim3((g.npts + 255) / 256), dim3(256), 0, s, g, b, X, xyz, g.npts, back);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// the pressure part of the force of one region (pn3 as k_force_pn leaves it) and
+// the gradient of sum_c omega[c] * sums[c] into gp (p), gm (metrics), lxyz (coordinates)
+int launch_force_vjp(const DRegion &g, const DBlock &b, const DFields &F, const DParams &P, const double *pn3,
+                     const double *xyz, const double ref[3], const double omega[6], double *lpn3, double *lxyz,
+                     double *gp, double *gm, hipStream_t s)
+{
+    int npts = g.npts;
+    if (npts <= 0) return 0;
+    Omega6 om;
+    for (int c = 0; c < 6; c++) om.w[c] = omega[c];
+    dim3 grid((npts + 127) / 128), blk(128);
+    hipLaunchKernelGGL(k_force_quad_vjp, grid, blk, 0, s, g, pn3, xyz, npts, make_double3(ref[0], ref[1], ref[2]), om,
+                       lpn3, lxyz);
+    hipLaunchKernelGGL(k_force_pn_vjp, grid, blk, 0, s, g, b, F, P, (const double *)lpn3, npts, gp, gm);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 }  // namespace wcns

@@ -167,9 +167,58 @@ DBlock transposed_view(const DBlock &b);
 // back = 0: dst (transposed) = src; back = 1: dst (normal) += src (transposed)
 int launch_transpose(int back, const BlockGrid &G, const TransList &L, hipStream_t s);
 
+// ------------------------------------------- tangent and adjoint of the residual
+// Work arrays of one block in the field-pool frame (total elements per array).
+// Metric arrays are in pool order kcx, kcy, kcz, etx, ety, etz, ctx, cty, ctz, vol.
+struct AdTan {
+    const double *dp;        // 5: direction of r, u, v, w, p
+    const double *dm;        // 10: direction of the metrics and volume
+    const double *dt;        // direction of T
+    const double *dvisl;     // direction of mu
+    const double *ddn;       // 12: direction of the node derivatives
+    double *dr;              // 5: tangent of the residual (accumulated)
+};
+struct AdAdj {
+    const double *w;         // 5: adjoint weights of the residual
+    double *gp;              // 5: gradient w.r.t. r, u, v, w, p (accumulated)
+    double *gm;              // 10: gradient w.r.t. the metrics and volume
+    double *gt;              // gradient w.r.t. T
+    double *gvisl;           // gradient w.r.t. mu
+    double *gdn;             // 12: gradient w.r.t. the node derivatives
+};
+int launch_rhs_inv_jvp(int dir, const DBlock &b, const DFields &F, const DParams &P, const AdTan &D, hipStream_t s);
+int launch_rhs_inv_vjp(int dir, const DBlock &b, const DFields &F, const DParams &P, const AdAdj &A, hipStream_t s);
+int launch_rhs_vis_jvp(int dir, const DBlock &b, const DFields &F, const DParams &P, const double *dn, const AdTan &D,
+                       hipStream_t s);
+int launch_rhs_vis_vjp(int dir, const DBlock &b, const DFields &F, const DParams &P, const double *dn, const AdAdj &A,
+                       hipStream_t s);
+int launch_dn_jvp(const DBlock &b, const DFields &F, const AdTan &D, double *ddn, hipStream_t s);
+int launch_dn_vjp(const DBlock &b, const DFields &F, const AdAdj &A, hipStream_t s);
+// T and mu of every node from p and rho: tangent (dt, dvisl from dp) and adjoint
+// (gt, gvisl into gp)
+int launch_tmu_jvp(const DBlock &b, const DFields &F, const DParams &P, double *dt, double *dvisl, const double *dp,
+                   hipStream_t s);
+int launch_tmu_vjp(const DBlock &b, const DFields &F, const DParams &P, const double *gt, const double *gvisl,
+                   double *gp, hipStream_t s);
+// compact arrays (nv of ni*nj*nk, i fastest) <-> pool arrays at the interior nodes
+int launch_interior(const DBlock &b, double *pool, long total, double *compact, int nv, int to_pool, hipStream_t s);
+// conservative metrics: tangent dM = (dM/dX) dX and adjoint gX += (dM/dX)^T gM;
+// work: gcl_ad_slots() G3-shaped arrays; X, dX, gX in PLOT3D order (3 arrays)
+int gcl_ad_slots();
+// (F: the primal metrics of the block, already formed from X)
+int launch_gcl_jvp(const DBlock &b, const DFields &F, const double *X, const double *dX, double *dm, double *work,
+                   hipStream_t s);
+int launch_gcl_vjp(const DBlock &b, const DFields &F, const double *X, const double *gm, double *gX, double *work,
+                   hipStream_t s);
+
 // wall force
 long force_quads(const DRegion &g);
 int force_parts(const DRegion &g);
+// adjoint of the pressure force (k_force_pn run with nvis = 0 leaves pn3)
+int launch_wall_xyz(const DRegion &g, const DBlock &b, double *X, double *xyz, int back, hipStream_t s);
+int launch_force_vjp(const DRegion &g, const DBlock &b, const DFields &F, const DParams &P, const double *pn3,
+                     const double *xyz, const double ref[3], const double omega[6], double *lpn3, double *lxyz,
+                     double *gp, double *gm, hipStream_t s);
 int launch_force_region(const DRegion &g, const DBlock &b, const DFields &F, const DParams &P, const double *xyz,
                         double *pn3, double *partial, double *out6, const double ref[3], hipStream_t s);
 

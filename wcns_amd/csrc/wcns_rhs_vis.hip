@@ -25,33 +25,41 @@ struct VLine {
     __device__ long at(int l) const { return base + (long)l * stride; }
 };
 
+// Node values along a pencil come from a source q(l), l the pencil index: an
+// array (ArrSrc), a unit vector (UnitSrc: probing a linear formula for its
+// coefficient of node `at`, which is how the adjoint transposes the linear
+// stencils), or an IO policy's variable (IoSrc).
+struct ArrSrc {
+    const double *a;
+    VLine L;
+    __device__ double operator()(int l) const { return a[L.at(l)]; }
+};
+struct UnitSrc {
+    int at;
+    __device__ double operator()(int l) const { return l == at ? 1.0 : 0.0; }
+};
+
 // VALUE_HALF_NODE (WCNSE5.f90:799-852), n > 4, of one node array at face f
-__device__ inline double vhn(const double *a, const VLine &L, int f)
+template <class S>
+__device__ inline auto vhn(const S &q, int n, int f) -> decltype(q(0))
 {
-    int n = L.n;
-    if (f >= 2 && f <= n - 2)
-        return (9.0 * (a[L.at(f)] + a[L.at(f + 1)]) + -1.0 * (a[L.at(f + 2)] + a[L.at(f - 1)])) / 16.0;
-    if (f == 1) return (5.0 * a[L.at(1)] + 15.0 * a[L.at(2)] + -5.0 * a[L.at(3)] + 1.0 * a[L.at(4)]) / 16.0;
-    if (f == n - 1)
-        return (5.0 * a[L.at(n)] + 15.0 * a[L.at(n - 1)] + -5.0 * a[L.at(n - 2)] + 1.0 * a[L.at(n - 3)]) / 16.0;
-    if (f == 0) return (35.0 * a[L.at(1)] - 35.0 * a[L.at(2)] + 21.0 * a[L.at(3)] - 5.0 * a[L.at(4)]) / 16.0;
-    return (35.0 * a[L.at(n)] - 35.0 * a[L.at(n - 1)] + 21.0 * a[L.at(n - 2)] - 5.0 * a[L.at(n - 3)]) / 16.0;
+    if (f >= 2 && f <= n - 2) return (9.0 * (q(f) + q(f + 1)) + -1.0 * (q(f + 2) + q(f - 1))) / 16.0;
+    if (f == 1) return (5.0 * q(1) + 15.0 * q(2) + -5.0 * q(3) + 1.0 * q(4)) / 16.0;
+    if (f == n - 1) return (5.0 * q(n) + 15.0 * q(n - 1) + -5.0 * q(n - 2) + 1.0 * q(n - 3)) / 16.0;
+    if (f == 0) return (35.0 * q(1) - 35.0 * q(2) + 21.0 * q(3) - 5.0 * q(4)) / 16.0;
+    return (35.0 * q(n) - 35.0 * q(n - 1) + 21.0 * q(n - 2) - 5.0 * q(n - 3)) / 16.0;
 }
 
 // node derivative along one direction at row l of a pencil (DUVWT_NODE_LINE_virtual)
-__device__ inline double dnode(const double *q, const VLine &L, int l, bool fl, bool fr)
+template <class S>
+__device__ inline auto dnode(const S &q, int n, int l, bool fl, bool fr) -> decltype(q(0))
 {
-    int n = L.n;
-    if (fl && l == 1)
-        return (-11. * q[L.at(1)] + 18. * q[L.at(2)] - 9. * q[L.at(3)] + 2. * q[L.at(4)]) / 6.0;
-    if (fl && l == 2)
-        return (-3.0 * q[L.at(1)] + -10.0 * q[L.at(2)] + 18.0 * q[L.at(3)] + -6.0 * q[L.at(4)] + 1.0 * q[L.at(5)]) / 12.0;
-    if (fr && l == n)
-        return -(-11. * q[L.at(n)] + 18. * q[L.at(n - 1)] - 9.0 * q[L.at(n - 2)] + 2. * q[L.at(n - 3)]) / 6.0;
+    if (fl && l == 1) return (-11. * q(1) + 18. * q(2) - 9. * q(3) + 2. * q(4)) / 6.0;
+    if (fl && l == 2) return (-3.0 * q(1) + -10.0 * q(2) + 18.0 * q(3) + -6.0 * q(4) + 1.0 * q(5)) / 12.0;
+    if (fr && l == n) return -(-11. * q(n) + 18. * q(n - 1) - 9.0 * q(n - 2) + 2. * q(n - 3)) / 6.0;
     if (fr && l == n - 1)
-        return -(-3.0 * q[L.at(n)] + -10.0 * q[L.at(n - 1)] + 18.0 * q[L.at(n - 2)] + -6.0 * q[L.at(n - 3)] +
-                 1.0 * q[L.at(n - 4)]) / 12.0;
-    return (8.0 * (q[L.at(l + 1)] - q[L.at(l - 1)]) + -1.0 * (q[L.at(l + 2)] - q[L.at(l - 2)])) / 12.0;
+        return -(-3.0 * q(n) + -10.0 * q(n - 1) + 18.0 * q(n - 2) + -6.0 * q(n - 3) + 1.0 * q(n - 4)) / 12.0;
+    return (8.0 * (q(l + 1) - q(l - 1)) + -1.0 * (q(l + 2) - q(l - 2))) / 12.0;
 }
 
 // flags of a pencil: uvwt(4,-2) < 0 after the density override (sub_tgh.f90:1923-1935)
@@ -71,7 +79,22 @@ __device__ inline VLine line_of(const DBlock &b, int dir, int i, int j, int k)
     return L;
 }
 
-// dn layout: component c (0..11 = 4*dir + {u,v,w,T}) of node g at dn[c*total + g]
+// dn layout: component c (0..11 = 4*dir + {u,v,w,T}) of node g at dn[c*total + g];
+// the derivatives of the four arrays q0..q3 (u, v, w, T for the solver, their
+// directions for the tangent), with the pencil flags of the state in F
+__device__ inline void node_deriv(const DBlock &b, const DFields &F, const double *const *q, double *dn, long t)
+{
+    int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
+    long g = gidx(b, i, j, k);
+    for (int dir = 0; dir < 3; dir++) {
+        VLine L = line_of(b, dir, i, j, k);
+        bool fl, fr;
+        pencil_flags(F, L, fl, fr);
+        int l = dir == 0 ? i : (dir == 1 ? j : k);
+        for (int m = 0; m < 4; m++) dn[(long)(4 * dir + m) * F.total + g] = dnode(ArrSrc{q[m], L}, L.n, l, fl, fr);
+    }
+}
+
 __global__ void k_vis_node_deriv(BlockGrid G, DFields F, double *dn)
 {
     DBlock b;
@@ -79,76 +102,152 @@ __global__ void k_vis_node_deriv(BlockGrid G, DFields F, double *dn)
     grid_thread(G, b, t);
     long n = (long)b.ni * b.nj * b.nk;
     if (t >= n) return;
-    int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
-    long g = gidx(b, i, j, k);
     const double *q[4] = {F.u, F.v, F.w, F.t};
-    for (int dir = 0; dir < 3; dir++) {
-        VLine L = line_of(b, dir, i, j, k);
-        bool fl, fr;
-        pencil_flags(F, L, fl, fr);
-        int l = dir == 0 ? i : (dir == 1 ? j : k);
-        for (int m = 0; m < 4; m++) dn[(long)(4 * dir + m) * F.total + g] = dnode(q[m], L, l, fl, fr);
-    }
+    node_deriv(b, F, q, dn, t);
 }
 
 // u,v,w,T at half node f (VALUE_line_half_virtual, sub_tgh.f90:2335-2387)
-__device__ inline double value_line_half(const double *q, const VLine &L, int f, bool fl, bool fr)
+template <class S>
+__device__ inline auto value_line_half(const S &q, int n, int f, bool fl, bool fr) -> decltype(q(0))
 {
-    int n = L.n;
-    if (fl && f == 0) return (35.0 * q[L.at(1)] + -35.0 * q[L.at(2)] + 21.0 * q[L.at(3)] + -5.0 * q[L.at(4)]) / 16.0;
-    if (fl && f == 1) return (5.0 * q[L.at(1)] + 15.0 * q[L.at(2)] + -5.0 * q[L.at(3)] + 1.0 * q[L.at(4)]) / 16.0;
-    if (fr && f == n)
-        return (35.0 * q[L.at(n)] + -35.0 * q[L.at(n - 1)] + 21.0 * q[L.at(n - 2)] + -5.0 * q[L.at(n - 3)]) / 16.0;
-    if (fr && f == n - 1)
-        return (5.0 * q[L.at(n)] + 15.0 * q[L.at(n - 1)] + -5.0 * q[L.at(n - 2)] + 1.0 * q[L.at(n - 3)]) / 16.0;
-    return (9.0 * (q[L.at(f)] + q[L.at(f + 1)]) + -1.0 * (q[L.at(f + 2)] + q[L.at(f - 1)])) / 16.0;
+    if (fl && f == 0) return (35.0 * q(1) + -35.0 * q(2) + 21.0 * q(3) + -5.0 * q(4)) / 16.0;
+    if (fl && f == 1) return (5.0 * q(1) + 15.0 * q(2) + -5.0 * q(3) + 1.0 * q(4)) / 16.0;
+    if (fr && f == n) return (35.0 * q(n) + -35.0 * q(n - 1) + 21.0 * q(n - 2) + -5.0 * q(n - 3)) / 16.0;
+    if (fr && f == n - 1) return (5.0 * q(n) + 15.0 * q(n - 1) + -5.0 * q(n - 2) + 1.0 * q(n - 3)) / 16.0;
+    return (9.0 * (q(f) + q(f + 1)) + -1.0 * (q(f + 2) + q(f - 1))) / 16.0;
 }
 
 // direct half-node derivative (DUVWT_half_line_virtual, sub_tgh.f90:2277-2330)
-__device__ inline double dhalf(const double *q, const VLine &L, int f, bool fl, bool fr)
+template <class S>
+__device__ inline auto dhalf(const S &q, int n, int f, bool fl, bool fr) -> decltype(q(0))
 {
-    int n = L.n;
-    if (fl && f == 0)
-        return (-71.0 * q[L.at(1)] + 141.0 * q[L.at(2)] + -93.0 * q[L.at(3)] + 23.0 * q[L.at(4)]) / 24.0;
-    if (fl && f == 1)
-        return (-22.0 * q[L.at(1)] + 17.0 * q[L.at(2)] + 9.0 * q[L.at(3)] + -5.0 * q[L.at(4)] + 1.0 * q[L.at(5)]) / 24.0;
-    if (fr && f == n)
-        return -(-71.0 * q[L.at(n)] + 141.0 * q[L.at(n - 1)] + -93.0 * q[L.at(n - 2)] + 23.0 * q[L.at(n - 3)]) / 24.0;
+    if (fl && f == 0) return (-71.0 * q(1) + 141.0 * q(2) + -93.0 * q(3) + 23.0 * q(4)) / 24.0;
+    if (fl && f == 1) return (-22.0 * q(1) + 17.0 * q(2) + 9.0 * q(3) + -5.0 * q(4) + 1.0 * q(5)) / 24.0;
+    if (fr && f == n) return -(-71.0 * q(n) + 141.0 * q(n - 1) + -93.0 * q(n - 2) + 23.0 * q(n - 3)) / 24.0;
     if (fr && f == n - 1)
-        return -(-22.0 * q[L.at(n)] + 17.0 * q[L.at(n - 1)] + 9.0 * q[L.at(n - 2)] + -5.0 * q[L.at(n - 3)] +
-                 1.0 * q[L.at(n - 4)]) / 24.0;
-    return (27.0 * (q[L.at(f + 1)] - q[L.at(f)]) + -1.0 * (q[L.at(f + 2)] - q[L.at(f - 1)])) / 24.0;
+        return -(-22.0 * q(n) + 17.0 * q(n - 1) + 9.0 * q(n - 2) + -5.0 * q(n - 3) + 1.0 * q(n - 4)) / 24.0;
+    return (27.0 * (q(f + 1) - q(f)) + -1.0 * (q(f + 2) - q(f - 1))) / 24.0;
 }
 
-template <int DIR>
-__device__ void face_visc(const DFields &F, const DParams &P, const double *dn, const VLine &L, int f, bool fl,
-                          bool fr, double cp_prl, double cp_prt, double *fv)
+// Variables a viscous face reads, as an IO policy numbers them.  Metrics in
+// kxyz_line order (kcx, etx, ctx, kcy, ety, cty, kcz, etz, ctz), sub_tgh.f90:1571-1581.
+enum { VU = 0, VT = 3, VDN = 4, VM = 16, VVOL = 25, VVISL = 26, VVIST = 27 };
+// pool index (kcx, kcy, kcz, etx, ..., ctz) of kxyz_line entry a
+__host__ __device__ constexpr int kxyz_pool(int a) { return (a % 3) * 3 + a / 3; }
+
+// The solver's IO: double loads, rows subtracted from R as re * dFv
+struct VisIO {
+    using T = double;
+    const double *q4[4];          // u, v, w, T
+    const double *dn;
+    const double *ms[9];
+    const double *vol, *visl, *vist;
+    double *rhs;
+    long total;
+    double re;
+    __device__ double val(int var, long g, int) const
+    {
+        if (var < 4) return q4[var][g];
+        if (var < VM) return dn[(long)(var - VDN) * total + g];
+        if (var < VVOL) return ms[var - VM][g];
+        if (var == VVOL) return vol[g];
+        if (var == VVISL) return visl[g];
+        return vist[g];
+    }
+    __device__ void row(long g, int, const double *df) const
+    {
+#pragma unroll
+        for (int m = 0; m < 5; m++) rhs[m * total + g] = rhs[m * total + g] - re * df[m];
+    }
+};
+
+// Tangent IO: D1 loads of (state, direction); mu_t is held fixed
+struct VisTanIO {
+    using T = D1;
+    VisIO s;
+    const double *d4[4];          // du, dv, dw, dT
+    const double *ddn, *dm, *dvisl;   // dm: metrics in pool order, vol at 9
+    double *drhs;
+    __device__ D1 val(int var, long g, int l) const
+    {
+        double x = s.val(var, g, l), dx;
+        if (var < 4) dx = d4[var][g];
+        else if (var < VM) dx = ddn[(long)(var - VDN) * s.total + g];
+        else if (var < VVOL) dx = dm[(long)kxyz_pool(var - VM) * s.total + g];
+        else if (var == VVOL) dx = dm[9 * s.total + g];
+        else if (var == VVISL) dx = dvisl[g];
+        else dx = 0.0;
+        return D1(x, dx);
+    }
+    __device__ void row(long g, int, const D1 *df) const
+    {
+#pragma unroll
+        for (int m = 0; m < 5; m++) drhs[m * s.total + g] = drhs[m * s.total + g] - s.re * df[m].d;
+    }
+};
+
+// Adjoint IO: variable `var` seeded on the nodes of one colour
+struct VisVjpIO {
+    using T = D1;
+    VisIO s;
+    const double *wt;
+    double *lam;
+    long stride;
+    int n, color, var;
+    __device__ D1 val(int v, long g, int l) const
+    {
+        return D1(s.val(v, g, l), (v == var && ad_mod(l) == color) ? 1.0 : 0.0);
+    }
+    __device__ void row(long g, int l, const D1 *df) const
+    {
+        const int lj = ad_node(l, color);
+        if (lj < -2 || lj > n + 3) return;
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 0; m < 5; m++) acc = acc + wt[m * s.total + g] * df[m].d;
+        const long gj = g + (long)(lj - l) * stride;
+        lam[gj] = lam[gj] - s.re * acc;
+    }
+};
+
+template <class IO>
+struct IoSrc {
+    const IO *io;
+    int var;
+    VLine L;
+    __device__ typename IO::T operator()(int l) const { return io->val(var, L.at(l), l); }
+};
+
+template <int DIR, class IO>
+__device__ void face_visc(const IO &io, const VLine &L, int f, bool fl, bool fr, bool turb, double cp_prl,
+                          double cp_prt, typename IO::T *fv)
 {
-    const double *q4[4] = {F.u, F.v, F.w, F.t};
-    double uvwt[4], du[12], kx[9];
+    using T = typename IO::T;
+    const int n = L.n;
+    auto src = [&](int var) { return IoSrc<IO>{&io, var, L}; };
+    T uvwt[4], du[12], kx[9];
 #pragma unroll
-    for (int m = 0; m < 4; m++) uvwt[m] = value_line_half(q4[m], L, f, fl, fr);
+    for (int m = 0; m < 4; m++) uvwt[m] = value_line_half(src(VU + m), n, f, fl, fr);
 #pragma unroll
-    for (int m = 0; m < 4; m++) du[4 * DIR + m] = dhalf(q4[m], L, f, fl, fr);
+    for (int m = 0; m < 4; m++) du[4 * DIR + m] = dhalf(src(VU + m), n, f, fl, fr);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         if (d == DIR) continue;
 #pragma unroll
-        for (int m = 0; m < 4; m++) du[4 * d + m] = vhn(dn + (long)(4 * d + m) * F.total, L, f);
+        for (int m = 0; m < 4; m++) du[4 * d + m] = vhn(src(VDN + 4 * d + m), n, f);
     }
-    // kxyz_line order: (kcx, etx, ctx, kcy, ety, cty, kcz, etz, ctz), sub_tgh.f90:1571-1581
-    const double *ms[9] = {F.kcx, F.etx, F.ctx, F.kcy, F.ety, F.cty, F.kcz, F.etz, F.ctz};
 #pragma unroll
-    for (int a = 0; a < 9; a++) kx[a] = vhn(ms[a], L, f);
-    double vol = vhn(F.vol, L, f);
+    for (int a = 0; a < 9; a++) kx[a] = vhn(src(VM + a), n, f);
+    T vol = vhn(src(VVOL), n, f);
     // viscosities (sub_tgh.f90:1562-1566): vslt1 = visl + vist, vslt2 = visl*cp/prl + vist*cp/prt;
     // laminar: vist = 0
-    double s1, s2;
+    T s1, s2;
     {
-        int n = L.n;
-        const bool turb = P.turb != 0;
-        auto V1 = [&](int l) { return F.visl[L.at(l)] + (turb ? F.vist[L.at(l)] : 0.0); };
-        auto V2 = [&](int l) { return F.visl[L.at(l)] * cp_prl + (turb ? F.vist[L.at(l)] * cp_prt : 0.0); };
+        auto V1 = [&](int l) { long g = L.at(l); return io.val(VVISL, g, l) + (turb ? io.val(VVIST, g, l) : T(0.0)); };
+        auto V2 = [&](int l) {
+            long g = L.at(l);
+            return io.val(VVISL, g, l) * cp_prl + (turb ? io.val(VVIST, g, l) * cp_prt : T(0.0));
+        };
         if (f >= 2 && f <= n - 2) {
             s1 = (9.0 * (V1(f) + V1(f + 1)) + -1.0 * (V1(f + 2) + V1(f - 1))) / 16.0;
             s2 = (9.0 * (V2(f) + V2(f + 1)) + -1.0 * (V2(f + 2) + V2(f - 1))) / 16.0;
@@ -167,7 +266,7 @@ __device__ void face_visc(const DFields &F, const DParams &P, const double *dn, 
         }
     }
     // DUVWT_DXYZ (WCNSE5.f90:854-890)
-    double d[12];
+    T d[12];
 #pragma unroll
     for (int m = 1; m <= 3; m++) {
         int m1 = m + m + m - 2, m2 = m1 + 1, m3 = m2 + 1;
@@ -177,22 +276,93 @@ __device__ void face_visc(const DFields &F, const DParams &P, const double *dn, 
         d[m + 8] = (kx[m1 - 1] * du[3] + kx[m2 - 1] * du[7] + kx[m3 - 1] * du[11]) / vol;
     }
     // FLUX_VIS_LINE_new (newvis.f90:283-349); the face normal is the pencil direction's metric
-    double vs = s1, kcp = s2;
-    double vscc = vs * (2.0 / 3.0);
-    double dudx = d[0], dudy = d[1], dudz = d[2], dvdx = d[3], dvdy = d[4], dvdz = d[5];
-    double dwdx = d[6], dwdy = d[7], dwdz = d[8];
-    double t1 = vscc * (2.0 * dudx - dvdy - dwdz);
-    double t5 = vscc * (2.0 * dvdy - dwdz - dudx);
-    double t9 = vscc * (2.0 * dwdz - dudx - dvdy);
-    double t2 = vs * (dudy + dvdx);
-    double t3 = vs * (dudz + dwdx);
-    double t6 = vs * (dvdz + dwdy);
-    double nx = kx[DIR], ny = kx[3 + DIR], nz = kx[6 + DIR];
+    T vs = s1, kcp = s2;
+    T vscc = vs * (2.0 / 3.0);
+    T dudx = d[0], dudy = d[1], dudz = d[2], dvdx = d[3], dvdy = d[4], dvdz = d[5];
+    T dwdx = d[6], dwdy = d[7], dwdz = d[8];
+    T t1 = vscc * (2.0 * dudx - dvdy - dwdz);
+    T t5 = vscc * (2.0 * dvdy - dwdz - dudx);
+    T t9 = vscc * (2.0 * dwdz - dudx - dvdy);
+    T t2 = vs * (dudy + dvdx);
+    T t3 = vs * (dudz + dwdx);
+    T t6 = vs * (dvdz + dwdy);
+    T nx = kx[DIR], ny = kx[3 + DIR], nz = kx[6 + DIR];
     fv[0] = 0.0;
     fv[1] = t1 * nx + t2 * ny + t3 * nz;
     fv[2] = t2 * nx + t5 * ny + t6 * nz;
     fv[3] = t3 * nx + t6 * ny + t9 * nz;
     fv[4] = uvwt[0] * fv[1] + uvwt[1] * fv[2] + uvwt[2] * fv[3] + kcp * (d[9] * nx + d[10] * ny + d[11] * nz);
+}
+
+__device__ inline VisIO vis_io(const DFields &F, const DParams &P, const double *dn)
+{
+    VisIO io;
+    io.q4[0] = F.u; io.q4[1] = F.v; io.q4[2] = F.w; io.q4[3] = F.t;
+    io.dn = dn;
+    io.ms[0] = F.kcx; io.ms[1] = F.etx; io.ms[2] = F.ctx;
+    io.ms[3] = F.kcy; io.ms[4] = F.ety; io.ms[5] = F.cty;
+    io.ms[6] = F.kcz; io.ms[7] = F.etz; io.ms[8] = F.ctz;
+    io.vol = F.vol; io.visl = F.visl; io.vist = F.vist;
+    io.rhs = F.rhs;
+    io.total = F.total;
+    io.re = P.re;
+    return io;
+}
+
+// one pencil's rows a..bb: six fluxes in registers give the FLUX_DXYZ rows
+// (WCNSE5.f90:229-251), handed to the IO policy
+template <int DIR, class IO>
+__device__ void vis_pencil(const IO &io, const VLine &L, bool fl, bool fr, int a, int bb, bool turb, double cp_prl,
+                           double cp_prt)
+{
+    using T = typename IO::T;
+    const int n = L.n;
+    int F0 = a - 3 < 0 ? 0 : a - 3;
+    int F1 = bb + 2 > n ? n : bb + 2;
+    if (a <= 2 && F1 < 3) F1 = 3;
+    if (bb >= n - 1 && F0 > n - 3) F0 = n - 3;
+
+    T fw[6][5];
+    for (int fc = F0; fc <= F1; fc++) {
+#pragma unroll
+        for (int l = 0; l < 5; l++)
+#pragma unroll
+            for (int m = 0; m < 5; m++) fw[l][m] = fw[l + 1][m];
+        face_visc<DIR>(io, L, fc, fl, fr, turb, cp_prl, cp_prt, fw[5]);
+        T df[5];
+        if (fc == 3) {
+            if (a <= 1) {
+#pragma unroll
+                for (int m = 0; m < 5; m++) df[m] = (-23.0 * fw[2][m] + 21.0 * fw[3][m] + 3.0 * fw[4][m] - fw[5][m]) / 24.0;
+                io.row(L.at(1), 1, df);
+            }
+            if (a <= 2 && bb >= 2) {
+#pragma unroll
+                for (int m = 0; m < 5; m++) df[m] = (fw[2][m] - 27.0 * fw[3][m] + 27.0 * fw[4][m] - fw[5][m]) / 24.0;
+                io.row(L.at(2), 2, df);
+            }
+        }
+        int row = fc - 2;
+        if (row >= 3 && row <= n - 2 && row >= a && row <= bb) {
+#pragma unroll
+            for (int m = 0; m < 5; m++)
+                df[m] = (2250.0 * (fw[3][m] - fw[2][m]) - 125.0 * (fw[4][m] - fw[1][m]) + 9.0 * (fw[5][m] - fw[0][m])) /
+                        1920.0;
+            io.row(L.at(row), row, df);
+        }
+        if (fc == n) {
+            if (bb >= n - 1 && a <= n - 1) {
+#pragma unroll
+                for (int m = 0; m < 5; m++) df[m] = -(fw[5][m] - 27.0 * fw[4][m] + 27.0 * fw[3][m] - fw[2][m]) / 24.0;
+                io.row(L.at(n - 1), n - 1, df);
+            }
+            if (bb >= n) {
+#pragma unroll
+                for (int m = 0; m < 5; m++) df[m] = -(-23.0 * fw[5][m] + 21.0 * fw[4][m] + 3.0 * fw[3][m] - fw[2][m]) / 24.0;
+                io.row(L.at(n), n, df);
+            }
+        }
+    }
 }
 
 // DIR0 = 3: the i direction run as j pencils of the transposed view (the same
@@ -217,70 +387,11 @@ __global__ void __launch_bounds__(128) k_rhs_vis(BlockGrid G, DFields F, DParams
     VLine L = line_of(b, DIR, i0, j0, k0);
     bool fl, fr;
     pencil_flags(F, L, fl, fr);
-    const double re = P.re;
     const double cp = 1.0 / ((P.gama - 1.0) * P.moo * P.moo);
-    const double cp_prl = cp / P.prl, cp_prt = cp / P.prt;
-
     int a = seg * SEG + 1;
     int bb = a + SEG - 1;
     if (bb > n) bb = n;
-    int F0 = a - 3 < 0 ? 0 : a - 3;
-    int F1 = bb + 2 > n ? n : bb + 2;
-    if (a <= 2 && F1 < 3) F1 = 3;
-    if (bb >= n - 1 && F0 > n - 3) F0 = n - 3;
-
-    double fw[6][5];
-    for (int fc = F0; fc <= F1; fc++) {
-#pragma unroll
-        for (int l = 0; l < 5; l++)
-#pragma unroll
-            for (int m = 0; m < 5; m++) fw[l][m] = fw[l + 1][m];
-        face_visc<DIR>(F, P, dn, L, fc, fl, fr, cp_prl, cp_prt, fw[5]);
-        // FLUX_DXYZ rows (WCNSE5.f90:229-251); R -= re * dfv
-        double df[5];
-        if (fc == 3) {
-            if (a <= 1) {
-#pragma unroll
-                for (int m = 0; m < 5; m++) df[m] = (-23.0 * fw[2][m] + 21.0 * fw[3][m] + 3.0 * fw[4][m] - fw[5][m]) / 24.0;
-                long g = L.at(1);
-#pragma unroll
-                for (int m = 0; m < 5; m++) F.rhs[m * F.total + g] = F.rhs[m * F.total + g] - re * df[m];
-            }
-            if (a <= 2 && bb >= 2) {
-#pragma unroll
-                for (int m = 0; m < 5; m++) df[m] = (fw[2][m] - 27.0 * fw[3][m] + 27.0 * fw[4][m] - fw[5][m]) / 24.0;
-                long g = L.at(2);
-#pragma unroll
-                for (int m = 0; m < 5; m++) F.rhs[m * F.total + g] = F.rhs[m * F.total + g] - re * df[m];
-            }
-        }
-        int row = fc - 2;
-        if (row >= 3 && row <= n - 2 && row >= a && row <= bb) {
-#pragma unroll
-            for (int m = 0; m < 5; m++)
-                df[m] = (2250.0 * (fw[3][m] - fw[2][m]) - 125.0 * (fw[4][m] - fw[1][m]) + 9.0 * (fw[5][m] - fw[0][m])) /
-                        1920.0;
-            long g = L.at(row);
-#pragma unroll
-            for (int m = 0; m < 5; m++) F.rhs[m * F.total + g] = F.rhs[m * F.total + g] - re * df[m];
-        }
-        if (fc == n) {
-            if (bb >= n - 1 && a <= n - 1) {
-#pragma unroll
-                for (int m = 0; m < 5; m++) df[m] = -(fw[5][m] - 27.0 * fw[4][m] + 27.0 * fw[3][m] - fw[2][m]) / 24.0;
-                long g = L.at(n - 1);
-#pragma unroll
-                for (int m = 0; m < 5; m++) F.rhs[m * F.total + g] = F.rhs[m * F.total + g] - re * df[m];
-            }
-            if (bb >= n) {
-#pragma unroll
-                for (int m = 0; m < 5; m++) df[m] = -(-23.0 * fw[5][m] + 21.0 * fw[4][m] + 3.0 * fw[3][m] - fw[2][m]) / 24.0;
-                long g = L.at(n);
-#pragma unroll
-                for (int m = 0; m < 5; m++) F.rhs[m * F.total + g] = F.rhs[m * F.total + g] - re * df[m];
-            }
-        }
-    }
+    vis_pencil<DIR>(vis_io(F, P, dn), L, fl, fr, a, bb, P.turb != 0, cp / P.prl, cp / P.prt);
 }
 
 // The same viscous residual split in two passes (used for the j, k and, on the
@@ -310,9 +421,8 @@ __global__ void __launch_bounds__(256) k_vis_face(BlockGrid G, DFields F, DParam
     bool fl, fr;
     pencil_flags(F, L, fl, fr);
     const double cp = 1.0 / ((P.gama - 1.0) * P.moo * P.moo);
-    const double cp_prl = cp / P.prl, cp_prt = cp / P.prt;
     double fv[5];
-    face_visc<DIR>(F, P, dn, L, f, fl, fr, cp_prl, cp_prt, fv);
+    face_visc<DIR>(vis_io(F, P, dn), L, f, fl, fr, P.turb != 0, cp / P.prl, cp / P.prt, fv);
     const long g = L.at(f);
 #pragma unroll
     for (int m = 0; m < 5; m++) fvb[m * F.total + g] = fv[m];
@@ -350,6 +460,136 @@ __global__ void __launch_bounds__(256) k_vis_rows(BlockGrid G, DFields F, DParam
                   9.0 * (fa(m, l + 2) - fa(m, l - 3))) / 1920.0;
         F.rhs[m * F.total + g] = F.rhs[m * F.total + g] - re * df;
     }
+}
+
+// ------------------------------------------------ tangent and adjoint (one block)
+
+// dR -= re * d(dFv/dxi) along DIR; the pencil segments of the solver
+template <int DIR>
+__global__ void __launch_bounds__(128) k_rhs_vis_jvp(DBlock b, DFields F, DParams P, const double *dn, AdTan D)
+{
+    const int n = DIR == 0 ? b.ni : (DIR == 1 ? b.nj : b.nk);
+    const int na = DIR == 0 ? b.nj : b.ni;
+    const int nbk = DIR == 2 ? b.nj : b.nk;
+    const long npen = (long)na * nbk;
+    const int nseg = (n + SEG - 1) / SEG;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npen * nseg) return;
+    int seg = (int)(t / npen);
+    long pen = t - (long)seg * npen;
+    int ia = (int)(pen % na) + 1, ib = (int)(pen / na) + 1;
+    int i0 = DIR == 0 ? 0 : ia, j0 = DIR == 1 ? 0 : (DIR == 0 ? ia : ib), k0 = DIR == 2 ? 0 : ib;
+    VLine L = line_of(b, DIR, i0, j0, k0);
+    bool fl, fr;
+    pencil_flags(F, L, fl, fr);
+    VisTanIO io;
+    io.s = vis_io(F, P, dn);
+    const long T5 = F.total;
+    for (int m = 0; m < 3; m++) io.d4[m] = D.dp + (1 + m) * T5;
+    io.d4[3] = D.dt;
+    io.ddn = D.ddn;
+    io.dm = D.dm;
+    io.dvisl = D.dvisl;
+    io.drhs = D.dr;
+    const double cp = 1.0 / ((P.gama - 1.0) * P.moo * P.moo);
+    int a = seg * SEG + 1;
+    int bb = a + SEG - 1;
+    if (bb > n) bb = n;
+    vis_pencil<DIR>(io, L, fl, fr, a, bb, P.turb != 0, cp / P.prl, cp / P.prt);
+}
+
+// seeds of the viscous adjoint along DIR: u, v, w, T, the eight cross-direction
+// node derivatives, nine metrics, the volume and mu
+constexpr int VIS_SEEDS = 23;
+template <int DIR>
+__device__ inline int vis_seed_var(int k)
+{
+    if (k < 4) return VU + k;
+    if (k < 12) {
+        int c = k - 4;                           // 0..7 over the two other directions
+        int dd = c / 4;                          // 0: first other direction, 1: second
+        int dir = DIR == 0 ? 1 + dd : (DIR == 1 ? 2 * dd : dd);
+        return VDN + 4 * dir + c % 4;
+    }
+    if (k < 21) return VM + (k - 12);
+    return k == 21 ? VVOL : VVISL;
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(128) k_rhs_vis_vjp(DBlock b, DFields F, DParams P, const double *dn, AdAdj A)
+{
+    const int na = DIR == 0 ? b.nj : b.ni;
+    const int nbk = DIR == 2 ? b.nj : b.nk;
+    const long npen = (long)na * nbk;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npen * NCOLOR * VIS_SEEDS) return;
+    const long pen = t % npen;
+    const int cs = (int)(t / npen);
+    const int color = cs % NCOLOR, k = cs / NCOLOR;
+    int ia = (int)(pen % na) + 1, ib = (int)(pen / na) + 1;
+    int i0 = DIR == 0 ? 0 : ia, j0 = DIR == 1 ? 0 : (DIR == 0 ? ia : ib), k0 = DIR == 2 ? 0 : ib;
+    VLine L = line_of(b, DIR, i0, j0, k0);
+    bool fl, fr;
+    pencil_flags(F, L, fl, fr);
+    VisVjpIO io;
+    io.s = vis_io(F, P, dn);
+    io.wt = A.w;
+    const int var = vis_seed_var<DIR>(k);
+    const long T5 = F.total;
+    if (var < VT) io.lam = A.gp + (1 + var) * T5;
+    else if (var == VT) io.lam = A.gt;
+    else if (var < VM) io.lam = A.gdn + (long)(var - VDN) * T5;
+    else if (var < VVOL) io.lam = A.gm + (long)kxyz_pool(var - VM) * T5;
+    else if (var == VVOL) io.lam = A.gm + 9 * T5;
+    else io.lam = A.gvisl;
+    io.stride = L.stride;
+    io.n = L.n;
+    io.color = color;
+    io.var = var;
+    const double cp = 1.0 / ((P.gama - 1.0) * P.moo * P.moo);
+    vis_pencil<DIR>(io, L, fl, fr, 1, L.n, P.turb != 0, cp / P.prl, cp / P.prt);
+}
+
+// tangent of the node derivatives: the same stencils on du, dv, dw, dT
+__global__ void k_dn_jvp(DBlock b, DFields F, AdTan D, double *ddn)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)b.ni * b.nj * b.nk) return;
+    const double *q[4] = {D.dp + F.total, D.dp + 2 * F.total, D.dp + 3 * F.total, D.dt};
+    node_deriv(b, F, q, ddn, t);
+}
+
+// adjoint of the node derivatives: every node of the ghosted block gathers, for
+// each direction along which its transverse indices are interior, the rows
+// l' = l-4..l+4 whose stencil holds it (the coefficient found by probing the
+// stencil with a unit vector)
+__global__ void k_dn_vjp(DBlock b, DFields F, AdAdj A)
+{
+    const long e = (long)(b.ni + 6) * (b.nj + 6) * (b.nk + 6);
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= e) return;
+    int i = (int)(t % (b.ni + 6)) - 2, j = (int)((t / (b.ni + 6)) % (b.nj + 6)) - 2;
+    int k = (int)(t / ((long)(b.ni + 6) * (b.nj + 6))) - 2;
+    const long g = gidx(b, i, j, k);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int dir = 0; dir < 3; dir++) {
+        bool in_i = i >= 1 && i <= b.ni, in_j = j >= 1 && j <= b.nj, in_k = k >= 1 && k <= b.nk;
+        bool ok = dir == 0 ? (in_j && in_k) : (dir == 1 ? (in_i && in_k) : (in_i && in_j));
+        if (!ok) continue;
+        VLine L = line_of(b, dir, i, j, k);
+        bool fl, fr;
+        pencil_flags(F, L, fl, fr);
+        int l = dir == 0 ? i : (dir == 1 ? j : k);
+        int r0 = l - 4 < 1 ? 1 : l - 4, r1 = l + 4 > L.n ? L.n : l + 4;
+        for (int r = r0; r <= r1; r++) {
+            double c = dnode(UnitSrc{l}, L.n, r, fl, fr);
+            if (c == 0.0) continue;
+            long gr = L.at(r);
+            for (int m = 0; m < 4; m++) acc[m] = acc[m] + c * A.gdn[(long)(4 * dir + m) * F.total + gr];
+        }
+    }
+    for (int m = 0; m < 3; m++) A.gp[(1 + m) * F.total + g] = A.gp[(1 + m) * F.total + g] + acc[m];
+    A.gt[g] = A.gt[g] + acc[3];
 }
 
 // LHS.f90:243-343 spectvisl (laminar) / LHS.f90:141-241 spectvist (mu + mu_t), interior
@@ -436,6 +676,45 @@ int launch_vis_rows(int dir, const BlockGrid &G, const DFields &F, const DParams
     else if (dir == 2) hipLaunchKernelGGL(k_vis_rows<2>, dim3(G.nwg), dim3(256), 0, s, G, F, P, fvb);
     else if (dir == 3) hipLaunchKernelGGL(k_vis_rows<3>, dim3(G.nwg), dim3(256), 0, s, G, F, P, fvb);
     else return -2;
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_rhs_vis_jvp(int dir, const DBlock &b, const DFields &F, const DParams &P, const double *dn, const AdTan &D,
+                       hipStream_t s)
+{
+    int n = dir == 0 ? b.ni : (dir == 1 ? b.nj : b.nk);
+    long npen = dir == 0 ? (long)b.nj * b.nk : (dir == 1 ? (long)b.ni * b.nk : (long)b.ni * b.nj);
+    long nt = npen * ((n + SEG - 1) / SEG);
+    dim3 grid((unsigned)((nt + 127) / 128)), blk(128);
+    if (dir == 0) hipLaunchKernelGGL(k_rhs_vis_jvp<0>, grid, blk, 0, s, b, F, P, dn, D);
+    if (dir == 1) hipLaunchKernelGGL(k_rhs_vis_jvp<1>, grid, blk, 0, s, b, F, P, dn, D);
+    if (dir == 2) hipLaunchKernelGGL(k_rhs_vis_jvp<2>, grid, blk, 0, s, b, F, P, dn, D);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_rhs_vis_vjp(int dir, const DBlock &b, const DFields &F, const DParams &P, const double *dn, const AdAdj &A,
+                       hipStream_t s)
+{
+    long npen = dir == 0 ? (long)b.nj * b.nk : (dir == 1 ? (long)b.ni * b.nk : (long)b.ni * b.nj);
+    long nt = npen * NCOLOR * VIS_SEEDS;
+    dim3 grid((unsigned)((nt + 127) / 128)), blk(128);
+    if (dir == 0) hipLaunchKernelGGL(k_rhs_vis_vjp<0>, grid, blk, 0, s, b, F, P, dn, A);
+    if (dir == 1) hipLaunchKernelGGL(k_rhs_vis_vjp<1>, grid, blk, 0, s, b, F, P, dn, A);
+    if (dir == 2) hipLaunchKernelGGL(k_rhs_vis_vjp<2>, grid, blk, 0, s, b, F, P, dn, A);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_dn_jvp(const DBlock &b, const DFields &F, const AdTan &D, double *ddn, hipStream_t s)
+{
+    long n = (long)b.ni * b.nj * b.nk;
+    hipLaunchKernelGGL(k_dn_jvp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, F, D, ddn);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_dn_vjp(const DBlock &b, const DFields &F, const AdAdj &A, hipStream_t s)
+{
+    long e = (long)(b.ni + 6) * (b.nj + 6) * (b.nk + 6);
+    hipLaunchKernelGGL(k_dn_vjp, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, b, F, A);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -31,7 +31,8 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_initial_q_dq_c_t", "wcns_timestep", "wcns_r_h_s_vis", "wcns_r_h_s_invis", "wcns_communicate_dq",
            "wcns_l_h_s", "wcns_communicate_pv", "wcns_residual", "wcns_step", "wcns_get_counter", "wcns_sync",
            "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get", "wcns_force",
-           "wcns_step_graph", "wcns_autotune"]
+           "wcns_step_graph", "wcns_autotune", "wcns_adj_residual", "wcns_adj_jvp", "wcns_adj_vjp",
+           "wcns_adj_force"]
 
 TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
@@ -89,6 +90,11 @@ def load_library(path=LIBPATH):
     L.wcns_force.argtypes = [vp, dptr]
     L.wcns_step_graph.argtypes = [vp, C.c_int]
     L.wcns_autotune.argtypes = [vp, C.c_int, iptr, iptr, dptr]
+    # adjoint entries take device pointers (vp)
+    L.wcns_adj_residual.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.wcns_adj_jvp.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp]
+    L.wcns_adj_vjp.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
+    L.wcns_adj_force.argtypes = [vp, C.c_int, vp, vp, dptr, dptr, vp, vp]
     _lib = L
     return L
 
@@ -255,6 +261,23 @@ class Solver:
         self._check(self.L.wcns_region_images(self.h, nb, nr, _i(img), _i(jmg), _i(kmg), _i(box)))
         cnt = int(np.prod(box[3:] - box[:3] + 1))
         return box, img[:cnt], jmg[:cnt], kmg[:cnt]
+
+    # ---- tangent and adjoint of the residual; arguments are device addresses
+    # (int, e.g. torch.Tensor.data_ptr(); 0 for an absent optional array)
+    def adj_residual(self, nb, P, X, R):
+        self._check(self.L.wcns_adj_residual(self.h, nb, P, X, R), "adj_residual")
+
+    def adj_jvp(self, nb, P, X, dP, dX, R, dR):
+        self._check(self.L.wcns_adj_jvp(self.h, nb, P, X, dP or None, dX or None, R or None, dR), "adj_jvp")
+
+    def adj_vjp(self, nb, P, X, W, gP, gX):
+        self._check(self.L.wcns_adj_vjp(self.h, nb, P, X, W, gP or None, gX or None), "adj_vjp")
+
+    def adj_force(self, nb, P, X, omega, gP=0, gX=0):
+        om = np.ascontiguousarray(omega, dtype=np.float64)
+        J = np.zeros(1)
+        self._check(self.L.wcns_adj_force(self.h, nb, P, X, _d(om), _d(J), gP or None, gX or None), "adj_force")
+        return float(J[0])
 
     def multipoint(self):
         ng = C.c_int()
