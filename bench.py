@@ -150,6 +150,9 @@ def main():
     ap.add_argument("--launch", default="graph", choices=["graph", "plain"],
                     help="timed steps replayed from a captured HIP graph, or plain launches")
     ap.add_argument("--breakdown", action="store_true", help="print the per-kernel table to stderr")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the residual, its tangent and adjoint and the force gradient instead "
+                         "(128x96x64 wall block, SA with frozen mu_t, M 0.7)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -169,6 +172,8 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n}))
         return
     from wcns_amd.solver import Solver
+    if args.adjoint:
+        return adjoint_bench(args)
 
     case, wl_name = build_case(args.workload)
     s = Solver(case, device=0, lhs_precision=1 if args.lhs == "fp32" else 0)
@@ -240,6 +245,40 @@ def main():
         "stage_ms_per_step": {k: round(v[0] / kt, 4) for k, v in tim.items()},
     }
     print(json.dumps(line))
+    s.close()
+
+
+def adjoint_bench(args):
+    """ms per call of wcns_adj_residual / _jvp / _vjp / _force (with gradient)
+    on the adjoint workload, inputs resident on the device."""
+    import torch
+    from wcns_amd import adjoint as A, cases
+    from wcns_amd.solver import Solver
+    case = cases.bump_channel_case(ni=128, nj=96, nk=64, dy_wall=1e-4, growth=1.08, reynolds=3e6, moo=0.7)
+    case.params.nlamtur = 1
+    s = Solver(case)
+    s.step(max(args.warmup, 1), history=False)
+    s.boundary_all()
+    s.sync()
+    P, X = A.state_tensors(s, 0)
+    g = torch.Generator().manual_seed(1)
+    dP = (torch.randn(P.shape, generator=g, dtype=torch.float64) * 1e-3).cuda()
+    dX = (torch.randn(X.shape, generator=g, dtype=torch.float64) * 1e-5).cuda()
+    W = torch.randn((5,) + tuple(X.shape[1:]), generator=g, dtype=torch.float64).cuda()
+    omega = [0.0, 1.0, 0.0, 0.0, 0.0, 0.0]
+    calls = {"residual": lambda: A.residual(s, 0, P, X), "jvp": lambda: A.jvp(s, 0, P, X, dP, dX),
+             "vjp": lambda: A.vjp(s, 0, P, X, W), "force_grad": lambda: A.force(s, 0, P, X, omega, grad=True)}
+    out = {}
+    for name, f in calls.items():
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            f()
+        torch.cuda.synchronize()
+        out[name] = round(1e3 * (time.perf_counter() - t0) / args.steps, 3)
+    print(json.dumps({"adjoint_ms_per_call": out, "nodes": case.nodes, "calls": args.steps,
+                      "config": "bump-channel 128x96x64 SA (frozen mu_t) Re3e6 M0.7 Roe WCNS-E5, FP64"}))
     s.close()
 
 

@@ -581,7 +581,10 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
     const int TJ = sb.TJ, TK = sb.TK, nTJ = sb.nTJ, nTK = sb.nTK, ni = b.ni;
     const int tile = wk.y;
     const int J = tile % nTJ, K = tile / nTJ;
-    const int wave = threadIdx.x / PIPE_COLS;
+    // wave-uniform: kept in a scalar register, so that the helper's choice of
+    // operand arrays by part is a scalar select, not a per-lane load of the
+    // pointer from the kernel-argument block
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PIPE_COLS);
     const int c = threadIdx.x % PIPE_COLS;          // column of this lane
     const int p = wave - 1;                         // helper part (chain: -1)
     const int jl = c % TJ, kl = c / TJ;
@@ -704,35 +707,51 @@ struct StageList {
     int n;
 };
 
+// STG_G arrays per pass: every thread issues its 4 * STG_G loads before the
+// first LDS write
+constexpr int STG_G = 4;
 template <typename T>
 __global__ void __launch_bounds__(256) k_skew_stage(DBlock b, SkIndex ix, StageList L, T *pool, long sktot)
 {
-    __shared__ T tile[STG_J][STG_S + 1];
+    __shared__ T tile[STG_G][STG_J][STG_S + 1];
     const int k = blockIdx.z;
     const int j0 = blockIdx.y * STG_J, s0 = blockIdx.x * STG_S;
     const int t = threadIdx.x;
-    for (int a = 0; a < L.n; a++) {
-        // load: row jr = t / 16 handles 4 hyperplanes each
+    for (int a0 = 0; a0 < L.n; a0 += STG_G) {
+        // load: row jr = t / 16 handles 4 hyperplanes of each array of the pass
+        T v[STG_G][4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            int jr = t / 16, sc = (t % 16) + 16 * q;
-            int j = j0 + jr, sv = s0 + sc, i = sv - j - k;
-            T v = T(0);
-            if (j <= b.nj + 1 && i >= 0 && i <= b.ni + 1) {
-                long g = gidx(b, i, j, k);
-                v = L.f32[a] ? (T)((const float *)L.src[a])[g] : (T)((const double *)L.src[a])[g];
+        for (int e = 0; e < STG_G; e++) {
+            const int a = a0 + e;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int jr = t / 16, sc = (t % 16) + 16 * q;
+                int j = j0 + jr, sv = s0 + sc, i = sv - j - k;
+                v[e][q] = T(0);
+                if (a < L.n && j <= b.nj + 1 && i >= 0 && i <= b.ni + 1) {
+                    long g = gidx(b, i, j, k);
+                    v[e][q] = L.f32[a] ? (T)((const float *)L.src[a])[g] : (T)((const double *)L.src[a])[g];
+                }
             }
-            tile[jr][sc] = v;
         }
+#pragma unroll
+        for (int e = 0; e < STG_G; e++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) tile[e][t / 16][(t % 16) + 16 * q] = v[e][q];
         __syncthreads();
         // store: hyperplane sc, rows jr contiguous
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            int jr = t % 16, sc = t / 16 + 16 * q;
-            int j = j0 + jr, sv = s0 + sc, i = sv - j - k;
-            if (j <= b.nj + 1 && i >= 0 && i <= b.ni + 1) {
-                bool on_shell = i == 0 || i == b.ni + 1 || j == 0 || j == b.nj + 1 || k == 0 || k == b.nk + 1;
-                if (!L.shell[a] || on_shell) pool[(long)L.dst[a] * sktot + ix(i, j, k)] = tile[jr][sc];
+        for (int e = 0; e < STG_G; e++) {
+            const int a = a0 + e;
+            if (a >= L.n) break;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int jr = t % 16, sc = t / 16 + 16 * q;
+                int j = j0 + jr, sv = s0 + sc, i = sv - j - k;
+                if (j <= b.nj + 1 && i >= 0 && i <= b.ni + 1) {
+                    bool on_shell = i == 0 || i == b.ni + 1 || j == 0 || j == b.nj + 1 || k == 0 || k == b.nk + 1;
+                    if (!L.shell[a] || on_shell) pool[(long)L.dst[a] * sktot + ix(i, j, k)] = tile[e][jr][sc];
+                }
             }
         }
         __syncthreads();
