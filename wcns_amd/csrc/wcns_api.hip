@@ -1516,8 +1516,8 @@ int wcns_multipoint(wcns_ctx *ctx, int *ngroups, int *npts, int *pts, int cap)
 
 // ------------------------------------------------ tangent and adjoint of the residual
 // AD pool: P-direction / P-gradient 5, metrics 10, T 1, mu 1, node derivatives 12,
-// residual tangent 5, adjoint weights 5
-enum { AD_P = 0, AD_M = 5, AD_T = 15, AD_MU = 16, AD_DN = 17, AD_R = 29, AD_W = 34, AD_N = 39 };
+// residual tangent 5, adjoint weights 5, per-face viscous flux-input weights 28
+enum { AD_P = 0, AD_M = 5, AD_T = 15, AD_MU = 16, AD_DN = 17, AD_R = 29, AD_W = 34, AD_G = 39, AD_N = 67 };
 
 static int ad_alloc(wcns_ctx *ctx)
 {
@@ -1644,7 +1644,7 @@ int wcns_adj_vjp(wcns_ctx *ctx, int nb, const double *P, const double *X, const 
     if (launch_interior(b.db, A + AD_W * T, T, const_cast<double *>(W), 5, 1, ctx->stream))
         return fail(ctx, WCNS_E_HIP, "weight copy");
     DFields F = ctx->fields();
-    AdAdj G{A + AD_W * T, A + AD_P * T, A + AD_M * T, A + AD_T * T, A + AD_MU * T, A + AD_DN * T};
+    AdAdj G{A + AD_W * T, A + AD_P * T, A + AD_M * T, A + AD_T * T, A + AD_MU * T, A + AD_DN * T, A + AD_G * T};
     for (int dir = 0; dir < 3; dir++)
         if (launch_rhs_inv_vjp(dir, b.db, F, ctx->dp, G, ctx->stream)) return fail(ctx, WCNS_E_HIP, "inviscid adjoint");
     if (ctx->prm.nvis == 1) {

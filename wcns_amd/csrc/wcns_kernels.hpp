@@ -185,6 +185,7 @@ struct AdAdj {
     double *gt;              // gradient w.r.t. T
     double *gvisl;           // gradient w.r.t. mu
     double *gdn;             // 12: gradient w.r.t. the node derivatives
+    double *gf;              // 28: per face, the viscous flux weight through the flux's inputs
 };
 int launch_rhs_inv_jvp(int dir, const DBlock &b, const DFields &F, const DParams &P, const AdTan &D, hipStream_t s);
 int launch_rhs_inv_vjp(int dir, const DBlock &b, const DFields &F, const DParams &P, const AdAdj &A, hipStream_t s);

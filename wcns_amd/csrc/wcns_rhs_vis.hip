@@ -186,30 +186,6 @@ struct VisTanIO {
     }
 };
 
-// Adjoint IO: variable `var` seeded on the nodes of one colour
-struct VisVjpIO {
-    using T = D1;
-    VisIO s;
-    const double *wt;
-    double *lam;
-    long stride;
-    int n, color, var;
-    __device__ D1 val(int v, long g, int l) const
-    {
-        return D1(s.val(v, g, l), (v == var && ad_mod(l) == color) ? 1.0 : 0.0);
-    }
-    __device__ void row(long g, int l, const D1 *df) const
-    {
-        const int lj = ad_node(l, color);
-        if (lj < -2 || lj > n + 3) return;
-        double acc = 0.0;
-#pragma unroll
-        for (int m = 0; m < 5; m++) acc = acc + wt[m * s.total + g] * df[m].d;
-        const long gj = g + (long)(lj - l) * stride;
-        lam[gj] = lam[gj] - s.re * acc;
-    }
-};
-
 template <class IO>
 struct IoSrc {
     const IO *io;
@@ -218,27 +194,35 @@ struct IoSrc {
     __device__ typename IO::T operator()(int l) const { return io->val(var, L.at(l), l); }
 };
 
+// The viscous flux of a half node in two parts: visc_inputs interpolates the
+// 28 quantities the flux is made of from the pencil's nodes (all linear in the
+// node values), visc_core forms the flux from them.  Layout of the inputs:
+// 0..3 u, v, w, T; 4..15 the derivatives du[4*dir + {u,v,w,T}] (the pencil
+// direction's from the half-node difference, the others interpolated from the
+// node derivatives); 16..24 the metrics in kxyz_line order; 25 the volume;
+// 26, 27 the two viscosities.
+constexpr int VIN = 28;
+
 template <int DIR, class IO>
-__device__ void face_visc(const IO &io, const VLine &L, int f, bool fl, bool fr, bool turb, double cp_prl,
-                          double cp_prt, typename IO::T *fv)
+__device__ inline void visc_inputs(const IO &io, const VLine &L, int f, bool fl, bool fr, bool turb, double cp_prl,
+                                   double cp_prt, typename IO::T *in)
 {
     using T = typename IO::T;
     const int n = L.n;
     auto src = [&](int var) { return IoSrc<IO>{&io, var, L}; };
-    T uvwt[4], du[12], kx[9];
 #pragma unroll
-    for (int m = 0; m < 4; m++) uvwt[m] = value_line_half(src(VU + m), n, f, fl, fr);
+    for (int m = 0; m < 4; m++) in[m] = value_line_half(src(VU + m), n, f, fl, fr);
 #pragma unroll
-    for (int m = 0; m < 4; m++) du[4 * DIR + m] = dhalf(src(VU + m), n, f, fl, fr);
+    for (int m = 0; m < 4; m++) in[4 + 4 * DIR + m] = dhalf(src(VU + m), n, f, fl, fr);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         if (d == DIR) continue;
 #pragma unroll
-        for (int m = 0; m < 4; m++) du[4 * d + m] = vhn(src(VDN + 4 * d + m), n, f);
+        for (int m = 0; m < 4; m++) in[4 + 4 * d + m] = vhn(src(VDN + 4 * d + m), n, f);
     }
 #pragma unroll
-    for (int a = 0; a < 9; a++) kx[a] = vhn(src(VM + a), n, f);
-    T vol = vhn(src(VVOL), n, f);
+    for (int a = 0; a < 9; a++) in[16 + a] = vhn(src(VM + a), n, f);
+    in[25] = vhn(src(VVOL), n, f);
     // viscosities (sub_tgh.f90:1562-1566): vslt1 = visl + vist, vslt2 = visl*cp/prl + vist*cp/prt;
     // laminar: vist = 0
     T s1, s2;
@@ -265,6 +249,15 @@ __device__ void face_visc(const IO &io, const VLine &L, int f, bool fl, bool fr,
             s2 = (35.0 * V2(n) - 35.0 * V2(n - 1) + 21.0 * V2(n - 2) - 5.0 * V2(n - 3)) / 16.0;
         }
     }
+    in[26] = s1;
+    in[27] = s2;
+}
+
+template <int DIR, typename T>
+__device__ inline void visc_core(const T *in, T *fv)
+{
+    const T *uvwt = in, *du = in + 4, *kx = in + 16;
+    const T vol = in[25];
     // DUVWT_DXYZ (WCNSE5.f90:854-890)
     T d[12];
 #pragma unroll
@@ -276,7 +269,7 @@ __device__ void face_visc(const IO &io, const VLine &L, int f, bool fl, bool fr,
         d[m + 8] = (kx[m1 - 1] * du[3] + kx[m2 - 1] * du[7] + kx[m3 - 1] * du[11]) / vol;
     }
     // FLUX_VIS_LINE_new (newvis.f90:283-349); the face normal is the pencil direction's metric
-    T vs = s1, kcp = s2;
+    T vs = in[26], kcp = in[27];
     T vscc = vs * (2.0 / 3.0);
     T dudx = d[0], dudy = d[1], dudz = d[2], dvdx = d[3], dvdy = d[4], dvdz = d[5];
     T dwdx = d[6], dwdy = d[7], dwdz = d[8];
@@ -292,6 +285,15 @@ __device__ void face_visc(const IO &io, const VLine &L, int f, bool fl, bool fr,
     fv[2] = t2 * nx + t5 * ny + t6 * nz;
     fv[3] = t3 * nx + t6 * ny + t9 * nz;
     fv[4] = uvwt[0] * fv[1] + uvwt[1] * fv[2] + uvwt[2] * fv[3] + kcp * (d[9] * nx + d[10] * ny + d[11] * nz);
+}
+
+template <int DIR, class IO>
+__device__ inline void face_visc(const IO &io, const VLine &L, int f, bool fl, bool fr, bool turb, double cp_prl,
+                                 double cp_prt, typename IO::T *fv)
+{
+    typename IO::T in[VIN];
+    visc_inputs<DIR>(io, L, f, fl, fr, turb, cp_prl, cp_prt, in);
+    visc_core<DIR>(in, fv);
 }
 
 __device__ inline VisIO vis_io(const DFields &F, const DParams &P, const double *dn)
@@ -498,56 +500,121 @@ __global__ void __launch_bounds__(128) k_rhs_vis_jvp(DBlock b, DFields F, DParam
     vis_pencil<DIR>(io, L, fl, fr, a, bb, P.turb != 0, cp / P.prl, cp / P.prt);
 }
 
-// seeds of the viscous adjoint along DIR: u, v, w, T, the eight cross-direction
-// node derivatives, nine metrics, the volume and mu
-constexpr int VIS_SEEDS = 23;
-template <int DIR>
-__device__ inline int vis_seed_var(int k)
+// Adjoint of the viscous residual along DIR, per face instead of per pencil:
+//  k_visadj_face: the weight of face f's flux, the transposed FLUX_DXYZ rows
+//    (coefficients found by probing the row formulas with a unit face), times
+//    the transposed Jacobian of the flux core w.r.t. its 28 inputs (formed in
+//    dual numbers, one input at a time, from the inputs interpolated afresh);
+//    the 28 results go to gf at the face's pencil node f;
+//  k_visadj_node: every node gathers them through the transposed
+//    interpolations of the faces whose stencils hold it (coefficients by
+//    probing with a unit node) into the gradients of u, v, w, T, the node
+//    derivatives, the metrics, the volume and mu.
+template <class S>
+__device__ inline auto vis_row(const S &fa, int n, int l) -> decltype(fa(0))
 {
-    if (k < 4) return VU + k;
-    if (k < 12) {
-        int c = k - 4;                           // 0..7 over the two other directions
-        int dd = c / 4;                          // 0: first other direction, 1: second
-        int dir = DIR == 0 ? 1 + dd : (DIR == 1 ? 2 * dd : dd);
-        return VDN + 4 * dir + c % 4;
-    }
-    if (k < 21) return VM + (k - 12);
-    return k == 21 ? VVOL : VVISL;
+    if (l == 1) return (-23.0 * fa(0) + 21.0 * fa(1) + 3.0 * fa(2) - fa(3)) / 24.0;
+    if (l == 2) return (fa(0) - 27.0 * fa(1) + 27.0 * fa(2) - fa(3)) / 24.0;
+    if (l == n - 1) return -(fa(n) - 27.0 * fa(n - 1) + 27.0 * fa(n - 2) - fa(n - 3)) / 24.0;
+    if (l == n) return -(-23.0 * fa(n) + 21.0 * fa(n - 1) + 3.0 * fa(n - 2) - fa(n - 3)) / 24.0;
+    return (2250.0 * (fa(l) - fa(l - 1)) - 125.0 * (fa(l + 1) - fa(l - 2)) + 9.0 * (fa(l + 2) - fa(l - 3))) / 1920.0;
 }
 
 template <int DIR>
-__global__ void __launch_bounds__(128) k_rhs_vis_vjp(DBlock b, DFields F, DParams P, const double *dn, AdAdj A)
+__global__ void __launch_bounds__(256) k_visadj_face(DBlock b, DFields F, DParams P, const double *dn, AdAdj A)
 {
+    const int n = DIR == 0 ? b.ni : (DIR == 1 ? b.nj : b.nk);
     const int na = DIR == 0 ? b.nj : b.ni;
     const int nbk = DIR == 2 ? b.nj : b.nk;
     const long npen = (long)na * nbk;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npen * NCOLOR * VIS_SEEDS) return;
-    const long pen = t % npen;
-    const int cs = (int)(t / npen);
-    const int color = cs % NCOLOR, k = cs / NCOLOR;
-    int ia = (int)(pen % na) + 1, ib = (int)(pen / na) + 1;
-    int i0 = DIR == 0 ? 0 : ia, j0 = DIR == 1 ? 0 : (DIR == 0 ? ia : ib), k0 = DIR == 2 ? 0 : ib;
-    VLine L = line_of(b, DIR, i0, j0, k0);
+    if (t >= npen * (n + 1)) return;
+    const int f = (int)(t / npen);
+    const long pen = t - (long)f * npen;
+    const int ia = (int)(pen % na) + 1, ib = (int)(pen / na) + 1;
+    const int i0 = DIR == 0 ? 0 : ia, j0 = DIR == 1 ? 0 : (DIR == 0 ? ia : ib), k0 = DIR == 2 ? 0 : ib;
+    const VLine L = line_of(b, DIR, i0, j0, k0);
     bool fl, fr;
     pencil_flags(F, L, fl, fr);
-    VisVjpIO io;
-    io.s = vis_io(F, P, dn);
-    io.wt = A.w;
-    const int var = vis_seed_var<DIR>(k);
-    const long T5 = F.total;
-    if (var < VT) io.lam = A.gp + (1 + var) * T5;
-    else if (var == VT) io.lam = A.gt;
-    else if (var < VM) io.lam = A.gdn + (long)(var - VDN) * T5;
-    else if (var < VVOL) io.lam = A.gm + (long)kxyz_pool(var - VM) * T5;
-    else if (var == VVOL) io.lam = A.gm + 9 * T5;
-    else io.lam = A.gvisl;
-    io.stride = L.stride;
-    io.n = L.n;
-    io.color = color;
-    io.var = var;
+    // weight of the face's flux: R -= re * (row formulas), rows f-2..f+3
+    double lf[5] = {0, 0, 0, 0, 0};
+    const int r0 = f - 2 < 1 ? 1 : f - 2, r1 = f + 3 > n ? n : f + 3;
+    for (int l = r0; l <= r1; l++) {
+        const double c = vis_row(UnitSrc{f}, n, l);
+        if (c == 0.0) continue;
+        const long gl = L.at(l);
+#pragma unroll
+        for (int m = 0; m < 5; m++) lf[m] = lf[m] - P.re * c * A.w[m * F.total + gl];
+    }
     const double cp = 1.0 / ((P.gama - 1.0) * P.moo * P.moo);
-    vis_pencil<DIR>(io, L, fl, fr, 1, L.n, P.turb != 0, cp / P.prl, cp / P.prt);
+    double in[VIN];
+    visc_inputs<DIR>(vis_io(F, P, dn), L, f, fl, fr, P.turb != 0, cp / P.prl, cp / P.prt, in);
+    const long g = L.at(f);
+    for (int q = 0; q < VIN; q++) {
+        D1 x[VIN], fv[5];
+#pragma unroll
+        for (int e = 0; e < VIN; e++) x[e] = D1(in[e], e == q ? 1.0 : 0.0);
+        visc_core<DIR>(x, fv);
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 1; m < 5; m++) acc = acc + lf[m] * fv[m].d;
+        A.gf[(long)q * F.total + g] = acc;
+    }
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(256) k_visadj_node(DBlock b, DFields F, DParams P, AdAdj A)
+{
+    const long e = (long)(b.ni + 6) * (b.nj + 6) * (b.nk + 6);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= e) return;
+    const int i = (int)(t % (b.ni + 6)) - 2, j = (int)((t / (b.ni + 6)) % (b.nj + 6)) - 2;
+    const int k = (int)(t / ((long)(b.ni + 6) * (b.nj + 6))) - 2;
+    const bool in_i = i >= 1 && i <= b.ni, in_j = j >= 1 && j <= b.nj, in_k = k >= 1 && k <= b.nk;
+    if (!(DIR == 0 ? (in_j && in_k) : (DIR == 1 ? (in_i && in_k) : (in_i && in_j)))) return;
+    const VLine L = line_of(b, DIR, i, j, k);
+    const int n = L.n;
+    const int l = DIR == 0 ? i : (DIR == 1 ? j : k);
+    bool fl, fr;
+    pencil_flags(F, L, fl, fr);
+    const double cp = 1.0 / ((P.gama - 1.0) * P.moo * P.moo);
+    const double cp_prl = cp / P.prl;
+    double gu[4] = {0, 0, 0, 0}, gd[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, gmt[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double gv = 0.0;
+    const int f0 = l - 4 < 0 ? 0 : l - 4, f1 = l + 4 > n ? n : l + 4;
+    for (int f = f0; f <= f1; f++) {
+        const double cv = value_line_half(UnitSrc{l}, n, f, fl, fr);
+        const double ch = dhalf(UnitSrc{l}, n, f, fl, fr);
+        const double cn = vhn(UnitSrc{l}, n, f);
+        if (cv == 0.0 && ch == 0.0 && cn == 0.0) continue;
+        const long gf = L.at(f);
+        auto G = [&](int q) { return A.gf[(long)q * F.total + gf]; };
+#pragma unroll
+        for (int m = 0; m < 4; m++) gu[m] = gu[m] + cv * G(m) + ch * G(4 + 4 * DIR + m);
+        if (cn != 0.0) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                if (d == DIR) continue;
+#pragma unroll
+                for (int m = 0; m < 4; m++) gd[4 * d + m] = gd[4 * d + m] + cn * G(4 + 4 * d + m);
+            }
+#pragma unroll
+            for (int a = 0; a < 9; a++) gmt[kxyz_pool(a)] = gmt[kxyz_pool(a)] + cn * G(16 + a);
+            gmt[9] = gmt[9] + cn * G(25);
+            gv = gv + cn * (G(26) + G(27) * cp_prl);
+        }
+    }
+    const long g = L.at(l), T5 = F.total;
+#pragma unroll
+    for (int m = 0; m < 3; m++) A.gp[(1 + m) * T5 + g] = A.gp[(1 + m) * T5 + g] + gu[m];
+    A.gt[g] = A.gt[g] + gu[3];
+    if (l >= 1 && l <= n) {
+#pragma unroll
+        for (int c = 0; c < 12; c++) A.gdn[c * T5 + g] = A.gdn[c * T5 + g] + gd[c];
+#pragma unroll
+        for (int c = 0; c < 10; c++) A.gm[c * T5 + g] = A.gm[c * T5 + g] + gmt[c];
+        A.gvisl[g] = A.gvisl[g] + gv;
+    }
 }
 
 // tangent of the node derivatives: the same stencils on du, dv, dw, dT
@@ -696,11 +763,19 @@ int launch_rhs_vis_vjp(int dir, const DBlock &b, const DFields &F, const DParams
                        hipStream_t s)
 {
     long npen = dir == 0 ? (long)b.nj * b.nk : (dir == 1 ? (long)b.ni * b.nk : (long)b.ni * b.nj);
-    long nt = npen * NCOLOR * VIS_SEEDS;
-    dim3 grid((unsigned)((nt + 127) / 128)), blk(128);
-    if (dir == 0) hipLaunchKernelGGL(k_rhs_vis_vjp<0>, grid, blk, 0, s, b, F, P, dn, A);
-    if (dir == 1) hipLaunchKernelGGL(k_rhs_vis_vjp<1>, grid, blk, 0, s, b, F, P, dn, A);
-    if (dir == 2) hipLaunchKernelGGL(k_rhs_vis_vjp<2>, grid, blk, 0, s, b, F, P, dn, A);
+    int n = dir == 0 ? b.ni : (dir == 1 ? b.nj : b.nk);
+    long nf = npen * (n + 1), e = (long)(b.ni + 6) * (b.nj + 6) * (b.nk + 6);
+    dim3 gf((unsigned)((nf + 255) / 256)), gn((unsigned)((e + 255) / 256)), blk(256);
+    if (dir == 0) {
+        hipLaunchKernelGGL(k_visadj_face<0>, gf, blk, 0, s, b, F, P, dn, A);
+        hipLaunchKernelGGL(k_visadj_node<0>, gn, blk, 0, s, b, F, P, A);
+    } else if (dir == 1) {
+        hipLaunchKernelGGL(k_visadj_face<1>, gf, blk, 0, s, b, F, P, dn, A);
+        hipLaunchKernelGGL(k_visadj_node<1>, gn, blk, 0, s, b, F, P, A);
+    } else {
+        hipLaunchKernelGGL(k_visadj_face<2>, gf, blk, 0, s, b, F, P, dn, A);
+        hipLaunchKernelGGL(k_visadj_node<2>, gn, blk, 0, s, b, F, P, A);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
