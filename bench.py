@@ -150,6 +150,10 @@ def main():
     ap.add_argument("--launch", default="graph", choices=["graph", "plain"],
                     help="timed steps replayed from a captured HIP graph, or plain launches")
     ap.add_argument("--breakdown", action="store_true", help="print the per-kernel table to stderr")
+    ap.add_argument("--cfl", type=float, default=None, help="override the workload's CFL number")
+    ap.add_argument("--converge", type=int, default=0, metavar="STEPS",
+                    help="run STEPS steps with the FP64 and with the FP32 implicit operator and report the "
+                         "residual drop of each against time (the metric's equal-drop comparison)")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the residual, its tangent and adjoint and the force gradient instead "
                          "(128x96x64 wall block, SA with frozen mu_t, M 0.7)")
@@ -174,8 +178,13 @@ def main():
     from wcns_amd.solver import Solver
     if args.adjoint:
         return adjoint_bench(args)
+    if args.converge:
+        return converge_bench(args)
 
     case, wl_name = build_case(args.workload)
+    if args.cfl:
+        case.params.cfl = args.cfl
+        wl_name += f" CFL {args.cfl:g}"
     s = Solver(case, device=0, lhs_precision=1 if args.lhs == "fp32" else 0)
     s.step(args.warmup, history=False)
     tile = None
@@ -246,6 +255,49 @@ def main():
     }
     print(json.dumps(line))
     s.close()
+
+
+def converge_bench(args, chunk=100):
+    """Residual drop (orders of magnitude of the RMS monitor below its first
+    value) against steps and seconds, FP64 and FP32 implicit operators, graph
+    replays of `chunk` steps between monitor reads."""
+    import math
+    import torch
+    from wcns_amd.solver import Solver
+    case, wl_name = build_case(args.workload)
+    if args.cfl:
+        case.params.cfl = args.cfl
+        wl_name += f" CFL {args.cfl:g}"
+    runs = {}
+    for lhs in ("fp64", "fp32"):
+        s = Solver(case, device=0, lhs_precision=1 if lhs == "fp32" else 0)
+        s.step(1, history=False)
+        r0 = s.residual_curve()[0]
+        if not args.no_tune:
+            s.autotune(reps=2)
+        hist, t_run, done = [], 0.0, 1
+        while done < args.converge:
+            n = min(chunk, args.converge - done)
+            s.sync()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s.step_graph(n)
+            s.sync()
+            t_run += time.perf_counter() - t0
+            done += n
+            rms = s.residual_curve()[0]
+            hist.append((done, round(t_run, 4), round(math.log10(r0 / rms), 3) if rms > 0 else None))
+        runs[lhs] = {"rms_first": r0, "history": hist, "ms_per_step": round(1e3 * t_run / (done - 1), 3)}
+        s.close()
+    # equal drop: the deepest drop both reached, and when each reached it
+    target = min(max(h[2] for h in runs[k]["history"] if h[2] is not None) for k in runs)
+    reach = {}
+    for k, r in runs.items():
+        hit = next((h for h in r["history"] if h[2] is not None and h[2] >= target), None)
+        reach[k] = {"steps": hit[0], "seconds": hit[1]} if hit else None
+    speed = reach["fp64"]["seconds"] / reach["fp32"]["seconds"] if reach["fp64"] and reach["fp32"] else None
+    print(json.dumps({"converge": {"workload": wl_name, "steps": args.converge, "equal_drop_orders": target,
+                                   "reached": reach, "fp32_over_fp64_time_speedup": speed, "runs": runs}}))
 
 
 def adjoint_bench(args):

@@ -34,14 +34,14 @@ def _solver(case):
     return Solver(case)
 
 
-def _state(case, steps=4):
+def _state(case, steps=4, nb=0):
     """A solver advanced a few steps, its ghosts refreshed from the interior."""
     from wcns_amd import adjoint as A
     s = _solver(case)
     s.step(steps, history=False)
     s.boundary_all()
     s.sync()
-    P, X = A.state_tensors(s, 0)
+    P, X = A.state_tensors(s, nb)
     return s, P, X
 
 
@@ -66,11 +66,18 @@ def _sa(**kw):
     return c
 
 
+def _sst(**kw):
+    c = _laminar(reynolds=1e5, **kw)
+    c.params.nlamtur = 2
+    return c
+
+
 CASES = {
     "laminar-roe-s41": lambda: _laminar(),
     "laminar-roe-s4": lambda: _laminar(nscheme=4),
     "laminar-sw-s41": lambda: _laminar(nflux=1, ni=20, nj=18, nk=12),
     "sa-frozen-roe-s41": lambda: _sa(),
+    "sst-frozen-roe-s41": lambda: _sst(),
 }
 
 
@@ -124,6 +131,34 @@ def test_dot_product(name):
         _, dRx = A.jvp(s, 0, P, X, None, dX)
         lx = float((W * dRx).sum())
         assert abs(lx - rx) <= 1e-12 * max(abs(lx), abs(rx))
+
+
+def test_multiblock_interface_block():
+    """Block 1 of a 2x2 viscous wall case: its interface ghosts are inputs like
+    any other; dot products at 1e-12 and Richardson differences."""
+    from wcns_amd import adjoint as A
+    case = cases.multiblock_case(nvis=1, wall=True, reynolds=1e4)
+    s, P, X = _state(case, nb=1)
+    for seed in (1, 2):
+        dP, dX = _directions(P, X, seed, 1e-4)
+        W = torch.randn((5,) + tuple(X.shape[1:]), dtype=torch.float64,
+                        generator=torch.Generator().manual_seed(50 + seed)).to(P.device)
+        _, dR = A.jvp(s, 1, P, X, dP, dX)
+        gP, gX = A.vjp(s, 1, P, X, W)
+        lhs = float((W * dR).sum())
+        rhs = float((gP * dP).sum()) + float((gX * dX).sum())
+        rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+        print(f"block 1 seed {seed}: dot-product rel {rel:.2e}", flush=True)
+        assert rel <= 1e-12
+    dP, dX = _directions(P, X, 9, 1e-2)
+    _, dR = A.jvp(s, 1, P, X, dP, dX)
+
+    def cd(h):
+        return (A.residual(s, 1, P + h * dP, X + h * dX) - A.residual(s, 1, P - h * dP, X - h * dX)) / (2 * h)
+    f1, f2 = cd(1e-7), cd(5e-8)
+    rich = float(((4 * f2 - f1) / 3 - dR).abs().max() / dR.abs().max())
+    print(f"block 1 Richardson rel err {rich:.2e}", flush=True)
+    assert rich <= 5e-6
 
 
 def _fd_check(s, A, P, X, dP, dX, h=1e-7):

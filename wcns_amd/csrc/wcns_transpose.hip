@@ -18,7 +18,10 @@ namespace {
 constexpr int TT = 32;      // tile edge
 constexpr int TROWS = 8;    // rows per pass: 256 threads = 32 x 8
 
-// BACK = 0: dst (transposed) = src (normal); BACK = 1: dst (normal) += src (transposed)
+// BACK = 0: dst (transposed) = src (normal); BACK = 1: dst (normal) += src (transposed).
+// TG arrays per pass: every thread issues its loads of the pass before the
+// first LDS write, so that each workgroup keeps 4 * TG loads per thread in flight.
+constexpr int TG = 4;
 template <int BACK>
 __global__ void __launch_bounds__(256) k_transpose(BlockGrid G, TransList L)
 {
@@ -31,34 +34,54 @@ __global__ void __launch_bounds__(256) k_transpose(BlockGrid G, TransList L)
     const int bi = rem % ti, bj = rem / ti;
     const long base = b.off + (long)plane * b.sy;
     const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
-    __shared__ double s[TT][TT + 1];
-    for (int a = 0; a < L.n; a++) {
-        const double *src = L.src[a];
-        double *dst = L.dst[a];
-        // s[jl][il] holds node (bi*TT + il, bj*TT + jl)
+    __shared__ double s[TG][TT][TT + 1];
+    for (int a0 = 0; a0 < L.n; a0 += TG) {
+        double v[TG][TT / TROWS];
+        // s[e][jl][il] holds node (bi*TT + il, bj*TT + jl) of array a0 + e
 #pragma unroll
-        for (int r = 0; r < TT / TROWS; r++) {
-            const int yl = ty + TROWS * r;
-            if (BACK == 0) {
-                const int i = bi * TT + tx, j = bj * TT + yl;
-                if (i < ei && j < ej) s[yl][tx] = src[base + i + (long)ei * j];
-            } else {
-                const int j = bj * TT + tx, i = bi * TT + yl;
-                if (i < ei && j < ej) s[tx][yl] = src[base + j + (long)ej * i];
+        for (int e = 0; e < TG; e++) {
+            const int a = a0 + e;
+#pragma unroll
+            for (int r = 0; r < TT / TROWS; r++) {
+                const int yl = ty + TROWS * r;
+                v[e][r] = 0.0;
+                if (a < L.n) {
+                    if (BACK == 0) {
+                        const int i = bi * TT + tx, j = bj * TT + yl;
+                        if (i < ei && j < ej) v[e][r] = L.src[a][base + i + (long)ei * j];
+                    } else {
+                        const int j = bj * TT + tx, i = bi * TT + yl;
+                        if (i < ei && j < ej) v[e][r] = L.src[a][base + j + (long)ej * i];
+                    }
+                }
             }
         }
+#pragma unroll
+        for (int e = 0; e < TG; e++)
+#pragma unroll
+            for (int r = 0; r < TT / TROWS; r++) {
+                const int yl = ty + TROWS * r;
+                if (BACK == 0) s[e][yl][tx] = v[e][r];
+                else s[e][tx][yl] = v[e][r];
+            }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < TT / TROWS; r++) {
-            const int yl = ty + TROWS * r;
-            if (BACK == 0) {
-                const int j = bj * TT + tx, i = bi * TT + yl;
-                if (i < ei && j < ej) dst[base + j + (long)ej * i] = s[tx][yl];
-            } else {
-                const int i = bi * TT + tx, j = bj * TT + yl;
-                if (i < ei && j < ej) {
-                    const long g = base + i + (long)ei * j;
-                    dst[g] = dst[g] + s[yl][tx];
+        for (int e = 0; e < TG; e++) {
+            const int a = a0 + e;
+            if (a >= L.n) break;
+            double *dst = L.dst[a];
+#pragma unroll
+            for (int r = 0; r < TT / TROWS; r++) {
+                const int yl = ty + TROWS * r;
+                if (BACK == 0) {
+                    const int j = bj * TT + tx, i = bi * TT + yl;
+                    if (i < ei && j < ej) dst[base + j + (long)ej * i] = s[e][tx][yl];
+                } else {
+                    const int i = bi * TT + tx, j = bj * TT + yl;
+                    if (i < ei && j < ej) {
+                        const long g = base + i + (long)ei * j;
+                        dst[g] = dst[g] + s[e][yl][tx];
+                    }
                 }
             }
         }
