@@ -289,6 +289,31 @@ def test_multiblock_40_steps():
     _cmp_all_blocks(s, o, case, ("r", "u", "v", "w", "p"), 1e-11)
 
 
+def _uneven_sa():
+    case = cases.uneven_blocks_case(nblk=(3, 2, 2), lo=13, hi=22, seed=3, nvis=1, wall=True)
+    case.params.nlamtur = 1
+    return case
+
+
+@pytest.mark.parametrize("make", ["uneven-sa", "mixed-bc-sa"])
+def test_surface_levels_match_region_sequence(monkeypatch, make):
+    """Boundary conditions and interface averaging launched as dependency
+    levels of all regions give the bits of one launch per region in the
+    reference's order (WCNS_SURFACE=sequence), over five whole steps."""
+    case = _uneven_sa() if make == "uneven-sa" else _sa_mixed_bc()
+    print(f"surface levels vs sequence: {make}, {len(case.blocks)} blocks", flush=True)
+    monkeypatch.setenv("WCNS_SURFACE", "sequence")
+    a = _solver(case)
+    monkeypatch.delenv("WCNS_SURFACE")
+    b = _solver(case)
+    ha = a.step(5)
+    hb = b.step(5)
+    assert np.array_equal(ha, hb)
+    for nb in range(len(case.blocks)):
+        for name in ("r", "u", "v", "w", "p", "qke", "vist"):
+            assert np.array_equal(a.field(nb, name), b.field(nb, name)), f"block {nb} {name}"
+
+
 # ---------------------------------------------------------------- Spalart-Allmaras
 def _sa_bump(**kw):
     case = cases.bump_channel_case(dy_wall=1e-3, growth=1.2, reynolds=1e5, **kw)

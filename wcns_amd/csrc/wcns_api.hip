@@ -72,6 +72,7 @@ struct wcns_ctx {
     float *dq32 = nullptr;                  // dQ of the FP32 implicit operator
     void *skew[2] = {nullptr, nullptr};     // plane-major sweep operands, FP64 / FP32 implicit operator
     std::vector<long> skoff;                // per block offset in each plane-major array
+    long *d_skoff = nullptr;                // the same, on the device
     long sktot = 0;
     SweepPlan plan{};                       // batched pipelined sweeps over all blocks
     std::vector<std::pair<int, int>> tile_shape;   // per block (TJ, TK)
@@ -93,8 +94,17 @@ struct wcns_ctx {
     double *dn_t = nullptr;                 // 12 slots; 0..3 <- j derivatives, 8..11 <- k derivatives
     double *df_t = nullptr;                 // 5: the i-direction contribution, transposed
     double *fvb = nullptr;                  // 5: viscous half-node fluxes of one direction
-    int *d_flag = nullptr;
+    int *d_flag = nullptr;                  // flag_cap nodes per block for the ordered repair
     int flag_cap = 512;
+    unsigned long long *d_nflag = nullptr;  // flagged-node count per block
+    // surface stages as dependency levels: regions (BC sequence, interfaces,
+    // physical faces) and per-level work lists; WCNS_SURFACE=sequence launches
+    // the BC and interface regions one at a time instead
+    bool surf_levels = true;
+    DRegion *d_regs = nullptr;
+    int2 *d_rwork = nullptr;
+    std::vector<int> bc_lvl, if_lvl;        // work offsets of each level, then the end
+    int zero_w0 = 0, zero_nw = 0;           // the physical faces' work entries
     double *d_partial = nullptr;
     int max_parts = 1;
     double *d_resblk = nullptr;
@@ -444,6 +454,7 @@ int wcns_create(wcns_ctx **out, int device, const wcns_params *prm)
     ctx->prm = *prm;
     if (const char *sw = getenv("WCNS_SWEEP")) ctx->tiled_sweeps = strcmp(sw, "plane") != 0;
     if (const char *id = getenv("WCNS_IDIR")) ctx->idir_t = strcmp(id, "direct") != 0;
+    if (const char *sf = getenv("WCNS_SURFACE")) ctx->surf_levels = strcmp(sf, "sequence") != 0;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return WCNS_E_HIP;
@@ -463,7 +474,7 @@ int wcns_destroy(wcns_ctx *ctx)
     for (int m = 0; m < 3; m++) { hipFree(ctx->sr[m]); hipFree(ctx->srv[m]); }
     void *pl[] = {ctx->d_sblocks, ctx->d_work, ctx->d_xch};
     for (void *p : pl) if (p) hipFree(p);
-    void *ps[] = {ctx->fvb, ctx->tpool, ctx->d_blocks_t, ctx->d_grid, ctx->qkw, ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag,
+    void *ps[] = {ctx->fvb, ctx->tpool, ctx->d_blocks_t, ctx->d_grid, ctx->qkw, ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag, ctx->d_nflag, ctx->d_regs, ctx->d_rwork, ctx->d_skoff,
                   ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
@@ -600,6 +611,113 @@ int wcns_set_lhs_precision(wcns_ctx *ctx, int prec)
     return WCNS_OK;
 }
 
+// Dependency levels of region operations that must give the results of
+// running them one after another (boundary_sequence's region order, or the
+// interface averaging's).  touch(g, rd, wr) lists the pool nodes operation g
+// reads and writes.  An operation goes one level above every earlier one that
+// wrote a node it reads or writes, or read a node it writes, so the
+// operations of one level touch no node that another of them writes.
+extern "C++" {
+template <typename Touch>
+static void region_levels(long total, const std::vector<DRegion> &seq, Touch touch, std::vector<int> &lvl)
+{
+    std::vector<uint16_t> lw(total, 0), lr(total, 0);   // 1 + level of the last write / read
+    std::vector<long> rd, wr;
+    lvl.assign(seq.size(), 0);
+    for (size_t n = 0; n < seq.size(); n++) {
+        rd.clear();
+        wr.clear();
+        touch(seq[n], rd, wr);
+        int L = 0;
+        for (long q : rd) L = std::max(L, (int)lw[q]);
+        for (long q : wr) L = std::max(L, std::max((int)lw[q], (int)lr[q]));
+        lvl[n] = L;
+        const uint16_t L1 = (uint16_t)(L + 1);
+        for (long q : rd) lr[q] = std::max(lr[q], L1);
+        for (long q : wr) lw[q] = std::max(lw[q], L1);
+    }
+}
+}
+
+// Work entries (region, first point) of the regions seq[n] + base, level by
+// level; a serial region is one entry (region, -1).
+static void level_work(const std::vector<DRegion> &seq, const std::vector<int> &lvl, int base,
+                       std::vector<int2> &work, std::vector<int> &start)
+{
+    int nl = 0;
+    for (int l : lvl) nl = std::max(nl, l + 1);
+    for (int l = 0; l < nl; l++) {
+        start.push_back((int)work.size());
+        for (size_t n = 0; n < seq.size(); n++) {
+            if (lvl[n] != l) continue;
+            if (seq[n].serial) { work.push_back(make_int2(base + (int)n, -1)); continue; }
+            for (int t = 0; t < seq[n].npts; t += REGION_CHUNK) work.push_back(make_int2(base + (int)n, t));
+        }
+    }
+    start.push_back((int)work.size());
+}
+
+static int build_region_plans(wcns_ctx *ctx)
+{
+    std::vector<DBlock> dbs;
+    for (auto &b : ctx->blocks) dbs.push_back(b.db);
+    std::vector<DRegion> bcs;
+    for (auto &b : ctx->blocks)
+        for (int r : b.bcorder) bcs.push_back(b.regs[r].d);
+    // bc_point: the face node and three ghost layers written; three interior
+    // layers read, and for an interface three layers of the partner block
+    auto bc_touch = [&](const DRegion &g, std::vector<long> &rd, std::vector<long> &wr) {
+        const DBlock &b = dbs[g.nb];
+        const int *l3 = g.l3;
+        for (int t = 0; t < g.npts; t++) {
+            int i, j, k;
+            region_point(g, t, i, j, k);
+            for (int a = 0; a <= 3; a++) wr.push_back(gidx(b, i + a * l3[0], j + a * l3[1], k + a * l3[2]));
+            for (int a = 1; a <= 3; a++) rd.push_back(gidx(b, i - a * l3[0], j - a * l3[1], k - a * l3[2]));
+            if (g.bctype < 0) {
+                const DBlock &bt = dbs[g.nbt];
+                const int *tl = g.t_l3;
+                int it, jt, kt;
+                region_image(g, i, j, k, it, jt, kt);
+                for (int a = 1; a <= 3; a++) rd.push_back(gidx(bt, it - a * tl[0], jt - a * tl[1], kt - a * tl[2]));
+            }
+        }
+    };
+    // avg2pm_point: the node written, its image read
+    auto if_touch = [&](const DRegion &g, std::vector<long> &rd, std::vector<long> &wr) {
+        for (int t = 0; t < g.npts; t++) {
+            int i, j, k, it, jt, kt;
+            region_point(g, t, i, j, k);
+            region_image(g, i, j, k, it, jt, kt);
+            wr.push_back(gidx(dbs[g.nb], i, j, k));
+            rd.push_back(gidx(dbs[g.nbt], it, jt, kt));
+        }
+    };
+    std::vector<int> lb, li;
+    region_levels(ctx->total, bcs, bc_touch, lb);
+    region_levels(ctx->total, ctx->iface, if_touch, li);
+    std::vector<DRegion> regs = bcs;
+    regs.insert(regs.end(), ctx->iface.begin(), ctx->iface.end());
+    regs.insert(regs.end(), ctx->phys.begin(), ctx->phys.end());
+    std::vector<int2> work;
+    ctx->bc_lvl.clear();
+    ctx->if_lvl.clear();
+    level_work(bcs, lb, 0, work, ctx->bc_lvl);
+    level_work(ctx->iface, li, (int)bcs.size(), work, ctx->if_lvl);
+    ctx->zero_w0 = (int)work.size();
+    std::vector<int> l0(ctx->phys.size(), 0);
+    std::vector<DRegion> ph = ctx->phys;
+    for (auto &g : ph) g.serial = 0;
+    std::vector<int> zs;
+    level_work(ph, l0, (int)(bcs.size() + ctx->iface.size()), work, zs);
+    ctx->zero_nw = (int)work.size() - ctx->zero_w0;
+    TRY(dalloc(ctx, &ctx->d_regs, std::max<size_t>(regs.size(), 1)));
+    TRY(dalloc(ctx, &ctx->d_rwork, std::max<size_t>(work.size(), 1)));
+    if (!regs.empty()) TRY(copy_sync(ctx, ctx->d_regs, regs.data(), sizeof(DRegion) * regs.size(), hipMemcpyHostToDevice));
+    if (!work.empty()) TRY(copy_sync(ctx, ctx->d_rwork, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice));
+    return WCNS_OK;
+}
+
 int wcns_setup(wcns_ctx *ctx)
 {
     if (!ctx) return WCNS_E_ARG;
@@ -682,7 +800,8 @@ int wcns_setup(wcns_ctx *ctx)
         ctx->tile_shape.push_back({TJ, std::max(1, std::min(64 / TJ, b.nk))});
     }
     TRY(dalloc(ctx, &ctx->counters, 8));
-    TRY(dalloc(ctx, &ctx->d_flag, ctx->flag_cap));
+    TRY(dalloc(ctx, &ctx->d_flag, (size_t)ctx->flag_cap * ctx->blocks.size()));
+    TRY(dalloc(ctx, &ctx->d_nflag, ctx->blocks.size()));
     std::vector<DBlock> hb;
     int maxp = 1;
     for (auto &b : ctx->blocks) { hb.push_back(b.db); maxp = std::max(maxp, residual_parts(b.db)); }
@@ -842,6 +961,8 @@ int wcns_setup(wcns_ctx *ctx)
     ctx->skoff.clear();
     ctx->sktot = 0;
     for (auto &b : ctx->blocks) { ctx->skoff.push_back(ctx->sktot); ctx->sktot += skew_size(b.db); }
+    TRY(dalloc(ctx, &ctx->d_skoff, ctx->skoff.size()));
+    TRY(copy_sync(ctx, ctx->d_skoff, ctx->skoff.data(), sizeof(long) * ctx->skoff.size(), hipMemcpyHostToDevice));
     if (ctx->tiled_sweeps) {
         TRY(alloc_skew(ctx, 0));
         if (ctx->lhs_prec == WCNS_LHS_FP32) TRY(alloc_skew(ctx, 1));
@@ -900,6 +1021,7 @@ int wcns_setup(wcns_ctx *ctx)
         TRY(copy_sync(ctx, ctx->d_grp_n, gn.data(), 4 * gn.size(), hipMemcpyHostToDevice));
         TRY(copy_sync(ctx, ctx->d_write_ok, ok.data(), 4 * ok.size(), hipMemcpyHostToDevice));
     }
+    TRY(build_region_plans(ctx));
     // walls: face-node coordinates in region-point order, force scratch
     size_t maxw = 1, maxq = 1;
     for (auto &w : ctx->walls) {
@@ -1007,6 +1129,13 @@ int wcns_boundary_all(wcns_ctx *ctx)
     std::vector<DBlock> dbs;
     for (auto &b : ctx->blocks) dbs.push_back(b.db);
     Scope sc(ctx, WCNS_T_BC);
+    if (ctx->surf_levels) {
+        for (size_t l = 0; l + 1 < ctx->bc_lvl.size(); l++)
+            if (launch_bc_level(ctx->d_regs, ctx->d_rwork + ctx->bc_lvl[l], ctx->bc_lvl[l + 1] - ctx->bc_lvl[l],
+                                ctx->d_blocks, F, ctx->dp, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "bc launch");
+        return WCNS_OK;
+    }
     for (auto &b : ctx->blocks)
         for (int r : b.bcorder)
             if (launch_bc_region(b.regs[r].d, dbs.data(), F, ctx->dp, ctx->stream))
@@ -1146,8 +1275,15 @@ static int communicate(wcns_ctx *ctx, int pv)
     int nvar = pv ? 6 : 5;
     if (launch_npp_gather(ctx->d_npp_pts, ctx->npp, ctx->d_blocks, F, ctx->d_npp_buf, nvar, !pv, ctx->stream))
         return fail(ctx, WCNS_E_HIP, "npp gather");
-    for (auto &g : ctx->iface)
-        if (launch_avg2pm(g, dbs.data(), F, pv, ctx->stream)) return fail(ctx, WCNS_E_HIP, "avg2pm");
+    if (ctx->surf_levels) {
+        for (size_t l = 0; l + 1 < ctx->if_lvl.size(); l++)
+            if (launch_avg2pm_level(ctx->d_regs, ctx->d_rwork + ctx->if_lvl[l], ctx->if_lvl[l + 1] - ctx->if_lvl[l],
+                                    ctx->d_blocks, F, pv, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "avg2pm");
+    } else {
+        for (auto &g : ctx->iface)
+            if (launch_avg2pm(g, dbs.data(), F, pv, ctx->stream)) return fail(ctx, WCNS_E_HIP, "avg2pm");
+    }
     if (launch_npp_assign(ctx->d_grp_start, ctx->d_grp_n, ctx->d_npp_pts, ctx->d_write_ok, ctx->ngroups,
                           ctx->d_npp_buf, ctx->d_blocks, F, nvar, !pv, ctx->stream))
         return fail(ctx, WCNS_E_HIP, "npp assign");
@@ -1173,14 +1309,15 @@ int wcns_l_h_s(wcns_ctx *ctx)
     std::vector<DBlock> dbs;
     for (auto &b : ctx->blocks) dbs.push_back(b.db);
     const int prec = F.dq32 ? 1 : 0;
-    for (auto &g : ctx->phys) launch_dq_zero_face(g, ctx->blocks[g.nb].db, F.rhs, F.total, ctx->stream);
+    if (launch_dq_zero_faces(ctx->d_regs, ctx->d_rwork + ctx->zero_w0, ctx->zero_nw, ctx->d_blocks, F.rhs, nullptr,
+                             F.total, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "zeroing the physical faces");
     if (ctx->tiled_sweeps) {
         {
-            Scope sc(ctx, WCNS_T_SWEEP_STAGE, (int)ctx->blocks.size());
-            for (size_t nb = 0; nb < ctx->blocks.size(); nb++)
-                if (launch_skew_stage(SKEW_STEP, ctx->blocks[nb].db, ctx->skoff[nb], F, prec, ctx->skew[prec],
+            Scope sc(ctx, WCNS_T_SWEEP_STAGE);
+            if (launch_skew_stage_all(SKEW_STEP, ctx->grid[GRID_SKEW], ctx->d_skoff, F, prec, ctx->skew[prec],
                                       ctx->sktot, ctx->stream))
-                    return fail(ctx, WCNS_E_HIP, "staging the sweep operands");
+                return fail(ctx, WCNS_E_HIP, "staging the sweep operands");
         }
         for (int kind = 0; kind < 4; kind++) {
             Scope sc(ctx, WCNS_T_SWEEP_LU_L + kind);
@@ -1196,17 +1333,18 @@ int wcns_l_h_s(wcns_ctx *ctx)
             }
         }
     }
-    for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
-        const DBlock &db = ctx->blocks[nb].db;
-        for (auto &g : ctx->phys)
-            if (g.nb == (int)nb) {
-                if (F.dq32) launch_dq_zero_face32(g, db, F.dq32, F.total, ctx->stream);
-                else launch_dq_zero_face(g, db, F.dq, F.total, ctx->stream);
-            }
-        if (launch_fill_u64(ctx->counters + 3, 1, 0ull, ctx->stream)) return fail(ctx, WCNS_E_HIP, "counter reset");
+    {
         Scope sc(ctx, WCNS_T_UPDATE);
-        if (launch_update(db, F, ctx->dp, ctx->d_flag, ctx->flag_cap, ctx->stream)) return fail(ctx, WCNS_E_HIP, "update");
-        if (launch_repair(db, F, ctx->dp, ctx->d_flag, ctx->flag_cap, ctx->stream)) return fail(ctx, WCNS_E_HIP, "repair");
+        if (launch_dq_zero_faces(ctx->d_regs, ctx->d_rwork + ctx->zero_w0, ctx->zero_nw, ctx->d_blocks, F.dq, F.dq32,
+                                 F.total, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "zeroing the physical faces");
+        if (launch_fill_u64(ctx->d_nflag, (long)ctx->blocks.size(), 0ull, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "counter reset");
+        if (launch_update(ctx->grid[GRID_NODES], F, ctx->dp, ctx->d_flag, ctx->d_nflag, ctx->flag_cap, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "update");
+        if (launch_repair(ctx->d_blocks, (int)ctx->blocks.size(), F, ctx->dp, ctx->d_flag, ctx->d_nflag,
+                          ctx->flag_cap, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "repair");
     }
     ctx->cur ^= 1;
     return WCNS_OK;
@@ -1401,10 +1539,9 @@ int wcns_autotune(wcns_ctx *ctx, int reps, int *tj, int *tk, double *ms)
     DFields F = ctx->fields();
     const int prec = F.dq32 ? 1 : 0;
     // the sweeps read the plane-major copy of the current step's operands
-    for (size_t nb = 0; nb < ctx->blocks.size(); nb++)
-        if (launch_skew_stage(SKEW_STEP, ctx->blocks[nb].db, ctx->skoff[nb], F, prec, ctx->skew[prec], ctx->sktot,
+    if (launch_skew_stage_all(SKEW_STEP, ctx->grid[GRID_SKEW], ctx->d_skoff, F, prec, ctx->skew[prec], ctx->sktot,
                               ctx->stream))
-            return fail(ctx, WCNS_E_HIP, "staging the sweep operands");
+        return fail(ctx, WCNS_E_HIP, "staging the sweep operands");
     const auto saved = ctx->tile_shape;
     double best = 1e300;
     int bj = 0, bk = 0;

@@ -61,7 +61,8 @@ enum {
     GRID_RHS_VIS_T = 14,          // viscous i pencils, likewise
     GRID_VFACE = 15,              // + d (0: i on the transposed view, 1: j, 2: k): viscous half nodes
     GRID_VROW = 18,               // + d: viscous FLUX_DXYZ rows
-    GRID_N = 21
+    GRID_SKEW = 21,               // plane-major staging tiles
+    GRID_N = 22
 };
 long grid_threads(int kind, const DBlock &b);
 int grid_block_size(int kind);
@@ -99,12 +100,16 @@ int launch_fill_u64(unsigned long long *a, long n, unsigned long long v, hipStre
 enum { SK_P = 0, SK_M = 5, SK_SR = 14, SK_SRV = 17, SK_RHS = 20, SK_COED = 25, SK_A = 26, SK_B = 31, SK_N = 36 };
 enum { SKEW_STEP = 0, SKEW_METRICS = 1 };
 long skew_size(const DBlock &b);
+long skew_stage_wgs(const DBlock &b);
+// the staging of every block in one launch over G = the GRID_SKEW map;
+// skoff: each block's offset in the plane-major arrays (device)
+int launch_skew_stage_all(int what, const BlockGrid &G, const long *skoff, const DFields &F, int prec, void *pool,
+                          long sktot, hipStream_t s);
 int launch_skew_stage(int what, const DBlock &b, long soff, const DFields &F, int prec, void *pool, long sktot,
                       hipStream_t s);
 int launch_sweep_batched(int kind, const SweepPlan &S, const DFields &F, const DParams &P, int prec, void *pool,
                          long sktot, hipStream_t s);
 int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s);
-int launch_dq_zero_face32(const DRegion &g, const DBlock &b, float *arr, long total, hipStream_t s);
 
 int launch_vis_node_deriv(const BlockGrid &G, const DFields &F, double *dn, hipStream_t s);
 int launch_rhs_vis_dir(int dir, const BlockGrid &G, const DFields &F, const DParams &P, const double *dn, hipStream_t s);
@@ -120,15 +125,29 @@ int launch_init_stage(const BlockGrid &nodes, const BlockGrid *ghost_t, const DF
                       hipStream_t s);
 int launch_timestep(const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s);
 int launch_zero_rhs(const DBlock &b, const DFields &F, hipStream_t s);
-int launch_dq_zero_face(const DRegion &g, const DBlock &b, double *arr, long total, hipStream_t s);
 int launch_npp_gather(const int *pts, int npts, const DBlock *dblocks, const DFields &F, double *buf,
                       int nvar, int use_vol, hipStream_t s);
 int launch_avg2pm(const DRegion &g, const DBlock *blocks, const DFields &F, int pv, hipStream_t s);
 int launch_npp_assign(const int *grp_start, const int *grp_n, const int *pts, const int *write_ok, int ngroups,
                       const double *buf, const DBlock *dblocks, const DFields &F, int nvar, int use_vol,
                       hipStream_t s);
-int launch_update(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s);
-int launch_repair(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s);
+// the limited update of every block, then the ordered repair of each block's
+// flagged nodes (flag_list holds cap entries per block, nflag one count each)
+int launch_update(const BlockGrid &nodes, const DFields &F, const DParams &P, int *flag_list,
+                  unsigned long long *nflag, int cap, hipStream_t s);
+int launch_repair(const DBlock *blocks, int nblocks, const DFields &F, const DParams &P, int *flag_list,
+                  const unsigned long long *nflag, int cap, hipStream_t s);
+// One dependency level of the boundary conditions / of the interface
+// averaging, and the dQ = 0 of every physical face: work[w] = (region, first
+// point), or (region, -1) for a region run in order on one thread; REGION_CHUNK
+// points per workgroup.
+constexpr int REGION_CHUNK = 128;
+int launch_bc_level(const DRegion *regs, const int2 *work, int nwork, const DBlock *blocks, const DFields &F,
+                    const DParams &P, hipStream_t s);
+int launch_avg2pm_level(const DRegion *regs, const int2 *work, int nwork, const DBlock *blocks, const DFields &F,
+                        int pv, hipStream_t s);
+int launch_dq_zero_faces(const DRegion *regs, const int2 *work, int nwork, const DBlock *blocks, double *arr,
+                         float *arr32, long total, hipStream_t s);
 int launch_residual(const DBlock &b, const DFields &F, double *partial, int nparts, hipStream_t s);
 int residual_parts(const DBlock &b);
 int launch_residual_final(const double *partial, int nparts, double *out, hipStream_t s);

@@ -222,6 +222,32 @@ __global__ void k_bc_region(DRegion g, DBlock b, DBlock bt, DFields F, DParams P
     bc_point(g, b, bt, F, P, i, j, k);
 }
 
+// One dependency level of a region sequence in one launch.  Workgroup w runs
+// points work[w].y .. work[w].y + blockDim.x - 1 of region work[w].x, or, with
+// work[w].y < 0, every point of that region in order on one thread (a region
+// that reads its own writes).  The host puts regions that touch no node
+// another region of the level writes on one level.
+__global__ void k_bc_batch(const DRegion *regs, const int2 *work, const DBlock *blocks, DFields F, DParams P)
+{
+    const int2 w = work[blockIdx.x];
+    const DRegion g = regs[w.x];
+    const DBlock b = blocks[g.nb];
+    const DBlock bt = blocks[g.bctype < 0 ? g.nbt : g.nb];
+    int i, j, k;
+    if (w.y < 0) {
+        if (threadIdx.x != 0) return;
+        for (int t = 0; t < g.npts; t++) {
+            region_point(g, t, i, j, k);
+            bc_point(g, b, bt, F, P, i, j, k);
+        }
+        return;
+    }
+    const int t = w.y + (int)threadIdx.x;
+    if (t >= g.npts) return;
+    region_point(g, t, i, j, k);
+    bc_point(g, b, bt, F, P, i, j, k);
+}
+
 // --------------------------------------------------------- init stage
 // set_dq_to_0 (PreProcess.f90:1189-1205) on the residual; with nvis == 1 also
 // get_c_t_only's temperature (RHS.f90:307-409) and compute_visl_ns
@@ -317,28 +343,6 @@ __global__ void k_timestep(BlockGrid G, DFields F, DParams P)
 }
 
 // ------------------------------------------------- interface averaging
-__global__ void k_dq_zero_face32(DRegion g, DBlock b, float *arr, long total)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.npts) return;
-    int i, j, k;
-    region_point(g, t, i, j, k);
-    long q = gidx(b, i, j, k);
-#pragma unroll
-    for (int m = 0; m < 5; m++) arr[m * total + q] = 0.0f;
-}
-
-__global__ void k_dq_zero_face(DRegion g, DBlock b, double *arr, long total)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.npts) return;
-    int i, j, k;
-    region_point(g, t, i, j, k);
-    long q = gidx(b, i, j, k);
-#pragma unroll
-    for (int m = 0; m < 5; m++) arr[m * total + q] = 0.0;
-}
-
 // parallel.f90:3028-3036 / 3174-3188: pre-averaging values of the multi-points
 __global__ void k_npp_gather(const int *pts, int npts, const DBlock *blocks, DFields F, double *buf,
                              int nvar, int use_vol)
@@ -400,6 +404,43 @@ __global__ void k_avg2pm(DRegion g, DBlock b, DBlock bt, DFields F, int pv)
     avg2pm_point(g, b, bt, F, pv, i, j, k);
 }
 
+__global__ void k_avg2pm_batch(const DRegion *regs, const int2 *work, const DBlock *blocks, DFields F, int pv)
+{
+    const int2 w = work[blockIdx.x];
+    const DRegion g = regs[w.x];
+    const DBlock b = blocks[g.nb];
+    const DBlock bt = blocks[g.nbt];
+    int i, j, k;
+    if (w.y < 0) {
+        if (threadIdx.x != 0) return;
+        for (int t = 0; t < g.npts; t++) {
+            region_point(g, t, i, j, k);
+            avg2pm_point(g, b, bt, F, pv, i, j, k);
+        }
+        return;
+    }
+    const int t = w.y + (int)threadIdx.x;
+    if (t >= g.npts) return;
+    region_point(g, t, i, j, k);
+    avg2pm_point(g, b, bt, F, pv, i, j, k);
+}
+
+// set_nonconnect_boundary_dq_0 (CIC.f90:2-17) on every physical face at once:
+// the same value everywhere, so the order of the regions does not matter.
+template <typename T>
+__global__ void k_dq_zero_batch(const DRegion *regs, const int2 *work, const DBlock *blocks, T *arr, long total)
+{
+    const int2 w = work[blockIdx.x];
+    const DRegion g = regs[w.x];
+    const int t = w.y + (int)threadIdx.x;
+    if (t >= g.npts) return;
+    int i, j, k;
+    region_point(g, t, i, j, k);
+    const long q = gidx(blocks[g.nb], i, j, k);
+#pragma unroll
+    for (int m = 0; m < 5; m++) arr[m * total + q] = T(0);
+}
+
 // parallel.f90:3122-3164 boundary_match_dq_3pp / :3282-3326 boundary_match_pv_3pp
 __global__ void k_npp_assign(const int *grp_start, const int *grp_n, const int *pts, const int *write_ok,
                              int ngroups, const double *buf, const DBlock *blocks, DFields F, int nvar,
@@ -431,11 +472,15 @@ __global__ void k_npp_assign(const int *grp_start, const int *grp_n, const int *
 // Sub2.f90:603-796 update_limited.  q is prim_to_q(prim) of the step's start
 // (get_q_only, RHS.f90:75-100), recomputed here instead of stored.  New
 // primitives go to the other buffer; nodes outside the limits are listed for
-// the ordered repair below.
-__global__ void k_update(DBlock b, DFields F, DParams P, int *flag_list, int cap)
+// the ordered repair below.  All blocks in one launch; block nb lists its
+// nodes in flag_list[nb * cap ...] and counts them in nflag[nb].
+__global__ void k_update(BlockGrid G, DFields F, DParams P, int *flag_list, unsigned long long *nflag, int cap)
 {
+    DBlock b;
+    long t;
+    grid_thread(G, b, t);
+    const int nb = G.wg[blockIdx.x].x;
     long n = (long)b.ni * b.nj * b.nk;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
     long g = gidx(b, i, j, k);
@@ -461,18 +506,22 @@ __global__ void k_update(DBlock b, DFields F, DParams P, int *flag_list, int cap
     pm = gamam1 * (em - 0.5 * rm * (um * um + vm * vm + wm * wm));
     F.rn[g] = rm; F.un[g] = um; F.vn[g] = vm; F.wn[g] = wm; F.pn[g] = pm;
     if (pm <= PMIN_LIMIT || rm <= RMIN_LIMIT || pm >= PMAX_LIMIT || rm >= RMAX_LIMIT) {
-        unsigned long long slot = atomicAdd(&F.counters[3], 1ull);
-        if (slot < (unsigned long long)cap) flag_list[slot] = (int)t;
+        unsigned long long slot = atomicAdd(&nflag[nb], 1ull);
+        if (slot < (unsigned long long)cap) flag_list[(long)nb * cap + slot] = (int)t;
     }
 }
 
 // Sub2.f90:661-767: a flagged node takes the mean of its six neighbours, those
 // earlier in lexicographic order already updated (new buffer), later ones and
-// ghosts as before the update (old buffer).  One thread, in order.
-__global__ void k_repair(DBlock b, DFields F, DParams P, int *flag_list, int cap)
+// ghosts as before the update (old buffer).  One thread per block, in order;
+// the repair reads only its own block, so the blocks run side by side.
+__global__ void k_repair(const DBlock *blocks, DFields F, DParams P, int *flags, const unsigned long long *nflag,
+                         int cap)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    unsigned long long nf = F.counters[3];
+    if (threadIdx.x != 0) return;
+    const DBlock b = blocks[blockIdx.x];
+    int *flag_list = flags + (long)blockIdx.x * cap;
+    unsigned long long nf = nflag[blockIdx.x];
     if (nf == 0) return;
     int n = (int)(nf < (unsigned long long)cap ? nf : (unsigned long long)cap);
     for (int a = 1; a < n; a++) {          // insertion sort: lexicographic = linear order
@@ -506,7 +555,7 @@ __global__ void k_repair(DBlock b, DFields F, DParams P, int *flag_list, int cap
         long g = gidx(b, i, j, k);
         F.rn[g] = rm; F.un[g] = um; F.vn[g] = vm; F.wn[g] = wm; F.pn[g] = pm;
     }
-    F.counters[1] += nf;
+    atomicAdd(&F.counters[1], nf);
     // Sub2.f90:781-785: too many abnormal nodes stops the run
     if (nf > (unsigned long long)(((long)b.ni * b.nj * b.nk) / 64) || nf >= 500ull) F.counters[4] = 1ull;
 }
@@ -627,6 +676,7 @@ long grid_threads(int kind, const DBlock &b)
     if (kind == GRID_SA_RHS) return b.ni > 2 && b.nj > 2 && b.nk > 2 ? (long)(b.ni - 2) * (b.nj - 2) * (b.nk - 2) : 0;
     if (kind == GRID_SA_UPD) return (long)(b.ni + 1) * (b.nj + 1) * (b.nk + 1);
     if (kind == GRID_TRANS) return transpose_tiles(b) * 256;
+    if (kind == GRID_SKEW) return skew_stage_wgs(b) * 256;
     // b is the transposed view here: its j pencils are the original i pencils
     if (kind == GRID_RHS_INV_T) return rhs_inv_threads(1, b);
     if (kind == GRID_RHS_VIS_T) return rhs_vis_threads(1, b);
@@ -638,7 +688,7 @@ long grid_threads(int kind, const DBlock &b)
 
 int grid_block_size(int kind)
 {
-    return (kind == GRID_NODES || kind == GRID_SA_UPD || kind == GRID_TRANS || kind >= GRID_VFACE) ? 256 : 128;
+    return (kind == GRID_NODES || kind == GRID_SA_UPD || kind == GRID_TRANS || kind >= GRID_VFACE) ? 256 : 128;   // GRID_SKEW too
 }
 
 int launch_zero_rhs(const DBlock &b, const DFields &F, hipStream_t s)
@@ -648,15 +698,29 @@ int launch_zero_rhs(const DBlock &b, const DFields &F, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_dq_zero_face(const DRegion &g, const DBlock &b, double *arr, long total, hipStream_t s)
+int launch_bc_level(const DRegion *regs, const int2 *work, int nwork, const DBlock *blocks, const DFields &F,
+                    const DParams &P, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dq_zero_face, grid1(g.npts, 128), dim3(128), 0, s, g, b, arr, total);
+    if (nwork > 0) hipLaunchKernelGGL(k_bc_batch, dim3(nwork), dim3(REGION_CHUNK), 0, s, regs, work, blocks, F, P);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_dq_zero_face32(const DRegion &g, const DBlock &b, float *arr, long total, hipStream_t s)
+int launch_avg2pm_level(const DRegion *regs, const int2 *work, int nwork, const DBlock *blocks, const DFields &F,
+                        int pv, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dq_zero_face32, grid1(g.npts, 128), dim3(128), 0, s, g, b, arr, total);
+    if (nwork > 0)
+        hipLaunchKernelGGL(k_avg2pm_batch, dim3(nwork), dim3(REGION_CHUNK), 0, s, regs, work, blocks, F, pv);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_dq_zero_faces(const DRegion *regs, const int2 *work, int nwork, const DBlock *blocks, double *arr,
+                         float *arr32, long total, hipStream_t s)
+{
+    if (nwork <= 0) return 0;
+    if (arr32) hipLaunchKernelGGL(k_dq_zero_batch<float>, dim3(nwork), dim3(REGION_CHUNK), 0, s, regs, work, blocks,
+                                  arr32, total);
+    else hipLaunchKernelGGL(k_dq_zero_batch<double>, dim3(nwork), dim3(REGION_CHUNK), 0, s, regs, work, blocks,
+                            arr, total);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -686,16 +750,17 @@ int launch_npp_assign(const int *grp_start, const int *grp_n, const int *pts, co
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_update(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s)
+int launch_update(const BlockGrid &nodes, const DFields &F, const DParams &P, int *flag_list,
+                  unsigned long long *nflag, int cap, hipStream_t s)
 {
-    long n = (long)b.ni * b.nj * b.nk;
-    hipLaunchKernelGGL(k_update, grid1(n, 256), dim3(256), 0, s, b, F, P, flag_list, cap);
+    if (nodes.nwg) hipLaunchKernelGGL(k_update, dim3(nodes.nwg), dim3(256), 0, s, nodes, F, P, flag_list, nflag, cap);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_repair(const DBlock &b, const DFields &F, const DParams &P, int *flag_list, int cap, hipStream_t s)
+int launch_repair(const DBlock *blocks, int nblocks, const DFields &F, const DParams &P, int *flag_list,
+                  const unsigned long long *nflag, int cap, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_repair, dim3(1), dim3(64), 0, s, b, F, P, flag_list, cap);
+    hipLaunchKernelGGL(k_repair, dim3(nblocks), dim3(64), 0, s, blocks, F, P, flag_list, nflag, cap);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

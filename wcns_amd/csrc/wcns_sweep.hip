@@ -710,12 +710,11 @@ struct StageList {
 // STG_G arrays per pass: every thread issues its 4 * STG_G loads before the
 // first LDS write
 constexpr int STG_G = 4;
+// One STG_S x STG_J tile of hyperplanes x rows in plane k of block b.
 template <typename T>
-__global__ void __launch_bounds__(256) k_skew_stage(DBlock b, SkIndex ix, StageList L, T *pool, long sktot)
+__device__ void skew_tile(const DBlock &b, const SkIndex &ix, const StageList &L, T *pool, long sktot, int k, int j0,
+                          int s0, T (&tile)[STG_G][STG_J][STG_S + 1])
 {
-    __shared__ T tile[STG_G][STG_J][STG_S + 1];
-    const int k = blockIdx.z;
-    const int j0 = blockIdx.y * STG_J, s0 = blockIdx.x * STG_S;
     const int t = threadIdx.x;
     for (int a0 = 0; a0 < L.n; a0 += STG_G) {
         // load: row jr = t / 16 handles 4 hyperplanes of each array of the pass
@@ -758,7 +757,34 @@ __global__ void __launch_bounds__(256) k_skew_stage(DBlock b, SkIndex ix, StageL
     }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k_skew_stage(DBlock b, SkIndex ix, StageList L, T *pool, long sktot)
+{
+    __shared__ T tile[STG_G][STG_J][STG_S + 1];
+    skew_tile(b, ix, L, pool, sktot, blockIdx.z, blockIdx.y * STG_J, blockIdx.x * STG_S, tile);
+}
+
+__host__ __device__ inline int skew_tiles_s(const DBlock &b) { return (b.ni + b.nj + b.nk + 4 + STG_S - 1) / STG_S; }
+__host__ __device__ inline int skew_tiles_j(const DBlock &b) { return (b.nj + 2 + STG_J - 1) / STG_J; }
+
+// All blocks in one launch: workgroup w stages tile G.wg[w].y of block
+// G.wg[w].x, numbered hyperplane tile fastest, then row tile, then plane k.
+template <typename T>
+__global__ void __launch_bounds__(256) k_skew_stage_all(BlockGrid G, const long *skoff, StageList L, T *pool,
+                                                        long sktot)
+{
+    __shared__ T tile[STG_G][STG_J][STG_S + 1];
+    const int2 w = G.wg[blockIdx.x];
+    const DBlock b = G.blocks[w.x];
+    const int gs = skew_tiles_s(b), gj = skew_tiles_j(b);
+    const int bs = w.y % gs, bj = (w.y / gs) % gj, k = w.y / (gs * gj);
+    const SkIndex ix{skoff[w.x], (b.nj + 2) * (b.nk + 2), b.nj + 2};
+    skew_tile(b, ix, L, pool, sktot, k, bj * STG_J, bs * STG_S, tile);
+}
+
 }  // namespace
+
+long skew_stage_wgs(const DBlock &b) { return (long)skew_tiles_s(b) * skew_tiles_j(b) * (b.nk + 2); }
 
 int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
 {
@@ -792,8 +818,7 @@ static SkIndex skew_index(const DBlock &b, long soff)
     return SkIndex{soff, (b.nj + 2) * (b.nk + 2), b.nj + 2};
 }
 
-int launch_skew_stage(int what, const DBlock &b, long soff, const DFields &F, int prec, void *pool, long sktot,
-                      hipStream_t s)
+static StageList stage_list(int what, const DFields &F, int prec)
 {
     StageList L{};
     auto add = [&](const void *src, int dst, int f32, int shell) {
@@ -816,11 +841,28 @@ int launch_skew_stage(int what, const DBlock &b, long soff, const DFields &F, in
             else add(F.dq + m * F.total, SK_A + m, 0, 1);
         }
     }
-    int ns = b.ni + b.nj + b.nk + 4;
-    dim3 g((ns + STG_S - 1) / STG_S, (b.nj + 2 + STG_J - 1) / STG_J, b.nk + 2);
+    return L;
+}
+
+int launch_skew_stage(int what, const DBlock &b, long soff, const DFields &F, int prec, void *pool, long sktot,
+                      hipStream_t s)
+{
+    const StageList L = stage_list(what, F, prec);
+    dim3 g(skew_tiles_s(b), skew_tiles_j(b), b.nk + 2);
     SkIndex ix = skew_index(b, soff);
     if (prec == 1) hipLaunchKernelGGL(k_skew_stage<float>, g, dim3(256), 0, s, b, ix, L, (float *)pool, sktot);
     else hipLaunchKernelGGL(k_skew_stage<double>, g, dim3(256), 0, s, b, ix, L, (double *)pool, sktot);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_skew_stage_all(int what, const BlockGrid &G, const long *skoff, const DFields &F, int prec, void *pool,
+                          long sktot, hipStream_t s)
+{
+    const StageList L = stage_list(what, F, prec);
+    if (G.nwg == 0) return 0;
+    if (prec == 1)
+        hipLaunchKernelGGL(k_skew_stage_all<float>, dim3(G.nwg), dim3(256), 0, s, G, skoff, L, (float *)pool, sktot);
+    else hipLaunchKernelGGL(k_skew_stage_all<double>, dim3(G.nwg), dim3(256), 0, s, G, skoff, L, (double *)pool, sktot);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
