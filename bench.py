@@ -248,7 +248,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.lhs == "fp64" else "f64+f32-lhs",
         "data": "synthetic (seeded generator wcns_amd.cases)",
         "config": {"workload": wl_name, "nodes": nodes, "blocks": len(case.blocks), "lhs": args.lhs,
-                   "launch": args.launch, "sweep_tile": list(tile[:2]) if tile else None, "residual_rms_end": rms},
+                   "launch": args.launch, "sweep_tile": list(tile[:2]) if tile else None,
+                   "sgs_paired_helpers": s.sweep_layout()[2], "residual_rms_end": rms},
         "roofline": roof, "cpu_baseline": cpu,
         "step_hbm_frac": round(STEP_BYTES_PER_NODE * nodes * args.steps / sec / (HBM_PEAK_GBS * 1e9), 5),
         "stage_ms_per_step": {k: round(v[0] / kt, 4) for k, v in tim.items()},

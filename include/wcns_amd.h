@@ -127,10 +127,17 @@ int wcns_step_graph(wcns_ctx *ctx, int nsteps);
 /* Tile-shape autotuner of the pipelined sweeps (no counterpart in the
  * reference): times the four sweeps of one step, reps times, for each
  * candidate (j, k) tile shape of at most 64 columns on the current state and
- * keeps the fastest for every block (the SA sweeps share it).  Leaves the
+ * keeps the fastest for every block (the SA sweeps share it).  For the SGS
+ * pair it also chooses between six helper wavefronts per tile and three that
+ * form two operand parts each (two tiles per CU), unless the environment
+ * variable WCNS_SGS_PAIR (0 or 1) fixed that at wcns_create.  Leaves the
  * primitive, SA and residual fields unchanged; the sweep unknowns dq are
  * scratch.  Returns the shape chosen and its time per sweep set in ms. */
 int wcns_autotune(wcns_ctx *ctx, int reps, int *tj, int *tk, double *ms);
+
+/* The pipelined sweeps' layout in use (no counterpart in the reference): the
+ * first block's tile shape and whether the SGS helpers form two parts each. */
+int wcns_sweep_layout(wcns_ctx *ctx, int *tj, int *tk, int *sgs_paired);
 
 /* Force and moment coefficients over every no-slip wall (bctype 2 or 2x):
  * out = cfx, cfy, cfz, cmx, cmy, cmz, cd, cl, xcp.  Replaces force + force_dif

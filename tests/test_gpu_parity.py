@@ -177,6 +177,26 @@ def test_tiled_sweeps_match_plane_sweeps(monkeypatch):
         assert np.array_equal(a.field(0, name), b.field(0, name))
 
 
+@pytest.mark.parametrize("lhs", [0, 1], ids=["fp64", "fp32"])
+def test_paired_sgs_helpers_match_unpaired(monkeypatch, lhs):
+    """The SGS sweeps with three helper wavefronts forming two operand parts
+    each give the bits of six helpers forming one, on an uneven multi-block SA
+    case, in both implicit-operator precisions."""
+    from wcns_amd.solver import Solver
+    case = _uneven_sa()
+    monkeypatch.setenv("WCNS_SGS_PAIR", "0")
+    a = Solver(case, lhs_precision=lhs)
+    monkeypatch.setenv("WCNS_SGS_PAIR", "1")
+    b = Solver(case, lhs_precision=lhs)
+    assert not a.sweep_layout()[2] and b.sweep_layout()[2]
+    ha = a.step(4)
+    hb = b.step(4)
+    assert np.array_equal(ha, hb)
+    for nb in range(len(case.blocks)):
+        for name in ("r", "u", "v", "w", "p"):
+            assert np.array_equal(a.field(nb, name), b.field(nb, name)), f"block {nb} {name}"
+
+
 def test_mixed_precision_lhs_converges_like_fp64():
     """FP32 implicit operator (FP64 residual, weights and update): same residual
     drop within a few steps of the FP64 operator on the same case."""

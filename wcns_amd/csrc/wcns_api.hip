@@ -75,6 +75,7 @@ struct wcns_ctx {
     long *d_skoff = nullptr;                // the same, on the device
     long sktot = 0;
     SweepPlan plan{};                       // batched pipelined sweeps over all blocks
+    bool sgs_forced = false;                // WCNS_SGS_PAIR fixes plan.sgs_paired; the autotuner keeps it
     std::vector<std::pair<int, int>> tile_shape;   // per block (TJ, TK)
     SweepBlock *d_sblocks = nullptr;
     int2 *d_work = nullptr;                 // forward list, then backward list
@@ -455,6 +456,7 @@ int wcns_create(wcns_ctx **out, int device, const wcns_params *prm)
     if (const char *sw = getenv("WCNS_SWEEP")) ctx->tiled_sweeps = strcmp(sw, "plane") != 0;
     if (const char *id = getenv("WCNS_IDIR")) ctx->idir_t = strcmp(id, "direct") != 0;
     if (const char *sf = getenv("WCNS_SURFACE")) ctx->surf_levels = strcmp(sf, "sequence") != 0;
+    if (const char *sp = getenv("WCNS_SGS_PAIR")) { ctx->plan.sgs_paired = atoi(sp) != 0; ctx->sgs_forced = true; }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return WCNS_E_HIP;
@@ -1530,6 +1532,15 @@ int wcns_get_counter(wcns_ctx *ctx, int which, long long *value)
     return WCNS_OK;
 }
 
+int wcns_sweep_layout(wcns_ctx *ctx, int *tj, int *tk, int *sgs_paired)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (tj) *tj = ctx->tile_shape.empty() ? 0 : ctx->tile_shape[0].first;
+    if (tk) *tk = ctx->tile_shape.empty() ? 0 : ctx->tile_shape[0].second;
+    if (sgs_paired) *sgs_paired = ctx->plan.sgs_paired;
+    return WCNS_OK;
+}
+
 int wcns_autotune(wcns_ctx *ctx, int reps, int *tj, int *tk, double *ms)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
@@ -1543,11 +1554,26 @@ int wcns_autotune(wcns_ctx *ctx, int reps, int *tj, int *tk, double *ms)
                               ctx->stream))
         return fail(ctx, WCNS_E_HIP, "staging the sweep operands");
     const auto saved = ctx->tile_shape;
+    const int saved_pair = ctx->plan.sgs_paired;
     double best = 1e300;
-    int bj = 0, bk = 0;
+    int bj = 0, bk = 0, bpair = saved_pair;
     hipEvent_t e0, e1;
     CK(hipEventCreate(&e0));
     CK(hipEventCreate(&e1));
+    // ms per repetition of sweep kinds k0..k1
+    auto time_kinds = [&](int k0, int k1, float &ms_out) -> int {
+        CK(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps; r++)
+            for (int kind = k0; kind <= k1; kind++)
+                if (launch_sweep_batched(kind, ctx->plan, F, ctx->dp, prec, ctx->skew[prec], ctx->sktot, ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "sweeps");
+        CK(hipEventRecord(e1, ctx->stream));
+        CK(hipEventSynchronize(e1));
+        float t = 0.f;
+        CK(hipEventElapsedTime(&t, e0, e1));
+        ms_out = t / reps;
+        return check_counters(ctx);
+    };
     for (auto &c : cand) {
         ctx->tile_shape.clear();
         for (auto &b : ctx->blocks) ctx->tile_shape.push_back({std::min(c[0], b.nj), std::min(c[1], b.nk)});
@@ -1555,18 +1581,21 @@ int wcns_autotune(wcns_ctx *ctx, int reps, int *tj, int *tk, double *ms)
         for (int kind = 0; kind < 4; kind++)      // warm-up
             if (launch_sweep_batched(kind, ctx->plan, F, ctx->dp, prec, ctx->skew[prec], ctx->sktot, ctx->stream))
                 return fail(ctx, WCNS_E_HIP, "sweeps");
-        CK(hipEventRecord(e0, ctx->stream));
-        for (int r = 0; r < reps; r++)
-            for (int kind = 0; kind < 4; kind++)
-                if (launch_sweep_batched(kind, ctx->plan, F, ctx->dp, prec, ctx->skew[prec], ctx->sktot, ctx->stream))
-                    return fail(ctx, WCNS_E_HIP, "sweeps");
-        CK(hipEventRecord(e1, ctx->stream));
-        CK(hipEventSynchronize(e1));
-        float t = 0.f;
-        CK(hipEventElapsedTime(&t, e0, e1));
-        TRY(check_counters(ctx));
-        if (t / reps < best) { best = t / reps; bj = c[0]; bk = c[1]; }
+        float t_lu = 0.f;
+        TRY(time_kinds(0, 1, t_lu));
+        // the SGS pair with seven wavefronts per tile, and with four (two per CU)
+        for (int pair = 0; pair < 2; pair++) {
+            if (ctx->sgs_forced && pair != saved_pair) continue;
+            ctx->plan.sgs_paired = pair;
+            float t_sgs = 0.f;
+            if (launch_sweep_batched(2, ctx->plan, F, ctx->dp, prec, ctx->skew[prec], ctx->sktot, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "sweeps");
+            TRY(time_kinds(2, 3, t_sgs));
+            if (t_lu + t_sgs < best) { best = t_lu + t_sgs; bj = c[0]; bk = c[1]; bpair = pair; }
+        }
+        ctx->plan.sgs_paired = saved_pair;
     }
+    ctx->plan.sgs_paired = bpair;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     ctx->tile_shape.clear();

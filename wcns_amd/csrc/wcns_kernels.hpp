@@ -86,6 +86,7 @@ struct SweepPlan {
     long nxch;
     int *err;                     // set when a bounded wait expires
     long spin_limit;
+    int sgs_paired;               // 1: SGS sweeps with three helper wavefronts of two parts each
 };
 long sweep_exchange_words(int TJ, int TK, int nTJ, int nTK, int ni);
 // An exchange word nobody has written yet: a signalling-NaN pattern that no

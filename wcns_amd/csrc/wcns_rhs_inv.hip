@@ -41,6 +41,10 @@ struct InvIO {
     }
 };
 
+// InvIO for the two-wave register budget: a type of its own, so that the
+// boundary functions it calls (not inlined) are compiled once per budget.
+struct InvIO2w : InvIO {};
+
 struct InvTanIO {
     using T = D1;
     static constexpr int SEGLEN = SEG;
@@ -480,9 +484,9 @@ __device__ inline void metric_triple(const DFields &F, const double *&mx, const 
     mz = DIR == 0 ? F.kcz : (DIR == 1 ? F.etz : F.ctz);
 }
 
-// DIR0 = 3: the i direction run as j pencils of the transposed view
-template <int DIR0, int FLUX>
-__global__ void __launch_bounds__(128) k_rhs_inv(BlockGrid G, DFields F, DParams P)
+// DIR0 = 3: the i direction run as j pencils of the transposed view.
+template <int DIR0, int FLUX, class IO>
+__device__ inline void rhs_inv_thread(const BlockGrid &G, const DFields &F, const DParams &P)
 {
     constexpr int DIR = DIR0 == 3 ? 1 : DIR0;
     DBlock b;
@@ -499,7 +503,7 @@ __global__ void __launch_bounds__(128) k_rhs_inv(BlockGrid G, DFields F, DParams
     int ia = (int)(pen % na) + 1;
     int ib = (int)(pen / na) + 1;
 
-    Pencil<InvIO> pc;
+    Pencil<IO> pc;
     pc.io.r = F.r; pc.io.u = F.u; pc.io.v = F.v; pc.io.w = F.w; pc.io.p = F.p;
     metric_triple<DIR>(F, pc.io.mx, pc.io.my, pc.io.mz);
     pc.io.rhs = F.rhs;
@@ -514,6 +518,23 @@ __global__ void __launch_bounds__(128) k_rhs_inv(BlockGrid G, DFields F, DParams
     inv_pencil<FLUX>(pc, a, bb, P.gama, P.efix, P.nscheme == 41, nerr, bad);
     if (nerr) atomicAdd(&F.counters[0], (unsigned long long)nerr);
     if (bad) atomicAdd(&F.counters[2], 1ull);
+}
+
+// Two register budgets of the same walk.  k_rhs_inv keeps everything in
+// registers (one wave per SIMD); k_rhs_inv_2w fits 256 registers, spilling a
+// few, so that two waves share a SIMD -- which pays only when the launch has
+// more than one wave per SIMD to give.
+template <int DIR0, int FLUX>
+__global__ void __launch_bounds__(128) k_rhs_inv(BlockGrid G, DFields F, DParams P)
+{
+    rhs_inv_thread<DIR0, FLUX, InvIO>(G, F, P);
+}
+
+template <int DIR0, int FLUX>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k_rhs_inv_2w(BlockGrid G, DFields F,
+                                                                                            DParams P)
+{
+    rhs_inv_thread<DIR0, FLUX, InvIO2w>(G, F, P);
 }
 
 // Tangent: dR += (dR_inv/dP) dP + (dR_inv/dM) dM along DIR for one block, the
@@ -595,21 +616,42 @@ long rhs_inv_threads(int dir, const DBlock &b)
     return npen * ((n + SEG - 1) / SEG);
 }
 
+template <int FLUX>
+static void launch_inv(int dir, bool two_waves, dim3 grid, dim3 blk, hipStream_t s, const BlockGrid &G,
+                       const DFields &F, const DParams &P)
+{
+    if (two_waves) {
+        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv_2w<0, FLUX>), grid, blk, 0, s, G, F, P);
+        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv_2w<1, FLUX>), grid, blk, 0, s, G, F, P);
+        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv_2w<2, FLUX>), grid, blk, 0, s, G, F, P);
+        if (dir == 3) hipLaunchKernelGGL((k_rhs_inv_2w<3, FLUX>), grid, blk, 0, s, G, F, P);
+    } else {
+        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, FLUX>), grid, blk, 0, s, G, F, P);
+        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, FLUX>), grid, blk, 0, s, G, F, P);
+        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, FLUX>), grid, blk, 0, s, G, F, P);
+        if (dir == 3) hipLaunchKernelGGL((k_rhs_inv<3, FLUX>), grid, blk, 0, s, G, F, P);
+    }
+}
+
+// SIMDs of the current device (four per CU)
+static int simd_count()
+{
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || ncu < 1)
+        ncu = 256;
+    return 4 * ncu;
+}
+
 int launch_rhs_inv_dir(int dir, const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s)
 {
     if (G.nwg == 0) return 0;
     dim3 grid((unsigned)G.nwg), blk(128);
-    if (P.nflux == 3) {
-        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 3>), grid, blk, 0, s, G, F, P);
-        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 3>), grid, blk, 0, s, G, F, P);
-        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 3>), grid, blk, 0, s, G, F, P);
-        if (dir == 3) hipLaunchKernelGGL((k_rhs_inv<3, 3>), grid, blk, 0, s, G, F, P);
-    } else {
-        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv<0, 1>), grid, blk, 0, s, G, F, P);
-        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv<1, 1>), grid, blk, 0, s, G, F, P);
-        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv<2, 1>), grid, blk, 0, s, G, F, P);
-        if (dir == 3) hipLaunchKernelGGL((k_rhs_inv<3, 1>), grid, blk, 0, s, G, F, P);
-    }
+    // two 64-wide waves per workgroup; the two-wave budget once they exceed
+    // 1.5 waves per SIMD
+    const bool two = 2L * G.nwg * 2 > 3L * simd_count();
+    if (P.nflux == 3) launch_inv<3>(dir, two, grid, blk, s, G, F, P);
+    else launch_inv<1>(dir, two, grid, blk, s, G, F, P);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

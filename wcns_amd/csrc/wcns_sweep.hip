@@ -389,29 +389,35 @@ constexpr int PIPE_MAX_THREADS = 7 * PIPE_COLS;  // chain + up to six helper wav
 
 // Operands of one tile's next hyperplane, formed by the helper wavefronts:
 // two slots, structure of arrays over the tile's columns.  The 13 stored
-// fields of a coefficient set are MxCoef without ae (= gama - 1).
-template <typename T>
+// fields of a coefficient set are MxCoef without ae (= gama - 1).  Each sweep
+// kind sizes only what it reads (a one-element stand-in otherwise): R for all
+// but lusgs_u, which reads its own dQ instead; the dQ outside the block and
+// the viscous radii for LU; the previous-sweep terms for SGS.  In FP64 the LU
+// kinds then take 70 KB, so that two workgroups fit in a CU's 160 KB.
+template <int KIND, typename T>
 struct PipeLds {
+    static constexpr int NR = KIND == 1 ? 1 : 5, NO = KIND == 1 ? 5 : 1;
+    static constexpr int NG = KIND < 2 ? 3 : 1, NF = KIND < 2 ? 1 : 3, NV = KIND < 2 ? 3 : 1;
     T cn[2][3][13][PIPE_COLS];
-    T fold[2][3][5][PIPE_COLS];
-    T rhs[2][5][PIPE_COLS];
+    T fold[2][NF][5][PIPE_COLS];
+    T rhs[2][NR][PIPE_COLS];
     T coed[2][PIPE_COLS];
-    T ghost[2][3][5][PIPE_COLS];
-    T own[2][5][PIPE_COLS];
-    T rv[2][3][PIPE_COLS];
+    T ghost[2][NG][5][PIPE_COLS];
+    T own[2][NO][PIPE_COLS];
+    T rv[2][NV][PIPE_COLS];
     T sh[2][5][PIPE_COLS];       // this step's results, for the j and k neighbours in the tile
 };
 
-template <typename T>
-__device__ inline void coef_put(PipeLds<T> &L, int slot, int d, int c, const MxCoef<T> &x)
+template <int KIND, typename T>
+__device__ inline void coef_put(PipeLds<KIND, T> &L, int slot, int d, int c, const MxCoef<T> &x)
 {
     const T v[13] = {x.l1, x.x1, x.x2, x.nx, x.ny, x.nz, x.ct, x.af, x.c2, x.um, x.vm, x.wm, x.hm};
 #pragma unroll
     for (int f = 0; f < 13; f++) L.cn[slot][d][f][c] = v[f];
 }
 
-template <typename T>
-__device__ inline void coef_get(const PipeLds<T> &L, int slot, int d, int c, T gama, MxCoef<T> &x)
+template <int KIND, typename T>
+__device__ inline void coef_get(const PipeLds<KIND, T> &L, int slot, int d, int c, T gama, MxCoef<T> &x)
 {
     x.l1 = L.cn[slot][d][0][c]; x.x1 = L.cn[slot][d][1][c]; x.x2 = L.cn[slot][d][2][c];
     x.nx = L.cn[slot][d][3][c]; x.ny = L.cn[slot][d][4][c]; x.nz = L.cn[slot][d][5][c];
@@ -478,33 +484,39 @@ __device__ inline void task_load(int p, const DBlock &b, const Operands<S> &O, c
 
 template <int KIND, typename T>
 __device__ inline void task_prep(int p, const DParams &P, int i, int j, int k, const DBlock &b, const TaskRaw<T> &r,
-                                 PipeLds<T> &L, int slot, int c)
+                                 PipeLds<KIND, T> &L, int slot, int c)
 {
     constexpr int off = sweep_off<KIND>();
     const T gama = (T)P.gama;
     if (p < 3) {
         MxCoef<T> x;
         mxdq_coef<T>(r.nb.prim, r.nb.gy, r.nb.rad, off < 0 ? T(1.0) : T(-1.0), gama, x);
-        coef_put<T>(L, slot, p, c, x);
+        coef_put(L, slot, p, c, x);
         if (p == 0) {
+            if constexpr (KIND != 1) {
 #pragma unroll
-            for (int m = 0; m < 5; m++) L.rhs[slot][m][c] = r.x[m];
+                for (int m = 0; m < 5; m++) L.rhs[slot][m][c] = r.x[m];
+            }
             L.coed[slot][c] = r.x[5];
-        } else if (p == 1 && KIND == 1) {
-#pragma unroll
-            for (int m = 0; m < 5; m++) L.own[slot][m][c] = r.x[m];
-        } else if (p == 2 && KIND < 2 && P.nvis > 0) {
-#pragma unroll
-            for (int d = 0; d < 3; d++) L.rv[slot][d][c] = r.x[d];
         }
-        if (KIND < 2) {
+        if constexpr (KIND == 1) {
+            if (p == 1) {
+#pragma unroll
+                for (int m = 0; m < 5; m++) L.own[slot][m][c] = r.x[m];
+            }
+        }
+        if constexpr (KIND < 2) {
+            if (p == 2 && P.nvis > 0) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) L.rv[slot][d][c] = r.x[d];
+            }
             int cc = (p == 0 ? i : (p == 1 ? j : k)) + off;
             if (cc < 1 || cc > (p == 0 ? b.ni : (p == 1 ? b.nj : b.nk))) {
 #pragma unroll
                 for (int m = 0; m < 5; m++) L.ghost[slot][p][m][c] = r.x[6 + m];
             }
         }
-    } else {
+    } else if constexpr (KIND >= 2) {
         MxCoef<T> x;
         mxdq_coef<T>(r.nb.prim, r.nb.gy, r.nb.rad, -off < 0 ? T(1.0) : T(-1.0), gama, x);
         T f[5];
@@ -514,21 +526,30 @@ __device__ inline void task_prep(int p, const DParams &P, int i, int j, int k, c
     }
 }
 
+// dQ of the neighbour outside the block: the staged ghost value for LU, 0 for SGS
+template <int KIND, typename T>
+__device__ inline T ghost_dq(const PipeLds<KIND, T> &L, int slot, int d, int m, int c)
+{
+    if constexpr (KIND < 2) return L.ghost[slot][d][m][c];
+    else return T(0.0);
+}
+
 // node_compute with the operands read from the LDS slot as they are needed
 // (the same operations in the same order).
 template <int KIND, typename T>
-__device__ inline void chain_compute(const DParams &P, const PipeLds<T> &L, int slot, int c, T (&nw)[3][5], T *out)
+__device__ inline void chain_compute(const DParams &P, const PipeLds<KIND, T> &L, int slot, int c, T (&nw)[3][5],
+                                     T *out)
 {
     const T gama = (T)P.gama;
     T f[3][5];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         MxCoef<T> x;
-        coef_get<T>(L, slot, d, c, gama, x);
+        coef_get(L, slot, d, c, gama, x);
         mxdq_apply<T>(x, nw[d], f[d]);
     }
     const T coed = L.coed[slot][c];
-    if (KIND < 2) {
+    if constexpr (KIND < 2) {
         T rhs0[5];
 #pragma unroll
         for (int m = 0; m < 5; m++) rhs0[m] = T(1.0) * (f[0][m] + f[1][m] + f[2][m]);
@@ -541,8 +562,10 @@ __device__ inline void chain_compute(const DParams &P, const PipeLds<T> &L, int 
             }
         }
 #pragma unroll
-        for (int m = 0; m < 5; m++)
-            out[m] = KIND == 0 ? (-L.rhs[slot][m][c] + rhs0[m]) * coed : L.own[slot][m][c] - rhs0[m] * coed;
+        for (int m = 0; m < 5; m++) {
+            if constexpr (KIND == 0) out[m] = (-L.rhs[slot][m][c] + rhs0[m]) * coed;
+            else out[m] = L.own[slot][m][c] - rhs0[m] * coed;
+        }
     } else {
 #pragma unroll
         for (int m = 0; m < 5; m++) {
@@ -563,17 +586,20 @@ __device__ inline void chain_compute(const DParams &P, const PipeLds<T> &L, int 
 // LDS and the neighbours' new dQ.  Wavefronts 1..H are helpers: helper lane
 // (c, p) forms part p of column c's operands for the next hyperplane (H = 3
 // for LU, 6 for SGS) and, meanwhile, loads those of the hyperplane after.
+// With PAIR (SGS only), three helpers form parts p and p + 3 each: four
+// wavefronts instead of seven, so that two workgroups fit in a CU.
 // O: the plane-major operands (every block at its own offset); A, B: the
 // plane-major dQ pair; OUT: the field-pool dQ that gs_pr_u (the last sweep of
 // the step) writes for the update.
-template <int KIND, typename T>
+template <int KIND, typename T, bool PAIR>
 __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBlock *blocks, const int2 *work,
                                                                  Operands<T> O, TPtrs<T> A, TPtrs<T> B,
                                                                  TPtrs<T> OUT, DParams P, int *err, long spin_limit)
 {
+    static_assert(!PAIR || KIND >= 2, "paired helpers are for the SGS sweeps");
     constexpr bool fwd = (KIND == 0 || KIND == 2);
     constexpr int off = sweep_off<KIND>();
-    constexpr int H = KIND < 2 ? 3 : 6;
+    constexpr int H = (KIND < 2 || PAIR) ? 3 : 6;
     const int2 wk = work[blockIdx.x];
     const SweepBlock sb = blocks[wk.x];
     const DBlock b = sb.b;
@@ -598,7 +624,7 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
     auto live = [&](int st) { int ii = i_at(st); return col && st < nsteps && ii >= 1 && ii <= ni; };
     const T gama = (T)P.gama;
 
-    __shared__ PipeLds<T> L;
+    __shared__ PipeLds<KIND, T> L;
 
     // chain-wavefront state
     const int jsrc = fwd ? (jl > 0 ? 0 : (J > 0 ? 1 : 2)) : (jl < jn - 1 ? 0 : (J < nTJ - 1 ? 1 : 2));
@@ -616,14 +642,19 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
     bool bad = false;                 // a bounded wait of this lane expired: stop waiting
 
     // helper state: the raw operands of the hyperplane after next
-    TaskRaw<T> raw;
+    TaskRaw<T> raw, raw2;
     const bool helper = p >= 0 && p < H;
     if (helper) {
         if (live(0)) {
             task_load<KIND, T>(p, b, O, ix, A, B, P, i_at(0), j, k, raw);
+            if constexpr (PAIR) task_load<KIND, T>(p + 3, b, O, ix, A, B, P, i_at(0), j, k, raw2);
             task_prep<KIND, T>(p, P, i_at(0), j, k, b, raw, L, 0, c);
+            if constexpr (PAIR) task_prep<KIND, T>(p + 3, P, i_at(0), j, k, b, raw2, L, 0, c);
         }
-        if (live(1)) task_load<KIND, T>(p, b, O, ix, A, B, P, i_at(1), j, k, raw);
+        if (live(1)) {
+            task_load<KIND, T>(p, b, O, ix, A, B, P, i_at(1), j, k, raw);
+            if constexpr (PAIR) task_load<KIND, T>(p + 3, b, O, ix, A, B, P, i_at(1), j, k, raw2);
+        }
     } else if (p < 0 && live(0)) {
         int i = i_at(0);
         if (jsrc == 1) xch_issue(xj_in + (long)(i - 1) * 5, rj);
@@ -641,7 +672,7 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
                     for (int m = 0; m < 5; m++) nw[0][m] = prev[m];
                 } else {
 #pragma unroll
-                    for (int m = 0; m < 5; m++) nw[0][m] = KIND < 2 ? L.ghost[slot][0][m][c] : T(0.0);
+                    for (int m = 0; m < 5; m++) nw[0][m] = ghost_dq(L, slot, 0, m, c);
                 }
                 if (jsrc == 0) {
 #pragma unroll
@@ -653,7 +684,7 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
                     }
                 } else {
 #pragma unroll
-                    for (int m = 0; m < 5; m++) nw[1][m] = KIND < 2 ? L.ghost[slot][1][m][c] : T(0.0);
+                    for (int m = 0; m < 5; m++) nw[1][m] = ghost_dq(L, slot, 1, m, c);
                 }
                 if (ksrc == 0) {
 #pragma unroll
@@ -665,7 +696,7 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
                     }
                 } else {
 #pragma unroll
-                    for (int m = 0; m < 5; m++) nw[2][m] = KIND < 2 ? L.ghost[slot][2][m][c] : T(0.0);
+                    for (int m = 0; m < 5; m++) nw[2][m] = ghost_dq(L, slot, 2, m, c);
                 }
                 T out[5];
                 chain_compute<KIND, T>(P, L, slot, c, nw, out);
@@ -685,8 +716,14 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
                 if (ksrc == 1) xch_issue(xk_in + (long)(i - 1) * 5, rk);
             }
         } else if (helper) {
-            if (live(step + 1)) task_prep<KIND, T>(p, P, i_at(step + 1), j, k, b, raw, L, slot ^ 1, c);
-            if (live(step + 2)) task_load<KIND, T>(p, b, O, ix, A, B, P, i_at(step + 2), j, k, raw);
+            if (live(step + 1)) {
+                task_prep<KIND, T>(p, P, i_at(step + 1), j, k, b, raw, L, slot ^ 1, c);
+                if constexpr (PAIR) task_prep<KIND, T>(p + 3, P, i_at(step + 1), j, k, b, raw2, L, slot ^ 1, c);
+            }
+            if (live(step + 2)) {
+                task_load<KIND, T>(p, b, O, ix, A, B, P, i_at(step + 2), j, k, raw);
+                if constexpr (PAIR) task_load<KIND, T>(p + 3, b, O, ix, A, B, P, i_at(step + 2), j, k, raw2);
+            }
         }
         __syncthreads();
     }
@@ -766,9 +803,13 @@ __global__ void __launch_bounds__(256) k_skew_stage(DBlock b, SkIndex ix, StageL
 
 __host__ __device__ inline int skew_tiles_s(const DBlock &b) { return (b.ni + b.nj + b.nk + 4 + STG_S - 1) / STG_S; }
 __host__ __device__ inline int skew_tiles_j(const DBlock &b) { return (b.nj + 2 + STG_J - 1) / STG_J; }
+// hyperplane tiles that cover rows j0..j0+STG_J-1 of one plane: nodes i = 0..ni+1
+// lie on s = j + k + i, so s0 = j0 + k onwards spans ni + 1 + STG_J planes
+__host__ __device__ inline int skew_tiles_row(const DBlock &b) { return (b.ni + 1 + STG_J + STG_S - 1) / STG_S; }
 
 // All blocks in one launch: workgroup w stages tile G.wg[w].y of block
 // G.wg[w].x, numbered hyperplane tile fastest, then row tile, then plane k.
+// The hyperplane tiles of a row tile start at its first plane with a node.
 template <typename T>
 __global__ void __launch_bounds__(256) k_skew_stage_all(BlockGrid G, const long *skoff, StageList L, T *pool,
                                                         long sktot)
@@ -776,15 +817,15 @@ __global__ void __launch_bounds__(256) k_skew_stage_all(BlockGrid G, const long 
     __shared__ T tile[STG_G][STG_J][STG_S + 1];
     const int2 w = G.wg[blockIdx.x];
     const DBlock b = G.blocks[w.x];
-    const int gs = skew_tiles_s(b), gj = skew_tiles_j(b);
+    const int gs = skew_tiles_row(b), gj = skew_tiles_j(b);
     const int bs = w.y % gs, bj = (w.y / gs) % gj, k = w.y / (gs * gj);
     const SkIndex ix{skoff[w.x], (b.nj + 2) * (b.nk + 2), b.nj + 2};
-    skew_tile(b, ix, L, pool, sktot, k, bj * STG_J, bs * STG_S, tile);
+    skew_tile(b, ix, L, pool, sktot, k, bj * STG_J, bj * STG_J + k + bs * STG_S, tile);
 }
 
 }  // namespace
 
-long skew_stage_wgs(const DBlock &b) { return (long)skew_tiles_s(b) * skew_tiles_j(b) * (b.nk + 2); }
+long skew_stage_wgs(const DBlock &b) { return (long)skew_tiles_row(b) * skew_tiles_j(b) * (b.nk + 2); }
 
 int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, hipStream_t s)
 {
@@ -885,12 +926,19 @@ static int launch_pipe(int kind, const SweepPlan &S, const DFields &F, const DPa
         OUT.c[m] = g3dq + m * F.total;
     }
     const int2 *work = (kind == 0 || kind == 2) ? S.work_fwd : S.work_bwd;
-    dim3 g(S.nwork), bl(PIPE_COLS * (kind < 2 ? 4 : 7));
+    const bool pair = kind >= 2 && S.sgs_paired;
+    dim3 g(S.nwork), bl(PIPE_COLS * ((kind < 2 || pair) ? 4 : 7));
     switch (kind) {
-    case 0: hipLaunchKernelGGL((k_sweep_pipe<0, T>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
-    case 1: hipLaunchKernelGGL((k_sweep_pipe<1, T>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
-    case 2: hipLaunchKernelGGL((k_sweep_pipe<2, T>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
-    default: hipLaunchKernelGGL((k_sweep_pipe<3, T>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
+    case 0: hipLaunchKernelGGL((k_sweep_pipe<0, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
+    case 1: hipLaunchKernelGGL((k_sweep_pipe<1, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
+    case 2:
+        if (pair) hipLaunchKernelGGL((k_sweep_pipe<2, T, true>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit);
+        else hipLaunchKernelGGL((k_sweep_pipe<2, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit);
+        break;
+    default:
+        if (pair) hipLaunchKernelGGL((k_sweep_pipe<3, T, true>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit);
+        else hipLaunchKernelGGL((k_sweep_pipe<3, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit);
+        break;
     }
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }

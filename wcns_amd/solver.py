@@ -32,7 +32,7 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_l_h_s", "wcns_communicate_pv", "wcns_residual", "wcns_step", "wcns_get_counter", "wcns_sync",
            "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get", "wcns_force",
            "wcns_step_graph", "wcns_autotune", "wcns_adj_residual", "wcns_adj_jvp", "wcns_adj_vjp",
-           "wcns_adj_force"]
+           "wcns_adj_force", "wcns_sweep_layout"]
 
 TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
@@ -90,6 +90,7 @@ def load_library(path=LIBPATH):
     L.wcns_force.argtypes = [vp, dptr]
     L.wcns_step_graph.argtypes = [vp, C.c_int]
     L.wcns_autotune.argtypes = [vp, C.c_int, iptr, iptr, dptr]
+    L.wcns_sweep_layout.argtypes = [vp, iptr, iptr, iptr]
     # adjoint entries take device pointers (vp)
     L.wcns_adj_residual.argtypes = [vp, C.c_int, vp, vp, vp]
     L.wcns_adj_jvp.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp]
@@ -201,6 +202,12 @@ class Solver:
         tj, tk, ms = C.c_int(), C.c_int(), C.c_double()
         self._check(self.L.wcns_autotune(self.h, reps, C.byref(tj), C.byref(tk), C.byref(ms)), "autotune")
         return tj.value, tk.value, ms.value
+
+    def sweep_layout(self):
+        """(TJ, TK, SGS helpers paired) of the pipelined sweeps in use (wcns_sweep_layout)."""
+        tj, tk, pr = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.L.wcns_sweep_layout(self.h, C.byref(tj), C.byref(tk), C.byref(pr)), "sweep layout")
+        return tj.value, tk.value, bool(pr.value)
 
     def step_graph(self, n=1):
         """n whole steps replayed from a captured HIP graph (wcns_step_graph)."""
