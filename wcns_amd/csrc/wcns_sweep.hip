@@ -70,14 +70,24 @@ struct G3Index {
     __device__ long operator()(int i, int j, int k) const { return gidx(b, i, j, k); }
 };
 
+// Plane-major index (s = i+j+k, j, k) of nodes 0..n+1.  Row j sits at j +
+// SK_JSHIFT of a row pitch that is a multiple of 16, so that the first row of
+// every tile (j = 1 + multiple of 16, or of 8, 4) starts on a 128-byte line,
+// and a 16-row tile reads one line per k instead of two.
+constexpr int SK_JSHIFT = 15;
 struct SkIndex {
     long off;
     int pl, nj2;
     __device__ long operator()(int i, int j, int k) const
     {
-        return off + (long)(i + j + k) * pl + j + (long)nj2 * k;
+        return off + (long)(i + j + k) * pl + (j + SK_JSHIFT) + (long)nj2 * k;
     }
 };
+__host__ __device__ inline int skew_pitch(const DBlock &b) { return (b.nj + 2 + SK_JSHIFT + 15) / 16 * 16; }
+__host__ __device__ inline SkIndex skew_index(const DBlock &b, long soff)
+{
+    return SkIndex{soff, skew_pitch(b) * (b.nk + 2), skew_pitch(b)};
+}
 
 // Raw operands of the split Jacobian of neighbour (i,j,k)+off*e_dir: the
 // neighbour's primitives, the metric at the neighbour with ghost indices
@@ -603,7 +613,7 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
     const int2 wk = work[blockIdx.x];
     const SweepBlock sb = blocks[wk.x];
     const DBlock b = sb.b;
-    const SkIndex ix{sb.soff, (b.nj + 2) * (b.nk + 2), b.nj + 2};
+    const SkIndex ix = skew_index(b, sb.soff);
     const int TJ = sb.TJ, TK = sb.TK, nTJ = sb.nTJ, nTK = sb.nTK, ni = b.ni;
     const int tile = wk.y;
     const int J = tile % nTJ, K = tile / nTJ;
@@ -764,7 +774,7 @@ __device__ void skew_tile(const DBlock &b, const SkIndex &ix, const StageList &L
                 int jr = t / 16, sc = (t % 16) + 16 * q;
                 int j = j0 + jr, sv = s0 + sc, i = sv - j - k;
                 v[e][q] = T(0);
-                if (a < L.n && j <= b.nj + 1 && i >= 0 && i <= b.ni + 1) {
+                if (a < L.n && j >= 0 && j <= b.nj + 1 && i >= 0 && i <= b.ni + 1) {
                     long g = gidx(b, i, j, k);
                     v[e][q] = L.f32[a] ? (T)((const float *)L.src[a])[g] : (T)((const double *)L.src[a])[g];
                 }
@@ -784,7 +794,7 @@ __device__ void skew_tile(const DBlock &b, const SkIndex &ix, const StageList &L
             for (int q = 0; q < 4; q++) {
                 int jr = t % 16, sc = t / 16 + 16 * q;
                 int j = j0 + jr, sv = s0 + sc, i = sv - j - k;
-                if (j <= b.nj + 1 && i >= 0 && i <= b.ni + 1) {
+                if (j >= 0 && j <= b.nj + 1 && i >= 0 && i <= b.ni + 1) {
                     bool on_shell = i == 0 || i == b.ni + 1 || j == 0 || j == b.nj + 1 || k == 0 || k == b.nk + 1;
                     if (!L.shell[a] || on_shell) pool[(long)L.dst[a] * sktot + ix(i, j, k)] = tile[e][jr][sc];
                 }
@@ -798,11 +808,12 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_skew_stage(DBlock b, SkIndex ix, StageList L, T *pool, long sktot)
 {
     __shared__ T tile[STG_G][STG_J][STG_S + 1];
-    skew_tile(b, ix, L, pool, sktot, blockIdx.z, blockIdx.y * STG_J, blockIdx.x * STG_S, tile);
+    skew_tile(b, ix, L, pool, sktot, blockIdx.z, (int)blockIdx.y * STG_J - SK_JSHIFT, blockIdx.x * STG_S, tile);
 }
 
 __host__ __device__ inline int skew_tiles_s(const DBlock &b) { return (b.ni + b.nj + b.nk + 4 + STG_S - 1) / STG_S; }
-__host__ __device__ inline int skew_tiles_j(const DBlock &b) { return (b.nj + 2 + STG_J - 1) / STG_J; }
+// row tiles j0 = STG_J * t - SK_JSHIFT, so that each stores whole lines
+__host__ __device__ inline int skew_tiles_j(const DBlock &b) { return (b.nj + 2 + SK_JSHIFT + STG_J - 1) / STG_J; }
 // hyperplane tiles that cover rows j0..j0+STG_J-1 of one plane: nodes i = 0..ni+1
 // lie on s = j + k + i, so s0 = j0 + k onwards spans ni + 1 + STG_J planes
 __host__ __device__ inline int skew_tiles_row(const DBlock &b) { return (b.ni + 1 + STG_J + STG_S - 1) / STG_S; }
@@ -819,8 +830,9 @@ __global__ void __launch_bounds__(256) k_skew_stage_all(BlockGrid G, const long 
     const DBlock b = G.blocks[w.x];
     const int gs = skew_tiles_row(b), gj = skew_tiles_j(b);
     const int bs = w.y % gs, bj = (w.y / gs) % gj, k = w.y / (gs * gj);
-    const SkIndex ix{skoff[w.x], (b.nj + 2) * (b.nk + 2), b.nj + 2};
-    skew_tile(b, ix, L, pool, sktot, k, bj * STG_J, bj * STG_J + k + bs * STG_S, tile);
+    const SkIndex ix = skew_index(b, skoff[w.x]);
+    const int j0 = bj * STG_J - SK_JSHIFT;
+    skew_tile(b, ix, L, pool, sktot, k, j0, j0 + k + bs * STG_S, tile);
 }
 
 }  // namespace
@@ -851,12 +863,7 @@ long sweep_exchange_words(int TJ, int TK, int nTJ, int nTK, int ni)
 
 long skew_size(const DBlock &b)
 {
-    return (long)(b.ni + b.nj + b.nk + 4) * (b.nj + 2) * (b.nk + 2);
-}
-
-static SkIndex skew_index(const DBlock &b, long soff)
-{
-    return SkIndex{soff, (b.nj + 2) * (b.nk + 2), b.nj + 2};
+    return (long)(b.ni + b.nj + b.nk + 4) * skew_pitch(b) * (b.nk + 2);
 }
 
 static StageList stage_list(int what, const DFields &F, int prec)
