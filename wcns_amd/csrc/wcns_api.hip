@@ -1326,6 +1326,12 @@ int wcns_l_h_s(wcns_ctx *ctx)
             if (launch_sweep_batched(kind, ctx->plan, F, ctx->dp, prec, ctx->skew[prec], ctx->sktot, ctx->stream))
                 return fail(ctx, WCNS_E_HIP, "sweeps");
         }
+        {
+            Scope sc(ctx, WCNS_T_SWEEP_STAGE);
+            if (launch_unskew_dq(ctx->grid[GRID_SKEW], ctx->d_skoff, prec, ctx->skew[prec], ctx->sktot, F.dq, F.dq32,
+                                 F.total, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "copying the sweeps' dQ");
+        }
     } else {
         for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
             const DBlock &db = ctx->blocks[nb].db;

@@ -102,6 +102,10 @@ enum { SK_P = 0, SK_M = 5, SK_SR = 14, SK_SRV = 17, SK_RHS = 20, SK_COED = 25, S
 enum { SKEW_STEP = 0, SKEW_METRICS = 1 };
 long skew_size(const DBlock &b);
 long skew_stage_wgs(const DBlock &b);
+// the pipelined sweeps' result (plane-major dQ) into the field pool's dq
+// (prec 0) or dq32 (prec 1), interior nodes of every block, over GRID_SKEW
+int launch_unskew_dq(const BlockGrid &G, const long *skoff, int prec, void *pool, long sktot, double *dq, float *dq32,
+                     long total, hipStream_t s);
 // the staging of every block in one launch over G = the GRID_SKEW map;
 // skoff: each block's offset in the plane-major arrays (device)
 int launch_skew_stage_all(int what, const BlockGrid &G, const long *skoff, const DFields &F, int prec, void *pool,

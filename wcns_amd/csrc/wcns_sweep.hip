@@ -599,12 +599,13 @@ __device__ inline void chain_compute(const DParams &P, const PipeLds<KIND, T> &L
 // With PAIR (SGS only), three helpers form parts p and p + 3 each: four
 // wavefronts instead of seven, so that two workgroups fit in a CU.
 // O: the plane-major operands (every block at its own offset); A, B: the
-// plane-major dQ pair; OUT: the field-pool dQ that gs_pr_u (the last sweep of
-// the step) writes for the update.
+// plane-major dQ pair.  lusgs_l/u write A, gs_pr_l reads A and writes B,
+// gs_pr_u reads B and writes A (which it does not read), from where
+// k_unskew_all copies the step's dQ to the field pool for the update.
 template <int KIND, typename T, bool PAIR>
 __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBlock *blocks, const int2 *work,
-                                                                 Operands<T> O, TPtrs<T> A, TPtrs<T> B,
-                                                                 TPtrs<T> OUT, DParams P, int *err, long spin_limit)
+                                                                 Operands<T> O, TPtrs<T> A, TPtrs<T> B, DParams P,
+                                                                 int *err, long spin_limit)
 {
     static_assert(!PAIR || KIND >= 2, "paired helpers are for the SGS sweeps");
     constexpr bool fwd = (KIND == 0 || KIND == 2);
@@ -646,7 +647,7 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
     double *xk_in = sb.xk + (ksrc == 1 ? ((long)(tile + off * nTJ) * TJ + jl) * rowlen : 0);
     double *xj_out = sb.xj + ((long)tile * TK + kl) * rowlen;
     double *xk_out = sb.xk + ((long)tile * TJ + jl) * rowlen;
-    const TPtrs<T> W = KIND == 2 ? B : (KIND == 3 ? OUT : A);
+    const TPtrs<T> W = KIND == 2 ? B : A;
     T prev[5] = {T(0), T(0), T(0), T(0), T(0)};
     unsigned long long rj[5], rk[5];
     bool bad = false;                 // a bounded wait of this lane expired: stop waiting
@@ -712,7 +713,7 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
                 chain_compute<KIND, T>(P, L, slot, c, nw, out);
                 if (putj) xch_put<T>(xj_out + (long)(i - 1) * 5, out);
                 if (putk) xch_put<T>(xk_out + (long)(i - 1) * 5, out);
-                const long g = KIND == 3 ? gidx(b, i, j, k) : ix(i, j, k);
+                const long g = ix(i, j, k);
 #pragma unroll
                 for (int m = 0; m < 5; m++) {
                     L.sh[slot][m][c] = out[m];
@@ -835,7 +836,68 @@ __global__ void __launch_bounds__(256) k_skew_stage_all(BlockGrid G, const long 
     skew_tile(b, ix, L, pool, sktot, k, j0, j0 + k + bs * STG_S, tile);
 }
 
+// The reverse copy for the sweeps' result: the plane-major dQ (src, five
+// arrays sktot apart) of interior nodes into the field pool (dst, five arrays
+// total apart).  Loads take whole lines of a hyperplane's rows, stores whole
+// runs of i, through the same LDS tile; the same work-list as the staging.
+template <typename T>
+__global__ void __launch_bounds__(256) k_unskew_all(BlockGrid G, const long *skoff, const T *src, long sktot, T *dst,
+                                                    long total)
+{
+    __shared__ T tile[STG_G][STG_J][STG_S + 1];
+    const int2 w = G.wg[blockIdx.x];
+    const DBlock b = G.blocks[w.x];
+    const int gs = skew_tiles_row(b), gj = skew_tiles_j(b);
+    const int bs = w.y % gs, bj = (w.y / gs) % gj, k = w.y / (gs * gj);
+    if (k < 1 || k > b.nk) return;                 // the whole workgroup
+    const SkIndex ix = skew_index(b, skoff[w.x]);
+    const int j0 = bj * STG_J - SK_JSHIFT, s0 = j0 + k + bs * STG_S;
+    const int t = threadIdx.x;
+    for (int a0 = 0; a0 < 5; a0 += STG_G) {
+        T v[STG_G][4];
+#pragma unroll
+        for (int e = 0; e < STG_G; e++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int a = a0 + e, jr = t % 16, sc = t / 16 + 16 * q;
+                const int j = j0 + jr, i = s0 + sc - j - k;
+                v[e][q] = T(0);
+                if (a < 5 && j >= 1 && j <= b.nj && i >= 1 && i <= b.ni) v[e][q] = src[a * sktot + ix(i, j, k)];
+            }
+#pragma unroll
+        for (int e = 0; e < STG_G; e++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) tile[e][t % 16][t / 16 + 16 * q] = v[e][q];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < STG_G; e++) {
+            const int a = a0 + e;
+            if (a >= 5) break;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int jr = t / 16, sc = (t % 16) + 16 * q;
+                const int j = j0 + jr, i = s0 + sc - j - k;
+                if (j >= 1 && j <= b.nj && i >= 1 && i <= b.ni) dst[a * total + gidx(b, i, j, k)] = tile[e][jr][sc];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+int launch_unskew_dq(const BlockGrid &G, const long *skoff, int prec, void *pool, long sktot, double *dq, float *dq32,
+                     long total, hipStream_t s)
+{
+    if (G.nwg == 0) return 0;
+    if (prec == 1)
+        hipLaunchKernelGGL(k_unskew_all<float>, dim3(G.nwg), dim3(256), 0, s, G, skoff,
+                           (const float *)pool + (long)SK_A * sktot, sktot, dq32, total);
+    else
+        hipLaunchKernelGGL(k_unskew_all<double>, dim3(G.nwg), dim3(256), 0, s, G, skoff,
+                           (const double *)pool + (long)SK_A * sktot, sktot, dq, total);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
 
 long skew_stage_wgs(const DBlock &b) { return (long)skew_tiles_row(b) * skew_tiles_j(b) * (b.nk + 2); }
 
@@ -916,7 +978,7 @@ int launch_skew_stage_all(int what, const BlockGrid &G, const long *skoff, const
 
 template <typename T>
 static int launch_pipe(int kind, const SweepPlan &S, const DFields &F, const DParams &P, T *pool, long sktot,
-                       T *g3dq, hipStream_t s)
+                       hipStream_t s)
 {
     Operands<T> O;
     for (int m = 0; m < 5; m++) { O.p[m] = pool + (SK_P + m) * sktot; O.rhs[m] = pool + (SK_RHS + m) * sktot; }
@@ -926,25 +988,24 @@ static int launch_pipe(int kind, const SweepPlan &S, const DFields &F, const DPa
         O.srv[d] = pool + (SK_SRV + d) * sktot;
     }
     O.coed = pool + SK_COED * sktot;
-    TPtrs<T> A, B, OUT;
+    TPtrs<T> A, B;
     for (int m = 0; m < 5; m++) {
         A.c[m] = pool + (SK_A + m) * sktot;
         B.c[m] = pool + (SK_B + m) * sktot;
-        OUT.c[m] = g3dq + m * F.total;
     }
     const int2 *work = (kind == 0 || kind == 2) ? S.work_fwd : S.work_bwd;
     const bool pair = kind >= 2 && S.sgs_paired;
     dim3 g(S.nwork), bl(PIPE_COLS * ((kind < 2 || pair) ? 4 : 7));
     switch (kind) {
-    case 0: hipLaunchKernelGGL((k_sweep_pipe<0, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
-    case 1: hipLaunchKernelGGL((k_sweep_pipe<1, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit); break;
+    case 0: hipLaunchKernelGGL((k_sweep_pipe<0, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, P, S.err, S.spin_limit); break;
+    case 1: hipLaunchKernelGGL((k_sweep_pipe<1, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, P, S.err, S.spin_limit); break;
     case 2:
-        if (pair) hipLaunchKernelGGL((k_sweep_pipe<2, T, true>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit);
-        else hipLaunchKernelGGL((k_sweep_pipe<2, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit);
+        if (pair) hipLaunchKernelGGL((k_sweep_pipe<2, T, true>), g, bl, 0, s, S.blocks, work, O, A, B, P, S.err, S.spin_limit);
+        else hipLaunchKernelGGL((k_sweep_pipe<2, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, P, S.err, S.spin_limit);
         break;
     default:
-        if (pair) hipLaunchKernelGGL((k_sweep_pipe<3, T, true>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit);
-        else hipLaunchKernelGGL((k_sweep_pipe<3, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, OUT, P, S.err, S.spin_limit);
+        if (pair) hipLaunchKernelGGL((k_sweep_pipe<3, T, true>), g, bl, 0, s, S.blocks, work, O, A, B, P, S.err, S.spin_limit);
+        else hipLaunchKernelGGL((k_sweep_pipe<3, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, P, S.err, S.spin_limit);
         break;
     }
     return hipGetLastError() == hipSuccess ? 0 : -3;
@@ -964,8 +1025,8 @@ int launch_sweep_batched(int kind, const SweepPlan &S, const DFields &F, const D
 {
     if (S.lanes > PIPE_COLS || S.lanes < 1 || S.nwork < 1) return -2;
     hipLaunchKernelGGL(k_fill_u64, grid1(S.nxch, 256), dim3(256), 0, s, S.xch, S.nxch, XCH_EMPTY);
-    if (prec == 1) return launch_pipe<float>(kind, S, F, P, (float *)pool, sktot, F.dq32, s);
-    return launch_pipe<double>(kind, S, F, P, (double *)pool, sktot, F.dq, s);
+    if (prec == 1) return launch_pipe<float>(kind, S, F, P, (float *)pool, sktot, s);
+    return launch_pipe<double>(kind, S, F, P, (double *)pool, sktot, s);
 }
 
 }  // namespace wcns
