@@ -67,7 +67,7 @@ SLOT_KERNEL = {"rhs_inv_i": "k_rhs_inv<3,", "rhs_inv_j": "k_rhs_inv<1,", "rhs_in
                "rhs_vis_i": "k_vis_face<3>", "rhs_vis_j": "k_vis_face<1>", "rhs_vis_k": "k_vis_face<2>",
                "vis_rows": "k_vis_rows<",
                "vis_deriv": "k_vis_node_deriv", "sweep_lu_l": "k_sweep_pipe<0,", "sweep_lu_u": "k_sweep_pipe<1,",
-               "sweep_gs_l": "k_sweep_pipe<2,", "sweep_gs_u": "k_sweep_pipe<3,", "sweep_stage": "k_skew_stage<",
+               "sweep_gs_l": "k_sweep_pipe<2,", "sweep_gs_u": "k_sweep_pipe<3,", "sweep_stage": "k_skew_stage_all<",
                "update": "k_update"}
 PROFILE_TAG = {"bump": "bump256x128x64", "cart": "cart256x128x64"}
 
