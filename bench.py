@@ -288,6 +288,7 @@ def converge_bench(args, chunk=100):
             done += n
             rms = s.residual_curve()[0]
             hist.append((done, round(t_run, 4), round(math.log10(r0 / rms), 3) if rms > 0 else None))
+            print(f"converge {lhs}: {done} steps, {t_run:.1f} s, drop {hist[-1][2]}", file=sys.stderr, flush=True)
         runs[lhs] = {"rms_first": r0, "history": hist, "ms_per_step": round(1e3 * t_run / (done - 1), 3)}
         s.close()
     # equal drop: the deepest drop both reached, and when each reached it
