@@ -93,6 +93,7 @@ struct wcns_ctx {
     double *mt[10] = {nullptr};             // kcx kcy kcz etx ety etz ctx cty ctz vol (set-up)
     double *pt[5] = {nullptr}, *t_t = nullptr, *visl_t = nullptr, *vist_t = nullptr;
     double *dn_t = nullptr;                 // 12 slots; 0..3 <- j derivatives, 8..11 <- k derivatives
+    bool pt_fresh = false;                  // pt[] holds the current primitives (no write since the copy)
     double *df_t = nullptr;                 // 5: the i-direction contribution, transposed
     double *fvb = nullptr;                  // 5: viscous half-node fluxes of one direction
     int *d_flag = nullptr;                  // flag_cap nodes per block for the ordered repair
@@ -189,7 +190,10 @@ int stage_transposed(wcns_ctx *ctx, const DFields &F, bool vis)
 {
     TransList L{};
     const double *src[5] = {F.r, F.u, F.v, F.w, F.p};
-    for (int m = 0; m < 5; m++) { L.src[L.n] = src[m]; L.dst[L.n++] = ctx->pt[m]; }
+    // the inviscid stage after the viscous one of the same step reuses its copies
+    if (vis || !ctx->pt_fresh)
+        for (int m = 0; m < 5; m++) { L.src[L.n] = src[m]; L.dst[L.n++] = ctx->pt[m]; }
+    ctx->pt_fresh = true;
     if (vis) {
         L.src[L.n] = F.t; L.dst[L.n++] = ctx->t_t;
         L.src[L.n] = F.visl; L.dst[L.n++] = ctx->visl_t;
@@ -1077,6 +1081,7 @@ int wcns_set_prim(wcns_ctx *ctx, int nb, const double *r, const double *u, const
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     if (nb < 0 || nb >= (int)ctx->blocks.size()) return WCNS_E_ARG;
+    ctx->pt_fresh = false;
     HBlock &b = ctx->blocks[nb];
     const double *src[5] = {r, u, v, w, p};
     for (int bb = 0; bb < 2; bb++)
@@ -1113,6 +1118,7 @@ int wcns_set_field(wcns_ctx *ctx, int nb, int which, const double *in)
 {
     if (!ctx || !ctx->setup_done || !in) return WCNS_E_STATE;
     if (nb < 0 || nb >= (int)ctx->blocks.size()) return WCNS_E_ARG;
+    ctx->pt_fresh = false;
     double *p;
     int nc;
     if (field_ptr(ctx, which, &p, &nc)) return fail(ctx, WCNS_E_ARG, "unknown field");
@@ -1127,6 +1133,7 @@ int wcns_set_field(wcns_ctx *ctx, int nb, int which, const double *in)
 int wcns_boundary_all(wcns_ctx *ctx)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    ctx->pt_fresh = false;
     DFields F = ctx->fields();
     std::vector<DBlock> dbs;
     for (auto &b : ctx->blocks) dbs.push_back(b.db);
@@ -1301,12 +1308,14 @@ int wcns_communicate_dq(wcns_ctx *ctx)
 int wcns_communicate_pv(wcns_ctx *ctx)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    ctx->pt_fresh = false;
     return communicate(ctx, 1);
 }
 
 int wcns_l_h_s(wcns_ctx *ctx)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    ctx->pt_fresh = false;
     DFields F = ctx->fields();
     std::vector<DBlock> dbs;
     for (auto &b : ctx->blocks) dbs.push_back(b.db);
@@ -1433,6 +1442,7 @@ static int capture_pair(wcns_ctx *ctx, hipGraphExec_t *out)
     hipError_t ee = hipStreamEndCapture(ctx->stream, &g);
     ctx->timing = tm;
     ctx->cur = c0;
+    ctx->pt_fresh = false;
     if (e || ee != hipSuccess) {
         if (g) hipGraphDestroy(g);
         return e ? e : fail(ctx, WCNS_E_HIP, std::string("stream capture: ") + hipGetErrorString(ee));
@@ -1716,6 +1726,7 @@ static int ad_check(wcns_ctx *ctx, int nb)
 // the block's primitives <- P, metrics <- GCL(X) (and their transposed copies)
 static int ad_state(wcns_ctx *ctx, int nb, const double *P, const double *X)
 {
+    ctx->pt_fresh = false;
     const HBlock &b = ctx->blocks[nb];
     DFields F = ctx->fields();
     double *dst[5] = {F.r, F.u, F.v, F.w, F.p};
