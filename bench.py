@@ -112,6 +112,11 @@ def build_case(workload):
                                         reynolds=3.0e6, moo=0.6)
         case.params.nlamtur = 1
         return case, f"64 uneven blocks (sides 24-96, seed 7) {case.nodes} nodes, SA Re3e6 M0.6 Roe WCNS-E5"
+    if workload == "mb64-sst":
+        case = cases.uneven_blocks_case(nblk=(4, 4, 4), lo=30, hi=120, seed=7, nvis=1, wall=True, nflux=3,
+                                        reynolds=1.0e7, moo=0.85)
+        case.params.nlamtur = 2
+        return case, f"64 uneven blocks (sides 30-120, seed 7) {case.nodes} nodes, SST Re1e7 M0.85 Roe WCNS-E5"
     if workload == "cart":
         return cases.cartesian_periodic_case(ni=256, nj=128, nk=64), \
             "cartesian-periodic 256x128x64 Euler M0.8 SW WCNS-E5"
