@@ -151,6 +151,8 @@ def main():
     ap.add_argument("--workload", default="bump")
     ap.add_argument("--lhs", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the run with the other implicit-operator precision")
     ap.add_argument("--no-tune", action="store_true", help="keep the default sweep tile shape")
     ap.add_argument("--launch", default="graph", choices=["graph", "plain"],
                     help="timed steps replayed from a captured HIP graph, or plain launches")
@@ -243,6 +245,37 @@ def main():
         for k, (ms, n) in sorted(tim.items(), key=lambda kv: -kv[1][0]):
             print(f"{k:12s} {ms / kt:9.3f} ms/step  {100 * ms / max(tot, 1e-9):5.1f}%  launches {n}",
                   file=sys.stderr)
+    layout = s.sweep_layout()
+    # the same workload with the other implicit-operator precision, timed the
+    # same way: the mixed-vs-FP64 pair of the metric (the equal-drop history of
+    # the two is bench.py --converge)
+    other = None
+    if not args.no_compare:
+        s.close()
+        olhs = "fp32" if args.lhs == "fp64" else "fp64"
+        s2 = Solver(case, device=0, lhs_precision=1 if olhs == "fp32" else 0)
+        s2.step(args.warmup, history=False)
+        if not args.no_tune:
+            s2.autotune(reps=2)
+        if args.launch == "graph":
+            s2.step_graph(2)
+        s2.sync()
+        torch.cuda.synchronize()
+        u0 = time.perf_counter()
+        if args.launch == "graph":
+            s2.step_graph(args.steps)
+        else:
+            s2.step(args.steps, history=False)
+        s2.sync()
+        torch.cuda.synchronize()
+        sec2 = time.perf_counter() - u0
+        v2 = case.nodes * args.steps / sec2 / 1e6
+        lay2 = s2.sweep_layout()
+        other = {"lhs": olhs, "value": round(v2, 2), "ms_per_step": round(1e3 * sec2 / args.steps, 3),
+                 "fp32_lhs_over_fp64": round(v2 / value if olhs == "fp32" else value / v2, 4),
+                 "sweep_tile": list(lay2[:2]), "sgs_paired_helpers": lay2[2],
+                 "residual_rms_end": s2.residual_curve()[0]}
+        s2.close()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(case)
@@ -254,7 +287,8 @@ def main():
         "data": "synthetic (seeded generator wcns_amd.cases)",
         "config": {"workload": wl_name, "nodes": nodes, "blocks": len(case.blocks), "lhs": args.lhs,
                    "launch": args.launch, "sweep_tile": list(tile[:2]) if tile else None,
-                   "sgs_paired_helpers": s.sweep_layout()[2], "residual_rms_end": rms},
+                   "sgs_paired_helpers": layout[2], "residual_rms_end": rms},
+        "other_lhs_precision": other,
         "roofline": roof, "cpu_baseline": cpu,
         "step_hbm_frac": round(STEP_BYTES_PER_NODE * nodes * args.steps / sec / (HBM_PEAK_GBS * 1e9), 5),
         "stage_ms_per_step": {k: round(v[0] / kt, 4) for k, v in tim.items()},
