@@ -197,6 +197,33 @@ def test_paired_sgs_helpers_match_unpaired(monkeypatch, lhs):
             assert np.array_equal(a.field(nb, name), b.field(nb, name)), f"block {nb} {name}"
 
 
+def test_inviscid_stage_sees_primitives_written_after_viscous_stage():
+    """The inviscid stage reuses the viscous stage's i<->j copies of the
+    primitives only while nothing has written them: a pressure changed with
+    set_field between the two stages must reach the inviscid residual.  Run A
+    (viscous, set_field, inviscid) against run B (set_field, inviscid), both
+    from a zeroed residual: identical bits."""
+    case = cases.bump_channel_case(ni=32, nj=24, nk=12, dy_wall=1e-3, growth=1.15, reynolds=1e4)
+    a = _solver(case)
+    b = _solver(case)
+    for s in (a, b):
+        s.boundary_all()
+        s.initial_q_dq_c_t()
+        s.timestep_tgh()
+    p = a.field(0, "p")
+    p2 = p * (1.0 + 0.01 * np.sin(np.arange(p.size).reshape(p.shape) * 0.37))
+    a.r_h_s_vis()                       # copies the primitives for the i direction
+    a.set_field(0, "p", p2)
+    a.initial_q_dq_c_t()                # R = 0 again
+    a.r_h_s_invis()
+    b.set_field(0, "p", p2)
+    b.initial_q_dq_c_t()
+    b.r_h_s_invis()
+    ra, rb = a.field(0, "rhs")[3:-3, 3:-3, 3:-3], b.field(0, "rhs")[3:-3, 3:-3, 3:-3]
+    assert np.array_equal(ra, rb)
+    assert np.any(ra != 0.0)
+
+
 def test_mixed_precision_lhs_converges_like_fp64():
     """FP32 implicit operator (FP64 residual, weights and update): same residual
     drop within a few steps of the FP64 operator on the same case."""
