@@ -197,6 +197,26 @@ def test_paired_sgs_helpers_match_unpaired(monkeypatch, lhs):
             assert np.array_equal(a.field(nb, name), b.field(nb, name)), f"block {nb} {name}"
 
 
+@pytest.mark.parametrize("model", [1, 2], ids=["sa", "sst"])
+def test_staged_turbulence_sweeps_match_field_pool_sweeps(monkeypatch, model):
+    """The scalar turbulence sweeps on operands staged plane-major give the
+    bits of the same sweeps reading the field pools, over four steps of an
+    uneven multi-block case with SA and with SST (two variables)."""
+    case = _uneven_sa()
+    case.params.nlamtur = model
+    monkeypatch.setenv("WCNS_SA_STAGED", "0")
+    a = _solver(case)
+    monkeypatch.setenv("WCNS_SA_STAGED", "1")
+    b = _solver(case)
+    monkeypatch.delenv("WCNS_SA_STAGED")
+    ha = a.step(4)
+    hb = b.step(4)
+    assert np.array_equal(ha, hb)
+    for nb in range(len(case.blocks)):
+        for name in ("r", "u", "v", "w", "p", "qke", "vist") + (("qkw",) if model == 2 else ()):
+            assert np.array_equal(a.field(nb, name), b.field(nb, name)), f"block {nb} {name}"
+
+
 def test_inviscid_stage_sees_primitives_written_after_viscous_stage():
     """The inviscid stage reuses the viscous stage's i<->j copies of the
     primitives only while nothing has written them: a pressure changed with

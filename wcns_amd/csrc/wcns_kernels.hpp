@@ -101,7 +101,32 @@ int launch_fill_u64(unsigned long long *a, long n, unsigned long long v, hipStre
 enum { SK_P = 0, SK_M = 5, SK_SR = 14, SK_SRV = 17, SK_RHS = 20, SK_COED = 25, SK_A = 26, SK_B = 31, SK_N = 36 };
 enum { SKEW_STEP = 0, SKEW_METRICS = 1 };
 long skew_size(const DBlock &b);
+// Plane-major index (s = i+j+k, j, k) of nodes 0..n+1.  Row j sits at j +
+// SK_JSHIFT of a row pitch that is a multiple of 16, so that the first row of
+// every tile (j = 1 + multiple of 16, or of 8, 4) starts on a 128-byte line,
+// and a 16-row tile reads one line per k instead of two.
+constexpr int SK_JSHIFT = 15;
+struct SkIndex {
+    long off;
+    int pl, nj2;
+    __device__ long operator()(int i, int j, int k) const
+    {
+        return off + (long)(i + j + k) * pl + (j + SK_JSHIFT) + (long)nj2 * k;
+    }
+};
+__host__ __device__ inline int skew_pitch(const DBlock &b) { return (b.nj + 2 + SK_JSHIFT + 15) / 16 * 16; }
+__host__ __device__ inline SkIndex skew_index(const DBlock &b, long soff)
+{
+    return SkIndex{soff, skew_pitch(b) * (b.nk + 2), skew_pitch(b)};
+}
 long skew_stage_wgs(const DBlock &b);
+// n FP64 field-pool arrays src[a] into plane-major arrays slot[a] of every
+// block (GRID_SKEW map), and ncomp plane-major arrays from slot0 on back into
+// the field pool (interior nodes, components total apart)
+int launch_skew_stage_f64(const BlockGrid &G, const long *skoff, const double *const *src, const int *slot, int n,
+                          double *pool, long sktot, hipStream_t s);
+int launch_unskew_f64(const BlockGrid &G, const long *skoff, const double *pool, long sktot, int slot0, int ncomp,
+                      double *dst, long total, hipStream_t s);
 // the pipelined sweeps' result (plane-major dQ) into the field pool's dq
 // (prec 0) or dq32 (prec 1), interior nodes of every block, over GRID_SKEW
 int launch_unskew_dq(const BlockGrid &G, const long *skoff, int prec, void *pool, long sktot, double *dq, float *dq32,
@@ -174,7 +199,9 @@ int launch_wall_dist(const DBlock &b, const double *x, const double *y, const do
 int launch_sa_init(const DBlock &b, const DFields &F, double q0, double w0, double vt, hipStream_t s);
 int launch_sa_rhs(const BlockGrid &nodes, const BlockGrid &inner, const DFields &F, const SaWork &W,
                   const DParams &P, hipStream_t s);
-int launch_sa_sweep(int fwd, int var, const SweepPlan &S, const SaWork &W, hipStream_t s);
+// pm: the FP64 plane-major pool when the operands were staged there (slots
+// SK_A .. SK_A+4: the unknown, the diagonal, un[0..2]), or null for the field pools
+int launch_sa_sweep(int fwd, int var, const SweepPlan &S, const SaWork &W, double *pm, long sktot, hipStream_t s);
 int launch_sa_update(const BlockGrid &G, const DFields &F, const SaWork &W, const DParams &P, hipStream_t s);
 
 // i<->j transposed copies (wcns_transpose.hip): block b's copy has b's offset

@@ -485,9 +485,14 @@ struct SaOp {
 // the value dqke holds there.  Forward: dq = (A+ dq_lower - dq) / spec;
 // backward: dq = dq - (A- dq_upper) / spec, A+- = (l1 +- |l1|)/2 of the
 // neighbour (mxdq_std_turbulence, turblence.f90:560-583).
-template <int FWD>
+// PM: the operands come from (and the unknown goes to) the plane-major copy
+// pm = slots SK_A.. of the FP64 plane-major pool, where the lanes of a tile
+// read whole rows of a hyperplane; otherwise from the field pools, where every
+// lane reads a line of its own.
+template <int FWD, bool PM>
 __global__ void __launch_bounds__(64) k_sa_sweep(const SweepBlock *blocks, const int2 *work, SaWork W,
-                                                 double *dqv, const double *specv, int *err, long spin_limit)
+                                                 double *dqv, const double *specv, double *pm, long sktot, int *err,
+                                                 long spin_limit)
 {
     constexpr int off = FWD ? -1 : 1;
     const double npn = FWD ? 1.0 : -1.0;
@@ -521,15 +526,25 @@ __global__ void __launch_bounds__(64) k_sa_sweep(const SweepBlock *blocks, const
     double *xj_out = sb.xj + ((long)tile * TK + kl) * rowlen;
     double *xk_out = sb.xk + ((long)tile * TJ + jl) * rowlen;
 
+    const SkIndex ix = skew_index(b, sb.soff);
     auto load = [&](int st, SaOp &o) {
         const int i = i_at(st);
         if (!inner(i)) return;
-        const long g = gidx(b, i, j, k);
-        o.d = dqv[g];
-        o.sp = specv[g];
-        o.ui = W.un[0][gidx(b, i + off, j, k)];
-        o.uj = W.un[1][gidx(b, i, j + off, k)];
-        o.uk = W.un[2][gidx(b, i, j, k + off)];
+        if constexpr (PM) {
+            const long g = ix(i, j, k);
+            o.d = pm[g];
+            o.sp = pm[sktot + g];
+            o.ui = pm[2 * sktot + ix(i + off, j, k)];
+            o.uj = pm[3 * sktot + ix(i, j + off, k)];
+            o.uk = pm[4 * sktot + ix(i, j, k + off)];
+        } else {
+            const long g = gidx(b, i, j, k);
+            o.d = dqv[g];
+            o.sp = specv[g];
+            o.ui = W.un[0][gidx(b, i + off, j, k)];
+            o.uj = W.un[1][gidx(b, i, j + off, k)];
+            o.uk = W.un[2][gidx(b, i, j, k + off)];
+        }
     };
     // operands of this step (o0) and the next two (o1, o2), rotated by value so
     // that no register array is indexed at run time
@@ -568,7 +583,8 @@ __global__ void __launch_bounds__(64) k_sa_sweep(const SweepBlock *blocks, const
                 const double ak = 0.5 * (o.uk + npn * fabs(o.uk));
                 const double rhs0 = ai * prev + aj * nj_ + ak * nk_;
                 out = FWD ? (rhs0 - o.d) / o.sp : o.d - rhs0 / o.sp;
-                dqv[gidx(b, i, j, k)] = out;
+                if constexpr (PM) pm[ix(i, j, k)] = out;
+                else dqv[gidx(b, i, j, k)] = out;
             }
             if (putj) xch_put1(xj_out + (i - 1), out);
             if (putk) xch_put1(xk_out + (i - 1), out);
@@ -673,7 +689,7 @@ int launch_sa_rhs(const BlockGrid &nodes, const BlockGrid &inner, const DFields 
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_sa_sweep(int fwd, int var, const SweepPlan &S, const SaWork &W, hipStream_t s)
+int launch_sa_sweep(int fwd, int var, const SweepPlan &S, const SaWork &W, double *pm, long sktot, hipStream_t s)
 {
     if (var < 0 || var > 1 || !W.dqke[var] || !W.spec[var]) return -2;
     if (S.lanes > 64 || S.lanes < 1 || S.nwork < 1) return -2;
@@ -681,11 +697,19 @@ int launch_sa_sweep(int fwd, int var, const SweepPlan &S, const SaWork &W, hipSt
     const int2 *work = fwd ? S.work_fwd : S.work_bwd;
     double *dqv = W.dqke[var];
     const double *specv = W.spec[var];
-    if (fwd)
-        hipLaunchKernelGGL(k_sa_sweep<1>, dim3(S.nwork), dim3(64), 0, s, S.blocks, work, W, dqv, specv, S.err,
+    double *p = pm ? pm + (long)SK_A * sktot : nullptr;
+    const dim3 g(S.nwork), bl(64);
+    if (pm && fwd)
+        hipLaunchKernelGGL((k_sa_sweep<1, true>), g, bl, 0, s, S.blocks, work, W, dqv, specv, p, sktot, S.err,
+                           S.spin_limit);
+    else if (pm)
+        hipLaunchKernelGGL((k_sa_sweep<0, true>), g, bl, 0, s, S.blocks, work, W, dqv, specv, p, sktot, S.err,
+                           S.spin_limit);
+    else if (fwd)
+        hipLaunchKernelGGL((k_sa_sweep<1, false>), g, bl, 0, s, S.blocks, work, W, dqv, specv, p, sktot, S.err,
                            S.spin_limit);
     else
-        hipLaunchKernelGGL(k_sa_sweep<0>, dim3(S.nwork), dim3(64), 0, s, S.blocks, work, W, dqv, specv, S.err,
+        hipLaunchKernelGGL((k_sa_sweep<0, false>), g, bl, 0, s, S.blocks, work, W, dqv, specv, p, sktot, S.err,
                            S.spin_limit);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }

@@ -70,24 +70,6 @@ struct G3Index {
     __device__ long operator()(int i, int j, int k) const { return gidx(b, i, j, k); }
 };
 
-// Plane-major index (s = i+j+k, j, k) of nodes 0..n+1.  Row j sits at j +
-// SK_JSHIFT of a row pitch that is a multiple of 16, so that the first row of
-// every tile (j = 1 + multiple of 16, or of 8, 4) starts on a 128-byte line,
-// and a 16-row tile reads one line per k instead of two.
-constexpr int SK_JSHIFT = 15;
-struct SkIndex {
-    long off;
-    int pl, nj2;
-    __device__ long operator()(int i, int j, int k) const
-    {
-        return off + (long)(i + j + k) * pl + (j + SK_JSHIFT) + (long)nj2 * k;
-    }
-};
-__host__ __device__ inline int skew_pitch(const DBlock &b) { return (b.nj + 2 + SK_JSHIFT + 15) / 16 * 16; }
-__host__ __device__ inline SkIndex skew_index(const DBlock &b, long soff)
-{
-    return SkIndex{soff, skew_pitch(b) * (b.nk + 2), skew_pitch(b)};
-}
 
 // Raw operands of the split Jacobian of neighbour (i,j,k)+off*e_dir: the
 // neighbour's primitives, the metric at the neighbour with ghost indices
@@ -841,8 +823,8 @@ __global__ void __launch_bounds__(256) k_skew_stage_all(BlockGrid G, const long 
 // total apart).  Loads take whole lines of a hyperplane's rows, stores whole
 // runs of i, through the same LDS tile; the same work-list as the staging.
 template <typename T>
-__global__ void __launch_bounds__(256) k_unskew_all(BlockGrid G, const long *skoff, const T *src, long sktot, T *dst,
-                                                    long total)
+__global__ void __launch_bounds__(256) k_unskew_all(BlockGrid G, const long *skoff, const T *src, long sktot,
+                                                    int ncomp, T *dst, long total)
 {
     __shared__ T tile[STG_G][STG_J][STG_S + 1];
     const int2 w = G.wg[blockIdx.x];
@@ -853,7 +835,7 @@ __global__ void __launch_bounds__(256) k_unskew_all(BlockGrid G, const long *sko
     const SkIndex ix = skew_index(b, skoff[w.x]);
     const int j0 = bj * STG_J - SK_JSHIFT, s0 = j0 + k + bs * STG_S;
     const int t = threadIdx.x;
-    for (int a0 = 0; a0 < 5; a0 += STG_G) {
+    for (int a0 = 0; a0 < ncomp; a0 += STG_G) {
         T v[STG_G][4];
 #pragma unroll
         for (int e = 0; e < STG_G; e++)
@@ -862,7 +844,7 @@ __global__ void __launch_bounds__(256) k_unskew_all(BlockGrid G, const long *sko
                 const int a = a0 + e, jr = t % 16, sc = t / 16 + 16 * q;
                 const int j = j0 + jr, i = s0 + sc - j - k;
                 v[e][q] = T(0);
-                if (a < 5 && j >= 1 && j <= b.nj && i >= 1 && i <= b.ni) v[e][q] = src[a * sktot + ix(i, j, k)];
+                if (a < ncomp && j >= 1 && j <= b.nj && i >= 1 && i <= b.ni) v[e][q] = src[a * sktot + ix(i, j, k)];
             }
 #pragma unroll
         for (int e = 0; e < STG_G; e++)
@@ -872,7 +854,7 @@ __global__ void __launch_bounds__(256) k_unskew_all(BlockGrid G, const long *sko
 #pragma unroll
         for (int e = 0; e < STG_G; e++) {
             const int a = a0 + e;
-            if (a >= 5) break;
+            if (a >= ncomp) break;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int jr = t / 16, sc = (t % 16) + 16 * q;
@@ -892,10 +874,20 @@ int launch_unskew_dq(const BlockGrid &G, const long *skoff, int prec, void *pool
     if (G.nwg == 0) return 0;
     if (prec == 1)
         hipLaunchKernelGGL(k_unskew_all<float>, dim3(G.nwg), dim3(256), 0, s, G, skoff,
-                           (const float *)pool + (long)SK_A * sktot, sktot, dq32, total);
+                           (const float *)pool + (long)SK_A * sktot, sktot, 5, dq32, total);
     else
         hipLaunchKernelGGL(k_unskew_all<double>, dim3(G.nwg), dim3(256), 0, s, G, skoff,
-                           (const double *)pool + (long)SK_A * sktot, sktot, dq, total);
+                           (const double *)pool + (long)SK_A * sktot, sktot, 5, dq, total);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_unskew_f64(const BlockGrid &G, const long *skoff, const double *pool, long sktot, int slot0, int ncomp,
+                      double *dst, long total, hipStream_t s)
+{
+    if (G.nwg == 0) return 0;
+    if (ncomp < 1 || ncomp > 5) return -2;
+    hipLaunchKernelGGL(k_unskew_all<double>, dim3(G.nwg), dim3(256), 0, s, G, skoff, pool + (long)slot0 * sktot,
+                       sktot, ncomp, dst, total);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -962,6 +954,18 @@ int launch_skew_stage(int what, const DBlock &b, long soff, const DFields &F, in
     SkIndex ix = skew_index(b, soff);
     if (prec == 1) hipLaunchKernelGGL(k_skew_stage<float>, g, dim3(256), 0, s, b, ix, L, (float *)pool, sktot);
     else hipLaunchKernelGGL(k_skew_stage<double>, g, dim3(256), 0, s, b, ix, L, (double *)pool, sktot);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_skew_stage_f64(const BlockGrid &G, const long *skoff, const double *const *src, const int *slot, int n,
+                          double *pool, long sktot, hipStream_t s)
+{
+    if (n < 1 || n > STG_MAX) return -2;
+    StageList L{};
+    for (int a = 0; a < n; a++) { L.src[a] = src[a]; L.dst[a] = slot[a]; L.f32[a] = 0; L.shell[a] = 0; }
+    L.n = n;
+    if (G.nwg == 0) return 0;
+    hipLaunchKernelGGL(k_skew_stage_all<double>, dim3(G.nwg), dim3(256), 0, s, G, skoff, L, pool, sktot);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
