@@ -633,7 +633,7 @@ int wcns_set_lhs_precision(wcns_ctx *ctx, int prec)
 // operations of one level touch no node that another of them writes.
 extern "C++" {
 template <typename Touch>
-static void region_levels(long total, const std::vector<DRegion> &seq, Touch touch, std::vector<int> &lvl)
+static bool region_levels(long total, const std::vector<DRegion> &seq, Touch touch, std::vector<int> &lvl)
 {
     std::vector<uint16_t> lw(total, 0), lr(total, 0);   // 1 + level of the last write / read
     std::vector<long> rd, wr;
@@ -645,11 +645,13 @@ static void region_levels(long total, const std::vector<DRegion> &seq, Touch tou
         int L = 0;
         for (long q : rd) L = std::max(L, (int)lw[q]);
         for (long q : wr) L = std::max(L, std::max((int)lw[q], (int)lr[q]));
+        if (L + 1 > 0xffff) return false;       // more levels than the 16-bit stamps hold
         lvl[n] = L;
         const uint16_t L1 = (uint16_t)(L + 1);
         for (long q : rd) lr[q] = std::max(lr[q], L1);
         for (long q : wr) lw[q] = std::max(lw[q], L1);
     }
+    return true;
 }
 }
 
@@ -708,8 +710,8 @@ static int build_region_plans(wcns_ctx *ctx)
         }
     };
     std::vector<int> lb, li;
-    region_levels(ctx->total, bcs, bc_touch, lb);
-    region_levels(ctx->total, ctx->iface, if_touch, li);
+    if (!region_levels(ctx->total, bcs, bc_touch, lb) || !region_levels(ctx->total, ctx->iface, if_touch, li))
+        return fail(ctx, WCNS_E_UNSUPPORTED, "more than 65534 dependency levels of boundary regions");
     std::vector<DRegion> regs = bcs;
     regs.insert(regs.end(), ctx->iface.begin(), ctx->iface.end());
     regs.insert(regs.end(), ctx->phys.begin(), ctx->phys.end());
