@@ -9,6 +9,7 @@
 // with i and j exchanged.  Their contribution comes back through a transposed
 // add into the residual.  Each workgroup moves one 32x32 tile of one k plane
 // of one block through LDS, so both the read and the write are coalesced.
+// Only what the view's pencils read is moved: every i, at interior j and k.
 #include "wcns_kernels.hpp"
 
 namespace wcns {
@@ -28,11 +29,14 @@ __global__ void __launch_bounds__(256) k_transpose(BlockGrid G, TransList L)
     const int2 wk = G.wg[blockIdx.x];
     const DBlock b = G.blocks[wk.x];
     const int ei = b.ni + 6, ej = b.nj + 6;
-    const int ti = (ei + TT - 1) / TT, tj = (ej + TT - 1) / TT;
+    const int ti = (ei + TT - 1) / TT, tj = (b.nj + TT - 1) / TT;
     const int tile = wk.y;
     const int plane = tile / (ti * tj), rem = tile % (ti * tj);
     const int bi = rem % ti, bj = rem / ti;
-    const long base = b.off + (long)plane * b.sy;
+    // every i (the pencil direction of the view, ghosts included), interior j
+    // and k: G3 rows 3..nj+2 of planes 3..nk+2
+    const long base = b.off + (long)(plane + 3) * b.sy;
+    const int jlo = 3, jhi = b.nj + 3;
     const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
     __shared__ double s[TG][TT][TT + 1];
     for (int a0 = 0; a0 < L.n; a0 += TG) {
@@ -47,11 +51,11 @@ __global__ void __launch_bounds__(256) k_transpose(BlockGrid G, TransList L)
                 v[e][r] = 0.0;
                 if (a < L.n) {
                     if (BACK == 0) {
-                        const int i = bi * TT + tx, j = bj * TT + yl;
-                        if (i < ei && j < ej) v[e][r] = L.src[a][base + i + (long)ei * j];
+                        const int i = bi * TT + tx, j = jlo + bj * TT + yl;
+                        if (i < ei && j < jhi) v[e][r] = L.src[a][base + i + (long)ei * j];
                     } else {
-                        const int j = bj * TT + tx, i = bi * TT + yl;
-                        if (i < ei && j < ej) v[e][r] = L.src[a][base + j + (long)ej * i];
+                        const int j = jlo + bj * TT + tx, i = bi * TT + yl;
+                        if (i < ei && j < jhi) v[e][r] = L.src[a][base + j + (long)ej * i];
                     }
                 }
             }
@@ -74,11 +78,11 @@ __global__ void __launch_bounds__(256) k_transpose(BlockGrid G, TransList L)
             for (int r = 0; r < TT / TROWS; r++) {
                 const int yl = ty + TROWS * r;
                 if (BACK == 0) {
-                    const int j = bj * TT + tx, i = bi * TT + yl;
-                    if (i < ei && j < ej) dst[base + j + (long)ej * i] = s[e][tx][yl];
+                    const int j = jlo + bj * TT + tx, i = bi * TT + yl;
+                    if (i < ei && j < jhi) dst[base + j + (long)ej * i] = s[e][tx][yl];
                 } else {
-                    const int i = bi * TT + tx, j = bj * TT + yl;
-                    if (i < ei && j < ej) {
+                    const int i = bi * TT + tx, j = jlo + bj * TT + yl;
+                    if (i < ei && j < jhi) {
                         const long g = base + i + (long)ei * j;
                         dst[g] = dst[g] + s[e][yl][tx];
                     }
@@ -93,8 +97,8 @@ __global__ void __launch_bounds__(256) k_transpose(BlockGrid G, TransList L)
 
 long transpose_tiles(const DBlock &b)
 {
-    const long ti = (b.ni + 6 + TT - 1) / TT, tj = (b.nj + 6 + TT - 1) / TT;
-    return ti * tj * (b.nk + 6);
+    const long ti = (b.ni + 6 + TT - 1) / TT, tj = (b.nj + TT - 1) / TT;
+    return ti * tj * b.nk;
 }
 
 DBlock transposed_view(const DBlock &b)
