@@ -217,6 +217,26 @@ def test_staged_turbulence_sweeps_match_field_pool_sweeps(monkeypatch, model):
             assert np.array_equal(a.field(nb, name), b.field(nb, name)), f"block {nb} {name}"
 
 
+def test_inviscid_register_budgets_match(monkeypatch):
+    """The inviscid walk under the two-wave register budget gives the bits of
+    the one-wave budget (each forced with WCNS_INV_WAVES), over four steps of
+    an uneven multi-block SA case with SW and with Roe fluxes."""
+    for nflux in (1, 3):
+        case = _uneven_sa()
+        case.params.nflux = nflux
+        monkeypatch.setenv("WCNS_INV_WAVES", "1")
+        a = _solver(case)
+        monkeypatch.setenv("WCNS_INV_WAVES", "2")
+        b = _solver(case)
+        monkeypatch.delenv("WCNS_INV_WAVES")
+        ha = a.step(4)
+        hb = b.step(4)
+        assert np.array_equal(ha, hb)
+        for nb in range(len(case.blocks)):
+            for name in ("r", "u", "v", "w", "p"):
+                assert np.array_equal(a.field(nb, name), b.field(nb, name)), f"flux {nflux} block {nb} {name}"
+
+
 def test_inviscid_stage_sees_primitives_written_after_viscous_stage():
     """The inviscid stage reuses the viscous stage's i<->j copies of the
     primitives only while nothing has written them: a pressure changed with

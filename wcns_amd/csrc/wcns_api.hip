@@ -469,6 +469,7 @@ int wcns_create(wcns_ctx **out, int device, const wcns_params *prm)
     if (const char *sf = getenv("WCNS_SURFACE")) ctx->surf_levels = strcmp(sf, "sequence") != 0;
     if (const char *sp = getenv("WCNS_SGS_PAIR")) { ctx->plan.sgs_paired = atoi(sp) != 0; ctx->sgs_forced = true; }
     if (const char *ss = getenv("WCNS_SA_STAGED")) ctx->sa_staged = atoi(ss) != 0;
+    if (const char *iw = getenv("WCNS_INV_WAVES")) ctx->dp.inv_waves = atoi(iw) == 2 ? 2 : (atoi(iw) == 1 ? 1 : 0);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return WCNS_E_HIP;

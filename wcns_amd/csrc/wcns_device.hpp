@@ -39,6 +39,7 @@ struct DParams {
     // freestream nu~ (SA), k, omega and mu_t (SST), namelist turbulence values
     double muoo, koo, omgoo, vtoo, kmaxlim, timedt_turb, xk, xb;
     int turb;
+    int inv_waves;   // register budget of the inviscid walk: 0 by launch size, 1 or 2 waves per SIMD
 };
 
 // Base pointers of the field pools.  Conservative 5-vectors (rhs, dq) are
@@ -52,7 +53,7 @@ struct DFields {
     double *dtdt, *sra, *srb, *src, *srva, *srvb, *srvc, *visl;
     double *coed;                       // LU-SGS diagonal 1/(1/dt + sum of radii), per node
     float *dq32;                        // FP32 sweep operand (mixed mode)
-    unsigned long long *counters;       // 0 fallbacks, 1 repairs, 2 roe, 3 nrepair list len
+    unsigned long long *counters;       // 0 fallbacks, 1 repairs, 2 roe, 4 repair abort
     double *qke, *vist, *dist;          // SA nu~ / SST k, mu_t, wall distance (null when laminar)
     double *qkw;                        // SST omega (null otherwise)
     long total;

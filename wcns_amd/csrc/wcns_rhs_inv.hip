@@ -649,7 +649,7 @@ int launch_rhs_inv_dir(int dir, const BlockGrid &G, const DFields &F, const DPar
     dim3 grid((unsigned)G.nwg), blk(128);
     // two 64-wide waves per workgroup; the two-wave budget once they exceed
     // 1.5 waves per SIMD
-    const bool two = 2L * G.nwg * 2 > 3L * simd_count();
+    const bool two = P.inv_waves ? P.inv_waves == 2 : 2L * G.nwg * 2 > 3L * simd_count();
     if (P.nflux == 3) launch_inv<3>(dir, two, grid, blk, s, G, F, P);
     else launch_inv<1>(dir, two, grid, blk, s, G, F, P);
     return hipGetLastError() == hipSuccess ? 0 : -3;
