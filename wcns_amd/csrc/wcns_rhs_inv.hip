@@ -575,8 +575,12 @@ __global__ void __launch_bounds__(128) k_rhs_inv_jvp(DBlock b, DFields F, DParam
 // of the colour, sum over rows of W . dR_row/d(variable at the node) to the
 // node's gradient (lamP: r,u,v,w,p; lamM: the pencil direction's metrics).
 constexpr int INV_SEEDS = 8;
+// Two waves per SIMD (256 registers, about 135 spilled): its launch is always
+// large (colours x seeds x pencils), and that budget measured 44.0 -> 41.8 ms
+// per adjoint call on the 128x96x64 SA block against about 450 registers at one.
 template <int DIR, int FLUX>
-__global__ void __launch_bounds__(128) k_rhs_inv_vjp(DBlock b, DFields F, DParams P, AdAdj A)
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k_rhs_inv_vjp(DBlock b, DFields F,
+                                                                                             DParams P, AdAdj A)
 {
     const int na = DIR == 0 ? b.nj : b.ni;
     const int nb = DIR == 2 ? b.nj : b.nk;
