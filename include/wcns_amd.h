@@ -109,20 +109,33 @@ int wcns_communicate_dq(wcns_ctx *ctx);      /* parallel.f90:3020-3164  */
 int wcns_l_h_s(wcns_ctx *ctx);               /* LHS.f90:1-34            */
 int wcns_communicate_pv(wcns_ctx *ctx);      /* parallel.f90:3166-3326  */
 /* residual_curve (PostProcess.f90:1195-1347): RMS and max of dq/(dtdt*vol)
- * over nodes 2..n-1; loc = i,j,k,m,nb (1-based).  Synchronises the stream. */
+ * over nodes 2..n-1 of the current dq; loc = i,j,k,m,nb (1-based).  Does not
+ * touch the history below.  Synchronises the stream. */
 int wcns_residual(wcns_ctx *ctx, double *rms, double *resmax, int loc[5]);
 
-/* nsteps whole steps; rms_hist (may be NULL) receives each step's RMS.
- * With rms_hist == NULL the residual is still computed on the device each
- * step but only read back at the end (no per-step host synchronisation). */
+/* nsteps whole steps in the order of Main.f90:85-109, residual_curve
+ * included: every step appends its monitor (rms, max, loc) to an on-device
+ * history of the last 8192 steps, with no host synchronisation in between.
+ * The host waits once per 8192 steps and at the end, where it checks the
+ * device error counters; a Roe c^2 <= 0 or an update_limited abort
+ * (WCNS_E_ROE / WCNS_E_REPAIR) is therefore reported at the end of that batch
+ * of steps, not at the step where it occurred.  rms_hist (may be NULL)
+ * receives each step's RMS. */
 int wcns_step(wcns_ctx *ctx, int nsteps, double *rms_hist);
 
-/* The same nsteps steps as wcns_step(ctx, nsteps, NULL), replayed from a HIP
- * graph that holds two whole steps (captured on first use, kept until the
- * implicit-operator precision changes; an odd last step runs as plain
- * launches).  No counterpart in the reference; the residual is not monitored
- * inside (call wcns_residual after).  Synchronises the stream at the end. */
+/* The same nsteps steps as wcns_step(ctx, nsteps, NULL), residual monitor
+ * included, replayed from a HIP graph that holds two whole steps (captured on
+ * first use, kept until the implicit-operator precision changes; an odd last
+ * step runs as plain launches).  No counterpart in the reference.
+ * Synchronises the stream at the end and checks the counters then. */
 int wcns_step_graph(wcns_ctx *ctx, int nsteps);
+
+/* The monitor history written by wcns_step / wcns_step_graph (the values
+ * saveres2file writes to res.dat every nwerror steps, PostProcess.f90:1349-1378):
+ * the last min(max, count, 8192) steps, oldest first; rms, resmax (max doubles
+ * each) and loc (5*max ints: i,j,k,m,nb) may each be NULL; *count receives the
+ * number of steps monitored since wcns_setup.  Synchronises the stream. */
+int wcns_residual_history(wcns_ctx *ctx, int max, double *rms, double *resmax, int *loc, long long *count);
 
 /* Tile-shape autotuner of the pipelined sweeps (no counterpart in the
  * reference): times the four sweeps of one step, reps times, for each

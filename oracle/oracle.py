@@ -53,6 +53,7 @@ def lib():
             getattr(L, n).argtypes = [C.c_void_p]
         L.o_residual.argtypes = [C.c_void_p, dptr, dptr, iptr]
         L.o_step.argtypes = [C.c_void_p, C.c_int, dptr]
+        L.o_set_threads.argtypes = [C.c_void_p, C.c_int]
         L.o_counter.restype = C.c_long
         L.o_counter.argtypes = [C.c_void_p, C.c_int]
         L.o_region_images.argtypes = [C.c_void_p, C.c_int, C.c_int, iptr, iptr, iptr, iptr]
@@ -188,6 +189,10 @@ class OracleCase:
     def set_field(self, nb, name, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
         lib().o_set_field(self.h, nb, FIELDS[name], _d(a))
+
+    def set_threads(self, n):
+        """OpenMP threads of the oracle's loops; the results do not depend on it."""
+        lib().o_set_threads(self.h, int(n))
 
     def freestream(self):
         out = np.zeros(8)

@@ -69,6 +69,7 @@ struct o_case {
     int ngroups;          /* groups with more than two points (multi-points) */
     ogroup *groups;
     long n_fallback, n_repair;
+    int nthreads;         /* OpenMP threads of the pencil, node and hyperplane loops (1: serial) */
 };
 
 /* ---------------------------------------------------------------- indexing */
@@ -620,8 +621,17 @@ o_case *o_case_create(const o_params *p)
     c->prm = *p;
     c->cap = 8;
     c->blk = (oblock *)calloc(c->cap, sizeof(oblock));
+    c->nthreads = 1;
     return c;
 }
+
+/* Threads of the CPU timing run.  Every parallel loop writes disjoint nodes
+ * and sums nothing in floating point across threads, and the sweeps run
+ * hyperplane by hyperplane (i+j+k = const), which hands every node the same
+ * operands as the lexicographic loops of LHS.f90; the results are therefore
+ * the serial ones bit for bit. */
+void o_set_threads(o_case *c, int n) { c->nthreads = n < 1 ? 1 : n; }
+int o_view_threads(const o_case *c) { return c->nthreads; }
 
 static void block_free(oblock *b)
 {
@@ -1500,6 +1510,7 @@ int o_initial_q_dq_c_t(o_case *c)
         oblock *b = &c->blk[nb];
         int ni = b->ni, nj = b->nj, nk = b->nk;
         /* interior */
+        #pragma omp parallel for collapse(2) num_threads(c->nthreads) if (c->nthreads > 1)
         for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
             long q = G(b, i, j, k);
             double a2 = gama * b->p[q] / b->r[q];
@@ -1535,6 +1546,7 @@ int o_initial_q_dq_c_t(o_case *c)
         }
         if (nvis == 1) {
             /* GasModel.f90:219-235, cmethod = 0 -> 1..n */
+            #pragma omp parallel for collapse(2) num_threads(c->nthreads) if (c->nthreads > 1)
             for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
                 long q = G(b, i, j, k);
                 double tm = b->t[q];
@@ -1542,6 +1554,7 @@ int o_initial_q_dq_c_t(o_case *c)
             }
         }
         /* get_q_only: interior with prim_to_q, ghost layer 1 with prim_to_q_bc */
+        #pragma omp parallel for collapse(2) num_threads(c->nthreads) if (c->nthreads > 1)
         for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
             long q = G(b, i, j, k);
             double prim[5] = {b->r[q], b->u[q], b->v[q], b->w[q], b->p[q]};
@@ -1595,6 +1608,7 @@ int o_timestep(o_case *c)
     for (int nb = 0; nb < c->nblocks; nb++) {
         oblock *b = &c->blk[nb];
         int ni = b->ni, nj = b->nj, nk = b->nk;
+        #pragma omp parallel for collapse(2) num_threads(c->nthreads) if (c->nthreads > 1)
         for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++)
             spect_point(b, i, j, k, i, j, k);
         for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) { spect_point(b, 0, j, k, 1, j, k); spect_point(b, ni + 1, j, k, ni, j, k); }
@@ -1605,6 +1619,7 @@ int o_timestep(o_case *c)
             o_spectvis(c, nb);
         }
         /* localdt0 */
+        #pragma omp parallel for collapse(2) num_threads(c->nthreads) if (c->nthreads > 1)
         for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) {
             long q = G(b, i, j, k);
             double ra = b->sra[q], rb = b->srb[q], rc = b->src[q];
@@ -1621,7 +1636,9 @@ int o_timestep(o_case *c)
     return 0;
 }
 
-/* RHS.f90:35-72 r_h_s_invis -> Convect.f90:2-649 invcode/inviscd3d */
+/* RHS.f90:35-72 r_h_s_invis -> Convect.f90:2-649 invcode/inviscd3d.  The
+ * pencils of one direction write disjoint nodes; with several threads each
+ * thread takes whole pencils with line buffers of its own. */
 int o_rhs_invis(o_case *c)
 {
     int scheme = c->prm.nscheme, flux = c->prm.nflux;
@@ -1630,41 +1647,49 @@ int o_rhs_invis(o_case *c)
         oblock *b = &c->blk[nb];
         int ni = b->ni, nj = b->nj, nk = b->nk;
         int nmax = ni > nj ? ni : nj; nmax = nmax > nk ? nmax : nk;
-        double *ql = (double *)calloc(6 * (nmax + 6) * 2, sizeof(double));
-        double *tr = (double *)calloc(5 * nmax, sizeof(double));
-        double *fc = (double *)calloc(5 * nmax, sizeof(double));
         for (int dir = 0; dir < 3; dir++) {
             int nd = dir == 0 ? ni : (dir == 1 ? nj : nk);
             if (nd <= NIJK2D) continue;
             int na = dir == 0 ? nj : ni;          /* inner loop extent */
             int nbo = dir == 2 ? nj : nk;         /* outer loop extent */
-            for (int ob = 1; ob <= nbo; ob++) for (int ib = 1; ib <= na; ib++) {
-                for (int l = -2; l <= nd + 3; l++) {
-                    int i = dir == 0 ? l : ib, j = dir == 1 ? l : (dir == 0 ? ib : ob), k = dir == 2 ? l : ob;
-                    long q = G(b, i, j, k);
-                    double *o = &ql[6 * (l + 2)];
-                    o[0] = b->r[q]; o[1] = b->u[q]; o[2] = b->v[q]; o[3] = b->w[q]; o[4] = b->p[q]; o[5] = gama;
+            long nfall = 0;
+            int err = 0;
+            #pragma omp parallel num_threads(c->nthreads) if (c->nthreads > 1) reduction(+:nfall) reduction(|:err)
+            {
+                double *ql = (double *)calloc(6 * (nmax + 6) * 2, sizeof(double));
+                double *tr = (double *)calloc(5 * nmax, sizeof(double));
+                double *fc = (double *)calloc(5 * nmax, sizeof(double));
+                #pragma omp for collapse(2) schedule(static)
+                for (int ob = 1; ob <= nbo; ob++) for (int ib = 1; ib <= na; ib++) {
+                    for (int l = -2; l <= nd + 3; l++) {
+                        int i = dir == 0 ? l : ib, j = dir == 1 ? l : (dir == 0 ? ib : ob), k = dir == 2 ? l : ob;
+                        long q = G(b, i, j, k);
+                        double *o = &ql[6 * (l + 2)];
+                        o[0] = b->r[q]; o[1] = b->u[q]; o[2] = b->v[q]; o[3] = b->w[q]; o[4] = b->p[q]; o[5] = gama;
+                    }
+                    for (int l = 1; l <= nd; l++) {
+                        int i = dir == 0 ? l : ib, j = dir == 1 ? l : (dir == 0 ? ib : ob), k = dir == 2 ? l : ob;
+                        long q = G(b, i, j, k);
+                        double *o = &tr[5 * (l - 1)];
+                        if (dir == 0) { o[0] = b->kcx[q]; o[1] = b->kcy[q]; o[2] = b->kcz[q]; }
+                        if (dir == 1) { o[0] = b->etx[q]; o[1] = b->ety[q]; o[2] = b->etz[q]; }
+                        if (dir == 2) { o[0] = b->ctx[q]; o[1] = b->cty[q]; o[2] = b->ctz[q]; }
+                        o[3] = 0.0; o[4] = b->vol[q];
+                    }
+                    int ne = o_wcns_line(scheme, flux, nd, nmax, efix, gama, tr, ql, fc);
+                    if (ne < 0) { err = 1; continue; }
+                    nfall += ne;
+                    for (int l = 1; l <= nd; l++) {
+                        int i = dir == 0 ? l : ib, j = dir == 1 ? l : (dir == 0 ? ib : ob), k = dir == 2 ? l : ob;
+                        long q = G(b, i, j, k);
+                        for (int m = 0; m < 5; m++) b->dq[5 * q + m] = b->dq[5 * q + m] + fc[5 * (l - 1) + m];
+                    }
                 }
-                for (int l = 1; l <= nd; l++) {
-                    int i = dir == 0 ? l : ib, j = dir == 1 ? l : (dir == 0 ? ib : ob), k = dir == 2 ? l : ob;
-                    long q = G(b, i, j, k);
-                    double *o = &tr[5 * (l - 1)];
-                    if (dir == 0) { o[0] = b->kcx[q]; o[1] = b->kcy[q]; o[2] = b->kcz[q]; }
-                    if (dir == 1) { o[0] = b->etx[q]; o[1] = b->ety[q]; o[2] = b->etz[q]; }
-                    if (dir == 2) { o[0] = b->ctx[q]; o[1] = b->cty[q]; o[2] = b->ctz[q]; }
-                    o[3] = 0.0; o[4] = b->vol[q];
-                }
-                int ne = o_wcns_line(scheme, flux, nd, nmax, efix, gama, tr, ql, fc);
-                if (ne < 0) { free(ql); free(tr); free(fc); return -2; }
-                c->n_fallback += ne;
-                for (int l = 1; l <= nd; l++) {
-                    int i = dir == 0 ? l : ib, j = dir == 1 ? l : (dir == 0 ? ib : ob), k = dir == 2 ? l : ob;
-                    long q = G(b, i, j, k);
-                    for (int m = 0; m < 5; m++) b->dq[5 * q + m] = b->dq[5 * q + m] + fc[5 * (l - 1) + m];
-                }
+                free(ql); free(tr); free(fc);
             }
+            c->n_fallback += nfall;
+            if (err) return -2;
         }
-        free(ql); free(tr); free(fc);
     }
     return 0;
 }
@@ -1966,6 +1991,77 @@ static int update_limited(o_case *c, oblock *b)
     return 0;
 }
 
+/* lusgs_l / lusgs_u node bodies for the hyperplane order (the same
+ * arithmetic as the lexicographic loops above, one node at a time) */
+static void lusgs_l_node(o_case *c, oblock *b, int i, int j, int k)
+{
+    double gama = c->prm.gama;
+    int nvis = c->prm.nvis;
+    long q = G(b, i, j, k);
+    double coed = getcoed(b, nvis, q);
+    double dqi[5], dqj[5], dqk[5], pi[5], pj[5], pk[5], gi[4], gj[4], gk[4], de[5], df[5], dg[5];
+    for (int m = 1; m <= 5; m++) { dqi[m-1] = DQ(b,m,i-1,j,k); dqj[m-1] = DQ(b,m,i,j-1,k); dqk[m-1] = DQ(b,m,i,j,k-1); }
+    getprim(b, i - 1, j, k, pi); getprim(b, i, j - 1, k, pj); getprim(b, i, j, k - 1, pk);
+    getrkec(b, i - 1, j, k, 0, gi); getrkec(b, i, j - 1, k, 1, gj); getrkec(b, i, j, k - 1, 2, gk);
+    double ra, rb, rc;
+    getrarbrc(b, i, j, k, -1, &ra, &rb, &rc);
+    o_mxdq_std(pi, gi, dqi, de, ra, 1, gama);
+    o_mxdq_std(pj, gj, dqj, df, rb, 1, gama);
+    o_mxdq_std(pk, gk, dqk, dg, rc, 1, gama);
+    double rhs0[5];
+    for (int m = 0; m < 5; m++) rhs0[m] = 1.0 * (de[m] + df[m] + dg[m]);
+    if (nvis > 0) {
+        double rva = b->srva[G(b,i-1,j,k)], rvb = b->srvb[G(b,i,j-1,k)], rvc = b->srvc[G(b,i,j,k-1)];
+        for (int m = 1; m <= 5; m++)
+            rhs0[m-1] = rhs0[m-1] + 0.5 * (rva * DQ(b,m,i-1,j,k) + rvb * DQ(b,m,i,j-1,k) + rvc * DQ(b,m,i,j,k-1));
+    }
+    for (int m = 1; m <= 5; m++) DQ(b,m,i,j,k) = (-DQ(b,m,i,j,k) + rhs0[m-1]) * coed;
+}
+
+static void lusgs_u_node(o_case *c, oblock *b, int i, int j, int k)
+{
+    double gama = c->prm.gama;
+    int nvis = c->prm.nvis;
+    long q = G(b, i, j, k);
+    double coed = getcoed(b, nvis, q);
+    double dqi[5], dqj[5], dqk[5], pi[5], pj[5], pk[5], gi[4], gj[4], gk[4], de[5], df[5], dg[5];
+    for (int m = 1; m <= 5; m++) { dqi[m-1] = DQ(b,m,i+1,j,k); dqj[m-1] = DQ(b,m,i,j+1,k); dqk[m-1] = DQ(b,m,i,j,k+1); }
+    getprim(b, i + 1, j, k, pi); getprim(b, i, j + 1, k, pj); getprim(b, i, j, k + 1, pk);
+    getrkec(b, i + 1, j, k, 0, gi); getrkec(b, i, j + 1, k, 1, gj); getrkec(b, i, j, k + 1, 2, gk);
+    double ra, rb, rc;
+    getrarbrc(b, i, j, k, 1, &ra, &rb, &rc);
+    o_mxdq_std(pi, gi, dqi, de, ra, -1, gama);
+    o_mxdq_std(pj, gj, dqj, df, rb, -1, gama);
+    o_mxdq_std(pk, gk, dqk, dg, rc, -1, gama);
+    double rhs0[5];
+    for (int m = 0; m < 5; m++) rhs0[m] = 1.0 * (de[m] + df[m] + dg[m]);
+    if (nvis > 0) {
+        double rva = b->srva[G(b,i+1,j,k)], rvb = b->srvb[G(b,i,j+1,k)], rvc = b->srvc[G(b,i,j,k+1)];
+        for (int m = 1; m <= 5; m++)
+            rhs0[m-1] = rhs0[m-1] - 0.5 * (rva * DQ(b,m,i+1,j,k) + rvb * DQ(b,m,i,j+1,k) + rvc * DQ(b,m,i,j,k+1));
+    }
+    for (int m = 1; m <= 5; m++) DQ(b,m,i,j,k) = DQ(b,m,i,j,k) - rhs0[m-1] * coed;
+}
+
+/* One sweep in hyperplane order: planes s = i+j+k ascending (fwd) or
+ * descending; the nodes of one plane read only nodes of the neighbouring
+ * planes, so they run in parallel.  kind 0 lusgs_l, 1 lusgs_u, 2 gs_pr. */
+static void sweep_planes(o_case *c, oblock *b, int kind, int fwd)
+{
+    int ni = b->ni, nj = b->nj, nk = b->nk;
+    for (int t = 0; t <= ni + nj + nk - 3; t++) {
+        int s = fwd ? 3 + t : ni + nj + nk - t;
+        #pragma omp parallel for collapse(2) schedule(static) num_threads(c->nthreads)
+        for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) {
+            int i = s - j - k;
+            if (i < 1 || i > ni) continue;
+            if (kind == 0) lusgs_l_node(c, b, i, j, k);
+            else if (kind == 1) lusgs_u_node(c, b, i, j, k);
+            else gs_pr_node(c, b, i, j, k);
+        }
+    }
+}
+
 /* LHS.f90:1-34 l_h_s_tgh -> LHS.f90:1559-1587 GS_PR_Single */
 int o_lhs(o_case *c)
 {
@@ -1976,12 +2072,19 @@ int o_lhs(o_case *c)
             long q = G(b, i, j, k);
             for (int m = 0; m < 5; m++) b->rhs_nb[5 * q + m] = b->dq[5 * q + m];
         }
-        lusgs_l(c, b);
-        lusgs_u(c, b);
-        for (int k = 1; k <= b->nk; k++) for (int j = 1; j <= b->nj; j++) for (int i = 1; i <= b->ni; i++)
-            gs_pr_node(c, b, i, j, k);
-        for (int k = b->nk; k >= 1; k--) for (int j = b->nj; j >= 1; j--) for (int i = b->ni; i >= 1; i--)
-            gs_pr_node(c, b, i, j, k);
+        if (c->nthreads > 1) {
+            sweep_planes(c, b, 0, 1);
+            sweep_planes(c, b, 1, 0);
+            sweep_planes(c, b, 2, 1);
+            sweep_planes(c, b, 2, 0);
+        } else {
+            lusgs_l(c, b);
+            lusgs_u(c, b);
+            for (int k = 1; k <= b->nk; k++) for (int j = 1; j <= b->nj; j++) for (int i = 1; i <= b->ni; i++)
+                gs_pr_node(c, b, i, j, k);
+            for (int k = b->nk; k >= 1; k--) for (int j = b->nj; j >= 1; j--) for (int i = b->ni; i >= 1; i--)
+                gs_pr_node(c, b, i, j, k);
+        }
         dq_to_zero_physical(b);
         int e = update_limited(c, b);
         if (e) return e;

@@ -63,6 +63,9 @@ int     o_lhs(o_case *c);
 int     o_communicate_pv(o_case *c);
 int     o_residual(o_case *c, double *rms, double *resmax, int loc[5]);
 int     o_step(o_case *c, int nsteps, double *rms_hist);
+/* OpenMP threads of the pencil, node and hyperplane loops (default 1); the
+ * results do not depend on it */
+void    o_set_threads(o_case *c, int n);
 long    o_counter(const o_case *c, int which);   /* 0 wcns fallbacks, 1 repairs */
 /* Spalart-Allmaras one-equation model (turblence.f90), per block, as called
  * from r_h_s_vis before the mean-flow viscous flux (RHS.f90:22) */

@@ -22,6 +22,7 @@ typedef struct {
 
 int o_view_block(o_case *c, int nb, oblock_view *v);
 void o_view_params(const o_case *c, o_params *p, double *visc);
+int o_view_threads(const o_case *c);
 double o_view_muoo(const o_case *c);
 void o_view_turb_inflow(const o_case *c, double *koo, double *omgoo);
 

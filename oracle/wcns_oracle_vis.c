@@ -157,6 +157,7 @@ int o_rhs_vis(o_case *c)
     double re = 1.0 / prm.reynolds;
     double cp = 1.0 / ((prm.gama - 1.0) * prm.moo * prm.moo);
     double cp_prl = cp / prm.prl, cp_prt = cp / prm.prt;
+    const int nthr = o_view_threads(c);
     for (int nb = 0;; nb++) {
         oblock_view B;
         if (o_view_block(c, nb, &B)) break;
@@ -170,12 +171,16 @@ int o_rhs_vis(o_case *c)
         double *duvwt = (double *)calloc(12 * N, sizeof(double));        /* duvwt(12,i,j,k) */
 #define DNI(m,i,j,k) duvwt[(m) + 12 * ((long)((i) - 1) + (long)ni * ((long)((j) - 1) + (long)nj * ((k) - 1)))]
 #define DN(m, ...) DNI(m, __VA_ARGS__)
-        double *uv = (double *)calloc(4 * (nmax + 6), sizeof(double));
-        double *dt = (double *)calloc(4 * (nmax + 1), sizeof(double));
-        /* UVWT_DER_4th_virtual */
+        /* UVWT_DER_4th_virtual; the pencils of one direction write disjoint
+         * nodes, so threads take whole pencils with line buffers of their own */
         for (int dir = 0; dir < 3; dir++) {
             int n = dir == 0 ? ni : (dir == 1 ? nj : nk);
             int na = dir == 0 ? nj : ni, nbb = dir == 2 ? nj : nk;
+            #pragma omp parallel num_threads(nthr) if (nthr > 1)
+            {
+            double *uv = (double *)calloc(4 * (nmax + 6), sizeof(double));
+            double *dt = (double *)calloc(4 * (nmax + 1), sizeof(double));
+            #pragma omp for collapse(2) schedule(static)
             for (int b2 = 1; b2 <= nbb; b2++) for (int b1 = 1; b1 <= na; b1++) {
 #define IJK(l) (dir == 0 ? (l) : b1), (dir == 1 ? (l) : (dir == 0 ? b1 : b2)), (dir == 2 ? (l) : b2)
                 int st, ed;
@@ -190,8 +195,16 @@ int o_rhs_vis(o_case *c)
                 for (int l = 1; l <= n; l++)
                     for (int m = 0; m < 4; m++) DN(m + 4 * dir, IJK(l)) = dt[m + 4 * (l - 1)];
             }
+            free(uv); free(dt);
+            }
         }
         /* the three directions of WCNSE5_VIS_virtual */
+        for (int dir = 0; dir < 3; dir++) {
+            int n = dir == 0 ? ni : (dir == 1 ? nj : nk);
+            int na = dir == 0 ? nj : ni, nbb = dir == 2 ? nj : nk;
+            #pragma omp parallel num_threads(nthr) if (nthr > 1)
+            {
+        double *uv = (double *)calloc(4 * (nmax + 6), sizeof(double));
         double *uvh = (double *)calloc(4 * (nmax + 1), sizeof(double));
         double *dmid = (double *)calloc(4 * (nmax + 1), sizeof(double));
         double *kx = (double *)calloc(9 * nmax, sizeof(double)), *kxh = (double *)calloc(9 * (nmax + 1), sizeof(double));
@@ -200,9 +213,7 @@ int o_rhs_vis(o_case *c)
         double *s2 = (double *)calloc(nmax, sizeof(double)), *s2h = (double *)calloc(nmax + 1, sizeof(double));
         double *dl = (double *)calloc(12 * nmax, sizeof(double)), *dh = (double *)calloc(12 * (nmax + 1), sizeof(double));
         double *fv = (double *)calloc(5 * (nmax + 1), sizeof(double)), *dfv = (double *)calloc(5 * nmax, sizeof(double));
-        for (int dir = 0; dir < 3; dir++) {
-            int n = dir == 0 ? ni : (dir == 1 ? nj : nk);
-            int na = dir == 0 ? nj : ni, nbb = dir == 2 ? nj : nk;
+            #pragma omp for collapse(2) schedule(static)
             for (int b2 = 1; b2 <= nbb; b2++) for (int b1 = 1; b1 <= na; b1++) {
                 for (int l = 1; l <= n; l++) {
                     long q = G(b, IJK(l));
@@ -257,9 +268,11 @@ int o_rhs_vis(o_case *c)
                 }
 #undef IJK
             }
-        }
-        free(duvwt); free(uv); free(dt); free(uvh); free(dmid); free(kx); free(kxh); free(vl); free(vlh);
+        free(uv); free(uvh); free(dmid); free(kx); free(kxh); free(vl); free(vlh);
         free(s1); free(s1h); free(s2); free(s2h); free(dl); free(dh); free(fv); free(dfv);
+            }
+        }
+        free(duvwt);
 #undef DN
     }
     return 0;
@@ -288,6 +301,8 @@ int o_spectvis(o_case *c, int nb)
         double coef1 = prm.csrv * coef;                                                  \
         b->srva[q] = coef1 * ri; b->srvb[q] = coef1 * rj; b->srvc[q] = coef1 * rk;       \
     } while (0)
+    const int nthr = o_view_threads(c);
+    #pragma omp parallel for collapse(2) num_threads(nthr) if (nthr > 1)
     for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) for (int i = 1; i <= ni; i++) PT(i, j, k, i, j, k);
     for (int k = 1; k <= nk; k++) for (int j = 1; j <= nj; j++) { PT(0, j, k, 1, j, k); PT(ni + 1, j, k, ni, j, k); }
     for (int k = 1; k <= nk; k++) for (int i = 1; i <= ni; i++) { PT(i, 0, k, i, 1, k); PT(i, nj + 1, k, i, nj, k); }

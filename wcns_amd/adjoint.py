@@ -32,13 +32,22 @@ def state_tensors(solver, nb, device="cuda"):
     return P.contiguous(), X.contiguous()
 
 
+def _check_tensor(name, t, shape, solver):
+    """t must be a contiguous float64 tensor of this shape on the solver's GPU:
+    the library reads and writes through its raw address."""
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous float64 GPU tensor")
+    dev = getattr(solver, "device", None)
+    if dev is not None and t.device.index != dev:
+        raise ValueError(f"{name} is on cuda:{t.device.index}, the solver on cuda:{dev}")
+
+
 def _check(P, X, solver, nb):
     ni, nj, nk = _blk(solver, nb)
-    if P.shape != (5, nk + 6, nj + 6, ni + 6) or X.shape != (3, nk, nj, ni):
-        raise ValueError(f"P {tuple(P.shape)} / X {tuple(X.shape)} do not fit block {nb} ({ni}x{nj}x{nk})")
-    for t in (P, X):
-        if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
-            raise ValueError("P and X must be contiguous float64 GPU tensors")
+    _check_tensor("P", P, (5, nk + 6, nj + 6, ni + 6), solver)
+    _check_tensor("X", X, (3, nk, nj, ni), solver)
 
 
 def residual(solver: Solver, nb, P, X):
@@ -58,6 +67,10 @@ def jvp(solver: Solver, nb, P, X, dP=None, dX=None):
     dR = torch.empty_like(R)
     dP = None if dP is None else dP.contiguous()
     dX = None if dX is None else dX.contiguous()
+    if dP is not None:
+        _check_tensor("dP", dP, P.shape, solver)
+    if dX is not None:
+        _check_tensor("dX", dX, X.shape, solver)
     torch.cuda.synchronize(P.device)
     solver.adj_jvp(nb, P.data_ptr(), X.data_ptr(), 0 if dP is None else dP.data_ptr(),
                    0 if dX is None else dX.data_ptr(), R.data_ptr(), dR.data_ptr())
@@ -67,7 +80,9 @@ def jvp(solver: Solver, nb, P, X, dP=None, dX=None):
 def vjp(solver: Solver, nb, P, X, W):
     """(gP, gX) = ((dR/dP)^T W, (dR/dX)^T W)."""
     _check(P, X, solver, nb)
+    ni, nj, nk = _blk(solver, nb)
     W = W.contiguous()
+    _check_tensor("W", W, (5, nk, nj, ni), solver)
     gP = torch.empty_like(P)
     gX = torch.empty_like(X)
     torch.cuda.synchronize(P.device)

@@ -114,11 +114,16 @@ struct wcns_ctx {
     int2 *d_rwork = nullptr;
     std::vector<int> bc_lvl, if_lvl;        // work offsets of each level, then the end
     int zero_w0 = 0, zero_nw = 0;           // the physical faces' work entries
+    // residual monitor: 3 partials per GRID_RESID workgroup, each block's
+    // first workgroup (nblocks + 1), per-block (sum, max, argmax), the latest
+    // entry (8 doubles) and the on-device history ring of whole steps
     double *d_partial = nullptr;
-    int max_parts = 1;
+    int *d_res_wg0 = nullptr;
     double *d_resblk = nullptr;
+    double *d_res_out = nullptr;
     double *d_hist = nullptr;
-    int hist_cap = 0;
+    unsigned long long *d_hist_n = nullptr;
+    int hist_cap = 8192;
     hipGraphExec_t graph[2] = {nullptr, nullptr};   // two-step graphs, by starting primitive buffer
     int graph_prec = -1;                            // implicit-operator precision they were captured with
 
@@ -490,7 +495,7 @@ int wcns_destroy(wcns_ctx *ctx)
     void *pl[] = {ctx->d_sblocks, ctx->d_work, ctx->d_xch};
     for (void *p : pl) if (p) hipFree(p);
     void *ps[] = {ctx->fvb, ctx->tpool, ctx->d_blocks_t, ctx->d_grid, ctx->qkw, ctx->qke, ctx->vist, ctx->dist, ctx->sa_pool, ctx->skew[0], ctx->skew[1], ctx->dq32, ctx->coed, ctx->d_sweep_err, ctx->dn, ctx->t, ctx->rhs, ctx->dq, ctx->dtdt, ctx->visl, ctx->counters, ctx->d_blocks, ctx->d_flag, ctx->d_nflag, ctx->d_regs, ctx->d_rwork, ctx->d_skoff,
-                  ctx->d_partial, ctx->d_resblk, ctx->d_hist, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
+                  ctx->d_partial, ctx->d_res_wg0, ctx->d_resblk, ctx->d_res_out, ctx->d_hist, ctx->d_hist_n, ctx->d_npp_pts, ctx->d_grp_start, ctx->d_grp_n,
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
     for (auto &w : ctx->walls) hipFree(w.xyz);
@@ -821,9 +826,8 @@ int wcns_setup(wcns_ctx *ctx)
     TRY(dalloc(ctx, &ctx->d_flag, (size_t)ctx->flag_cap * ctx->blocks.size()));
     TRY(dalloc(ctx, &ctx->d_nflag, ctx->blocks.size()));
     std::vector<DBlock> hb;
-    int maxp = 1;
-    for (auto &b : ctx->blocks) { hb.push_back(b.db); maxp = std::max(maxp, residual_parts(b.db)); }
-    ctx->max_parts = maxp;
+    std::vector<int> res_wg0(1, 0);
+    for (auto &b : ctx->blocks) { hb.push_back(b.db); res_wg0.push_back(res_wg0.back() + residual_parts(b.db)); }
     TRY(dalloc(ctx, &ctx->d_blocks, hb.size()));
     TRY(copy_sync(ctx, ctx->d_blocks, hb.data(), sizeof(DBlock) * hb.size(), hipMemcpyHostToDevice));
     // transposed views of the blocks (i <-> j), for the i-direction residuals
@@ -855,7 +859,8 @@ int wcns_setup(wcns_ctx *ctx)
             // it, so launches of at most XCD_SLICE_MIN workgroups keep the
             // plain order.
             const long n = (long)lst.size(), q = n / XCDS, r = n % XCDS;
-            const bool slices = kind == GRID_TRANS || n > XCD_SLICE_MIN;
+            // the residual monitor's partials are combined per block in list order
+            const bool slices = kind != GRID_RESID && (kind == GRID_TRANS || n > XCD_SLICE_MIN);
             for (long pos = 0; pos < n; pos++) {
                 const long x = pos % XCDS, idx = pos / XCDS;
                 const long first = x * q + std::min(x, r);
@@ -885,8 +890,14 @@ int wcns_setup(wcns_ctx *ctx)
         ctx->dn_t = q; q += 12 * T;
         ctx->df_t = q;
     }
-    TRY(dalloc(ctx, &ctx->d_partial, 3 * maxp));
+    TRY(dalloc(ctx, &ctx->d_partial, 3 * (size_t)res_wg0.back()));
+    TRY(dalloc(ctx, &ctx->d_res_wg0, res_wg0.size()));
+    TRY(copy_sync(ctx, ctx->d_res_wg0, res_wg0.data(), sizeof(int) * res_wg0.size(), hipMemcpyHostToDevice));
     TRY(dalloc(ctx, &ctx->d_resblk, 3 * ctx->blocks.size()));
+    TRY(dalloc(ctx, &ctx->d_res_out, 8));
+    TRY(dalloc(ctx, &ctx->d_hist, 8 * (size_t)ctx->hist_cap));
+    TRY(dalloc(ctx, &ctx->d_hist_n, 1));
+    CK(hipMemsetAsync(ctx->d_hist_n, 0, sizeof(unsigned long long), ctx->stream));
 
     // freestream (PreProcess.f90:968-1113, height < 0)
     DParams &d = ctx->dp;
@@ -1423,42 +1434,55 @@ int wcns_l_h_s(wcns_ctx *ctx)
     return WCNS_OK;
 }
 
+// residual_curve of the current dq on the stream; push appends it to the
+// on-device history (the per-step monitor of Main.f90:109), no host wait
+static int launch_monitor(wcns_ctx *ctx, bool push)
+{
+    Scope sc(ctx, WCNS_T_RESIDUAL, 3);
+    if (launch_residual_all(ctx->grid[GRID_RESID], ctx->fields(), ctx->d_partial, ctx->d_res_wg0,
+                            (int)ctx->blocks.size(), ctx->d_resblk, ctx->d_blocks, ctx->d_res_out,
+                            push ? ctx->d_hist : nullptr, ctx->d_hist_n, ctx->hist_cap, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "residual monitor launch");
+    return WCNS_OK;
+}
+
+static void entry_out(const double *e, double *rms, double *resmax, int *loc)
+{
+    if (rms) *rms = e[0];
+    if (resmax) *resmax = e[1];
+    if (loc) for (int m = 0; m < 5; m++) loc[m] = (int)e[2 + m];
+}
+
 int wcns_residual(wcns_ctx *ctx, double *rms, double *resmax, int loc[5])
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
-    DFields F = ctx->fields();
-    std::vector<double> res(3 * ctx->blocks.size());
-    Scope *sc = new Scope(ctx, WCNS_T_RESIDUAL);
-    for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
-        const DBlock &db = ctx->blocks[nb].db;
-        int np = residual_parts(db);
-        launch_residual(db, F, ctx->d_partial, np, ctx->stream);
-        launch_residual_final(ctx->d_partial, np, ctx->d_resblk + 3 * nb, ctx->stream);
-    }
-    delete sc;
-    CK(hipMemcpyAsync(res.data(), ctx->d_resblk, 8 * res.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TRY(launch_monitor(ctx, false));
+    double e[8];
+    CK(hipMemcpyAsync(e, ctx->d_res_out, sizeof e, hipMemcpyDeviceToHost, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     TRY(check_counters(ctx));
-    double sum = 0.0, mx = 0.0;
-    long npt = 0;
-    int l[5] = {2, 2, 2, 1, 1};
-    for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
-        const HBlock &b = ctx->blocks[nb];
-        sum = sum + res[3 * nb];
-        npt += (long)(b.ni - 2) * (b.nj - 2) * (b.nk - 2) * 5;
-        if (mx < res[3 * nb + 1]) {
-            mx = res[3 * nb + 1];
-            long a = (long)res[3 * nb + 2];
-            int m = (int)(a % 5);
-            long t = a / 5;
-            int mi = b.ni - 2, mj = b.nj - 2;
-            l[0] = (int)(t % mi) + 2; l[1] = (int)((t / mi) % mj) + 2; l[2] = (int)(t / ((long)mi * mj)) + 2;
-            l[3] = m + 1; l[4] = (int)nb + 1;
-        }
+    entry_out(e, rms, resmax, loc);
+    return WCNS_OK;
+}
+
+int wcns_residual_history(wcns_ctx *ctx, int max, double *rms, double *resmax, int *loc, long long *count)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (max < 0 || (max > 0 && !rms && !resmax && !loc)) return WCNS_E_ARG;
+    unsigned long long n = 0;
+    CK(hipMemcpyAsync(&n, ctx->d_hist_n, sizeof n, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (count) *count = (long long)n;
+    const long want = std::min<long>(std::min<long>(max, (long)n), ctx->hist_cap);
+    if (want <= 0) return WCNS_OK;
+    std::vector<double> h(8 * (size_t)ctx->hist_cap);
+    CK(hipMemcpyAsync(h.data(), ctx->d_hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    for (long q = 0; q < want; q++) {
+        const unsigned long long idx = n - (unsigned long long)want + (unsigned long long)q;
+        const double *e = h.data() + 8 * (size_t)(idx % (unsigned long long)ctx->hist_cap);
+        entry_out(e, rms ? rms + q : nullptr, resmax ? resmax + q : nullptr, loc ? loc + 5 * q : nullptr);
     }
-    if (rms) *rms = sqrt(sum / (double)npt);
-    if (resmax) *resmax = mx;
-    if (loc) memcpy(loc, l, sizeof l);
     return WCNS_OK;
 }
 
@@ -1478,7 +1502,7 @@ static int step_once(wcns_ctx *ctx)
     TRY(wcns_communicate_dq(ctx));
     TRY(wcns_l_h_s(ctx));
     TRY(wcns_communicate_pv(ctx));
-    return WCNS_OK;
+    return launch_monitor(ctx, true);
 }
 
 // Captures two consecutive steps (the primitive buffers alternate, so a pair
@@ -1539,18 +1563,14 @@ int wcns_step(wcns_ctx *ctx, int nsteps, double *rms_hist)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     if (nsteps < 0) return WCNS_E_ARG;
-    for (int s = 0; s < nsteps; s++) {
-        TRY(step_once(ctx));
-        if (rms_hist) {
-            double rms, rmax;
-            int loc[5];
-            TRY(wcns_residual(ctx, &rms, &rmax, loc));
-            rms_hist[s] = rms;
-        }
-    }
-    if (!rms_hist) {
+    // the monitor of every step goes to the device history; the host reads
+    // it once per hist_cap steps at most (before the ring wraps)
+    for (int s0 = 0; s0 < nsteps; s0 += ctx->hist_cap) {
+        const int ns = std::min(nsteps - s0, ctx->hist_cap);
+        for (int s = 0; s < ns; s++) TRY(step_once(ctx));
         CK(hipStreamSynchronize(ctx->stream));
         TRY(check_counters(ctx));
+        if (rms_hist) TRY(wcns_residual_history(ctx, ns, rms_hist + s0, nullptr, nullptr, nullptr));
     }
     return WCNS_OK;
 }

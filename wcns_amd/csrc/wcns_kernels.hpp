@@ -62,7 +62,8 @@ enum {
     GRID_VFACE = 15,              // + d (0: i on the transposed view, 1: j, 2: k): viscous half nodes
     GRID_VROW = 18,               // + d: viscous FLUX_DXYZ rows
     GRID_SKEW = 21,               // plane-major staging tiles
-    GRID_N = 22
+    GRID_RESID = 22,              // residual monitor: residual_parts(b) workgroups per block, in block order
+    GRID_N = 23
 };
 long grid_threads(int kind, const DBlock &b);
 int grid_block_size(int kind);
@@ -178,9 +179,15 @@ int launch_avg2pm_level(const DRegion *regs, const int2 *work, int nwork, const 
                         int pv, hipStream_t s);
 int launch_dq_zero_faces(const DRegion *regs, const int2 *work, int nwork, const DBlock *blocks, double *arr,
                          float *arr32, long total, hipStream_t s);
-int launch_residual(const DBlock &b, const DFields &F, double *partial, int nparts, hipStream_t s);
 int residual_parts(const DBlock &b);
-int launch_residual_final(const double *partial, int nparts, double *out, hipStream_t s);
+// residual_curve (PostProcess.f90:1195-1293) of every block over G = the
+// GRID_RESID list (wg0: first workgroup of each block, nblocks + 1 entries):
+// partial (3 per workgroup) -> resblk (sum, max, argmax per block) -> out[8] =
+// rms, max, i, j, k, m, nb, entry number; with hist != nullptr the entry is also
+// appended to the ring hist[cap][8] at *count % cap and *count is advanced.
+int launch_residual_all(const BlockGrid &G, const DFields &F, double *partial, const int *wg0, int nblocks,
+                        double *resblk, const DBlock *blocks, double *out, double *hist, unsigned long long *count,
+                        int cap, hipStream_t s);
 int launch_copy_interior(const DBlock &b, const double *src, double *dst, hipStream_t s);
 
 // Turbulence model scratch of one step (field-pool frame, total elements each);

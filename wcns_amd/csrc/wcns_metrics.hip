@@ -374,7 +374,7 @@ int launch_gcl_jvp(const DBlock &b, const DFields &F, const double *X, const dou
 {
     long slot = g3_slot(b), nn = pencil_nodes(b);
     double *D = work, *dD = work + 9 * slot, *dPm = work + 18 * slot, *dC = work + 27 * slot;
-    hipMemsetAsync(work, 0, sizeof(double) * 45 * slot, s);
+    if (hipMemsetAsync(work, 0, sizeof(double) * 45 * slot, s) != hipSuccess) return -3;
     gcl_pass_a(b, X, D, s);
     for (int c = 0; c < 3; c++)
         for (int dir = 0; dir < 3; dir++)
@@ -398,7 +398,7 @@ int launch_gcl_vjp(const DBlock &b, const DFields &F, const double *X, const dou
 {
     long slot = g3_slot(b), nn = pencil_nodes(b);
     double *D = work, *lamC = work + 9 * slot, *lamD = work + 27 * slot, *lamPm = work + 36 * slot;
-    hipMemsetAsync(work, 0, sizeof(double) * 45 * slot, s);
+    if (hipMemsetAsync(work, 0, sizeof(double) * 45 * slot, s) != hipSuccess) return -3;
     gcl_pass_a(b, X, D, s);
     hipLaunchKernelGGL(k_gcl_rd, grid1(nn, 256), dim3(256), 0, s, b, F, (const double *)D, gm, F.total, lamC, lamD,
                        slot);

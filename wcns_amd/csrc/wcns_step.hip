@@ -564,16 +564,31 @@ __global__ void k_repair(const DBlock *blocks, DFields F, DParams P, int *flags,
 // PostProcess.f90:1240-1293: sum and max of (|dq|/(dtdt*vol))^2 over nodes 2..n-1.
 constexpr int RES_THREADS = 256;
 
-__global__ void __launch_bounds__(RES_THREADS) k_residual(DBlock b, DFields F, double *partial)
+// workgroups of the residual monitor per block: about 8 interior nodes per
+// thread, at most 1024 workgroups
+__host__ __device__ inline int residual_parts_dev(const DBlock &b)
+{
+    long n = (long)(b.ni - 2) * (b.nj - 2) * (b.nk - 2);
+    long p = (n + RES_THREADS * 8 - 1) / (RES_THREADS * 8);
+    return (int)(p < 1 ? 1 : (p > 1024 ? 1024 : p));
+}
+
+// The same monitor for every block in one launch over the GRID_RESID list:
+// block b owns workgroups wg0[b] .. wg0[b+1]-1 (in order, never permuted), and
+// each of them strides over the block's interior nodes 2..n-1.
+__global__ void __launch_bounds__(RES_THREADS) k_residual_all(BlockGrid G, DFields F, double *partial)
 {
     __shared__ double ssum[RES_THREADS];
     __shared__ double smax[RES_THREADS];
     __shared__ long sarg[RES_THREADS];
-    int mi = b.ni - 2, mj = b.nj - 2, mk = b.nk - 2;
-    long n = (long)mi * mj * mk;
+    const int2 w = G.wg[blockIdx.x];
+    const DBlock b = G.blocks[w.x];
+    const int mi = b.ni - 2, mj = b.nj - 2, mk = b.nk - 2;
+    const long n = (long)mi * mj * mk;
+    const long stride = (long)residual_parts_dev(b) * RES_THREADS;
     double sum = 0.0, mx = 0.0;
     long arg = -1;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    for (long t = (long)w.y * RES_THREADS + threadIdx.x; t < n; t += stride) {
         int i = (int)(t % mi) + 2, j = (int)((t / mi) % mj) + 2, k = (int)(t / ((long)mi * mj)) + 2;
         long g = gidx(b, i, j, k);
         double dtime1 = 1.0 / (F.dtdt[g] * F.vol[g]);
@@ -586,11 +601,11 @@ __global__ void __launch_bounds__(RES_THREADS) k_residual(DBlock b, DFields F, d
     }
     ssum[threadIdx.x] = sum; smax[threadIdx.x] = mx; sarg[threadIdx.x] = arg;
     __syncthreads();
-    for (int w = RES_THREADS / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            ssum[threadIdx.x] += ssum[threadIdx.x + w];
-            double om = smax[threadIdx.x + w];
-            long oa = sarg[threadIdx.x + w];
+    for (int h = RES_THREADS / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + h];
+            double om = smax[threadIdx.x + h];
+            long oa = sarg[threadIdx.x + h];
             if (om > smax[threadIdx.x] || (om == smax[threadIdx.x] && oa >= 0 && (sarg[threadIdx.x] < 0 || oa < sarg[threadIdx.x]))) {
                 smax[threadIdx.x] = om; sarg[threadIdx.x] = oa;
             }
@@ -604,15 +619,59 @@ __global__ void __launch_bounds__(RES_THREADS) k_residual(DBlock b, DFields F, d
     }
 }
 
-__global__ void k_residual_final(const double *partial, int nparts, double *out)
+// One workgroup per block: its partials in order -> resblk[3b..3b+2] = (sum, max, argmax).
+__global__ void __launch_bounds__(64) k_residual_blocks(const double *partial, const int *wg0, double *resblk)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (threadIdx.x != 0) return;
+    const int b = blockIdx.x;
     double s = 0.0, mx = 0.0, arg = -1.0;
-    for (int p = 0; p < nparts; p++) {
+    for (int p = wg0[b]; p < wg0[b + 1]; p++) {
         s += partial[3 * p];
         if (partial[3 * p + 1] > mx) { mx = partial[3 * p + 1]; arg = partial[3 * p + 2]; }
     }
-    out[0] = s; out[1] = mx; out[2] = arg;
+    resblk[3 * b] = s; resblk[3 * b + 1] = mx; resblk[3 * b + 2] = arg;
+}
+
+// residual_curve's combination over the blocks (PostProcess.f90:1280-1292):
+// the sums added in block order, the first block with the largest maximum
+// names the location.  out = rms, max, i, j, k, m, nb (1-based), step number.
+// With hist != nullptr the entry is also appended to the history ring
+// hist[cap][8] at position *count % cap, and *count is advanced.
+__global__ void k_residual_hist(const double *resblk, const DBlock *blocks, int nblocks, double *out,
+                                double *hist, unsigned long long *count, int cap)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double sum = 0.0, mx = 0.0;
+    long npt = 0;
+    double l[5] = {2.0, 2.0, 2.0, 1.0, 1.0};
+    for (int nb = 0; nb < nblocks; nb++) {
+        const DBlock b = blocks[nb];
+        sum = sum + resblk[3 * nb];
+        npt += (long)(b.ni - 2) * (b.nj - 2) * (b.nk - 2) * 5;
+        if (mx < resblk[3 * nb + 1]) {
+            mx = resblk[3 * nb + 1];
+            const long a = (long)resblk[3 * nb + 2];
+            const long t = a / 5;
+            const int mi = b.ni - 2, mj = b.nj - 2;
+            l[0] = (double)(t % mi + 2);
+            l[1] = (double)((t / mi) % mj + 2);
+            l[2] = (double)(t / ((long)mi * mj) + 2);
+            l[3] = (double)(a % 5 + 1);
+            l[4] = (double)(nb + 1);
+        }
+    }
+    double e[8];
+    e[0] = sqrt(sum / (double)npt);
+    e[1] = mx;
+    for (int m = 0; m < 5; m++) e[2 + m] = l[m];
+    const unsigned long long n = hist ? *count : 0ull;
+    e[7] = (double)(n + 1);
+    for (int m = 0; m < 8; m++) out[m] = e[m];
+    if (hist) {
+        double *h = hist + 8 * (long)(n % (unsigned long long)cap);
+        for (int m = 0; m < 8; m++) h[m] = e[m];
+        *count = n + 1;
+    }
 }
 
 __global__ void k_zero_interior5(DBlock b, double *arr, long total)
@@ -677,6 +736,7 @@ long grid_threads(int kind, const DBlock &b)
     if (kind == GRID_SA_UPD) return (long)(b.ni + 1) * (b.nj + 1) * (b.nk + 1);
     if (kind == GRID_TRANS) return transpose_tiles(b) * 256;
     if (kind == GRID_SKEW) return skew_stage_wgs(b) * 256;
+    if (kind == GRID_RESID) return (long)residual_parts_dev(b) * RES_THREADS;
     // b is the transposed view here: its j pencils are the original i pencils
     if (kind == GRID_RHS_INV_T) return rhs_inv_threads(1, b);
     if (kind == GRID_RHS_VIS_T) return rhs_vis_threads(1, b);
@@ -688,7 +748,8 @@ long grid_threads(int kind, const DBlock &b)
 
 int grid_block_size(int kind)
 {
-    return (kind == GRID_NODES || kind == GRID_SA_UPD || kind == GRID_TRANS || kind >= GRID_VFACE) ? 256 : 128;   // GRID_SKEW too
+    // GRID_SKEW and GRID_RESID too
+    return (kind == GRID_NODES || kind == GRID_SA_UPD || kind == GRID_TRANS || kind >= GRID_VFACE) ? 256 : 128;
 }
 
 int launch_zero_rhs(const DBlock &b, const DFields &F, hipStream_t s)
@@ -764,22 +825,18 @@ int launch_repair(const DBlock *blocks, int nblocks, const DFields &F, const DPa
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int residual_parts(const DBlock &b)
-{
-    long n = (long)(b.ni - 2) * (b.nj - 2) * (b.nk - 2);
-    long p = (n + RES_THREADS * 8 - 1) / (RES_THREADS * 8);
-    return (int)(p < 1 ? 1 : (p > 1024 ? 1024 : p));
-}
+int residual_parts(const DBlock &b) { return residual_parts_dev(b); }
 
-int launch_residual(const DBlock &b, const DFields &F, double *partial, int nparts, hipStream_t s)
+int launch_residual_all(const BlockGrid &G, const DFields &F, double *partial, const int *wg0, int nblocks,
+                        double *resblk, const DBlock *blocks, double *out, double *hist, unsigned long long *count,
+                        int cap, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_residual, dim3(nparts), dim3(RES_THREADS), 0, s, b, F, partial);
-    return hipGetLastError() == hipSuccess ? 0 : -3;
-}
-
-int launch_residual_final(const double *partial, int nparts, double *out, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(64), 0, s, partial, nparts, out);
+    if (G.nwg <= 0 || nblocks <= 0) return -3;
+    hipLaunchKernelGGL(k_residual_all, dim3(G.nwg), dim3(RES_THREADS), 0, s, G, F, partial);
+    if (hipGetLastError() != hipSuccess) return -3;
+    hipLaunchKernelGGL(k_residual_blocks, dim3(nblocks), dim3(64), 0, s, partial, wg0, resblk);
+    if (hipGetLastError() != hipSuccess) return -3;
+    hipLaunchKernelGGL(k_residual_hist, dim3(1), dim3(64), 0, s, resblk, blocks, nblocks, out, hist, count, cap);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -32,7 +32,7 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_l_h_s", "wcns_communicate_pv", "wcns_residual", "wcns_step", "wcns_get_counter", "wcns_sync",
            "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get", "wcns_force",
            "wcns_step_graph", "wcns_autotune", "wcns_adj_residual", "wcns_adj_jvp", "wcns_adj_vjp",
-           "wcns_adj_force", "wcns_sweep_layout"]
+           "wcns_adj_force", "wcns_sweep_layout", "wcns_residual_history"]
 
 TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
@@ -82,6 +82,7 @@ def load_library(path=LIBPATH):
         getattr(L, n).argtypes = [vp]
     L.wcns_residual.argtypes = [vp, dptr, dptr, iptr]
     L.wcns_step.argtypes = [vp, C.c_int, dptr]
+    L.wcns_residual_history.argtypes = [vp, C.c_int, dptr, dptr, iptr, C.POINTER(C.c_longlong)]
     L.wcns_get_counter.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong)]
     L.wcns_region_images.argtypes = [vp, C.c_int, C.c_int, iptr, iptr, iptr, iptr]
     L.wcns_multipoint.argtypes = [vp, iptr, iptr, iptr, C.c_int]
@@ -119,6 +120,7 @@ class Solver:
     def __init__(self, case, device=0, lhs_precision=0):
         self.L = load_library()
         self.case = case
+        self.device = device
         prm = Params()
         for k, _ in Params._fields_:
             setattr(prm, k, getattr(case.params, k))
@@ -196,6 +198,18 @@ class Solver:
             return hist[:n]
         self._check(self.L.wcns_step(self.h, n, None), "step")
         return None
+
+    def residual_history(self, n):
+        """The monitor of the last n steps run by step / step_graph, oldest first:
+        (rms (n,), max (n,), loc (n, 5) = i, j, k, m, nb), and the number of
+        steps monitored since set-up (wcns_residual_history)."""
+        rms, rmax = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        loc = np.zeros((max(n, 1), 5), dtype=np.int32)
+        cnt = C.c_longlong()
+        self._check(self.L.wcns_residual_history(self.h, n, _d(rms), _d(rmax), _i(loc), C.byref(cnt)),
+                    "residual_history")
+        m = min(n, cnt.value, 8192)
+        return rms[:m], rmax[:m], loc[:m], cnt.value
 
     def autotune(self, reps=3):
         """Pick the sweep tile shape by timing (wcns_autotune); returns (TJ, TK, ms per sweep set)."""
