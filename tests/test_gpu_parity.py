@@ -526,6 +526,12 @@ def test_graph_replay_matches_plain_launches(name):
             np.testing.assert_array_equal(a.field(nb, nm), b.field(nb, nm), err_msg=f"block {nb} {nm}")
     ra, rb = a.residual_curve(), b.residual_curve()
     assert ra[0] == rb[0] and ra[1] == rb[1]
+    # every step's monitor went to the device history, graph replays included
+    ha, hb = a.residual_history(9), b.residual_history(9)
+    assert ha[3] == 9 and hb[3] == 9
+    for x, y in zip(ha[:3], hb[:3]):
+        np.testing.assert_array_equal(x, y)
+    assert ha[0][-1] == ra[0] and ha[1][-1] == ra[1]
 
 
 @pytest.mark.parametrize("name", ["bump-laminar", "sa-multiblock"])

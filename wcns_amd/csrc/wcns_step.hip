@@ -619,17 +619,37 @@ __global__ void __launch_bounds__(RES_THREADS) k_residual_all(BlockGrid G, DFiel
     }
 }
 
-// One workgroup per block: its partials in order -> resblk[3b..3b+2] = (sum, max, argmax).
-__global__ void __launch_bounds__(64) k_residual_blocks(const double *partial, const int *wg0, double *resblk)
+// One workgroup per block: its partials -> resblk[3b..3b+2] = (sum, max,
+// argmax), in a fixed order (strided per thread, then a tree), the largest
+// maximum with the smallest node index winning ties as in the serial loop.
+__global__ void __launch_bounds__(RES_THREADS) k_residual_blocks(const double *partial, const int *wg0, double *resblk)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ double ssum[RES_THREADS];
+    __shared__ double smax[RES_THREADS];
+    __shared__ long sarg[RES_THREADS];
     const int b = blockIdx.x;
-    double s = 0.0, mx = 0.0, arg = -1.0;
-    for (int p = wg0[b]; p < wg0[b + 1]; p++) {
+    double s = 0.0, mx = 0.0;
+    long arg = -1;
+    for (int p = wg0[b] + threadIdx.x; p < wg0[b + 1]; p += RES_THREADS) {
         s += partial[3 * p];
-        if (partial[3 * p + 1] > mx) { mx = partial[3 * p + 1]; arg = partial[3 * p + 2]; }
+        const double om = partial[3 * p + 1];
+        const long oa = (long)partial[3 * p + 2];
+        if (om > mx || (om == mx && oa >= 0 && (arg < 0 || oa < arg))) { mx = om; arg = oa; }
     }
-    resblk[3 * b] = s; resblk[3 * b + 1] = mx; resblk[3 * b + 2] = arg;
+    ssum[threadIdx.x] = s; smax[threadIdx.x] = mx; sarg[threadIdx.x] = arg;
+    __syncthreads();
+    for (int h = RES_THREADS / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + h];
+            double om = smax[threadIdx.x + h];
+            long oa = sarg[threadIdx.x + h];
+            if (om > smax[threadIdx.x] || (om == smax[threadIdx.x] && oa >= 0 && (sarg[threadIdx.x] < 0 || oa < sarg[threadIdx.x]))) {
+                smax[threadIdx.x] = om; sarg[threadIdx.x] = oa;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { resblk[3 * b] = ssum[0]; resblk[3 * b + 1] = smax[0]; resblk[3 * b + 2] = (double)sarg[0]; }
 }
 
 // residual_curve's combination over the blocks (PostProcess.f90:1280-1292):
@@ -834,7 +854,7 @@ int launch_residual_all(const BlockGrid &G, const DFields &F, double *partial, c
     if (G.nwg <= 0 || nblocks <= 0) return -3;
     hipLaunchKernelGGL(k_residual_all, dim3(G.nwg), dim3(RES_THREADS), 0, s, G, F, partial);
     if (hipGetLastError() != hipSuccess) return -3;
-    hipLaunchKernelGGL(k_residual_blocks, dim3(nblocks), dim3(64), 0, s, partial, wg0, resblk);
+    hipLaunchKernelGGL(k_residual_blocks, dim3(nblocks), dim3(RES_THREADS), 0, s, partial, wg0, resblk);
     if (hipGetLastError() != hipSuccess) return -3;
     hipLaunchKernelGGL(k_residual_hist, dim3(1), dim3(64), 0, s, resblk, blocks, nblocks, out, hist, count, cap);
     return hipGetLastError() == hipSuccess ? 0 : -3;
