@@ -141,6 +141,33 @@ def test_flux_roe_against_reference_fresh_inputs():
         np.testing.assert_array_equal(O.flux_roe(pl, pr, nv), mgs.flux_roe(lib, pl, pr, nv))
 
 
+@pytest.mark.skipif(not os.path.exists(REF_SA), reason="serial reference library not built here")
+def test_flux_roe_c2_nonpositive_stops_both():
+    """A Roe average with c^2 <= 0 (a negative face pressure, as an
+    extrapolated WCNS state can give): the reference's flux_Roe stops the run
+    (flux.f90:178-181), so it runs in a child process; the oracle raises."""
+    import subprocess
+    import sys
+    import textwrap
+    pl = [1.0, 0.3, 0.0, 0.0, -2.0]
+    pr = [1.0, 0.2, 0.1, 0.0, 0.7]
+    nv = [1.0, 0.0, 0.0, 0.0]
+    with pytest.raises(ArithmeticError):
+        O.flux_roe(np.array(pl), np.array(pr), nv)
+    script = textwrap.dedent(f"""
+        import importlib.util, numpy as np
+        spec = importlib.util.spec_from_file_location("mgs", {os.path.join(HERE, "golden", "make_golden_serial.py")!r})
+        m = importlib.util.module_from_spec(spec); spec.loader.exec_module(m)
+        lib = m.load()
+        m.flux_roe(lib, np.array({pl}), np.array({pr}), np.array({nv}))
+        print("FLUX RETURNED")
+    """)
+    env = dict(os.environ, PYTHONPATH=os.path.dirname(HERE))
+    p = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, env=env, timeout=120)
+    assert "math error in subroutine flux_Roe" in p.stdout
+    assert "FLUX RETURNED" not in p.stdout
+
+
 def _oracle_sst_step(mgs, dims, fields):
     oc = O.OracleCase(mgs.sst_case(*dims))
     for n in mgs.SA_FIELDS + ["qkw"]:

@@ -119,6 +119,7 @@ struct wcns_ctx {
     // entry (8 doubles) and the on-device history ring of whole steps
     double *d_partial = nullptr;
     int *d_res_wg0 = nullptr;
+    int res_nwg = 0;                        // residual workgroups of all blocks
     double *d_resblk = nullptr;
     double *d_res_out = nullptr;
     double *d_hist = nullptr;
@@ -890,6 +891,7 @@ int wcns_setup(wcns_ctx *ctx)
         ctx->dn_t = q; q += 12 * T;
         ctx->df_t = q;
     }
+    ctx->res_nwg = res_wg0.back();
     TRY(dalloc(ctx, &ctx->d_partial, 3 * (size_t)res_wg0.back()));
     TRY(dalloc(ctx, &ctx->d_res_wg0, res_wg0.size()));
     TRY(copy_sync(ctx, ctx->d_res_wg0, res_wg0.data(), sizeof(int) * res_wg0.size(), hipMemcpyHostToDevice));
@@ -1167,6 +1169,10 @@ int wcns_set_field(wcns_ctx *ctx, int nb, int which, const double *in)
     CK(hipStreamSynchronize(ctx->stream));
     for (int m = 0; m < nc; m++)
         TRY(copy_sync(ctx, p + m * ctx->total + b.db.off, in + m * b.g3, 8 * b.g3, hipMemcpyHostToDevice));
+    // a primitive goes to both buffers, as wcns_set_prim does: the update
+    // writes only interior nodes into the other buffer
+    if (which >= 0 && which <= 4)
+        TRY(copy_sync(ctx, ctx->prim[ctx->cur ^ 1][which] + b.db.off, in, 8 * b.g3, hipMemcpyHostToDevice));
     return WCNS_OK;
 }
 

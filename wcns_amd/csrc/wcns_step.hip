@@ -9,9 +9,14 @@ namespace wcns {
 namespace {
 
 // ------------------------------------------------------------------ BCs
+// A boundary value goes to both primitive buffers.  The update writes only
+// the interior nodes into the other buffer, so after the step's buffer swap
+// the ghosts read back are the ones this step's boundary conditions set, as
+// in the reference (whose update leaves the ghosts alone).
 __device__ inline void setp(const DFields &F, long g, double r, double u, double v, double w, double p)
 {
     F.r[g] = r; F.u[g] = u; F.v[g] = v; F.w[g] = w; F.p[g] = p;
+    F.rn[g] = r; F.un[g] = u; F.vn[g] = v; F.wn[g] = w; F.pn[g] = p;
 }
 
 // turbulence variable m (nu~ or k, then SST omega) and its BC_farfield_turbulence
