@@ -113,6 +113,19 @@ int wcns_communicate_pv(wcns_ctx *ctx);      /* parallel.f90:3166-3326  */
  * touch the history below.  Synchronises the stream. */
 int wcns_residual(wcns_ctx *ctx, double *rms, double *resmax, int loc[5]);
 
+/* The same as wcns_residual, with its scratch (per-workgroup partials, the
+ * per-block sums and the result) in a fixed device array compiled into the
+ * library instead of buffers that wcns_setup allocates.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid hipMalloc);
+ * it calls no allocating routine.  Limit: at most 65536 monitor workgroups
+ * (about 8 interior nodes per thread, 256 threads, at most 1024 per block) and
+ * 4096 blocks; larger cases get WCNS_E_ARG.  Every context on one GPU shares
+ * that GPU's array (1.7 MB), so calls that overlap on one GPU race; each call
+ * rewrites every scratch word it reads, so nothing needs clearing between
+ * calls.  Synchronises the stream. */
+int wcns_residual_noalloc(wcns_ctx *ctx, double *rms, double *resmax, int loc[5]);
+
 /* nsteps whole steps in the order of Main.f90:85-109, residual_curve
  * included: every step appends its monitor (rms, max, loc) to an on-device
  * history of the last 8192 steps, with no host synchronisation in between.

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <set>
 #include <string>
 #include <tuple>
@@ -48,6 +49,38 @@ struct HBlock {
 
 }  // namespace
 
+// Scratch of wcns_residual_noalloc: a file-scope device array, so that the
+// entry point needs no allocation.  Every GPU holds its own copy until the
+// process ends; host code reaches it only through the device address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch_ptr),
+// never through the array's name.  Layout: 3 partials per residual workgroup
+// (at most NOALLOC_WGS), then 3 per block (at most NOALLOC_BLOCKS), then the
+// 8-double result.
+constexpr int NOALLOC_WGS = 65536;
+constexpr int NOALLOC_BLOCKS = 4096;
+constexpr size_t NOALLOC_DOUBLES = 3 * (size_t)NOALLOC_WGS + 3 * (size_t)NOALLOC_BLOCKS + 8;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+double *noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The scratch array's address on `device` (which must be current), looked up
+// once per device; nullptr if the lookup fails.
+double *noalloc_scratch_ptr(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (!noalloc_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        noalloc_ptr[device] = (double *)p;
+    }
+    return noalloc_ptr[device];
+}
+}  // namespace
+
 // accelerator complex dies of an MI355X, each with its own L2; workgroup ids
 // are dispatched to them in turn
 constexpr long XCDS = 8;
@@ -56,6 +89,7 @@ constexpr long XCD_SLICE_MIN = 32768;
 struct wcns_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    double *noalloc_scratch = nullptr;      // this device's copy of `scratch` (device address)
     wcns_params prm{};
     DParams dp{};
     std::vector<HBlock> blocks;
@@ -477,6 +511,13 @@ int wcns_create(wcns_ctx **out, int device, const wcns_params *prm)
     if (const char *ss = getenv("WCNS_SA_STAGED")) ctx->sa_staged = atoi(ss) != 0;
     if (const char *iw = getenv("WCNS_INV_WAVES")) ctx->dp.inv_waves = atoi(iw) == 2 ? 2 : (atoi(iw) == 1 ? 1 : 0);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return WCNS_E_HIP;
+    }
+    // the no-allocation scratch of this device (device current: hipSetDevice above)
+    ctx->noalloc_scratch = noalloc_scratch_ptr(device);
+    if (!ctx->noalloc_scratch) {
+        hipStreamDestroy(ctx->stream);
         delete ctx;
         return WCNS_E_HIP;
     }
@@ -1465,6 +1506,32 @@ int wcns_residual(wcns_ctx *ctx, double *rms, double *resmax, int loc[5])
     TRY(launch_monitor(ctx, false));
     double e[8];
     CK(hipMemcpyAsync(e, ctx->d_res_out, sizeof e, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    TRY(check_counters(ctx));
+    entry_out(e, rms, resmax, loc);
+    return WCNS_OK;
+}
+
+int wcns_residual_noalloc(wcns_ctx *ctx, double *rms, double *resmax, int loc[5])
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (ctx->res_nwg > NOALLOC_WGS || (int)ctx->blocks.size() > NOALLOC_BLOCKS)
+        return fail(ctx, WCNS_E_ARG, "wcns_residual_noalloc: the case needs " + std::to_string(ctx->res_nwg) +
+                                         " monitor workgroups and " + std::to_string(ctx->blocks.size()) +
+                                         " blocks; its scratch holds at most " + std::to_string(NOALLOC_WGS) +
+                                         " and " + std::to_string(NOALLOC_BLOCKS));
+    CK(hipSetDevice(ctx->device));
+    double *partial = ctx->noalloc_scratch;
+    double *resblk = partial + 3 * (size_t)NOALLOC_WGS;
+    double *res_out = resblk + 3 * (size_t)NOALLOC_BLOCKS;
+    {
+        Scope sc(ctx, WCNS_T_RESIDUAL, 3);
+        if (launch_residual_all(ctx->grid[GRID_RESID], ctx->fields(), partial, ctx->d_res_wg0, (int)ctx->blocks.size(),
+                                resblk, ctx->d_blocks, res_out, nullptr, nullptr, 0, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "residual monitor launch");
+    }
+    double e[8];
+    CK(hipMemcpyAsync(e, res_out, sizeof e, hipMemcpyDeviceToHost, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     TRY(check_counters(ctx));
     entry_out(e, rms, resmax, loc);

@@ -32,7 +32,8 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_l_h_s", "wcns_communicate_pv", "wcns_residual", "wcns_step", "wcns_get_counter", "wcns_sync",
            "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get", "wcns_force",
            "wcns_step_graph", "wcns_autotune", "wcns_adj_residual", "wcns_adj_jvp", "wcns_adj_vjp",
-           "wcns_adj_force", "wcns_sweep_layout", "wcns_residual_history"]
+           "wcns_adj_force", "wcns_sweep_layout", "wcns_residual_history",
+           "wcns_residual_noalloc"]
 
 TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
@@ -81,6 +82,7 @@ def load_library(path=LIBPATH):
               "wcns_communicate_dq", "wcns_l_h_s", "wcns_communicate_pv", "wcns_sync"):
         getattr(L, n).argtypes = [vp]
     L.wcns_residual.argtypes = [vp, dptr, dptr, iptr]
+    L.wcns_residual_noalloc.argtypes = [vp, dptr, dptr, iptr]
     L.wcns_step.argtypes = [vp, C.c_int, dptr]
     L.wcns_residual_history.argtypes = [vp, C.c_int, dptr, dptr, iptr, C.POINTER(C.c_longlong)]
     L.wcns_get_counter.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong)]
