@@ -172,6 +172,35 @@ int wcns_sweep_layout(wcns_ctx *ctx, int *tj, int *tk, int *sgs_paired);
  * one-sided wall derivatives of getuvwtder (Sub1.f90:383-480). */
 int wcns_force(wcns_ctx *ctx, double out[9]);
 
+/* wcns_force's coefficients appended to an on-device history of the last
+ * 8192 records, with no host wait: what force() writes to force.dat every
+ * nforce steps (PostProcess.f90:1415-1503).  No counterpart in the reference
+ * (which writes each line at once). */
+int wcns_force_record(wcns_ctx *ctx);
+/* the last min(max, count, 8192) records, oldest first, 9 doubles each
+ * (cfx, cfy, cfz, cmx, cmy, cmz, cd, cl, xcp); *count receives the number of
+ * records since wcns_setup.  Synchronises the stream. */
+int wcns_force_history(wcns_ctx *ctx, int max, double *out, long long *count);
+
+/* ---- restart files in the reference's formats ---------------------------
+ * wcns_save_flow writes `flowname` as savefile / write_flow_file_parallel do
+ * (PostProcess.f90:1380-1396, parallel.f90:4073-4145): Fortran unformatted
+ * sequential records, first (nout, wholetime, roo, uoo, voo, woo, poo, moo),
+ * then per block r, u, v, w, p, t of every node i, j, k (i fastest); with a
+ * turbulence model also flowname.turb as write_turbulence_parallel does
+ * (parallel.f90:4676-4746): per block (qke(1..nlamtur), mu_t) per node.
+ * wcns_load_flow reads both back as read_flow_file / read_turbulence_parallel
+ * do (PreProcess.f90:1542-1616, parallel.f90:4603-4674): interior nodes only,
+ * the ghosts keep their values (the next wcns_boundary_all sets them); T as
+ * initial_t forms it from the primitives when t_from_file is 0 (what the
+ * reference's restart does, PreProcess.f90:62), or as the file holds it when
+ * t_from_file is 1: the temperature of the step before, which the wall
+ * conditions read, so that a saved run resumes bit for bit.  nout and
+ * wholetime (either may be NULL) receive the header values.  A file whose
+ * records do not fit the case gives WCNS_E_ARG. */
+int wcns_save_flow(wcns_ctx *ctx, const char *flowname, int nout, double wholetime);
+int wcns_load_flow(wcns_ctx *ctx, const char *flowname, int t_from_file, int *nout, double *wholetime);
+
 /* device error/diagnostic counters: 0 WCNS positivity fallbacks
  * (WCNSE5.f90:2224-2245), 1 repaired nodes (Sub2.f90:661-767), 2 Roe c^2<=0 */
 int wcns_get_counter(wcns_ctx *ctx, int which, long long *value);

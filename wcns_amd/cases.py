@@ -364,3 +364,97 @@ def uneven_blocks_case(nblk=(4, 4, 4), lo=24, hi=96, seed=7, **kw):
     rng = np.random.default_rng(seed)
     draw = lambda n: tuple(int(lo + (hi - lo) * u ** (1.0 / 3.0)) for u in rng.random(n))
     return multiblock_case(draw(nblk[0]), draw(nblk[1]), draw(nblk[2]), seed=seed, **kw)
+
+
+# ------------------------------------------------------------- file writers
+# The reference's input formats, so that a case generated here can be run by
+# the reference's own host (or the Fortran driver in wcns_amd/fortran) as is.
+
+def _frec(f, payload: bytes):
+    """One Fortran unformatted sequential record: byte count, data, byte count."""
+    n = np.array([len(payload)], dtype="<i4").tobytes()
+    f.write(n + payload + n)
+
+
+def write_plot3d(case, path):
+    """Multi-block PLOT3D grid as read_plot3d_grid reads it (PreProcess.f90:365-412):
+    nblocks; one record of (idim, jdim, kdim) per block; then per block one
+    record of x, y, z over k, j, i (i fastest)."""
+    with open(path, "wb") as f:
+        _frec(f, np.array([len(case.blocks)], "<i4").tobytes())
+        for b in case.blocks:
+            _frec(f, np.array([b.ni, b.nj, b.nk], "<i4").tobytes())
+        for b in case.blocks:
+            xyz = [np.asarray(a, dtype="<f8").transpose(2, 1, 0).ravel() for a in (b.x, b.y, b.z)]
+            _frec(f, np.concatenate(xyz).tobytes())
+
+
+def write_inp(case, path, nprocs=1):
+    """The boundary file in the "1976" format of read_bc_parallel
+    (parallel.f90:852-929): marker, process count, block count, then per block
+    its process id, dimensions, name, region count and region lines (with the
+    target line and ibcwin for bctype -1)."""
+    lines = ["%8d" % 1976, "%8d" % nprocs, "%8d" % len(case.blocks)]
+    for nb, b in enumerate(case.blocks):
+        lines.append("%8d" % (nb % nprocs + 1))
+        lines.append("%8d%8d%8d" % (b.ni, b.nj, b.nk))
+        lines.append(" " + b.name)
+        lines.append("%8d" % len(b.regions))
+        for r in b.regions:
+            lines.append("".join("%8d" % v for v in r.s_raw) + "%8d" % r.bctype)
+            if r.bctype < 0:
+                lines.append("        " + "".join("%8d" % v for v in r.t_raw) + "%8d%8d" % (r.nbt, r.ibcwin))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_param(case, path, gridname="grid.x", bcname="grid.inp", flowname="flow.dat",
+                forcename="force.dat", errhis="res.dat", nstart=1, nomax=100, nwerror=1, nforce=100,
+                ndisk=1000):
+    """param.dat with the namelist groups the hot path reads (PreProcess.f90:80-159),
+    in the reference's $group ... / layout."""
+    p = case.params
+    turb = {1: "SA", 2: "SST"}.get(p.nlamtur, "SA")
+
+    def g(name, items):
+        body = "\n".join(f"{k:<10} = {v}" for k, v in items)
+        return f"${name}\n{body}\n/\n"
+    txt = "".join([
+        g("inflow", [("moo", repr(p.moo)), ("attack", repr(p.attack)), ("sideslip", repr(p.sideslip)),
+                     ("reynolds", repr(p.reynolds)), ("twall", repr(p.twall)), ("height", "-1.0"),
+                     ("tref", repr(p.tref)), ("ndim", "3")]),
+        g("force", [("nwholefield", str(p.nwholefield)), ("sref", repr(p.sref)), ("lfref", repr(p.lfref)),
+                    ("lref", repr(p.lref)), ("xref", repr(p.xref)), ("yref", repr(p.yref)),
+                    ("zref", repr(p.zref))]),
+        g("filename", [("gridname", f"'{gridname}'"), ("bcname", f"'{bcname}'"), ("flowname", f"'{flowname}'"),
+                       ("forcename", f"'{forcename}'"), ("errhis", f"'{errhis}'")]),
+        g("control", [("nstart", str(nstart)), ("nomax", str(nomax)), ("ndisk", str(ndisk)),
+                      ("nforce", str(nforce)), ("nwerror", str(nwerror)), ("method", "1")]),
+        g("step", [("ntmst", str(p.ntmst)), ("cfl", repr(p.cfl)), ("timedt_rate", repr(p.timedt_rate)),
+                   ("timedt_turb", repr(p.timedt_turb))]),
+        g("flowtype", [("nvis", str(p.nvis)), ("nchem", "0"), ("ntmodel", "1")]),
+        g("turbulence", [("nlamtur", str(p.nlamtur)), ("nameturb", f"'{turb}'"), ("kmaxlim", repr(p.kmaxlim)),
+                         ("ncmpcor", "0"), ("ndes", "0")]),
+        g("technic", [("nscheme", str(p.nscheme)), ("nflux", str(p.nflux)), ("efix", repr(p.efix)),
+                      ("csrv", repr(p.csrv))]),
+        g("interplate", [("xk", repr(p.xk)), ("xb", repr(p.xb))]),
+    ])
+    with open(path, "w") as f:
+        f.write(txt)
+
+
+def write_flow(case, path, nout=0, wholetime=0.0):
+    """The case's initial field as a restart file in the savefile format
+    (PostProcess.f90:1380-1396): header (nout, wholetime, roo, uoo, voo, woo,
+    poo, moo), then per block r, u, v, w, p, t of the interior nodes (i
+    fastest), t = moo^2 gamma p / rho as initial_t forms it."""
+    roo, uoo, voo, woo, poo = case.freestream()
+    p = case.params
+    with open(path, "wb") as f:
+        _frec(f, np.array([nout], "<i4").tobytes() +
+              np.array([wholetime, roo, uoo, voo, woo, poo, p.moo], "<f8").tobytes())
+        for nb, b in enumerate(case.blocks):
+            r, u, v, w, pr = (np.asarray(a)[3:-3, 3:-3, 3:-3] for a in case.initial_prim(nb))
+            t = p.moo * p.moo * (p.gama * pr / r)
+            rec = np.stack([a.transpose(2, 1, 0) for a in (r, u, v, w, pr, t)], axis=-1)
+            _frec(f, np.ascontiguousarray(rec, dtype="<f8").tobytes())

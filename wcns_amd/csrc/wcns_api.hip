@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -167,6 +168,10 @@ struct wcns_ctx {
     struct Wall { DRegion d; double *xyz; };  // no-slip walls with their face-node coordinates
     std::vector<Wall> walls;
     double *d_force_pn = nullptr, *d_force_part = nullptr, *d_force_out = nullptr;
+    double *d_force_last = nullptr;         // the latest coefficients (9)
+    double *d_fhist = nullptr;              // force history ring (fhist_cap x 9), and its count
+    unsigned long long *d_fhist_n = nullptr;
+    int fhist_cap = 8192;
     // tangent / adjoint of the residual, allocated on first use
     double *ad_pool = nullptr;              // AD_N arrays of total elements
     double *ad_work = nullptr;              // metric slots of the largest block
@@ -541,7 +546,8 @@ int wcns_destroy(wcns_ctx *ctx)
                   ctx->d_write_ok, ctx->d_npp_buf};
     for (void *p : ps) if (p) hipFree(p);
     for (auto &w : ctx->walls) hipFree(w.xyz);
-    void *fs[] = {ctx->d_force_pn, ctx->d_force_part, ctx->d_force_out, ctx->ad_pool, ctx->ad_work, ctx->ad_force};
+    void *fs[] = {ctx->d_force_pn, ctx->d_force_part, ctx->d_force_out, ctx->ad_pool, ctx->ad_work, ctx->ad_force,
+                  ctx->d_force_last, ctx->d_fhist, ctx->d_fhist_n};
     for (void *p : fs) if (p) hipFree(p);
     for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
     for (int c = 0; c < 2; c++)
@@ -1131,6 +1137,10 @@ int wcns_setup(wcns_ctx *ctx)
     TRY(dalloc(ctx, &ctx->d_force_pn, 3 * maxw));
     TRY(dalloc(ctx, &ctx->d_force_part, 6 * maxq));
     TRY(dalloc(ctx, &ctx->d_force_out, 6 * std::max<size_t>(ctx->walls.size(), 1)));
+    TRY(dalloc(ctx, &ctx->d_force_last, 9));
+    TRY(dalloc(ctx, &ctx->d_fhist, 9 * (size_t)ctx->fhist_cap));
+    TRY(dalloc(ctx, &ctx->d_fhist_n, 1));
+    CK(hipMemsetAsync(ctx->d_fhist_n, 0, sizeof(unsigned long long), ctx->stream));
     ctx->setup_done = true;
     return WCNS_OK;
 }
@@ -1648,10 +1658,10 @@ int wcns_step(wcns_ctx *ctx, int nsteps, double *rms_hist)
     return WCNS_OK;
 }
 
-int wcns_force(wcns_ctx *ctx, double out[9])
+// force + force_dif of every no-slip wall on the stream into d_force_last, and
+// with push also into the force history; no host wait
+static int launch_forces(wcns_ctx *ctx, bool push)
 {
-    if (!ctx || !out) return WCNS_E_ARG;
-    if (!ctx->setup_done) return WCNS_E_STATE;
     if (ctx->prm.sref == 0.0 || ctx->prm.lfref == 0.0) return fail(ctx, WCNS_E_ARG, "sref and lfref must be non-zero");
     DFields F = ctx->fields();
     const double ref[3] = {ctx->prm.xref, ctx->prm.yref, ctx->prm.zref};
@@ -1661,28 +1671,216 @@ int wcns_force(wcns_ctx *ctx, double out[9])
                                 ctx->d_force_part, ctx->d_force_out + 6 * w, ref, ctx->stream))
             return fail(ctx, WCNS_E_HIP, "force launch failed");
     }
-    std::vector<double> part(6 * ctx->walls.size());
-    if (!part.empty()) {
-        CK(hipMemcpyAsync(part.data(), ctx->d_force_out, 8 * part.size(), hipMemcpyDeviceToHost, ctx->stream));
-    }
-    CK(hipStreamSynchronize(ctx->stream));
-    // PostProcess.f90:1415-1477: region sums in (block, region) order, then coefficients
-    double f[6] = {0, 0, 0, 0, 0, 0};
-    for (size_t w = 0; w < ctx->walls.size(); w++)
-        for (int m = 0; m < 6; m++) f[m] = f[m] + part[6 * w + m];
+    // PostProcess.f90:1458-1477 (force): the angles as init_inflow converts them
     const wcns_params &p = ctx->prm;
-    double scf = 1.0 / p.sref, vcm = 1.0 / (p.sref * p.lfref);
-    int wf = 2 - p.nwholefield;
-    for (int m = 0; m < 3; m++) f[m] = wf * f[m] * scf;
-    for (int m = 3; m < 6; m++) f[m] = wf * f[m] * vcm;
-    const double pai = 3.14159265;   // Globalvar.f90:236, as the freestream set-up converts attack
-    double att = p.attack * pai / 180.0, ssl = p.sideslip * pai / 180.0;
-    double sina = sin(att), cosa = cos(att), sinb = sin(ssl), cosb = cos(ssl);
-    double cl = f[1] * cosa - f[0] * sina;
-    double cd = cosb * (f[1] * sina + f[0] * cosa) + sinb * f[2];
-    double xcp = (f[1] >= 0 ? 1.0 : -1.0) * f[5] / (fabs(f[1]) + SMALL);
-    for (int m = 0; m < 6; m++) out[m] = f[m];
-    out[6] = cd; out[7] = cl; out[8] = xcp;
+    const double pai = 3.14159265;   // Globalvar.f90:236
+    const double att = p.attack * pai / 180.0, ssl = p.sideslip * pai / 180.0;
+    ForceCoef c;
+    c.wf = (double)(2 - p.nwholefield);
+    c.scf = 1.0 / p.sref;
+    c.vcm = 1.0 / (p.sref * p.lfref);
+    c.sina = sin(att); c.cosa = cos(att); c.sinb = sin(ssl); c.cosb = cos(ssl);
+    if (launch_force_coef(ctx->d_force_out, (int)ctx->walls.size(), c, ctx->d_force_last, push ? ctx->d_fhist : nullptr,
+                          ctx->d_fhist_n, ctx->fhist_cap, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "force coefficient launch failed");
+    return WCNS_OK;
+}
+
+int wcns_force(wcns_ctx *ctx, double out[9])
+{
+    if (!ctx || !out) return WCNS_E_ARG;
+    if (!ctx->setup_done) return WCNS_E_STATE;
+    TRY(launch_forces(ctx, false));
+    CK(hipMemcpyAsync(out, ctx->d_force_last, 9 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    return WCNS_OK;
+}
+
+int wcns_force_record(wcns_ctx *ctx)
+{
+    if (!ctx) return WCNS_E_ARG;
+    if (!ctx->setup_done) return WCNS_E_STATE;
+    return launch_forces(ctx, true);
+}
+
+int wcns_force_history(wcns_ctx *ctx, int max, double *out, long long *count)
+{
+    if (!ctx || max < 0 || (max > 0 && !out)) return WCNS_E_ARG;
+    if (!ctx->setup_done) return WCNS_E_STATE;
+    unsigned long long n = 0;
+    CK(hipMemcpyAsync(&n, ctx->d_fhist_n, sizeof n, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (count) *count = (long long)n;
+    const long want = std::min<long>(std::min<long>(max, (long)n), ctx->fhist_cap);
+    if (want <= 0) return WCNS_OK;
+    std::vector<double> h(9 * (size_t)ctx->fhist_cap);
+    CK(hipMemcpyAsync(h.data(), ctx->d_fhist, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    for (long q = 0; q < want; q++) {
+        const unsigned long long idx = n - (unsigned long long)want + (unsigned long long)q;
+        memcpy(out + 9 * q, h.data() + 9 * (size_t)(idx % (unsigned long long)ctx->fhist_cap), 9 * sizeof(double));
+    }
+    return WCNS_OK;
+}
+
+// ------------------------------------------------------------ restart files
+// Fortran unformatted sequential records, as the reference's flang/gfortran
+// build writes them: a 4-byte byte count, the data, the count again.
+static bool rec_write(FILE *f, const void *d, size_t n)
+{
+    if (n > 0x7fffffffu) return false;
+    const uint32_t m = (uint32_t)n;
+    return fwrite(&m, 4, 1, f) == 1 && (n == 0 || fwrite(d, 1, n, f) == n) && fwrite(&m, 4, 1, f) == 1;
+}
+
+static bool rec_read(FILE *f, std::vector<unsigned char> &buf)
+{
+    uint32_t m = 0, m2 = 0;
+    if (fread(&m, 4, 1, f) != 1) return false;
+    buf.resize(m);
+    if (m && fread(buf.data(), 1, m, f) != m) return false;
+    return fread(&m2, 4, 1, f) == 1 && m2 == m;
+}
+
+// a whole G3 field of block nb into host memory
+static int fetch(wcns_ctx *ctx, int nb, int which, std::vector<double> &a)
+{
+    a.resize(ctx->blocks[nb].g3);
+    return wcns_get_field(ctx, nb, which, a.data());
+}
+
+int wcns_save_flow(wcns_ctx *ctx, const char *flowname, int nout, double wholetime)
+{
+    if (!ctx || !flowname) return WCNS_E_ARG;
+    if (!ctx->setup_done) return WCNS_E_STATE;
+    FILE *f = fopen(flowname, "wb");
+    if (!f) return fail(ctx, WCNS_E_ARG, std::string("cannot open ") + flowname);
+    unsigned char head[4 + 7 * 8];
+    const int32_t no = nout;
+    const double hv[7] = {wholetime, ctx->dp.roo, ctx->dp.uoo, ctx->dp.voo, ctx->dp.woo, ctx->dp.poo, ctx->prm.moo};
+    memcpy(head, &no, 4);
+    memcpy(head + 4, hv, sizeof hv);
+    bool ok = rec_write(f, head, sizeof head);
+    std::vector<double> fld[6], rec;
+    for (size_t nb = 0; ok && nb < ctx->blocks.size(); nb++) {
+        const HBlock &b = ctx->blocks[nb];
+        for (int m = 0; m < 6; m++) {
+            int e = fetch(ctx, (int)nb, m, fld[m]);
+            if (e) { fclose(f); return e; }
+        }
+        rec.clear();
+        for (int k = 1; k <= b.nk; k++) for (int j = 1; j <= b.nj; j++) for (int i = 1; i <= b.ni; i++) {
+            const long q = gidx(b.db, i, j, k) - b.db.off;
+            for (int m = 0; m < 6; m++) rec.push_back(fld[m][q]);
+        }
+        ok = rec_write(f, rec.data(), rec.size() * sizeof(double));
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) return fail(ctx, WCNS_E_ARG, std::string("writing ") + flowname + " failed");
+    if (ctx->qke) {
+        const std::string tn = std::string(flowname) + ".turb";
+        FILE *g = fopen(tn.c_str(), "wb");
+        if (!g) return fail(ctx, WCNS_E_ARG, "cannot open " + tn);
+        const int nl = ctx->qkw ? 2 : 1;
+        std::vector<double> qk, qw, vt;
+        for (size_t nb = 0; ok && nb < ctx->blocks.size(); nb++) {
+            const HBlock &b = ctx->blocks[nb];
+            int e = fetch(ctx, (int)nb, 26, qk);
+            if (!e && nl == 2) e = fetch(ctx, (int)nb, 28, qw);
+            if (!e) e = fetch(ctx, (int)nb, 15, vt);
+            if (e) { fclose(g); return e; }
+            rec.clear();
+            for (int k = 1; k <= b.nk; k++) for (int j = 1; j <= b.nj; j++) for (int i = 1; i <= b.ni; i++) {
+                const long q = gidx(b.db, i, j, k) - b.db.off;
+                rec.push_back(qk[q]);
+                if (nl == 2) rec.push_back(qw[q]);
+                rec.push_back(vt[q]);
+            }
+            ok = rec_write(g, rec.data(), rec.size() * sizeof(double));
+        }
+        ok = (fclose(g) == 0) && ok;
+        if (!ok) return fail(ctx, WCNS_E_ARG, "writing " + tn + " failed");
+    }
+    return WCNS_OK;
+}
+
+int wcns_load_flow(wcns_ctx *ctx, const char *flowname, int t_from_file, int *nout, double *wholetime)
+{
+    if (!ctx || !flowname) return WCNS_E_ARG;
+    if (!ctx->setup_done) return WCNS_E_STATE;
+    FILE *f = fopen(flowname, "rb");
+    if (!f) return fail(ctx, WCNS_E_ARG, std::string("cannot open ") + flowname);
+    std::vector<unsigned char> buf;
+    if (!rec_read(f, buf) || buf.size() != 4 + 7 * 8) {
+        fclose(f);
+        return fail(ctx, WCNS_E_ARG, std::string(flowname) + ": not a flow file of this case (header)");
+    }
+    int32_t no;
+    double hv[7];
+    memcpy(&no, buf.data(), 4);
+    memcpy(hv, buf.data() + 4, sizeof hv);
+    std::vector<double> fld[6];
+    for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
+        const HBlock &b = ctx->blocks[nb];
+        const size_t n = (size_t)b.ni * b.nj * b.nk;
+        if (!rec_read(f, buf) || buf.size() != n * 6 * sizeof(double)) {
+            fclose(f);
+            return fail(ctx, WCNS_E_ARG, std::string(flowname) + ": block " + std::to_string(nb + 1) +
+                                             " does not match the case");
+        }
+        const double *d = reinterpret_cast<const double *>(buf.data());
+        for (int m = 0; m < 6; m++) {
+            int e = fetch(ctx, (int)nb, m, fld[m]);
+            if (e) { fclose(f); return e; }
+        }
+        size_t o = 0;
+        for (int k = 1; k <= b.nk; k++) for (int j = 1; j <= b.nj; j++) for (int i = 1; i <= b.ni; i++, o++) {
+            const long q = gidx(b.db, i, j, k) - b.db.off;
+            for (int m = 0; m < 6; m++) fld[m][q] = d[6 * o + m];
+        }
+        // read_flow_file (PreProcess.f90:1542-1616) sets the interior; the
+        // temperature then follows initial_t as at set-up (PreProcess.f90:62),
+        // or with t_from_file stays as the file holds it
+        int e = wcns_set_prim(ctx, (int)nb, fld[0].data(), fld[1].data(), fld[2].data(), fld[3].data(),
+                              fld[4].data());
+        if (!e && (t_from_file || ctx->prm.nvis != 1)) e = wcns_set_field(ctx, (int)nb, 5, fld[5].data());
+        if (e) { fclose(f); return e; }
+    }
+    fclose(f);
+    if (ctx->qke) {
+        const std::string tn = std::string(flowname) + ".turb";
+        FILE *g = fopen(tn.c_str(), "rb");
+        if (!g) return fail(ctx, WCNS_E_ARG, "cannot open " + tn);
+        const int nl = ctx->qkw ? 2 : 1;
+        std::vector<double> qk, qw, vt;
+        for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
+            const HBlock &b = ctx->blocks[nb];
+            const size_t n = (size_t)b.ni * b.nj * b.nk;
+            if (!rec_read(g, buf) || buf.size() != n * (nl + 1) * sizeof(double)) {
+                fclose(g);
+                return fail(ctx, WCNS_E_ARG, tn + ": block " + std::to_string(nb + 1) + " does not match the case");
+            }
+            const double *d = reinterpret_cast<const double *>(buf.data());
+            int e = fetch(ctx, (int)nb, 26, qk);
+            if (!e && nl == 2) e = fetch(ctx, (int)nb, 28, qw);
+            if (!e) e = fetch(ctx, (int)nb, 15, vt);
+            if (e) { fclose(g); return e; }
+            size_t o = 0;
+            for (int k = 1; k <= b.nk; k++) for (int j = 1; j <= b.nj; j++) for (int i = 1; i <= b.ni; i++, o++) {
+                const long q = gidx(b.db, i, j, k) - b.db.off;
+                qk[q] = d[(nl + 1) * o];
+                if (nl == 2) qw[q] = d[(nl + 1) * o + 1];
+                vt[q] = d[(nl + 1) * o + nl];
+            }
+            e = wcns_set_field(ctx, (int)nb, 26, qk.data());
+            if (!e && nl == 2) e = wcns_set_field(ctx, (int)nb, 28, qw.data());
+            if (!e) e = wcns_set_field(ctx, (int)nb, 15, vt.data());
+            if (e) { fclose(g); return e; }
+        }
+        fclose(g);
+    }
+    if (nout) *nout = no;
+    if (wholetime) *wholetime = hv[0];
     return WCNS_OK;
 }
 

@@ -306,7 +306,41 @@ __global__ void k_wall_xyz(DRegion g, DBlock b, double *X, double *xyz, int npts
     }
 }
 
+// force (PostProcess.f90:1429-1477): the regions' sums added in (block,
+// region) order, scaled to coefficients, cl, cd and xcp.  out = cfx, cfy, cfz,
+// cmx, cmy, cmz, cd, cl, xcp; with hist != nullptr the 9 values are also
+// appended to the ring hist[cap][9] at *count % cap.
+__global__ void k_force_coef(const double *out6, int nwalls, ForceCoef c, double *out, double *hist,
+                             unsigned long long *count, int cap)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double f[6] = {0, 0, 0, 0, 0, 0};
+    for (int w = 0; w < nwalls; w++)
+        for (int m = 0; m < 6; m++) f[m] = f[m] + out6[6 * w + m];
+    for (int m = 0; m < 3; m++) f[m] = c.wf * f[m] * c.scf;
+    for (int m = 3; m < 6; m++) f[m] = c.wf * f[m] * c.vcm;
+    double e[9];
+    for (int m = 0; m < 6; m++) e[m] = f[m];
+    e[7] = f[1] * c.cosa - f[0] * c.sina;
+    e[6] = c.cosb * (f[1] * c.sina + f[0] * c.cosa) + c.sinb * f[2];
+    e[8] = (f[1] >= 0 ? 1.0 : -1.0) * f[5] / (fabs(f[1]) + SMALL);
+    for (int m = 0; m < 9; m++) out[m] = e[m];
+    if (hist) {
+        const unsigned long long n = *count;
+        double *h = hist + 9 * (long)(n % (unsigned long long)cap);
+        for (int m = 0; m < 9; m++) h[m] = e[m];
+        *count = n + 1;
+    }
+}
+
 }  // namespace
+
+int launch_force_coef(const double *out6, int nwalls, const ForceCoef &c, double *out, double *hist,
+                      unsigned long long *count, int cap, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_force_coef, dim3(1), dim3(64), 0, s, out6, nwalls, c, out, hist, count, cap);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
 
 long force_quads(const DRegion &g)
 {

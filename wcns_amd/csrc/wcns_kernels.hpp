@@ -275,6 +275,12 @@ long force_quads(const DRegion &g);
 int force_parts(const DRegion &g);
 // adjoint of the pressure force (k_force_pn run with nvis = 0 leaves pn3)
 int launch_wall_xyz(const DRegion &g, const DBlock &b, double *X, double *xyz, int back, hipStream_t s);
+// the force coefficients from the regions' sums (6 per region, in order):
+// wf = 2 - nwholefield, scf = 1/sref, vcm = 1/(sref lfref), and the sines and
+// cosines of the angle of attack and sideslip
+struct ForceCoef { double wf, scf, vcm, sina, cosa, sinb, cosb; };
+int launch_force_coef(const double *out6, int nwalls, const ForceCoef &c, double *out, double *hist,
+                      unsigned long long *count, int cap, hipStream_t s);
 int launch_force_vjp(const DRegion &g, const DBlock &b, const DFields &F, const DParams &P, const double *pn3,
                      const double *xyz, const double ref[3], const double omega[6], double *lpn3, double *lxyz,
                      double *gp, double *gm, hipStream_t s);

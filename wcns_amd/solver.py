@@ -33,7 +33,7 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get", "wcns_force",
            "wcns_step_graph", "wcns_autotune", "wcns_adj_residual", "wcns_adj_jvp", "wcns_adj_vjp",
            "wcns_adj_force", "wcns_sweep_layout", "wcns_residual_history",
-           "wcns_residual_noalloc"]
+           "wcns_residual_noalloc", "wcns_force_record", "wcns_force_history", "wcns_save_flow", "wcns_load_flow"]
 
 TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
@@ -91,6 +91,10 @@ def load_library(path=LIBPATH):
     L.wcns_timing.argtypes = [vp, C.c_int]
     L.wcns_timing_get.argtypes = [vp, C.c_int, dptr, C.POINTER(C.c_longlong)]
     L.wcns_force.argtypes = [vp, dptr]
+    L.wcns_force_record.argtypes = [vp]
+    L.wcns_force_history.argtypes = [vp, C.c_int, dptr, C.POINTER(C.c_longlong)]
+    L.wcns_save_flow.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
+    L.wcns_load_flow.argtypes = [vp, C.c_char_p, C.c_int, iptr, dptr]
     L.wcns_step_graph.argtypes = [vp, C.c_int]
     L.wcns_autotune.argtypes = [vp, C.c_int, iptr, iptr, dptr]
     L.wcns_sweep_layout.argtypes = [vp, iptr, iptr, iptr]
@@ -270,6 +274,30 @@ class Solver:
         out = np.zeros(9)
         self._check(self.L.wcns_force(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
+
+    def force_record(self):
+        """Append the force coefficients to the on-device history (no host wait)."""
+        self._check(self.L.wcns_force_record(self.h), "force_record")
+
+    def force_history(self, n):
+        """The last n force records, oldest first: (array (m, 9), records so far)."""
+        out = np.zeros((max(n, 1), 9))
+        cnt = C.c_longlong()
+        self._check(self.L.wcns_force_history(self.h, n, _d(out), C.byref(cnt)), "force_history")
+        return out[:min(n, cnt.value, 8192)], cnt.value
+
+    def save_flow(self, path, nout=0, wholetime=0.0):
+        """Write the restart file (and path.turb with a turbulence model) in the reference's format."""
+        self._check(self.L.wcns_save_flow(self.h, os.fsencode(path), nout, wholetime), "save_flow")
+
+    def load_flow(self, path, t_from_file=False):
+        """Read a restart file written by save_flow or by the reference; returns
+        (nout, wholetime).  T follows initial_t as in the reference's restart, or
+        with t_from_file stays as saved (an exact resume)."""
+        no, wt = C.c_int(), C.c_double()
+        self._check(self.L.wcns_load_flow(self.h, os.fsencode(path), int(bool(t_from_file)), C.byref(no),
+                                          C.byref(wt)), "load_flow")
+        return no.value, wt.value
 
     def counter(self, which):
         v = C.c_longlong()
