@@ -64,6 +64,8 @@ def lib():
                                 C.c_double, C.c_double]
         L.o_flux_roe.argtypes = [dptr, dptr, C.c_double, C.c_double, C.c_double, C.c_double, dptr,
                                  C.c_double, C.c_double]
+        L.o_flux_vanleer.argtypes = [dptr, dptr, C.c_double, C.c_double, C.c_double, C.c_double, dptr,
+                                     C.c_double, C.c_double]
         L.o_wcns_line.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dptr,
                                   dptr, dptr]
         L.o_force.argtypes = [C.c_void_p, dptr]
@@ -114,6 +116,15 @@ def flux_roe(ql, qr, nv, efix=0.1, gama=1.4):
                          nv[0], nv[1], nv[2], nv[3], _d(f), efix, gama)
     if e:
         raise ArithmeticError("Roe average c2 <= 0")
+    return f
+
+
+def flux_vanleer(pl, pr, nv, efix=0.1, gama=1.4):
+    f = np.zeros(5)
+    e = lib().o_flux_vanleer(_d(np.ascontiguousarray(pl, float)), _d(np.ascontiguousarray(pr, float)),
+                             nv[0], nv[1], nv[2], nv[3], _d(f), efix, gama)
+    if e:
+        raise ArithmeticError("van Leer: gama p / rho < 0")
     return f
 
 

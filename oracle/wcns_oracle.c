@@ -273,10 +273,71 @@ int o_flux_roe(const double *qlin, const double *qrin, double nx, double ny, dou
     return 0;
 }
 
+/* flux.f90:410-501 flux_vanleer_pn: van Leer's split flux of one state,
+ * npn = 1 (F+) or -1 (F-).  Returns -1 where gama p / rho < 0 (the reference
+ * stops there, flux.f90:437-441). */
+static int flux_vanleer_pn(const double *prim, double nx, double ny, double nz, double nt, double *f,
+                           double gama, int npn)
+{
+    double rm = prim[0], um = prim[1], vm = prim[2], wm = prim[3], pm = prim[4];
+    double u2 = 0.5 * (um * um + vm * vm + wm * wm);
+    double hint = gethm(prim, gama);
+    double hm = hint + u2;
+    double c2 = gama * pm / rm;
+    if (c2 < 0.0) return -1;
+    double cm = sqrt(c2);
+    double ct = nx * um + ny * vm + nz * wm + nt;
+    double cgm = sqrt(nx * nx + ny * ny + nz * nz) + SML_SSS;
+    double cgm1 = 1.0 / cgm;
+    double nxa = nx * cgm1, nya = ny * cgm1, nza = nz * cgm1, cta = ct * cgm1;
+    double bmvl = cta / cm;
+    if (npn == 1) {
+        if (fabs(bmvl) < 1.0) {
+            double ctcm = (-cta + 2.0 * cm) / gama;
+            f[0] = 0.25 * cgm * rm * cm * ((bmvl + 1.0) * (bmvl + 1.0));
+            f[1] = f[0] * (um + nxa * ctcm); f[2] = f[0] * (vm + nya * ctcm);
+            f[3] = f[0] * (wm + nza * ctcm); f[4] = f[0] * hm;
+        } else if (bmvl >= 1.0) {
+            double rmct = rm * cta;
+            f[0] = cgm * rmct; f[1] = cgm * (rmct * um + pm * nxa); f[2] = cgm * (rmct * vm + pm * nya);
+            f[3] = cgm * (rmct * wm + pm * nza); f[4] = cgm * rmct * hm;
+        } else {
+            for (int n = 0; n < 5; n++) f[n] = 0.0;
+        }
+    } else {
+        if (fabs(bmvl) < 1.0) {
+            double ctcm = (-cta - 2.0 * cm) / gama;
+            f[0] = -0.25 * cgm * rm * cm * ((bmvl - 1.0) * (bmvl - 1.0));
+            f[1] = f[0] * (um + nxa * ctcm); f[2] = f[0] * (vm + nya * ctcm);
+            f[3] = f[0] * (wm + nza * ctcm); f[4] = f[0] * hm;
+        } else if (bmvl <= -1.0) {
+            double rmct = rm * cta;
+            f[0] = cgm * rmct; f[1] = cgm * (rmct * um + pm * nxa); f[2] = cgm * (rmct * vm + pm * nya);
+            f[3] = cgm * (rmct * wm + pm * nza); f[4] = cgm * rmct * hm;
+        } else {
+            for (int n = 0; n < 5; n++) f[n] = 0.0;
+        }
+    }
+    return 0;
+}
+
+/* flux.f90:395-408 flux_vanleer: F+(qL) + F-(qR) */
+int o_flux_vanleer(const double *priml, const double *primr, double nx, double ny, double nz, double nt,
+                   double *f, double efix, double gama)
+{
+    double fl[5], fr[5];
+    (void)efix;
+    if (flux_vanleer_pn(priml, nx, ny, nz, nt, fl, gama, 1)) return -1;
+    if (flux_vanleer_pn(primr, nx, ny, nz, nt, fr, gama, -1)) return -1;
+    for (int m = 0; m < 5; m++) f[m] = fl[m] + fr[m];
+    return 0;
+}
+
 static int face_flux(int flux, const double *ql, const double *qr, double kx, double ky,
                      double kz, double kt, double *f, double efix, double gama)
 {
     if (flux == 3) return o_flux_roe(ql, qr, kx, ky, kz, kt, f, efix, gama);
+    if (flux == 2) return o_flux_vanleer(ql, qr, kx, ky, kz, kt, f, efix, gama);
     o_flux_sw(ql, qr, kx, ky, kz, kt, f, efix, gama);
     return 0;
 }
@@ -390,18 +451,27 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
     o_flux_dxyz(5, nmax, ni, f, df);
 #define DFO(m,i) df[((i) - 1) * 5 + (m) - 1]
 
-    if (scheme == 41 && ist > 1) {
-        /* WCNSE5.f90:2269-2513: left physical boundary, rows 1-3 */
+    if ((scheme == 41 || scheme == 42) && ist > 1) {
+        /* WCNSE5.f90:2269-2513: left physical boundary, rows 1-3; scheme 42
+         * (WCNS_E_5_42, WCNSE5.f90:2766-3441) differs only in the four
+         * linear face states below (WCNSE5.f90:2967-2986) */
         for (int i = 1; i <= 4; i++)
             for (int m = 1; m <= NL; m++) { u_l_old[m - 1][i] = QWL(m, i); u_r_old[m - 1][i] = QWR(m, i); }
         for (int m = 1; m <= NL; m++) {
-            QWL(m, 0) = (3.0 * QL(m, 1) - 1.0 * QL(m, 2) - 0. * QL(m, 3)) / 2.0;
+            if (scheme == 42) {
+                QWL(m, 0) = (677.0 * QL(m, 1) - 421.0 * QL(m, 2) + 99.0 * QL(m, 3) + 13.0 * QL(m, 4)) / 368.0;
+                QWL(m, 1) = (5.0 * QL(m, 1) + 15.0 * QL(m, 2) - 5.0 * QL(m, 3) + QL(m, 4)) / 16.0;
+                QWL(m, 2) = (-QL(m, 1) + 9.0 * QL(m, 2) + 9.0 * QL(m, 3) - QL(m, 4)) / 16.0;
+                QWL(m, 3) = (QL(m, 1) - 5.0 * QL(m, 2) + 15.0 * QL(m, 3) + 5.0 * QL(m, 4)) / 16.0;
+            } else {
+                QWL(m, 0) = (3.0 * QL(m, 1) - 1.0 * QL(m, 2) - 0. * QL(m, 3)) / 2.0;
+                QWL(m, 1) = (19.0 * QL(m, 1) + 25.0 * QL(m, 2) - 2. * QL(m, 3)) / 42.0;
+                QWL(m, 2) = (4.0 * QL(m, 1) + 3.0 * QL(m, 2) + 9.0 * QL(m, 3) + 2. * QL(m, 4)) / 18.0;
+                QWL(m, 3) = (-2.0 * QL(m, 1) + 3.0 * QL(m, 2) + 3.0 * QL(m, 3) + 2. * QL(m, 4)) / 6.0;
+            }
             QWR(m, 0) = QWL(m, 0);
-            QWL(m, 1) = (19.0 * QL(m, 1) + 25.0 * QL(m, 2) - 2. * QL(m, 3)) / 42.0;
             QWR(m, 1) = QWL(m, 1);
-            QWL(m, 2) = (4.0 * QL(m, 1) + 3.0 * QL(m, 2) + 9.0 * QL(m, 3) + 2. * QL(m, 4)) / 18.0;
             QWR(m, 2) = QWL(m, 2);
-            QWL(m, 3) = (-2.0 * QL(m, 1) + 3.0 * QL(m, 2) + 3.0 * QL(m, 3) + 2. * QL(m, 4)) / 6.0;
             QWR(m, 3) = QWL(m, 3);
         }
         for (int i = 0; i <= 3; i++) {
@@ -477,20 +547,28 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
             DFO(m, 3) = (FL(m, 1) - 27.0 * FL(m, 2) + 27.0 * FL(m, 3) - FL(m, 4)) / 24.0;
     }
 
-    if (scheme == 41 && ied < ni) {
-        /* WCNSE5.f90:2518-2751: right physical boundary, rows ni-2..ni */
+    if ((scheme == 41 || scheme == 42) && ied < ni) {
+        /* WCNSE5.f90:2518-2751: right physical boundary, rows ni-2..ni; scheme
+         * 42's face states WCNSE5.f90:3207-3226 */
         for (int i = 1; i <= 4; i++) {
             int i1 = ni - i;
             for (int m = 1; m <= NL; m++) { u_l_old[m - 1][9 - i] = QWL(m, i1); u_r_old[m - 1][9 - i] = QWR(m, i1); }
         }
         for (int m = 1; m <= NL; m++) {
-            QWR(m, ni)     = (3.0 * QL(m, ni) - 1.0 * QL(m, ni - 1) - 0. * QL(m, ni - 2)) / 2.0;
+            if (scheme == 42) {
+                QWR(m, ni)     = (677.0 * QL(m, ni) - 421.0 * QL(m, ni - 1) + 99.0 * QL(m, ni - 2) + 13.0 * QL(m, ni - 3)) / 368.0;
+                QWR(m, ni - 1) = (5.0 * QL(m, ni) + 15.0 * QL(m, ni - 1) - 5.0 * QL(m, ni - 2) + QL(m, ni - 3)) / 16.0;
+                QWR(m, ni - 2) = (-QL(m, ni) + 9.0 * QL(m, ni - 1) + 9.0 * QL(m, ni - 2) - QL(m, ni - 3)) / 16.0;
+                QWR(m, ni - 3) = (QL(m, ni) - 5.0 * QL(m, ni - 1) + 15.0 * QL(m, ni - 2) + 5.0 * QL(m, ni - 3)) / 16.0;
+            } else {
+                QWR(m, ni)     = (3.0 * QL(m, ni) - 1.0 * QL(m, ni - 1) - 0. * QL(m, ni - 2)) / 2.0;
+                QWR(m, ni - 1) = (19.0 * QL(m, ni) + 25.0 * QL(m, ni - 1) - 2. * QL(m, ni - 2)) / 42.0;
+                QWR(m, ni - 2) = (4.0 * QL(m, ni) + 3.0 * QL(m, ni - 1) + 9.0 * QL(m, ni - 2) + 2. * QL(m, ni - 3)) / 18.0;
+                QWR(m, ni - 3) = (-2.0 * QL(m, ni) + 3.0 * QL(m, ni - 1) + 3.0 * QL(m, ni - 2) + 2. * QL(m, ni - 3)) / 6.0;
+            }
             QWL(m, ni)     = QWR(m, ni);
-            QWR(m, ni - 1) = (19.0 * QL(m, ni) + 25.0 * QL(m, ni - 1) - 2. * QL(m, ni - 2)) / 42.0;
             QWL(m, ni - 1) = QWR(m, ni - 1);
-            QWR(m, ni - 2) = (4.0 * QL(m, ni) + 3.0 * QL(m, ni - 1) + 9.0 * QL(m, ni - 2) + 2. * QL(m, ni - 3)) / 18.0;
             QWL(m, ni - 2) = QWR(m, ni - 2);
-            QWR(m, ni - 3) = (-2.0 * QL(m, ni) + 3.0 * QL(m, ni - 1) + 3.0 * QL(m, ni - 2) + 2. * QL(m, ni - 3)) / 6.0;
             QWL(m, ni - 3) = QWR(m, ni - 3);
         }
         for (int i = ni; i >= ni - 3; i--) {

@@ -84,6 +84,9 @@ void o_value_half_node(int n, int nmax, int ni, const double *q, double *q_half)
 void o_flux_dxyz(int nl, int nmax, int ni, const double *f, double *df);
 void o_flux_sw(const double *priml, const double *primr, double nx, double ny, double nz,
                double nt, double *f, double efix, double gama);
+/* van Leer (flux.f90:395-501); -1 where gama p / rho < 0 */
+int  o_flux_vanleer(const double *priml, const double *primr, double nx, double ny, double nz,
+                    double nt, double *f, double efix, double gama);
 int  o_flux_roe(const double *ql, const double *qr, double nx, double ny, double nz,
                 double nt, double *f, double efix, double gama);
 /* scheme 4 or 41; flux 1 (SW) or 3 (Roe); q is q(1:6,-2:nmax+3,1:2), trxyz(5,nmax), df(5,nmax) */

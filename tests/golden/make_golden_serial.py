@@ -86,6 +86,23 @@ def wcns_line_roe(lib, fn, trxyz, qline):
     return df.T.copy()
 
 
+def wcns_line_flux(lib, fn, ffn, trxyz, qline):
+    """One reference WCNS line (fn: wcns_e_5_, wcns_e_5_41_ or wcns_e_5_42_)
+    with the flux routine ffn (flux_sw_, flux_vanleer_ or flux_roe_)."""
+    ni = trxyz.shape[0]
+    tr = np.asfortranarray(trxyz.T.copy())
+    q = np.zeros((6, ni + 6, 2), order="F")
+    q[:5, :, 0] = qline.T
+    q[5, :, :] = GAMA
+    df = np.zeros((5, ni), order="F")
+    fx = C.cast(getattr(lib, ffn), C.c_void_p)
+    f = getattr(lib, fn)
+    f.argtypes = None
+    f(dv(0.0), dv(0.0), ip(ni), ip(ni), ip(5), ip(1), ip(2), fx, C.c_void_p(0),
+      dv(EFIX), dp(tr), dp(q), dp(df))
+    return df.T.copy()
+
+
 def sa_step(lib, dims, scal, fields):
     """One reference SA block step on C-frame fields; returns (qke, vist)."""
     arrs = [np.ascontiguousarray(fields[n], dtype=np.float64) for n in SA_FIELDS]

@@ -76,12 +76,13 @@ CASES = {
     "laminar-roe-s41": lambda: _laminar(),
     "laminar-roe-s4": lambda: _laminar(nscheme=4),
     "laminar-sw-s41": lambda: _laminar(nflux=1, ni=20, nj=18, nk=12),
+    "laminar-vl-s42": lambda: _laminar(nflux=2, nscheme=42, ni=20, nj=18, nk=12),
     "sa-frozen-roe-s41": lambda: _sa(),
     "sst-frozen-roe-s41": lambda: _sst(),
 }
 
 
-@pytest.mark.parametrize("name", ["laminar-roe-s41", "laminar-sw-s41"])
+@pytest.mark.parametrize("name", ["laminar-roe-s41", "laminar-sw-s41", "laminar-vl-s42"])
 def test_adjoint_primal_is_the_oracle_residual(name):
     from wcns_amd import adjoint as A
     case = CASES[name]()

@@ -44,6 +44,12 @@ CASES = [
     dict(ni=24, nj=16, nk=12, nscheme=4, nflux=1),
     dict(ni=20, nj=14, nk=13, nscheme=41, nflux=3),
     dict(ni=16, nj=20, nk=12, nscheme=4, nflux=3),
+    # van Leer (nflux 2) and WCNS_E_5_42 (nscheme 42), Convect.f90:20-37
+    dict(ni=24, nj=16, nk=12, nscheme=41, nflux=2),
+    dict(ni=20, nj=14, nk=13, nscheme=42, nflux=1),
+    dict(ni=24, nj=16, nk=12, nscheme=42, nflux=2),
+    dict(ni=16, nj=20, nk=12, nscheme=42, nflux=3),
+    dict(ni=16, nj=20, nk=12, nscheme=4, nflux=2),
 ]
 
 
@@ -108,6 +114,7 @@ def test_cartesian_64x32x16_100_steps():
 VIS_CASES = [
     dict(ni=24, nj=16, nk=12, nflux=3, nscheme=41),
     dict(ni=20, nj=18, nk=13, nflux=1, nscheme=4),
+    dict(ni=24, nj=16, nk=12, nflux=2, nscheme=42),
 ]
 
 

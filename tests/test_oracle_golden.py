@@ -168,6 +168,47 @@ def test_flux_roe_c2_nonpositive_stops_both():
     assert "FLUX RETURNED" not in p.stdout
 
 
+@pytest.mark.skipif(not os.path.exists(REF_SA), reason="serial reference library not built here")
+def test_flux_vanleer_against_reference_fresh_inputs():
+    """van Leer's splitting (flux.f90:395-501) on states that cover the
+    subsonic branch and both supersonic branches of each half."""
+    mgs = _serial_module()
+    lib = mgs.load()
+    rng = np.random.default_rng(4343)
+    for _ in range(600):
+        pl = np.array([0.2 + 2 * rng.random(), *(3 * rng.standard_normal(3)), 0.1 + 3 * rng.random()])
+        pr = np.array([0.2 + 2 * rng.random(), *(3 * rng.standard_normal(3)), 0.1 + 3 * rng.random()])
+        nv = np.array([*rng.standard_normal(3), 0.0])
+        ref = np.zeros(5)
+        lib.flux_vanleer_(mgs.dp(pl.copy()), mgs.dp(pr.copy()), mgs.ip(5), mgs.dv(nv[0]), mgs.dv(nv[1]),
+                          mgs.dv(nv[2]), mgs.dv(nv[3]), mgs.dp(ref), mgs.dv(mgs.EFIX), mgs.dv(mgs.GAMA))
+        np.testing.assert_array_equal(O.flux_vanleer(pl, pr, nv), ref)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SA), reason="serial reference library not built here")
+@pytest.mark.parametrize("scheme,fn", [(4, "wcns_e_5_"), (41, "wcns_e_5_41_"), (42, "wcns_e_5_42_")])
+@pytest.mark.parametrize("flux,ffn", [(1, "flux_sw_"), (2, "flux_vanleer_"), (3, "flux_roe_")])
+def test_wcns_schemes_and_fluxes_against_reference(scheme, fn, flux, ffn):
+    """Whole WCNS lines (WCNS_E_5, _41, _42 of WCNSE5.f90) with each flux the
+    reference dispatches to them (Convect.f90:20-37), on smooth and rough
+    lines with and without physical boundaries at either end."""
+    import importlib.util
+    mgs = _serial_module()
+    lib = mgs.load()
+    spec = importlib.util.spec_from_file_location("mg", os.path.join(HERE, "golden", "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    rng = np.random.default_rng(1000 * scheme + flux)
+    for n in (5, 7, 12, 20, 33):
+        for bl in (0, 1):
+            for br in (0, 1):
+                q = mg.smooth_line(rng, n, bl, br, amp=0.2, rough=bool(rng.integers(0, 2)))
+                tr = mg.metrics_line(rng, n)
+                ref = mgs.wcns_line_flux(lib, fn, ffn, tr, q)
+                mine, _ = O.wcns_line(scheme, flux, tr, q)
+                np.testing.assert_array_equal(mine, ref)
+
+
 def _oracle_sst_step(mgs, dims, fields):
     oc = O.OracleCase(mgs.sst_case(*dims))
     for n in mgs.SA_FIELDS + ["qkw"]:

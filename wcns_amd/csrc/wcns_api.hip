@@ -485,7 +485,7 @@ int check_counters(wcns_ctx *ctx)
     unsigned long long c[8];
     CK(hipMemcpyAsync(c, ctx->counters, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
-    if (c[2]) return fail(ctx, WCNS_E_ROE, "sqrt(c2) is a math error in flux_Roe (flux.f90:179-182)");
+    if (c[2]) return fail(ctx, WCNS_E_ROE, "sqrt(c2) is a math error in flux_Roe / flux_vanleer (flux.f90:179-182, 437-441)");
     if (c[4]) return fail(ctx, WCNS_E_REPAIR, "too many abnormal nodes in update_limited (Sub2.f90:781-785)");
     if (ctx->d_sweep_err) {
         int e = 0;
@@ -526,7 +526,7 @@ int wcns_create(wcns_ctx **out, int device, const wcns_params *prm)
         delete ctx;
         return WCNS_E_HIP;
     }
-    if (prm->nscheme != 4 && prm->nscheme != 41) { ctx->err = "nscheme must be 4 or 41"; }
+    if (prm->nscheme != 4 && prm->nscheme != 41 && prm->nscheme != 42) { ctx->err = "nscheme must be 4, 41 or 42"; }
     *out = ctx;
     return WCNS_OK;
 }
@@ -796,8 +796,9 @@ int wcns_setup(wcns_ctx *ctx)
     CK(hipSetDevice(ctx->device));
     CK(hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     const wcns_params &p = ctx->prm;
-    if (p.nscheme != 4 && p.nscheme != 41) return fail(ctx, WCNS_E_UNSUPPORTED, "nscheme must be 4 or 41");
-    if (p.nflux != 1 && p.nflux != 3) return fail(ctx, WCNS_E_UNSUPPORTED, "nflux must be 1 (SW) or 3 (Roe)");
+    if (p.nscheme != 4 && p.nscheme != 41 && p.nscheme != 42)
+        return fail(ctx, WCNS_E_UNSUPPORTED, "nscheme must be 4, 41 or 42");
+    if (p.nflux < 1 || p.nflux > 3) return fail(ctx, WCNS_E_UNSUPPORTED, "nflux must be 1 (SW), 2 (van Leer) or 3 (Roe)");
     if (p.nvis != 0 && p.nvis != 1) return fail(ctx, WCNS_E_ARG, "nvis must be 0 or 1");
     if (p.nlamtur >= 0 && !((p.nlamtur == 1 || p.nlamtur == 2) && p.nvis == 1))
         return fail(ctx, WCNS_E_UNSUPPORTED, "turbulence models: SA (nlamtur = 1) or SST (nlamtur = 2), with nvis = 1");

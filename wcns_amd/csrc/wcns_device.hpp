@@ -201,10 +201,70 @@ __device__ inline bool flux_roe(const T *qlin, const T *qrin, T nx, T ny, T nz, 
     return ok;
 }
 
+// flux.f90:410-501 flux_vanleer_pn (kt = 0), npn = +1 (F+) or -1 (F-).
+// Returns false where gama p / rho < 0 (the reference stops, flux.f90:437-441).
+template <typename T>
+__device__ inline bool flux_vl_pn(const T *prim, T nx, T ny, T nz, T *f, double gama, int npn)
+{
+    T rm = prim[0], um = prim[1], vm = prim[2], wm = prim[3], pm = prim[4];
+    T u2 = 0.5 * (um * um + vm * vm + wm * wm);
+    T hint = gama / (gama - 1.0) * pm / rm;
+    T hm = hint + u2;
+    T c2 = gama * pm / rm;
+    bool ok = !(c2 < 0.0);
+    T cm = sqrt(c2);
+    T ct = nx * um + ny * vm + nz * wm + 0.0;
+    T cgm = sqrt(nx * nx + ny * ny + nz * nz) + SML_SSS;
+    T cgm1 = 1.0 / cgm;
+    T nxa = nx * cgm1, nya = ny * cgm1, nza = nz * cgm1, cta = ct * cgm1;
+    T bmvl = cta / cm;
+    if (fabs(bmvl) < 1.0) {
+        T ctcm, f0;
+        if (npn == 1) {
+            ctcm = (-cta + 2.0 * cm) / gama;
+            f0 = 0.25 * cgm * rm * cm * ((bmvl + 1.0) * (bmvl + 1.0));
+        } else {
+            ctcm = (-cta - 2.0 * cm) / gama;
+            f0 = -0.25 * cgm * rm * cm * ((bmvl - 1.0) * (bmvl - 1.0));
+        }
+        f[0] = f0;
+        f[1] = f0 * (um + nxa * ctcm);
+        f[2] = f0 * (vm + nya * ctcm);
+        f[3] = f0 * (wm + nza * ctcm);
+        f[4] = f0 * hm;
+    } else if ((npn == 1 && bmvl >= 1.0) || (npn == -1 && bmvl <= -1.0)) {
+        T rmct = rm * cta;
+        f[0] = cgm * rmct;
+        f[1] = cgm * (rmct * um + pm * nxa);
+        f[2] = cgm * (rmct * vm + pm * nya);
+        f[3] = cgm * (rmct * wm + pm * nza);
+        f[4] = cgm * rmct * hm;
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; m++) f[m] = T(0.0);
+    }
+    return ok;
+}
+
+// flux.f90:395-408 flux_vanleer: F+(qL) + F-(qR)
+template <typename T>
+__device__ inline bool flux_vl(const T *ql, const T *qr, T nx, T ny, T nz, T *f, double gama)
+{
+    T fl[5], fr[5];
+    bool ok = flux_vl_pn(ql, nx, ny, nz, fl, gama, 1);
+    ok = flux_vl_pn(qr, nx, ny, nz, fr, gama, -1) && ok;
+#pragma unroll
+    for (int m = 0; m < 5; m++) f[m] = fl[m] + fr[m];
+    return ok;
+}
+
+// The face flux of nflux = 1 (Steger-Warming), 2 (van Leer) or 3 (Roe),
+// as invcode dispatches it (Convect.f90:20-37).  False: c^2 < 0 (Roe, van Leer).
 template <int FLUX, typename T>
 __device__ inline bool face_flux(const T *ql, const T *qr, T nx, T ny, T nz, T *f, double efix, double gama)
 {
     if (FLUX == 3) return flux_roe(ql, qr, nx, ny, nz, f, efix, gama);
+    if (FLUX == 2) return flux_vl(ql, qr, nx, ny, nz, f, gama);
     flux_sw(ql, qr, nx, ny, nz, f, efix, gama);
     return true;
 }

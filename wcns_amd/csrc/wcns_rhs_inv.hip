@@ -197,7 +197,7 @@ __device__ inline void checked_flux(const Pencil<IO> &pc, int f, typename IO::T 
 // The WCNS_E_5_41 left boundary rows 1..3 (WCNSE5.f90:2269-2513).
 template <int FLUX, class IO>
 __device__ __noinline__ void left41(const Pencil<IO> &pc, typename IO::T *df1, typename IO::T *df2,
-                                    typename IO::T *df3, double efix, double gama, int &nerr, int &bad)
+                                    typename IO::T *df3, double efix, double gama, bool s42, int &nerr, int &bad)
 {
     using T = typename IO::T;
     T q1[5], q2[5], q3[5], q4[5], urold[5][5], tmp[5], nx[3], f[6][5], ql[5], qr[5];
@@ -206,14 +206,21 @@ __device__ __noinline__ void left41(const Pencil<IO> &pc, typename IO::T *df1, t
 #pragma unroll
     for (int m = 0; m < 5; m++) urold[1][m] = (5.0 * q1[m] + 15.0 * q2[m] - 5.0 * q3[m] + q4[m]) / 16.0;
     for (int i = 2; i <= 4; i++) wcns_at(pc, i + 1, tmp, urold[i]);
-    // (1) faces 0..3 -> df(1)
+    // (1) faces 0..3 -> df(1); scheme 42's states WCNSE5.f90:2967-2986
     for (int i = 0; i <= 3; i++) {
         for (int m = 0; m < 5; m++) {
             T v;
-            if (i == 0) v = (3.0 * q1[m] - 1.0 * q2[m] - 0. * q3[m]) / 2.0;
-            else if (i == 1) v = (19.0 * q1[m] + 25.0 * q2[m] - 2. * q3[m]) / 42.0;
-            else if (i == 2) v = (4.0 * q1[m] + 3.0 * q2[m] + 9.0 * q3[m] + 2. * q4[m]) / 18.0;
-            else v = (-2.0 * q1[m] + 3.0 * q2[m] + 3.0 * q3[m] + 2. * q4[m]) / 6.0;
+            if (s42) {
+                if (i == 0) v = (677.0 * q1[m] - 421.0 * q2[m] + 99.0 * q3[m] + 13.0 * q4[m]) / 368.0;
+                else if (i == 1) v = (5.0 * q1[m] + 15.0 * q2[m] - 5.0 * q3[m] + q4[m]) / 16.0;
+                else if (i == 2) v = (-q1[m] + 9.0 * q2[m] + 9.0 * q3[m] - q4[m]) / 16.0;
+                else v = (q1[m] - 5.0 * q2[m] + 15.0 * q3[m] + 5.0 * q4[m]) / 16.0;
+            } else {
+                if (i == 0) v = (3.0 * q1[m] - 1.0 * q2[m] - 0. * q3[m]) / 2.0;
+                else if (i == 1) v = (19.0 * q1[m] + 25.0 * q2[m] - 2. * q3[m]) / 42.0;
+                else if (i == 2) v = (4.0 * q1[m] + 3.0 * q2[m] + 9.0 * q3[m] + 2. * q4[m]) / 18.0;
+                else v = (-2.0 * q1[m] + 3.0 * q2[m] + 3.0 * q3[m] + 2. * q4[m]) / 6.0;
+            }
             ql[m] = v; qr[m] = v;
         }
         half_metric(pc, i, nx);
@@ -257,7 +264,7 @@ __device__ __noinline__ void left41(const Pencil<IO> &pc, typename IO::T *df1, t
 // The WCNS_E_5_41 right boundary rows n-2..n (WCNSE5.f90:2518-2751).  f[k] holds face n-k.
 template <int FLUX, class IO>
 __device__ __noinline__ void right41(const Pencil<IO> &pc, typename IO::T *dfn, typename IO::T *dfn1,
-                                     typename IO::T *dfn2, double efix, double gama, int &nerr, int &bad)
+                                     typename IO::T *dfn2, double efix, double gama, bool s42, int &nerr, int &bad)
 {
     using T = typename IO::T;
     int n = pc.n;
@@ -267,15 +274,22 @@ __device__ __noinline__ void right41(const Pencil<IO> &pc, typename IO::T *dfn, 
 #pragma unroll
     for (int m = 0; m < 5; m++) ulold[8][m] = (5.0 * qn[m] + 15.0 * qn1[m] - 5.0 * qn2[m] + qn3[m]) / 16.0;
     for (int i = 2; i <= 4; i++) wcns_at(pc, n - i, ulold[9 - i], tmp);
-    // (4) faces n..n-3 -> df(n)
+    // (4) faces n..n-3 -> df(n); scheme 42's states WCNSE5.f90:3207-3226
     for (int i = n; i >= n - 3; i--) {
         int o = n - i;
         for (int m = 0; m < 5; m++) {
             T v;
-            if (o == 0) v = (3.0 * qn[m] - 1.0 * qn1[m] - 0. * qn2[m]) / 2.0;
-            else if (o == 1) v = (19.0 * qn[m] + 25.0 * qn1[m] - 2. * qn2[m]) / 42.0;
-            else if (o == 2) v = (4.0 * qn[m] + 3.0 * qn1[m] + 9.0 * qn2[m] + 2. * qn3[m]) / 18.0;
-            else v = (-2.0 * qn[m] + 3.0 * qn1[m] + 3.0 * qn2[m] + 2. * qn3[m]) / 6.0;
+            if (s42) {
+                if (o == 0) v = (677.0 * qn[m] - 421.0 * qn1[m] + 99.0 * qn2[m] + 13.0 * qn3[m]) / 368.0;
+                else if (o == 1) v = (5.0 * qn[m] + 15.0 * qn1[m] - 5.0 * qn2[m] + qn3[m]) / 16.0;
+                else if (o == 2) v = (-qn[m] + 9.0 * qn1[m] + 9.0 * qn2[m] - qn3[m]) / 16.0;
+                else v = (qn[m] - 5.0 * qn1[m] + 15.0 * qn2[m] + 5.0 * qn3[m]) / 16.0;
+            } else {
+                if (o == 0) v = (3.0 * qn[m] - 1.0 * qn1[m] - 0. * qn2[m]) / 2.0;
+                else if (o == 1) v = (19.0 * qn[m] + 25.0 * qn1[m] - 2. * qn2[m]) / 42.0;
+                else if (o == 2) v = (4.0 * qn[m] + 3.0 * qn1[m] + 9.0 * qn2[m] + 2. * qn3[m]) / 18.0;
+                else v = (-2.0 * qn[m] + 3.0 * qn1[m] + 3.0 * qn2[m] + 2. * qn3[m]) / 6.0;
+            }
             ql[m] = v; qr[m] = v;
         }
         half_metric(pc, i, nx);
@@ -338,8 +352,8 @@ __device__ __noinline__ void closure_state(const Pencil<IO> &pc, int which, type
 // One pencil's rows a..bb (DIR only selects nothing here: the pencil carries
 // its own base, stride and length).  The product instance adds the rows to R.
 template <int FLUX, class IO>
-__device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double gama, double efix, bool s41, int &nerr,
-                                  int &bad)
+__device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double gama, double efix, bool s41, bool s42,
+                                  int &nerr, int &bad)
 {
     using T = typename IO::T;
     const int n = pc.n;
@@ -452,14 +466,14 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
 
     if (s41 && lclos && a == 1) {
         T d1[5], d2[5], d3[5];
-        left41<FLUX>(pc, d1, d2, d3, efix, gama, nerr, bad);
+        left41<FLUX>(pc, d1, d2, d3, efix, gama, s42, nerr, bad);
         emit_row(pc, 1, d1);
         emit_row(pc, 2, d2);
         emit_row(pc, 3, d3);
     }
     if (s41 && rclos && bb == n) {
         T dn[5], dn1[5], dn2[5];
-        right41<FLUX>(pc, dn, dn1, dn2, efix, gama, nerr, bad);
+        right41<FLUX>(pc, dn, dn1, dn2, efix, gama, s42, nerr, bad);
         emit_row(pc, n, dn);
         emit_row(pc, n - 1, dn1);
         emit_row(pc, n - 2, dn2);
@@ -515,7 +529,7 @@ __device__ inline void rhs_inv_thread(const BlockGrid &G, const DFields &F, cons
     int bb = a + SEG - 1;
     if (bb > n) bb = n;
     int nerr = 0, bad = 0;
-    inv_pencil<FLUX>(pc, a, bb, P.gama, P.efix, P.nscheme == 41, nerr, bad);
+    inv_pencil<FLUX>(pc, a, bb, P.gama, P.efix, P.nscheme >= 41, P.nscheme == 42, nerr, bad);
     if (nerr) atomicAdd(&F.counters[0], (unsigned long long)nerr);
     if (bad) atomicAdd(&F.counters[2], 1ull);
 }
@@ -567,7 +581,7 @@ __global__ void __launch_bounds__(128) k_rhs_inv_jvp(DBlock b, DFields F, DParam
     int bb = a + SEG - 1;
     if (bb > n) bb = n;
     int nerr = 0, bad = 0;
-    inv_pencil<FLUX>(pc, a, bb, P.gama, P.efix, P.nscheme == 41, nerr, bad);
+    inv_pencil<FLUX>(pc, a, bb, P.gama, P.efix, P.nscheme >= 41, P.nscheme == 42, nerr, bad);
 }
 
 // Adjoint: thread (pencil, colour, seed variable) walks the whole pencil with
@@ -605,7 +619,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k
     io.stride = pc.stride;
     io.n = pc.n;
     int nerr = 0, bad = 0;
-    inv_pencil<FLUX>(pc, 1, pc.n, P.gama, P.efix, P.nscheme == 41, nerr, bad);
+    inv_pencil<FLUX>(pc, 1, pc.n, P.gama, P.efix, P.nscheme >= 41, P.nscheme == 42, nerr, bad);
 }
 
 }  // namespace
@@ -655,6 +669,7 @@ int launch_rhs_inv_dir(int dir, const BlockGrid &G, const DFields &F, const DPar
     // 1.5 waves per SIMD
     const bool two = P.inv_waves ? P.inv_waves == 2 : 2L * G.nwg * 2 > 3L * simd_count();
     if (P.nflux == 3) launch_inv<3>(dir, two, grid, blk, s, G, F, P);
+    else if (P.nflux == 2) launch_inv<2>(dir, two, grid, blk, s, G, F, P);
     else launch_inv<1>(dir, two, grid, blk, s, G, F, P);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
@@ -670,6 +685,10 @@ int launch_rhs_inv_jvp(int dir, const DBlock &b, const DFields &F, const DParams
         if (dir == 0) hipLaunchKernelGGL((k_rhs_inv_jvp<0, 3>), grid, blk, 0, s, b, F, P, D);
         if (dir == 1) hipLaunchKernelGGL((k_rhs_inv_jvp<1, 3>), grid, blk, 0, s, b, F, P, D);
         if (dir == 2) hipLaunchKernelGGL((k_rhs_inv_jvp<2, 3>), grid, blk, 0, s, b, F, P, D);
+    } else if (P.nflux == 2) {
+        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv_jvp<0, 2>), grid, blk, 0, s, b, F, P, D);
+        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv_jvp<1, 2>), grid, blk, 0, s, b, F, P, D);
+        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv_jvp<2, 2>), grid, blk, 0, s, b, F, P, D);
     } else {
         if (dir == 0) hipLaunchKernelGGL((k_rhs_inv_jvp<0, 1>), grid, blk, 0, s, b, F, P, D);
         if (dir == 1) hipLaunchKernelGGL((k_rhs_inv_jvp<1, 1>), grid, blk, 0, s, b, F, P, D);
@@ -689,6 +708,10 @@ int launch_rhs_inv_vjp(int dir, const DBlock &b, const DFields &F, const DParams
         if (dir == 0) hipLaunchKernelGGL((k_rhs_inv_vjp<0, 3>), grid, blk, 0, s, b, F, P, A);
         if (dir == 1) hipLaunchKernelGGL((k_rhs_inv_vjp<1, 3>), grid, blk, 0, s, b, F, P, A);
         if (dir == 2) hipLaunchKernelGGL((k_rhs_inv_vjp<2, 3>), grid, blk, 0, s, b, F, P, A);
+    } else if (P.nflux == 2) {
+        if (dir == 0) hipLaunchKernelGGL((k_rhs_inv_vjp<0, 2>), grid, blk, 0, s, b, F, P, A);
+        if (dir == 1) hipLaunchKernelGGL((k_rhs_inv_vjp<1, 2>), grid, blk, 0, s, b, F, P, A);
+        if (dir == 2) hipLaunchKernelGGL((k_rhs_inv_vjp<2, 2>), grid, blk, 0, s, b, F, P, A);
     } else {
         if (dir == 0) hipLaunchKernelGGL((k_rhs_inv_vjp<0, 1>), grid, blk, 0, s, b, F, P, A);
         if (dir == 1) hipLaunchKernelGGL((k_rhs_inv_vjp<1, 1>), grid, blk, 0, s, b, F, P, A);
