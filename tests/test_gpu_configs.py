@@ -9,8 +9,9 @@ and branches that the stage-by-stage tests do not reach:
 * the FP32 implicit operator against the FP64 oracle, with a stated bound;
 * the inviscid wall (BC 2 with nvis = 0) at Mach 2;
 * forced failure paths: a repaired node and the WCNS positivity fallback, the
-  too-many-repairs stop (WCNS_E_REPAIR), and a Roe average with c^2 <= 0
-  (WCNS_E_ROE), each with the oracle's counters or error on the same input.
+  too-many-repairs stop (WCNS_E_REPAIR), a Roe average with c^2 <= 0
+  (WCNS_E_ROE) and a folded grid (WCNS_E_JACOBIAN), each with the oracle's
+  counters or error on the same input.
 
 FP64 tolerance: relative 1e-10 against each field's scale after 20 steps (the
 turbulence models' pow/tanh/sqrt differ between the GPU and host libm by a few
@@ -174,6 +175,22 @@ def test_forced_abort_reports_repair_error():
     assert s.counter(1) == o.counter(1) > (24 * 16 * 12) // 64
     o.close()
     s.close()
+
+
+def test_folded_grid_reports_jacobian_error():
+    """A node moved past two of its i neighbours folds the cells around it:
+    check_grid_derivative stops on the non-positive Jacobian
+    (PreProcess.f90:2637-2666); the library returns WCNS_E_JACOBIAN from
+    wcns_setup and names the count of such nodes, the oracle fails its set-up."""
+    from wcns_amd.solver import Solver, WcnsError
+    case = cases.cartesian_periodic_case(ni=24, nj=16, nk=12, seed=3)
+    b = case.blocks[0]
+    b.x[10, 8, 6] = b.x[13, 8, 6]
+    with pytest.raises(WcnsError, match="non-positive Jacobian") as ei:
+        Solver(case)
+    assert ei.value.code == -6
+    with pytest.raises(RuntimeError, match="-11"):
+        O.OracleCase(case)
 
 
 def test_roe_c2_nonpositive_reports_error():

@@ -205,12 +205,17 @@ def uniform_init(case, nb):
 
 
 def bump_channel_case(ni=256, nj=128, nk=64, bump=0.05, dy_wall=1.0e-5, growth=1.1, lx=4.0, lz=0.5,
-                      reynolds=1.0e5, moo=0.5, nflux=3, nscheme=41, cfl=5.0, seed=0):
+                      reynolds=1.0e5, moo=0.5, nflux=3, nscheme=41, cfl=5.0, seed=0, top_bc=4):
     """Sinusoidal-bump channel: x in [-1.5, 2.5] (i), a bump y_w = bump*sin^2(pi x)
     on 0 <= x <= 1 of the bottom wall (BC 2, adiabatic no-slip, j = 1), wall-normal
     spacing dy_wall growing geometrically by `growth` (j), span lz periodic (k,
-    self-connected).  Far field (BC 4) on inflow, outflow and top.  Laminar
-    Navier-Stokes at `reynolds` (per unit length), Mach `moo`, Roe flux, WCNS-E5."""
+    self-connected).  Far field (BC 4) on inflow and outflow; the top takes
+    `top_bc`: far field (4), or a slip plane (3, symmetry), which keeps the
+    far-field treatment off a face that the flow runs along (there the normal
+    velocity changes sign from step to step and the far-field BC switches
+    between its inflow and outflow branches, a limit cycle that stops the
+    residual).  Laminar Navier-Stokes at `reynolds` (per unit length), Mach
+    `moo`, Roe flux, WCNS-E5."""
     x = -1.5 + lx * np.arange(ni) / (ni - 1)
     s = np.concatenate([[0.0], np.cumsum(dy_wall * growth ** np.arange(nj - 1))])
     H = s[-1]
@@ -221,7 +226,7 @@ def bump_channel_case(ni=256, nj=128, nk=64, bump=0.05, dy_wall=1.0e-5, growth=1
     Y = np.repeat(Y, nk, 2)
     Z = np.broadcast_to(z[None, None, :], (ni, nj, nk)).copy()
     regs = [Region((1, 1, 1, nj, 1, nk), 4), Region((ni, ni, 1, nj, 1, nk), 4),
-            Region((1, ni, 1, 1, 1, nk), 2), Region((1, ni, nj, nj, 1, nk), 4),
+            Region((1, ni, 1, 1, 1, nk), 2), Region((1, ni, nj, nj, 1, nk), top_bc),
             Region((1, ni, 1, nj, 1, 1), -1, (1, ni, 1, nj, nk, nk), 1, 6),
             Region((1, ni, 1, nj, nk, nk), -1, (1, ni, 1, nj, 1, 1), 1, 5)]
     blk = Block(ni, nj, nk, X, Y, Z, regs, name="bump")
