@@ -132,3 +132,83 @@ class WallForce(torch.autograd.Function):
     def backward(ctx, gJ):
         gP, gX = ctx.saved_tensors
         return None, None, gJ * gP, gJ * gX, None
+
+
+# ------------------------------------------------ a whole step (inviscid cases)
+# The state is the primitives r, u, v, w, p of every block in the solver's
+# field pool: a (5, total) float64 tensor, block nb's (nk+6, nj+6, ni+6)
+# ghosted nodes (i fastest) from offset[nb] on.  Only the interior nodes are
+# the state: ghost values of inputs are not read, those of outputs are zero.
+
+def pool_state(solver, device="cuda"):
+    """The solver's current primitives as a pool-layout (5, total) tensor."""
+    total, offs = solver.pool_layout()
+    P = torch.zeros((5, total), dtype=torch.float64)
+    for nb, blk in enumerate(solver.case.blocks):
+        n = (blk.ni + 6) * (blk.nj + 6) * (blk.nk + 6)
+        for m, f in enumerate(("r", "u", "v", "w", "p")):
+            P[m, offs[nb]:offs[nb] + n] = torch.from_numpy(solver.field(nb, f).ravel())
+    return P.to(device).contiguous()
+
+
+def pool_block(solver, A, nb):
+    """Block nb's part of a pool-layout tensor as (5, nk+6, nj+6, ni+6)."""
+    total, offs = solver.pool_layout()
+    blk = solver.case.blocks[nb]
+    n = (blk.ni + 6) * (blk.nj + 6) * (blk.nk + 6)
+    return A[:, offs[nb]:offs[nb] + n].reshape(5, blk.nk + 6, blk.nj + 6, blk.ni + 6)
+
+
+def _check_pool(name, t, solver):
+    total, _ = solver.pool_layout()
+    _check_tensor(name, t, (5, total), solver)
+
+
+def step_jvp(solver: Solver, P, dP):
+    """(G(P), G'(P) dP) for one pseudo-time step G (wcns_adj_step_jvp)."""
+    _check_pool("P", P, solver)
+    _check_pool("dP", dP, solver)
+    Pn, dPn = torch.empty_like(P), torch.empty_like(P)
+    torch.cuda.synchronize(P.device)
+    solver.adj_step_jvp(P.data_ptr(), dP.data_ptr(), Pn.data_ptr(), dPn.data_ptr())
+    return Pn, dPn
+
+
+def step_vjp(solver: Solver, P, Wn):
+    """G'(P)^T Wn (wcns_adj_step_vjp)."""
+    _check_pool("P", P, solver)
+    _check_pool("Wn", Wn, solver)
+    gP = torch.empty_like(P)
+    torch.cuda.synchronize(P.device)
+    solver.adj_step_vjp(P.data_ptr(), Wn.data_ptr(), gP.data_ptr())
+    return gP
+
+
+def multistep_gradient(solver: Solver, P0, nsteps, every, omega, budget_gb=0.0):
+    """J = omega . (pressure force coefficients) after nsteps steps from P0 and
+    dJ/dP0, with checkpoints every `every` steps (wcns_adj_multistep); returns
+    (J, gP0, GB of checkpoints and work arrays)."""
+    _check_pool("P0", P0, solver)
+    g = torch.empty_like(P0)
+    torch.cuda.synchronize(P0.device)
+    J, used = solver.adj_multistep(P0.data_ptr(), nsteps, every, omega, budget_gb, g.data_ptr())
+    return J, g, used
+
+
+class _Step(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, P, solver):
+        Pn, _ = step_jvp(solver, P, torch.zeros_like(P))
+        ctx.save_for_backward(P)
+        ctx.solver = solver
+        return Pn
+
+    @staticmethod
+    def backward(ctx, Wn):
+        (P,) = ctx.saved_tensors
+        return step_vjp(ctx.solver, P, Wn.contiguous()), None
+
+
+def step(solver: Solver, P):
+    """One pseudo-time step as a differentiable function of the state."""
+    return _Step.apply(P, solver)

@@ -22,7 +22,8 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=o
          "-Wno-unused-result", "-Wno-unused-variable"]
 
 SOURCES = ["wcns_api.hip", "wcns_rhs_inv.hip", "wcns_rhs_vis.hip", "wcns_step.hip", "wcns_metrics.hip", "wcns_force.hip",
-           "wcns_sweep.hip", "wcns_sa.hip", "wcns_transpose.hip", "wcns_adjoint.hip"]
+           "wcns_sweep.hip", "wcns_sa.hip", "wcns_transpose.hip", "wcns_adjoint.hip",
+           "wcns_adstep.hip"]
 
 
 def _compile(src):

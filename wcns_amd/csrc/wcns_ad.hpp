@@ -13,6 +13,7 @@ namespace wcns {
 // keep the double overloads visible beside the D1 ones below
 using ::sqrt;
 using ::fabs;
+using ::pow;
 
 struct D1 {
     double v, d;
@@ -56,6 +57,12 @@ __device__ inline D1 sqrt(D1 a)
     return D1(s, a.d / (2.0 * s));
 }
 __device__ inline D1 fabs(D1 a) { return a.v < 0.0 ? -a : a; }
+// a^e for a constant exponent (the far-field boundary's isentropic relations)
+__device__ inline D1 pow(D1 a, double e)
+{
+    double p = ::pow(a.v, e);
+    return D1(p, e * (p / a.v) * a.d);
+}
 
 // value part of a double or a D1
 __device__ inline double val(double a) { return a; }

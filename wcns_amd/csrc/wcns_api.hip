@@ -176,6 +176,14 @@ struct wcns_ctx {
     double *ad_pool = nullptr;              // AD_N arrays of total elements
     double *ad_work = nullptr;              // metric slots of the largest block
     double *ad_force = nullptr;             // traction / coordinate gradients and coordinates of a wall
+    // tangent / adjoint of a whole step, allocated on first use: AS_N arrays
+    // of total elements, and the boundary tape (15 doubles per point of every
+    // region of the BC sequence, at as_tape_off[region])
+    double *as_pool = nullptr;
+    double *as_tape = nullptr;
+    std::vector<long> as_tape_off;
+    int *as_ustart = nullptr, *as_uent = nullptr;   // multi-point entries per node (CSR)
+    int as_nuniq = 0;
     int npp = 0, ngroups = 0;
     int *d_npp_pts = nullptr, *d_grp_start = nullptr, *d_grp_n = nullptr, *d_write_ok = nullptr;
     double *d_npp_buf = nullptr;
@@ -547,7 +555,7 @@ int wcns_destroy(wcns_ctx *ctx)
     for (void *p : ps) if (p) hipFree(p);
     for (auto &w : ctx->walls) hipFree(w.xyz);
     void *fs[] = {ctx->d_force_pn, ctx->d_force_part, ctx->d_force_out, ctx->ad_pool, ctx->ad_work, ctx->ad_force,
-                  ctx->d_force_last, ctx->d_fhist, ctx->d_fhist_n};
+                  ctx->d_force_last, ctx->d_fhist, ctx->d_fhist_n, ctx->as_pool, ctx->as_tape, ctx->as_ustart, ctx->as_uent};
     for (void *p : fs) if (p) hipFree(p);
     for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
     for (int c = 0; c < 2; c++)
@@ -2258,6 +2266,415 @@ int wcns_adj_force(wcns_ctx *ctx, int nb, const double *P, const double *X, cons
         CK(hipStreamSynchronize(ctx->stream));
     }
     return WCNS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ tangent and adjoint of a whole step
+// Pool of the step's derivatives (AS_N arrays of total elements): the step's
+// primitives after the boundary conditions (PF, 5), the four sweeps' dQ (X,
+// 20), an all-zero array, the adjoint weights of the new primitives (LPN, 5),
+// and a work region shared by the tangent and the adjoint passes.
+enum { AS_PF = 0, AS_X = 5, AS_ZERO = 25, AS_LPN = 26, AS_W = 31 };
+// tangent work: primitives after the BCs 5, residual 5, radii + coed 4, sweeps 20, result 5
+enum { AT_P = 0, AT_R = 5, AT_S = 10, AT_X = 14, AT_PN = 34 };
+// adjoint work: primitives after the BCs 5, dQ 5, sweep outputs 20, multipliers 20, R 5, coed 1, radii 3
+enum { AA_P = 0, AA_D = 5, AA_XB = 10, AA_MU = 30, AA_RB = 50, AA_CB = 55, AA_SR = 56, AA_N = 59 };
+enum { AS_N = AS_W + AA_N };
+
+// the BC sequence of boundary_all, region by region
+template <typename Fn>
+static int for_bc_regions(wcns_ctx *ctx, bool reverse, Fn fn)
+{
+    std::vector<const DRegion *> seq;
+    for (auto &b : ctx->blocks)
+        for (int r : b.bcorder) seq.push_back(&b.regs[r].d);
+    for (size_t a = 0; a < seq.size(); a++) {
+        size_t q = reverse ? seq.size() - 1 - a : a;
+        int e = fn(*seq[q], q);
+        if (e) return e;
+    }
+    return WCNS_OK;
+}
+
+static int as_check(wcns_ctx *ctx)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (ctx->prm.nvis != 0 || ctx->prm.nlamtur >= 0)
+        return fail(ctx, WCNS_E_UNSUPPORTED, "the step tangent and adjoint cover inviscid steps (nvis = 0)");
+    if (ctx->lhs_prec != WCNS_LHS_FP64)
+        return fail(ctx, WCNS_E_UNSUPPORTED, "the step tangent and adjoint differentiate the FP64 implicit operator");
+    for (auto &b : ctx->blocks)
+        if (b.ni <= 4 || b.nj <= 4 || b.nk <= 4)
+            return fail(ctx, WCNS_E_UNSUPPORTED, "the step adjoint needs more than 4 nodes per direction");
+    TRY(ad_alloc(ctx));
+    if (ctx->as_pool) return WCNS_OK;
+    TRY(dalloc(ctx, &ctx->as_pool, (size_t)AS_N * ctx->total));
+    long off = 0;
+    ctx->as_tape_off.clear();
+    for_bc_regions(ctx, false, [&](const DRegion &g, size_t) {
+        ctx->as_tape_off.push_back(off);
+        off += 15L * g.npts;
+        return 0;
+    });
+    TRY(dalloc(ctx, &ctx->as_tape, (size_t)off));
+    // the multi-point entries grouped by node, in entry order
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> by_node;
+    const int np = (int)ctx->h_npp_pts.size() / 4;
+    for (int e = 0; e < np; e++) {
+        const int *p = &ctx->h_npp_pts[4 * e];
+        by_node[{p[0], p[1], p[2], p[3]}].push_back(e);
+    }
+    std::vector<int> us{0}, ue;
+    for (auto &kv : by_node) {
+        for (int e : kv.second) ue.push_back(e);
+        us.push_back((int)ue.size());
+    }
+    ctx->as_nuniq = (int)by_node.size();
+    TRY(dalloc(ctx, &ctx->as_ustart, us.size()));
+    TRY(dalloc(ctx, &ctx->as_uent, std::max<size_t>(ue.size(), 1)));
+    TRY(copy_sync(ctx, ctx->as_ustart, us.data(), 4 * us.size(), hipMemcpyHostToDevice));
+    if (!ue.empty()) TRY(copy_sync(ctx, ctx->as_uent, ue.data(), 4 * ue.size(), hipMemcpyHostToDevice));
+    return WCNS_OK;
+}
+
+static int as_fill(wcns_ctx *ctx, double *a, long n)
+{
+    return launch_fill_u64(reinterpret_cast<unsigned long long *>(a), n, 0ull, ctx->stream)
+               ? fail(ctx, WCNS_E_HIP, "clearing the step adjoint arrays")
+               : WCNS_OK;
+}
+
+static int as_copy(wcns_ctx *ctx, double *const dst[5], const double *src)
+{
+    for (int m = 0; m < 5; m++)
+        CK(hipMemcpyAsync(dst[m], src + (size_t)m * ctx->total, 8 * (size_t)ctx->total, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    return WCNS_OK;
+}
+
+static int as_copy_out(wcns_ctx *ctx, double *dst, double *const src[5])
+{
+    for (int m = 0; m < 5; m++)
+        CK(hipMemcpyAsync(dst + (size_t)m * ctx->total, src[m], 8 * (size_t)ctx->total, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    return WCNS_OK;
+}
+
+static int zero_faces(wcns_ctx *ctx, double *arr)
+{
+    return launch_dq_zero_faces(ctx->d_regs, ctx->d_rwork + ctx->zero_w0, ctx->zero_nw, ctx->d_blocks, arr, nullptr,
+                                ctx->total, ctx->stream)
+               ? fail(ctx, WCNS_E_HIP, "zeroing the physical faces")
+               : WCNS_OK;
+}
+
+// the interface averagings in their sequence order on the arrays of F
+static int communicate_on(wcns_ctx *ctx, const DFields &F, int pv)
+{
+    std::vector<DBlock> dbs;
+    for (auto &b : ctx->blocks) dbs.push_back(b.db);
+    int nvar = pv ? 6 : 5;
+    if (launch_npp_gather(ctx->d_npp_pts, ctx->npp, ctx->d_blocks, F, ctx->d_npp_buf, nvar, !pv, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "npp gather");
+    for (auto &g : ctx->iface)
+        if (launch_avg2pm(g, dbs.data(), F, pv, ctx->stream)) return fail(ctx, WCNS_E_HIP, "avg2pm");
+    if (launch_npp_assign(ctx->d_grp_start, ctx->d_grp_n, ctx->d_npp_pts, ctx->d_write_ok, ctx->ngroups,
+                          ctx->d_npp_buf, ctx->d_blocks, F, nvar, !pv, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "npp assign");
+    return WCNS_OK;
+}
+
+// their transpose on the 5 arrays lam (dQ with use_vol, else the primitives)
+static int communicate_t(wcns_ctx *ctx, double *lam, int use_vol)
+{
+    std::vector<DBlock> dbs;
+    for (auto &b : ctx->blocks) dbs.push_back(b.db);
+    const double *vol = ctx->metric[9];
+    if (launch_npp_assign_t(ctx->d_grp_start, ctx->d_grp_n, ctx->d_npp_pts, ctx->d_write_ok, ctx->ngroups,
+                            ctx->d_blocks, vol, lam, ctx->total, ctx->d_npp_buf, use_vol, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "npp assign transpose");
+    for (size_t a = ctx->iface.size(); a-- > 0;)
+        if (launch_avg2pm_t(ctx->iface[a], dbs.data(), vol, lam, ctx->total, use_vol, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "avg2pm transpose");
+    if (launch_npp_gather_t(ctx->d_npp_pts, ctx->as_ustart, ctx->as_uent, ctx->as_nuniq, ctx->d_blocks, vol, lam,
+                            ctx->total, ctx->d_npp_buf, use_vol, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "npp gather transpose");
+    return WCNS_OK;
+}
+
+static DFields with_prims(DFields F, double *p)
+{
+    const long T = F.total;
+    F.r = p; F.u = p + T; F.v = p + 2 * T; F.w = p + 3 * T; F.p = p + 4 * T;
+    return F;
+}
+
+// One step of the state in the current primitive buffer (P, if given, is
+// copied there first), as step_once computes it but with the sweeps into
+// separate arrays and the boundary values taped; with dP its tangent too.
+// Leaves PF, X, R, the radii and coed of the step, and the new state current.
+static int as_forward(wcns_ctx *ctx, const double *P, const double *dP)
+{
+    const long T = ctx->total;
+    double *S = ctx->as_pool, *W = S + (size_t)AS_W * T;
+    DFields F = ctx->fields();
+    double *cur[5] = {F.r, F.u, F.v, F.w, F.p};
+    if (P) TRY(as_copy(ctx, cur, P));
+    const bool tan = dP != nullptr;
+    if (tan) {
+        TRY(as_fill(ctx, W, (long)AA_N * T));
+        double *tp[5];
+        for (int m = 0; m < 5; m++) tp[m] = W + (AT_P + m) * T;
+        TRY(as_copy(ctx, tp, dP));
+    }
+    std::vector<DBlock> dbs;
+    for (auto &b : ctx->blocks) dbs.push_back(b.db);
+    TRY(for_bc_regions(ctx, false, [&](const DRegion &g, size_t q) {
+        return launch_bc_fwd(g, dbs.data(), F, ctx->dp, tan ? W + AT_P * T : nullptr, ctx->as_tape + ctx->as_tape_off[q],
+                             ctx->stream)
+                   ? fail(ctx, WCNS_E_HIP, "boundary tangent launch")
+                   : WCNS_OK;
+    }));
+    TRY(as_copy_out(ctx, S + AS_PF * T, cur));
+    TRY(wcns_initial_q_dq_c_t(ctx));
+    TRY(wcns_timestep(ctx));
+    TRY(wcns_r_h_s_invis(ctx));
+    TRY(communicate_on(ctx, F, 0));
+    TRY(zero_faces(ctx, F.rhs));
+    if (tan) {
+        if (launch_ts_jvp(ctx->grid[GRID_NODES], F, ctx->dp, W + AT_P * T, W + AT_S * T, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "radii tangent");
+        TRY(ad_zero(ctx, AD_M, AD_R - AD_M));
+        double *A = ctx->ad_pool;
+        AdTan D{W + AT_P * T, A + AD_M * T, A + AD_T * T, A + AD_MU * T, A + AD_DN * T, W + AT_R * T};
+        for (auto &b : ctx->blocks)
+            for (int dir = 0; dir < 3; dir++)
+                if (launch_rhs_inv_jvp(dir, b.db, F, ctx->dp, D, ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "inviscid tangent");
+        DFields Ft = F;
+        Ft.rhs = W + AT_R * T;
+        TRY(communicate_on(ctx, Ft, 0));
+        TRY(zero_faces(ctx, W + AT_R * T));
+    }
+    AdSweep A{};
+    A.x = S + AS_X * T;
+    if (tan) {
+        A.tx = W + AT_X * T;
+        A.tp = W + AT_P * T;
+        A.tsr = W + AT_S * T;
+        A.tcoed = W + (AT_S + 3) * T;
+        A.trhs = W + AT_R * T;
+        A.zero = S + AS_ZERO * T;
+    }
+    for (auto &b : ctx->blocks)
+        if (launch_ad_sweeps_fwd(b.db, F, ctx->dp, A, ctx->stream)) return fail(ctx, WCNS_E_HIP, "sweeps");
+    // the update's dQ: the last sweep's, zero on the physical faces (x3 itself
+    // is kept: the adjoint needs the values the sweep used)
+    const double *x3 = S + (AS_X + 15) * T;
+    CK(hipMemcpyAsync(F.dq, x3, 8 * 5 * (size_t)T, hipMemcpyDeviceToDevice, ctx->stream));
+    TRY(zero_faces(ctx, F.dq));
+    if (tan) TRY(zero_faces(ctx, W + (AT_X + 15) * T));
+    if (launch_fill_u64(ctx->d_nflag, (long)ctx->blocks.size(), 0ull, ctx->stream) ||
+        launch_update(ctx->grid[GRID_NODES], F, ctx->dp, ctx->d_flag, ctx->d_nflag, ctx->flag_cap, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "update");
+    if (tan && launch_upd_jvp(ctx->grid[GRID_NODES], ctx->dp, S + AS_PF * T, F.dq, W + AT_P * T, W + (AT_X + 15) * T,
+                              W + AT_PN * T, T, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "update tangent");
+    ctx->cur ^= 1;
+    ctx->pt_fresh = false;
+    TRY(communicate_on(ctx, ctx->fields(), 1));
+    if (tan) {
+        DFields Ft = with_prims(ctx->fields(), W + AT_PN * T);
+        Ft.t = S + AS_ZERO * T;    // the temperature's average is not part of an inviscid state
+        TRY(communicate_on(ctx, Ft, 1));   // (averages of zeros: the array stays zero)
+    }
+    std::vector<unsigned long long> nf(ctx->blocks.size());
+    CK(hipMemcpyAsync(nf.data(), ctx->d_nflag, 8 * nf.size(), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    TRY(check_counters(ctx));
+    for (auto n : nf)
+        if (n) return fail(ctx, WCNS_E_UNSUPPORTED, "a node left the update limits: the repaired step is not differentiable");
+    return WCNS_OK;
+}
+
+// The adjoint of the step from state P (5 pool arrays, current buffer if
+// null) for weights Wn of the new primitives, into gP (interior nodes).
+static int as_reverse(wcns_ctx *ctx, const double *P, const double *Wn, double *gP)
+{
+    const long T = ctx->total;
+    double *S = ctx->as_pool, *W = S + (size_t)AS_W * T;
+    TRY(as_forward(ctx, P, nullptr));
+    double *lpn = S + AS_LPN * T;
+    if (Wn != lpn) CK(hipMemcpyAsync(lpn, Wn, 8 * 5 * (size_t)T, hipMemcpyDeviceToDevice, ctx->stream));
+    TRY(as_fill(ctx, W, (long)AA_N * T));
+    TRY(communicate_t(ctx, lpn, 0));
+    double *lp = W + AA_P * T, *ld = W + AA_D * T;
+    if (launch_upd_vjp(ctx->grid[GRID_NODES], ctx->dp, S + AS_PF * T, ctx->dq, lpn, lp, ld, T, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "update adjoint");
+    TRY(zero_faces(ctx, ld));
+    CK(hipMemcpyAsync(W + (AA_XB + 15) * T, ld, 8 * 5 * (size_t)T, hipMemcpyDeviceToDevice, ctx->stream));
+    const DFields Fs = with_prims(ctx->fields(), S + AS_PF * T);
+    AdSweep A{};
+    A.x = S + AS_X * T;
+    A.mu = W + AA_MU * T;
+    A.xb = W + AA_XB * T;
+    A.rb = W + AA_RB * T;
+    A.cb = W + AA_CB * T;
+    A.lp = lp;
+    A.lsr = W + AA_SR * T;
+    for (size_t nb = ctx->blocks.size(); nb-- > 0;)
+        if (launch_ad_sweeps_rev(ctx->blocks[nb].db, Fs, ctx->dp, A, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "sweeps adjoint");
+    if (launch_ts_vjp(ctx->grid[GRID_NODES], Fs, ctx->dp, A.lsr, A.cb, lp, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "radii adjoint");
+    TRY(zero_faces(ctx, A.rb));
+    TRY(communicate_t(ctx, A.rb, 1));
+    TRY(ad_zero(ctx, AD_M, AD_R - AD_M));
+    double *AD = ctx->ad_pool;
+    AdAdj G{A.rb, lp, AD + AD_M * T, AD + AD_T * T, AD + AD_MU * T, AD + AD_DN * T, AD + AD_G * T};
+    for (auto &b : ctx->blocks)
+        for (int dir = 0; dir < 3; dir++)
+            if (launch_rhs_inv_vjp(dir, b.db, Fs, ctx->dp, G, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "inviscid adjoint");
+    std::vector<DBlock> dbs;
+    for (auto &b : ctx->blocks) dbs.push_back(b.db);
+    TRY(for_bc_regions(ctx, true, [&](const DRegion &g, size_t q) {
+        return launch_bc_rev(g, dbs.data(), Fs, ctx->dp, lp, ctx->as_tape + ctx->as_tape_off[q], ctx->stream)
+                   ? fail(ctx, WCNS_E_HIP, "boundary adjoint launch")
+                   : WCNS_OK;
+    }));
+    for (auto &b : ctx->blocks)
+        if (launch_zero_ghosts5(b.db, lp, T, ctx->stream)) return fail(ctx, WCNS_E_HIP, "ghost clear");
+    CK(hipMemcpyAsync(gP, lp, 8 * 5 * (size_t)T, hipMemcpyDeviceToDevice, ctx->stream));
+    return WCNS_OK;
+}
+
+// J = sum_c omega_c C_c (pressure part of the wall force, scaled as
+// wcns_force) of the current state, and dJ/dP into g (5 pool arrays)
+static int as_force(wcns_ctx *ctx, const double omega[6], double *J, double *g)
+{
+    if (ctx->prm.sref == 0.0 || ctx->prm.lfref == 0.0) return fail(ctx, WCNS_E_ARG, "sref and lfref must be non-zero");
+    const long T = ctx->total;
+    DFields F = ctx->fields();
+    DParams dpp = ctx->dp;
+    dpp.nvis = 0;
+    const wcns_params &p = ctx->prm;
+    const double ref[3] = {p.xref, p.yref, p.zref};
+    const int wf = 2 - p.nwholefield;
+    double om[6];
+    for (int c = 0; c < 6; c++) om[c] = omega[c] * wf * (c < 3 ? 1.0 / p.sref : 1.0 / (p.sref * p.lfref));
+    TRY(as_fill(ctx, g, 5 * T));
+    TRY(ad_zero(ctx, AD_M, AD_R - AD_M));
+    double *AD = ctx->ad_pool;
+    double sum = 0.0;
+    for (auto &wr : ctx->walls) {
+        const DBlock &db = ctx->blocks[wr.d.nb].db;
+        const int np = wr.d.npts;
+        double *lpn3 = ctx->ad_force, *lxyz = lpn3 + 3 * (size_t)np;
+        if (launch_force_region(wr.d, db, F, dpp, wr.xyz, ctx->d_force_pn, ctx->d_force_part, ctx->d_force_out, ref,
+                                ctx->stream) ||
+            launch_force_vjp(wr.d, db, F, dpp, ctx->d_force_pn, wr.xyz, ref, om, lpn3, lxyz, g, AD + AD_M * T,
+                             ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "force adjoint");
+        double part[6];
+        CK(hipMemcpyAsync(part, ctx->d_force_out, sizeof part, hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        for (int c = 0; c < 6; c++) sum = sum + om[c] * part[c];
+    }
+    *J = sum;
+    return WCNS_OK;
+}
+
+extern "C" {
+
+int wcns_pool_layout(wcns_ctx *ctx, long *total, long *offsets)
+{
+    if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (total) *total = ctx->total;
+    if (offsets)
+        for (size_t nb = 0; nb < ctx->blocks.size(); nb++) offsets[nb] = ctx->blocks[nb].db.off;
+    return WCNS_OK;
+}
+
+int wcns_adj_step_jvp(wcns_ctx *ctx, const double *P, const double *dP, double *Pn, double *dPn)
+{
+    TRY(as_check(ctx));
+    if (!dP) return fail(ctx, WCNS_E_ARG, "null direction");
+    TRY(as_forward(ctx, P, dP));
+    DFields F = ctx->fields();
+    double *cur[5] = {F.r, F.u, F.v, F.w, F.p};
+    if (Pn) TRY(as_copy_out(ctx, Pn, cur));
+    if (dPn) {
+        const long T = ctx->total;
+        CK(hipMemcpyAsync(dPn, ctx->as_pool + (size_t)(AS_W + AT_PN) * T, 8 * 5 * (size_t)T, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+        for (auto &b : ctx->blocks)
+            if (launch_zero_ghosts5(b.db, dPn, T, ctx->stream)) return fail(ctx, WCNS_E_HIP, "ghost clear");
+    }
+    CK(hipStreamSynchronize(ctx->stream));
+    return WCNS_OK;
+}
+
+int wcns_adj_step_vjp(wcns_ctx *ctx, const double *P, const double *Wn, double *gP)
+{
+    TRY(as_check(ctx));
+    if (!Wn || !gP) return fail(ctx, WCNS_E_ARG, "null array");
+    TRY(as_reverse(ctx, P, Wn, gP));
+    CK(hipStreamSynchronize(ctx->stream));
+    return WCNS_OK;
+}
+
+int wcns_adj_multistep(wcns_ctx *ctx, const double *P0, int nsteps, int every, const double omega[6],
+                       double budget_gb, double *J, double *gP0, double *used_gb)
+{
+    TRY(as_check(ctx));
+    if (!P0 || !omega || !J || nsteps < 1 || every < 1) return fail(ctx, WCNS_E_ARG, "bad arguments");
+    const long T = ctx->total;
+    const int nck = (nsteps + every - 1) / every, seg = std::min(every, nsteps);
+    const size_t state = 5 * (size_t)T;
+    const double store = 8.0 * state * (nck + seg + 1);
+    const double work = 8.0 * (double)T * (AS_N + AD_N);
+    if (used_gb) *used_gb = (store + work) / 1e9;
+    if (budget_gb > 0.0 && store + work > budget_gb * 1e9)
+        return fail(ctx, WCNS_E_ARG, "checkpoints and work arrays need " + std::to_string((store + work) / 1e9) +
+                                         " GB, above the budget");
+    double *ck = nullptr;
+    TRY(dalloc(ctx, &ck, state * (nck + seg + 1)));
+    double *sg = ck + state * nck, *lam = sg + state * seg;
+    auto run = [&]() -> int {
+        DFields F = ctx->fields();
+        double *cur[5] = {F.r, F.u, F.v, F.w, F.p};
+        TRY(as_copy(ctx, cur, P0));
+        for (int n = 0; n < nsteps; n++) {
+            if (n % every == 0) {
+                F = ctx->fields();
+                double *c5[5] = {F.r, F.u, F.v, F.w, F.p};
+                TRY(as_copy_out(ctx, ck + state * (n / every), c5));
+            }
+            TRY(as_forward(ctx, nullptr, nullptr));
+        }
+        TRY(as_force(ctx, omega, J, lam));
+        // each segment: its states replayed from the checkpoint, then reversed
+        for (int c = nck - 1; c >= 0; c--) {
+            const int s0 = c * every, s1 = std::min(nsteps, s0 + every);
+            CK(hipMemcpyAsync(sg, ck + state * c, 8 * state, hipMemcpyDeviceToDevice, ctx->stream));
+            for (int n = s0; n + 1 < s1; n++) {
+                TRY(as_forward(ctx, sg + state * (n - s0), nullptr));
+                F = ctx->fields();
+                double *c5[5] = {F.r, F.u, F.v, F.w, F.p};
+                TRY(as_copy_out(ctx, sg + state * (n + 1 - s0), c5));
+            }
+            for (int n = s1 - 1; n >= s0; n--) TRY(as_reverse(ctx, sg + state * (n - s0), lam, lam));
+        }
+        CK(hipMemcpyAsync(gP0, lam, 8 * state, hipMemcpyDeviceToDevice, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        return WCNS_OK;
+    };
+    int e = run();
+    hipStreamSynchronize(ctx->stream);
+    hipFree(ck);
+    return e;
 }
 
 }  // extern "C"

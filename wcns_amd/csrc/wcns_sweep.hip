@@ -32,6 +32,7 @@
 //                  layer) staged once per step by k_skew_stage, so the lanes of
 //                  a tile read a few contiguous rows per array at every step
 //                  instead of one cache line per lane.
+#include <type_traits>
 #include "wcns_kernels.hpp"
 
 namespace wcns {
@@ -1031,6 +1032,405 @@ int launch_sweep_batched(int kind, const SweepPlan &S, const DFields &F, const D
     hipLaunchKernelGGL(k_fill_u64, grid1(S.nxch, 256), dim3(256), 0, s, S.xch, S.nxch, XCH_EMPTY);
     if (prec == 1) return launch_pipe<float>(kind, S, F, P, (float *)pool, sktot, s);
     return launch_pipe<double>(kind, S, F, P, (double *)pool, sktot, s);
+}
+
+
+// ------------------------------------------------ tangent and adjoint of the sweeps
+// The four sweeps of one step, for the derivatives of a whole step.  Each
+// sweep writes its own array (x_0..x_3): the values of the in-place schedule,
+// since there lusgs_u reads lusgs_l's dQ at its own node and each SGS sweep
+// reads the previous sweep's dQ at the nodes still ahead of it.  In those
+// terms, with A+(n,d) / A-(n,d) the split Jacobian of node n in direction d
+// with npn = +1 / -1 (mxdq_apply on mxdq_coef of n's primitives, metric row d
+// and radius d) and c the diagonal coed:
+//   x0_i = c_i (-R_i + sum_d A+(i-e_d) x0_{i-e_d})                  ascending
+//   x1_i = x0_i - c_i sum_d A-(i+e_d) x1_{i+e_d}                     descending
+//   x2_i = c_i (sum_d A+(i-e_d) x2_{i-e_d} - A-(i+e_d) x1_{i+e_d} - R_i)   ascending
+//   x3_i = c_i (sum_d A+(i-e_d) x2_{i-e_d} - A-(i+e_d) x3_{i+e_d} - R_i)   descending
+// with every term of a neighbour outside 1..n zero (its dQ is).  The tangent
+// runs the solver's own node arithmetic in D1; the adjoint solves the
+// transposed systems in the opposite order, each node applying its own
+// transposed matrices to its neighbours' multipliers, then differentiates
+// mu^T A(n,d) x_n with respect to n's primitives and radius.
+namespace {
+
+template <int KIND>
+__device__ inline void raw_merge(const NodeRaw<KIND, double> &a, const NodeRaw<KIND, double> &t, NodeRaw<KIND, D1> &r)
+{
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            r.nn[d].prim[m] = D1(a.nn[d].prim[m], t.nn[d].prim[m]);
+            r.no[d].prim[m] = D1(a.no[d].prim[m], t.no[d].prim[m]);
+            r.dqo[d][m] = D1(a.dqo[d][m], t.dqo[d][m]);
+            r.ghost[d][m] = D1(a.ghost[d][m], t.ghost[d][m]);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            r.nn[d].gy[c] = D1(a.nn[d].gy[c], t.nn[d].gy[c]);
+            r.no[d].gy[c] = D1(a.no[d].gy[c], t.no[d].gy[c]);
+        }
+        r.nn[d].rad = D1(a.nn[d].rad, t.nn[d].rad);
+        r.no[d].rad = D1(a.no[d].rad, t.no[d].rad);
+        r.rv[d] = D1(a.rv[d], t.rv[d]);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        r.own[m] = D1(a.own[m], t.own[m]);
+        r.rhs[m] = D1(a.rhs[m], t.rhs[m]);
+    }
+    r.coed = D1(a.coed, t.coed);
+}
+
+__device__ inline bool plane_node(const DBlock &b, int s, int &i, int &j, int &k)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int jlo = s - b.ni - b.nk, jhi = s - 2;
+    jlo = jlo < 1 ? 1 : jlo;
+    jhi = jhi > b.nj ? b.nj : jhi;
+    if (jhi < jlo) return false;
+    long nj_ = jhi - jlo + 1;
+    if (t >= nj_ * b.nk) return false;
+    j = jlo + (int)(t % nj_);
+    k = (int)(t / nj_) + 1;
+    i = s - j - k;
+    return i >= 1 && i <= b.ni;
+}
+
+// One hyperplane of sweep KIND into X (and its tangent Xt when TAN).  A is the
+// array lusgs_l/u read outside the block and lusgs_u at its own node (x0),
+// or gs_pr_l's previous sweep (x1); B gs_pr_u's previous sweep (x2).
+template <int KIND, bool TAN>
+__global__ void __launch_bounds__(256) k_ad_sweep(DBlock b, Operands<double> O, Operands<double> Ot, DqPtrs A,
+                                                  DqPtrs B, DqPtrs At, DqPtrs Bt, DqPtrs X, DqPtrs Xt, DParams P,
+                                                  int s)
+{
+    int i, j, k;
+    if (!plane_node(b, s, i, j, k)) return;
+    constexpr int off = sweep_off<KIND>();
+    using T = typename std::conditional<TAN, D1, double>::type;
+    G3Index ix{b};
+    NodeIn<KIND, T> in;
+    NodeRaw<KIND, double> rv{};
+    node_load<KIND, double>(b, O, ix, A, B, P, i, j, k, rv);
+    if constexpr (TAN) {
+        NodeRaw<KIND, double> rt{};
+        node_load<KIND, double>(b, Ot, ix, At, Bt, P, i, j, k, rt);
+        NodeRaw<KIND, D1> r;
+        raw_merge<KIND>(rv, rt, r);
+        node_prep<KIND, D1>(P, r, in);
+    } else {
+        node_prep<KIND, double>(P, rv, in);
+    }
+    const int nn[3] = {b.ni, b.nj, b.nk};
+    const int ijk[3] = {i, j, k};
+    T nw[3][5];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int c = ijk[d] + off;
+        if (c < 1 || c > nn[d]) {
+            outside<KIND, T>(in, d, nw[d]);
+        } else {
+            long gn = ix(i + (d == 0 ? off : 0), j + (d == 1 ? off : 0), k + (d == 2 ? off : 0));
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                if constexpr (TAN) nw[d][m] = D1(X.c[m][gn], Xt.c[m][gn]);
+                else nw[d][m] = X.c[m][gn];
+            }
+        }
+    }
+    T out[5];
+    node_compute<KIND, T>(P, in, nw, out);
+    const long g = ix(i, j, k);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        X.c[m][g] = val(out[m]);
+        if constexpr (TAN) Xt.c[m][g] = out[m].d;
+    }
+}
+
+// g = M^T mu for the split Jacobian M of mxdq_apply: M = l1 I + x1 (c2 n w^T -
+// u h^T) + x2 (n h^T - u w^T), w = (ct, -nx, -ny, -nz, 0), h = (af, -ae um,
+// -ae vm, -ae wm, ae), n = (0, nx, ny, nz, ct), u = (1, um, vm, wm, hm).
+__device__ inline void mxdq_apply_t(const MxCoef<double> &c, const double *mu, double *g)
+{
+    double nm = c.nx * mu[1] + c.ny * mu[2] + c.nz * mu[3] + c.ct * mu[4];
+    double um = mu[0] + c.um * mu[1] + c.vm * mu[2] + c.wm * mu[3] + c.hm * mu[4];
+    double aw = c.x1 * c.c2 * nm - c.x2 * um;
+    double ah = c.x2 * nm - c.x1 * um;
+    g[0] = c.l1 * mu[0] + aw * c.ct + ah * c.af;
+    g[1] = c.l1 * mu[1] - aw * c.nx - ah * c.ae * c.um;
+    g[2] = c.l1 * mu[2] - aw * c.ny - ah * c.ae * c.vm;
+    g[3] = c.l1 * mu[3] - aw * c.nz - ah * c.ae * c.wm;
+    g[4] = c.l1 * mu[4] + ah * c.ae;
+}
+
+// the operands of node g's own split Jacobian in direction D
+template <int D, typename T>
+__device__ inline void own_raw(const Operands<double> &O, long g, T *prim, T *gy, T &rad)
+{
+#pragma unroll
+    for (int m = 0; m < 5; m++) prim[m] = O.p[m][g];
+#pragma unroll
+    for (int c = 0; c < 3; c++) gy[c] = O.m[D][c][g];
+    rad = O.sr[D][g];
+}
+
+struct TSweep {
+    const double *x[4][5];   // the four sweeps' dQ
+    double *mu[4][5];        // their multipliers
+    double *xb[4][5];        // adjoints of their outputs
+    double *rb[5];           // adjoint of R (accumulated)
+    double *cb;              // adjoint of coed (accumulated)
+};
+
+// One hyperplane of the transposed sweep KIND (processed in the opposite
+// order to the sweep): z_i = xbar_i +- sum_d A(i,d)^T mu_{i-+e_d}, mu_i = c_i z_i,
+// and the adjoints of R, coed and (lusgs_u) x0 that z carries.
+template <int KIND>
+__global__ void __launch_bounds__(256) k_ad_tsweep(DBlock b, Operands<double> O, TSweep S, DParams P, int s)
+{
+    int i, j, k;
+    if (!plane_node(b, s, i, j, k)) return;
+    G3Index ix{b};
+    const long g = ix(i, j, k);
+    constexpr bool up = (KIND == 0 || KIND == 2);   // multipliers of the nodes ahead, A+ of this node
+    const double npn = up ? 1.0 : -1.0;
+    const int nn[3] = {b.ni, b.nj, b.nk};
+    const int ijk[3] = {i, j, k};
+    double z[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) z[m] = S.xb[KIND][m][g];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int c = ijk[d] + (up ? 1 : -1);
+        if (c < 1 || c > nn[d]) continue;
+        const long gn = ix(i + (d == 0 ? (up ? 1 : -1) : 0), j + (d == 1 ? (up ? 1 : -1) : 0),
+                           k + (d == 2 ? (up ? 1 : -1) : 0));
+        double prim[5], gy[3], rad;
+        if (d == 0) own_raw<0, double>(O, g, prim, gy, rad);
+        else if (d == 1) own_raw<1, double>(O, g, prim, gy, rad);
+        else own_raw<2, double>(O, g, prim, gy, rad);
+        MxCoef<double> cf;
+        mxdq_coef<double>(prim, gy, rad, npn, P.gama, cf);
+        double mu[5], tg[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++) mu[m] = S.mu[KIND][m][gn];
+        mxdq_apply_t(cf, mu, tg);
+#pragma unroll
+        for (int m = 0; m < 5; m++) z[m] = up ? z[m] + tg[m] : z[m] - tg[m];
+    }
+    const double coed = O.coed[g];
+    double cb = 0.0;
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const double mu = coed * z[m];
+        S.mu[KIND][m][g] = mu;
+        double sm;   // d x_i / d coed
+        if (KIND == 1) {
+            S.xb[0][m][g] = S.xb[0][m][g] + z[m];
+            sm = (S.x[1][m][g] - S.x[0][m][g]) / coed;
+        } else {
+            S.rb[m][g] = S.rb[m][g] - mu;
+            sm = S.x[KIND][m][g] / coed;
+        }
+        cb = cb + z[m] * sm;
+    }
+    S.cb[g] = S.cb[g] + cb;
+}
+
+// After the transposed gs_pr_u: xbar2_n += sum_d A+(n,d)^T mu3_{n+e_d}; after
+// the transposed gs_pr_l: xbar1_n -= sum_d A-(n,d)^T mu2_{n-e_d}.
+template <int KIND>
+__global__ void __launch_bounds__(256) k_ad_tgather(DBlock b, Operands<double> O, TSweep S, DParams P)
+{
+    const long n = (long)b.ni * b.nj * b.nk;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
+    G3Index ix{b};
+    const long g = ix(i, j, k);
+    constexpr bool up = KIND == 3;
+    constexpr int dst = KIND == 3 ? 2 : 1;
+    const int nn[3] = {b.ni, b.nj, b.nk};
+    const int ijk[3] = {i, j, k};
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int c = ijk[d] + (up ? 1 : -1);
+        if (c < 1 || c > nn[d]) continue;
+        const long gn = ix(i + (d == 0 ? (up ? 1 : -1) : 0), j + (d == 1 ? (up ? 1 : -1) : 0),
+                           k + (d == 2 ? (up ? 1 : -1) : 0));
+        double prim[5], gy[3], rad;
+        if (d == 0) own_raw<0, double>(O, g, prim, gy, rad);
+        else if (d == 1) own_raw<1, double>(O, g, prim, gy, rad);
+        else own_raw<2, double>(O, g, prim, gy, rad);
+        MxCoef<double> cf;
+        mxdq_coef<double>(prim, gy, rad, up ? 1.0 : -1.0, P.gama, cf);
+        double mu[5], tg[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++) mu[m] = S.mu[KIND][m][gn];
+        mxdq_apply_t(cf, mu, tg);
+#pragma unroll
+        for (int m = 0; m < 5; m++) acc[m] = acc[m] + tg[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 5; m++) S.xb[dst][m][g] = up ? S.xb[dst][m][g] + acc[m] : S.xb[dst][m][g] - acc[m];
+}
+
+// sum over pairs p of wt_p mu_p^T A(coef) x_p, in D1, along seed q of
+// (primitives 0..4, radius 5)
+__device__ inline double coef_pairs(const double *prim, const double *gy, double rad, double npn, double gama,
+                                    int q, int np, const double (*mu)[5], const double (*x)[5])
+{
+    D1 pr[5], gd[3];
+#pragma unroll
+    for (int m = 0; m < 5; m++) pr[m] = D1(prim[m], q == m ? 1.0 : 0.0);
+#pragma unroll
+    for (int c = 0; c < 3; c++) gd[c] = D1(gy[c]);
+    MxCoef<D1> cf;
+    mxdq_coef<D1>(pr, gd, D1(rad, q == 5 ? 1.0 : 0.0), D1(npn), D1(gama), cf);
+    double s = 0.0;
+    for (int p = 0; p < np; p++) {
+        D1 xx[5], f[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++) xx[m] = D1(x[p][m]);
+        mxdq_apply<D1>(cf, xx, f);
+#pragma unroll
+        for (int m = 0; m < 5; m++) s = s + mu[p][m] * f[m].d;
+    }
+    return s;
+}
+
+// Gradients of sum mu^T A(n,d) x_n over the sweeps' pairs w.r.t. node n's
+// primitives (lp) and spectral radii (lsr), accumulated:
+//   A+(n,d): (mu0, x0), (mu2, x2), (mu3, x2) at n+e_d;
+//   A-(n,d): (-mu1, x1), (-mu2, x1), (-mu3, x3) at n-e_d.
+__global__ void __launch_bounds__(256) k_ad_tcoef(DBlock b, Operands<double> O, TSweep S, DParams P, double *lp,
+                                                  double *lsr, long total)
+{
+    const long n = (long)b.ni * b.nj * b.nk;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
+    G3Index ix{b};
+    const long g = ix(i, j, k);
+    const int nn[3] = {b.ni, b.nj, b.nk};
+    const int ijk[3] = {i, j, k};
+    double gp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double prim[5], gy[3], rad;
+        if (d == 0) own_raw<0, double>(O, g, prim, gy, rad);
+        else if (d == 1) own_raw<1, double>(O, g, prim, gy, rad);
+        else own_raw<2, double>(O, g, prim, gy, rad);
+        double gr = 0.0;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int c = ijk[d] + (side == 0 ? 1 : -1);
+            if (c < 1 || c > nn[d]) continue;
+            const long gn = ix(i + (d == 0 ? (side == 0 ? 1 : -1) : 0), j + (d == 1 ? (side == 0 ? 1 : -1) : 0),
+                               k + (d == 2 ? (side == 0 ? 1 : -1) : 0));
+            double mu[3][5], x[3][5];
+            const int ka[3] = {0, 2, 3}, xa[3] = {0, 2, 2};      // side 0: A+
+            const int kb[3] = {1, 2, 3}, xbb[3] = {1, 1, 3};     // side 1: A-
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                const int km = side == 0 ? ka[p] : kb[p], kx = side == 0 ? xa[p] : xbb[p];
+#pragma unroll
+                for (int m = 0; m < 5; m++) {
+                    const double mv = km == 0 ? S.mu[0][m][gn] : km == 1 ? S.mu[1][m][gn] :
+                                      km == 2 ? S.mu[2][m][gn] : S.mu[3][m][gn];
+                    mu[p][m] = side == 0 ? mv : -mv;
+                    x[p][m] = kx == 0 ? S.x[0][m][g] : kx == 1 ? S.x[1][m][g] : kx == 2 ? S.x[2][m][g] : S.x[3][m][g];
+                }
+            }
+            const double npn = side == 0 ? 1.0 : -1.0;
+            for (int q = 0; q < 6; q++) {
+                const double v = coef_pairs(prim, gy, rad, npn, P.gama, q, 3, mu, x);
+                if (q < 5) gp[q] = gp[q] + v;
+                else gr = gr + v;
+            }
+        }
+        lsr[d * total + g] = lsr[d * total + g] + gr;
+    }
+#pragma unroll
+    for (int m = 0; m < 5; m++) lp[m * total + g] = lp[m * total + g] + gp[m];
+}
+
+Operands<double> tan_operands(const DFields &F, const AdSweep &A)
+{
+    Operands<double> O;
+    const long T = F.total;
+    for (int m = 0; m < 5; m++) { O.p[m] = A.tp + m * T; O.rhs[m] = A.trhs + m * T; }
+    for (int d = 0; d < 3; d++) {
+        for (int c = 0; c < 3; c++) O.m[d][c] = A.zero;
+        O.sr[d] = A.tsr + d * T;
+        O.srv[d] = A.zero;
+    }
+    O.coed = A.tcoed;
+    return O;
+}
+
+DqPtrs ptrs5(double *base, long total)
+{
+    DqPtrs p;
+    for (int m = 0; m < 5; m++) p.c[m] = base + m * total;
+    return p;
+}
+
+}  // namespace
+
+int launch_ad_sweeps_fwd(const DBlock &b, const DFields &F, const DParams &P, const AdSweep &A, hipStream_t s)
+{
+    const int smin = 3, smax = b.ni + b.nj + b.nk;
+    const long T = F.total;
+    dim3 g = grid1((long)b.nj * b.nk, 256), bl(256);
+    Operands<double> O = g3_operands(F);
+    const bool tan = A.tx != nullptr;
+    Operands<double> Ot = tan ? tan_operands(F, A) : O;
+    DqPtrs X[4], Xt[4];
+    for (int q = 0; q < 4; q++) {
+        X[q] = ptrs5(A.x + 5 * q * T, T);
+        Xt[q] = tan ? ptrs5(A.tx + 5 * q * T, T) : X[q];
+    }
+#define AD_SWEEP(KIND, AA, BB, AAt, BBt)                                                                             \
+    if (tan) hipLaunchKernelGGL((k_ad_sweep<KIND, true>), g, bl, 0, s, b, O, Ot, AA, BB, AAt, BBt, X[KIND], Xt[KIND], \
+                                P, p);                                                                               \
+    else hipLaunchKernelGGL((k_ad_sweep<KIND, false>), g, bl, 0, s, b, O, Ot, AA, BB, AAt, BBt, X[KIND], Xt[KIND], P, p)
+    for (int p = smin; p <= smax; p++) { AD_SWEEP(0, X[0], X[0], Xt[0], Xt[0]); }
+    for (int p = smax; p >= smin; p--) { AD_SWEEP(1, X[0], X[0], Xt[0], Xt[0]); }
+    for (int p = smin; p <= smax; p++) { AD_SWEEP(2, X[1], X[1], Xt[1], Xt[1]); }
+    for (int p = smax; p >= smin; p--) { AD_SWEEP(3, X[2], X[2], Xt[2], Xt[2]); }
+#undef AD_SWEEP
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_ad_sweeps_rev(const DBlock &b, const DFields &F, const DParams &P, const AdSweep &A, hipStream_t s)
+{
+    const int smin = 3, smax = b.ni + b.nj + b.nk;
+    const long T = F.total;
+    const long nn = (long)b.ni * b.nj * b.nk;
+    dim3 g = grid1((long)b.nj * b.nk, 256), gn = grid1(nn, 256), bl(256);
+    Operands<double> O = g3_operands(F);
+    TSweep S;
+    for (int q = 0; q < 4; q++)
+        for (int m = 0; m < 5; m++) {
+            S.x[q][m] = A.x + (5 * q + m) * T;
+            S.mu[q][m] = A.mu + (5 * q + m) * T;
+            S.xb[q][m] = A.xb + (5 * q + m) * T;
+        }
+    for (int m = 0; m < 5; m++) S.rb[m] = A.rb + m * T;
+    S.cb = A.cb;
+    for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_ad_tsweep<3>, g, bl, 0, s, b, O, S, P, p);
+    hipLaunchKernelGGL(k_ad_tgather<3>, gn, bl, 0, s, b, O, S, P);
+    for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_ad_tsweep<2>, g, bl, 0, s, b, O, S, P, p);
+    hipLaunchKernelGGL(k_ad_tgather<2>, gn, bl, 0, s, b, O, S, P);
+    for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_ad_tsweep<1>, g, bl, 0, s, b, O, S, P, p);
+    for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_ad_tsweep<0>, g, bl, 0, s, b, O, S, P, p);
+    hipLaunchKernelGGL(k_ad_tcoef, gn, bl, 0, s, b, O, S, P, A.lp, A.lsr, T);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 }  // namespace wcns

@@ -33,7 +33,8 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_region_images", "wcns_multipoint", "wcns_timing", "wcns_timing_get", "wcns_force",
            "wcns_step_graph", "wcns_autotune", "wcns_adj_residual", "wcns_adj_jvp", "wcns_adj_vjp",
            "wcns_adj_force", "wcns_sweep_layout", "wcns_residual_history",
-           "wcns_residual_noalloc", "wcns_force_record", "wcns_force_history", "wcns_save_flow", "wcns_load_flow"]
+           "wcns_residual_noalloc", "wcns_force_record", "wcns_force_history", "wcns_save_flow", "wcns_load_flow",
+           "wcns_pool_layout", "wcns_adj_step_jvp", "wcns_adj_step_vjp", "wcns_adj_multistep"]
 
 TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
@@ -103,6 +104,10 @@ def load_library(path=LIBPATH):
     L.wcns_adj_jvp.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp]
     L.wcns_adj_vjp.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
     L.wcns_adj_force.argtypes = [vp, C.c_int, vp, vp, dptr, dptr, vp, vp]
+    L.wcns_pool_layout.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+    L.wcns_adj_step_jvp.argtypes = [vp, vp, vp, vp, vp]
+    L.wcns_adj_step_vjp.argtypes = [vp, vp, vp, vp]
+    L.wcns_adj_multistep.argtypes = [vp, vp, C.c_int, C.c_int, dptr, C.c_double, dptr, vp, dptr]
     _lib = L
     return L
 
@@ -329,6 +334,28 @@ class Solver:
         J = np.zeros(1)
         self._check(self.L.wcns_adj_force(self.h, nb, P, X, _d(om), _d(J), gP or None, gX or None), "adj_force")
         return float(J[0])
+
+    # ---- tangent and adjoint of a whole (inviscid) step over the field pool:
+    # 5 arrays of `total` elements, block nb's ghosted nodes from offsets[nb]
+    def pool_layout(self):
+        total = C.c_long()
+        offs = np.zeros(len(self.case.blocks), dtype=np.int64)
+        self._check(self.L.wcns_pool_layout(self.h, C.byref(total), offs.ctypes.data_as(C.POINTER(C.c_long))),
+                    "pool_layout")
+        return total.value, offs
+
+    def adj_step_jvp(self, P, dP, Pn=0, dPn=0):
+        self._check(self.L.wcns_adj_step_jvp(self.h, P or None, dP, Pn or None, dPn or None), "adj_step_jvp")
+
+    def adj_step_vjp(self, P, Wn, gP):
+        self._check(self.L.wcns_adj_step_vjp(self.h, P or None, Wn, gP), "adj_step_vjp")
+
+    def adj_multistep(self, P0, nsteps, every, omega, budget_gb, gP0):
+        om = np.ascontiguousarray(omega, dtype=np.float64)
+        J, used = np.zeros(1), np.zeros(1)
+        self._check(self.L.wcns_adj_multistep(self.h, P0, nsteps, every, _d(om), budget_gb, _d(J), gP0, _d(used)),
+                    "adj_multistep")
+        return float(J[0]), float(used[0])
 
     def multipoint(self):
         ng = C.c_int()
