@@ -192,6 +192,10 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="time the residual, its tangent and adjoint and the force gradient instead "
                          "(128x96x64 wall block, SA with frozen mu_t, M 0.7)")
+    ap.add_argument("--multistep", type=int, default=0, metavar="STEPS",
+                    help="time the reverse-mode wall-force sensitivity through STEPS whole steps of an inviscid "
+                         "workload (default bump-euler), checkpoints every --every steps")
+    ap.add_argument("--every", type=int, default=10, help="checkpoint interval of --multistep")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -213,6 +217,8 @@ def main():
     from wcns_amd.solver import Solver
     if args.adjoint:
         return adjoint_bench(args)
+    if args.multistep:
+        return multistep_bench(args)
     if args.converge:
         return converge_bench(args)
 
@@ -403,6 +409,39 @@ def adjoint_bench(args):
         out[name] = round(1e3 * (time.perf_counter() - t0) / args.steps, 3)
     print(json.dumps({"adjoint_ms_per_call": out, "nodes": case.nodes, "calls": args.steps,
                       "config": "bump-channel 128x96x64 SA (frozen mu_t) Re3e6 M0.7 Roe WCNS-E5, FP64"}))
+    s.close()
+
+
+def multistep_bench(args):
+    """Seconds of wcns_adj_multistep: STEPS forward steps with a checkpoint
+    every --every steps, J = the wall's pressure cfy, and the reverse pass
+    (each segment replayed from its checkpoint, then every step's adjoint),
+    against the same STEPS plain solver steps; with the HBM the checkpoints
+    and work arrays take."""
+    import torch
+    from wcns_amd import adjoint as A
+    from wcns_amd.solver import Solver
+    wl = args.workload if args.workload != "bump" else "bump-euler"
+    case, wl_name = build_case(wl)
+    s = Solver(case)
+    s.step(max(args.warmup, 1), history=False)
+    P0 = A.pool_state(s)
+    om = [0.0, 1.0, 0.0, 0.0, 0.0, 0.0]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(args.multistep, history=False)
+    s.sync()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J, g, used = A.multistep_gradient(s, P0, args.multistep, args.every, om)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    print(json.dumps({"multistep_adjoint": {"workload": wl_name, "nodes": case.nodes, "steps": args.multistep,
+                                            "checkpoint_every": args.every, "J_cfy": J,
+                                            "grad_norm": float(g.norm()), "seconds": round(t_adj, 3),
+                                            "plain_steps_seconds": round(t_fwd, 3),
+                                            "ratio_to_plain_steps": round(t_adj / t_fwd, 2),
+                                            "hbm_gb_checkpoints_and_work": round(used, 3)}}))
     s.close()
 
 

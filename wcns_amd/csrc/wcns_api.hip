@@ -2467,8 +2467,21 @@ static int as_forward(wcns_ctx *ctx, const double *P, const double *dP)
         A.trhs = W + AT_R * T;
         A.zero = S + AS_ZERO * T;
     }
-    for (auto &b : ctx->blocks)
-        if (launch_ad_sweeps_fwd(b.db, F, ctx->dp, A, ctx->stream)) return fail(ctx, WCNS_E_HIP, "sweeps");
+    if (!tan && ctx->tiled_sweeps) {
+        // the solver's pipelined sweeps, each sweep's dQ copied out of the
+        // plane-major pair (lusgs_l/u and gs_pr_u leave it in A, gs_pr_l in B)
+        void *pool = ctx->skew[0];
+        if (launch_skew_stage_all(SKEW_STEP, ctx->grid[GRID_SKEW], ctx->d_skoff, F, 0, pool, ctx->sktot, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "staging the sweep operands");
+        for (int kind = 0; kind < 4; kind++)
+            if (launch_sweep_batched(kind, ctx->plan, F, ctx->dp, 0, pool, ctx->sktot, ctx->stream) ||
+                launch_unskew_f64(ctx->grid[GRID_SKEW], ctx->d_skoff, (const double *)pool, ctx->sktot,
+                                  kind == 2 ? SK_B : SK_A, 5, A.x + 5 * kind * T, T, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "sweeps");
+    } else {
+        for (auto &b : ctx->blocks)
+            if (launch_ad_sweeps_fwd(b.db, F, ctx->dp, A, ctx->stream)) return fail(ctx, WCNS_E_HIP, "sweeps");
+    }
     // the update's dQ: the last sweep's, zero on the physical faces (x3 itself
     // is kept: the adjoint needs the values the sweep used)
     const double *x3 = S + (AS_X + 15) * T;
@@ -2646,24 +2659,29 @@ int wcns_adj_multistep(wcns_ctx *ctx, const double *P0, int nsteps, int every, c
         DFields F = ctx->fields();
         double *cur[5] = {F.r, F.u, F.v, F.w, F.p};
         TRY(as_copy(ctx, cur, P0));
+        // the states by the solver's own step (the derivative's primal equals it bit for bit)
         for (int n = 0; n < nsteps; n++) {
             if (n % every == 0) {
                 F = ctx->fields();
                 double *c5[5] = {F.r, F.u, F.v, F.w, F.p};
                 TRY(as_copy_out(ctx, ck + state * (n / every), c5));
             }
-            TRY(as_forward(ctx, nullptr, nullptr));
+            TRY(step_once(ctx));
         }
+        TRY(check_counters(ctx));
         TRY(as_force(ctx, omega, J, lam));
         // each segment: its states replayed from the checkpoint, then reversed
         for (int c = nck - 1; c >= 0; c--) {
             const int s0 = c * every, s1 = std::min(nsteps, s0 + every);
             CK(hipMemcpyAsync(sg, ck + state * c, 8 * state, hipMemcpyDeviceToDevice, ctx->stream));
+            F = ctx->fields();
+            double *c5[5] = {F.r, F.u, F.v, F.w, F.p};
+            TRY(as_copy(ctx, c5, sg));
             for (int n = s0; n + 1 < s1; n++) {
-                TRY(as_forward(ctx, sg + state * (n - s0), nullptr));
+                TRY(step_once(ctx));
                 F = ctx->fields();
-                double *c5[5] = {F.r, F.u, F.v, F.w, F.p};
-                TRY(as_copy_out(ctx, sg + state * (n + 1 - s0), c5));
+                double *c6[5] = {F.r, F.u, F.v, F.w, F.p};
+                TRY(as_copy_out(ctx, sg + state * (n + 1 - s0), c6));
             }
             for (int n = s1 - 1; n >= s0; n--) TRY(as_reverse(ctx, sg + state * (n - s0), lam, lam));
         }
