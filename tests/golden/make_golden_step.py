@@ -1,0 +1,102 @@
+"""Writes tests/golden/step_golden.npz: whole pseudo-time steps of the
+reference's own serial step routines (oracle/_ref/libref_step.so, built from
+the reference sources by `make -C oracle ref`), so that the oracle's steps are
+pinned by committed vectors where the reference is not built.
+
+Cases (every step: boundary_all, initial_q_dq_c_t, timestep_tgh, r_h_s_vis
+when viscous, r_h_s_invis, l_h_s_tgh, residual_curve; the serial reference has
+no interface averaging, which exists only in its MPI build):
+  cart    the reference's CPU-runnable configuration, 64x32x16, Euler, SW,
+          scheme 41, M 0.8, CFL 5: 100 steps;
+  bump    laminar bump channel 24x40x8, Roe, scheme 41: 5 steps, then the wall
+          force (force + force_dif) after boundary_all;
+  sa      2x2 blocks, SA, no-slip wall, symmetry top, inflow / outflow,
+          interfaces, periodic span: 4 steps and the force (the reference
+          is given the oracle's wall distance: its own routine is part of the
+          MPI build);
+  repair  the Cartesian case with two nodes above rmax_limit and two
+          near-vacuum pressures (ordered repair, WCNS positivity fallback):
+          2 steps.
+Stored per case: the residual monitor of every step (rms, max, location) and
+the SHA-256 of the final r, u, v, w, p over the interior nodes of every block
+(little-endian float64, block by block, k, j, i order), plus the force.
+
+Run from the repository root: python tests/golden/make_golden_step.py
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", ".."))
+
+from wcns_amd import cases  # noqa: E402
+
+STAGES_INV = ["boundary_all", "initial_q_dq_c_t", "timestep", "rhs_invis", "lhs"]
+STAGES_VIS = ["boundary_all", "initial_q_dq_c_t", "timestep", "rhs_vis", "rhs_invis", "lhs"]
+
+
+def golden_cases():
+    out = {}
+    out["cart"] = (cases.cartesian_periodic_case(ni=64, nj=32, nk=16, seed=1), 100, False)
+    out["bump"] = (cases.bump_channel_case(ni=24, nj=40, nk=8, dy_wall=1e-3, growth=1.15), 5, True)
+    sa = cases.multiblock_case(nvis=1, wall=True, periodic_i=False, ilo_bc=5, ihi_bc=6, jhi_bc=3, reynolds=1e5)
+    sa.params.nlamtur = 1
+    sa.params.nwholefield = 0
+    out["sa"] = (sa, 4, True)
+    out["repair"] = (cases.cartesian_periodic_case(ni=24, nj=16, nk=12, seed=3), 2, False)
+    return out
+
+
+def poke_repair(target):
+    """The forced branches of the repair case, on any object with field/set_field."""
+    for name, nodes, value in (("r", [(6, 6, 6), (15, 9, 4)], 2.0e3), ("p", [(10, 4, 8), (11, 4, 8)], 1.0e-7)):
+        a = target.field(0, name)
+        for i, j, k in nodes:
+            a[k + 2, j + 2, i + 2] = value
+        target.set_field(0, name, a)
+
+
+def state_hash(case, target):
+    h = hashlib.sha256()
+    for nb in range(len(case.blocks)):
+        for n in ("r", "u", "v", "w", "p"):
+            a = np.ascontiguousarray(target.field(nb, n)[3:-3, 3:-3, 3:-3], dtype="<f8")
+            h.update(a.tobytes())
+    return np.frombuffer(h.digest(), dtype=np.uint8).copy()
+
+
+def main():
+    from oracle import oracle as O
+    from oracle import ref_step as R
+    if not R.available():
+        raise SystemExit("oracle/_ref/libref_step.so is not built (make -C oracle ref)")
+    data = {}
+    for tag, (case, nsteps, vis) in golden_cases().items():
+        dist = None
+        if case.params.nlamtur > 0:
+            o = O.OracleCase(case)
+            dist = [o.field(nb, "dist") for nb in range(len(case.blocks))]
+            o.close()
+        r = R.RefStep(case, dist=dist)
+        if tag == "repair":
+            poke_repair(r)
+        hist = []
+        for _ in range(nsteps):
+            for st in (STAGES_VIS if vis else STAGES_INV):
+                r.stage(st)
+            rms, rmax, loc = r.residual()
+            hist.append([rms, rmax] + [float(x) for x in loc])
+        data[f"{tag}_hist"] = np.array(hist)
+        data[f"{tag}_hash"] = state_hash(case, r)
+        if vis:
+            r.stage("boundary_all")
+            data[f"{tag}_force"] = r.force()
+        print(tag, nsteps, "steps, last rms", hist[-1][0], flush=True)
+    np.savez_compressed(os.path.join(HERE, "step_golden.npz"), **data)
+
+
+if __name__ == "__main__":
+    main()

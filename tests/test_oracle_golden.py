@@ -288,3 +288,36 @@ def test_wall_turbulence_values_against_reference(nlamtur):
     if nlamtur == 2:
         np.testing.assert_array_equal(oc.field(0, "qkw")[sl], wo[sl])
         assert np.any(wo[sl][:, 0, :] != 0.0)
+
+
+# ------------------------------------------------ whole steps of the reference
+STEP_GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "step_golden.npz")
+
+
+@pytest.mark.parametrize("tag", ["cart", "bump", "sa", "repair"])
+def test_whole_steps_against_reference_fixtures(tag):
+    """The oracle's steps against the reference's own serial step routines
+    (tests/golden/make_golden_step.py): every residual-monitor entry (rms, max
+    and its location), the final interior state (SHA-256 of its bytes) and the
+    wall force, bit for bit; the same stage sequence, without the interface
+    averaging that only the reference's MPI build has."""
+    import hashlib
+    from tests.golden.make_golden_step import golden_cases, poke_repair, state_hash, STAGES_INV, STAGES_VIS
+    g = np.load(STEP_GOLDEN)
+    case, nsteps, vis = golden_cases()[tag]
+    o = O.OracleCase(case)
+    o.set_threads(4)
+    if tag == "repair":
+        poke_repair(o)
+    hist = []
+    for _ in range(nsteps):
+        for st in (STAGES_VIS if vis else STAGES_INV):
+            o.stage(st)
+        rms, rmax, loc = o.residual()
+        hist.append([rms, rmax] + [float(x) for x in loc])
+    assert np.array_equal(np.array(hist), g[f"{tag}_hist"])
+    assert np.array_equal(state_hash(case, o), g[f"{tag}_hash"])
+    if vis:
+        o.stage("boundary_all")
+        assert np.array_equal(o.force(), g[f"{tag}_force"])
+    o.close()
