@@ -269,29 +269,35 @@ int wcns_adj_force(wcns_ctx *ctx, int nb, const double *P, const double *X, cons
                    double *J, double *gP, double *gX);
 
 /* ---- tangent and adjoint of a whole pseudo-time step (no counterpart in the reference) ----
- * G(P) is one step of Main.f90:85-109 for an inviscid case (nvis = 0, FP64
- * implicit operator): boundary_all, initial_q_dq_c_t, timestep_tgh,
- * r_h_s_invis, communicate_dq_npp, l_h_s_tgh (GS_PR_Single: lusgs_l/u and
- * gs_pr_l/u, LHS.f90:381-577,1777-2046; update_limited, Sub2.f90:603-796) and
- * communicate_pv_npp, as wcns_step computes it.  The state is the primitives
- * r, u, v, w, p at the interior nodes of every block: 5 arrays of the field
- * pool (wcns_pool_layout: total elements each, block nb's (ni+6)(nj+6)(nk+6)
- * nodes from its offset on, i fastest), DEVICE memory; ghost values of P and
- * dP are not read, and those of Pn, dPn and gP are zero.  Every derivative is
- * of the arithmetic as computed, boundary conditions and interface
- * averagings included; a step in which a node is repaired is rejected
+ * G is one step of Main.f90:85-109 (FP64 implicit operator) as wcns_step
+ * computes it: boundary_all, initial_q_dq_c_t, timestep_tgh, r_h_s_vis,
+ * communicate_dq_npp, r_h_s_invis, communicate_dq_npp, l_h_s_tgh
+ * (GS_PR_Single: lusgs_l/u and gs_pr_l/u, LHS.f90:381-577,1777-2046;
+ * update_limited, Sub2.f90:603-796) and communicate_pv_npp.
+ * The state is nstate arrays of the field pool (wcns_pool_layout: total
+ * elements each, block nb's (ni+6)(nj+6)(nk+6) nodes from offsets[nb], i
+ * fastest), DEVICE memory: r, u, v, w, p, then with nvis = 1 T (which the
+ * no-slip wall reads one step late), then with a turbulence model nu~ or k,
+ * mu_t, and for SST omega.  The derivatives are taken w.r.t. the first nvar
+ * of them (r..p, and T when viscous) at the interior nodes: the turbulence
+ * model's variables are carried along but held (frozen mu_t), ghost values
+ * of P and dP are not read, and those of dPn and gP are zero; dP, dPn, Wn
+ * and gP have nvar arrays, P and Pn nstate.  Every derivative is of the
+ * arithmetic as computed, boundary conditions and interface averagings
+ * included; a step in which a node is repaired is rejected
  * (WCNS_E_UNSUPPORTED).  Each call leaves the state at G(P). */
-int wcns_pool_layout(wcns_ctx *ctx, long *total, long *offsets);
+int wcns_pool_layout(wcns_ctx *ctx, long *total, long *offsets, int *nstate, int *nvar);
 /* Pn = G(P), dPn = G'(P) dP (P NULL: the current state) */
 int wcns_adj_step_jvp(wcns_ctx *ctx, const double *P, const double *dP, double *Pn, double *dPn);
 /* gP = G'(P)^T Wn */
 int wcns_adj_step_vjp(wcns_ctx *ctx, const double *P, const double *Wn, double *gP);
 /* Reverse-mode sensitivity of J = sum_c omega[c] C[c] (the pressure part of
- * wcns_force's cfx..cmz over the no-slip walls) after nsteps steps from P0,
- * with respect to P0: checkpoints every `every` steps, each segment replayed
- * from its checkpoint and reversed.  Rejected (WCNS_E_ARG) when checkpoints
- * and work arrays need more than budget_gb (budget_gb <= 0: no limit);
- * used_gb (may be NULL) receives what they need. */
+ * wcns_force's cfx..cmz over the no-slip walls) after nsteps steps from P0
+ * (nstate arrays), with respect to P0's nvar arrays: checkpoints of the
+ * whole state every `every` steps, each segment replayed from its
+ * checkpoint and reversed.  Rejected (WCNS_E_ARG) when checkpoints and work
+ * arrays need more than budget_gb (budget_gb <= 0: no limit); used_gb (may
+ * be NULL) receives what they need. */
 int wcns_adj_multistep(wcns_ctx *ctx, const double *P0, int nsteps, int every, const double omega[6],
                        double budget_gb, double *J, double *gP0, double *used_gb);
 

@@ -134,41 +134,61 @@ class WallForce(torch.autograd.Function):
         return None, None, gJ * gP, gJ * gX, None
 
 
-# ------------------------------------------------ a whole step (inviscid cases)
-# The state is the primitives r, u, v, w, p of every block in the solver's
-# field pool: a (5, total) float64 tensor, block nb's (nk+6, nj+6, ni+6)
-# ghosted nodes (i fastest) from offset[nb] on.  Only the interior nodes are
-# the state: ghost values of inputs are not read, those of outputs are zero.
+# ------------------------------------------------ a whole step
+# The state is the field pool's arrays of every block: (nstate, total)
+# float64 tensors, block nb's (nk+6, nj+6, ni+6) ghosted nodes (i fastest)
+# from offset[nb] on.  r, u, v, w, p, then for a viscous case T, then with a
+# turbulence model nu~ or k, mu_t (and SST omega); the first nvar
+# (Solver.state_layout) are differentiated, at the interior nodes: directions
+# and weights are (nvar, total), ghost values of inputs are not read and
+# those of outputs are zero.
+
+STATE_FIELDS = ("r", "u", "v", "w", "p", "t", "qke", "vist", "qkw")
+
+
+def _state_names(solver):
+    prm = solver.case.params
+    names = ["r", "u", "v", "w", "p"]
+    if prm.nvis == 1:
+        names.append("t")
+    if prm.nlamtur in (1, 2):
+        names += ["qke", "vist"]
+    if prm.nlamtur == 2:
+        names.append("qkw")
+    return names
+
 
 def pool_state(solver, device="cuda"):
-    """The solver's current primitives as a pool-layout (5, total) tensor."""
+    """The solver's current state as a pool-layout (nstate, total) tensor."""
     total, offs = solver.pool_layout()
-    P = torch.zeros((5, total), dtype=torch.float64)
+    names = _state_names(solver)
+    P = torch.zeros((len(names), total), dtype=torch.float64)
     for nb, blk in enumerate(solver.case.blocks):
         n = (blk.ni + 6) * (blk.nj + 6) * (blk.nk + 6)
-        for m, f in enumerate(("r", "u", "v", "w", "p")):
+        for m, f in enumerate(names):
             P[m, offs[nb]:offs[nb] + n] = torch.from_numpy(solver.field(nb, f).ravel())
     return P.to(device).contiguous()
 
 
 def pool_block(solver, A, nb):
-    """Block nb's part of a pool-layout tensor as (5, nk+6, nj+6, ni+6)."""
+    """Block nb's part of a pool-layout tensor as (rows, nk+6, nj+6, ni+6)."""
     total, offs = solver.pool_layout()
     blk = solver.case.blocks[nb]
     n = (blk.ni + 6) * (blk.nj + 6) * (blk.nk + 6)
-    return A[:, offs[nb]:offs[nb] + n].reshape(5, blk.nk + 6, blk.nj + 6, blk.ni + 6)
+    return A[:, offs[nb]:offs[nb] + n].reshape(A.shape[0], blk.nk + 6, blk.nj + 6, blk.ni + 6)
 
 
-def _check_pool(name, t, solver):
+def _check_pool(name, t, solver, rows):
     total, _ = solver.pool_layout()
-    _check_tensor(name, t, (5, total), solver)
+    _check_tensor(name, t, (rows, total), solver)
 
 
 def step_jvp(solver: Solver, P, dP):
     """(G(P), G'(P) dP) for one pseudo-time step G (wcns_adj_step_jvp)."""
-    _check_pool("P", P, solver)
-    _check_pool("dP", dP, solver)
-    Pn, dPn = torch.empty_like(P), torch.empty_like(P)
+    ns, nv = solver.state_layout()
+    _check_pool("P", P, solver, ns)
+    _check_pool("dP", dP, solver, nv)
+    Pn, dPn = torch.empty_like(P), torch.empty_like(dP)
     torch.cuda.synchronize(P.device)
     solver.adj_step_jvp(P.data_ptr(), dP.data_ptr(), Pn.data_ptr(), dPn.data_ptr())
     return Pn, dPn
@@ -176,9 +196,10 @@ def step_jvp(solver: Solver, P, dP):
 
 def step_vjp(solver: Solver, P, Wn):
     """G'(P)^T Wn (wcns_adj_step_vjp)."""
-    _check_pool("P", P, solver)
-    _check_pool("Wn", Wn, solver)
-    gP = torch.empty_like(P)
+    ns, nv = solver.state_layout()
+    _check_pool("P", P, solver, ns)
+    _check_pool("Wn", Wn, solver, nv)
+    gP = torch.empty_like(Wn)
     torch.cuda.synchronize(P.device)
     solver.adj_step_vjp(P.data_ptr(), Wn.data_ptr(), gP.data_ptr())
     return gP
@@ -188,8 +209,9 @@ def multistep_gradient(solver: Solver, P0, nsteps, every, omega, budget_gb=0.0):
     """J = omega . (pressure force coefficients) after nsteps steps from P0 and
     dJ/dP0, with checkpoints every `every` steps (wcns_adj_multistep); returns
     (J, gP0, GB of checkpoints and work arrays)."""
-    _check_pool("P0", P0, solver)
-    g = torch.empty_like(P0)
+    ns, nv = solver.state_layout()
+    _check_pool("P0", P0, solver, ns)
+    g = torch.empty((nv, P0.shape[1]), dtype=torch.float64, device=P0.device)
     torch.cuda.synchronize(P0.device)
     J, used = solver.adj_multistep(P0.data_ptr(), nsteps, every, omega, budget_gb, g.data_ptr())
     return J, g, used
@@ -198,7 +220,8 @@ def multistep_gradient(solver: Solver, P0, nsteps, every, omega, budget_gb=0.0):
 class _Step(torch.autograd.Function):
     @staticmethod
     def forward(ctx, P, solver):
-        Pn, _ = step_jvp(solver, P, torch.zeros_like(P))
+        ns, nv = solver.state_layout()
+        Pn, _ = step_jvp(solver, P, torch.zeros((nv, P.shape[1]), dtype=P.dtype, device=P.device))
         ctx.save_for_backward(P)
         ctx.solver = solver
         return Pn
@@ -206,9 +229,13 @@ class _Step(torch.autograd.Function):
     @staticmethod
     def backward(ctx, Wn):
         (P,) = ctx.saved_tensors
-        return step_vjp(ctx.solver, P, Wn.contiguous()), None
+        ns, nv = ctx.solver.state_layout()
+        g = torch.zeros_like(P)
+        g[:nv] = step_vjp(ctx.solver, P, Wn[:nv].contiguous())
+        return g, None
 
 
 def step(solver: Solver, P):
-    """One pseudo-time step as a differentiable function of the state."""
+    """One pseudo-time step as a differentiable function of the state (the
+    turbulence model's rows carried along with zero derivative)."""
     return _Step.apply(P, solver)

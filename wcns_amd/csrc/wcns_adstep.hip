@@ -1,7 +1,7 @@
-// wcns_adstep.hip -- tangent and adjoint of the parts of an inviscid
-// pseudo-time step around the residual and the sweeps: the boundary
-// conditions (BC.f90 as in bc_point), the spectral radii and LU-SGS diagonal
-// (Convect.f90:1338-1461, LHS.f90:602-645), the limited update
+// wcns_adstep.hip -- tangent and adjoint of the parts of a pseudo-time step
+// around the residual and the sweeps: the boundary conditions (BC.f90 as in
+// bc_point), the temperature, the inviscid and viscous spectral radii and the
+// LU-SGS diagonal (Convect.f90:1338-1461, LHS.f90:141-343,602-645), the limited update
 // (Sub2.f90:603-796) and the transposed interface averagings
 // (parallel.f90:3020-3326).  Each pointwise piece is written once, generic in
 // the scalar type: double reproduces the solver's arithmetic, D1 gives its
@@ -37,7 +37,9 @@ __device__ inline void bc_nodes(const DRegion &g, const DBlock &b, const DBlock 
     }
 }
 
-// outputs a point writes (bit q: o[q]) and how many of its inputs matter
+// outputs a point writes (bit q: o[q]), how many of its input nodes matter,
+// and how many variables of each: the no-slip wall also reads T (the value
+// the previous step left), every other BC only r, u, v, w, p
 __device__ inline int bc_mask(int bctype) { return bctype < 0 ? 0xE : 0xF; }
 __device__ inline int bc_nin(int bctype)
 {
@@ -45,11 +47,13 @@ __device__ inline int bc_nin(int bctype)
     if (bctype == 4 || bctype == 6) return 1;
     return 3;
 }
+__device__ inline int bc_nvar(int bctype, const DParams &P) { return (bctype == 2 && P.nvis == 1) ? 6 : 5; }
 
-// The values one point writes from the values it reads (inviscid walls only).
-// nrm: the face's unit normal (getnxyz) for BC 4 and BC 2.
+// The values one point writes (r, u, v, w, p) from the values it reads
+// (r, u, v, w, p, T of each input node).  nrm: the face's unit normal
+// (getnxyz) for BC 4 and BC 2.
 template <typename T>
-__device__ inline void bc_eval(const DRegion &g, const DParams &P, const double *nrm, const T (&in)[3][5],
+__device__ inline void bc_eval(const DRegion &g, const DParams &P, const double *nrm, const T (&in)[3][6],
                                T (&o)[4][5])
 {
     const double gama = P.gama;
@@ -115,9 +119,35 @@ __device__ inline void bc_eval(const DRegion &g, const DParams &P, const double 
 #pragma unroll
             for (int m = 0; m < 5; m++) o[q][m] = in[0][m];
         return;
-    case 2:     // BC.f90:1412-1486 inviscid_wall + dif_average
+    case 2:
+        if (P.nvis == 1) {   // BC.f90:1244-1409 viscid_wall (adiabatic, or fixed wall temperature)
+            const double moocp = P.moo * P.moo * gama;
+            T pres, tres;
+            if (P.twall_fixed) {
+                pres = (4.0 * in[0][4] - in[1][4]) / 3.0;
+                if (pres < PMIN_LIMIT) pres = in[0][4];
+                tres = T(P.twall_nd);
+            } else {
+                pres = (4.0 * in[0][4] - in[1][4]) / 3.0;
+                tres = (4.0 * in[0][5] - in[1][5]) / 3.0;
+                if (pres < PMIN_LIMIT) pres = in[0][4];
+                if (tres < TMIN_LIMIT) tres = in[0][5];
+            }
+            T dres = moocp * pres / tres;
+            o[0][0] = dres; o[0][1] = T(0.0); o[0][2] = T(0.0); o[0][3] = T(0.0); o[0][4] = pres;
+#pragma unroll
+            for (int l = 0; l < 3; l++) {
+                o[l + 1][0] = l == 2 ? -in[l][0] : in[l][0];
+                o[l + 1][1] = -in[l][1];
+                o[l + 1][2] = -in[l][2];
+                o[l + 1][3] = -in[l][3];
+                o[l + 1][4] = in[l][4];
+            }
+            return;
+        }
+        // fall through: BC.f90:1412-1486 inviscid_wall + dif_average
     case 3: {   // BC.f90:227-457 boundary_3
-        const bool wall = g.bctype == 2;
+        const bool wall = g.bctype == 2;   // (nvis = 0 here)
         const double nx = wall ? nrm[0] : g.nrm[0], ny = wall ? nrm[1] : g.nrm[1], nz = wall ? nrm[2] : g.nrm[2];
 #pragma unroll
         for (int l = 0; l < 3; l++) {
@@ -151,15 +181,15 @@ __device__ void bc_fwd_point(const DRegion &g, const DBlock &b, const DBlock &bt
     region_point(g, t, i, j, k);
     long o[4], in[3];
     bc_nodes(g, b, bt, i, j, k, o, in);
-    double *pc[5] = {F.r, F.u, F.v, F.w, F.p}, *po[5] = {F.rn, F.un, F.vn, F.wn, F.pn};
+    double *pc[6] = {F.r, F.u, F.v, F.w, F.p, F.t}, *po[5] = {F.rn, F.un, F.vn, F.wn, F.pn};
     const long T5 = F.total;
-    D1 x[3][5], y[4][5];
+    D1 x[3][6], y[4][5];
 #pragma unroll
     for (int l = 0; l < 3; l++)
 #pragma unroll
-        for (int m = 0; m < 5; m++) {
+        for (int m = 0; m < 6; m++) {
             const double v = pc[m][in[l]];
-            tape[((long)t * 3 + l) * 5 + m] = v;
+            tape[((long)t * 3 + l) * 6 + m] = v;
             x[l][m] = D1(v, tan ? tan[m * T5 + in[l]] : 0.0);
         }
     double nrm[3];
@@ -198,21 +228,21 @@ __device__ void bc_rev_point(const DRegion &g, const DBlock &b, const DBlock &bt
                 lam[m * T5 + o[q]] = 0.0;
             }
         }
-    double xv[3][5];
+    double xv[3][6];
 #pragma unroll
     for (int l = 0; l < 3; l++)
 #pragma unroll
-        for (int m = 0; m < 5; m++) xv[l][m] = tape[((long)t * 3 + l) * 5 + m];
+        for (int m = 0; m < 6; m++) xv[l][m] = tape[((long)t * 3 + l) * 6 + m];
     double nrm[3];
     bc_normal(g, F, o[0], nrm);
-    const int nin = bc_nin(g.bctype);
+    const int nin = bc_nin(g.bctype), nvar = bc_nvar(g.bctype, P);
     for (int l = 0; l < nin; l++)
-        for (int m = 0; m < 5; m++) {
-            D1 x[3][5], y[4][5];
+        for (int m = 0; m < nvar; m++) {
+            D1 x[3][6], y[4][5];
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
-                for (int c = 0; c < 5; c++) x[a][c] = D1(xv[a][c], (a == l && c == m) ? 1.0 : 0.0);
+                for (int c = 0; c < 6; c++) x[a][c] = D1(xv[a][c], (a == l && c == m) ? 1.0 : 0.0);
             bc_eval<D1>(g, P, nrm, x, y);
             double s = 0.0;
 #pragma unroll
@@ -250,22 +280,48 @@ __global__ void k_bc_rev(DRegion g, DBlock b, DBlock bt, DFields F, DParams P, d
 }
 
 // ------------------------------------------------ radii, dt and coed
+// The node's T and mu (initial_q_dq_c_t: get_c_t_only RHS.f90:307-409,
+// compute_visl_ns GasModel.f90:219-235)
 template <typename T>
-__device__ inline void ts_node(const DParams &P, const T *pr, const DFields &F, long g, T &sra, T &srb, T &src,
-                               T &coed)
+__device__ inline void t_mu(const DParams &P, const T *pr, T &tm, T &mu)
+{
+    double moo2 = P.moo * P.moo;
+    T a2 = P.gama * pr[4] / pr[0];
+    tm = moo2 * a2;
+    mu = tm * sqrt(tm) * (1.0 + P.visc) / (tm + P.visc);
+}
+
+// spectinv + spectvisl/vist + localdt0 + getcoed of one node: out = sra,
+// srb, src, srva, srvb, srvc, coed.  The viscous radii (LHS.f90:141-343)
+// follow the primitives through mu(T(p, r)) when csrv > 0 (mu_t held);
+// otherwise they are the stored values.
+template <typename T>
+__device__ inline void tsv_node(const DParams &P, const T *pr, const DFields &F, long g, T (&out)[7])
 {
     T um = pr[1], vm = pr[2], wm = pr[3];
     T cm = sqrt(P.gama * pr[4] / pr[0]);
     double kx = F.kcx[g], ky = F.kcy[g], kz = F.kcz[g];
     double ex = F.etx[g], ey = F.ety[g], ez = F.etz[g];
     double cx = F.ctx[g], cy = F.cty[g], cz = F.ctz[g];
+    T sv[3] = {T(F.srva[g]), T(F.srvb[g]), T(F.srvc[g])};
+    if (P.nvis == 1 && P.csrv > 0.0) {
+        T tm, mu;
+        t_mu<T>(P, pr, tm, mu);
+        T vis = P.turb ? mu + F.vist[g] : mu;
+        T rm_vol = pr[0] * F.vol[g];
+        T coef = 2.0 * vis / (P.reynolds * rm_vol + SMALL);
+        T coef1 = P.csrv * coef;
+        sv[0] = coef1 * (kx * kx + ky * ky + kz * kz);
+        sv[1] = coef1 * (ex * ex + ey * ey + ez * ez);
+        sv[2] = coef1 * (cx * cx + cy * cy + cz * cz);
+    }
     T cita = kx * um + ky * vm + kz * wm + 0.0;
     T citb = ex * um + ey * vm + ez * wm + 0.0;
     T citc = cx * um + cy * vm + cz * wm + 0.0;
-    sra = fabs(cita) + cm * ::sqrt(kx * kx + ky * ky + kz * kz);
-    srb = fabs(citb) + cm * ::sqrt(ex * ex + ey * ey + ez * ez);
-    src = fabs(citc) + cm * ::sqrt(cx * cx + cy * cy + cz * cz);
-    double rv = F.srva[g] + F.srvb[g] + F.srvc[g];
+    T sra = fabs(cita) + cm * ::sqrt(kx * kx + ky * ky + kz * kz);
+    T srb = fabs(citb) + cm * ::sqrt(ex * ex + ey * ey + ez * ez);
+    T src = fabs(citc) + cm * ::sqrt(cx * cx + cy * cy + cz * cz);
+    T rv = sv[0] + sv[1] + sv[2];
     T rabc = sra + srb + src + rv;
     T dt;
     if (P.timedt_rate > SMALL) {
@@ -276,9 +332,11 @@ __device__ inline void ts_node(const DParams &P, const T *pr, const DFields &F, 
         dt = P.cfl / rabc;
     }
     T rad_ns = sra + srb + src;
-    if (P.nvis >= 1) rad_ns = rad_ns + F.srva[g] + F.srvb[g] + F.srvc[g];
+    if (P.nvis >= 1) rad_ns = rad_ns + sv[0] + sv[1] + sv[2];
     T ct = 1.0 / dt;
-    coed = 1.0 / (ct + 1.0 * 1.0 * rad_ns);
+    out[0] = sra; out[1] = srb; out[2] = src;
+    out[3] = sv[0]; out[4] = sv[1]; out[5] = sv[2];
+    out[6] = 1.0 / (ct + 1.0 * 1.0 * rad_ns);
 }
 
 __device__ inline bool node_of(const BlockGrid &G, DBlock &b, long &g)
@@ -291,6 +349,7 @@ __device__ inline bool node_of(const BlockGrid &G, DBlock &b, long &g)
     return true;
 }
 
+// tangent ts (7 arrays, the order of tsv_node's outputs) from tp
 __global__ void k_ts_jvp(BlockGrid G, DFields F, DParams P, const double *tp, double *ts)
 {
     DBlock b;
@@ -298,33 +357,72 @@ __global__ void k_ts_jvp(BlockGrid G, DFields F, DParams P, const double *tp, do
     if (!node_of(G, b, g)) return;
     const long T5 = F.total;
     const double *pc[5] = {F.r, F.u, F.v, F.w, F.p};
-    D1 pr[5];
+    D1 pr[5], out[7];
 #pragma unroll
     for (int m = 0; m < 5; m++) pr[m] = D1(pc[m][g], tp[m * T5 + g]);
-    D1 a, bb, c, d;
-    ts_node<D1>(P, pr, F, g, a, bb, c, d);
-    ts[g] = a.d;
-    ts[T5 + g] = bb.d;
-    ts[2 * T5 + g] = c.d;
-    ts[3 * T5 + g] = d.d;
+    tsv_node<D1>(P, pr, F, g, out);
+#pragma unroll
+    for (int q = 0; q < 7; q++) ts[q * T5 + g] = out[q].d;
 }
 
-__global__ void k_ts_vjp(BlockGrid G, DFields F, DParams P, const double *ls, const double *lc, double *lp)
+// lp += from the weights of the radii (ls, 3), viscous radii (lv, 3) and coed (lc)
+__global__ void k_ts_vjp(BlockGrid G, DFields F, DParams P, const double *ls, const double *lv, const double *lc,
+                         double *lp)
 {
     DBlock b;
     long g;
     if (!node_of(G, b, g)) return;
     const long T5 = F.total;
-    const double w0 = ls[g], w1 = ls[T5 + g], w2 = ls[2 * T5 + g], w3 = lc[g];
-    if (w0 == 0.0 && w1 == 0.0 && w2 == 0.0 && w3 == 0.0) return;
+    double w[7];
+#pragma unroll
+    for (int q = 0; q < 3; q++) { w[q] = ls[q * T5 + g]; w[3 + q] = lv[q * T5 + g]; }
+    w[6] = lc[g];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < 7; q++) any = any || w[q] != 0.0;
+    if (!any) return;
     const double *pc[5] = {F.r, F.u, F.v, F.w, F.p};
     for (int q = 0; q < 5; q++) {
-        D1 pr[5];
+        D1 pr[5], out[7];
 #pragma unroll
         for (int m = 0; m < 5; m++) pr[m] = D1(pc[m][g], m == q ? 1.0 : 0.0);
-        D1 a, bb, c, d;
-        ts_node<D1>(P, pr, F, g, a, bb, c, d);
-        lp[q * T5 + g] = lp[q * T5 + g] + (w0 * a.d + w1 * bb.d + w2 * c.d + w3 * d.d);
+        tsv_node<D1>(P, pr, F, g, out);
+        double s = 0.0;
+#pragma unroll
+        for (int o = 0; o < 7; o++) s = s + w[o] * out[o].d;
+        lp[q * T5 + g] = lp[q * T5 + g] + s;
+    }
+}
+
+// the state's temperature after the step: T(p, r) of initial_q_dq_c_t at the
+// interior nodes (before the primitive averaging)
+__global__ void k_tout_jvp(BlockGrid G, DFields F, DParams P, const double *pf, const double *tp, double *tt)
+{
+    DBlock b;
+    long g;
+    if (!node_of(G, b, g)) return;
+    const long T5 = F.total;
+    D1 pr[5], tm, mu;
+#pragma unroll
+    for (int m = 0; m < 5; m++) pr[m] = D1(pf[m * T5 + g], tp[m * T5 + g]);
+    t_mu<D1>(P, pr, tm, mu);
+    tt[g] = tm.d;
+}
+
+__global__ void k_tout_vjp(BlockGrid G, DFields F, DParams P, const double *pf, const double *lt, double *lp)
+{
+    DBlock b;
+    long g;
+    if (!node_of(G, b, g)) return;
+    const long T5 = F.total;
+    const double w = lt[g];
+    if (w == 0.0) return;
+    for (int q = 0; q < 5; q++) {
+        D1 pr[5], tm, mu;
+#pragma unroll
+        for (int m = 0; m < 5; m++) pr[m] = D1(pf[m * T5 + g], m == q ? 1.0 : 0.0);
+        t_mu<D1>(P, pr, tm, mu);
+        lp[q * T5 + g] = lp[q * T5 + g] + w * tm.d;
     }
 }
 
@@ -402,61 +500,61 @@ __global__ void k_upd_vjp(BlockGrid G, DParams P, const double *pf, const double
 // forward: value(qs) = 0.5 (value(qt) vp + value(qs)), vp = vol(qs)/vol(qt)
 // for dQ, 1 for the primitives
 __device__ inline void avg2pm_t_point(const DRegion &g, const DBlock &b, const DBlock &bt, const double *vol,
-                                      double *lam, long total, int use_vol, int t)
+                                      double *lam, long total, int use_vol, int nv, int t)
 {
     int i, j, k, it, jt, kt;
     region_point(g, t, i, j, k);
     region_image(g, i, j, k, it, jt, kt);
     const long qs = gidx(b, i, j, k), qt = gidx(bt, it, jt, kt);
     const double vp = use_vol ? vol[qs] / vol[qt] : 1.0;
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
+    for (int m = 0; m < nv; m++) {
         const double l = lam[m * total + qs];
         lam[m * total + qt] = lam[m * total + qt] + 0.5 * vp * l;
         lam[m * total + qs] = 0.5 * l;
     }
 }
 
-__global__ void k_avg2pm_t(DRegion g, DBlock b, DBlock bt, const double *vol, double *lam, long total, int use_vol)
+__global__ void k_avg2pm_t(DRegion g, DBlock b, DBlock bt, const double *vol, double *lam, long total, int use_vol,
+                           int nv)
 {
     if (g.serial) {
         if (blockIdx.x != 0 || threadIdx.x != 0) return;
-        for (int t = g.npts - 1; t >= 0; t--) avg2pm_t_point(g, b, bt, vol, lam, total, use_vol, t);
+        for (int t = g.npts - 1; t >= 0; t--) avg2pm_t_point(g, b, bt, vol, lam, total, use_vol, nv, t);
         return;
     }
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.npts) return;
-    avg2pm_t_point(g, b, bt, vol, lam, total, use_vol, t);
+    avg2pm_t_point(g, b, bt, vol, lam, total, use_vol, nv, t);
 }
 
 // forward: every written member of a group takes mean(buf) (* vol for dQ)
 __global__ void k_npp_assign_t(const int *grp_start, const int *grp_n, const int *pts, const int *write_ok,
                                int ngroups, const DBlock *blocks, const double *vol, double *lam, long total,
-                               double *bufb, int use_vol)
+                               double *bufb, int use_vol, int nv)
 {
     int gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= ngroups) return;
     const int s0 = grp_start[gi], n = grp_n[gi];
-    double S[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double S[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int s = 0; s < n; s++) {
         if (!write_ok[s0 + s]) continue;
         const int *p = pts + 4 * (s0 + s);
         const long q = gidx(blocks[p[0]], p[1], p[2], p[3]);
         const double f = use_vol ? vol[q] : 1.0;
-        for (int m = 0; m < 5; m++) {
+        for (int m = 0; m < nv; m++) {
             S[m] = S[m] + lam[m * total + q] * f;
             lam[m * total + q] = 0.0;
         }
     }
     for (int s = 0; s < n; s++)
-        for (int m = 0; m < 5; m++) bufb[5 * (s0 + s) + m] = S[m] / (n * 1.0);
+        for (int m = 0; m < nv; m++) bufb[6 * (s0 + s) + m] = S[m] / (n * 1.0);
 }
 
 // forward: buf(s) = value(q_s) (/ vol for dQ), before the pairwise averages.
 // A node can be a member of several groups, so the entries are taken per
 // node: node u owns entries ent[ustart[u] .. ustart[u+1]-1], summed in order.
 __global__ void k_npp_gather_t(const int *pts, const int *ustart, const int *ent, int nuniq, const DBlock *blocks,
-                               const double *vol, double *lam, long total, const double *bufb, int use_vol)
+                               const double *vol, double *lam, long total, const double *bufb, int use_vol, int nv)
 {
     int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nuniq) return;
@@ -464,14 +562,14 @@ __global__ void k_npp_gather_t(const int *pts, const int *ustart, const int *ent
     const int *p = pts + 4 * ent[e0];
     const long q = gidx(blocks[p[0]], p[1], p[2], p[3]);
     const double f = use_vol ? vol[q] : 1.0;
-    for (int m = 0; m < 5; m++) {
+    for (int m = 0; m < nv; m++) {
         double a = 0.0;
-        for (int e = e0; e < e1; e++) a = a + bufb[5 * ent[e] + m] / f;
+        for (int e = e0; e < e1; e++) a = a + bufb[6 * ent[e] + m] / f;
         lam[m * total + q] = lam[m * total + q] + a;
     }
 }
 
-__global__ void k_zero_ghosts5(DBlock b, double *arr, long total)
+__global__ void k_zero_ghosts(DBlock b, double *arr, long total, int nv)
 {
     const long e = (long)(b.ni + 6) * (b.nj + 6) * (b.nk + 6);
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -479,7 +577,7 @@ __global__ void k_zero_ghosts5(DBlock b, double *arr, long total)
     const int i = (int)(t % (b.ni + 6)) - 2, j = (int)((t / (b.ni + 6)) % (b.nj + 6)) - 2;
     const int k = (int)(t / ((long)(b.ni + 6) * (b.nj + 6))) - 2;
     if (i >= 1 && i <= b.ni && j >= 1 && j <= b.nj && k >= 1 && k <= b.nk) return;
-    for (int m = 0; m < 5; m++) arr[m * total + b.off + t] = 0.0;
+    for (int m = 0; m < nv; m++) arr[m * total + b.off + t] = 0.0;
 }
 
 }  // namespace
@@ -511,11 +609,27 @@ int launch_ts_jvp(const BlockGrid &G, const DFields &F, const DParams &P, const 
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_ts_vjp(const BlockGrid &G, const DFields &F, const DParams &P, const double *ls, const double *lc,
-                  double *lp, hipStream_t s)
+int launch_ts_vjp(const BlockGrid &G, const DFields &F, const DParams &P, const double *ls, const double *lv,
+                  const double *lc, double *lp, hipStream_t s)
 {
     if (G.nwg == 0) return 0;
-    hipLaunchKernelGGL(k_ts_vjp, dim3(G.nwg), dim3(256), 0, s, G, F, P, ls, lc, lp);
+    hipLaunchKernelGGL(k_ts_vjp, dim3(G.nwg), dim3(256), 0, s, G, F, P, ls, lv, lc, lp);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_tout_jvp(const BlockGrid &G, const DFields &F, const DParams &P, const double *pf, const double *tp,
+                    double *tt, hipStream_t s)
+{
+    if (G.nwg == 0) return 0;
+    hipLaunchKernelGGL(k_tout_jvp, dim3(G.nwg), dim3(256), 0, s, G, F, P, pf, tp, tt);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_tout_vjp(const BlockGrid &G, const DFields &F, const DParams &P, const double *pf, const double *lt,
+                    double *lp, hipStream_t s)
+{
+    if (G.nwg == 0) return 0;
+    hipLaunchKernelGGL(k_tout_vjp, dim3(G.nwg), dim3(256), 0, s, G, F, P, pf, lt, lp);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -536,37 +650,38 @@ int launch_upd_vjp(const BlockGrid &G, const DParams &P, const double *pf, const
 }
 
 int launch_avg2pm_t(const DRegion &g, const DBlock *blocks, const double *vol, double *lam, long total, int use_vol,
-                    hipStream_t s)
+                    int nv, hipStream_t s)
 {
     const DBlock &b = blocks[g.nb], &bt = blocks[g.nbt];
-    if (g.serial) hipLaunchKernelGGL(k_avg2pm_t, dim3(1), dim3(64), 0, s, g, b, bt, vol, lam, total, use_vol);
-    else hipLaunchKernelGGL(k_avg2pm_t, grid1(g.npts, 128), dim3(128), 0, s, g, b, bt, vol, lam, total, use_vol);
+    if (g.serial) hipLaunchKernelGGL(k_avg2pm_t, dim3(1), dim3(64), 0, s, g, b, bt, vol, lam, total, use_vol, nv);
+    else hipLaunchKernelGGL(k_avg2pm_t, grid1(g.npts, 128), dim3(128), 0, s, g, b, bt, vol, lam, total, use_vol, nv);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 int launch_npp_assign_t(const int *grp_start, const int *grp_n, const int *pts, const int *write_ok, int ngroups,
                         const DBlock *blocks, const double *vol, double *lam, long total, double *bufb, int use_vol,
-                        hipStream_t s)
+                        int nv, hipStream_t s)
 {
     if (ngroups <= 0) return 0;
     hipLaunchKernelGGL(k_npp_assign_t, grid1(ngroups, 128), dim3(128), 0, s, grp_start, grp_n, pts, write_ok,
-                       ngroups, blocks, vol, lam, total, bufb, use_vol);
+                       ngroups, blocks, vol, lam, total, bufb, use_vol, nv);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 int launch_npp_gather_t(const int *pts, const int *ustart, const int *ent, int nuniq, const DBlock *blocks,
-                        const double *vol, double *lam, long total, const double *bufb, int use_vol, hipStream_t s)
+                        const double *vol, double *lam, long total, const double *bufb, int use_vol, int nv,
+                        hipStream_t s)
 {
     if (nuniq <= 0) return 0;
     hipLaunchKernelGGL(k_npp_gather_t, grid1(nuniq, 128), dim3(128), 0, s, pts, ustart, ent, nuniq, blocks, vol, lam,
-                       total, bufb, use_vol);
+                       total, bufb, use_vol, nv);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_zero_ghosts5(const DBlock &b, double *arr, long total, hipStream_t s)
+int launch_zero_ghosts(const DBlock &b, double *arr, long total, int nv, hipStream_t s)
 {
     const long e = (long)(b.ni + 6) * (b.nj + 6) * (b.nk + 6);
-    hipLaunchKernelGGL(k_zero_ghosts5, grid1(e, 256), dim3(256), 0, s, b, arr, total);
+    hipLaunchKernelGGL(k_zero_ghosts, grid1(e, 256), dim3(256), 0, s, b, arr, total, nv);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

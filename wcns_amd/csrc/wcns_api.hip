@@ -2273,13 +2273,16 @@ int wcns_adj_force(wcns_ctx *ctx, int nb, const double *P, const double *X, cons
 // ------------------------------------------------ tangent and adjoint of a whole step
 // Pool of the step's derivatives (AS_N arrays of total elements): the step's
 // primitives after the boundary conditions (PF, 5), the four sweeps' dQ (X,
-// 20), an all-zero array, the adjoint weights of the new primitives (LPN, 5),
-// and a work region shared by the tangent and the adjoint passes.
-enum { AS_PF = 0, AS_X = 5, AS_ZERO = 25, AS_LPN = 26, AS_W = 31 };
-// tangent work: primitives after the BCs 5, residual 5, radii + coed 4, sweeps 20, result 5
-enum { AT_P = 0, AT_R = 5, AT_S = 10, AT_X = 14, AT_PN = 34 };
-// adjoint work: primitives after the BCs 5, dQ 5, sweep outputs 20, multipliers 20, R 5, coed 1, radii 3
-enum { AA_P = 0, AA_D = 5, AA_XB = 10, AA_MU = 30, AA_RB = 50, AA_CB = 55, AA_SR = 56, AA_N = 59 };
+// 20), an all-zero array, the adjoint weights of the new state (LPN, 6), μ_t
+// as the radii read it (VT), and a work region shared by the tangent and the
+// adjoint passes.
+enum { AS_PF = 0, AS_X = 5, AS_ZERO = 25, AS_LPN = 26, AS_VT = 32, AS_W = 33 };
+// tangent work: state after the BCs 6 (r..p, T), residual 5, radii, viscous
+// radii and coed 7, sweeps 20, result 6 (r..p, T)
+enum { AT_P = 0, AT_R = 6, AT_S = 11, AT_X = 18, AT_PN = 38, AT_N = 44 };
+// adjoint work: state after the BCs 6, dQ 5, sweep outputs 20, multipliers
+// 20, R 5, coed 1, radii 3, viscous radii 3
+enum { AA_P = 0, AA_D = 6, AA_XB = 11, AA_MU = 31, AA_RB = 51, AA_CB = 56, AA_SR = 57, AA_SRV = 60, AA_N = 63 };
 enum { AS_N = AS_W + AA_N };
 
 // the BC sequence of boundary_all, region by region
@@ -2297,11 +2300,23 @@ static int for_bc_regions(wcns_ctx *ctx, bool reverse, Fn fn)
     return WCNS_OK;
 }
 
+// Differentiated variables of the state (r, u, v, w, p, and T for a viscous
+// case, which the no-slip wall reads one step late) and the whole state the
+// step carries (with a turbulence model also nu~ or k, mu_t, and omega).
+static int as_nvar(const wcns_ctx *ctx) { return ctx->prm.nvis == 1 ? 6 : 5; }
+static std::vector<double *> state_ptrs(wcns_ctx *ctx)
+{
+    DFields F = ctx->fields();
+    std::vector<double *> v = {F.r, F.u, F.v, F.w, F.p};
+    if (ctx->prm.nvis == 1) v.push_back(F.t);
+    if (ctx->dp.turb) { v.push_back(F.qke); v.push_back(F.vist); }
+    if (ctx->dp.turb == 2) v.push_back(F.qkw);
+    return v;
+}
+
 static int as_check(wcns_ctx *ctx)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
-    if (ctx->prm.nvis != 0 || ctx->prm.nlamtur >= 0)
-        return fail(ctx, WCNS_E_UNSUPPORTED, "the step tangent and adjoint cover inviscid steps (nvis = 0)");
     if (ctx->lhs_prec != WCNS_LHS_FP64)
         return fail(ctx, WCNS_E_UNSUPPORTED, "the step tangent and adjoint differentiate the FP64 implicit operator");
     for (auto &b : ctx->blocks)
@@ -2314,7 +2329,7 @@ static int as_check(wcns_ctx *ctx)
     ctx->as_tape_off.clear();
     for_bc_regions(ctx, false, [&](const DRegion &g, size_t) {
         ctx->as_tape_off.push_back(off);
-        off += 15L * g.npts;
+        off += 18L * g.npts;
         return 0;
     });
     TRY(dalloc(ctx, &ctx->as_tape, (size_t)off));
@@ -2345,17 +2360,18 @@ static int as_fill(wcns_ctx *ctx, double *a, long n)
                : WCNS_OK;
 }
 
-static int as_copy(wcns_ctx *ctx, double *const dst[5], const double *src)
+// n pool arrays from / to n consecutive arrays of a flat buffer
+static int as_copy(wcns_ctx *ctx, double *const *dst, const double *src, int n)
 {
-    for (int m = 0; m < 5; m++)
+    for (int m = 0; m < n; m++)
         CK(hipMemcpyAsync(dst[m], src + (size_t)m * ctx->total, 8 * (size_t)ctx->total, hipMemcpyDeviceToDevice,
                           ctx->stream));
     return WCNS_OK;
 }
 
-static int as_copy_out(wcns_ctx *ctx, double *dst, double *const src[5])
+static int as_copy_out(wcns_ctx *ctx, double *dst, double *const *src, int n)
 {
-    for (int m = 0; m < 5; m++)
+    for (int m = 0; m < n; m++)
         CK(hipMemcpyAsync(dst + (size_t)m * ctx->total, src[m], 8 * (size_t)ctx->total, hipMemcpyDeviceToDevice,
                           ctx->stream));
     return WCNS_OK;
@@ -2385,20 +2401,20 @@ static int communicate_on(wcns_ctx *ctx, const DFields &F, int pv)
     return WCNS_OK;
 }
 
-// their transpose on the 5 arrays lam (dQ with use_vol, else the primitives)
-static int communicate_t(wcns_ctx *ctx, double *lam, int use_vol)
+// their transpose on nv arrays lam (dQ with use_vol, else the primitives and T)
+static int communicate_t(wcns_ctx *ctx, double *lam, int use_vol, int nv)
 {
     std::vector<DBlock> dbs;
     for (auto &b : ctx->blocks) dbs.push_back(b.db);
     const double *vol = ctx->metric[9];
     if (launch_npp_assign_t(ctx->d_grp_start, ctx->d_grp_n, ctx->d_npp_pts, ctx->d_write_ok, ctx->ngroups,
-                            ctx->d_blocks, vol, lam, ctx->total, ctx->d_npp_buf, use_vol, ctx->stream))
+                            ctx->d_blocks, vol, lam, ctx->total, ctx->d_npp_buf, use_vol, nv, ctx->stream))
         return fail(ctx, WCNS_E_HIP, "npp assign transpose");
     for (size_t a = ctx->iface.size(); a-- > 0;)
-        if (launch_avg2pm_t(ctx->iface[a], dbs.data(), vol, lam, ctx->total, use_vol, ctx->stream))
+        if (launch_avg2pm_t(ctx->iface[a], dbs.data(), vol, lam, ctx->total, use_vol, nv, ctx->stream))
             return fail(ctx, WCNS_E_HIP, "avg2pm transpose");
     if (launch_npp_gather_t(ctx->d_npp_pts, ctx->as_ustart, ctx->as_uent, ctx->as_nuniq, ctx->d_blocks, vol, lam,
-                            ctx->total, ctx->d_npp_buf, use_vol, ctx->stream))
+                            ctx->total, ctx->d_npp_buf, use_vol, nv, ctx->stream))
         return fail(ctx, WCNS_E_HIP, "npp gather transpose");
     return WCNS_OK;
 }
@@ -2410,23 +2426,65 @@ static DFields with_prims(DFields F, double *p)
     return F;
 }
 
-// One step of the state in the current primitive buffer (P, if given, is
-// copied there first), as step_once computes it but with the sweeps into
-// separate arrays and the boundary values taped; with dP its tangent too.
-// Leaves PF, X, R, the radii and coed of the step, and the new state current.
+// The residual's tangent (viscous part, averaging, inviscid part, averaging
+// again, as step_once forms R) into tr, from the tangent tp of the state
+// after the BCs
+static int as_rhs_jvp(wcns_ctx *ctx, const DFields &F, const double *tp, double *tr)
+{
+    const long T = ctx->total;
+    double *A = ctx->ad_pool;
+    TRY(ad_zero(ctx, AD_M, AD_R - AD_M));
+    AdTan D{tp, A + AD_M * T, A + AD_T * T, A + AD_MU * T, A + AD_DN * T, tr};
+    if (ctx->prm.nvis == 1) {
+        for (auto &b : ctx->blocks) {
+            if (launch_tmu_jvp(b.db, F, ctx->dp, A + AD_T * T, A + AD_MU * T, tp, ctx->stream) ||
+                launch_dn_jvp(b.db, F, D, A + AD_DN * T, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "viscous tangent");
+            for (int dir = 0; dir < 3; dir++)
+                if (launch_rhs_vis_jvp(dir, b.db, F, ctx->dp, ctx->dn, D, ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "viscous tangent");
+        }
+        DFields Ft = F;
+        Ft.rhs = tr;
+        TRY(communicate_on(ctx, Ft, 0));
+    }
+    for (auto &b : ctx->blocks)
+        for (int dir = 0; dir < 3; dir++)
+            if (launch_rhs_inv_jvp(dir, b.db, F, ctx->dp, D, ctx->stream)) return fail(ctx, WCNS_E_HIP, "inviscid tangent");
+    DFields Ft = F;
+    Ft.rhs = tr;
+    TRY(communicate_on(ctx, Ft, 0));
+    return zero_faces(ctx, tr);
+}
+
+// One step of the state in the current buffers (P, if given, is copied there
+// first: state_ptrs' arrays), as step_once computes it but with the sweeps'
+// outputs kept and the boundary values taped; with dP (as_nvar arrays) its
+// tangent too.  Leaves PF, X, R, the radii and coed of the step, and the new
+// state current.
 static int as_forward(wcns_ctx *ctx, const double *P, const double *dP)
 {
     const long T = ctx->total;
+    const int nv = as_nvar(ctx);
+    const bool vis = ctx->prm.nvis == 1;
     double *S = ctx->as_pool, *W = S + (size_t)AS_W * T;
     DFields F = ctx->fields();
+    std::vector<double *> st = state_ptrs(ctx);
+    if (P) TRY(as_copy(ctx, st.data(), P, (int)st.size()));
     double *cur[5] = {F.r, F.u, F.v, F.w, F.p};
-    if (P) TRY(as_copy(ctx, cur, P));
     const bool tan = dP != nullptr;
     if (tan) {
         TRY(as_fill(ctx, W, (long)AA_N * T));
-        double *tp[5];
-        for (int m = 0; m < 5; m++) tp[m] = W + (AT_P + m) * T;
-        TRY(as_copy(ctx, tp, dP));
+        double *tp[6];
+        for (int m = 0; m < nv; m++) tp[m] = W + (AT_P + m) * T;
+        TRY(as_copy(ctx, tp, dP, nv));
+    }
+    if (ctx->dp.turb) {
+        // the turbulence models' boundary values by the solver's own pass, then
+        // the primitives as they were for the taped pass below
+        TRY(as_copy_out(ctx, S + AS_PF * T, cur, 5));
+        TRY(wcns_boundary_all(ctx));
+        TRY(as_copy(ctx, cur, S + AS_PF * T, 5));
     }
     std::vector<DBlock> dbs;
     for (auto &b : ctx->blocks) dbs.push_back(b.db);
@@ -2436,34 +2494,28 @@ static int as_forward(wcns_ctx *ctx, const double *P, const double *dP)
                    ? fail(ctx, WCNS_E_HIP, "boundary tangent launch")
                    : WCNS_OK;
     }));
-    TRY(as_copy_out(ctx, S + AS_PF * T, cur));
+    TRY(as_copy_out(ctx, S + AS_PF * T, cur, 5));
+    if (ctx->dp.turb) CK(hipMemcpyAsync(S + AS_VT * T, F.vist, 8 * (size_t)T, hipMemcpyDeviceToDevice, ctx->stream));
     TRY(wcns_initial_q_dq_c_t(ctx));
     TRY(wcns_timestep(ctx));
+    if (tan && launch_ts_jvp(ctx->grid[GRID_NODES], F, ctx->dp, W + AT_P * T, W + AT_S * T, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "radii tangent");
+    if (vis) {
+        TRY(wcns_r_h_s_vis(ctx));
+        TRY(communicate_on(ctx, F, 0));
+    }
     TRY(wcns_r_h_s_invis(ctx));
     TRY(communicate_on(ctx, F, 0));
     TRY(zero_faces(ctx, F.rhs));
-    if (tan) {
-        if (launch_ts_jvp(ctx->grid[GRID_NODES], F, ctx->dp, W + AT_P * T, W + AT_S * T, ctx->stream))
-            return fail(ctx, WCNS_E_HIP, "radii tangent");
-        TRY(ad_zero(ctx, AD_M, AD_R - AD_M));
-        double *A = ctx->ad_pool;
-        AdTan D{W + AT_P * T, A + AD_M * T, A + AD_T * T, A + AD_MU * T, A + AD_DN * T, W + AT_R * T};
-        for (auto &b : ctx->blocks)
-            for (int dir = 0; dir < 3; dir++)
-                if (launch_rhs_inv_jvp(dir, b.db, F, ctx->dp, D, ctx->stream))
-                    return fail(ctx, WCNS_E_HIP, "inviscid tangent");
-        DFields Ft = F;
-        Ft.rhs = W + AT_R * T;
-        TRY(communicate_on(ctx, Ft, 0));
-        TRY(zero_faces(ctx, W + AT_R * T));
-    }
+    if (tan) TRY(as_rhs_jvp(ctx, F, W + AT_P * T, W + AT_R * T));
     AdSweep A{};
     A.x = S + AS_X * T;
     if (tan) {
         A.tx = W + AT_X * T;
         A.tp = W + AT_P * T;
         A.tsr = W + AT_S * T;
-        A.tcoed = W + (AT_S + 3) * T;
+        A.tsrv = W + (AT_S + 3) * T;
+        A.tcoed = W + (AT_S + 6) * T;
         A.trhs = W + AT_R * T;
         A.zero = S + AS_ZERO * T;
     }
@@ -2494,13 +2546,17 @@ static int as_forward(wcns_ctx *ctx, const double *P, const double *dP)
     if (tan && launch_upd_jvp(ctx->grid[GRID_NODES], ctx->dp, S + AS_PF * T, F.dq, W + AT_P * T, W + (AT_X + 15) * T,
                               W + AT_PN * T, T, ctx->stream))
         return fail(ctx, WCNS_E_HIP, "update tangent");
+    if (tan && vis && launch_tout_jvp(ctx->grid[GRID_NODES], F, ctx->dp, S + AS_PF * T, W + AT_P * T,
+                                      W + (AT_PN + 5) * T, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "temperature tangent");
     ctx->cur ^= 1;
     ctx->pt_fresh = false;
     TRY(communicate_on(ctx, ctx->fields(), 1));
     if (tan) {
         DFields Ft = with_prims(ctx->fields(), W + AT_PN * T);
-        Ft.t = S + AS_ZERO * T;    // the temperature's average is not part of an inviscid state
-        TRY(communicate_on(ctx, Ft, 1));   // (averages of zeros: the array stays zero)
+        // the averaged T is part of a viscous state; an inviscid one averages zeros
+        Ft.t = vis ? W + (AT_PN + 5) * T : S + AS_ZERO * T;
+        TRY(communicate_on(ctx, Ft, 1));
     }
     std::vector<unsigned long long> nf(ctx->blocks.size());
     CK(hipMemcpyAsync(nf.data(), ctx->d_nflag, 8 * nf.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -2511,23 +2567,27 @@ static int as_forward(wcns_ctx *ctx, const double *P, const double *dP)
     return WCNS_OK;
 }
 
-// The adjoint of the step from state P (5 pool arrays, current buffer if
-// null) for weights Wn of the new primitives, into gP (interior nodes).
+// The adjoint of the step from state P (current buffers if null) for the
+// weights Wn of the new state's as_nvar arrays, into gP (interior nodes).
 static int as_reverse(wcns_ctx *ctx, const double *P, const double *Wn, double *gP)
 {
     const long T = ctx->total;
+    const int nv = as_nvar(ctx);
+    const bool vis = ctx->prm.nvis == 1;
     double *S = ctx->as_pool, *W = S + (size_t)AS_W * T;
     TRY(as_forward(ctx, P, nullptr));
     double *lpn = S + AS_LPN * T;
-    if (Wn != lpn) CK(hipMemcpyAsync(lpn, Wn, 8 * 5 * (size_t)T, hipMemcpyDeviceToDevice, ctx->stream));
+    if (Wn != lpn) CK(hipMemcpyAsync(lpn, Wn, 8 * nv * (size_t)T, hipMemcpyDeviceToDevice, ctx->stream));
     TRY(as_fill(ctx, W, (long)AA_N * T));
-    TRY(communicate_t(ctx, lpn, 0));
+    TRY(communicate_t(ctx, lpn, 0, nv));
     double *lp = W + AA_P * T, *ld = W + AA_D * T;
     if (launch_upd_vjp(ctx->grid[GRID_NODES], ctx->dp, S + AS_PF * T, ctx->dq, lpn, lp, ld, T, ctx->stream))
         return fail(ctx, WCNS_E_HIP, "update adjoint");
+    DFields Fs = with_prims(ctx->fields(), S + AS_PF * T);
+    if (vis && launch_tout_vjp(ctx->grid[GRID_NODES], Fs, ctx->dp, S + AS_PF * T, lpn + 5 * T, lp, ctx->stream))
+        return fail(ctx, WCNS_E_HIP, "temperature adjoint");
     TRY(zero_faces(ctx, ld));
     CK(hipMemcpyAsync(W + (AA_XB + 15) * T, ld, 8 * 5 * (size_t)T, hipMemcpyDeviceToDevice, ctx->stream));
-    const DFields Fs = with_prims(ctx->fields(), S + AS_PF * T);
     AdSweep A{};
     A.x = S + AS_X * T;
     A.mu = W + AA_MU * T;
@@ -2536,20 +2596,42 @@ static int as_reverse(wcns_ctx *ctx, const double *P, const double *Wn, double *
     A.cb = W + AA_CB * T;
     A.lp = lp;
     A.lsr = W + AA_SR * T;
+    A.lsrv = W + AA_SRV * T;
     for (size_t nb = ctx->blocks.size(); nb-- > 0;)
         if (launch_ad_sweeps_rev(ctx->blocks[nb].db, Fs, ctx->dp, A, ctx->stream))
             return fail(ctx, WCNS_E_HIP, "sweeps adjoint");
-    if (launch_ts_vjp(ctx->grid[GRID_NODES], Fs, ctx->dp, A.lsr, A.cb, lp, ctx->stream))
-        return fail(ctx, WCNS_E_HIP, "radii adjoint");
+    {
+        // the radii saw μ_t before the turbulence model's update of the step
+        DFields Fv = Fs;
+        if (ctx->dp.turb) Fv.vist = S + AS_VT * T;
+        if (launch_ts_vjp(ctx->grid[GRID_NODES], Fv, ctx->dp, A.lsr, A.lsrv, A.cb, lp, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "radii adjoint");
+    }
     TRY(zero_faces(ctx, A.rb));
-    TRY(communicate_t(ctx, A.rb, 1));
-    TRY(ad_zero(ctx, AD_M, AD_R - AD_M));
+    TRY(communicate_t(ctx, A.rb, 1, 5));
     double *AD = ctx->ad_pool;
+    TRY(ad_zero(ctx, AD_M, AD_N - AD_M));
     AdAdj G{A.rb, lp, AD + AD_M * T, AD + AD_T * T, AD + AD_MU * T, AD + AD_DN * T, AD + AD_G * T};
     for (auto &b : ctx->blocks)
         for (int dir = 0; dir < 3; dir++)
             if (launch_rhs_inv_vjp(dir, b.db, Fs, ctx->dp, G, ctx->stream))
                 return fail(ctx, WCNS_E_HIP, "inviscid adjoint");
+    if (vis) {
+        // the viscous part was averaged once more: its weights go through the
+        // transpose again; T and mu of the state after the BCs, as the
+        // residual used them (the primitive averaging has since moved T)
+        TRY(communicate_t(ctx, A.rb, 1, 5));
+        if (launch_init_stage(ctx->grid[GRID_NODES], &ctx->grid[GRID_GHOST_T], Fs, ctx->dp, ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "init launch");
+        for (auto &b : ctx->blocks) {
+            for (int dir = 0; dir < 3; dir++)
+                if (launch_rhs_vis_vjp(dir, b.db, Fs, ctx->dp, ctx->dn, G, ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "viscous adjoint");
+            if (launch_dn_vjp(b.db, Fs, G, ctx->stream) ||
+                launch_tmu_vjp(b.db, Fs, ctx->dp, AD + AD_T * T, AD + AD_MU * T, lp, ctx->stream))
+                return fail(ctx, WCNS_E_HIP, "viscous adjoint");
+        }
+    }
     std::vector<DBlock> dbs;
     for (auto &b : ctx->blocks) dbs.push_back(b.db);
     TRY(for_bc_regions(ctx, true, [&](const DRegion &g, size_t q) {
@@ -2558,13 +2640,13 @@ static int as_reverse(wcns_ctx *ctx, const double *P, const double *Wn, double *
                    : WCNS_OK;
     }));
     for (auto &b : ctx->blocks)
-        if (launch_zero_ghosts5(b.db, lp, T, ctx->stream)) return fail(ctx, WCNS_E_HIP, "ghost clear");
-    CK(hipMemcpyAsync(gP, lp, 8 * 5 * (size_t)T, hipMemcpyDeviceToDevice, ctx->stream));
+        if (launch_zero_ghosts(b.db, lp, T, nv, ctx->stream)) return fail(ctx, WCNS_E_HIP, "ghost clear");
+    CK(hipMemcpyAsync(gP, lp, 8 * nv * (size_t)T, hipMemcpyDeviceToDevice, ctx->stream));
     return WCNS_OK;
 }
 
 // J = sum_c omega_c C_c (pressure part of the wall force, scaled as
-// wcns_force) of the current state, and dJ/dP into g (5 pool arrays)
+// wcns_force) of the current state, and dJ/dP into g (as_nvar pool arrays)
 static int as_force(wcns_ctx *ctx, const double omega[6], double *J, double *g)
 {
     if (ctx->prm.sref == 0.0 || ctx->prm.lfref == 0.0) return fail(ctx, WCNS_E_ARG, "sref and lfref must be non-zero");
@@ -2577,7 +2659,7 @@ static int as_force(wcns_ctx *ctx, const double omega[6], double *J, double *g)
     const int wf = 2 - p.nwholefield;
     double om[6];
     for (int c = 0; c < 6; c++) om[c] = omega[c] * wf * (c < 3 ? 1.0 / p.sref : 1.0 / (p.sref * p.lfref));
-    TRY(as_fill(ctx, g, 5 * T));
+    TRY(as_fill(ctx, g, (long)as_nvar(ctx) * T));
     TRY(ad_zero(ctx, AD_M, AD_R - AD_M));
     double *AD = ctx->ad_pool;
     double sum = 0.0;
@@ -2601,12 +2683,14 @@ static int as_force(wcns_ctx *ctx, const double omega[6], double *J, double *g)
 
 extern "C" {
 
-int wcns_pool_layout(wcns_ctx *ctx, long *total, long *offsets)
+int wcns_pool_layout(wcns_ctx *ctx, long *total, long *offsets, int *nstate, int *nvar)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     if (total) *total = ctx->total;
     if (offsets)
         for (size_t nb = 0; nb < ctx->blocks.size(); nb++) offsets[nb] = ctx->blocks[nb].db.off;
+    if (nstate) *nstate = (int)state_ptrs(ctx).size();
+    if (nvar) *nvar = as_nvar(ctx);
     return WCNS_OK;
 }
 
@@ -2615,15 +2699,15 @@ int wcns_adj_step_jvp(wcns_ctx *ctx, const double *P, const double *dP, double *
     TRY(as_check(ctx));
     if (!dP) return fail(ctx, WCNS_E_ARG, "null direction");
     TRY(as_forward(ctx, P, dP));
-    DFields F = ctx->fields();
-    double *cur[5] = {F.r, F.u, F.v, F.w, F.p};
-    if (Pn) TRY(as_copy_out(ctx, Pn, cur));
+    const long T = ctx->total;
+    std::vector<double *> st = state_ptrs(ctx);
+    if (Pn) TRY(as_copy_out(ctx, Pn, st.data(), (int)st.size()));
     if (dPn) {
-        const long T = ctx->total;
-        CK(hipMemcpyAsync(dPn, ctx->as_pool + (size_t)(AS_W + AT_PN) * T, 8 * 5 * (size_t)T, hipMemcpyDeviceToDevice,
+        const int nv = as_nvar(ctx);
+        CK(hipMemcpyAsync(dPn, ctx->as_pool + (size_t)(AS_W + AT_PN) * T, 8 * nv * (size_t)T, hipMemcpyDeviceToDevice,
                           ctx->stream));
         for (auto &b : ctx->blocks)
-            if (launch_zero_ghosts5(b.db, dPn, T, ctx->stream)) return fail(ctx, WCNS_E_HIP, "ghost clear");
+            if (launch_zero_ghosts(b.db, dPn, T, nv, ctx->stream)) return fail(ctx, WCNS_E_HIP, "ghost clear");
     }
     CK(hipStreamSynchronize(ctx->stream));
     return WCNS_OK;
@@ -2644,27 +2728,26 @@ int wcns_adj_multistep(wcns_ctx *ctx, const double *P0, int nsteps, int every, c
     TRY(as_check(ctx));
     if (!P0 || !omega || !J || nsteps < 1 || every < 1) return fail(ctx, WCNS_E_ARG, "bad arguments");
     const long T = ctx->total;
+    const int ns = (int)state_ptrs(ctx).size(), nv = as_nvar(ctx);
     const int nck = (nsteps + every - 1) / every, seg = std::min(every, nsteps);
-    const size_t state = 5 * (size_t)T;
-    const double store = 8.0 * state * (nck + seg + 1);
+    const size_t state = (size_t)ns * T;
+    const double store = 8.0 * (state * (nck + seg) + (size_t)nv * T);
     const double work = 8.0 * (double)T * (AS_N + AD_N);
     if (used_gb) *used_gb = (store + work) / 1e9;
     if (budget_gb > 0.0 && store + work > budget_gb * 1e9)
         return fail(ctx, WCNS_E_ARG, "checkpoints and work arrays need " + std::to_string((store + work) / 1e9) +
                                          " GB, above the budget");
     double *ck = nullptr;
-    TRY(dalloc(ctx, &ck, state * (nck + seg + 1)));
+    TRY(dalloc(ctx, &ck, state * (nck + seg) + (size_t)nv * T));
     double *sg = ck + state * nck, *lam = sg + state * seg;
     auto run = [&]() -> int {
-        DFields F = ctx->fields();
-        double *cur[5] = {F.r, F.u, F.v, F.w, F.p};
-        TRY(as_copy(ctx, cur, P0));
+        std::vector<double *> st = state_ptrs(ctx);
+        TRY(as_copy(ctx, st.data(), P0, ns));
         // the states by the solver's own step (the derivative's primal equals it bit for bit)
         for (int n = 0; n < nsteps; n++) {
             if (n % every == 0) {
-                F = ctx->fields();
-                double *c5[5] = {F.r, F.u, F.v, F.w, F.p};
-                TRY(as_copy_out(ctx, ck + state * (n / every), c5));
+                st = state_ptrs(ctx);
+                TRY(as_copy_out(ctx, ck + state * (n / every), st.data(), ns));
             }
             TRY(step_once(ctx));
         }
@@ -2674,18 +2757,16 @@ int wcns_adj_multistep(wcns_ctx *ctx, const double *P0, int nsteps, int every, c
         for (int c = nck - 1; c >= 0; c--) {
             const int s0 = c * every, s1 = std::min(nsteps, s0 + every);
             CK(hipMemcpyAsync(sg, ck + state * c, 8 * state, hipMemcpyDeviceToDevice, ctx->stream));
-            F = ctx->fields();
-            double *c5[5] = {F.r, F.u, F.v, F.w, F.p};
-            TRY(as_copy(ctx, c5, sg));
+            st = state_ptrs(ctx);
+            TRY(as_copy(ctx, st.data(), sg, ns));
             for (int n = s0; n + 1 < s1; n++) {
                 TRY(step_once(ctx));
-                F = ctx->fields();
-                double *c6[5] = {F.r, F.u, F.v, F.w, F.p};
-                TRY(as_copy_out(ctx, sg + state * (n + 1 - s0), c6));
+                st = state_ptrs(ctx);
+                TRY(as_copy_out(ctx, sg + state * (n + 1 - s0), st.data(), ns));
             }
             for (int n = s1 - 1; n >= s0; n--) TRY(as_reverse(ctx, sg + state * (n - s0), lam, lam));
         }
-        CK(hipMemcpyAsync(gP0, lam, 8 * state, hipMemcpyDeviceToDevice, ctx->stream));
+        CK(hipMemcpyAsync(gP0, lam, 8 * nv * (size_t)T, hipMemcpyDeviceToDevice, ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
         return WCNS_OK;
     };

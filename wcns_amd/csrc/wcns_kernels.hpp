@@ -146,14 +146,15 @@ int launch_sweep(int kind, const DBlock &b, const DFields &F, const DParams &P, 
 // arrays of F.total doubles; a group of 5 is component-major).  The forward
 // writes the sweeps' dQ x (20: sweep q, component m at (5q+m)*total) and,
 // with tx set, their tangents from those of the primitives (tp, 5), radii
-// (tsr, 3), coed (tcoed) and residual (trhs, 5); zero is an all-zero array.
-// The reverse takes the adjoint of x3 in xb[15..19] and leaves the
-// multipliers (mu, 20), the adjoints of x0..x2 (xb), and accumulates those of
-// R (rb, 5), coed (cb), the primitives (lp, 5) and the radii (lsr, 3).
+// (tsr, 3), viscous radii (tsrv, 3, or null), coed (tcoed) and residual
+// (trhs, 5); zero is an all-zero array.  The reverse takes the adjoint of x3
+// in xb[15..19] and leaves the multipliers (mu, 20), the adjoints of x0..x2
+// (xb), and accumulates those of R (rb, 5), coed (cb), the primitives (lp, 5),
+// the radii (lsr, 3) and the viscous radii (lsrv, 3).
 struct AdSweep {
     double *x, *tx;
-    const double *tp, *tsr, *tcoed, *trhs, *zero;
-    double *mu, *xb, *rb, *cb, *lp, *lsr;
+    const double *tp, *tsr, *tsrv, *tcoed, *trhs, *zero;
+    double *mu, *xb, *rb, *cb, *lp, *lsr, *lsrv;
 };
 int launch_ad_sweeps_fwd(const DBlock &b, const DFields &F, const DParams &P, const AdSweep &A, hipStream_t s);
 int launch_ad_sweeps_rev(const DBlock &b, const DFields &F, const DParams &P, const AdSweep &A, hipStream_t s);
@@ -161,18 +162,25 @@ int launch_ad_sweeps_rev(const DBlock &b, const DFields &F, const DParams &P, co
 // Tangent and adjoint of the rest of an (inviscid) step, wcns_adstep.hip.
 // Boundary regions one at a time, in the order of the solver's sequence: the
 // forward sets the ghosts and face nodes as bc_point does, in D1 when tan is
-// set (its tangent follows), and records the values each point reads
-// (15 doubles per point) on the tape; the reverse moves the adjoint of every
-// value a point wrote to the values it read.
+// set (6 arrays: r, u, v, w, p, T; its tangent follows), and records the
+// values each point reads (18 doubles per point) on the tape; the reverse
+// moves the adjoint of every value a point wrote to the values it read.
 int launch_bc_fwd(const DRegion &g, const DBlock *blocks, const DFields &F, const DParams &P, double *tan,
                   double *tape, hipStream_t s);
 int launch_bc_rev(const DRegion &g, const DBlock *blocks, const DFields &F, const DParams &P, double *lam,
                   const double *tape, hipStream_t s);
-// spectral radii and coed (k_timestep): tangent ts (4: sra, srb, src, coed)
-// from tp; adjoint lp += from ls (3) and lc
+// spectral radii, viscous radii and coed (k_spectvis + k_timestep): tangent
+// ts (7: sra, srb, src, srva, srvb, srvc, coed) from tp; adjoint lp += from
+// ls (3), lv (3) and lc
 int launch_ts_jvp(const BlockGrid &G, const DFields &F, const DParams &P, const double *tp, double *ts, hipStream_t s);
-int launch_ts_vjp(const BlockGrid &G, const DFields &F, const DParams &P, const double *ls, const double *lc,
-                  double *lp, hipStream_t s);
+int launch_ts_vjp(const BlockGrid &G, const DFields &F, const DParams &P, const double *ls, const double *lv,
+                  const double *lc, double *lp, hipStream_t s);
+// the temperature initial_q_dq_c_t forms from primitives pf: tangent tt from
+// tp; adjoint lp += from lt
+int launch_tout_jvp(const BlockGrid &G, const DFields &F, const DParams &P, const double *pf, const double *tp,
+                    double *tt, hipStream_t s);
+int launch_tout_vjp(const BlockGrid &G, const DFields &F, const DParams &P, const double *pf, const double *lt,
+                    double *lp, hipStream_t s);
 // limited update (k_update) of primitives pf by dq: tangent tpn; adjoint
 // lp += and ld = from lpn
 int launch_upd_jvp(const BlockGrid &G, const DParams &P, const double *pf, const double *dq, const double *tpf,
@@ -180,18 +188,19 @@ int launch_upd_jvp(const BlockGrid &G, const DParams &P, const double *pf, const
 int launch_upd_vjp(const BlockGrid &G, const DParams &P, const double *pf, const double *dq, const double *lpn,
                    double *lp, double *ld, long total, hipStream_t s);
 // transposed interface averagings (k_npp_gather / k_avg2pm / k_npp_assign),
-// on 5 arrays lam; use_vol as for dQ
+// on nv (5 or 6) arrays lam; use_vol as for dQ; bufb holds 6 per entry
 int launch_avg2pm_t(const DRegion &g, const DBlock *blocks, const double *vol, double *lam, long total, int use_vol,
-                    hipStream_t s);
+                    int nv, hipStream_t s);
 int launch_npp_assign_t(const int *grp_start, const int *grp_n, const int *pts, const int *write_ok, int ngroups,
                         const DBlock *blocks, const double *vol, double *lam, long total, double *bufb, int use_vol,
-                        hipStream_t s);
+                        int nv, hipStream_t s);
 // (entries of the same node, which may sit in several groups, summed per node
 // in order: node u owns ent[ustart[u] .. ustart[u+1]-1])
 int launch_npp_gather_t(const int *pts, const int *ustart, const int *ent, int nuniq, const DBlock *blocks,
-                        const double *vol, double *lam, long total, const double *bufb, int use_vol, hipStream_t s);
-// ghost nodes of 5 arrays to zero
-int launch_zero_ghosts5(const DBlock &b, double *arr, long total, hipStream_t s);
+                        const double *vol, double *lam, long total, const double *bufb, int use_vol, int nv,
+                        hipStream_t s);
+// ghost nodes of nv arrays to zero
+int launch_zero_ghosts(const DBlock &b, double *arr, long total, int nv, hipStream_t s);
 
 int launch_vis_node_deriv(const BlockGrid &G, const DFields &F, double *dn, hipStream_t s);
 int launch_rhs_vis_dir(int dir, const BlockGrid &G, const DFields &F, const DParams &P, const double *dn, hipStream_t s);

@@ -1183,11 +1183,15 @@ struct TSweep {
     double *xb[4][5];        // adjoints of their outputs
     double *rb[5];           // adjoint of R (accumulated)
     double *cb;              // adjoint of coed (accumulated)
+    double *lv[3];           // adjoint of the viscous radii (accumulated; LU, nvis = 1)
 };
 
 // One hyperplane of the transposed sweep KIND (processed in the opposite
 // order to the sweep): z_i = xbar_i +- sum_d A(i,d)^T mu_{i-+e_d}, mu_i = c_i z_i,
-// and the adjoints of R, coed and (lusgs_u) x0 that z carries.
+// and the adjoints of R, coed and (lusgs_u) x0 that z carries.  With viscous
+// radii rv the LU sweeps add 0.5 rv_d(n) x_n for each neighbour n (lusgs_l)
+// or subtract it inside the product with c (lusgs_u): z also takes
+// 0.5 rv_d(i) mu_{i-+e_d}, and rv_d(i) the adjoint 0.5 mu_{i-+e_d} . x_i.
 template <int KIND>
 __global__ void __launch_bounds__(256) k_ad_tsweep(DBlock b, Operands<double> O, TSweep S, DParams P, int s)
 {
@@ -1218,6 +1222,17 @@ __global__ void __launch_bounds__(256) k_ad_tsweep(DBlock b, Operands<double> O,
 #pragma unroll
         for (int m = 0; m < 5; m++) mu[m] = S.mu[KIND][m][gn];
         mxdq_apply_t(cf, mu, tg);
+        if (KIND < 2 && P.nvis > 0) {
+            const double rv = d == 0 ? O.srv[0][g] : (d == 1 ? O.srv[1][g] : O.srv[2][g]);
+            double lv = 0.0;
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                tg[m] = KIND == 0 ? tg[m] + 0.5 * rv * mu[m] : tg[m] - 0.5 * rv * mu[m];
+                lv = lv + mu[m] * S.x[KIND][m][g];
+            }
+            double *lvd = d == 0 ? S.lv[0] : (d == 1 ? S.lv[1] : S.lv[2]);
+            lvd[g] = lvd[g] + 0.5 * lv;
+        }
 #pragma unroll
         for (int m = 0; m < 5; m++) z[m] = up ? z[m] + tg[m] : z[m] - tg[m];
     }
@@ -1367,7 +1382,7 @@ Operands<double> tan_operands(const DFields &F, const AdSweep &A)
     for (int d = 0; d < 3; d++) {
         for (int c = 0; c < 3; c++) O.m[d][c] = A.zero;
         O.sr[d] = A.tsr + d * T;
-        O.srv[d] = A.zero;
+        O.srv[d] = A.tsrv ? A.tsrv + d * T : A.zero;
     }
     O.coed = A.tcoed;
     return O;
@@ -1423,6 +1438,7 @@ int launch_ad_sweeps_rev(const DBlock &b, const DFields &F, const DParams &P, co
         }
     for (int m = 0; m < 5; m++) S.rb[m] = A.rb + m * T;
     S.cb = A.cb;
+    for (int d = 0; d < 3; d++) S.lv[d] = A.lsrv + d * T;
     for (int p = smin; p <= smax; p++) hipLaunchKernelGGL(k_ad_tsweep<3>, g, bl, 0, s, b, O, S, P, p);
     hipLaunchKernelGGL(k_ad_tgather<3>, gn, bl, 0, s, b, O, S, P);
     for (int p = smax; p >= smin; p--) hipLaunchKernelGGL(k_ad_tsweep<2>, g, bl, 0, s, b, O, S, P, p);

@@ -104,7 +104,7 @@ def load_library(path=LIBPATH):
     L.wcns_adj_jvp.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp, vp]
     L.wcns_adj_vjp.argtypes = [vp, C.c_int, vp, vp, vp, vp, vp]
     L.wcns_adj_force.argtypes = [vp, C.c_int, vp, vp, dptr, dptr, vp, vp]
-    L.wcns_pool_layout.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+    L.wcns_pool_layout.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long), iptr, iptr]
     L.wcns_adj_step_jvp.argtypes = [vp, vp, vp, vp, vp]
     L.wcns_adj_step_vjp.argtypes = [vp, vp, vp, vp]
     L.wcns_adj_multistep.argtypes = [vp, vp, C.c_int, C.c_int, dptr, C.c_double, dptr, vp, dptr]
@@ -335,14 +335,23 @@ class Solver:
         self._check(self.L.wcns_adj_force(self.h, nb, P, X, _d(om), _d(J), gP or None, gX or None), "adj_force")
         return float(J[0])
 
-    # ---- tangent and adjoint of a whole (inviscid) step over the field pool:
-    # 5 arrays of `total` elements, block nb's ghosted nodes from offsets[nb]
+    # ---- tangent and adjoint of a whole step over the field pool: arrays of
+    # `total` elements, block nb's ghosted nodes from offsets[nb]; nstate of
+    # them carry the state, the first nvar are differentiated
     def pool_layout(self):
         total = C.c_long()
         offs = np.zeros(len(self.case.blocks), dtype=np.int64)
-        self._check(self.L.wcns_pool_layout(self.h, C.byref(total), offs.ctypes.data_as(C.POINTER(C.c_long))),
-                    "pool_layout")
+        ns, nv = C.c_int(), C.c_int()
+        self._check(self.L.wcns_pool_layout(self.h, C.byref(total), offs.ctypes.data_as(C.POINTER(C.c_long)),
+                                            C.byref(ns), C.byref(nv)), "pool_layout")
         return total.value, offs
+
+    def state_layout(self):
+        """(nstate, nvar) of the step's state (wcns_pool_layout)."""
+        total = C.c_long()
+        ns, nv = C.c_int(), C.c_int()
+        self._check(self.L.wcns_pool_layout(self.h, C.byref(total), None, C.byref(ns), C.byref(nv)), "pool_layout")
+        return ns.value, nv.value
 
     def adj_step_jvp(self, P, dP, Pn=0, dPn=0):
         self._check(self.L.wcns_adj_step_jvp(self.h, P or None, dP, Pn or None, dPn or None), "adj_step_jvp")
