@@ -27,8 +27,9 @@ bump-euler (the bump channel at wall spacing 1e-3 and growth 1.08, Euler,
 with a slip top face instead of far field: the steady workload of --converge),
 bump-slip (the laminar bump channel with the slip top), mb64-sa (64
 uneven blocks, sides 24-96, SA, about 30 M nodes, inflow / outflow in i) and
-mb64-sst (sides 30-120, SST, Re 1e7, M 0.85); use --no-cpu for the 64-block
-cases.
+mb64-sst (sides 30-120, SST, Re 1e7, M 0.85) and mb64-sst30 (the same at
+sides 24-96, about 30 M nodes: the size whose multi-step adjoint fits one GPU
+beside the solver); use --no-cpu for the 64-block cases.
 """
 import argparse
 import json
@@ -129,6 +130,11 @@ def build_case(workload):
     if workload == "bump-slip":
         return cases.bump_channel_case(ni=256, nj=128, nk=64, top_bc=3), \
             "bump-channel 256x128x64 laminar Re1e5 M0.5 Roe WCNS-E5, slip top"
+    if workload == "mb64-sst30":
+        case = cases.uneven_blocks_case(nblk=(4, 4, 4), lo=24, hi=96, seed=7, nvis=1, wall=True, nflux=3,
+                                        reynolds=1.0e7, moo=0.85, periodic_i=False, ilo_bc=5, ihi_bc=6)
+        case.params.nlamtur = 2
+        return case, f"64 uneven blocks (sides 24-96, seed 7) {case.nodes} nodes, SST Re1e7 M0.85 Roe WCNS-E5"
     if workload == "cart":
         return cases.cartesian_periodic_case(ni=256, nj=128, nk=64), \
             "cartesian-periodic 256x128x64 Euler M0.8 SW WCNS-E5"
