@@ -93,6 +93,13 @@ long sweep_exchange_words(int TJ, int TK, int nTJ, int nTK, int ni);
 // An exchange word nobody has written yet: a signalling-NaN pattern that no
 // arithmetic result has.  Every slot is reset to it before each sweep.
 constexpr unsigned long long XCH_EMPTY = 0x7ff4deadbeef0001ull;
+// Exchange words seen as global memory.  The slot pointers come from structs
+// in memory, whose address space the compiler cannot infer: through them the
+// loads and stores would be flat instructions, which also count in the LDS
+// counter that every workgroup barrier drains, so each step would wait for
+// its own exchange traffic to reach memory.
+typedef __attribute__((address_space(1))) unsigned long long xch_word;
+__device__ inline xch_word *xch_ptr(double *slot) { return (xch_word *)(slot); }
 int launch_fill_xch(const SweepPlan &S, hipStream_t s);
 // fills n words with v by a kernel (no memset node, so it may sit in a captured graph)
 int launch_fill_u64(unsigned long long *a, long n, unsigned long long v, hipStream_t s);

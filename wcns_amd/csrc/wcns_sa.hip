@@ -446,20 +446,20 @@ __global__ void __launch_bounds__(128) k_sst_rhs(BlockGrid G, DFields F, SaWork 
 // ------------------------------------------------------------ the sweeps
 __device__ inline void xch_put1(double *slot, double v)
 {
-    __hip_atomic_store(reinterpret_cast<unsigned long long *>(slot), (unsigned long long)__double_as_longlong(v),
+    __hip_atomic_store(xch_ptr(slot), (unsigned long long)__double_as_longlong(v),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ inline unsigned long long xch_issue1(double *slot)
 {
-    return __hip_atomic_load(reinterpret_cast<unsigned long long *>(slot), __ATOMIC_RELAXED,
+    return __hip_atomic_load(xch_ptr(slot), __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Bounded wait for a slot read issued earlier (see the five-variable sweeps).
 __device__ inline bool xch_finish1(double *slot, unsigned long long raw, double &v, long spin_limit, int *err)
 {
-    unsigned long long *p = reinterpret_cast<unsigned long long *>(slot);
+    xch_word *p = xch_ptr(slot);
     long spins = 0;
     unsigned long long x = raw;
     while (x == XCH_EMPTY) {

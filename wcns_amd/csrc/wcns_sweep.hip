@@ -337,7 +337,7 @@ __global__ void k_fill_u64(unsigned long long *a, long n, unsigned long long v)
 template <typename T>
 __device__ inline void xch_put(double *slot, const T *v)
 {
-    unsigned long long *p = reinterpret_cast<unsigned long long *>(slot);
+    xch_word *p = xch_ptr(slot);
 #pragma unroll
     for (int m = 0; m < 5; m++)
         __hip_atomic_store(p + m, (unsigned long long)__double_as_longlong((double)v[m]), __ATOMIC_RELAXED,
@@ -346,59 +346,70 @@ __device__ inline void xch_put(double *slot, const T *v)
 
 __device__ inline void xch_issue(double *slot, unsigned long long *raw)
 {
-    unsigned long long *p = reinterpret_cast<unsigned long long *>(slot);
+    xch_word *p = xch_ptr(slot);
 #pragma unroll
     for (int m = 0; m < 5; m++) raw[m] = __hip_atomic_load(p + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Complete a slot read issued earlier: re-read the words still empty.  The
-// wait is bounded: after spin_limit re-reads, or once any lane of the launch
-// has given up (err set), it stops and reports failure.
+// Complete a slot read issued earlier.  The common case -- all five words
+// already written when they were read -- is one test on the registers, so that
+// the only wait is for those five loads; the words still empty are re-read
+// one by one.  The wait is bounded: after spin_limit re-reads, or once any
+// lane of the launch has given up (err set), it stops and reports failure.
 template <typename T>
 __device__ inline bool xch_finish(double *slot, unsigned long long *raw, T *v, long spin_limit, int *err)
 {
-    unsigned long long *p = reinterpret_cast<unsigned long long *>(slot);
-    long spins = 0;
-    bool ok = true;
+    bool ready = true;
 #pragma unroll
-    for (int m = 0; m < 5; m++) {
-        unsigned long long x = raw[m];
-        while (x == XCH_EMPTY && ok) {
-            if (++spins > spin_limit ||
-                ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-                ok = false;
-                break;
+    for (int m = 0; m < 5; m++) ready = ready && raw[m] != XCH_EMPTY;
+    bool ok = true;
+    if (!ready) {
+        xch_word *p = xch_ptr(slot);
+        long spins = 0;
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            while (raw[m] == XCH_EMPTY && ok) {
+                if (++spins > spin_limit ||
+                    ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                    ok = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                raw[m] = __hip_atomic_load(p + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            __builtin_amdgcn_s_sleep(1);
-            x = __hip_atomic_load(p + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        v[m] = (T)__longlong_as_double((long long)x);
     }
+#pragma unroll
+    for (int m = 0; m < 5; m++) v[m] = (T)__longlong_as_double((long long)raw[m]);
     return ok;
 }
 
 constexpr int PIPE_COLS = 64;                  // tile columns = lanes of the chain wavefront
-constexpr int PIPE_MAX_THREADS = 7 * PIPE_COLS;  // chain + up to six helper wavefronts
+constexpr int PIPE_MAX_THREADS = 8 * PIPE_COLS;  // chain, up to six helper wavefronts, exchange
 
 // Operands of one tile's next hyperplane, formed by the helper wavefronts:
 // two slots, structure of arrays over the tile's columns.  The 13 stored
 // fields of a coefficient set are MxCoef without ae (= gama - 1).  Each sweep
 // kind sizes only what it reads (a one-element stand-in otherwise): R for all
 // but lusgs_u, which reads its own dQ instead; the dQ outside the block and
-// the viscous radii for LU; the previous-sweep terms for SGS.  In FP64 the LU
-// kinds then take 70 KB, so that two workgroups fit in a CU's 160 KB.
+// the viscous radii for LU; the previous-sweep terms for SGS.  An array a
+// kind does not read is a single element.  The exchange wavefront puts the
+// upstream tiles' results for the tile's first row and column into xo.  In
+// FP64 every kind then takes at most 79 KB, so that two workgroups fit in a
+// CU's 160 KB.
+template <bool USE, typename A, typename T>
+using LdsArr = std::conditional_t<USE, A, T[1]>;
 template <int KIND, typename T>
 struct PipeLds {
-    static constexpr int NR = KIND == 1 ? 1 : 5, NO = KIND == 1 ? 5 : 1;
-    static constexpr int NG = KIND < 2 ? 3 : 1, NF = KIND < 2 ? 1 : 3, NV = KIND < 2 ? 3 : 1;
     T cn[2][3][13][PIPE_COLS];
-    T fold[2][NF][5][PIPE_COLS];
-    T rhs[2][NR][PIPE_COLS];
+    LdsArr<(KIND >= 2), T[2][3][5][PIPE_COLS], T> fold;
+    LdsArr<(KIND != 1), T[2][5][PIPE_COLS], T> rhs;
     T coed[2][PIPE_COLS];
-    T ghost[2][NG][5][PIPE_COLS];
-    T own[2][NO][PIPE_COLS];
-    T rv[2][NV][PIPE_COLS];
+    LdsArr<(KIND < 2), T[2][3][5][PIPE_COLS], T> ghost;
+    LdsArr<(KIND == 1), T[2][5][PIPE_COLS], T> own;
+    LdsArr<(KIND < 2), T[2][3][PIPE_COLS], T> rv;
     T sh[2][5][PIPE_COLS];       // this step's results, for the j and k neighbours in the tile
+    T xo[2][2][5][PIPE_COLS];    // the j and k neighbours' results from the upstream tiles
 };
 
 template <int KIND, typename T>
@@ -631,33 +642,26 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
     double *xj_out = sb.xj + ((long)tile * TK + kl) * rowlen;
     double *xk_out = sb.xk + ((long)tile * TJ + jl) * rowlen;
     const TPtrs<T> W = KIND == 2 ? B : A;
-    T prev[5] = {T(0), T(0), T(0), T(0), T(0)};
-    unsigned long long rj[5], rk[5];
-    bool bad = false;                 // a bounded wait of this lane expired: stop waiting
 
-    // helper state: the raw operands of the hyperplane after next
-    TaskRaw<T> raw, raw2;
-    const bool helper = p >= 0 && p < H;
-    if (helper) {
-        if (live(0)) {
-            task_load<KIND, T>(p, b, O, ix, A, B, P, i_at(0), j, k, raw);
-            if constexpr (PAIR) task_load<KIND, T>(p + 3, b, O, ix, A, B, P, i_at(0), j, k, raw2);
-            task_prep<KIND, T>(p, P, i_at(0), j, k, b, raw, L, 0, c);
-            if constexpr (PAIR) task_prep<KIND, T>(p + 3, P, i_at(0), j, k, b, raw2, L, 0, c);
-        }
-        if (live(1)) {
-            task_load<KIND, T>(p, b, O, ix, A, B, P, i_at(1), j, k, raw);
-            if constexpr (PAIR) task_load<KIND, T>(p + 3, b, O, ix, A, B, P, i_at(1), j, k, raw2);
-        }
-    } else if (p < 0 && live(0)) {
-        int i = i_at(0);
-        if (jsrc == 1) xch_issue(xj_in + (long)(i - 1) * 5, rj);
-        if (ksrc == 1) xch_issue(xk_in + (long)(i - 1) * 5, rk);
-    }
-    __syncthreads();
-    for (int step = 0; step < nsteps; step++) {
-        const int slot = step & 1;
-        if (p < 0) {
+    // Roles, each a whole wavefront with a loop of its own (so that one
+    // role's registers are free in the others'); every wavefront meets
+    // 1 + nsteps barriers.
+    //   chain     (p < 0): updates the tile's node of each hyperplane from
+    //             LDS only -- it issues no global load, so it never waits for
+    //             memory (on gfx950 a wait for a load also waits for every
+    //             older store, the chain's exchange stores included);
+    //   helpers   (0 <= p < H): form the operands of the next hyperplane;
+    //   exchange  (p == H): completes the upstream tiles' words for the next
+    //             hyperplane into xo, reading them two steps ahead.
+    // Everything read from memory is issued two steps before it is used.  The
+    // loops are unrolled by two so that each of two register sets keeps its
+    // own registers (a copy from one set to the other would wait for the
+    // loads in flight).
+    if (p < 0) {
+        T prev[5] = {T(0), T(0), T(0), T(0), T(0)};
+        __syncthreads();
+        for (int step = 0; step < nsteps; step++) {
+            const int slot = step & 1;
             if (live(step)) {
                 const int i = i_at(step);
                 T nw[3][5];
@@ -668,29 +672,12 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
 #pragma unroll
                     for (int m = 0; m < 5; m++) nw[0][m] = ghost_dq(L, slot, 0, m, c);
                 }
-                if (jsrc == 0) {
 #pragma unroll
-                    for (int m = 0; m < 5; m++) nw[1][m] = L.sh[slot ^ 1][m][c + off];
-                } else if (jsrc == 1) {
-                    if (!xch_finish<T>(xj_in + (long)(i - 1) * 5, rj, nw[1], bad ? 0 : spin_limit, err)) {
-                        if (!bad) atomicExch(err, 1);
-                        bad = true;
-                    }
-                } else {
-#pragma unroll
-                    for (int m = 0; m < 5; m++) nw[1][m] = ghost_dq(L, slot, 1, m, c);
-                }
-                if (ksrc == 0) {
-#pragma unroll
-                    for (int m = 0; m < 5; m++) nw[2][m] = L.sh[slot ^ 1][m][c + off * TJ];
-                } else if (ksrc == 1) {
-                    if (!xch_finish<T>(xk_in + (long)(i - 1) * 5, rk, nw[2], bad ? 0 : spin_limit, err)) {
-                        if (!bad) atomicExch(err, 1);
-                        bad = true;
-                    }
-                } else {
-#pragma unroll
-                    for (int m = 0; m < 5; m++) nw[2][m] = ghost_dq(L, slot, 2, m, c);
+                for (int m = 0; m < 5; m++) {
+                    nw[1][m] = jsrc == 0 ? L.sh[slot ^ 1][m][c + off]
+                                         : (jsrc == 1 ? L.xo[slot][0][m][c] : ghost_dq(L, slot, 1, m, c));
+                    nw[2][m] = ksrc == 0 ? L.sh[slot ^ 1][m][c + off * TJ]
+                                         : (ksrc == 1 ? L.xo[slot][1][m][c] : ghost_dq(L, slot, 2, m, c));
                 }
                 T out[5];
                 chain_compute<KIND, T>(P, L, slot, c, nw, out);
@@ -704,22 +691,102 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
                     W.c[m][g] = out[m];
                 }
             }
-            if (live(step + 1)) {
-                const int i = i_at(step + 1);
+            __syncthreads();
+        }
+    } else if (p == H) {
+        bool bad = false;             // a bounded wait of this lane expired: stop waiting
+        unsigned long long rjA[5], rkA[5], rjB[5], rkB[5];
+        auto xissue = [&](int st, unsigned long long (&rj)[5], unsigned long long (&rk)[5]) {
+            if (live(st)) {
+                const int i = i_at(st);
                 if (jsrc == 1) xch_issue(xj_in + (long)(i - 1) * 5, rj);
                 if (ksrc == 1) xch_issue(xk_in + (long)(i - 1) * 5, rk);
             }
-        } else if (helper) {
-            if (live(step + 1)) {
-                task_prep<KIND, T>(p, P, i_at(step + 1), j, k, b, raw, L, slot ^ 1, c);
-                if constexpr (PAIR) task_prep<KIND, T>(p + 3, P, i_at(step + 1), j, k, b, raw2, L, slot ^ 1, c);
+        };
+        // complete the words of hyperplane st into LDS slot st & 1
+        auto xfinish = [&](int st, unsigned long long (&rj)[5], unsigned long long (&rk)[5]) {
+            if (live(st)) {
+                const int i = i_at(st), sl = st & 1;
+                T v[5];
+                if (jsrc == 1) {
+                    if (!xch_finish<T>(xj_in + (long)(i - 1) * 5, rj, v, bad ? 0 : spin_limit, err)) {
+                        if (!bad) atomicExch(err, 1);
+                        bad = true;
+                    }
+#pragma unroll
+                    for (int m = 0; m < 5; m++) L.xo[sl][0][m][c] = v[m];
+                }
+                if (ksrc == 1) {
+                    if (!xch_finish<T>(xk_in + (long)(i - 1) * 5, rk, v, bad ? 0 : spin_limit, err)) {
+                        if (!bad) atomicExch(err, 1);
+                        bad = true;
+                    }
+#pragma unroll
+                    for (int m = 0; m < 5; m++) L.xo[sl][1][m][c] = v[m];
+                }
             }
-            if (live(step + 2)) {
-                task_load<KIND, T>(p, b, O, ix, A, B, P, i_at(step + 2), j, k, raw);
-                if constexpr (PAIR) task_load<KIND, T>(p + 3, b, O, ix, A, B, P, i_at(step + 2), j, k, raw2);
-            }
-        }
+        };
+        xissue(0, rjA, rkA);
+        xfinish(0, rjA, rkA);
+        xissue(1, rjB, rkB);
+        xissue(2, rjA, rkA);
         __syncthreads();
+        auto xstep = [&](int step, unsigned long long (&rj)[5], unsigned long long (&rk)[5]) {
+            xfinish(step + 1, rj, rk);
+            xissue(step + 3, rj, rk);
+            __syncthreads();
+        };
+        for (int step = 0; step < nsteps; step += 2) {
+            xstep(step, rjB, rkB);
+            if (step + 1 < nsteps) xstep(step + 1, rjA, rkA);
+        }
+    } else {
+        // One loop per helper part, the part a compile-time constant: a part
+        // chosen at run time made the compiler keep the loaded operands in a
+        // private array behind a part-dependent offset, which waits for every
+        // load as soon as it is issued.
+        auto helper_loop = [&](auto PC) {
+            constexpr int PP = decltype(PC)::value;
+            TaskRaw<T> rawA, rawB, raw2A, raw2B;
+            auto hload = [&](int st, TaskRaw<T> &r, TaskRaw<T> &r2) {
+                if (live(st)) {
+                    task_load<KIND, T>(PP, b, O, ix, A, B, P, i_at(st), j, k, r);
+                    if constexpr (PAIR) task_load<KIND, T>(PP + 3, b, O, ix, A, B, P, i_at(st), j, k, r2);
+                }
+            };
+            auto hprep = [&](int st, const TaskRaw<T> &r, const TaskRaw<T> &r2, int sl) {
+                if (live(st)) {
+                    task_prep<KIND, T>(PP, P, i_at(st), j, k, b, r, L, sl, c);
+                    if constexpr (PAIR) task_prep<KIND, T>(PP + 3, P, i_at(st), j, k, b, r2, L, sl, c);
+                }
+            };
+            hload(0, rawA, raw2A);
+            hprep(0, rawA, raw2A, 0);
+            hload(1, rawB, raw2B);
+            hload(2, rawA, raw2A);
+            __syncthreads();
+            // form hyperplane step + 1 from r/r2, reload them with hyperplane step + 3
+            auto helper_step = [&](int step, TaskRaw<T> &r, TaskRaw<T> &r2) {
+                hprep(step + 1, r, r2, (step & 1) ^ 1);
+                hload(step + 3, r, r2);
+                __syncthreads();
+            };
+            for (int step = 0; step < nsteps; step += 2) {
+                helper_step(step, rawB, raw2B);
+                if (step + 1 < nsteps) helper_step(step + 1, rawA, raw2A);
+            }
+        };
+        switch (p) {
+        case 0: helper_loop(std::integral_constant<int, 0>{}); break;
+        case 1: helper_loop(std::integral_constant<int, 1>{}); break;
+        case 2: helper_loop(std::integral_constant<int, 2>{}); break;
+        case 3: if constexpr (H > 3) helper_loop(std::integral_constant<int, 3>{}); break;
+        case 4: if constexpr (H > 4) helper_loop(std::integral_constant<int, 4>{}); break;
+        case 5: if constexpr (H > 5) helper_loop(std::integral_constant<int, 5>{}); break;
+        default:
+            for (int step = 0; step <= nsteps; step++) __syncthreads();
+            break;
+        }
     }
 }
 
@@ -1000,7 +1067,7 @@ static int launch_pipe(int kind, const SweepPlan &S, const DFields &F, const DPa
     }
     const int2 *work = (kind == 0 || kind == 2) ? S.work_fwd : S.work_bwd;
     const bool pair = kind >= 2 && S.sgs_paired;
-    dim3 g(S.nwork), bl(PIPE_COLS * ((kind < 2 || pair) ? 4 : 7));
+    dim3 g(S.nwork), bl(PIPE_COLS * ((kind < 2 || pair) ? 5 : 8));
     switch (kind) {
     case 0: hipLaunchKernelGGL((k_sweep_pipe<0, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, P, S.err, S.spin_limit); break;
     case 1: hipLaunchKernelGGL((k_sweep_pipe<1, T, false>), g, bl, 0, s, S.blocks, work, O, A, B, P, S.err, S.spin_limit); break;
