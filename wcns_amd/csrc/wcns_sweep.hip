@@ -538,48 +538,76 @@ __device__ inline T ghost_dq(const PipeLds<KIND, T> &L, int slot, int d, int m, 
     else return T(0.0);
 }
 
-// node_compute with the operands read from the LDS slot as they are needed
-// (the same operations in the same order).
+// The chain's operands of one node, read from the LDS slot all at once
+// before any arithmetic, so that their latencies overlap.
 template <int KIND, typename T>
-__device__ inline void chain_compute(const DParams &P, const PipeLds<KIND, T> &L, int slot, int c, T (&nw)[3][5],
-                                     T *out)
+struct ChainIn {
+    MxCoef<T> x[3];
+    T coed, rhs[5], own[5], rv[3], fold[3][5];
+};
+
+template <int KIND, typename T>
+__device__ inline void chain_read(const DParams &P, const PipeLds<KIND, T> &L, int slot, int c, ChainIn<KIND, T> &q)
 {
     const T gama = (T)P.gama;
+#pragma unroll
+    for (int d = 0; d < 3; d++) coef_get(L, slot, d, c, gama, q.x[d]);
+    q.coed = L.coed[slot][c];
+    if constexpr (KIND != 1) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) q.rhs[m] = L.rhs[slot][m][c];
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; m++) q.own[m] = L.own[slot][m][c];
+    }
+    if constexpr (KIND < 2) {
+        if (P.nvis > 0) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) q.rv[d] = L.rv[slot][d][c];
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int m = 0; m < 5; m++) q.fold[d][m] = L.fold[slot][d][m][c];
+    }
+}
+
+// node_compute on the operands of chain_read (the same operations in the
+// same order).
+template <int KIND, typename T>
+__device__ inline void chain_math(const DParams &P, const ChainIn<KIND, T> &q, T (&nw)[3][5], T *out)
+{
     T f[3][5];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        MxCoef<T> x;
-        coef_get(L, slot, d, c, gama, x);
-        mxdq_apply<T>(x, nw[d], f[d]);
-    }
-    const T coed = L.coed[slot][c];
+    for (int d = 0; d < 3; d++) mxdq_apply<T>(q.x[d], nw[d], f[d]);
+    const T coed = q.coed;
     if constexpr (KIND < 2) {
         T rhs0[5];
 #pragma unroll
         for (int m = 0; m < 5; m++) rhs0[m] = T(1.0) * (f[0][m] + f[1][m] + f[2][m]);
         if (P.nvis > 0) {
-            const T rv0 = L.rv[slot][0][c], rv1 = L.rv[slot][1][c], rv2 = L.rv[slot][2][c];
 #pragma unroll
             for (int m = 0; m < 5; m++) {
-                T v = T(0.5) * (rv0 * nw[0][m] + rv1 * nw[1][m] + rv2 * nw[2][m]);
+                T v = T(0.5) * (q.rv[0] * nw[0][m] + q.rv[1] * nw[1][m] + q.rv[2] * nw[2][m]);
                 rhs0[m] = KIND == 0 ? rhs0[m] + v : rhs0[m] - v;
             }
         }
 #pragma unroll
         for (int m = 0; m < 5; m++) {
-            if constexpr (KIND == 0) out[m] = (-L.rhs[slot][m][c] + rhs0[m]) * coed;
-            else out[m] = L.own[slot][m][c] - rhs0[m] * coed;
+            if constexpr (KIND == 0) out[m] = (-q.rhs[m] + rhs0[m]) * coed;
+            else out[m] = q.own[m] - rhs0[m] * coed;
         }
     } else {
 #pragma unroll
         for (int m = 0; m < 5; m++) {
-            const T fo0 = L.fold[slot][0][m][c], fo1 = L.fold[slot][1][m][c], fo2 = L.fold[slot][2][m][c];
+            const T fo0 = q.fold[0][m], fo1 = q.fold[1][m], fo2 = q.fold[2][m];
             T r0;
             if (KIND == 2)
                 r0 = T(1.0) * (f[0][m] + f[1][m] + f[2][m] - fo0 - fo1 - fo2);
             else
                 r0 = T(1.0) * (fo0 + fo1 + fo2 - f[0][m] - f[1][m] - f[2][m]);
-            out[m] = (r0 - L.rhs[slot][m][c]) * coed;
+            out[m] = (r0 - q.rhs[m]) * coed;
         }
     }
 }
@@ -659,28 +687,32 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
     // loads in flight).
     if (p < 0) {
         T prev[5] = {T(0), T(0), T(0), T(0), T(0)};
+        // where the j and k neighbours' dQ come from, fixed for the lane: LDS
+        // index of the in-tile neighbour (or the lane's own, when it reads xo
+        // or the block's ghost instead), so that every read below is
+        // unconditional and only the choice is made per lane
+        const int cj = jsrc == 0 ? c + off : c, ck = ksrc == 0 ? c + off * TJ : c;
         __syncthreads();
         for (int step = 0; step < nsteps; step++) {
             const int slot = step & 1;
             if (live(step)) {
                 const int i = i_at(step);
+                ChainIn<KIND, T> q;
+                chain_read<KIND, T>(P, L, slot, c, q);
+                const bool iin = i + off >= 1 && i + off <= ni;
                 T nw[3][5];
-                if (i + off >= 1 && i + off <= ni) {
-#pragma unroll
-                    for (int m = 0; m < 5; m++) nw[0][m] = prev[m];
-                } else {
-#pragma unroll
-                    for (int m = 0; m < 5; m++) nw[0][m] = ghost_dq(L, slot, 0, m, c);
-                }
 #pragma unroll
                 for (int m = 0; m < 5; m++) {
-                    nw[1][m] = jsrc == 0 ? L.sh[slot ^ 1][m][c + off]
-                                         : (jsrc == 1 ? L.xo[slot][0][m][c] : ghost_dq(L, slot, 1, m, c));
-                    nw[2][m] = ksrc == 0 ? L.sh[slot ^ 1][m][c + off * TJ]
-                                         : (ksrc == 1 ? L.xo[slot][1][m][c] : ghost_dq(L, slot, 2, m, c));
+                    const T g0 = ghost_dq(L, slot, 0, m, c), g1 = ghost_dq(L, slot, 1, m, c);
+                    const T g2 = ghost_dq(L, slot, 2, m, c);
+                    const T sj = L.sh[slot ^ 1][m][cj], sk = L.sh[slot ^ 1][m][ck];
+                    const T xj = L.xo[slot][0][m][c], xk = L.xo[slot][1][m][c];
+                    nw[0][m] = iin ? prev[m] : g0;
+                    nw[1][m] = jsrc == 0 ? sj : (jsrc == 1 ? xj : g1);
+                    nw[2][m] = ksrc == 0 ? sk : (ksrc == 1 ? xk : g2);
                 }
                 T out[5];
-                chain_compute<KIND, T>(P, L, slot, c, nw, out);
+                chain_math<KIND, T>(P, q, nw, out);
                 if (putj) xch_put<T>(xj_out + (long)(i - 1) * 5, out);
                 if (putk) xch_put<T>(xk_out + (long)(i - 1) * 5, out);
                 const long g = ix(i, j, k);
