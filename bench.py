@@ -78,9 +78,10 @@ SLOT_KERNEL = {"rhs_inv_i": "k_rhs_inv<3,", "rhs_inv_j": "k_rhs_inv<1,", "rhs_in
 PROFILE_TAG = {"bump": "bump256x128x64", "cart": "cart256x128x64"}
 
 
-def measured_traffic(workload, lhs, slot):
+def measured_traffic(workload, lhs, slot, sgs_paired=False):
     """HBM bytes per launch of the slot's kernel from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of this workload (profiles/), or None."""
+    FETCH_SIZE/WRITE_SIZE summary of this workload (profiles/), or None.  A
+    sweep slot names its exact instance (element type, paired SGS helpers)."""
     tag = PROFILE_TAG.get(workload)
     name = SLOT_KERNEL.get(slot)
     if not tag or not name:
@@ -88,12 +89,25 @@ def measured_traffic(workload, lhs, slot):
     path = os.path.join(REPO, "profiles", f"{tag}_{lhs}_hbm_traffic.md")
     if not os.path.exists(path):
         return None, None
+    kinds = {"sweep_lu_l": 0, "sweep_lu_u": 1, "sweep_gs_l": 2, "sweep_gs_u": 3}
+    exact = slot in kinds
+    if exact:
+        kind = kinds[slot]
+        name = (f"k_sweep_pipe<{kind},{'double' if lhs == 'fp64' else 'float'},"
+                f"{'true' if sgs_paired and kind >= 2 else 'false'}>")
     key = name.replace(" ", "")
+    col = None
     for line in open(path):
         cells = [c.strip().strip("`") for c in line.split("|")]
-        if len(cells) > 4 and cells[1].replace(" ", "").startswith(key):
+        if col is None and "HBM MB/launch (scaled)" in cells:
+            col = cells.index("HBM MB/launch (scaled)")
+            continue
+        if col is None or len(cells) <= col:
+            continue
+        k = cells[1].replace(" ", "")
+        if (k == key) if exact else k.startswith(key):
             try:
-                return float(cells[4]) * 1e6, os.path.relpath(path, REPO)
+                return float(cells[col]) * 1e6, os.path.relpath(path, REPO)
             except ValueError:
                 return None, None
     return None, None
@@ -276,7 +290,7 @@ def main():
         per_launch = alg_bytes_per_node(dom, lhs_bytes) * nodes * kt / dlaunch
         avg_s = dms / dlaunch / 1e3
         ach = per_launch / avg_s / 1e9
-        traffic, tsrc = measured_traffic(args.workload, args.lhs, dom)
+        traffic, tsrc = measured_traffic(args.workload, args.lhs, dom, bool(s.sweep_layout()[2]))
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc, "kernel": dom,
                 "alg_bytes_per_launch": per_launch, "avg_launch_ms": round(dms / dlaunch, 4)}
