@@ -177,6 +177,31 @@ def test_forced_abort_reports_repair_error():
     s.close()
 
 
+def test_expired_sweep_wait_reports_error(monkeypatch):
+    """The pipelined sweeps' waits for the upstream tiles' exchange words are
+    bounded.  With the bound set from the host to one re-read
+    (WCNS_SPIN_LIMIT=1) the downstream tiles give up at the start of the
+    sweep, every wavefront still runs to the end of it, and the step returns
+    WCNS_E_HIP; a context with the default bound then steps normally."""
+    from wcns_amd.solver import Solver, WcnsError
+    case = cases.cartesian_periodic_case(ni=32, nj=48, nk=24, seed=3)
+    monkeypatch.setenv("WCNS_SPIN_LIMIT", "1")
+    s = Solver(case)
+    print("sweep layout", s.sweep_layout(), flush=True)
+    with pytest.raises(WcnsError) as ei:
+        s.step(1)
+    assert ei.value.code == -3 and "wait expired" in str(ei.value)
+    s.close()
+    monkeypatch.delenv("WCNS_SPIN_LIMIT")
+    s2 = Solver(case)
+    o = _oracle(case)
+    hs = s2.step(2)
+    ho = o.step(2)
+    assert np.max(np.abs(hs - ho) / ho) <= 1e-12
+    o.close()
+    s2.close()
+
+
 def test_folded_grid_reports_jacobian_error():
     """A node moved past two of its i neighbours folds the cells around it:
     check_grid_derivative stops on the non-positive Jacobian

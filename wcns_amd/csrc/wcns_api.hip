@@ -115,6 +115,7 @@ struct wcns_ctx {
     long *d_skoff = nullptr;                // the same, on the device
     long sktot = 0;
     SweepPlan plan{};                       // batched pipelined sweeps over all blocks
+    long spin_limit = 2000000;              // re-reads before a sweep's wait gives up (WCNS_SPIN_LIMIT)
     bool sgs_forced = false;                // WCNS_SGS_PAIR fixes plan.sgs_paired; the autotuner keeps it
     std::vector<std::pair<int, int>> tile_shape;   // per block (TJ, TK)
     SweepBlock *d_sblocks = nullptr;
@@ -521,6 +522,7 @@ int wcns_create(wcns_ctx **out, int device, const wcns_params *prm)
     if (const char *id = getenv("WCNS_IDIR")) ctx->idir_t = strcmp(id, "direct") != 0;
     if (const char *sf = getenv("WCNS_SURFACE")) ctx->surf_levels = strcmp(sf, "sequence") != 0;
     if (const char *sp = getenv("WCNS_SGS_PAIR")) { ctx->plan.sgs_paired = atoi(sp) != 0; ctx->sgs_forced = true; }
+    if (const char *sl = getenv("WCNS_SPIN_LIMIT")) ctx->spin_limit = std::max(0L, atol(sl));
     if (const char *ss = getenv("WCNS_SA_STAGED")) ctx->sa_staged = atoi(ss) != 0;
     if (const char *iw = getenv("WCNS_INV_WAVES")) ctx->dp.inv_waves = atoi(iw) == 2 ? 2 : (atoi(iw) == 1 ? 1 : 0);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -672,7 +674,7 @@ static int build_sweep_plan(wcns_ctx *ctx)
     P.xch = ctx->d_xch;
     P.nxch = nx;
     P.err = ctx->d_sweep_err;
-    P.spin_limit = 2000000;
+    P.spin_limit = ctx->spin_limit;
     return 0;
 }
 
