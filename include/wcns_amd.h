@@ -201,6 +201,28 @@ int wcns_force_history(wcns_ctx *ctx, int max, double *out, long long *count);
 int wcns_save_flow(wcns_ctx *ctx, const char *flowname, int nout, double wholetime);
 int wcns_load_flow(wcns_ctx *ctx, const char *flowname, int t_from_file, int *nout, double *wholetime);
 
+/* Tecplot output of post_processing's nplot choices (PostProcess.f90:37-55),
+ * in the binary format (version 102) that the reference's mod_tecios writes
+ * itself (parallel.f90:1-173), to the file `path` (the caller names it as
+ * the reference does: tecname for the surfaces, its stem + "_wall.plt" /
+ * "_vol.plt"):
+ *   WCNS_TEC_SURFACE  tecout_bin_surface (PostProcess.f90:97-336): one zone
+ *                     per boundary region of type 2, 3, 6 (not 4, 5, 7x or
+ *                     interfaces); X Y Z R U V W P T M (+ vT1 (vT2) muT with
+ *                     a turbulence model), the velocities of a wall taken
+ *                     from the node inside it;
+ *   WCNS_TEC_WALL     tecout_bin_wall (:338-582): one zone per wall region;
+ *                     X Y Z cp, and with viscosity cfx cfy cfz cf qw (the
+ *                     traction -2 s_lr (tau n) Re, its magnitude and the heat
+ *                     flux, formed on the device).  The reference's qw reads a
+ *                     local cp it never sets (:356, 475); here cp is the gas
+ *                     model's 1/((gama-1) moo^2) (sub_tgh.f90:868);
+ *   WCNS_TEC_VOLUME   tecout_bin_volume (:584-772): one zone per block. */
+#define WCNS_TEC_SURFACE 0
+#define WCNS_TEC_WALL    1
+#define WCNS_TEC_VOLUME  2
+int wcns_tecplot(wcns_ctx *ctx, const char *path, int kind);
+
 /* device error/diagnostic counters: 0 WCNS positivity fallbacks
  * (WCNSE5.f90:2224-2245), 1 repaired nodes (Sub2.f90:661-767), 2 Roe c^2<=0 */
 int wcns_get_counter(wcns_ctx *ctx, int which, long long *value);

@@ -51,6 +51,7 @@ def lib():
         L.ref_step_stage.argtypes = [C.c_int]
         L.ref_step_residual.argtypes = [dptr, iptr]
         L.ref_step_force.argtypes = [dptr, C.c_int, C.c_char_p, dptr]
+        L.ref_step_tecout.argtypes = [C.c_int, C.c_char_p]
         _lib = L
     return _lib
 
@@ -148,6 +149,14 @@ class RefStep:
             fn = os.path.join(td, "force.dat").encode()
             assert lib().ref_step_force(_d(fref), prm.nwholefield, fn, _d(out)) == 0
         return out
+
+    def tecout(self, kind, tecname):
+        """The reference's Tecplot writer `kind` (0 surface, 1 wall, 2 volume)
+        on the loaded state, with tecname set to `tecname` (a path); returns
+        the path of the file it wrote."""
+        assert lib().ref_step_tecout(kind, tecname.encode()) == 0
+        stem = tecname[:tecname.rfind(".")] if "." in os.path.basename(tecname) else tecname
+        return tecname if kind == 0 else stem + ("_wall.plt" if kind == 1 else "_vol.plt")
 
     def turb_inflow(self):
         out = np.zeros(4)

@@ -292,6 +292,37 @@ integer(c_int) function ref_step_force(fref, nwhole, fname, out) bind(C, name="r
   ref_step_force = 0
 end function ref_step_force
 
+! The reference's Tecplot writers, on the loaded state: kind 0
+! tecout_bin_surface (PostProcess.f90:97-336, the file tecname), 1
+! tecout_bin_wall (:338-582, tecname's stem + "_wall.plt"), 2
+! tecout_bin_volume (:584-772, tecname's stem + "_vol.plt").
+integer(c_int) function ref_step_tecout(kind, fname) bind(C, name="ref_step_tecout")
+  use iso_c_binding
+  use global_variables, only: tecname
+  implicit none
+  integer(c_int), value, intent(in) :: kind
+  character(kind=c_char), intent(in) :: fname(*)
+  integer :: l
+  external :: tecout_bin_surface, tecout_bin_wall, tecout_bin_volume
+  tecname = ' '
+  l = 0
+  do while (fname(l + 1) /= c_null_char .and. l < len(tecname))
+    l = l + 1
+    tecname(l:l) = fname(l)
+  end do
+  ref_step_tecout = 0
+  select case (kind)
+  case (0)
+    call tecout_bin_surface
+  case (1)
+    call tecout_bin_wall
+  case (2)
+    call tecout_bin_volume
+  case default
+    ref_step_tecout = -1
+  end select
+end function ref_step_tecout
+
 ! The turbulence freestream values: muoo, koo, omgoo, vtoo.
 integer(c_int) function ref_step_turb_inflow(out) bind(C, name="ref_step_turb_inflow")
   use iso_c_binding

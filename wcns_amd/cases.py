@@ -415,7 +415,7 @@ def write_inp(case, path, nprocs=1):
 
 def write_param(case, path, gridname="grid.x", bcname="grid.inp", flowname="flow.dat",
                 forcename="force.dat", errhis="res.dat", nstart=1, nomax=100, nwerror=1, nforce=100,
-                ndisk=1000):
+                ndisk=1000, tecname="tec.plt", nplot=0):
     """param.dat with the namelist groups the hot path reads (PreProcess.f90:80-159),
     in the reference's $group ... / layout."""
     p = case.params
@@ -432,9 +432,11 @@ def write_param(case, path, gridname="grid.x", bcname="grid.inp", flowname="flow
                     ("lref", repr(p.lref)), ("xref", repr(p.xref)), ("yref", repr(p.yref)),
                     ("zref", repr(p.zref))]),
         g("filename", [("gridname", f"'{gridname}'"), ("bcname", f"'{bcname}'"), ("flowname", f"'{flowname}'"),
-                       ("forcename", f"'{forcename}'"), ("errhis", f"'{errhis}'")]),
+                       ("forcename", f"'{forcename}'"), ("errhis", f"'{errhis}'"),
+                       ("tecname", f"'{tecname}'")]),
         g("control", [("nstart", str(nstart)), ("nomax", str(nomax)), ("ndisk", str(ndisk)),
-                      ("nforce", str(nforce)), ("nwerror", str(nwerror)), ("method", "1")]),
+                      ("nforce", str(nforce)), ("nwerror", str(nwerror)), ("method", "1"),
+                      ("nplot", str(nplot))]),
         g("step", [("ntmst", str(p.ntmst)), ("cfl", repr(p.cfl)), ("timedt_rate", repr(p.timedt_rate)),
                    ("timedt_turb", repr(p.timedt_turb))]),
         g("flowtype", [("nvis", str(p.nvis)), ("nchem", "0"), ("ntmodel", "1")]),

@@ -1815,6 +1815,182 @@ int wcns_save_flow(wcns_ctx *ctx, const char *flowname, int nout, double wholeti
     return WCNS_OK;
 }
 
+// Tecplot binary (version 102) as the reference's mod_tecios writes it
+// (parallel.f90:1-173): every character as an int32, strings 0-terminated,
+// zone and end-of-header markers as float32, one ordered block-packed zone
+// per block or boundary region, all values as doubles.
+namespace {
+struct TecFile {
+    FILE *f;
+    bool ok = true;
+    void raw(const void *p, size_t n) { ok = ok && fwrite(p, 1, n, f) == n; }
+    void i32(int32_t v) { raw(&v, 4); }
+    void f32(float v) { raw(&v, 4); }
+    void str(const std::string &t)
+    {
+        for (unsigned char ch : t) i32(ch == ',' ? 0 : ch);   // tecio_ini turns the commas into separators
+        i32(0);
+    }
+};
+}  // namespace
+
+int wcns_tecplot(wcns_ctx *ctx, const char *path, int kind)
+{
+    if (!ctx || !path || kind < WCNS_TEC_SURFACE || kind > WCNS_TEC_VOLUME) return WCNS_E_ARG;
+    if (!ctx->setup_done) return WCNS_E_STATE;
+    const int nl = ctx->prm.nlamtur, nvis = ctx->prm.nvis;
+    std::string names;
+    int nv;
+    if (kind == WCNS_TEC_WALL) {
+        nv = nvis == 1 ? 9 : 4;
+        names = nvis == 1 ? "X,Y,Z,cp,cfx,cfy,cfz,cf,qw" : "X,Y,Z,cp";
+    } else {
+        nv = 10;
+        names = "X,Y,Z,R,U,V,W,P,T,M";
+        if (nl >= 0) {
+            nv += nl + 1;
+            names += nl == 0 ? ",muT" : (nl == 1 ? ",vT1,muT" : ",vT1,vT2,muT");
+        }
+    }
+    // zones: (block, region or -1 for the whole block)
+    std::vector<std::pair<int, int>> zones;
+    for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
+        if (kind == WCNS_TEC_VOLUME) { zones.push_back({(int)nb, -1}); continue; }
+        const auto &regs = ctx->blocks[nb].regs;
+        for (size_t nr = 0; nr < regs.size(); nr++) {
+            const int bt = regs[nr].bctype;
+            const bool draw = kind == WCNS_TEC_WALL ? (bt == 2 || bt == 20 || bt == 21)
+                                                   : (bt / 10 != 7 && bt != 4 && bt != 5 && bt > 0);
+            if (draw) zones.push_back({(int)nb, (int)nr});
+        }
+    }
+    auto box = [&](const std::pair<int, int> &z, int lo[3], int hi[3]) {
+        const HBlock &b = ctx->blocks[z.first];
+        if (z.second < 0) {
+            lo[0] = lo[1] = lo[2] = 1;
+            hi[0] = b.ni; hi[1] = b.nj; hi[2] = b.nk;
+        } else {
+            const DRegion &d = b.regs[z.second].d;
+            for (int m = 0; m < 3; m++) { lo[m] = d.s_lo[m]; hi[m] = d.s_hi[m]; }
+        }
+    };
+    TecFile T{fopen(path, "wb")};
+    if (!T.f) return fail(ctx, WCNS_E_ARG, std::string("cannot open ") + path);
+    T.raw("#!TDV102", 8);
+    T.i32(1);
+    T.str("WCNS_SOLVER");
+    T.i32(nv);
+    T.str(names);
+    for (auto &z : zones) {
+        int lo[3], hi[3];
+        box(z, lo, hi);
+        std::string zn = "BLK" + std::to_string(z.first + 1);
+        if (z.second >= 0)
+            zn += "BC" + std::to_string(z.second + 1) + "T" +
+                  std::to_string(ctx->blocks[z.first].regs[z.second].bctype);
+        T.f32(299.0f);
+        T.str(zn);
+        T.i32(-1);      // zone colour: Tecplot's choice
+        T.i32(0);       // ordered
+        T.i32(0);       // block packing
+        T.i32(0);       // values at the nodes
+        T.i32(0);       // no face connections
+        T.i32(hi[0] - lo[0] + 1);
+        T.i32(hi[1] - lo[1] + 1);
+        T.i32(hi[2] - lo[2] + 1);
+        T.i32(0);       // no auxiliary data
+    }
+    T.f32(357.0f);
+    const double gama = ctx->prm.gama;
+    std::vector<double> fl[6], qk, qw2, vt, wp, col;
+    for (auto &z : zones) {
+        const HBlock &b = ctx->blocks[z.first];
+        int lo[3], hi[3];
+        box(z, lo, hi);
+        T.f32(299.0f);
+        for (int m = 0; m < nv; m++) T.i32(2);   // doubles
+        T.i32(0);       // no shared variables
+        T.i32(-1);      // no shared connectivity
+        auto each = [&](auto fn) {
+            for (int k = lo[2]; k <= hi[2]; k++)
+                for (int j = lo[1]; j <= hi[1]; j++)
+                    for (int i = lo[0]; i <= hi[0]; i++) fn(i, j, k);
+        };
+        auto grid = [&](const std::vector<double> &a) {
+            col.clear();
+            each([&](int i, int j, int k) {
+                col.push_back(a[(size_t)(i - 1) + (size_t)b.ni * ((size_t)(j - 1) + (size_t)b.nj * (k - 1))]);
+            });
+            T.raw(col.data(), 8 * col.size());
+        };
+        grid(b.x);
+        grid(b.y);
+        grid(b.z);
+        if (kind == WCNS_TEC_WALL) {
+            const DRegion &d = b.regs[z.second].d;
+            double *dw = nullptr;
+            TRY(dalloc(ctx, &dw, 6 * (size_t)d.npts));
+            const double cpk = 1.0 / ((gama - 1.0) * ctx->prm.moo * ctx->prm.moo);
+            int e = launch_wall_plot(d, b.db, ctx->fields(), ctx->dp, cpk, dw, ctx->stream);
+            wp.resize(6 * (size_t)d.npts);
+            if (!e) e = copy_sync(ctx, wp.data(), dw, 8 * wp.size(), hipMemcpyDeviceToHost);
+            hipFree(dw);
+            if (e) { fclose(T.f); return fail(ctx, WCNS_E_HIP, "wall surface values"); }
+            // region_point numbers the face nodes i fastest, as the file does
+            for (int c = 0; c < (nvis == 1 ? 6 : 1); c++) T.raw(wp.data() + (size_t)c * d.npts, 8 * (size_t)d.npts);
+            continue;
+        }
+        for (int m = 0; m < 6; m++) {
+            int e = fetch(ctx, z.first, m, fl[m]);
+            if (e) { fclose(T.f); return e; }
+        }
+        if (nl >= 1) {
+            int e = fetch(ctx, z.first, 26, qk);
+            if (!e && nl == 2) e = fetch(ctx, z.first, 28, qw2);
+            if (!e) e = fetch(ctx, z.first, 15, vt);
+            if (e) { fclose(T.f); return e; }
+        }
+        // velocities of a wall surface from the node inside it (PostProcess.f90:233-249)
+        int ip[3] = {0, 0, 0};
+        if (kind == WCNS_TEC_SURFACE) {
+            const DRegion &d = b.regs[z.second].d;
+            const int bt = b.regs[z.second].bctype;
+            if (bt == 2 || bt == 20 || bt == 21) ip[d.s_nd - 1] = d.s_lr;
+        }
+        auto node = [&](const std::vector<double> &a, int i, int j, int k) { return a[gidx(b.db, i, j, k) - b.db.off]; };
+        auto field = [&](const std::vector<double> &a, bool inner) {
+            col.clear();
+            each([&](int i, int j, int k) {
+                col.push_back(inner ? node(a, i - ip[0], j - ip[1], k - ip[2]) : node(a, i, j, k));
+            });
+            T.raw(col.data(), 8 * col.size());
+        };
+        field(fl[0], false);
+        field(fl[1], true);
+        field(fl[2], true);
+        field(fl[3], true);
+        field(fl[4], false);
+        field(fl[5], false);
+        // M = sqrt(u2 / c2) of the node's own values (PostProcess.f90:686-689)
+        col.clear();
+        each([&](int i, int j, int k) {
+            const double r = node(fl[0], i, j, k), u = node(fl[1], i, j, k), v = node(fl[2], i, j, k);
+            const double w = node(fl[3], i, j, k), p = node(fl[4], i, j, k);
+            const double c2 = gama * p / r, u2 = u * u + v * v + w * w;
+            col.push_back(sqrt(u2 / c2));
+        });
+        T.raw(col.data(), 8 * col.size());
+        if (nl >= 1) {
+            field(qk, false);
+            if (nl == 2) field(qw2, false);
+            field(vt, false);
+        }
+    }
+    const bool ok = (fclose(T.f) == 0) && T.ok;
+    if (!ok) return fail(ctx, WCNS_E_ARG, std::string("writing ") + path + " failed");
+    return WCNS_OK;
+}
+
 int wcns_load_flow(wcns_ctx *ctx, const char *flowname, int t_from_file, int *nout, double *wholetime)
 {
     if (!ctx || !flowname) return WCNS_E_ARG;

@@ -80,6 +80,67 @@ __global__ void k_force_pn(DRegion g, DBlock b, DFields F, DParams P, double *pn
     pn3[2 * npts + t] = fz;
 }
 
+// tecout_bin_wall's values of one face node (PostProcess.f90:451-478):
+// cp = 2 (p - poo); for Navier-Stokes also the viscous traction
+// -2 s_lr (tau n) Re, its magnitude and the wall heat flux
+// -s_lr (mu cp / Pr) (n . grad T) Re.  cpk is the cp of that heat flux
+// (the reference reads a local there that it never sets; the caller passes
+// the gas model's 1/((gama-1) moo^2), sub_tgh.f90:868).  out: six arrays
+// of npts, cp, fnx, fny, fnz, cf, qw.
+__global__ void k_wall_plot(DRegion g, DBlock b, DFields F, DParams P, double cpk, double *out, int npts)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts) return;
+    int i, j, k;
+    region_point(g, t, i, j, k);
+    long q = gidx(b, i, j, k);
+    out[t] = 2.0 * (F.p[q] - P.poo);
+    if (P.nvis != 1) return;
+    double nx, ny, nz;
+    getnxyz(F, q, g.l3, nx, ny, nz);
+    const double *uvwt[4] = {F.u, F.v, F.w, F.t};
+    double dk[4], de[4], dc[4];
+    for (int m = 0; m < 4; m++) {
+        dk[m] = uvwt_der(uvwt[m], b, i, j, k, 0);
+        de[m] = uvwt_der(uvwt[m], b, i, j, k, 1);
+        dc[m] = uvwt_der(uvwt[m], b, i, j, k, 2);
+    }
+    double kx = F.kcx[q], ky = F.kcy[q], kz = F.kcz[q];
+    double ex = F.etx[q], ey = F.ety[q], ez = F.etz[q];
+    double cx = F.ctx[q], cy = F.cty[q], cz = F.ctz[q], vj = F.vol[q];
+    // Sub1.f90:251-270 getduvwdxyz
+    double dudx = (kx * dk[0] + ex * de[0] + cx * dc[0]) / vj;
+    double dudy = (ky * dk[0] + ey * de[0] + cy * dc[0]) / vj;
+    double dudz = (kz * dk[0] + ez * de[0] + cz * dc[0]) / vj;
+    double dvdx = (kx * dk[1] + ex * de[1] + cx * dc[1]) / vj;
+    double dvdy = (ky * dk[1] + ey * de[1] + cy * dc[1]) / vj;
+    double dvdz = (kz * dk[1] + ez * de[1] + cz * dc[1]) / vj;
+    double dwdx = (kx * dk[2] + ex * de[2] + cx * dc[2]) / vj;
+    double dwdy = (ky * dk[2] + ey * de[2] + cy * dc[2]) / vj;
+    double dwdz = (kz * dk[2] + ez * de[2] + cz * dc[2]) / vj;
+    // Sub1.f90:181-198 stress (tyx = txy, tzx = txz, tzy = tyz)
+    const double c23 = 0.66666666667;
+    double vis = F.visl[q];
+    double txx = vis * c23 * (2.0 * dudx - dvdy - dwdz);
+    double tyy = vis * c23 * (2.0 * dvdy - dwdz - dudx);
+    double tzz = vis * c23 * (2.0 * dwdz - dudx - dvdy);
+    double txy = vis * (dudy + dvdx), txz = vis * (dudz + dwdx), tyz = vis * (dvdz + dwdy);
+    double pn = 2.0 * g.s_lr;
+    double fnx = -pn * (nx * txx + ny * txy + nz * txz) * P.re;
+    double fny = -pn * (nx * txy + ny * tyy + nz * tyz) * P.re;
+    double fnz = -pn * (nx * txz + ny * tyz + nz * tzz) * P.re;
+    double dtdx = (kx * dk[3] + ex * de[3] + cx * dc[3]) / vj;
+    double dtdy = (ky * dk[3] + ey * de[3] + cy * dc[3]) / vj;
+    double dtdz = (kz * dk[3] + ez * de[3] + cz * dc[3]) / vj;
+    double kcp = vis * cpk / P.prl;
+    double qw = -(double)g.s_lr * kcp * (nx * dtdx + ny * dtdy + nz * dtdz) * P.re;
+    out[npts + t] = fnx;
+    out[2 * npts + t] = fny;
+    out[3 * npts + t] = fnz;
+    out[4 * npts + t] = sqrt(fnx * fnx + fny * fny + fnz * fnz);
+    out[5 * npts + t] = qw;
+}
+
 // region-local corner offsets of quad (a, c, e), corners 1..4 of force_dif
 // (PostProcess.f90:1577-1613)
 __device__ inline void quad_corners(const DRegion &g, int a, int c, int e, int o[4][3])
@@ -365,6 +426,14 @@ int launch_force_region(const DRegion &g, const DBlock &b, const DFields &F, con
                            make_double3(ref[0], ref[1], ref[2]), partial);
     hipLaunchKernelGGL(k_force_final, dim3(1), dim3(64), 0, s, (const double *)partial, np, out6);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_wall_plot(const DRegion &g, const DBlock &b, const DFields &F, const DParams &P, double cpk, double *out,
+                     hipStream_t s)
+{
+    if (g.npts <= 0) return 0;
+    hipLaunchKernelGGL(k_wall_plot, dim3((g.npts + 255) / 256), dim3(256), 0, s, g, b, F, P, cpk, out, g.npts);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
 int launch_wall_xyz(const DRegion &g, const DBlock &b, double *X, double *xyz, int back, hipStream_t s)

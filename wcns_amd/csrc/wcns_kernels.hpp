@@ -348,6 +348,10 @@ int launch_wall_xyz(const DRegion &g, const DBlock &b, double *X, double *xyz, i
 struct ForceCoef { double wf, scf, vcm, sina, cosa, sinb, cosb; };
 int launch_force_coef(const double *out6, int nwalls, const ForceCoef &c, double *out, double *hist,
                       unsigned long long *count, int cap, hipStream_t s);
+// tecout_bin_wall's surface values of one wall region (six arrays of g.npts:
+// cp, fnx, fny, fnz, cf, qw; cp only without viscosity)
+int launch_wall_plot(const DRegion &g, const DBlock &b, const DFields &F, const DParams &P, double cpk, double *out,
+                     hipStream_t s);
 int launch_force_vjp(const DRegion &g, const DBlock &b, const DFields &F, const DParams &P, const double *pn3,
                      const double *xyz, const double ref[3], const double omega[6], double *lpn3, double *lxyz,
                      double *gp, double *gm, hipStream_t s);

@@ -9,6 +9,7 @@ module wcns_amd_c
   integer(c_int), parameter :: WCNS_E_NOMEM = -4, WCNS_E_BC = -5, WCNS_E_JACOBIAN = -6, WCNS_E_ROE = -7
   integer(c_int), parameter :: WCNS_E_REPAIR = -8, WCNS_E_UNSUPPORTED = -9
   integer(c_int), parameter :: WCNS_LHS_FP64 = 0, WCNS_LHS_FP32 = 1
+  integer(c_int), parameter :: WCNS_TEC_SURFACE = 0, WCNS_TEC_WALL = 1, WCNS_TEC_VOLUME = 2
 
   type, bind(C) :: wcns_params
      real(c_double) :: moo, reynolds, attack, sideslip, tref, twall
@@ -173,6 +174,13 @@ module wcns_amd_c
        character(kind=c_char), intent(in) :: flowname(*)
        integer(c_int), value :: nout
        real(c_double), value :: wholetime
+     end function
+     ! Tecplot binary output of post_processing's nplot choices (PostProcess.f90:37-55)
+     integer(c_int) function wcns_tecplot(ctx, path, kind) bind(C, name="wcns_tecplot")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int), value :: kind
      end function
      integer(c_int) function wcns_load_flow(ctx, flowname, t_from_file, nout, wholetime) &
           bind(C, name="wcns_load_flow")

@@ -271,6 +271,20 @@ contains
     if ((it / ndisk) * ndisk == it .or. ng == nomax) then
       rc2 = wcns_save_flow(wcns_ctx_h, trim(flowname) // c_null_char, int(it, c_int), wholetime)
       if (rc2 /= 0) call wcns_abort('wcns_save_flow', rc2)
+      ! the Tecplot files of nplot (PostProcess.f90:37-55)
+      select case (nplot)
+      case (0, 3)
+        call tecplot(WCNS_TEC_SURFACE)
+        call tecplot(WCNS_TEC_WALL)
+      case (1)
+        call tecplot(WCNS_TEC_WALL)
+      case (2)
+        call tecplot(WCNS_TEC_VOLUME)
+      case (4, 5)
+        write(*, *) 'nplot = ', nplot, ': the two-dimensional Tecplot output is not supported'
+      case default
+        write(*, *) 'nplot = ', nplot, ': no such output option'
+      end select
     end if
     if ((it / nforce) * nforce == it) then
       rc2 = wcns_force(wcns_ctx_h, fc)
@@ -280,4 +294,25 @@ contains
 20    format(1x, i6, 2x, e10.4, 9(f12.5, 1x))
     end if
   end subroutine post
+
+  ! one Tecplot file, named as the reference names it: tecname for the
+  ! surfaces, its stem + "_wall.plt" / "_vol.plt" (PostProcess.f90:375-380, 626-631)
+  subroutine tecplot(kind)
+    integer(c_int), intent(in) :: kind
+    integer(c_int) :: rc2
+    character(len=130) :: fname
+    integer :: m
+    m = index(tecname, '.', back=.true.)
+    if (kind == WCNS_TEC_SURFACE) then
+      fname = tecname
+    else if (m == 0) then
+      fname = trim(adjustl(tecname))
+    else
+      fname = trim(adjustl(tecname(1:m-1)))
+    end if
+    if (kind == WCNS_TEC_WALL) fname = trim(fname) // '_wall.plt'
+    if (kind == WCNS_TEC_VOLUME) fname = trim(fname) // '_vol.plt'
+    rc2 = wcns_tecplot(wcns_ctx_h, trim(fname) // c_null_char, kind)
+    if (rc2 /= 0) call wcns_abort('wcns_tecplot', rc2)
+  end subroutine tecplot
 end program wcns_drive

@@ -34,7 +34,7 @@ EXPORTS = ["wcns_create", "wcns_destroy", "wcns_last_error", "wcns_add_block", "
            "wcns_step_graph", "wcns_autotune", "wcns_adj_residual", "wcns_adj_jvp", "wcns_adj_vjp",
            "wcns_adj_force", "wcns_sweep_layout", "wcns_residual_history",
            "wcns_residual_noalloc", "wcns_force_record", "wcns_force_history", "wcns_save_flow", "wcns_load_flow",
-           "wcns_pool_layout", "wcns_adj_step_jvp", "wcns_adj_step_vjp", "wcns_adj_multistep"]
+           "wcns_pool_layout", "wcns_adj_step_jvp", "wcns_adj_step_vjp", "wcns_adj_multistep", "wcns_tecplot"]
 
 TIMING_SLOTS = ["bc", "init", "timestep", "vis_deriv", "rhs_inv_i", "rhs_inv_j", "rhs_inv_k", "comm_dq",
                 "sweep_lu_l", "sweep_lu_u", "sweep_gs_l", "sweep_gs_u", "update", "comm_pv", "residual",
@@ -96,6 +96,7 @@ def load_library(path=LIBPATH):
     L.wcns_force_history.argtypes = [vp, C.c_int, dptr, C.POINTER(C.c_longlong)]
     L.wcns_save_flow.argtypes = [vp, C.c_char_p, C.c_int, C.c_double]
     L.wcns_load_flow.argtypes = [vp, C.c_char_p, C.c_int, iptr, dptr]
+    L.wcns_tecplot.argtypes = [vp, C.c_char_p, C.c_int]
     L.wcns_step_graph.argtypes = [vp, C.c_int]
     L.wcns_autotune.argtypes = [vp, C.c_int, iptr, iptr, dptr]
     L.wcns_sweep_layout.argtypes = [vp, iptr, iptr, iptr]
@@ -294,6 +295,12 @@ class Solver:
     def save_flow(self, path, nout=0, wholetime=0.0):
         """Write the restart file (and path.turb with a turbulence model) in the reference's format."""
         self._check(self.L.wcns_save_flow(self.h, os.fsencode(path), nout, wholetime), "save_flow")
+
+    def tecplot(self, path, kind):
+        """Write a Tecplot binary file as the reference's post_processing does:
+        kind 0 surfaces (tecout_bin_surface), 1 walls (tecout_bin_wall),
+        2 volume (tecout_bin_volume)."""
+        self._check(self.L.wcns_tecplot(self.h, os.fsencode(path), kind), "tecplot")
 
     def load_flow(self, path, t_from_file=False):
         """Read a restart file written by save_flow or by the reference; returns
