@@ -5,6 +5,8 @@ Tolerances: FP64 path, relative 1e-12 against the field's scale (the stated
 bar); stages that involve no transcendental function are expected to be
 bit-identical and are checked with 0 ulp where noted.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -304,6 +306,49 @@ def test_wall_force_matches_oracle():
     scale = np.max(np.abs(fo[:8]))
     assert np.max(np.abs(fs[:8] - fo[:8])) <= 1e-10 * scale
     assert abs(fs[8] - fo[8]) <= 1e-10 * abs(fo[8])
+
+
+def _steps_to_drop(s, orders, chunk=100, limit=4000):
+    """Steps the solver until its residual monitor has dropped `orders` decades
+    from its first value (checked every `chunk` steps); returns the steps."""
+    h0, n = None, 0
+    while n < limit:
+        h = s.step(chunk)
+        n += chunk
+        h0 = h[0] if h0 is None else h0
+        if np.log10(h0 / h[-1]) >= orders:
+            return n, np.log10(h0 / h[-1])
+    raise AssertionError(f"no {orders}-order drop in {limit} steps")
+
+
+def test_converged_wall_force_matches_oracle():
+    """Converged state: the steady Euler bump channel (48x24x12, slip top,
+    CFL 20) run to an 8-order residual drop.  The FP64 path's wall-force
+    coefficients match the oracle's after the same steps within 1e-10; the
+    solver with the FP32 implicit operator reaches the same 8-order drop and
+    its coefficients match within 1e-8."""
+    from wcns_amd.solver import Solver
+    case = cases.bump_channel_case(ni=48, nj=24, nk=12, dy_wall=0.02, growth=1.1, top_bc=3, cfl=20.0)
+    case.params.nvis = 0
+    s = Solver(case)
+    n64, d64 = _steps_to_drop(s, 8.0)
+    o = O.OracleCase(case)
+    o.set_threads(min(16, os.cpu_count() or 1))
+    ho = o.step(n64)
+    fs, fo = s.force(), o.force()
+    scale = np.max(np.abs(fo[:8]))
+    err64 = np.max(np.abs(fs[:8] - fo[:8])) / scale
+    b = Solver(case, lhs_precision=1)
+    n32, d32 = _steps_to_drop(b, 8.0)
+    fb = b.force()
+    err32 = np.max(np.abs(fb[:8] - fo[:8])) / scale
+    print(f"fp64: {d64:.2f} orders in {n64} steps, oracle {np.log10(ho[0] / ho[-1]):.2f}; force rel err {err64:.2e}"
+          f" | fp32 operator: {d32:.2f} orders in {n32} steps, force rel err {err32:.2e}", flush=True)
+    assert err64 <= 1e-10 and abs(fs[8] - fo[8]) <= 1e-10 * abs(fo[8])
+    assert err32 <= 1e-8 and abs(fb[8] - fo[8]) <= 1e-8 * abs(fo[8])
+    o.close()
+    b.close()
+    s.close()
 
 
 def test_pipelined_sweeps_match_plane_sweeps_full_size(monkeypatch):
