@@ -22,6 +22,12 @@ from a few further plain steps with HIP events recorded on the library's own
 stream; the dominant kernel's roofline entry is its algorithmic HBM bytes per
 launch over its mean launch time, against 8 TB/s.
 
+The line's "equal_drop" object is the metric's equal-residual-drop
+comparison: the steady workload bump-euler (below) at CFL 20, run once with
+the FP64 and once with the FP32 implicit operator until the monitor has
+dropped 8 orders, graph replays of 100 steps between monitor reads; the wall
+time to that drop of each and their ratio (--no-equal-drop skips it).
+
 Extra workloads: bump-sa (the bump channel with the Spalart-Allmaras model),
 bump-euler (the bump channel at wall spacing 1e-3 and growth 1.08, Euler,
 with a slip top face instead of far field: the steady workload of --converge),
@@ -209,6 +215,8 @@ def main():
     ap.add_argument("--converge", type=int, default=0, metavar="STEPS",
                     help="run STEPS steps with the FP64 and with the FP32 implicit operator and report the "
                          "residual drop of each against time (the metric's equal-drop comparison)")
+    ap.add_argument("--no-equal-drop", action="store_true",
+                    help="skip the FP64 / FP32 runs to an 8-order drop on the steady Euler bump channel")
     ap.add_argument("--adjoint", action="store_true",
                     help="time the residual, its tangent and adjoint and the force gradient instead "
                          "(128x96x64 wall block, SA with frozen mu_t, M 0.7)")
@@ -333,6 +341,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(case)
+    drop = None if args.no_equal_drop else equal_drop_line(args)
     line = {
         "metric": "Mcell-iter/s (RHS+LU-SGS), mixed vs FP64 at equal residual drop; % HBM peak",
         "value": round(value, 2), "unit": "Mcell-iter/s", "n_gpus": 1, "steps": args.steps,
@@ -343,6 +352,7 @@ def main():
                    "launch": args.launch, "sweep_tile": list(tile[:2]) if tile else None,
                    "sgs_paired_helpers": layout[2], "residual_rms_end": rms},
         "other_lhs_precision": other,
+        "equal_drop": drop,
         "roofline": roof, "cpu_baseline": cpu,
         "step_hbm_frac": round(STEP_BYTES_PER_NODE * nodes * args.steps / sec / (HBM_PEAK_GBS * 1e9), 5),
         "stage_ms_per_step": {k: round(v[0] / kt, 4) for k, v in tim.items()},
@@ -351,51 +361,76 @@ def main():
     s.close()
 
 
-def converge_bench(args, chunk=100):
+def drop_run(case, lhs, max_steps, stop_at=None, tune=True, chunk=100):
     """Residual drop (orders of magnitude of the RMS monitor below its first
-    value) against steps and seconds, FP64 and FP32 implicit operators, graph
-    replays of `chunk` steps between monitor reads."""
+    value) against steps and seconds for one implicit-operator precision:
+    one plain step, then graph replays of `chunk` steps between monitor
+    reads, until `max_steps` or until the drop reaches `stop_at`."""
     import math
     import torch
     from wcns_amd.solver import Solver
-    case, wl_name = build_case(args.workload)
-    if args.cfl:
-        case.params.cfl = args.cfl
-        wl_name += f" CFL {args.cfl:g}"
-    runs = {}
-    for lhs in ("fp64", "fp32"):
-        s = Solver(case, device=0, lhs_precision=1 if lhs == "fp32" else 0)
-        s.step(1, history=False)
-        r0 = s.residual_curve()[0]
-        if not args.no_tune:
-            s.autotune(reps=2)
-        hist, t_run, done = [], 0.0, 1
-        while done < args.converge:
-            n = min(chunk, args.converge - done)
-            s.sync()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            s.step_graph(n)
-            s.sync()
-            t_run += time.perf_counter() - t0
-            done += n
-            rms = s.residual_curve()[0]
-            hist.append((done, round(t_run, 4), round(math.log10(r0 / rms), 3) if rms > 0 else None))
-            print(f"converge {lhs}: {done} steps, {t_run:.1f} s, drop {hist[-1][2]}", file=sys.stderr, flush=True)
-        runs[lhs] = {"rms_first": r0, "history": hist, "ms_per_step": round(1e3 * t_run / (done - 1), 3)}
-        s.close()
-    # equal drop: 8 orders (the convergence target) when both reach it, else
-    # the deepest drop both reach; and when each reached it
+    s = Solver(case, device=0, lhs_precision=1 if lhs == "fp32" else 0)
+    s.step(1, history=False)
+    r0 = s.residual_curve()[0]
+    if tune:
+        s.autotune(reps=2)
+    hist, t_run, done = [], 0.0, 1
+    while done < max_steps:
+        n = min(chunk, max_steps - done)
+        s.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.step_graph(n)
+        s.sync()
+        t_run += time.perf_counter() - t0
+        done += n
+        rms = s.residual_curve()[0]
+        hist.append((done, round(t_run, 4), round(math.log10(r0 / rms), 3) if rms > 0 else None))
+        print(f"converge {lhs}: {done} steps, {t_run:.1f} s, drop {hist[-1][2]}", file=sys.stderr, flush=True)
+        if stop_at is not None and hist[-1][2] is not None and hist[-1][2] >= stop_at:
+            break
+    s.close()
+    return {"rms_first": r0, "history": hist, "ms_per_step": round(1e3 * t_run / (done - 1), 3)}
+
+
+def equal_drop(runs, goal=8.0):
+    """The drop both precisions reach (`goal` orders when both do, else the
+    deepest common one) and the step and second at which each reached it."""
     deepest = min(max(h[2] for h in runs[k]["history"] if h[2] is not None) for k in runs)
-    target = min(8.0, deepest)
+    target = min(goal, deepest)
     reach = {}
     for k, r in runs.items():
         hit = next((h for h in r["history"] if h[2] is not None and h[2] >= target), None)
         reach[k] = {"steps": hit[0], "seconds": hit[1]} if hit else None
     speed = reach["fp64"]["seconds"] / reach["fp32"]["seconds"] if reach["fp64"] and reach["fp32"] else None
+    return target, deepest, reach, speed
+
+
+def converge_bench(args):
+    """Residual drop against steps and seconds, FP64 and FP32 implicit
+    operators, over args.converge steps of the workload."""
+    case, wl_name = build_case(args.workload)
+    if args.cfl:
+        case.params.cfl = args.cfl
+        wl_name += f" CFL {args.cfl:g}"
+    runs = {lhs: drop_run(case, lhs, args.converge, tune=not args.no_tune) for lhs in ("fp64", "fp32")}
+    target, deepest, reach, speed = equal_drop(runs)
     print(json.dumps({"converge": {"workload": wl_name, "steps": args.converge, "equal_drop_orders": target,
                                    "deepest_common_drop": deepest,
                                    "reached": reach, "fp32_over_fp64_time_speedup": speed, "runs": runs}}))
+
+
+def equal_drop_line(args, goal=8.0, max_steps=6000):
+    """The metric's equal-drop comparison on the steady Euler bump channel
+    (256x128x64, slip top, CFL 20): each precision runs until its monitor
+    has dropped `goal` orders; the wall time to that drop, FP64 over FP32."""
+    case, wl_name = build_case("bump-euler")
+    case.params.cfl = 20.0
+    runs = {lhs: drop_run(case, lhs, max_steps, stop_at=goal, tune=not args.no_tune) for lhs in ("fp64", "fp32")}
+    target, _, reach, speed = equal_drop(runs, goal)
+    return {"workload": wl_name + " CFL 20", "drop_orders": target, "reached": reach,
+            "fp32_over_fp64_time_speedup": round(speed, 4) if speed else None,
+            "ms_per_step": {k: r["ms_per_step"] for k, r in runs.items()}}
 
 
 def adjoint_bench(args):
