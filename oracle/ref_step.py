@@ -48,6 +48,7 @@ def lib():
         L.ref_step_turb_inflow.argtypes = [dptr]
         L.ref_step_images.argtypes = [C.c_int, C.c_int, iptr, iptr, iptr]
         L.ref_step_field.argtypes = [C.c_int, C.c_int, C.c_int, dptr]
+        L.ref_step_poison.argtypes = [C.c_long, C.c_double]
         L.ref_step_stage.argtypes = [C.c_int]
         L.ref_step_residual.argtypes = [dptr, iptr]
         L.ref_step_force.argtypes = [dptr, C.c_int, C.c_char_p, dptr]
@@ -133,6 +134,14 @@ class RefStep:
 
     def stage(self, name):
         assert lib().ref_step_stage(STAGES[name]) == 0
+
+    def poison(self, nb, value):
+        """Leave `value` in the heap block that the next r_h_s_invis with
+        flux 4 allocates for block nb's pressure ratios (rpres), so that the
+        entries the reference reads without writing hold it."""
+        blk = self.case.blocks[nb]
+        n = 3 * (blk.ni + 3) * (blk.nj + 3) * (blk.nk + 3)
+        assert lib().ref_step_poison(n, value) == 0
 
     def residual(self):
         out = np.zeros(2)

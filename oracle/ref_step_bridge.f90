@@ -339,7 +339,7 @@ integer(c_int) function ref_step_bcindex() bind(C, name="ref_step_bcindex")
   use global_variables
   implicit none
   integer, parameter :: bclists(6) = (/-1, 4, 5, 6, 2, 3/)
-  integer :: nb, nr, i, ibl, nrmax
+  integer :: nb, nr, i, ibl, nrmax, n1, n2, n3, m, j, k
   ref_step_bcindex = 0
   do nb = 1, nblocks
     nrmax = mb_bc(nb)%nregions
@@ -354,6 +354,27 @@ integer(c_int) function ref_step_bcindex() bind(C, name="ref_step_bcindex")
       end do
     end do
     if (i /= nrmax) ref_step_bcindex = -1
+  end do
+  ! the face-node region flags of allocate_other_variable and fill_bc_flag
+  ! (PreProcess.f90:695, 761-766, 820-857): per face, the last region in
+  ! the order above whose box holds the node
+  allocate(mb_flg(nblocks, 6))
+  do nb = 1, nblocks
+    n1 = mb_dim(nb, 1); n2 = mb_dim(nb, 2); n3 = mb_dim(nb, 3)
+    allocate(mb_flg(nb, 1)%a3d(1:1, 1:n2, 1:n3), mb_flg(nb, 2)%a3d(n1:n1, 1:n2, 1:n3))
+    allocate(mb_flg(nb, 3)%a3d(1:n1, 1:1, 1:n3), mb_flg(nb, 4)%a3d(1:n1, n2:n2, 1:n3))
+    allocate(mb_flg(nb, 5)%a3d(1:n1, 1:n2, 1:1), mb_flg(nb, 6)%a3d(1:n1, 1:n2, n3:n3))
+    do ibl = 1, mb_bc(nb)%nregions
+      nr = mb_bc(nb)%bcindexs(ibl)
+      m = 2 * mb_bc(nb)%bc(nr)%s_nd + (mb_bc(nb)%bc(nr)%s_lr - 1) / 2
+      do k = mb_bc(nb)%bc(nr)%s_st(3), mb_bc(nb)%bc(nr)%s_ed(3)
+        do j = mb_bc(nb)%bc(nr)%s_st(2), mb_bc(nb)%bc(nr)%s_ed(2)
+          do i = mb_bc(nb)%bc(nr)%s_st(1), mb_bc(nb)%bc(nr)%s_ed(1)
+            mb_flg(nb, m)%a3d(i, j, k) = nr
+          end do
+        end do
+      end do
+    end do
   end do
 end function ref_step_bcindex
 
@@ -498,3 +519,23 @@ integer(c_int) function ref_step_residual(out, loc) bind(C, name="ref_step_resid
   loc = n_error
   ref_step_residual = 0
 end function ref_step_residual
+
+! Allocates, fills with v and frees an array of n reals, twice, so that the
+! heap's next allocation of that size (inviscd3d's rpres, Convect.f90:79)
+! starts from those values: the entries of rpres its loops never write
+! (Convect.f90:81-132) but its rpmin reads (Convect.f90:169-177, 335-343,
+! 499-507) then hold v.  Test infrastructure for pinning those reads.
+integer(c_int) function ref_step_poison(n, v) bind(C, name="ref_step_poison")
+  use iso_c_binding
+  implicit none
+  integer(c_long), value, intent(in) :: n
+  real(c_double), value, intent(in) :: v
+  real, pointer :: a(:)
+  integer :: rep
+  do rep = 1, 2
+    allocate(a(n))
+    a = v
+    deallocate(a)
+  end do
+  ref_step_poison = 0
+end function ref_step_poison

@@ -333,6 +333,148 @@ int o_flux_vanleer(const double *priml, const double *primr, double nx, double n
     return 0;
 }
 
+/* WCNSE5.f90:4858-4912 pave: Roe average of two primitive states */
+void o_pave(const double *ql, const double *qr, double *qa, double gama)
+{
+    double gam1 = gama - 1.0, gam2 = gama / gam1;
+    double rl = ql[0], ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4];
+    double rr = qr[0], ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4];
+    double vl2 = ul * ul + vl * vl + wl * wl;
+    double vr2 = ur * ur + vr * vr + wr * wr;
+    double hl = gam2 * pl / rl + 0.5 * vl2;
+    double hr = gam2 * pr / rr + 0.5 * vr2;
+    double rls = sqrt(rl), rrs = sqrt(rr);
+    double ra = rls * rrs;
+    double oors = 1.0 / (rls + rrs);
+    rls = rls * oors;
+    rrs = rrs * oors;
+    double ua = rls * ul + rrs * ur, va = rls * vl + rrs * vr, wa = rls * wl + rrs * wr;
+    double ha = rls * hl + rrs * hr;
+    double va2 = ua * ua + va * va + wa * wa;
+    qa[0] = ra; qa[1] = ua; qa[2] = va; qa[3] = wa;
+    qa[4] = (ha - 0.5 * va2) * ra / gam2;
+}
+
+/* WCNSE5.f90:4960-5004 LxQ: left eigenvectors of the face normal gykb(1:3)
+ * at the primitive state prim, applied to dq (gykb(4) is normalised there
+ * and never used) */
+void o_lxq(const double *prim, const double *gykb, const double *dq, double *f, double gama)
+{
+    double nx = gykb[0], ny = gykb[1], nz = gykb[2];
+    double sn = sqrt(nx * nx + ny * ny + nz * nz);
+    double oon = 1.0 / (sn > SML_SSS ? sn : SML_SSS);
+    nx = nx * oon; ny = ny * oon; nz = nz * oon;
+    double rm = prim[0], pm = prim[4];
+    double c2 = gama * pm / rm;
+    double cm = sqrt(c2);
+    double dqn = nx * dq[1] + ny * dq[2] + nz * dq[3];
+    double wp1 = dq[0] - dq[4] / c2;
+    double wp2 = dq[4] / (rm * cm);
+    f[0] = nz * dq[2] - ny * dq[3] + nx * wp1;
+    f[1] = -nz * dq[1] + nx * dq[3] + ny * wp1;
+    f[2] = ny * dq[1] - nx * dq[2] + nz * wp1;
+    f[3] = dqn + wp2;
+    f[4] = -dqn + wp2;
+}
+
+/* WCNSE5.f90:5006-5050 RxQ: the right eigenvectors, the inverse of LxQ */
+void o_rxq(const double *prim, const double *gykb, const double *dq, double *f, double gama)
+{
+    double nx = gykb[0], ny = gykb[1], nz = gykb[2];
+    double sn = sqrt(nx * nx + ny * ny + nz * nz);
+    double oon = 1.0 / (sn > SML_SSS ? sn : SML_SSS);
+    nx = nx * oon; ny = ny * oon; nz = nz * oon;
+    double rm = prim[0], pm = prim[4];
+    double c2 = gama * pm / rm;
+    double cm = sqrt(c2);
+    double dqn = nx * dq[0] + ny * dq[1] + nz * dq[2];
+    double wp1 = 0.5 * rm * (dq[3] + dq[4]);
+    double wp2 = 0.5 * (dq[3] - dq[4]);
+    f[0] = dqn + wp1 / cm;
+    f[1] = -nz * dq[1] + ny * dq[2] + nx * wp2;
+    f[2] = nz * dq[0] - nx * dq[2] + ny * wp2;
+    f[3] = -ny * dq[0] + nx * dq[1] + nz * wp2;
+    f[4] = wp1 * cm;
+}
+
+/* flux.f90:246-393 flux_roemx: the shock-stable Roe flux of Kim and Kim
+ * (J. Comput. Phys. 2005; "RoeM"), whose pressure-wiggle control uses the
+ * face's minimum pressure ratio rpmin (Convect.f90:78-200) */
+void o_flux_roemx(const double *ql, const double *qr, double sx, double sy, double sz, double rpmin,
+                  double *flr, double gama)
+{
+    double sn = sqrt(sx * sx + sy * sy + sz * sz);
+    if (sn < SML_SSS) sn = SML_SSS;
+    double osn = 1.0 / sn;
+    double nx = sx * osn, ny = sy * osn, nz = sz * osn;
+    double rl = ql[0] + SMALL, ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4] + SMALL;
+    double rr = qr[0] + SMALL, ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4] + SMALL;
+    double vl2 = ul * ul + vl * vl + wl * wl;
+    double vr2 = ur * ur + vr * vr + wr * wr;
+    double ael = gama - 1.0, aer = gama - 1.0;
+    double el = pl / (rl * ael), er = pr / (rr * aer);
+    el = el + 0.5 * vl2;
+    double hl = el + pl / rl;
+    er = er + 0.5 * vr2;
+    double hr = er + pr / rr;
+    double vnl = nx * ul + ny * vl + nz * wl;
+    double vnr = nx * ur + ny * vr + nz * wr;
+    double dq[5], qi[5], mxdq[5];
+    dq[0] = rr - rl;
+    dq[1] = rr * ur - rl * ul;
+    dq[2] = rr * vr - rl * vl;
+    dq[3] = rr * wr - rl * wl;
+    dq[4] = rr * hr - rl * hl;
+    double rls = sqrt(rl), rrs = sqrt(rr);
+    qi[0] = rls * rrs;
+    double oors = 1.0 / (rls + rrs);
+    rls = rls * oors;
+    rrs = rrs * oors;
+    qi[1] = rls * ul + rrs * ur;
+    qi[2] = rls * vl + rrs * vr;
+    qi[3] = rls * wl + rrs * wr;
+    qi[4] = rls * hl + rrs * hr;
+    double gami = rls * gama + rrs * gama;
+    double vi2 = qi[1] * qi[1] + qi[2] * qi[2] + qi[3] * qi[3];
+    double vi2p2 = 0.5 * vi2;
+    double ci2 = (gami - 1.0) * (qi[4] - vi2p2);
+    double ci = sqrt(ci2);
+    double vni = qi[1] * nx + qi[2] * ny + qi[3] * nz;
+    double b1 = 0.0, b2 = 0.0;
+    if (vni + ci > b1) b1 = vni + ci;
+    if (vnr + ci > b1) b1 = vnr + ci;
+    if (vni - ci < b2) b2 = vni - ci;
+    if (vnl - ci < b2) b2 = vnl - ci;
+    double oob12 = 1.0 / (b1 - b2);
+    b1 = b1 * oob12;
+    double b1b2 = b1 * b2;
+    b2 = b2 * oob12;
+    double mi = vni / ci, mia = fabs(mi);
+    double fden = vi2 <= SMALL ? 1.0 : pow(mia, 1.0 - rpmin);
+    double rpmlr = pl / pr < pr / pl ? pl / pr : pr / pl;
+    double gcdc = mia <= SMALL ? 1.0 : pow(mia, 1.0 - rpmlr);
+    double dvn = vnr - vnl, dvx = ur - ul, dvy = vr - vl, dvz = wr - wl, dps = pr - pl, dh = hr - hl;
+    double rmp = dq[0] - fden * dps / ci2;
+    mxdq[0] = rmp;
+    mxdq[1] = rmp * qi[1] + qi[0] * (dvx - nx * dvn);
+    mxdq[2] = rmp * qi[2] + qi[0] * (dvy - ny * dvn);
+    mxdq[3] = rmp * qi[3] + qi[0] * (dvz - nz * dvn);
+    mxdq[4] = rmp * qi[4] + qi[0] * dh;
+    double rvnl = b1 * rl * vnl;
+    double rvnr = -b2 * rr * vnr;
+    double plr = b1 * pl - b2 * pr;
+    flr[0] = rvnl + rvnr + b1b2 * dq[0];
+    flr[1] = rvnl * ul + rvnr * ur + nx * plr + b1b2 * dq[1];
+    flr[2] = rvnl * vl + rvnr * vr + ny * plr + b1b2 * dq[2];
+    flr[3] = rvnl * wl + rvnr * wr + nz * plr + b1b2 * dq[3];
+    flr[4] = rvnl * hl + rvnr * hr + b1b2 * dq[4];
+    double cmxd = -gcdc * b1b2 / (1.0 + mia);
+    for (int m = 0; m < 5; m++) flr[m] = (flr[m] + cmxd * mxdq[m]) * sn;
+}
+
+static int face_flux_at(int flux, const double *ql, const double *qr, double kx, double ky,
+                        double kz, double kt, double *f, double efix, double gama, const double *rpmin, int face);
+
 static int face_flux(int flux, const double *ql, const double *qr, double kx, double ky,
                      double kz, double kt, double *f, double efix, double gama)
 {
@@ -342,6 +484,44 @@ static int face_flux(int flux, const double *ql, const double *qr, double kx, do
     return 0;
 }
 
+static int face_flux_at(int flux, const double *ql, const double *qr, double kx, double ky,
+                        double kz, double kt, double *f, double efix, double gama, const double *rpmin, int face)
+{
+    if (flux == 4) {
+        /* the face index reaches flux_roemx through common /rpind/ i, the
+         * loop index of every caller (WCNSE5.f90:2797, 4186) */
+        o_flux_roemx(ql, qr, kx, ky, kz, rpmin[face], f, gama);
+        return 0;
+    }
+    return face_flux(flux, ql, qr, kx, ky, kz, kt, f, efix, gama);
+}
+
+/* WCNSE5.f90:5052-5123 VALUE_HALF_NODE_45 for n > 4: VALUE_HALF_NODE's faces
+ * 0..ni plus faces -2, -1, ni+1, ni+2 from the ghost metrics.  q(5,-3:ni+4),
+ * q_half(5,-2:ni+2) (index i+2). */
+void o_value_half_node_45(int ni, const double *q, double *q_half)
+{
+#define QQ(m,i) q[(m) + 5 * ((i) + 3)]
+#define QH(m,i) q_half[(m) + 5 * ((i) + 2)]
+    for (int m = 0; m < 5; m++) {
+        for (int i = 2; i <= ni - 2; i++)
+            QH(m, i) = (9.0 * (QQ(m, i) + QQ(m, i + 1)) + -1.0 * (QQ(m, i + 2) + QQ(m, i - 1))) / 16.0;
+        QH(m, 1) = (5.0 * QQ(m, 1) + 15.0 * QQ(m, 2) + -5.0 * QQ(m, 3) + 1.0 * QQ(m, 4)) / 16.0;
+        QH(m, ni - 1) = (5.0 * QQ(m, ni) + 15.0 * QQ(m, ni - 1) + -5.0 * QQ(m, ni - 2) + 1.0 * QQ(m, ni - 3)) / 16.0;
+        QH(m, 0) = (35.0 * QQ(m, 1) - 35.0 * QQ(m, 2) + 21.0 * QQ(m, 3) - 5.0 * QQ(m, 4)) / 16.0;
+        QH(m, ni) = (35.0 * QQ(m, ni) - 35.0 * QQ(m, ni - 1) + 21.0 * QQ(m, ni - 2) - 5.0 * QQ(m, ni - 3)) / 16.0;
+        QH(m, -1) = (5.0 * QQ(m, 0) + 15.0 * QQ(m, -1) + -5.0 * QQ(m, -2) + 1.0 * QQ(m, -3)) / 16.0;
+        QH(m, -2) = (9.0 * (QQ(m, -2) + QQ(m, -1)) + -1.0 * (QQ(m, 0) + QQ(m, -3))) / 16.0;
+        QH(m, ni + 1) = (5.0 * QQ(m, ni + 1) + 15.0 * QQ(m, ni + 2) + -5.0 * QQ(m, ni + 3) + 1.0 * QQ(m, ni + 4)) / 16.0;
+        QH(m, ni + 2) = (9.0 * (QQ(m, ni + 2) + QQ(m, ni + 3)) + -1.0 * (QQ(m, ni + 4) + QQ(m, ni + 1))) / 16.0;
+    }
+#undef QQ
+#undef QH
+}
+
+static int wcns_line_core(int scheme, int flux, int ni, int nmax, double efix, double gama,
+                          const double *trxyz, const double *trx1, const double *q, const double *rpmin, double *df);
+
 /*
  * WCNSE5.f90:2-206 WCNS_E_5 (scheme 4) and WCNSE5.f90:2087-2759 WCNS_E_5_41.
  * q(1:6,-2:nmax+3,1:2) (only the first plane is read), trxyz(5,nmax), df(5,nmax).
@@ -350,12 +530,37 @@ static int face_flux(int flux, const double *ql, const double *qr, double kx, do
 int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama,
                 const double *trxyz, const double *q, double *df)
 {
+    return wcns_line_core(scheme, flux, ni, nmax, efix, gama, trxyz, NULL, q, NULL, df);
+}
+
+/*
+ * The line routine of every scheme with every flux: schemes 4, 41, 42 as
+ * o_wcns_line, and WCNS_E_5_45 (WCNSE5.f90:3442-4155) and WCNS_E_5_47
+ * (WCNSE5.f90:4158-4854), whose metric line trx1(5,-3:ni+4) carries the four
+ * ghost metrics inviscd3d gathers at either end (Convect.f90:180-297); flux 4
+ * (flux_roemx) reads rpmin(0:ni).  trx1 and rpmin may be NULL when unused.
+ */
+int o_wcns_line_ex(int scheme, int flux, int ni, int nmax, double efix, double gama,
+                   const double *trx1, const double *q, const double *rpmin, double *df)
+{
+    double *tr = (double *)malloc(sizeof(double) * 5 * (ni > 0 ? ni : 1));
+    for (int i = 1; i <= ni; i++)
+        for (int m = 0; m < 5; m++) tr[5 * (i - 1) + m] = trx1[5 * (i + 3) + m];
+    int r = wcns_line_core(scheme, flux, ni, nmax, efix, gama, tr, trx1, q, rpmin, df);
+    free(tr);
+    return r;
+}
+
+static int wcns_line_core(int scheme, int flux, int ni, int nmax, double efix, double gama,
+                          const double *trxyz, const double *trx1, const double *q, const double *rpmin, double *df)
+{
 #define QL(m,i)  q[((m) - 1) + 6 * ((i) + 2)]
     const double CL[3] = {1.0 / 16.0, 10.0 / 16.0, 5.0 / 16.0};
     const double CR[3] = {5.0 / 16.0, 10.0 / 16.0, 1.0 / 16.0};
     const double EPS = 1.0e-6;
     int np = nmax + 1;
-    double *nx   = (double *)malloc(sizeof(double) * 5 * np);
+    double *nxbuf = (double *)malloc(sizeof(double) * 5 * (np + 4));
+    double *nx   = nxbuf + 10;           /* faces -2..nmax+2 */
     double *f    = (double *)malloc(sizeof(double) * 5 * np);
     double *qwl  = (double *)malloc(sizeof(double) * 5 * np);
     double *qwr  = (double *)malloc(sizeof(double) * 5 * np);
@@ -376,11 +581,140 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
     double ql[5], qr[5], flr[5];
     double u_l_old[5][9], u_r_old[5][9];
 
-    o_value_half_node(5, nmax, ni, trxyz, nx);
+    if (scheme == 45) o_value_half_node_45(ni, trx1, nxbuf);
+    else o_value_half_node(5, nmax, ni, trxyz, nx);
     int ist = (QL(1, -2) < SMALL) ? 2 : 0;
     int ied = (QL(1, ni + 3) < SMALL) ? ni - 2 : ni;
+    const int charw = scheme == 45 || scheme == 47;
 
-    for (int m = 1; m <= NL; m++) {
+    if (scheme == 45) {
+        /* WCNSE5.f90:3492-3545: characteristic jumps cq(i) = L(face i)
+         * (q(i+1) - q(i)) with |rho| in the Roe average, i = ist-2..ied+2;
+         * the differences and weights of cq at i = ist..ied+1 */
+        double *cq = (double *)malloc(sizeof(double) * 5 * (ni + 6));
+#define CQ(m,i) cq[((m) - 1) + 5 * ((i) + 2)]
+        for (int i = ist - 2; i <= ied + 2; i++) {
+            double qal[5], qar[5], qtmp[5], qave[5];
+            for (int m = 1; m <= NL; m++) { qal[m - 1] = QL(m, i); qar[m - 1] = QL(m, i + 1); }
+            qal[0] = fabs(qal[0]);
+            qar[0] = fabs(qar[0]);
+            for (int m = 0; m < NL; m++) qtmp[m] = qar[m] - qal[m];
+            o_pave(qal, qar, qave, gama);
+            o_lxq(qave, &NX(1, i), qtmp, &CQ(1, i), gama);
+        }
+        for (int i = ist; i <= ied + 1; i++)
+            for (int m = 1; m <= NL; m++) {
+                double bl[3], br[3], IS;
+                GG(m, 1, i) = 0.5 * (3.0 * CQ(m, i - 1) - CQ(m, i - 2));
+                GG(m, 2, i) = 0.5 * (CQ(m, i) + CQ(m, i - 1));
+                GG(m, 3, i) = 0.5 * (3.0 * CQ(m, i) - CQ(m, i + 1));
+                SS(m, 1, i) = CQ(m, i - 1) - CQ(m, i - 2);
+                SS(m, 2, i) = CQ(m, i) - CQ(m, i - 1);
+                SS(m, 3, i) = CQ(m, i + 1) - CQ(m, i);
+                for (int n = 1; n <= 3; n++) {
+                    IS = GG(m, n, i) * GG(m, n, i) + SS(m, n, i) * SS(m, n, i);
+                    double EIS2 = (EPS + IS) * (EPS + IS);
+                    bl[n - 1] = CL[n - 1] / EIS2;
+                    br[n - 1] = CR[n - 1] / EIS2;
+                }
+                IS = bl[0] + bl[1] + bl[2];
+                for (int n = 1; n <= 3; n++) WL(m, n, i) = bl[n - 1] / IS;
+                IS = br[0] + br[1] + br[2];
+                for (int n = 1; n <= 3; n++) WR(m, n, i) = br[n - 1] / IS;
+            }
+#undef CQ
+        free(cq);
+        /* WCNSE5.f90:3550-3580: both states of face i back through R(face i) */
+        for (int i = ist; i <= ied; i++) {
+            int i1 = i + 1;
+            double qal[5], qar[5], qa[5], qb[5], qave[5];
+            for (int m = 1; m <= NL; m++) {
+                qal[m - 1] = 0.125 * (WL(m, 1, i) * (SS(m, 1, i) + 4.0 * GG(m, 1, i)) +
+                                      WL(m, 2, i) * (SS(m, 2, i) + 4.0 * GG(m, 2, i)) +
+                                      WL(m, 3, i) * (SS(m, 3, i) + 4.0 * GG(m, 3, i)));
+                qar[m - 1] = 0.125 * (WR(m, 1, i1) * (SS(m, 1, i1) - 4.0 * GG(m, 1, i1)) +
+                                      WR(m, 2, i1) * (SS(m, 2, i1) - 4.0 * GG(m, 2, i1)) +
+                                      WR(m, 3, i1) * (SS(m, 3, i1) - 4.0 * GG(m, 3, i1)));
+            }
+            double pa[5], pb[5];
+            for (int m = 1; m <= NL; m++) { pa[m - 1] = QL(m, i); pb[m - 1] = QL(m, i1); }
+            o_pave(pa, pb, qave, gama);
+            o_rxq(qave, &NX(1, i), qal, qa, gama);
+            o_rxq(qave, &NX(1, i), qar, qb, gama);
+            for (int m = 1; m <= NL; m++) { QWL(m, i) = QL(m, i) + qa[m - 1]; QWR(m, i) = QL(m, i1) + qb[m - 1]; }
+        }
+    }
+    if (scheme == 47) {
+        /* WCNSE5.f90:4200-4278: differences in primitive variables, then
+         * everything at node i in the characteristic variables of node i's
+         * own state and metric; both states back through R(node) */
+        for (int m = 1; m <= NL; m++) {
+            int i = ist - 1;
+            SS(m, 2, i) = QL(m, i - 1) - 2.0 * QL(m, i) + QL(m, i + 1);
+            SS(m, 3, i) = QL(m, i) - 2.0 * QL(m, i + 1) + QL(m, i + 2);
+            for (i = ist; i <= ied + 1; i++) {
+                GG(m, 1, i) = 0.5 * (QL(m, i - 2) - 4.0 * QL(m, i - 1) + 3.0 * QL(m, i));
+                GG(m, 2, i) = 0.5 * (QL(m, i + 1) - QL(m, i - 1));
+                GG(m, 3, i) = 0.5 * (-3.0 * QL(m, i) + 4.0 * QL(m, i + 1) - QL(m, i + 2));
+                SS(m, 1, i) = SS(m, 2, i - 1);
+                SS(m, 2, i) = SS(m, 3, i - 1);
+                SS(m, 3, i) = QL(m, i) - 2.0 * QL(m, i + 1) + QL(m, i + 2);
+            }
+        }
+        double *qch = (double *)malloc(sizeof(double) * 5 * (ni + 4));   /* L q at i = -1..ni+2 */
+#define QC(m,i) qch[((m) - 1) + 5 * ((i) + 1)]
+        for (int i = ist; i <= ied + 1; i++) {
+            double prim[5], v[5], IS0[5][3];
+            const double *mt = &trx1[5 * (i + 3)];
+            for (int m = 1; m <= NL; m++) prim[m - 1] = QL(m, i);
+            o_lxq(prim, mt, prim, &QC(1, i), gama);
+            for (int n = 1; n <= 3; n++) {
+                double gch[5], sch[5];
+                for (int m = 1; m <= NL; m++) v[m - 1] = GG(m, n, i);
+                o_lxq(prim, mt, v, gch, gama);
+                for (int m = 1; m <= NL; m++) v[m - 1] = SS(m, n, i);
+                o_lxq(prim, mt, v, sch, gama);
+                for (int m = 1; m <= NL; m++) {
+                    GG(m, n, i) = gch[m - 1];
+                    SS(m, n, i) = sch[m - 1];
+                    IS0[m - 1][n - 1] = gch[m - 1] * gch[m - 1] + sch[m - 1] * sch[m - 1];
+                }
+            }
+            for (int m = 1; m <= NL; m++) {
+                double bl[3], br[3], IS;
+                for (int n = 1; n <= 3; n++) {
+                    double EIS2 = (EPS + IS0[m - 1][n - 1]) * (EPS + IS0[m - 1][n - 1]);
+                    bl[n - 1] = CL[n - 1] / EIS2;
+                    br[n - 1] = CR[n - 1] / EIS2;
+                }
+                IS = bl[0] + bl[1] + bl[2];
+                for (int n = 1; n <= 3; n++) WL(m, n, i) = bl[n - 1] / IS;
+                IS = br[0] + br[1] + br[2];
+                for (int n = 1; n <= 3; n++) WR(m, n, i) = br[n - 1] / IS;
+            }
+        }
+        for (int i = ist; i <= ied; i++) {
+            int i1 = i + 1;
+            double a[5], b[5], pa[5], pb[5], oa[5], ob[5];
+            for (int m = 1; m <= NL; m++) {
+                a[m - 1] = QC(m, i) + 0.125 * (WL(m, 1, i) * (SS(m, 1, i) + 4.0 * GG(m, 1, i)) +
+                                               WL(m, 2, i) * (SS(m, 2, i) + 4.0 * GG(m, 2, i)) +
+                                               WL(m, 3, i) * (SS(m, 3, i) + 4.0 * GG(m, 3, i)));
+                b[m - 1] = QC(m, i1) + 0.125 * (WR(m, 1, i1) * (SS(m, 1, i1) - 4.0 * GG(m, 1, i1)) +
+                                                WR(m, 2, i1) * (SS(m, 2, i1) - 4.0 * GG(m, 2, i1)) +
+                                                WR(m, 3, i1) * (SS(m, 3, i1) - 4.0 * GG(m, 3, i1)));
+                pa[m - 1] = QL(m, i);
+                pb[m - 1] = QL(m, i1);
+            }
+            o_rxq(pa, &trx1[5 * (i + 3)], a, oa, gama);
+            o_rxq(pb, &trx1[5 * (i1 + 3)], b, ob, gama);
+            for (int m = 1; m <= NL; m++) { QWL(m, i) = oa[m - 1]; QWR(m, i) = ob[m - 1]; }
+        }
+#undef QC
+        free(qch);
+    }
+
+    for (int m = 1; m <= NL && !charw; m++) {
         int i = ist - 1;
         SS(m, 2, i) = QL(m, i - 1) - 2.0 * QL(m, i) + QL(m, i + 1);
         SS(m, 3, i) = QL(m, i) - 2.0 * QL(m, i + 1) + QL(m, i + 2);
@@ -404,7 +738,7 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
             for (int n = 1; n <= 3; n++) WR(m, n, i) = br[n - 1] / IS;
         }
     }
-    for (int i = ist; i <= ied; i++) {
+    for (int i = ist; i <= ied && !charw; i++) {
         int i1 = i + 1;
         for (int m = 1; m <= NL; m++) {
             QWL(m, i) = QL(m, i) + 0.125 * (WL(m, 1, i) * (SS(m, 1, i) + 4.0 * GG(m, 1, i)) +
@@ -445,20 +779,20 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
             int ii = i + 1 < ni ? i + 1 : ni;
             for (int m = 1; m <= NL; m++) qr[m - 1] = QL(m, ii);
         }
-        bad |= face_flux(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama);
+        bad |= face_flux_at(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama, rpmin, i);
         for (int m = 1; m <= NL; m++) FL(m, i) = flr[m - 1];
     }
     o_flux_dxyz(5, nmax, ni, f, df);
 #define DFO(m,i) df[((i) - 1) * 5 + (m) - 1]
 
-    if ((scheme == 41 || scheme == 42) && ist > 1) {
+    if ((scheme == 41 || scheme >= 42) && ist > 1) {
         /* WCNSE5.f90:2269-2513: left physical boundary, rows 1-3; scheme 42
          * (WCNS_E_5_42, WCNSE5.f90:2766-3441) differs only in the four
          * linear face states below (WCNSE5.f90:2967-2986) */
         for (int i = 1; i <= 4; i++)
             for (int m = 1; m <= NL; m++) { u_l_old[m - 1][i] = QWL(m, i); u_r_old[m - 1][i] = QWR(m, i); }
         for (int m = 1; m <= NL; m++) {
-            if (scheme == 42) {
+            if (scheme != 41) {
                 QWL(m, 0) = (677.0 * QL(m, 1) - 421.0 * QL(m, 2) + 99.0 * QL(m, 3) + 13.0 * QL(m, 4)) / 368.0;
                 QWL(m, 1) = (5.0 * QL(m, 1) + 15.0 * QL(m, 2) - 5.0 * QL(m, 3) + QL(m, 4)) / 16.0;
                 QWL(m, 2) = (-QL(m, 1) + 9.0 * QL(m, 2) + 9.0 * QL(m, 3) - QL(m, 4)) / 16.0;
@@ -485,7 +819,7 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
                 nerror++;
                 for (int m = 1; m <= NL; m++) qr[m - 1] = (QL(m, ii) + QL(m, i + 1)) / 2.;
             }
-            bad |= face_flux(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama);
+            bad |= face_flux_at(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama, rpmin, i);
             for (int m = 1; m <= NL; m++) FL(m, i) = flr[m - 1];
         }
         for (int m = 1; m <= NL; m++)
@@ -515,7 +849,7 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
                 int ii = i + 1 < ni ? i + 1 : ni;
                 for (int m = 1; m <= NL; m++) qr[m - 1] = QL(m, ii);
             }
-            bad |= face_flux(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama);
+            bad |= face_flux_at(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama, rpmin, i);
             for (int m = 1; m <= NL; m++) FL(m, i) = flr[m - 1];
         }
         for (int m = 1; m <= NL; m++)
@@ -540,14 +874,14 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
                 int ii = i + 1 < ni ? i + 1 : ni;
                 for (int m = 1; m <= NL; m++) qr[m - 1] = QL(m, ii);
             }
-            bad |= face_flux(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama);
+            bad |= face_flux_at(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama, rpmin, i);
             for (int m = 1; m <= NL; m++) FL(m, i) = flr[m - 1];
         }
         for (int m = 1; m <= NL; m++)
             DFO(m, 3) = (FL(m, 1) - 27.0 * FL(m, 2) + 27.0 * FL(m, 3) - FL(m, 4)) / 24.0;
     }
 
-    if ((scheme == 41 || scheme == 42) && ied < ni) {
+    if ((scheme == 41 || scheme >= 42) && ied < ni) {
         /* WCNSE5.f90:2518-2751: right physical boundary, rows ni-2..ni; scheme
          * 42's face states WCNSE5.f90:3207-3226 */
         for (int i = 1; i <= 4; i++) {
@@ -555,7 +889,7 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
             for (int m = 1; m <= NL; m++) { u_l_old[m - 1][9 - i] = QWL(m, i1); u_r_old[m - 1][9 - i] = QWR(m, i1); }
         }
         for (int m = 1; m <= NL; m++) {
-            if (scheme == 42) {
+            if (scheme != 41) {
                 QWR(m, ni)     = (677.0 * QL(m, ni) - 421.0 * QL(m, ni - 1) + 99.0 * QL(m, ni - 2) + 13.0 * QL(m, ni - 3)) / 368.0;
                 QWR(m, ni - 1) = (5.0 * QL(m, ni) + 15.0 * QL(m, ni - 1) - 5.0 * QL(m, ni - 2) + QL(m, ni - 3)) / 16.0;
                 QWR(m, ni - 2) = (-QL(m, ni) + 9.0 * QL(m, ni - 1) + 9.0 * QL(m, ni - 2) - QL(m, ni - 3)) / 16.0;
@@ -582,7 +916,7 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
                 nerror++;
                 for (int m = 1; m <= NL; m++) qr[m - 1] = (QL(m, 1) + QL(m, i)) / 2.0;
             }
-            bad |= face_flux(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama);
+            bad |= face_flux_at(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama, rpmin, i);
             for (int m = 1; m <= NL; m++) FL(m, ni - i) = flr[m - 1];
         }
         for (int m = 1; m <= NL; m++)
@@ -612,7 +946,7 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
                 int ii = i + 1 < ni ? i + 1 : ni;
                 for (int m = 1; m <= NL; m++) qr[m - 1] = QL(m, ii);
             }
-            bad |= face_flux(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama);
+            bad |= face_flux_at(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama, rpmin, i);
             for (int m = 1; m <= NL; m++) FL(m, ni - i) = flr[m - 1];
         }
         for (int m = 1; m <= NL; m++)
@@ -637,14 +971,14 @@ int o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama
                 int ii = i + 1 < ni ? i + 1 : ni;
                 for (int m = 1; m <= NL; m++) qr[m - 1] = QL(m, ii);
             }
-            bad |= face_flux(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama);
+            bad |= face_flux_at(flux, ql, qr, NX(1, i), NX(2, i), NX(3, i), NX(4, i), flr, efix, gama, rpmin, i);
             for (int m = 1; m <= NL; m++) FL(m, ni - i) = flr[m - 1];
         }
         for (int m = 1; m <= NL; m++)
             DFO(m, ni - 2) = -(FL(m, 1) - 27.0 * FL(m, 2) + 27.0 * FL(m, 3) - FL(m, 4)) / 24.0;
     }
 #undef DFO
-    free(nx); free(f); free(qwl); free(qwr); free(g); free(s); free(wl); free(wr);
+    free(nxbuf); free(f); free(qwl); free(qwr); free(g); free(s); free(wl); free(wr);
 #undef QL
 #undef NX
 #undef FL
@@ -1714,6 +2048,150 @@ int o_timestep(o_case *c)
     return 0;
 }
 
+/* The metric of direction dir (0 kc, 1 et, 2 ct) at G index q */
+static void metric_of(const oblock *b, int dir, long q, double *o)
+{
+    if (dir == 0) { o[0] = b->kcx[q]; o[1] = b->kcy[q]; o[2] = b->kcz[q]; }
+    if (dir == 1) { o[0] = b->etx[q]; o[1] = b->ety[q]; o[2] = b->etz[q]; }
+    if (dir == 2) { o[0] = b->ctx[q]; o[1] = b->cty[q]; o[2] = b->ctz[q]; }
+}
+
+/*
+ * Convect.f90:180-297 (and the same for j, k): the four ghost metrics of a
+ * pencil of direction dir at either end, trx1(1:3, -3..0) and (1:3, n+1..n+4)
+ * (index i+3).  The region that owns the pencil's end node is the last one in
+ * set_bc_index order whose box holds it on that face (fill_bc_flag,
+ * PreProcess.f90:820-857).  An interface gives the target block's metrics of
+ * its own normal direction, 1..4 nodes inside its face from the image node,
+ * signed by the target side (t_lr on the low end, -t_lr on the high end); any
+ * other region repeats the end node's metric.  kt and vol of the ghosts are
+ * not read by any scheme (kt = 0 on a static grid; vol is not used).
+ */
+static void ghost_metrics(const o_case *c, const oblock *b, int dir, int ia, int ib, double *trx1)
+{
+    int nd = dir == 0 ? b->ni : (dir == 1 ? b->nj : b->nk);
+    for (int side = 0; side < 2; side++) {
+        int l = side == 0 ? 1 : nd;
+        int i = dir == 0 ? l : ia, j = dir == 1 ? l : (dir == 0 ? ia : ib), k = dir == 2 ? l : ib;
+        const oregion *own = NULL;
+        for (int n = 0; n < b->nregions; n++) {
+            const oregion *g = &b->reg[b->bcindex[n]];
+            if (g->s_nd != dir + 1 || g->s_lr != (side == 0 ? -1 : 1)) continue;
+            if (i < g->s_st[0] || i > g->s_ed[0] || j < g->s_st[1] || j > g->s_ed[1] ||
+                k < g->s_st[2] || k > g->s_ed[2]) continue;
+            own = g;
+        }
+        for (int i1 = 1; i1 <= 4; i1++) {
+            int slot = side == 0 ? 1 - i1 : nd + i1;
+            double *o = &trx1[5 * (slot + 3)];
+            if (own && own->bctype < 0) {
+                const oblock *bt = &c->blk[own->nbt - 1];
+                long r = rbox(own, i, j, k);
+                int it = own->image[r] - i1 * own->t_lr3d[0];
+                int jt = own->jmage[r] - i1 * own->t_lr3d[1];
+                int kt = own->kmage[r] - i1 * own->t_lr3d[2];
+                double mt[3];
+                metric_of(bt, own->t_nd - 1, G(bt, it, jt, kt), mt);
+                double sg = side == 0 ? own->t_lr : -own->t_lr;
+                for (int m = 0; m < 3; m++) o[m] = sg * mt[m];
+            } else {
+                for (int m = 0; m < 3; m++) o[m] = trx1[5 * (l + 3) + m];
+            }
+            o[3] = 0.0;
+            o[4] = 0.0;
+        }
+    }
+}
+
+static inline double dmin(double a, double b) { return b < a ? b : a; }
+
+/*
+ * Convect.f90:78-132: the pressure ratios min(p_l/p_r, p_r/p_l) between
+ * neighbours along each direction, rp[d] over -1..n+1 (index (i+1) +
+ * (ni+3)((j+1) + (nj+3)(k+1))), the two ghost entries at a physical face
+ * (ghost density below small) replaced by the first interior one.  The
+ * reference's loops never write direction 2 and 3 at i = -1, 1 and 3 at
+ * j = -1, 1 and 2 at k = -1, yet its rpmin reads them at the faces 0: there it
+ * reads whatever the heap held.  Here they hold +inf, so that the minimum is
+ * over the ratios that exist -- the reference's result whenever that memory
+ * held no smaller value.
+ */
+static double *pressure_ratios(const oblock *b)
+{
+    int ni = b->ni, nj = b->nj, nk = b->nk;
+    long sx = ni + 3, sy = (long)(ni + 3) * (nj + 3), sz = sy * (nk + 3);
+    double *rp = (double *)malloc(sizeof(double) * 3 * sz);
+    for (long q = 0; q < 3 * sz; q++) rp[q] = HUGE_VAL;
+#define RP(d,i,j,k) rp[(d) * sz + ((i) + 1) + sx * ((j) + 1) + sy * ((k) + 1)]
+    for (int k = 0; k <= nk + 1; k++)
+        for (int j = 0; j <= nj + 1; j++) {
+            for (int i = -1; i <= ni + 1; i++) {
+                double pl = b->p[G(b, i, j, k)], pr = b->p[G(b, i + 1, j, k)];
+                RP(0, i, j, k) = dmin(pl / pr, pr / pl);
+            }
+            if (b->r[G(b, -2, j, k)] < SMALL) { RP(0, 0, j, k) = RP(0, 1, j, k); RP(0, -1, j, k) = RP(0, 1, j, k); }
+            if (b->r[G(b, ni + 3, j, k)] < SMALL) {
+                RP(0, ni, j, k) = RP(0, ni - 1, j, k);
+                RP(0, ni + 1, j, k) = RP(0, ni - 1, j, k);
+            }
+        }
+    for (int k = 0; k <= nk + 1; k++)
+        for (int i = 0; i <= ni + 1; i++) {
+            for (int j = -1; j <= nj + 1; j++) {
+                double pl = b->p[G(b, i, j, k)], pr = b->p[G(b, i, j + 1, k)];
+                RP(1, i, j, k) = dmin(pl / pr, pr / pl);
+            }
+            if (b->r[G(b, i, -2, k)] < SMALL) { RP(1, i, 0, k) = RP(1, i, 1, k); RP(1, i, -1, k) = RP(1, i, 1, k); }
+            if (b->r[G(b, i, nj + 3, k)] < SMALL) {
+                RP(1, i, nj, k) = RP(1, i, nj - 1, k);
+                RP(1, i, nj + 1, k) = RP(1, i, nj - 1, k);
+            }
+        }
+    for (int j = 0; j <= nj + 1; j++)
+        for (int i = 0; i <= ni + 1; i++) {
+            for (int k = -1; k <= nk + 1; k++) {
+                double pl = b->p[G(b, i, j, k)], pr = b->p[G(b, i, j, k + 1)];
+                RP(2, i, j, k) = dmin(pl / pr, pr / pl);
+            }
+            if (b->r[G(b, i, j, -2)] < SMALL) { RP(2, i, j, 0) = RP(2, i, j, 1); RP(2, i, j, -1) = RP(2, i, j, 1); }
+            if (b->r[G(b, i, j, nk + 3)] < SMALL) {
+                RP(2, i, j, nk) = RP(2, i, j, nk - 1);
+                RP(2, i, j, nk + 1) = RP(2, i, j, nk - 1);
+            }
+        }
+    return rp;
+}
+
+/* Convect.f90:169-177 (i), 335-343 (j), 499-507 (k): the face's own ratio,
+ * then the least of the four ratios of each other direction around it */
+static void rpmin_line(const oblock *b, const double *rp, int dir, int ia, int ib, double *rpmin)
+{
+    int ni = b->ni, nj = b->nj, nk = b->nk;
+    long sx = ni + 3, sy = (long)(ni + 3) * (nj + 3), sz = sy * (nk + 3);
+    int nd = dir == 0 ? ni : (dir == 1 ? nj : nk);
+    for (int f = 0; f <= nd; f++) {
+        double v;
+        if (dir == 0) {
+            int i = f, j = ia, k = ib;
+            v = RP(0, i, j, k);
+            v = dmin(dmin(dmin(dmin(RP(1, i, j, k), RP(1, i, j + 1, k)), RP(1, i - 1, j, k)), RP(1, i - 1, j + 1, k)), v);
+            v = dmin(dmin(dmin(dmin(RP(2, i, j, k), RP(2, i, j, k + 1)), RP(2, i - 1, j, k)), RP(2, i - 1, j, k + 1)), v);
+        } else if (dir == 1) {
+            int i = ia, j = f, k = ib;
+            v = RP(1, i, j, k);
+            v = dmin(dmin(dmin(dmin(RP(0, i, j, k), RP(0, i + 1, j, k)), RP(0, i, j - 1, k)), RP(0, i + 1, j - 1, k)), v);
+            v = dmin(dmin(dmin(dmin(RP(2, i, j, k), RP(2, i, j, k + 1)), RP(2, i, j - 1, k)), RP(2, i, j - 1, k + 1)), v);
+        } else {
+            int i = ia, j = ib, k = f;
+            v = RP(2, i, j, k);
+            v = dmin(dmin(dmin(dmin(RP(0, i, j, k), RP(0, i + 1, j, k)), RP(0, i, j, k - 1)), RP(0, i + 1, j, k - 1)), v);
+            v = dmin(dmin(dmin(dmin(RP(1, i, j, k), RP(1, i, j + 1, k)), RP(1, i, j, k - 1)), RP(1, i, j + 1, k - 1)), v);
+        }
+        rpmin[f] = v;
+    }
+#undef RP
+}
+
 /* RHS.f90:35-72 r_h_s_invis -> Convect.f90:2-649 invcode/inviscd3d.  The
  * pencils of one direction write disjoint nodes; with several threads each
  * thread takes whole pencils with line buffers of its own. */
@@ -1725,6 +2203,7 @@ int o_rhs_invis(o_case *c)
         oblock *b = &c->blk[nb];
         int ni = b->ni, nj = b->nj, nk = b->nk;
         int nmax = ni > nj ? ni : nj; nmax = nmax > nk ? nmax : nk;
+        double *rp = flux == 4 ? pressure_ratios(b) : NULL;
         for (int dir = 0; dir < 3; dir++) {
             int nd = dir == 0 ? ni : (dir == 1 ? nj : nk);
             if (nd <= NIJK2D) continue;
@@ -1735,7 +2214,8 @@ int o_rhs_invis(o_case *c)
             #pragma omp parallel num_threads(c->nthreads) if (c->nthreads > 1) reduction(+:nfall) reduction(|:err)
             {
                 double *ql = (double *)calloc(6 * (nmax + 6) * 2, sizeof(double));
-                double *tr = (double *)calloc(5 * nmax, sizeof(double));
+                double *tr = (double *)calloc(5 * (nmax + 8), sizeof(double));   /* nodes -3..n+4 */
+                double *rpm = (double *)calloc(nmax + 1, sizeof(double));
                 double *fc = (double *)calloc(5 * nmax, sizeof(double));
                 #pragma omp for collapse(2) schedule(static)
                 for (int ob = 1; ob <= nbo; ob++) for (int ib = 1; ib <= na; ib++) {
@@ -1748,13 +2228,13 @@ int o_rhs_invis(o_case *c)
                     for (int l = 1; l <= nd; l++) {
                         int i = dir == 0 ? l : ib, j = dir == 1 ? l : (dir == 0 ? ib : ob), k = dir == 2 ? l : ob;
                         long q = G(b, i, j, k);
-                        double *o = &tr[5 * (l - 1)];
-                        if (dir == 0) { o[0] = b->kcx[q]; o[1] = b->kcy[q]; o[2] = b->kcz[q]; }
-                        if (dir == 1) { o[0] = b->etx[q]; o[1] = b->ety[q]; o[2] = b->etz[q]; }
-                        if (dir == 2) { o[0] = b->ctx[q]; o[1] = b->cty[q]; o[2] = b->ctz[q]; }
+                        double *o = &tr[5 * (l + 3)];
+                        metric_of(b, dir, q, o);
                         o[3] = 0.0; o[4] = b->vol[q];
                     }
-                    int ne = o_wcns_line(scheme, flux, nd, nmax, efix, gama, tr, ql, fc);
+                    if (scheme == 45 || scheme == 47) ghost_metrics(c, b, dir, ib, ob, tr);
+                    if (flux == 4) rpmin_line(b, rp, dir, ib, ob, rpm);
+                    int ne = o_wcns_line_ex(scheme, flux, nd, nmax, efix, gama, tr, ql, rpm, fc);
                     if (ne < 0) { err = 1; continue; }
                     nfall += ne;
                     for (int l = 1; l <= nd; l++) {
@@ -1763,11 +2243,12 @@ int o_rhs_invis(o_case *c)
                         for (int m = 0; m < 5; m++) b->dq[5 * q + m] = b->dq[5 * q + m] + fc[5 * (l - 1) + m];
                     }
                 }
-                free(ql); free(tr); free(fc);
+                free(ql); free(tr); free(rpm); free(fc);
             }
             c->n_fallback += nfall;
-            if (err) return -2;
+            if (err) { free(rp); return -2; }
         }
+        free(rp);
     }
     return 0;
 }

@@ -92,6 +92,14 @@ int  o_flux_roe(const double *ql, const double *qr, double nx, double ny, double
 /* scheme 4 or 41; flux 1 (SW) or 3 (Roe); q is q(1:6,-2:nmax+3,1:2), trxyz(5,nmax), df(5,nmax) */
 int  o_wcns_line(int scheme, int flux, int ni, int nmax, double efix, double gama,
                  const double *trxyz, const double *q, double *df);
+int  o_wcns_line_ex(int scheme, int flux, int ni, int nmax, double efix, double gama,
+                    const double *trx1, const double *q, const double *rpmin, double *df);
+void o_value_half_node_45(int ni, const double *q, double *q_half);
+void o_pave(const double *ql, const double *qr, double *qa, double gama);
+void o_lxq(const double *prim, const double *gykb, const double *dq, double *f, double gama);
+void o_rxq(const double *prim, const double *gykb, const double *dq, double *f, double gama);
+void o_flux_roemx(const double *ql, const double *qr, double sx, double sy, double sz, double rpmin,
+                  double *flr, double gama);
 void o_mxdq_std(const double *prim, const double *gykb, const double *dq, double *f,
                 double rad, int npn, double gama);
 

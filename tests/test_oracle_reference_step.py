@@ -228,3 +228,72 @@ def test_forced_abort_stops_both():
     assert O.lib().o_lhs(o.h) == -3
     assert o.counter(1) > (24 * 16 * 12) // 64
     o.close()
+
+
+def _char_cases(nscheme, nflux):
+    """A periodic Cartesian box (interfaces to itself in i and k), two blocks
+    with a slip wall at M 2, and four viscous blocks with a no-slip wall,
+    symmetry top and inflow / outflow."""
+    a = cases.cartesian_periodic_case(ni=20, nj=14, nk=12, seed=3, nscheme=nscheme, nflux=nflux)
+    b = cases.multiblock_case(nx=(14, 12), ny=(12,), nz=(10,), nvis=0, wall=True, moo=2.0, nflux=nflux,
+                              nscheme=nscheme)
+    c = cases.multiblock_case(nvis=1, wall=True, periodic_i=False, ilo_bc=5, ihi_bc=6, jhi_bc=3, reynolds=1e5,
+                              nscheme=nscheme, nflux=nflux)
+    return {"periodic": a, "wall-M2": b, "viscous-4-blocks": c}
+
+
+@pytest.mark.parametrize("nscheme,nflux", [(45, 1), (45, 2), (45, 3), (47, 1), (47, 2), (47, 3)])
+def test_characteristic_schemes_against_reference(nscheme, nflux):
+    """WCNS_E_5_45 (characteristic jumps through the Roe-averaged face
+    eigenvectors, WCNSE5.f90:3442-4155) and WCNS_E_5_47 (characteristic
+    variables of each node's own state, WCNSE5.f90:4158-4854), whose pencils
+    read four ghost metrics at each end from the interface partner
+    (Convect.f90:180-297): two steps stage by stage on periodic, multi-block,
+    wall and viscous cases."""
+    for tag, case in _char_cases(nscheme, nflux).items():
+        o, r = _pair(case)
+        for s in range(2):
+            _step(case, o, r, f"{nscheme}/{nflux} {tag} step {s + 1}")
+        o.close()
+
+
+def _step_poisoned(case, o, r, tag, value):
+    for st in ["boundary_all", "initial_q_dq_c_t", "timestep"] + (["rhs_vis"] if case.params.nvis == 1 else []) + \
+            ["rhs_invis", "lhs"]:
+        if st == "rhs_invis":
+            r.poison(0, value)
+        r.stage(st)
+        o.stage(st)
+        _check(case, o, r, STAGE_FIELDS[st], f"{tag} {st}")
+
+
+@pytest.mark.parametrize("nscheme", [42, 47])
+def test_roemx_flux_against_reference(nscheme):
+    """flux_roemx (flux.f90:246-393) with the minimum pressure ratio around
+    each face (Convect.f90:78-177).  At the faces 0 the reference reads
+    pressure-ratio entries its loops never write; with the heap block those
+    entries come from filled with 1e300 beforehand (ref_step_poison), its
+    steps equal the oracle's bit for bit, the ratios that exist deciding the
+    minimum as in the oracle.  Filled with 0, the results differ: the reads
+    are reached."""
+    case = cases.cartesian_periodic_case(ni=20, nj=14, nk=12, seed=3, nscheme=nscheme, nflux=4)
+    o, r = _pair(case)
+    for s in range(2):
+        _step_poisoned(case, o, r, f"roemx {nscheme} step {s + 1}", 1.0e300)
+    o.close()
+    o, r = _pair(case)
+    for st in ["boundary_all", "initial_q_dq_c_t", "timestep"]:
+        r.stage(st)
+        o.stage(st)
+    r.poison(0, 0.0)
+    r.stage("rhs_invis")
+    o.stage("rhs_invis")
+    assert not np.array_equal(r.field(0, "dq"), o.field(0, "dq"))
+    o.close()
+    # equal-sized blocks reuse the block the previous one freed
+    case = cases.multiblock_case(nx=(12, 12), ny=(12, 12), nz=(10,), nvis=1, wall=True, periodic_i=False, ilo_bc=5,
+                                 ihi_bc=6, jhi_bc=3, reynolds=1e5, nscheme=nscheme, nflux=4)
+    o, r = _pair(case)
+    for s in range(2):
+        _step_poisoned(case, o, r, f"roemx {nscheme} viscous step {s + 1}", 1.0e300)
+    o.close()
