@@ -138,6 +138,11 @@ struct wcns_ctx {
     int ncu = 256;                          // compute units of the device
     int sa_staged = -1;                     // WCNS_SA_STAGED=0|1 fixes the turbulence sweeps' operand path
     double *df_t = nullptr;                 // 5: the i-direction contribution, transposed
+    // schemes 45 / 47: pencil-end ghost metrics and each block's first record;
+    // flux 4: the pressure ratios (3 x total)
+    double *d_gmet = nullptr;
+    long *d_gmoff = nullptr;
+    double *d_rpres = nullptr;
     double *fvb = nullptr;                  // 5: viscous half-node fluxes of one direction
     int *d_flag = nullptr;                  // flag_cap nodes per block for the ordered repair
     int flag_cap = 512;
@@ -222,6 +227,7 @@ struct wcns_ctx {
         F.visl = visl; F.counters = counters; F.total = total; F.coed = coed;
         F.dq32 = lhs_prec == WCNS_LHS_FP32 ? dq32 : nullptr;
         F.qke = qke; F.vist = vist; F.dist = dist; F.qkw = qkw;
+        F.gmet = d_gmet; F.gmoff = d_gmoff; F.rpres = d_rpres;
         return F;
     }
 };
@@ -536,7 +542,8 @@ int wcns_create(wcns_ctx **out, int device, const wcns_params *prm)
         delete ctx;
         return WCNS_E_HIP;
     }
-    if (prm->nscheme != 4 && prm->nscheme != 41 && prm->nscheme != 42) { ctx->err = "nscheme must be 4, 41 or 42"; }
+    if (prm->nscheme != 4 && prm->nscheme != 41 && prm->nscheme != 42 && prm->nscheme != 45 && prm->nscheme != 47)
+        ctx->err = "nscheme must be 4, 41, 42, 45 or 47";
     *out = ctx;
     return WCNS_OK;
 }
@@ -557,7 +564,8 @@ int wcns_destroy(wcns_ctx *ctx)
     for (void *p : ps) if (p) hipFree(p);
     for (auto &w : ctx->walls) hipFree(w.xyz);
     void *fs[] = {ctx->d_force_pn, ctx->d_force_part, ctx->d_force_out, ctx->ad_pool, ctx->ad_work, ctx->ad_force,
-                  ctx->d_force_last, ctx->d_fhist, ctx->d_fhist_n, ctx->as_pool, ctx->as_tape, ctx->as_ustart, ctx->as_uent};
+                  ctx->d_force_last, ctx->d_fhist, ctx->d_fhist_n, ctx->as_pool, ctx->as_tape, ctx->as_ustart, ctx->as_uent,
+                  ctx->d_gmet, ctx->d_gmoff, ctx->d_rpres};
     for (void *p : fs) if (p) hipFree(p);
     for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
     for (int c = 0; c < 2; c++)
@@ -806,9 +814,13 @@ int wcns_setup(wcns_ctx *ctx)
     CK(hipSetDevice(ctx->device));
     CK(hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
     const wcns_params &p = ctx->prm;
-    if (p.nscheme != 4 && p.nscheme != 41 && p.nscheme != 42)
-        return fail(ctx, WCNS_E_UNSUPPORTED, "nscheme must be 4, 41 or 42");
-    if (p.nflux < 1 || p.nflux > 3) return fail(ctx, WCNS_E_UNSUPPORTED, "nflux must be 1 (SW), 2 (van Leer) or 3 (Roe)");
+    if (p.nscheme != 4 && p.nscheme != 41 && p.nscheme != 42 && p.nscheme != 45 && p.nscheme != 47)
+        return fail(ctx, WCNS_E_UNSUPPORTED, "nscheme must be 4, 41, 42, 45 or 47");
+    if (p.nflux < 1 || p.nflux > 4)
+        return fail(ctx, WCNS_E_UNSUPPORTED, "nflux must be 1 (SW), 2 (van Leer), 3 (Roe) or 4 (RoeMx)");
+    // invcode dispatches flux_roemx from schemes 42 and 47 only (Convect.f90:34, 47)
+    if (p.nflux == 4 && p.nscheme != 42 && p.nscheme != 47)
+        return fail(ctx, WCNS_E_UNSUPPORTED, "nflux 4 (RoeMx) goes with nscheme 42 or 47");
     if (p.nvis != 0 && p.nvis != 1) return fail(ctx, WCNS_E_ARG, "nvis must be 0 or 1");
     if (p.nlamtur >= 0 && !((p.nlamtur == 1 || p.nlamtur == 2) && p.nvis == 1))
         return fail(ctx, WCNS_E_UNSUPPORTED, "turbulence models: SA (nlamtur = 1) or SST (nlamtur = 2), with nvis = 1");
@@ -1064,6 +1076,36 @@ int wcns_setup(wcns_ctx *ctx)
             double s2 = sqrt(nx * nx + ny * ny + nz * nz);
             g.nrm[0] = nx / s2; g.nrm[1] = ny / s2; g.nrm[2] = nz / s2;
         }
+    // schemes 45 / 47: the ghost metrics at both ends of every pencil
+    // (Convect.f90:180-297), region by region in the BC order
+    if (d.nscheme == 45 || d.nscheme == 47) {
+        std::vector<long> gmoff;
+        long tot = 0;
+        for (auto &b : ctx->blocks) {
+            gmoff.push_back(tot);
+            tot += 24 * ((long)b.nj * b.nk + (long)b.ni * b.nk + (long)b.ni * b.nj);
+        }
+        TRY(dalloc(ctx, &ctx->d_gmet, (size_t)tot));
+        TRY(dalloc(ctx, &ctx->d_gmoff, gmoff.size()));
+        TRY(copy_sync(ctx, ctx->d_gmoff, gmoff.data(), sizeof(long) * gmoff.size(), hipMemcpyHostToDevice));
+        F = ctx->fields();
+        for (size_t nb = 0; nb < ctx->blocks.size(); nb++) {
+            const HBlock &b = ctx->blocks[nb];
+            for (int r : b.bcorder) {
+                const DRegion &g = b.regs[r].d;
+                if (launch_ghost_metrics(g, b.db, ctx->blocks[g.nbt].db, F, ctx->d_gmet, gmoff[nb], ctx->stream))
+                    return fail(ctx, WCNS_E_HIP, "ghost metrics");
+            }
+        }
+    }
+    // flux 4: the pressure-ratio arrays; the entries the per-step kernel
+    // never writes stay +inf
+    if (d.nflux == 4) {
+        TRY(dalloc(ctx, &ctx->d_rpres, 3 * (size_t)T));
+        if (launch_fill_u64(reinterpret_cast<unsigned long long *>(ctx->d_rpres), 3 * T, 0x7FF0000000000000ull,
+                            ctx->stream))
+            return fail(ctx, WCNS_E_HIP, "pressure-ratio fill");
+    }
     // plane-major sweep operands
     ctx->skoff.clear();
     ctx->sktot = 0;
@@ -1388,8 +1430,14 @@ int wcns_r_h_s_invis(wcns_ctx *ctx)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     DFields F = ctx->fields();
+    const bool var = inv_variant(ctx->dp);
+    if (ctx->dp.nflux == 4) {
+        Scope sc(ctx, WCNS_T_INV_I);
+        for (auto &b : ctx->blocks)
+            if (launch_pressure_ratios(b.db, F, ctx->stream)) return fail(ctx, WCNS_E_HIP, "pressure ratios");
+    }
     for (int dir = 0; dir < 3; dir++) {
-        if (dir == 0 && ctx->idir_t) {
+        if (dir == 0 && ctx->idir_t && !var) {
             {
                 Scope st(ctx, WCNS_T_TRANSPOSE);
                 if (stage_transposed(ctx, F, false)) return fail(ctx, WCNS_E_HIP, "transposing the inviscid inputs");
@@ -2251,6 +2299,8 @@ static int ad_check(wcns_ctx *ctx, int nb)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     if (nb < 0 || nb >= (int)ctx->blocks.size()) return fail(ctx, WCNS_E_ARG, "block index");
+    if (inv_variant(ctx->dp))
+        return fail(ctx, WCNS_E_UNSUPPORTED, "tangent and adjoint: schemes 4, 41, 42 with fluxes 1-3");
     const HBlock &b = ctx->blocks[nb];
     if (b.ni <= 4 || b.nj <= 4 || b.nk <= 4)
         return fail(ctx, WCNS_E_UNSUPPORTED, "the residual adjoint needs more than 4 nodes per direction");
@@ -2495,6 +2545,8 @@ static std::vector<double *> state_ptrs(wcns_ctx *ctx)
 static int as_check(wcns_ctx *ctx)
 {
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
+    if (inv_variant(ctx->dp))
+        return fail(ctx, WCNS_E_UNSUPPORTED, "tangent and adjoint: schemes 4, 41, 42 with fluxes 1-3");
     if (ctx->lhs_prec != WCNS_LHS_FP64)
         return fail(ctx, WCNS_E_UNSUPPORTED, "the step tangent and adjoint differentiate the FP64 implicit operator");
     for (auto &b : ctx->blocks)

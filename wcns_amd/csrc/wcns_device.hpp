@@ -56,6 +56,12 @@ struct DFields {
     unsigned long long *counters;       // 0 fallbacks, 1 repairs, 2 roe, 4 repair abort
     double *qke, *vist, *dist;          // SA nu~ / SST k, mu_t, wall distance (null when laminar)
     double *qkw;                        // SST omega (null otherwise)
+    // schemes 45 / 47: four ghost metrics at each end of every pencil
+    // (12 doubles a pencil end; gmoff: each block's first); flux 4: the
+    // pressure ratios of the three directions (3 x total)
+    const double *gmet;
+    const long *gmoff;
+    double *rpres;
     long total;
 };
 
@@ -267,6 +273,145 @@ __device__ inline bool face_flux(const T *ql, const T *qr, T nx, T ny, T nz, T *
     if (FLUX == 2) return flux_vl(ql, qr, nx, ny, nz, f, gama);
     flux_sw(ql, qr, nx, ny, nz, f, efix, gama);
     return true;
+}
+
+// ------------------------------------------- characteristic schemes, RoeMx
+// WCNSE5.f90:4858-4912 pave: Roe average of two primitive states.
+__device__ inline void pave(const double *ql, const double *qr, double *qa, double gama)
+{
+    const double gam1 = gama - 1.0, gam2 = gama / gam1;
+    const double vl2 = ql[1] * ql[1] + ql[2] * ql[2] + ql[3] * ql[3];
+    const double vr2 = qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3];
+    const double hl = gam2 * ql[4] / ql[0] + 0.5 * vl2;
+    const double hr = gam2 * qr[4] / qr[0] + 0.5 * vr2;
+    double rls = sqrt(ql[0]), rrs = sqrt(qr[0]);
+    const double ra = rls * rrs;
+    const double oors = 1.0 / (rls + rrs);
+    rls = rls * oors;
+    rrs = rrs * oors;
+    const double ua = rls * ql[1] + rrs * qr[1], va = rls * ql[2] + rrs * qr[2], wa = rls * ql[3] + rrs * qr[3];
+    const double ha = rls * hl + rrs * hr;
+    const double va2 = ua * ua + va * va + wa * wa;
+    qa[0] = ra; qa[1] = ua; qa[2] = va; qa[3] = wa;
+    qa[4] = (ha - 0.5 * va2) * ra / gam2;
+}
+
+// The unit normal of LxQ / RxQ (WCNSE5.f90:4971-4978).
+__device__ inline void char_normal(const double *gykb, double &nx, double &ny, double &nz)
+{
+    double sn = sqrt(gykb[0] * gykb[0] + gykb[1] * gykb[1] + gykb[2] * gykb[2]);
+    const double oon = 1.0 / (sn > SML_SSS ? sn : SML_SSS);
+    nx = gykb[0] * oon; ny = gykb[1] * oon; nz = gykb[2] * oon;
+}
+
+// WCNSE5.f90:4960-5004 LxQ: the left eigenvectors of the normal gykb at the
+// primitive state prim, applied to dq.
+__device__ inline void lxq(const double *prim, const double *gykb, const double *dq, double *f, double gama)
+{
+    double nx, ny, nz;
+    char_normal(gykb, nx, ny, nz);
+    const double c2 = gama * prim[4] / prim[0];
+    const double cm = sqrt(c2);
+    const double dqn = nx * dq[1] + ny * dq[2] + nz * dq[3];
+    const double wp1 = dq[0] - dq[4] / c2;
+    const double wp2 = dq[4] / (prim[0] * cm);
+    f[0] = nz * dq[2] - ny * dq[3] + nx * wp1;
+    f[1] = -nz * dq[1] + nx * dq[3] + ny * wp1;
+    f[2] = ny * dq[1] - nx * dq[2] + nz * wp1;
+    f[3] = dqn + wp2;
+    f[4] = -dqn + wp2;
+}
+
+// WCNSE5.f90:5006-5050 RxQ: the right eigenvectors, LxQ's inverse.
+__device__ inline void rxq(const double *prim, const double *gykb, const double *dq, double *f, double gama)
+{
+    double nx, ny, nz;
+    char_normal(gykb, nx, ny, nz);
+    const double c2 = gama * prim[4] / prim[0];
+    const double cm = sqrt(c2);
+    const double dqn = nx * dq[0] + ny * dq[1] + nz * dq[2];
+    const double wp1 = 0.5 * prim[0] * (dq[3] + dq[4]);
+    const double wp2 = 0.5 * (dq[3] - dq[4]);
+    f[0] = dqn + wp1 / cm;
+    f[1] = -nz * dq[1] + ny * dq[2] + nx * wp2;
+    f[2] = nz * dq[0] - nx * dq[2] + ny * wp2;
+    f[3] = -ny * dq[0] + nx * dq[1] + nz * wp2;
+    f[4] = wp1 * cm;
+}
+
+// flux.f90:246-393 flux_roemx: the shock-stable Roe flux of Kim and Kim
+// (RoeM), its pressure-wiggle control weighted by the face's least pressure
+// ratio rpmin (Convect.f90:78-200).
+__device__ inline void flux_roemx(const double *ql, const double *qr, double sx, double sy, double sz, double rpmin,
+                                  double *flr, double gama)
+{
+    double sn = sqrt(sx * sx + sy * sy + sz * sz);
+    sn = sn < SML_SSS ? SML_SSS : sn;
+    const double osn = 1.0 / sn;
+    const double nx = sx * osn, ny = sy * osn, nz = sz * osn;
+    const double rl = ql[0] + SMALL, ul = ql[1], vl = ql[2], wl = ql[3], pl = ql[4] + SMALL;
+    const double rr = qr[0] + SMALL, ur = qr[1], vr = qr[2], wr = qr[3], pr = qr[4] + SMALL;
+    const double vl2 = ul * ul + vl * vl + wl * wl;
+    const double vr2 = ur * ur + vr * vr + wr * wr;
+    const double ae = gama - 1.0;
+    double el = pl / (rl * ae), er = pr / (rr * ae);
+    el = el + 0.5 * vl2;
+    const double hl = el + pl / rl;
+    er = er + 0.5 * vr2;
+    const double hr = er + pr / rr;
+    const double vnl = nx * ul + ny * vl + nz * wl;
+    const double vnr = nx * ur + ny * vr + nz * wr;
+    double dq[5], qi[5], mxdq[5];
+    dq[0] = rr - rl;
+    dq[1] = rr * ur - rl * ul;
+    dq[2] = rr * vr - rl * vl;
+    dq[3] = rr * wr - rl * wl;
+    dq[4] = rr * hr - rl * hl;
+    double rls = sqrt(rl), rrs = sqrt(rr);
+    qi[0] = rls * rrs;
+    const double oors = 1.0 / (rls + rrs);
+    rls = rls * oors;
+    rrs = rrs * oors;
+    qi[1] = rls * ul + rrs * ur;
+    qi[2] = rls * vl + rrs * vr;
+    qi[3] = rls * wl + rrs * wr;
+    qi[4] = rls * hl + rrs * hr;
+    const double gami = rls * gama + rrs * gama;
+    const double vi2 = qi[1] * qi[1] + qi[2] * qi[2] + qi[3] * qi[3];
+    const double ci2 = (gami - 1.0) * (qi[4] - 0.5 * vi2);
+    const double ci = sqrt(ci2);
+    const double vni = qi[1] * nx + qi[2] * ny + qi[3] * nz;
+    double b1 = 0.0, b2 = 0.0;
+    if (vni + ci > b1) b1 = vni + ci;
+    if (vnr + ci > b1) b1 = vnr + ci;
+    if (vni - ci < b2) b2 = vni - ci;
+    if (vnl - ci < b2) b2 = vnl - ci;
+    const double oob12 = 1.0 / (b1 - b2);
+    b1 = b1 * oob12;
+    const double b1b2 = b1 * b2;
+    b2 = b2 * oob12;
+    const double mia = fabs(vni / ci);
+    const double fden = vi2 <= SMALL ? 1.0 : pow(mia, 1.0 - rpmin);
+    const double rpmlr = pl / pr < pr / pl ? pl / pr : pr / pl;
+    const double gcdc = mia <= SMALL ? 1.0 : pow(mia, 1.0 - rpmlr);
+    const double dvn = vnr - vnl, dps = pr - pl;
+    const double rmp = dq[0] - fden * dps / ci2;
+    mxdq[0] = rmp;
+    mxdq[1] = rmp * qi[1] + qi[0] * ((ur - ul) - nx * dvn);
+    mxdq[2] = rmp * qi[2] + qi[0] * ((vr - vl) - ny * dvn);
+    mxdq[3] = rmp * qi[3] + qi[0] * ((wr - wl) - nz * dvn);
+    mxdq[4] = rmp * qi[4] + qi[0] * (hr - hl);
+    const double rvnl = b1 * rl * vnl;
+    const double rvnr = -b2 * rr * vnr;
+    const double plr = b1 * pl - b2 * pr;
+    flr[0] = rvnl + rvnr + b1b2 * dq[0];
+    flr[1] = rvnl * ul + rvnr * ur + nx * plr + b1b2 * dq[1];
+    flr[2] = rvnl * vl + rvnr * vr + ny * plr + b1b2 * dq[2];
+    flr[3] = rvnl * wl + rvnr * wr + nz * plr + b1b2 * dq[3];
+    flr[4] = rvnl * hl + rvnr * hr + b1b2 * dq[4];
+    const double cmxd = -gcdc * b1b2 / (1.0 + mia);
+#pragma unroll
+    for (int m = 0; m < 5; m++) flr[m] = (flr[m] + cmxd * mxdq[m]) * sn;
 }
 
 // WCNSE5.f90:2138-2187 (one variable at one node): nonlinear weights from the

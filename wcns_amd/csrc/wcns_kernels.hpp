@@ -69,6 +69,12 @@ long grid_threads(int kind, const DBlock &b);
 int grid_block_size(int kind);
 
 int launch_rhs_inv_dir(int dir, const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s);
+// schemes 45 / 47 or flux 4: the walk of the direct pencils with ghost
+// metrics / pressure ratios
+bool inv_variant(const DParams &P);
+int launch_pressure_ratios(const DBlock &b, const DFields &F, hipStream_t s);
+int launch_ghost_metrics(const DRegion &g, const DBlock &b, const DBlock &bt, const DFields &F, double *gm, long base,
+                         hipStream_t s);
 // The pipelined sweeps of all blocks run in one persistent launch per sweep:
 // every block's (j,k) columns are cut into TJ x TK tiles, one workgroup per
 // tile, and the workgroups follow a work list that is a topological order of

@@ -106,6 +106,14 @@ struct Pencil {
     long base;                     // index of node 0 of the pencil
     long stride;                   // index step along the pencil
     int n;
+    // schemes 45 / 47: the pencil's ghost metrics, nodes 0,-1,-2,-3 (gml) and
+    // n+1..n+4 (gmr), three components each
+    const double *gml, *gmr;
+    // flux 4: pressure ratios of direction d at rp + d * tot; own direction
+    // dir, the other two xd[] with their index steps xs[]
+    const double *rp;
+    long tot, xs[2];
+    int dir, xd[2];
 };
 
 template <class IO>
@@ -156,14 +164,194 @@ __device__ inline void half_metric(const Pencil<IO> &pc, int f, typename IO::T *
     }
 }
 
-// WCNS qwl(node) or qwr(node-1) straight from global memory (boundary rows).
+// The metric triple of node l, l in -3..n+4: the ghost records beyond the ends.
 template <class IO>
-__device__ inline void wcns_at(const Pencil<IO> &pc, int node, typename IO::T *qwl, typename IO::T *qwr)
+__device__ inline void load_mx(const Pencil<IO> &pc, int l, typename IO::T *t)
+{
+    if (l >= 1 && l <= pc.n) { load_m(pc, l, t); return; }
+    const double *g = l < 1 ? pc.gml + 3 * (-l) : pc.gmr + 3 * (l - pc.n - 1);
+#pragma unroll
+    for (int m = 0; m < 3; m++) t[m] = g[m];
+}
+
+// VALUE_HALF_NODE_45 (WCNSE5.f90:5052-5123) at face f in -2..n+2: faces 0..n
+// as VALUE_HALF_NODE, the two beyond each end from the ghost metrics.
+template <class IO>
+__device__ inline void half_metric_x(const Pencil<IO> &pc, int f, typename IO::T *nx)
+{
+    using T = typename IO::T;
+    const int n = pc.n;
+    if (f >= 0 && f <= n) { half_metric(pc, f, nx); return; }
+    T a[3], b[3], c[3], d[3];
+    if (f == -1 || f == n + 1) {
+        const int o = f == -1 ? 0 : n + 1, sg = f == -1 ? -1 : 1;
+        load_mx(pc, o, a); load_mx(pc, o + sg, b); load_mx(pc, o + 2 * sg, c); load_mx(pc, o + 3 * sg, d);
+#pragma unroll
+        for (int m = 0; m < 3; m++) nx[m] = (5.0 * a[m] + 15.0 * b[m] + -5.0 * c[m] + 1.0 * d[m]) / 16.0;
+    } else if (f == -2) {
+        load_mx(pc, -2, a); load_mx(pc, -1, b); load_mx(pc, 0, c); load_mx(pc, -3, d);
+#pragma unroll
+        for (int m = 0; m < 3; m++) nx[m] = (9.0 * (a[m] + b[m]) + -1.0 * (c[m] + d[m])) / 16.0;
+    } else {
+        load_mx(pc, n + 2, a); load_mx(pc, n + 3, b); load_mx(pc, n + 4, c); load_mx(pc, n + 1, d);
+#pragma unroll
+        for (int m = 0; m < 3; m++) nx[m] = (9.0 * (a[m] + b[m]) + -1.0 * (c[m] + d[m])) / 16.0;
+    }
+}
+
+__device__ inline double dmin(double a, double b) { return b < a ? b : a; }
+
+// Convect.f90:169-177 / 335-343 / 499-507: the least pressure ratio around
+// face f -- its own, then four of each other direction (the face's node and
+// the one before it, at the cross index and the next).
+template <class IO>
+__device__ inline double face_rpmin(const Pencil<IO> &pc, int f)
+{
+    const long g = pc.base + (long)f * pc.stride;
+    double v = pc.rp[(long)pc.dir * pc.tot + g];
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+        const double *a = pc.rp + (long)pc.xd[x] * pc.tot;
+        const long sx = pc.xs[x];
+        v = dmin(dmin(dmin(dmin(a[g], a[g + sx]), a[g - pc.stride]), a[g - pc.stride + sx]), v);
+    }
+    return v;
+}
+
+// The face flux of invcode's dispatch (Convect.f90:18-47) at face f.
+template <int FLUX, class IO>
+__device__ inline bool flux_at(const Pencil<IO> &pc, int f, const typename IO::T *ql, const typename IO::T *qr,
+                               const typename IO::T *nx, typename IO::T *flr, double efix, double gama)
+{
+    if constexpr (FLUX == 4) {
+        flux_roemx(ql, qr, nx[0], nx[1], nx[2], face_rpmin(pc, f), flr, gama);
+        return true;
+    } else {
+        return face_flux<FLUX>(ql, qr, nx[0], nx[1], nx[2], flr, efix, gama);
+    }
+}
+
+// WCNS_E_5_45 (WCNSE5.f90:3492-3505): cq(i), the jump q(i+1) - q(i) in the
+// characteristic variables of face i (Roe average of the |rho| states, face
+// metric nxf).
+__device__ inline void char45_jump(const double *qa, const double *qb, const double *nxf, double gama, double *cq)
+{
+    double al[5], ar[5], d[5], qave[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) { al[m] = qa[m]; ar[m] = qb[m]; }
+    al[0] = fabs(al[0]);
+    ar[0] = fabs(ar[0]);
+#pragma unroll
+    for (int m = 0; m < 5; m++) d[m] = ar[m] - al[m];
+    pave(al, ar, qave, gama);
+    lxq(qave, nxf, d, cq, gama);
+}
+
+// The nonlinear weights of one variable from its three (g, s) pairs
+// (WCNSE5.f90:2149-2170, the same in every scheme).
+__device__ inline void wcns_weights(double g1, double g2, double g3, double s1, double s2, double s3, double *wl,
+                                    double *wr)
+{
+    const double EPS = 1.0e-6;
+    double is1 = g1 * g1 + s1 * s1, is2 = g2 * g2 + s2 * s2, is3 = g3 * g3 + s3 * s3;
+    double e1 = (EPS + is1) * (EPS + is1), e2 = (EPS + is2) * (EPS + is2), e3 = (EPS + is3) * (EPS + is3);
+    double bl1 = (1.0 / 16.0) / e1, bl2 = (10.0 / 16.0) / e2, bl3 = (5.0 / 16.0) / e3;
+    double br1 = (5.0 / 16.0) / e1, br2 = (10.0 / 16.0) / e2, br3 = (1.0 / 16.0) / e3;
+    double sl = bl1 + bl2 + bl3, sr = br1 + br2 + br3;
+    wl[0] = bl1 / sl; wl[1] = bl2 / sl; wl[2] = bl3 / sl;
+    wr[0] = br1 / sr; wr[1] = br2 / sr; wr[2] = br3 / sr;
+}
+
+// WCNS_E_5_45 at node nd (WCNSE5.f90:3512-3580) from cq(nd-2..nd+1) and the
+// nodes nd-2..nd+2: qwl(nd) through face nd's eigenvectors, qwr(nd-1)
+// through face nd-1's.
+__device__ inline void char45_node(const double (*cq)[5], const double (*qw)[5], const double *nxm1,
+                                   const double *nx0, double gama, double *qwl, double *qwr)
+{
+    double al[5], ar[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        double g1 = 0.5 * (3.0 * cq[1][m] - cq[0][m]);
+        double g2 = 0.5 * (cq[2][m] + cq[1][m]);
+        double g3 = 0.5 * (3.0 * cq[2][m] - cq[3][m]);
+        double s1 = cq[1][m] - cq[0][m], s2 = cq[2][m] - cq[1][m], s3 = cq[3][m] - cq[2][m];
+        double wl[3], wr[3];
+        wcns_weights(g1, g2, g3, s1, s2, s3, wl, wr);
+        al[m] = 0.125 * (wl[0] * (s1 + 4.0 * g1) + wl[1] * (s2 + 4.0 * g2) + wl[2] * (s3 + 4.0 * g3));
+        ar[m] = 0.125 * (wr[0] * (s1 - 4.0 * g1) + wr[1] * (s2 - 4.0 * g2) + wr[2] * (s3 - 4.0 * g3));
+    }
+    double qave[5], t[5];
+    pave(qw[2], qw[3], qave, gama);
+    rxq(qave, nx0, al, t, gama);
+#pragma unroll
+    for (int m = 0; m < 5; m++) qwl[m] = qw[2][m] + t[m];
+    pave(qw[1], qw[2], qave, gama);
+    rxq(qave, nxm1, ar, t, gama);
+#pragma unroll
+    for (int m = 0; m < 5; m++) qwr[m] = qw[2][m] + t[m];
+}
+
+// WCNS_E_5_47 at node nd (WCNSE5.f90:4200-4278): differences of the
+// primitives, taken with the node's own value into the characteristic
+// variables of the node's state and metric mt, weighted there, and both
+// states back through the same eigenvectors.
+__device__ inline void char47_node(const double (*qw)[5], const double *mt, double gama, double *qwl, double *qwr)
+{
+    const double *q0 = qw[2];
+    double lq[5], g[3][5], s[3][5], lg[3][5], ls[3][5];
+    lxq(q0, mt, q0, lq, gama);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        g[0][m] = 0.5 * (qw[0][m] - 4.0 * qw[1][m] + 3.0 * qw[2][m]);
+        g[1][m] = 0.5 * (qw[3][m] - qw[1][m]);
+        g[2][m] = 0.5 * (-3.0 * qw[2][m] + 4.0 * qw[3][m] - qw[4][m]);
+        s[0][m] = qw[0][m] - 2.0 * qw[1][m] + qw[2][m];
+        s[1][m] = qw[1][m] - 2.0 * qw[2][m] + qw[3][m];
+        s[2][m] = qw[2][m] - 2.0 * qw[3][m] + qw[4][m];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        lxq(q0, mt, g[k], lg[k], gama);
+        lxq(q0, mt, s[k], ls[k], gama);
+    }
+    double a[5], b[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        double wl[3], wr[3];
+        wcns_weights(lg[0][m], lg[1][m], lg[2][m], ls[0][m], ls[1][m], ls[2][m], wl, wr);
+        a[m] = lq[m] + 0.125 * (wl[0] * (ls[0][m] + 4.0 * lg[0][m]) + wl[1] * (ls[1][m] + 4.0 * lg[1][m]) +
+                                wl[2] * (ls[2][m] + 4.0 * lg[2][m]));
+        b[m] = lq[m] + 0.125 * (wr[0] * (ls[0][m] - 4.0 * lg[0][m]) + wr[1] * (ls[1][m] - 4.0 * lg[1][m]) +
+                                wr[2] * (ls[2][m] - 4.0 * lg[2][m]));
+    }
+    rxq(q0, mt, a, qwl, gama);
+    rxq(q0, mt, b, qwr, gama);
+}
+
+// The scheme's qwl(node) and qwr(node-1) straight from global memory
+// (boundary rows): SCH 0 the variable-wise WCNS of schemes 4, 41, 42.
+template <int SCH, class IO>
+__device__ inline void wcns_at(const Pencil<IO> &pc, int node, typename IO::T *qwl, typename IO::T *qwr, double gama)
 {
     typename IO::T q[5][5];
     for (int l = 0; l < 5; l++) load_q(pc, node - 2 + l, q[l]);
+    if constexpr (SCH == 47) {
+        double mt[3];
+        load_mx(pc, node, mt);
+        char47_node(q, mt, gama, qwl, qwr);
+    } else if constexpr (SCH == 45) {
+        double cq[4][5], nxf[3], nxm1[3], nx0[3];
+        for (int l = 0; l < 4; l++) {
+            half_metric_x(pc, node - 2 + l, nxf);
+            char45_jump(q[l], q[l + 1], nxf, gama, cq[l]);
+        }
+        half_metric_x(pc, node - 1, nxm1);
+        half_metric_x(pc, node, nx0);
+        char45_node(cq, q, nxm1, nx0, gama, qwl, qwr);
+    } else {
 #pragma unroll
-    for (int m = 0; m < 5; m++) wcns_node(q[0][m], q[1][m], q[2][m], q[3][m], q[4][m], qwl[m], qwr[m]);
+        for (int m = 0; m < 5; m++) wcns_node(q[0][m], q[1][m], q[2][m], q[3][m], q[4][m], qwl[m], qwr[m]);
+    }
 }
 
 // Fallback-checked face flux (WCNSE5.f90:2200-2262).  mode 0: ql<-q(max(f,1)),
@@ -191,11 +379,11 @@ __device__ inline void checked_flux(const Pencil<IO> &pc, int f, typename IO::T 
             if (fr) { for (int m = 0; m < 5; m++) qr[m] = (qa[m] + qb[m]) / 2.0; nerr++; }
         }
     }
-    if (!face_flux<FLUX>(ql, qr, nx[0], nx[1], nx[2], flr, efix, gama)) bad = 1;
+    if (!flux_at<FLUX>(pc, f, ql, qr, nx, flr, efix, gama)) bad = 1;
 }
 
 // The WCNS_E_5_41 left boundary rows 1..3 (WCNSE5.f90:2269-2513).
-template <int FLUX, class IO>
+template <int SCH, int FLUX, class IO>
 __device__ __noinline__ void left41(const Pencil<IO> &pc, typename IO::T *df1, typename IO::T *df2,
                                     typename IO::T *df3, double efix, double gama, bool s42, int &nerr, int &bad)
 {
@@ -205,7 +393,7 @@ __device__ __noinline__ void left41(const Pencil<IO> &pc, typename IO::T *df1, t
     // u_r_old(1..4): qwr after the scheme-4 closures; qwr(1) = (5,15,-5,1)/16, qwr(2..4) WCNS
 #pragma unroll
     for (int m = 0; m < 5; m++) urold[1][m] = (5.0 * q1[m] + 15.0 * q2[m] - 5.0 * q3[m] + q4[m]) / 16.0;
-    for (int i = 2; i <= 4; i++) wcns_at(pc, i + 1, tmp, urold[i]);
+    for (int i = 2; i <= 4; i++) wcns_at<SCH>(pc, i + 1, tmp, urold[i], gama);
     // (1) faces 0..3 -> df(1); scheme 42's states WCNSE5.f90:2967-2986
     for (int i = 0; i <= 3; i++) {
         for (int m = 0; m < 5; m++) {
@@ -262,7 +450,7 @@ __device__ __noinline__ void left41(const Pencil<IO> &pc, typename IO::T *df1, t
 }
 
 // The WCNS_E_5_41 right boundary rows n-2..n (WCNSE5.f90:2518-2751).  f[k] holds face n-k.
-template <int FLUX, class IO>
+template <int SCH, int FLUX, class IO>
 __device__ __noinline__ void right41(const Pencil<IO> &pc, typename IO::T *dfn, typename IO::T *dfn1,
                                      typename IO::T *dfn2, double efix, double gama, bool s42, int &nerr, int &bad)
 {
@@ -273,7 +461,7 @@ __device__ __noinline__ void right41(const Pencil<IO> &pc, typename IO::T *dfn, 
     // u_l_old(9-i) = qwl(n-i), i=1..4: qwl(n-1) = (5,15,-5,1)/16 closure, qwl(n-2..n-4) WCNS
 #pragma unroll
     for (int m = 0; m < 5; m++) ulold[8][m] = (5.0 * qn[m] + 15.0 * qn1[m] - 5.0 * qn2[m] + qn3[m]) / 16.0;
-    for (int i = 2; i <= 4; i++) wcns_at(pc, n - i, ulold[9 - i], tmp);
+    for (int i = 2; i <= 4; i++) wcns_at<SCH>(pc, n - i, ulold[9 - i], tmp, gama);
     // (4) faces n..n-3 -> df(n); scheme 42's states WCNSE5.f90:3207-3226
     for (int i = n; i >= n - 3; i--) {
         int o = n - i;
@@ -351,7 +539,7 @@ __device__ __noinline__ void closure_state(const Pencil<IO> &pc, int which, type
 
 // One pencil's rows a..bb (DIR only selects nothing here: the pencil carries
 // its own base, stride and length).  The product instance adds the rows to R.
-template <int FLUX, class IO>
+template <int FLUX, class IO, int SCH = 0>
 __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double gama, double efix, bool s41, bool s42,
                                   int &nerr, int &bad)
 {
@@ -372,18 +560,47 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
     T qwl_prev[5];
 
     for (int l = 0; l < 4; l++) load_q(pc, F0 - 2 + l, qw[l + 1]);   // qw[1..4] = F0-2..F0+1
+    // scheme 45: cq(nd-2..nd+1), rolled with the node window
+    double cqw[SCH == 45 ? 4 : 1][5];
+    if constexpr (SCH == 45) {
+        for (int l = 0; l < 3; l++) {
+            double nxf[3];
+            half_metric_x(pc, F0 - 2 + l, nxf);
+            char45_jump(qw[1 + l], qw[2 + l], nxf, gama, cqw[1 + l]);
+        }
+    }
     for (int nd = F0; nd <= F1 + 1; nd++) {
 #pragma unroll
         for (int l = 0; l < 4; l++)
 #pragma unroll
             for (int m = 0; m < 5; m++) qw[l][m] = qw[l + 1][m];
         load_q(pc, nd + 2, qw[4]);
+        if constexpr (SCH == 45) {
+#pragma unroll
+            for (int l = 0; l < 3; l++)
+#pragma unroll
+                for (int m = 0; m < 5; m++) cqw[l][m] = cqw[l + 1][m];
+            double nxf[3];
+            half_metric_x(pc, nd + 1, nxf);
+            char45_jump(qw[3], qw[4], nxf, gama, cqw[3]);
+        }
         T qwl_cur[5], qwr_m1[5];
         bool need = (nd >= ist && nd <= ied + 1);
         if (need) {
+            if constexpr (SCH == 47) {
+                double mt[3];
+                load_mx(pc, nd, mt);
+                char47_node(qw, mt, gama, qwl_cur, qwr_m1);
+            } else if constexpr (SCH == 45) {
+                double nxm1[3], nx0[3];
+                half_metric_x(pc, nd - 1, nxm1);
+                half_metric_x(pc, nd, nx0);
+                char45_node(cqw, qw, nxm1, nx0, gama, qwl_cur, qwr_m1);
+            } else {
 #pragma unroll
-            for (int m = 0; m < 5; m++)
-                wcns_node(qw[0][m], qw[1][m], qw[2][m], qw[3][m], qw[4][m], qwl_cur[m], qwr_m1[m]);
+                for (int m = 0; m < 5; m++)
+                    wcns_node(qw[0][m], qw[1][m], qw[2][m], qw[3][m], qw[4][m], qwl_cur[m], qwr_m1[m]);
+            }
         }
         // closure overrides of qwl(nd) and qwr(nd-1); the right side wins on overlap
         {
@@ -419,7 +636,7 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
             for (int l = 0; l < 5; l++)
 #pragma unroll
                 for (int m = 0; m < 5; m++) fw[l][m] = fw[l + 1][m];
-            if (!face_flux<FLUX>(ql, qr, nx[0], nx[1], nx[2], fw[5], efix, gama)) bad = 1;
+            if (!flux_at<FLUX>(pc, fc, ql, qr, nx, fw[5], efix, gama)) bad = 1;
             // emit the rows that became computable (FLUX_DXYZ, WCNSE5.f90:229-251)
             T df[5];
             if (fc == 3) {
@@ -466,14 +683,14 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
 
     if (s41 && lclos && a == 1) {
         T d1[5], d2[5], d3[5];
-        left41<FLUX>(pc, d1, d2, d3, efix, gama, s42, nerr, bad);
+        left41<SCH, FLUX>(pc, d1, d2, d3, efix, gama, s42, nerr, bad);
         emit_row(pc, 1, d1);
         emit_row(pc, 2, d2);
         emit_row(pc, 3, d3);
     }
     if (s41 && rclos && bb == n) {
         T dn[5], dn1[5], dn2[5];
-        right41<FLUX>(pc, dn, dn1, dn2, efix, gama, s42, nerr, bad);
+        right41<SCH, FLUX>(pc, dn, dn1, dn2, efix, gama, s42, nerr, bad);
         emit_row(pc, n, dn);
         emit_row(pc, n - 1, dn1);
         emit_row(pc, n - 2, dn2);
@@ -498,8 +715,30 @@ __device__ inline void metric_triple(const DFields &F, const double *&mx, const 
     mz = DIR == 0 ? F.kcz : (DIR == 1 ? F.etz : F.ctz);
 }
 
+// The ghost-metric record of pencil (ia, ib) of direction d at end `side`
+// (0 low, 1 high): per block, direction by direction, the low end's pencils
+// then the high end's, 12 doubles each.
+__device__ inline long gm_record(const DBlock &b, long base, int d, int side, int ia, int ib)
+{
+    const long a0 = (long)b.nj * b.nk, a1 = (long)b.ni * b.nk, a2 = (long)b.ni * b.nj;
+    const long na = d == 0 ? b.nj : b.ni;
+    long off = base;
+    if (d >= 1) off += 2 * 12 * a0;
+    if (d >= 2) off += 2 * 12 * a1;
+    off += (long)side * 12 * (d == 0 ? a0 : (d == 1 ? a1 : a2));
+    return off + 12 * ((long)(ib - 1) * na + (ia - 1));
+}
+
+__device__ inline void metric_triple_of(const DFields &F, int d, const double *&mx, const double *&my,
+                                        const double *&mz)
+{
+    mx = d == 0 ? F.kcx : (d == 1 ? F.etx : F.ctx);
+    my = d == 0 ? F.kcy : (d == 1 ? F.ety : F.cty);
+    mz = d == 0 ? F.kcz : (d == 1 ? F.etz : F.ctz);
+}
+
 // DIR0 = 3: the i direction run as j pencils of the transposed view.
-template <int DIR0, int FLUX, class IO>
+template <int DIR0, int FLUX, class IO, int SCH = 0>
 __device__ inline void rhs_inv_thread(const BlockGrid &G, const DFields &F, const DParams &P)
 {
     constexpr int DIR = DIR0 == 3 ? 1 : DIR0;
@@ -524,12 +763,26 @@ __device__ inline void rhs_inv_thread(const BlockGrid &G, const DFields &F, cons
     pc.io.total = F.total;
     pc.r = F.r;
     pencil_at<DIR>(b, ia, ib, pc.base, pc.stride, pc.n);
+    if constexpr (SCH != 0) {
+        const long base = F.gmoff[G.wg[blockIdx.x].x];
+        pc.gml = F.gmet + gm_record(b, base, DIR, 0, ia, ib);
+        pc.gmr = F.gmet + gm_record(b, base, DIR, 1, ia, ib);
+    }
+    if constexpr (FLUX == 4) {
+        pc.rp = F.rpres;
+        pc.tot = F.total;
+        pc.dir = DIR;
+        pc.xd[0] = DIR == 0 ? 1 : 0;
+        pc.xd[1] = DIR == 2 ? 1 : 2;
+        pc.xs[0] = DIR == 0 ? b.sx : 1;
+        pc.xs[1] = DIR == 2 ? b.sx : b.sy;
+    }
 
     int a = seg * SEG + 1;
     int bb = a + SEG - 1;
     if (bb > n) bb = n;
     int nerr = 0, bad = 0;
-    inv_pencil<FLUX>(pc, a, bb, P.gama, P.efix, P.nscheme >= 41, P.nscheme == 42, nerr, bad);
+    inv_pencil<FLUX, IO, SCH>(pc, a, bb, P.gama, P.efix, P.nscheme >= 41, P.nscheme >= 42, nerr, bad);
     if (nerr) atomicAdd(&F.counters[0], (unsigned long long)nerr);
     if (bad) atomicAdd(&F.counters[2], 1ull);
 }
@@ -549,6 +802,85 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) k
                                                                                             DParams P)
 {
     rhs_inv_thread<DIR0, FLUX, InvIO2w>(G, F, P);
+}
+
+// Schemes 45 / 47 (SCH) and the RoeMx flux (FLUX 4), direct pencils only.
+template <int DIR, int FLUX, int SCH>
+__global__ void __launch_bounds__(128) k_rhs_inv_var(BlockGrid G, DFields F, DParams P)
+{
+    rhs_inv_thread<DIR, FLUX, InvIO, SCH>(G, F, P);
+}
+
+// Convect.f90:78-132: the pressure ratios of direction D along line (a, b)
+// (the two other indices, each over 0..n+1), written over -1..n+1 with the
+// ghost pair of a physical end (ghost density below small) taken from the
+// first interior entry, in the reference's order.  Entries no line writes
+// keep the +inf they were given once.
+template <int D>
+__global__ void __launch_bounds__(256) k_pressure_ratios(DBlock b, DFields F)
+{
+    const int n = D == 0 ? b.ni : (D == 1 ? b.nj : b.nk);
+    const int na = (D == 0 ? b.nj : b.ni) + 2, nb = (D == 2 ? b.nj : b.nk) + 2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)na * nb) return;
+    const int la = (int)(t % na), lb = (int)(t / na);
+    const int i0 = D == 0 ? 0 : la, j0 = D == 1 ? 0 : (D == 0 ? la : lb), k0 = D == 2 ? 0 : lb;
+    const long g0 = gidx(b, i0, j0, k0);
+    const long st = D == 0 ? 1 : (D == 1 ? b.sx : b.sy);
+    double *rp = F.rpres + (long)D * F.total;
+    for (int l = -1; l <= n + 1; l++) {
+        const double pl = F.p[g0 + l * st], pr = F.p[g0 + (l + 1) * st];
+        rp[g0 + l * st] = dmin(pl / pr, pr / pl);
+    }
+    if (F.r[g0 - 2 * st] < SMALL) {
+        rp[g0] = rp[g0 + st];
+        rp[g0 - st] = rp[g0 + st];
+    }
+    if (F.r[g0 + (n + 3) * st] < SMALL) {
+        rp[g0 + n * st] = rp[g0 + (n - 1) * st];
+        rp[g0 + (n + 1) * st] = rp[g0 + (n - 1) * st];
+    }
+}
+
+// One region's share of the ghost metrics (Convect.f90:180-297): for every
+// face node, the four ghosts of the pencil that ends there -- the target
+// block's metrics of its own normal direction 1..4 nodes inside its face,
+// signed by the target side, for an interface; the end node's own metric
+// repeated for any other region.  Launched region by region in the BC
+// order, so that the last region holding a node decides (fill_bc_flag,
+// PreProcess.f90:820-857).
+__global__ void __launch_bounds__(256) k_ghost_metrics(DRegion g, DBlock b, DBlock bt, DFields F, double *gm,
+                                                       long base)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.npts) return;
+    int i, j, k;
+    region_point(g, t, i, j, k);
+    const int d = g.s_nd - 1, side = g.s_lr < 0 ? 0 : 1;
+    const int ia = d == 0 ? j : i, ib = d == 2 ? j : k;
+    double *o = gm + gm_record(b, base, d, side, ia, ib);
+    const double *mx, *my, *mz;
+    if (g.bctype < 0) {
+        int it, jt, kt;
+        region_image(g, i, j, k, it, jt, kt);
+        const int td = g.t_l3[0] ? 0 : (g.t_l3[1] ? 1 : 2);
+        const double sg = side == 0 ? (double)g.t_l3[td] : -(double)g.t_l3[td];
+        metric_triple_of(F, td, mx, my, mz);
+        for (int L = 1; L <= 4; L++) {
+            const long q = gidx(bt, it - L * g.t_l3[0], jt - L * g.t_l3[1], kt - L * g.t_l3[2]);
+            o[3 * (L - 1)] = sg * mx[q];
+            o[3 * (L - 1) + 1] = sg * my[q];
+            o[3 * (L - 1) + 2] = sg * mz[q];
+        }
+    } else {
+        metric_triple_of(F, d, mx, my, mz);
+        const long q = gidx(b, i, j, k);
+        for (int L = 1; L <= 4; L++) {
+            o[3 * (L - 1)] = mx[q];
+            o[3 * (L - 1) + 1] = my[q];
+            o[3 * (L - 1) + 2] = mz[q];
+        }
+    }
 }
 
 // Tangent: dR += (dR_inv/dP) dP + (dR_inv/dM) dM along DIR for one block, the
@@ -661,16 +993,66 @@ static int simd_count()
     return 4 * ncu;
 }
 
+template <int FLUX, int SCH>
+static void launch_var(int dir, dim3 grid, dim3 blk, hipStream_t s, const BlockGrid &G, const DFields &F,
+                       const DParams &P)
+{
+    if (dir == 0) hipLaunchKernelGGL((k_rhs_inv_var<0, FLUX, SCH>), grid, blk, 0, s, G, F, P);
+    if (dir == 1) hipLaunchKernelGGL((k_rhs_inv_var<1, FLUX, SCH>), grid, blk, 0, s, G, F, P);
+    if (dir == 2) hipLaunchKernelGGL((k_rhs_inv_var<2, FLUX, SCH>), grid, blk, 0, s, G, F, P);
+}
+
+template <int SCH>
+static void launch_var_flux(int nflux, int dir, dim3 grid, dim3 blk, hipStream_t s, const BlockGrid &G,
+                            const DFields &F, const DParams &P)
+{
+    if (nflux == 4) launch_var<4, SCH>(dir, grid, blk, s, G, F, P);
+    else if (nflux == 3) launch_var<3, SCH>(dir, grid, blk, s, G, F, P);
+    else if (nflux == 2) launch_var<2, SCH>(dir, grid, blk, s, G, F, P);
+    else launch_var<1, SCH>(dir, grid, blk, s, G, F, P);
+}
+
+bool inv_variant(const DParams &P) { return P.nscheme == 45 || P.nscheme == 47 || P.nflux == 4; }
+
 int launch_rhs_inv_dir(int dir, const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s)
 {
     if (G.nwg == 0) return 0;
     dim3 grid((unsigned)G.nwg), blk(128);
+    if (inv_variant(P)) {
+        // the variants run on the direct pencils only (dir 0..2)
+        if (dir > 2) return -3;
+        if (P.nscheme == 45) launch_var_flux<45>(P.nflux, dir, grid, blk, s, G, F, P);
+        else if (P.nscheme == 47) launch_var_flux<47>(P.nflux, dir, grid, blk, s, G, F, P);
+        else launch_var<4, 0>(dir, grid, blk, s, G, F, P);
+        return hipGetLastError() == hipSuccess ? 0 : -3;
+    }
     // two 64-wide waves per workgroup; the two-wave budget once they exceed
     // 1.5 waves per SIMD
     const bool two = P.inv_waves ? P.inv_waves == 2 : 2L * G.nwg * 2 > 3L * simd_count();
     if (P.nflux == 3) launch_inv<3>(dir, two, grid, blk, s, G, F, P);
     else if (P.nflux == 2) launch_inv<2>(dir, two, grid, blk, s, G, F, P);
     else launch_inv<1>(dir, two, grid, blk, s, G, F, P);
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_pressure_ratios(const DBlock &b, const DFields &F, hipStream_t s)
+{
+    for (int d = 0; d < 3; d++) {
+        const long na = (d == 0 ? b.nj : b.ni) + 2, nb = (d == 2 ? b.nj : b.nk) + 2;
+        dim3 grid((unsigned)((na * nb + 255) / 256)), blk(256);
+        if (d == 0) hipLaunchKernelGGL(k_pressure_ratios<0>, grid, blk, 0, s, b, F);
+        if (d == 1) hipLaunchKernelGGL(k_pressure_ratios<1>, grid, blk, 0, s, b, F);
+        if (d == 2) hipLaunchKernelGGL(k_pressure_ratios<2>, grid, blk, 0, s, b, F);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_ghost_metrics(const DRegion &g, const DBlock &b, const DBlock &bt, const DFields &F, double *gm, long base,
+                         hipStream_t s)
+{
+    if (g.npts <= 0) return 0;
+    hipLaunchKernelGGL(k_ghost_metrics, dim3((unsigned)((g.npts + 255) / 256)), dim3(256), 0, s, g, b, bt, F, gm,
+                       base);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
