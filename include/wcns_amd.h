@@ -42,8 +42,9 @@ typedef struct {
     double cfl, timedt_rate, efix, csrv, gama, prl, prt;
     int nvis;      /* 0 Euler, 1 Navier-Stokes                          */
     int nlamtur;   /* -1 laminar, 1 Spalart-Allmaras, 2 SST k-omega      */
-    int nscheme;   /* 4 (WCNS_E_5) or 41 (WCNS_E_5_41)                   */
-    int nflux;     /* 1 Steger-Warming, 3 Roe                            */
+    int nscheme;   /* 4 (WCNS_E_5), 41, 42, 45 or 47 (WCNS_E_5_4x)        */
+    int nflux;     /* 1 Steger-Warming, 2 van Leer, 3 Roe, 4 RoeMx (with  */
+                   /* nscheme 42 or 47, Convect.f90:34, 47)                */
     int ntmst;     /* 0 local time step                                  */
     /* namelist force (param.dat): reference area, length and moment centre */
     double sref, lfref, xref, yref, zref;

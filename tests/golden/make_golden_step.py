@@ -16,7 +16,17 @@ no interface averaging, which exists only in its MPI build):
           MPI build);
   repair  the Cartesian case with two nodes above rmax_limit and two
           near-vacuum pressures (ordered repair, WCNS positivity fallback):
-          2 steps.
+          2 steps;
+  s45, s47wall, s45vis
+          WCNS_E_5_45 / _47 (characteristic interpolation, ghost metrics
+          from the interface partners): a 20x14x12 periodic box with SW, two
+          blocks with a slip wall at M 2 with Roe, four viscous blocks with
+          van Leer: 2 steps each;
+  roemx42, roemx47
+          the RoeMx flux with schemes 42 and 47 on the periodic box, the heap
+          block of inviscd3d's pressure ratios filled with 1e300 before every
+          r_h_s_invis (the entries its loops never write then hold a value
+          that no ratio undercuts): 2 steps.
 Stored per case: the residual monitor of every step (rms, max, location) and
 the SHA-256 of the final r, u, v, w, p over the interior nodes of every block
 (little-endian float64, block by block, k, j, i order), plus the force.
@@ -47,7 +57,17 @@ def golden_cases():
     sa.params.nwholefield = 0
     out["sa"] = (sa, 4, True)
     out["repair"] = (cases.cartesian_periodic_case(ni=24, nj=16, nk=12, seed=3), 2, False)
+    out["s45"] = (cases.cartesian_periodic_case(ni=20, nj=14, nk=12, seed=3, nscheme=45, nflux=1), 2, False)
+    out["s47wall"] = (cases.multiblock_case(nx=(14, 12), ny=(12,), nz=(10,), nvis=0, wall=True, moo=2.0, nflux=3,
+                                            nscheme=47), 2, False)
+    out["s45vis"] = (cases.multiblock_case(nvis=1, wall=True, periodic_i=False, ilo_bc=5, ihi_bc=6, jhi_bc=3,
+                                           reynolds=1e5, nscheme=45, nflux=2), 2, True)
+    out["roemx42"] = (cases.cartesian_periodic_case(ni=20, nj=14, nk=12, seed=3, nscheme=42, nflux=4), 2, False)
+    out["roemx47"] = (cases.cartesian_periodic_case(ni=20, nj=14, nk=12, seed=3, nscheme=47, nflux=4), 2, False)
     return out
+
+
+ROEMX_FILL = 1.0e300
 
 
 def poke_repair(target):
@@ -86,6 +106,8 @@ def main():
         hist = []
         for _ in range(nsteps):
             for st in (STAGES_VIS if vis else STAGES_INV):
+                if st == "rhs_invis" and case.params.nflux == 4:
+                    r.poison(0, ROEMX_FILL)
                 r.stage(st)
             rms, rmax, loc = r.residual()
             hist.append([rms, rmax] + [float(x) for x in loc])

@@ -294,7 +294,7 @@ def test_wall_turbulence_values_against_reference(nlamtur):
 STEP_GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "step_golden.npz")
 
 
-@pytest.mark.parametrize("tag", ["cart", "bump", "sa", "repair"])
+@pytest.mark.parametrize("tag", ["cart", "bump", "sa", "repair", "s45", "s47wall", "s45vis", "roemx42", "roemx47"])
 def test_whole_steps_against_reference_fixtures(tag):
     """The oracle's steps against the reference's own serial step routines
     (tests/golden/make_golden_step.py): every residual-monitor entry (rms, max
