@@ -6,6 +6,9 @@ and branches that the stage-by-stage tests do not reach:
   Mach 0.85 (the forward part of the multi-step adjoint workload), streamwise
   inflow (BC 5) / outflow (BC 6), a no-slip wall, far field on top, 1:1
   interfaces, periodic span: 20 steps replayed from the HIP graph;
+* the bench workload itself, the 256x128x64 bump channel at wall spacing
+  1e-5 and growth 1.1: 3 steps (plain, then graph replays with the tuned
+  sweep tiles) against the oracle;
 * the FP32 implicit operator against the FP64 oracle, with a stated bound;
 * the inviscid wall (BC 2 with nvis = 0) at Mach 2;
 * forced failure paths: a repaired node and the WCNS positivity fallback, the
@@ -88,6 +91,32 @@ def test_uneven_64_blocks_graph_20_steps(moo, model, reynolds):
     assert s.counter(0) == o.counter(0) and s.counter(1) == o.counter(1)
     o.close()
     s.close()
+
+
+def test_bump_channel_full_size_against_oracle():
+    """The bench workload itself: the 256x128x64 bump channel (bump 0.05 L,
+    wall spacing 1e-5 L, growth 1.1, laminar Re 1e5, M 0.5, Roe, scheme 41),
+    one plain step, the tile autotuner, then 2 steps from the HIP graph with
+    the tuned sweep tiles, against 3 oracle steps: the residual monitor within 1e-10, the primitives within
+    1e-10 of each field's scale, the fallback and repair counters equal."""
+    from wcns_amd.solver import Solver
+    case = cases.bump_channel_case(ni=256, nj=128, nk=64)
+    print("bump 256x128x64: GPU", flush=True)
+    s = Solver(case)
+    s.step(1, history=False)
+    s.autotune(reps=1)           # tile shapes timed on the stepped state
+    s.step_graph(2)
+    hs = s.residual_history(3)[0]
+    print("bump 256x128x64: oracle", flush=True)
+    o = _oracle(case)
+    ho = o.step(3)
+    rel = np.max(np.abs(hs - ho) / ho)
+    e = _max_rel(s, o, case)
+    print(f"monitor rel {rel:.2e}, primitives rel {e:.2e}", flush=True)
+    assert rel <= 1e-10 and e <= 1e-10
+    assert s.counter(0) == o.counter(0) and s.counter(1) == o.counter(1)
+    s.close()
+    o.close()
 
 
 def test_fp32_operator_error_bound():
