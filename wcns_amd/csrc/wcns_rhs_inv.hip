@@ -28,6 +28,7 @@ struct InvIO {
     using T = double;
     static constexpr int SEGLEN = SEG;
     static constexpr bool COUNT = true;
+    static constexpr bool PRE_ROW = true;    // rows are plain adds to rhs: read one iteration ahead
     const double *r, *u, *v, *w, *p;
     const double *mx, *my, *mz;    // metric triple of the pencil direction
     double *rhs;
@@ -39,6 +40,16 @@ struct InvIO {
 #pragma unroll
         for (int c = 0; c < 5; c++) rhs[c * total + g] = rhs[c * total + g] + df[c];
     }
+    __device__ void row_read(long g, double *o) const
+    {
+#pragma unroll
+        for (int c = 0; c < 5; c++) o[c] = rhs[c * total + g];
+    }
+    __device__ void row_write(long g, const double *pre, const double *df) const
+    {
+#pragma unroll
+        for (int c = 0; c < 5; c++) rhs[c * total + g] = pre[c] + df[c];
+    }
 };
 
 // InvIO for the two-wave register budget: a type of its own, so that the
@@ -49,6 +60,7 @@ struct InvTanIO {
     using T = D1;
     static constexpr int SEGLEN = SEG;
     static constexpr bool COUNT = false;
+    static constexpr bool PRE_ROW = false;
     const double *r, *u, *v, *w, *p, *mx, *my, *mz;
     const double *dr, *du, *dv, *dw, *dp, *dmx, *dmy, *dmz;
     double *drhs;
@@ -72,6 +84,7 @@ struct InvVjpIO {
     using T = D1;
     static constexpr int SEGLEN = 1 << 30;     // one thread walks the whole pencil
     static constexpr bool COUNT = false;
+    static constexpr bool PRE_ROW = false;
     const double *r, *u, *v, *w, *p, *mx, *my, *mz;
     const double *wt;                          // adjoint weights of R (5 arrays)
     double *lam;
@@ -560,6 +573,15 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
     T qwl_prev[5];
 
     for (int l = 0; l < 4; l++) load_q(pc, F0 - 2 + l, qw[l + 1]);   // qw[1..4] = F0-2..F0+1
+    // one iteration ahead: the next node, the next metric of the rolled
+    // window mw = m(nd-2..nd+1), and (solver) the next interior residual row
+    T qnext[5], mw[4][3], mnext[3];
+    load_q(pc, F0 + 2, qnext);
+    for (int l = 0; l < 3; l++) load_m(pc, F0 - 2 + l, mw[l + 1]);
+    load_m(pc, F0 + 1, mnext);
+    double rpre[IO::PRE_ROW ? 5 : 1];
+    if constexpr (IO::PRE_ROW)
+        if (F0 - 3 >= 1) pc.io.row_read(pc.base + (long)(F0 - 3) * pc.stride, rpre);
     // scheme 45: cq(nd-2..nd+1), rolled with the node window
     double cqw[SCH == 45 ? 4 : 1][5];
     if constexpr (SCH == 45) {
@@ -574,7 +596,18 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
         for (int l = 0; l < 4; l++)
 #pragma unroll
             for (int m = 0; m < 5; m++) qw[l][m] = qw[l + 1][m];
-        load_q(pc, nd + 2, qw[4]);
+#pragma unroll
+        for (int m = 0; m < 5; m++) qw[4][m] = qnext[m];
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+#pragma unroll
+            for (int m = 0; m < 3; m++) mw[l][m] = mw[l + 1][m];
+#pragma unroll
+        for (int m = 0; m < 3; m++) mw[3][m] = mnext[m];
+        if (nd <= F1) {
+            load_q(pc, nd + 3, qnext);
+            load_m(pc, nd + 2, mnext);
+        }
         if constexpr (SCH == 45) {
 #pragma unroll
             for (int l = 0; l < 3; l++)
@@ -623,7 +656,14 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
         int fc = nd - 1;
         if (fc >= F0) {
             T nx[3];
-            half_metric(pc, fc, nx);
+            if (fc >= 2 && fc <= n - 2) {
+                // VALUE_HALF_NODE's interior formula on the rolled window
+#pragma unroll
+                for (int m = 0; m < 3; m++)
+                    nx[m] = (9.0 * (mw[1][m] + mw[2][m]) + -1.0 * (mw[3][m] + mw[0][m])) / 16.0;
+            } else {
+                half_metric(pc, fc, nx);
+            }
             T ql[5], qr[5];
 #pragma unroll
             for (int m = 0; m < 5; m++) { ql[m] = qwl_prev[m]; qr[m] = qwr_m1[m]; }
@@ -660,7 +700,8 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
                 for (int m = 0; m < 5; m++)
                     df[m] = (2250.0 * (fw[3][m] - fw[2][m]) - 125.0 * (fw[4][m] - fw[1][m]) +
                              9.0 * (fw[5][m] - fw[0][m])) / 1920.0;
-                emit_row(pc, row, df);
+                if constexpr (IO::PRE_ROW) pc.io.row_write(pc.base + (long)row * pc.stride, rpre, df);
+                else emit_row(pc, row, df);
             }
             if (fc == n) {
                 if (bb >= n - 1 && a <= n - 1 && !(s41 && rclos)) {
@@ -679,6 +720,9 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
         }
 #pragma unroll
         for (int m = 0; m < 5; m++) qwl_prev[m] = qwl_cur[m];
+        // the interior row of the next iteration, nd-2
+        if constexpr (IO::PRE_ROW)
+            if (nd - 2 >= 1 && nd - 2 <= n) pc.io.row_read(pc.base + (long)(nd - 2) * pc.stride, rpre);
     }
 
     if (s41 && lclos && a == 1) {
