@@ -575,12 +575,17 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
     for (int l = 0; l < 4; l++) load_q(pc, F0 - 2 + l, qw[l + 1]);   // qw[1..4] = F0-2..F0+1
     // one iteration ahead: the next node, the next metric of the rolled
     // window mw = m(nd-2..nd+1), and (solver) the next interior residual row
+    // (solver instances only: in the dual-number tangent and adjoint walks the
+    // extra registers spill)
+    constexpr bool PF = IO::PRE_ROW;
     T qnext[5], mw[4][3], mnext[3];
-    load_q(pc, F0 + 2, qnext);
-    for (int l = 0; l < 3; l++) load_m(pc, F0 - 2 + l, mw[l + 1]);
-    load_m(pc, F0 + 1, mnext);
-    double rpre[IO::PRE_ROW ? 5 : 1];
-    if constexpr (IO::PRE_ROW)
+    if constexpr (PF) {
+        load_q(pc, F0 + 2, qnext);
+        for (int l = 0; l < 3; l++) load_m(pc, F0 - 2 + l, mw[l + 1]);
+        load_m(pc, F0 + 1, mnext);
+    }
+    double rpre[PF ? 5 : 1];
+    if constexpr (PF)
         if (F0 - 3 >= 1) pc.io.row_read(pc.base + (long)(F0 - 3) * pc.stride, rpre);
     // scheme 45: cq(nd-2..nd+1), rolled with the node window
     double cqw[SCH == 45 ? 4 : 1][5];
@@ -596,17 +601,21 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
         for (int l = 0; l < 4; l++)
 #pragma unroll
             for (int m = 0; m < 5; m++) qw[l][m] = qw[l + 1][m];
+        if constexpr (PF) {
 #pragma unroll
-        for (int m = 0; m < 5; m++) qw[4][m] = qnext[m];
+            for (int m = 0; m < 5; m++) qw[4][m] = qnext[m];
 #pragma unroll
-        for (int l = 0; l < 3; l++)
+            for (int l = 0; l < 3; l++)
 #pragma unroll
-            for (int m = 0; m < 3; m++) mw[l][m] = mw[l + 1][m];
+                for (int m = 0; m < 3; m++) mw[l][m] = mw[l + 1][m];
 #pragma unroll
-        for (int m = 0; m < 3; m++) mw[3][m] = mnext[m];
-        if (nd <= F1) {
-            load_q(pc, nd + 3, qnext);
-            load_m(pc, nd + 2, mnext);
+            for (int m = 0; m < 3; m++) mw[3][m] = mnext[m];
+            if (nd <= F1) {
+                load_q(pc, nd + 3, qnext);
+                load_m(pc, nd + 2, mnext);
+            }
+        } else {
+            load_q(pc, nd + 2, qw[4]);
         }
         if constexpr (SCH == 45) {
 #pragma unroll
@@ -656,7 +665,7 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
         int fc = nd - 1;
         if (fc >= F0) {
             T nx[3];
-            if (fc >= 2 && fc <= n - 2) {
+            if (PF && fc >= 2 && fc <= n - 2) {
                 // VALUE_HALF_NODE's interior formula on the rolled window
 #pragma unroll
                 for (int m = 0; m < 3; m++)
