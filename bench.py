@@ -22,6 +22,10 @@ from a few further plain steps with HIP events recorded on the library's own
 stream; the dominant kernel's roofline entry is its algorithmic HBM bytes per
 launch over its mean launch time, against 8 TB/s.
 
+"cpu_baseline" is the C oracle on all host cores; "cpu_reference" times the
+reference's own serial step routines (built from its sources into
+oracle/_ref/libref_step.so) on one core, on the same workload.
+
 The line's "equal_drop" object is the metric's equal-residual-drop
 comparison: the steady workload bump-euler (below) at CFL 20, run once with
 the FP64 and once with the FP32 implicit operator until the monitor has
@@ -197,6 +201,56 @@ def cpu_baseline(case, budget_s=20.0):
                       f"{dt:.1f} s on {nthr} host threads (oracle/wcns_oracle.c, gcc -O2 -fopenmp)"}
 
 
+def cpu_reference(workload, budget_s=10.0):
+    """The reference's own step routines, compiled from its sources in their
+    serial configuration (oracle/_ref/libref_step.so: flang -O2, one rank, one
+    core), on the same workload: whole steps of boundary_all, initial_q_dq_c_t,
+    timestep_tgh, r_h_s_vis, r_h_s_invis, l_h_s_tgh and residual_curve after
+    one untimed step, until about budget_s seconds have passed.  It runs in a
+    child process, since the reference ends its process on a fatal condition
+    (Fortran STOP).  None where that library is not built or the run fails."""
+    import subprocess
+    from oracle import ref_step as R
+    if not R.available():
+        return None
+    try:
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-reference-child", workload,
+                            "--cpu-reference-budget", str(budget_s)], capture_output=True, text=True,
+                           timeout=20 * budget_s + 120)
+    except subprocess.TimeoutExpired:
+        return None
+    lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+    return json.loads(lines[-1]) if p.returncode == 0 and lines else None
+
+
+def cpu_reference_child(workload, budget_s):
+    from oracle import ref_step as R
+    case, _ = build_case(workload)
+    if case.params.nlamtur > 0:
+        print("{}")          # its SA / SST steps need the oracle's wall distance: not timed here
+        return
+    stages = ["boundary_all", "initial_q_dq_c_t", "timestep"] + (["rhs_vis"] if case.params.nvis == 1 else []) + \
+        ["rhs_invis", "lhs"]
+    r = R.RefStep(case)
+
+    def step():
+        for st in stages:
+            r.stage(st)
+        r.residual()
+    step()
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        step()
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 20:
+            break
+    dt = time.perf_counter() - t0
+    print(json.dumps({"value": case.nodes * n / dt / 1e6, "unit": "Mcell-iter/s", "cores": 1, "kind": "reference",
+                      "sample": f"{n} full step(s) of the same {case.nodes}-node workload after one untimed step, "
+                                f"{dt:.1f} s on one host core (the reference's serial step routines, flang -O2)"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,7 +280,11 @@ def main():
     ap.add_argument("--every", type=int, default=10, help="checkpoint interval of --multistep")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters)")
+    ap.add_argument("--cpu-reference-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-reference-budget", type=float, default=10.0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_reference_child:
+        return cpu_reference_child(args.cpu_reference_child, args.cpu_reference_budget)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(a multi-GPU run would shard the grid by block, one rank per GPU)", file=sys.stderr)
@@ -338,9 +396,10 @@ def main():
                  "sweep_tile": list(lay2[:2]), "sgs_paired_helpers": lay2[2],
                  "residual_rms_end": s2.residual_curve()[0]}
         s2.close()
-    cpu = None
+    cpu = cref = None
     if not args.no_cpu:
         cpu = cpu_baseline(case)
+        cref = cpu_reference(args.workload)
     drop = None if args.no_equal_drop else equal_drop_line(args)
     line = {
         "metric": "Mcell-iter/s (RHS+LU-SGS), mixed vs FP64 at equal residual drop; % HBM peak",
@@ -353,7 +412,7 @@ def main():
                    "sgs_paired_helpers": layout[2], "residual_rms_end": rms},
         "other_lhs_precision": other,
         "equal_drop": drop,
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "cpu_reference": cref,
         "step_hbm_frac": round(STEP_BYTES_PER_NODE * nodes * args.steps / sec / (HBM_PEAK_GBS * 1e9), 5),
         "stage_ms_per_step": {k: round(v[0] / kt, 4) for k, v in tim.items()},
     }
