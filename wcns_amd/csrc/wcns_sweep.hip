@@ -643,7 +643,13 @@ __global__ void __launch_bounds__(PIPE_MAX_THREADS) k_sweep_pipe(const SweepBloc
     // wave-uniform: kept in a scalar register, so that the helper's choice of
     // operand arrays by part is a scalar select, not a per-lane load of the
     // pointer from the kernel-argument block
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PIPE_COLS);
+    // Roles by wavefront: 0 the chain, 1..H the helpers, H+1 the exchange.
+    // A workgroup's wavefronts go to the CU's four SIMDs in turn, so wave 4
+    // shares the chain's SIMD: with eight wavefronts (SGS, six helpers) the
+    // mostly sleeping exchange wavefront takes that place and the helpers
+    // move up, so that no helper competes with the chain for issue slots.
+    const int hw = __builtin_amdgcn_readfirstlane(threadIdx.x / PIPE_COLS);
+    const int wave = H == 6 ? (hw == 4 ? H + 1 : (hw > 4 ? hw - 1 : hw)) : hw;
     const int c = threadIdx.x % PIPE_COLS;          // column of this lane
     const int p = wave - 1;                         // helper part (chain: -1)
     const int jl = c % TJ, kl = c / TJ;
