@@ -252,7 +252,9 @@ DFields fields_t(const wcns_ctx *ctx, const DFields &F)
     return T;
 }
 
-// Copies the i-direction inputs to the transposed view, zeroes df_t.
+// Copies the i-direction inputs to the transposed view.  The i-direction
+// kernels of the view write every interior row of df_t (its ghost rows keep
+// the zeros of the allocation), so it needs no clearing.
 int stage_transposed(wcns_ctx *ctx, const DFields &F, bool vis)
 {
     TransList L{};
@@ -269,8 +271,7 @@ int stage_transposed(wcns_ctx *ctx, const DFields &F, bool vis)
         for (int m = 0; m < 4; m++) { L.src[L.n] = ctx->dn + (4 + m) * F.total; L.dst[L.n++] = ctx->dn_t + m * F.total; }
         for (int m = 0; m < 4; m++) { L.src[L.n] = ctx->dn + (8 + m) * F.total; L.dst[L.n++] = ctx->dn_t + (8 + m) * F.total; }
     }
-    if (launch_transpose(0, ctx->grid[GRID_TRANS], L, ctx->stream)) return -3;
-    return launch_fill_u64(reinterpret_cast<unsigned long long *>(ctx->df_t), 5 * F.total, 0ull, ctx->stream);
+    return launch_transpose(0, ctx->grid[GRID_TRANS], L, ctx->stream) ? -3 : 0;
 }
 
 // rhs += the transposed i-direction contribution

@@ -33,12 +33,16 @@ struct InvIO {
     const double *mx, *my, *mz;    // metric triple of the pencil direction
     double *rhs;
     long total;
+    // store: rhs is the transposed i-direction contribution, whose every
+    // interior row one thread writes once (0 + df: the same bits as adding to
+    // the zeroed buffer); otherwise the rows are added to rhs
+    bool store;
     __device__ void q(long g, int, double *o) const { o[0] = r[g]; o[1] = u[g]; o[2] = v[g]; o[3] = w[g]; o[4] = p[g]; }
     __device__ void m(long g, int, double *o) const { o[0] = mx[g]; o[1] = my[g]; o[2] = mz[g]; }
     __device__ void row(long g, int, const double *df) const
     {
 #pragma unroll
-        for (int c = 0; c < 5; c++) rhs[c * total + g] = rhs[c * total + g] + df[c];
+        for (int c = 0; c < 5; c++) rhs[c * total + g] = (store ? 0.0 : rhs[c * total + g]) + df[c];
     }
     __device__ void row_read(long g, double *o) const
     {
@@ -48,7 +52,7 @@ struct InvIO {
     __device__ void row_write(long g, const double *pre, const double *df) const
     {
 #pragma unroll
-        for (int c = 0; c < 5; c++) rhs[c * total + g] = pre[c] + df[c];
+        for (int c = 0; c < 5; c++) rhs[c * total + g] = (store ? 0.0 : pre[c]) + df[c];
     }
 };
 
@@ -586,7 +590,7 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
     }
     double rpre[PF ? 5 : 1];
     if constexpr (PF)
-        if (F0 - 3 >= 1) pc.io.row_read(pc.base + (long)(F0 - 3) * pc.stride, rpre);
+        if (F0 - 3 >= 1 && !pc.io.store) pc.io.row_read(pc.base + (long)(F0 - 3) * pc.stride, rpre);
     // scheme 45: cq(nd-2..nd+1), rolled with the node window
     double cqw[SCH == 45 ? 4 : 1][5];
     if constexpr (SCH == 45) {
@@ -731,7 +735,7 @@ __device__ inline void inv_pencil(const Pencil<IO> &pc, int a, int bb, double ga
         for (int m = 0; m < 5; m++) qwl_prev[m] = qwl_cur[m];
         // the interior row of the next iteration, nd-2
         if constexpr (IO::PRE_ROW)
-            if (nd - 2 >= 1 && nd - 2 <= n) pc.io.row_read(pc.base + (long)(nd - 2) * pc.stride, rpre);
+            if (nd - 2 >= 1 && nd - 2 <= n && !pc.io.store) pc.io.row_read(pc.base + (long)(nd - 2) * pc.stride, rpre);
     }
 
     if (s41 && lclos && a == 1) {
@@ -814,6 +818,7 @@ __device__ inline void rhs_inv_thread(const BlockGrid &G, const DFields &F, cons
     metric_triple<DIR>(F, pc.io.mx, pc.io.my, pc.io.mz);
     pc.io.rhs = F.rhs;
     pc.io.total = F.total;
+    pc.io.store = DIR0 == 3;
     pc.r = F.r;
     pencil_at<DIR>(b, ia, ib, pc.base, pc.stride, pc.n);
     if constexpr (SCH != 0) {

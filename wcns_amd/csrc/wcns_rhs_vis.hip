@@ -460,7 +460,9 @@ __global__ void __launch_bounds__(256) k_vis_rows(BlockGrid G, DFields F, DParam
         else
             df = (2250.0 * (fa(m, l) - fa(m, l - 1)) - 125.0 * (fa(m, l + 1) - fa(m, l - 2)) +
                   9.0 * (fa(m, l + 2) - fa(m, l - 3))) / 1920.0;
-        F.rhs[m * F.total + g] = F.rhs[m * F.total + g] - re * df;
+        // DIR0 = 3: the transposed i-direction contribution, every interior
+        // row written once here (0 - re df: the bits of the zeroed buffer's)
+        F.rhs[m * F.total + g] = (DIR0 == 3 ? 0.0 : F.rhs[m * F.total + g]) - re * df;
     }
 }
 
