@@ -1319,8 +1319,7 @@ int wcns_timestep(wcns_ctx *ctx)
     if (!ctx || !ctx->setup_done) return WCNS_E_STATE;
     DFields F = ctx->fields();
     Scope sc(ctx, WCNS_T_TIMESTEP);
-    if (ctx->prm.nvis == 1 && ctx->prm.csrv > 0.0 && launch_spectvis(ctx->grid[GRID_NODES], F, ctx->dp, ctx->stream))
-        return fail(ctx, WCNS_E_HIP, "spectvis launch");
+    // with nvis = 1 and csrv > 0 the time-step kernel also forms the viscous radii
     if (launch_timestep(ctx->grid[GRID_NODES], F, ctx->dp, ctx->stream)) return fail(ctx, WCNS_E_HIP, "timestep launch");
     return WCNS_OK;
 }

@@ -182,7 +182,7 @@ int launch_bc_fwd(const DRegion &g, const DBlock *blocks, const DFields &F, cons
                   double *tape, hipStream_t s);
 int launch_bc_rev(const DRegion &g, const DBlock *blocks, const DFields &F, const DParams &P, double *lam,
                   const double *tape, hipStream_t s);
-// spectral radii, viscous radii and coed (k_spectvis + k_timestep): tangent
+// spectral radii, viscous radii and coed (k_timestep): tangent
 // ts (7: sra, srb, src, srva, srvb, srvc, coed) from tp; adjoint lp += from
 // ls (3), lv (3) and lc
 int launch_ts_jvp(const BlockGrid &G, const DFields &F, const DParams &P, const double *tp, double *ts, hipStream_t s);
@@ -217,7 +217,6 @@ int launch_zero_ghosts(const DBlock &b, double *arr, long total, int nv, hipStre
 
 int launch_vis_node_deriv(const BlockGrid &G, const DFields &F, double *dn, hipStream_t s);
 int launch_rhs_vis_dir(int dir, const BlockGrid &G, const DFields &F, const DParams &P, const double *dn, hipStream_t s);
-int launch_spectvis(const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s);
 // the viscous residual of one direction in two passes (dir 1: j, 2: k, 3: i on the
 // transposed view); fvb: 5 pool arrays of half-node fluxes
 int launch_vis_faces(int dir, const BlockGrid &G, const DFields &F, const DParams &P, const double *dn, double *fvb,

@@ -1,6 +1,6 @@
 // wcns_rhs_vis.hip -- laminar viscous residual R -= Re^-1 dFv/dxi along i, j, k
-// (reference src/RHS.f90:2-30 -> src/sub_tgh.f90:1511-1810 WCNSE5_VIS_virtual)
-// and the viscous spectral radii (src/LHS.f90:243-343 spectvisl).
+// (reference src/RHS.f90:2-30 -> src/sub_tgh.f90:1511-1810 WCNSE5_VIS_virtual).
+// The viscous spectral radii (src/LHS.f90:141-343) are formed by k_timestep.
 //
 // Pass 1 (k_vis_node_deriv): fourth-order node derivatives of u, v, w, T along
 // each direction (UVWT_DER_4th_virtual, sub_tgh.f90:1909-2029), 12 values per
@@ -661,32 +661,6 @@ __global__ void k_dn_vjp(DBlock b, DFields F, AdAdj A)
     A.gt[g] = A.gt[g] + acc[3];
 }
 
-// LHS.f90:243-343 spectvisl (laminar) / LHS.f90:141-241 spectvist (mu + mu_t), interior
-// nodes (ghost-layer radii only ever multiply a zero dq)
-__global__ void k_spectvis(BlockGrid G, DFields F, DParams P)
-{
-    DBlock b;
-    long t;
-    grid_thread(G, b, t);
-    long n = (long)b.ni * b.nj * b.nk;
-    if (t >= n) return;
-    int i = (int)(t % b.ni) + 1, j = (int)((t / b.ni) % b.nj) + 1, k = (int)(t / ((long)b.ni * b.nj)) + 1;
-    long g = gidx(b, i, j, k);
-    double rm_vol = F.r[g] * F.vol[g];
-    double kx = F.kcx[g], ky = F.kcy[g], kz = F.kcz[g];
-    double ex = F.etx[g], ey = F.ety[g], ez = F.etz[g];
-    double cx = F.ctx[g], cy = F.cty[g], cz = F.ctz[g];
-    double ri = kx * kx + ky * ky + kz * kz;
-    double rj = ex * ex + ey * ey + ez * ez;
-    double rk = cx * cx + cy * cy + cz * cz;
-    double vis = P.turb ? F.visl[g] + F.vist[g] : F.visl[g];
-    double coef = 2.0 * vis / (P.reynolds * rm_vol + SMALL);
-    double coef1 = P.csrv * coef;
-    F.srva[g] = coef1 * ri;
-    F.srvb[g] = coef1 * rj;
-    F.srvc[g] = coef1 * rk;
-}
-
 
 }  // namespace
 
@@ -792,13 +766,6 @@ int launch_dn_vjp(const DBlock &b, const DFields &F, const AdAdj &A, hipStream_t
 {
     long e = (long)(b.ni + 6) * (b.nj + 6) * (b.nk + 6);
     hipLaunchKernelGGL(k_dn_vjp, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, b, F, A);
-    return hipGetLastError() == hipSuccess ? 0 : -3;
-}
-
-int launch_spectvis(const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s)
-{
-    if (G.nwg == 0) return 0;
-    hipLaunchKernelGGL(k_spectvis, dim3(G.nwg), dim3(256), 0, s, G, F, P);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -304,7 +304,12 @@ __global__ void k_init_ghost_t(BlockGrid G, DFields F, DParams P, int dir)
 }
 
 // ---------------------------------------------------- spectral radius, dt
-// Convect.f90:1338-1461 spectinv (interior) + LHS.f90:1343-1377 localdt0
+// Convect.f90:1338-1461 spectinv (interior) + LHS.f90:1343-1377 localdt0;
+// VIS: also the viscous radii of spectvisl (laminar, LHS.f90:243-343) /
+// spectvist (mu + mu_t, LHS.f90:141-241) of the node, formed here rather than
+// read back from a kernel of their own; interior nodes only (ghost-layer radii
+// only ever multiply a zero dq)
+template <bool VIS>
 __global__ void k_timestep(BlockGrid G, DFields F, DParams P)
 {
     DBlock b;
@@ -328,7 +333,24 @@ __global__ void k_timestep(BlockGrid G, DFields F, DParams P)
     F.sra[g] = sra;
     F.srb[g] = srb;
     F.src[g] = src;
-    double rv = F.srva[g] + F.srvb[g] + F.srvc[g];
+    double sva, svb, svc;
+    if constexpr (VIS) {
+        const double rm_vol = F.r[g] * F.vol[g];
+        const double vis = P.turb ? F.visl[g] + F.vist[g] : F.visl[g];
+        const double coef = 2.0 * vis / (P.reynolds * rm_vol + SMALL);
+        const double coef1 = P.csrv * coef;
+        sva = coef1 * (kx * kx + ky * ky + kz * kz);
+        svb = coef1 * (ex * ex + ey * ey + ez * ez);
+        svc = coef1 * (cx * cx + cy * cy + cz * cz);
+        F.srva[g] = sva;
+        F.srvb[g] = svb;
+        F.srvc[g] = svc;
+    } else {
+        sva = F.srva[g];
+        svb = F.srvb[g];
+        svc = F.srvc[g];
+    }
+    double rv = sva + svb + svc;
     double rabc = sra + srb + src + rv;
     double dt;
     if (P.timedt_rate > SMALL) {
@@ -341,7 +363,7 @@ __global__ void k_timestep(BlockGrid G, DFields F, DParams P)
     if (F.coed) {
         // LHS.f90:602-645 getcoed, the same for all four sweeps of the step
         double rad_ns = sra + srb + src;
-        if (P.nvis >= 1) rad_ns = rad_ns + F.srva[g] + F.srvb[g] + F.srvc[g];
+        if (P.nvis >= 1) rad_ns = rad_ns + sva + svb + svc;
         double ct = 1.0 / dt;
         F.coed[g] = 1.0 / (ct + 1.0 * 1.0 * rad_ns);
     }
@@ -737,7 +759,9 @@ int launch_init_stage(const BlockGrid &nodes, const BlockGrid *ghost_t, const DF
 
 int launch_timestep(const BlockGrid &G, const DFields &F, const DParams &P, hipStream_t s)
 {
-    if (G.nwg) hipLaunchKernelGGL(k_timestep, dim3(G.nwg), dim3(256), 0, s, G, F, P);
+    const bool vis = P.nvis == 1 && P.csrv > 0.0;
+    if (G.nwg && vis) hipLaunchKernelGGL(k_timestep<true>, dim3(G.nwg), dim3(256), 0, s, G, F, P);
+    if (G.nwg && !vis) hipLaunchKernelGGL(k_timestep<false>, dim3(G.nwg), dim3(256), 0, s, G, F, P);
     return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
